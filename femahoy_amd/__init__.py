@@ -1,0 +1,365 @@
+"""femahoy_amd: the iterative solve of femahoy's `fem` crate on one AMD MI355X.
+
+The host side of the drop-in boundary, over the C ABI of include/fem_solver.h
+(libfemahoy_amd.so, built in-tree by `make -C femahoy_amd`). It mirrors the reference's
+solver interface (crates/fem/src/solve.rs):
+
+    Solver.new(box, method, ...)      Solver::new (solve.rs:92)
+    solver.with_max_iter(n)           Solver::with_max_iter (solve.rs:187)
+    solver.method()                   Solver::method (solve.rs:191)
+    solver.solve(b, x, cancelled)     Solver::solve (solve.rs:199), returning Stats (solve.rs:81)
+
+with the same error texts ("conjugate gradients did not converge: ...", "cancelled"). The
+library is the only compute path: if it is missing, importing works but every solver call
+raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass, field
+from typing import Callable, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfemahoy_amd.so")
+
+HEAT_HEX8, HEAT_TET4, ELASTIC_TET4, ELASTIC_TET10 = 0, 1, 2, 3
+KINDS = {"heat_hex8": HEAT_HEX8, "heat_tet4": HEAT_TET4, "elastic_tet4": ELASTIC_TET4, "elastic_tet10": ELASTIC_TET10}
+METHODS = {None: -1, "direct": 0, "multigrid": 1}
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2}
+FACES = ("x0", "x1", "y0", "y1", "z0", "z1")
+
+PCG_TOL = 1e-10        # solve.rs:42
+PCG_MAX_ITER = 500     # solve.rs:43
+STEADY_DIRECT_LIMIT = 3000  # solve.rs:18
+
+
+class FemError(RuntimeError):
+    """An error of the library, with the reference's text where the reference has one."""
+
+
+class _Box(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("n", C.c_int32 * 3),
+        ("length", C.c_double * 3),
+        ("coef0", C.POINTER(C.c_double)),
+        ("coef1", C.POINTER(C.c_double)),
+        ("fixed", C.c_int32 * 6),
+        ("value", C.c_double * 6),
+        ("source", C.c_double),
+        ("body", C.c_double * 3),
+        ("traction", C.c_double * 18),
+    ]
+
+
+class _Options(C.Structure):
+    _fields_ = [
+        ("tol", C.c_double),
+        ("max_iter", C.c_uint64),
+        ("method", C.c_int32),
+        ("precision", C.c_int32),
+        ("crossover", C.c_uint64),
+        ("levels", C.c_int32),
+        ("cheb_degree", C.c_int32),
+        ("cheb_lower", C.c_double),
+        ("cheb_upper", C.c_double),
+        ("power_steps", C.c_int32),
+        ("coarse_max", C.c_uint64),
+        ("refine", C.c_int32),
+        ("device", C.c_int32),
+    ]
+
+
+class _Stats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_uint64),
+        ("residual", C.c_double),
+        ("true_residual", C.c_double),
+        ("seconds", C.c_double),
+        ("setup_seconds", C.c_double),
+        ("levels", C.c_uint64),
+        ("coarse_unknowns", C.c_uint64),
+        ("restarts", C.c_uint64),
+        ("method", C.c_int32),
+        ("precision", C.c_int32),
+    ]
+
+
+CANCEL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)
+
+EXPORTS = (
+    "fem_options_default", "fem_solver_new_box", "fem_solver_solve", "fem_solver_solve_device",
+    "fem_solver_method", "fem_solver_set_max_iter", "fem_solver_set_tol", "fem_solver_free",
+    "fem_solver_free_count", "fem_solver_unknown_count", "fem_solver_free_list", "fem_solver_point_grid",
+    "fem_solver_rhs", "fem_solver_apply", "fem_solver_precondition", "fem_solver_level_info",
+    "fem_solver_device_name", "fem_model_new_box", "fem_model_point_count", "fem_model_unknown_count",
+    "fem_model_free_count", "fem_model_free_list", "fem_model_point_grid", "fem_model_load", "fem_model_free",
+)
+
+_lib = None
+
+
+def build(force: bool = False) -> None:
+    """Compile libfemahoy_amd.so for gfx950 with hipcc (make)."""
+    cmd = ["make", "-s", "-C", HERE] + (["-B"] if force else []) + ["libfemahoy_amd.so"]
+    subprocess.run(cmd, check=True)
+
+
+def lib() -> C.CDLL:
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FemError(f"{LIB_PATH} is not built: run `make -C {HERE}` (hipcc, gfx950)")
+    L = C.CDLL(LIB_PATH)
+    P, SZ, ERR = C.c_void_p, C.c_size_t, C.c_char_p
+    F64P = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+    I64P = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+    U32P = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+    sig = {
+        "fem_options_default": (None, [C.POINTER(_Options)]),
+        "fem_solver_new_box": (C.c_int, [C.POINTER(_Box), C.POINTER(_Options), C.POINTER(P), ERR, SZ]),
+        "fem_solver_solve": (C.c_int, [P, F64P, F64P, CANCEL_FN, P, C.POINTER(_Stats), ERR, SZ]),
+        "fem_solver_solve_device": (C.c_int, [P, P, P, CANCEL_FN, P, C.POINTER(_Stats), ERR, SZ]),
+        "fem_solver_method": (C.c_int, [P]),
+        "fem_solver_set_max_iter": (None, [P, C.c_uint64]),
+        "fem_solver_set_tol": (None, [P, C.c_double]),
+        "fem_solver_free": (None, [P]),
+        "fem_solver_free_count": (C.c_uint64, [P]),
+        "fem_solver_unknown_count": (C.c_uint64, [P]),
+        "fem_solver_free_list": (C.c_int, [P, I64P]),
+        "fem_solver_point_grid": (C.c_int, [P, U32P]),
+        "fem_solver_rhs": (C.c_int, [P, F64P, ERR, SZ]),
+        "fem_solver_apply": (C.c_int, [P, F64P, F64P, ERR, SZ]),
+        "fem_solver_precondition": (C.c_int, [P, F64P, F64P, ERR, SZ]),
+        "fem_solver_level_info": (C.c_int, [P, C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
+        "fem_solver_device_name": (C.c_char_p, [P]),
+        "fem_model_new_box": (C.c_int, [C.POINTER(_Box), C.POINTER(P), ERR, SZ]),
+        "fem_model_point_count": (C.c_uint64, [P]),
+        "fem_model_unknown_count": (C.c_uint64, [P]),
+        "fem_model_free_count": (C.c_uint64, [P]),
+        "fem_model_free_list": (C.c_int, [P, I64P]),
+        "fem_model_point_grid": (C.c_int, [P, U32P]),
+        "fem_model_load": (C.c_int, [P, F64P]),
+        "fem_model_free": (None, [P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _err():
+    return C.create_string_buffer(2048)
+
+
+@dataclass
+class Box:
+    """A structured box model: `n` cells on [0, length], one material value per cell (hex8) or per
+    tetrahedron (6 per cell, in the reference's numbering of tetrahedra)."""
+
+    kind: str
+    n: Sequence[int]
+    length: Sequence[float]
+    coef0: np.ndarray
+    coef1: Optional[np.ndarray] = None
+    fixed: dict = field(default_factory=dict)       # heat: face -> temperature; elasticity: face -> True (clamped)
+    source: float = 0.0
+    body: Sequence[float] = (0.0, 0.0, 0.0)
+    traction: dict = field(default_factory=dict)    # face -> (tx, ty, tz)
+
+    def cells(self) -> int:
+        return int(np.prod(self.n))
+
+    def _struct(self):
+        b = _Box()
+        b.kind = KINDS[self.kind]
+        for d in range(3):
+            b.n[d] = int(self.n[d])
+            b.length[d] = float(self.length[d])
+        count = self.cells() * (1 if self.kind == "heat_hex8" else 6)
+        c0 = np.ascontiguousarray(np.broadcast_to(np.asarray(self.coef0, np.float64), (count,)))
+        keep = [c0]
+        b.coef0 = c0.ctypes.data_as(C.POINTER(C.c_double))
+        if self.coef1 is not None:
+            c1 = np.ascontiguousarray(np.broadcast_to(np.asarray(self.coef1, np.float64), (count,)))
+            keep.append(c1)
+            b.coef1 = c1.ctypes.data_as(C.POINTER(C.c_double))
+        for f, v in self.fixed.items():
+            i = FACES.index(f)
+            b.fixed[i] = 1
+            b.value[i] = 0.0 if v is True else float(v)
+        b.source = float(self.source)
+        for d in range(3):
+            b.body[d] = float(self.body[d])
+        for f, t in self.traction.items():
+            i = FACES.index(f)
+            for d in range(3):
+                b.traction[3 * i + d] = float(t[d])
+        return b, keep
+
+
+@dataclass
+class Stats:
+    """solve.rs:81-86 Stats (iterations, residual) and what the device solve adds."""
+
+    iterations: int
+    residual: float
+    true_residual: float
+    seconds: float
+    setup_seconds: float
+    levels: int
+    coarse_unknowns: int
+    restarts: int
+    method: str
+    precision: str
+
+
+class Model:
+    """The numbering, free unknowns and loads of a box model (host only, no device)."""
+
+    def __init__(self, box: Box):
+        b, keep = box._struct()
+        h = C.c_void_p()
+        err = _err()
+        if lib().fem_model_new_box(C.byref(b), C.byref(h), err, 2048):
+            raise FemError(err.value.decode())
+        self.h = h
+        del keep
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().fem_model_free(self.h)
+            self.h = None
+
+    def points(self) -> int:
+        return lib().fem_model_point_count(self.h)
+
+    def free_list(self) -> np.ndarray:
+        out = np.zeros(lib().fem_model_free_count(self.h), np.int64)
+        lib().fem_model_free_list(self.h, out)
+        return out
+
+    def point_grid(self) -> np.ndarray:
+        out = np.zeros(3 * self.points(), np.uint32)
+        lib().fem_model_point_grid(self.h, out)
+        return out.reshape(-1, 3)
+
+    def load(self) -> np.ndarray:
+        out = np.zeros(lib().fem_model_unknown_count(self.h))
+        lib().fem_model_load(self.h, out)
+        return out
+
+
+class Solver:
+    """Solver::new / solve / method / with_max_iter over the device library."""
+
+    def __init__(self, box: Box, method: Optional[str] = None, precision: str = "fp32", tol: float = PCG_TOL,
+                 max_iter: int = PCG_MAX_ITER, crossover: int = STEADY_DIRECT_LIMIT, levels: int = 0,
+                 cheb_degree: int = 3, cheb_lower: float = 0.1, cheb_upper: float = 1.1, power_steps: int = 20,
+                 coarse_max: int = 6000, refine: bool = True, device: int = 0):
+        o = _Options()
+        lib().fem_options_default(C.byref(o))
+        o.tol, o.max_iter, o.method, o.precision = tol, max_iter, METHODS[method], PRECISIONS[precision]
+        o.crossover, o.levels, o.cheb_degree = crossover, levels, cheb_degree
+        o.cheb_lower, o.cheb_upper, o.power_steps = cheb_lower, cheb_upper, power_steps
+        o.coarse_max, o.refine, o.device = coarse_max, int(refine), device
+        b, keep = box._struct()
+        h = C.c_void_p()
+        err = _err()
+        if lib().fem_solver_new_box(C.byref(b), C.byref(o), C.byref(h), err, 2048):
+            raise FemError(err.value.decode())
+        del keep
+        self.h = h
+        self.n = int(lib().fem_solver_free_count(self.h))
+
+    new = classmethod(lambda cls, *a, **k: cls(*a, **k))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().fem_solver_free(self.h)
+            self.h = None
+
+    def method(self) -> str:
+        return ("direct", "multigrid")[lib().fem_solver_method(self.h)]
+
+    def with_max_iter(self, max_iter: int) -> "Solver":
+        lib().fem_solver_set_max_iter(self.h, max_iter)
+        return self
+
+    def set_tol(self, tol: float) -> "Solver":
+        lib().fem_solver_set_tol(self.h, tol)
+        return self
+
+    def free_list(self) -> np.ndarray:
+        out = np.zeros(self.n, np.int64)
+        lib().fem_solver_free_list(self.h, out)
+        return out
+
+    def rhs(self) -> np.ndarray:
+        out = np.zeros(self.n)
+        err = _err()
+        if lib().fem_solver_rhs(self.h, out, err, 2048):
+            raise FemError(err.value.decode())
+        return out
+
+    def apply(self, x) -> np.ndarray:
+        y = np.zeros(self.n)
+        err = _err()
+        if lib().fem_solver_apply(self.h, np.ascontiguousarray(x, np.float64), y, err, 2048):
+            raise FemError(err.value.decode())
+        return y
+
+    def precondition(self, r) -> np.ndarray:
+        z = np.zeros(self.n)
+        err = _err()
+        if lib().fem_solver_precondition(self.h, np.ascontiguousarray(r, np.float64), z, err, 2048):
+            raise FemError(err.value.decode())
+        return z
+
+    def level_info(self, level: int):
+        n = C.c_uint64()
+        lm = C.c_double()
+        if lib().fem_solver_level_info(self.h, level, C.byref(n), C.byref(lm)):
+            raise FemError("no such level")
+        return int(n.value), float(lm.value)
+
+    def device_name(self) -> str:
+        return lib().fem_solver_device_name(self.h).decode()
+
+    def solve(self, b, x=None, cancelled: Optional[Callable[[], bool]] = None):
+        """Solve A x = b; `x` is the start value (zeros if None). Returns (x, Stats)."""
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(self.n) if x is None else np.array(x, np.float64, copy=True)
+        if b.shape != (self.n,) or x.shape != (self.n,):
+            raise FemError(f"b and x need {self.n} values")
+        cb = CANCEL_FN(lambda _ctx: 1 if cancelled() else 0) if cancelled else CANCEL_FN()
+        st = _Stats()
+        err = _err()
+        rc = lib().fem_solver_solve(self.h, b, x, cb, None, C.byref(st), err, 2048)
+        if rc:
+            raise FemError(err.value.decode())
+        return x, _stats(st)
+
+    def solve_device(self, d_b: int, d_x: int, cancelled: Optional[Callable[[], bool]] = None) -> Stats:
+        """Solve with b and x already in device memory (raw HIP pointers)."""
+        cb = CANCEL_FN(lambda _ctx: 1 if cancelled() else 0) if cancelled else CANCEL_FN()
+        st = _Stats()
+        err = _err()
+        if lib().fem_solver_solve_device(self.h, C.c_void_p(d_b), C.c_void_p(d_x), cb, None, C.byref(st), err, 2048):
+            raise FemError(err.value.decode())
+        return _stats(st)
+
+
+def _stats(st: _Stats) -> Stats:
+    return Stats(int(st.iterations), float(st.residual), float(st.true_residual), float(st.seconds),
+                 float(st.setup_seconds), int(st.levels), int(st.coarse_unknowns), int(st.restarts),
+                 ("direct", "multigrid")[st.method] if st.method in (0, 1) else "?",
+                 {v: k for k, v in PRECISIONS.items()}.get(st.precision, "?"))

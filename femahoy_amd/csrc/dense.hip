@@ -1,0 +1,394 @@
+// The dense coarse level on the FP64 matrix cores.
+//
+// Factor: right-looking blocked LDL^T with 64 x 64 blocks. Each step factors the diagonal block
+// in LDS (one workgroup), then forms the block column L_ik = A_ik L_kk^-T D_k^-1 and the trailing
+// update A_ij -= (L_ik D_k) L_jk^T as 64 x 64 x 64 products on v_mfma_f64_16x16x4f64.
+// Inverse: U = L^-T by a blocked triangular solve on the identity (the same products), then
+// A^-1 = (U D^-1/2)(U D^-1/2)^T, one product over the triangular K range of each tile.
+// Per cycle the coarse solve is x = A^-1 b, one wave per row, in the V-cycle's precision.
+//
+// The pivot test is the reference's (crates/fem/src/cholesky.rs:22 and 92-99): a pivot at or
+// below 1e-10 of its diagonal in A stops the factor with "the matrix is not positive definite".
+#include "dense.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+namespace fem {
+
+namespace {
+
+constexpr int NB = 64;
+constexpr int KC = 16;
+constexpr double PIVOT = 1e-10;
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+struct GemmArgs {
+    const double* A;
+    int lda;
+    const double* B;
+    int ldb;
+    double* C;
+    int ldc;
+    int K;
+    double alpha, beta;
+    int mode;     // 0 rectangle, 1 lower triangle of tiles, 2 symmetric product (K from the tile)
+    int ti0, tj0; // tile offsets
+    int count;    // mode 1: tiles per side
+    double* W;    // optional second output W = C * colscale[col] (ld 64)
+    const double* colscale;
+};
+
+__global__ void __launch_bounds__(256) k_gemm_nt(GemmArgs g) {
+    __shared__ double sA[NB][KC + 1];
+    __shared__ double sB[NB][KC + 1];
+    int ti, tj;
+    if (g.mode == 1) {
+        // blockIdx.x -> (a, b) with b <= a
+        int t = blockIdx.x, a = 0;
+        while (t > a) {
+            t -= a + 1;
+            ++a;
+        }
+        ti = g.ti0 + a;
+        tj = g.tj0 + t;
+    } else {
+        ti = g.ti0 + blockIdx.x;
+        tj = g.tj0 + blockIdx.y;
+    }
+    int kofs = 0, K = g.K;
+    if (g.mode == 2) {
+        kofs = (ti > tj ? ti : tj) * NB;
+        K = g.K - kofs;
+    }
+    const double* Ab = g.A + (size_t)ti * NB * g.lda + kofs;
+    const double* Bb = g.B + (size_t)tj * NB * g.ldb + kofs;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    double4_t acc[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < K; k0 += KC) {
+#pragma unroll
+        for (int e = 0; e < (NB * KC) / 256; ++e) {
+            const int idx = tid + 256 * e;
+            const int r = idx / KC, c = idx % KC;
+            sA[r][c] = Ab[(size_t)r * g.lda + k0 + c];
+            sB[r][c] = Bb[(size_t)r * g.ldb + k0 + c];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < KC; kk += 4) {
+            double af[2], bf[2];
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi) af[bi] = sA[wr + bi * 16 + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) bf[bj] = sB[wc + bj * 16 + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj)
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[bi], bf[bj], acc[bi][bj], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    double* Cb = g.C + (size_t)ti * NB * g.ldc + (size_t)tj * NB;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                // v_mfma_f64_16x16x4f64 result: lane l, register r holds row (l / 16) + 4 r,
+                // column l % 16 of the 16 x 16 block.
+                const int row = wr + bi * 16 + (lane >> 4) + 4 * r;
+                const int col = wc + bj * 16 + (lane & 15);
+                double* c = Cb + (size_t)row * g.ldc + col;
+                const double v = (g.alpha != 0.0 ? g.alpha * *c : 0.0) + g.beta * acc[bi][bj][r];
+                *c = v;
+                if (g.W) g.W[((size_t)ti * NB + row) * NB + col] = v * g.colscale[col];
+            }
+}
+
+// Factors the diagonal block kb in LDS: A_kk = L D L^T (L unit lower). Writes L into the block
+// (strict lower part), D into d[kb*64 ..], L^-1 into linv[kb] and D^-1 L^-1 into bk.
+__global__ void __launch_bounds__(256) k_ldlt_diag(double* A, int lda, int kb, const double* orig_diag, double* d,
+                                                   double* linv, double* bk, int* status) {
+    __shared__ double S[NB][NB + 1];
+    const int tid = threadIdx.x;
+    double* blk = A + (size_t)kb * NB * lda + (size_t)kb * NB;
+    for (int e = tid; e < NB * NB; e += 256) {
+        const int r = e / NB, c = e % NB;
+        S[r][c] = r >= c ? blk[(size_t)r * lda + c] : 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < NB; ++j) {
+        double dj = S[j][j];
+        if (!(dj > PIVOT * fabs(orig_diag[kb * NB + j]))) {
+            if (tid == 0 && *status == 0) *status = kb * NB + j + 1;
+            dj = 1.0;  // keep going with a harmless pivot; the host reports the failure
+        }
+        const int m = NB - 1 - j;
+        // trailing update of the lower triangle: S[r][c] -= S[r][j] S[c][j] / dj, j < c <= r
+        for (int e = tid; e < m * m; e += 256) {
+            const int r = j + 1 + e / m, c = j + 1 + e % m;
+            if (c <= r) S[r][c] -= S[r][j] * S[c][j] / dj;
+        }
+        __syncthreads();
+        if (tid > j && tid < NB) S[tid][j] /= dj;
+        if (tid == 0) S[j][j] = dj;
+        __syncthreads();
+    }
+    // L^-1 (unit lower), one column per thread; each thread reads back only its own column.
+    double* li = linv + (size_t)kb * NB * NB;
+    if (tid < NB) {
+        const int c = tid;
+        for (int r = 0; r < NB; ++r) li[r * NB + c] = r == c ? 1.0 : 0.0;
+        for (int r = c + 1; r < NB; ++r) {
+            double s = 0.0;
+            for (int k = c; k < r; ++k) s += S[r][k] * li[k * NB + c];
+            li[r * NB + c] = -s;
+        }
+        for (int r = 0; r < NB; ++r) bk[r * NB + c] = li[r * NB + c] / S[r][r];
+    }
+    for (int e = tid; e < NB * NB; e += 256) {
+        const int r = e / NB, c = e % NB;
+        if (r > c) blk[(size_t)r * lda + c] = S[r][c];
+    }
+    if (tid < NB) d[kb * NB + tid] = S[tid][tid];
+}
+
+__global__ void k_pad(const double* a, int n, double* A, int np) {
+    const size_t e = blockIdx.x * (size_t)256 + threadIdx.x;
+    if (e >= (size_t)np * np) return;
+    const int r = (int)(e / np), c = (int)(e % np);
+    A[e] = (r < n && c < n) ? a[(size_t)r * n + c] : (r == c ? 1.0 : 0.0);
+}
+
+__global__ void k_diag_of(const double* A, int np, double* dg) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r < np) dg[r] = A[(size_t)r * np + r];
+}
+
+__global__ void k_identity(double* U, int np) {
+    const size_t e = blockIdx.x * (size_t)256 + threadIdx.x;
+    if (e >= (size_t)np * np) return;
+    U[e] = (e / np == e % np) ? 1.0 : 0.0;
+}
+
+// V = U D^-1/2 (columns scaled).
+__global__ void k_scale_cols(double* U, int np, const double* d) {
+    const size_t e = blockIdx.x * (size_t)256 + threadIdx.x;
+    if (e >= (size_t)np * np) return;
+    U[e] *= 1.0 / sqrt(d[e % np]);
+}
+
+template <class T>
+__global__ void k_extract(const double* Ainv, int np, int n, T* out) {
+    const size_t e = blockIdx.x * (size_t)256 + threadIdx.x;
+    if (e >= (size_t)n * n) return;
+    out[e] = (T)Ainv[(e / n) * np + e % n];
+}
+
+// x[map[i]] = sum_j inv[i][j] b[map[j]]: one wave per row, 4 rows per block, b staged in LDS
+// (STAGE) or, for a large direct solve, read through the caches.
+template <class T, bool STAGE>
+__global__ void __launch_bounds__(256) k_coarse_gemv(const T* __restrict__ inv, int n, const int64_t* __restrict__ map,
+                                                     const T* __restrict__ b, T* __restrict__ x) {
+    extern __shared__ unsigned char smem[];
+    T* sb = reinterpret_cast<T*>(smem);
+    if (STAGE) {
+        for (int j = threadIdx.x; j < n; j += 256) sb[j] = b[map[j]];
+        __syncthreads();
+    }
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    const T* a = inv + (size_t)row * n;
+    T s = T(0);
+    for (int j = lane; j < n; j += 64) s += a[j] * (STAGE ? sb[j] : b[map[j]]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) x[map[row]] = s;
+}
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Error(std::string("HIP error ") + hipGetErrorString(e) + " in " + what);
+}
+
+void launch_gemm(const GemmArgs& g, dim3 grid, hipStream_t s) {
+    if (grid.x == 0 || grid.y == 0) return;
+    hipLaunchKernelGGL(k_gemm_nt, grid, dim3(256), 0, s, g);
+    check(hipGetLastError(), "k_gemm_nt");
+}
+
+}  // namespace
+
+void dense_gemm_nt(int m, int n, int k, double alpha, const double* A, int lda, double beta, const double* B, int ldb,
+                   double* C, int ldc, hipStream_t s) {
+    if (m % NB || n % NB || k % KC) throw Error("dense_gemm_nt needs sizes in multiples of 64");
+    GemmArgs g{A, lda, B, ldb, C, ldc, k, alpha, beta, 0, 0, 0, 0, nullptr, nullptr};
+    launch_gemm(g, dim3(m / NB, n / NB), s);
+}
+
+void DenseCoarse::factor(double* a, int n, bool fp32, hipStream_t s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    n_ = n;
+    fp32_ = fp32;
+    const int nb = (n + NB - 1) / NB;
+    np_ = nb * NB;
+    const int np = np_;
+    const size_t nn = (size_t)np * np;
+    DArray<double> A(nn), U(nn), P((size_t)np * NB), linv((size_t)nb * NB * NB), bk((size_t)NB * NB), d(np), dg(np);
+    DArray<int> status(1);
+    FEM_HIP(hipMemsetAsync(status.data(), 0, sizeof(int), s));
+    const unsigned blocks = (unsigned)((nn + 255) / 256);
+    hipLaunchKernelGGL(k_pad, dim3(blocks), dim3(256), 0, s, a, n, A.data(), np);
+    hipLaunchKernelGGL(k_diag_of, dim3((np + 255) / 256), dim3(256), 0, s, A.data(), np, dg.data());
+    for (int kb = 0; kb < nb; ++kb) {
+        hipLaunchKernelGGL(k_ldlt_diag, dim3(1), dim3(256), 0, s, A.data(), np, kb, dg.data(), d.data(), linv.data(),
+                           bk.data(), status.data());
+        const int below = nb - kb - 1;
+        if (below == 0) break;
+        // L_ik = A_ik (D^-1 L^-1)^T ; W_ik = L_ik D_k
+        GemmArgs pg{A.data() + (size_t)kb * NB, np, bk.data(), NB, A.data() + (size_t)kb * NB, np, NB, 0.0, 1.0, 0,
+                    kb + 1, 0, 0, P.data(), d.data() + (size_t)kb * NB};
+        launch_gemm(pg, dim3(below, 1), s);
+        // A_ij -= W_ik L_jk^T for kb < j <= i
+        GemmArgs ug{P.data(), NB, A.data() + (size_t)kb * NB, np, A.data(), np, NB, 1.0, -1.0, 1, kb + 1, kb + 1, below,
+                    nullptr, nullptr};
+        launch_gemm(ug, dim3(below * (below + 1) / 2, 1), s);
+    }
+    int bad = 0;
+    FEM_HIP(hipMemcpyAsync(&bad, status.data(), sizeof(int), hipMemcpyDeviceToHost, s));
+    FEM_HIP(hipStreamSynchronize(s));
+    if (bad) {
+        const int col = bad - 1;
+        double diag = 0.0;
+        FEM_HIP(hipMemcpy(&diag, dg.data() + col, sizeof(double), hipMemcpyDeviceToHost));
+        char buf[256];
+        std::snprintf(buf, sizeof buf,
+                      "the matrix is not positive definite: the pivot of column %d is 0 or less to the precision of "
+                      "the arithmetic (its diagonal is %g)",
+                      col, diag);
+        throw Error(buf);
+    }
+    // U = L^-T: solve on the identity, working on the transpose (all products are A B^T).
+    hipLaunchKernelGGL(k_identity, dim3(blocks), dim3(256), 0, s, U.data(), np);
+    for (int k = 0; k < nb; ++k) {
+        // U_jk = U_jk Linv_kk^T for j <= k
+        GemmArgs s1{U.data() + (size_t)k * NB, np, linv.data() + (size_t)k * NB * NB, NB, U.data() + (size_t)k * NB, np,
+                    NB, 0.0, 1.0, 0, 0, 0, 0, nullptr, nullptr};
+        launch_gemm(s1, dim3(k + 1, 1), s);
+        // U_ji -= U_jk L_ik^T for j <= k < i
+        const int right = nb - k - 1;
+        if (right == 0) continue;
+        GemmArgs s2{U.data() + (size_t)k * NB, np, A.data() + (size_t)k * NB, np, U.data(), np, NB, 1.0, -1.0, 0, 0,
+                    k + 1, 0, nullptr, nullptr};
+        launch_gemm(s2, dim3(k + 1, right), s);
+    }
+    hipLaunchKernelGGL(k_scale_cols, dim3(blocks), dim3(256), 0, s, U.data(), np, d.data());
+    // A^-1 = V V^T with V upper triangular: K from max(i, j) of the tile.
+    DArray<double> inv(nn);
+    GemmArgs fg{U.data(), np, U.data(), np, inv.data(), np, np, 0.0, 1.0, 2, 0, 0, 0, nullptr, nullptr};
+    launch_gemm(fg, dim3(nb, nb), s);
+    const unsigned eb = (unsigned)(((size_t)n * n + 255) / 256);
+    if (fp32) {
+        inv32_.resize((size_t)n * n);
+        hipLaunchKernelGGL(k_extract<float>, dim3(eb), dim3(256), 0, s, inv.data(), np, n, inv32_.data());
+    } else {
+        inv64_.resize((size_t)n * n);
+        hipLaunchKernelGGL(k_extract<double>, dim3(eb), dim3(256), 0, s, inv.data(), np, n, inv64_.data());
+    }
+    FEM_HIP(hipGetLastError());
+    FEM_HIP(hipStreamSynchronize(s));
+    factor_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const double dn = (double)np;
+    factor_flops_ = dn * dn * dn / 3.0 + dn * dn * dn / 3.0 + dn * dn * dn / 3.0;
+}
+
+template <class T>
+void DenseCoarse::solve(const T* b, T* x, hipStream_t s) const {
+    const T* inv;
+    if constexpr (sizeof(T) == 4) {
+        if (!fp32_) throw Error("the coarse inverse is FP64");
+        inv = inv32_.data();
+    } else {
+        if (fp32_) throw Error("the coarse inverse is FP32");
+        inv = inv64_.data();
+    }
+    const size_t shm = (size_t)n_ * sizeof(T);
+    if (shm <= 48 * 1024)
+        hipLaunchKernelGGL((k_coarse_gemv<T, true>), dim3((n_ + 3) / 4), dim3(256), shm, s, inv, n_, map_.data(), b, x);
+    else
+        hipLaunchKernelGGL((k_coarse_gemv<T, false>), dim3((n_ + 3) / 4), dim3(256), 0, s, inv, n_, map_.data(), b, x);
+}
+
+template void DenseCoarse::solve<float>(const float*, float*, hipStream_t) const;
+template void DenseCoarse::solve<double>(const double*, double*, hipStream_t) const;
+
+}  // namespace fem
+
+// ---------------------------------------------------------------------------------------------
+// Test hooks (include/fem_testing.h).
+#include <cstring>
+
+#include "../../include/fem_testing.h"
+
+namespace {
+template <class F>
+int hook(char* err, size_t errlen, F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        if (err && errlen) {
+            std::strncpy(err, e.what(), errlen - 1);
+            err[errlen - 1] = 0;
+        }
+        return 1;
+    }
+}
+}  // namespace
+
+extern "C" int fem_test_gemm_nt(int m, int n, int k, const double* A, const double* B, double* C, char* err,
+                                size_t errlen) {
+    return hook(err, errlen, [&] {
+        fem::DArray<double> a((size_t)m * k), b((size_t)n * k), c((size_t)m * n);
+        a.upload(A, (size_t)m * k);
+        b.upload(B, (size_t)n * k);
+        fem::dense_gemm_nt(m, n, k, 0.0, a.data(), k, 1.0, b.data(), k, c.data(), n, nullptr);
+        c.download(C, (size_t)m * n);
+        FEM_HIP(hipDeviceSynchronize());
+    });
+}
+
+extern "C" int fem_test_dense_inverse(int n, const double* A, double* Ainv, char* err, size_t errlen) {
+    return hook(err, errlen, [&] {
+        fem::DArray<double> a((size_t)n * n);
+        a.upload(A, (size_t)n * n);
+        fem::DenseCoarse dc;
+        dc.factor(a.data(), n, false, nullptr);
+        // x = inv b for the unit vectors, through the solve path: map = identity
+        std::vector<int64_t> id(n);
+        for (int i = 0; i < n; ++i) id[i] = i;
+        dc.map().resize(n);
+        dc.map().upload(id.data(), n);
+        fem::DArray<double> e(n), x(n);
+        std::vector<double> col(n);
+        for (int j = 0; j < n; ++j) {
+            std::vector<double> unit(n, 0.0);
+            unit[j] = 1.0;
+            e.upload(unit.data(), n);
+            dc.solve<double>(e.data(), x.data(), nullptr);
+            x.download(col.data(), n);
+            FEM_HIP(hipDeviceSynchronize());
+            for (int i = 0; i < n; ++i) Ainv[(size_t)i * n + j] = col[i];
+        }
+    });
+}

@@ -1,0 +1,48 @@
+// The dense coarse level: an FP64 LDL^T factor on the FP64 matrix cores (MFMA), the explicit
+// inverse A^-1 = L^-T D^-1 L^-1 formed from it, and the per-cycle solve as a matrix-vector
+// product with that inverse, in the V-cycle's precision.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "common.hpp"
+
+namespace fem {
+
+class DenseCoarse {
+   public:
+    // Factors the n x n symmetric matrix `a` (FP64, row-major, device memory, overwritten) and
+    // forms its inverse. `fp32` keeps the inverse in FP32 for an FP32 V-cycle. Throws fem::Error
+    // with the reference's "not positive definite" text when a pivot fails.
+    void factor(double* a, int n, bool fp32, hipStream_t s);
+
+    // x[map[i]] = sum_j inv[i][j] b[map[j]] for the n dense unknowns; map holds the grid unknown
+    // of each dense unknown. T is float or double (the storage of the inverse must match).
+    template <class T>
+    void solve(const T* b, T* x, hipStream_t s) const;
+
+    int n() const { return n_; }
+    const DArray<int64_t>& map() const { return map_; }
+    DArray<int64_t>& map() { return map_; }
+    double factor_seconds() const { return factor_seconds_; }
+    double factor_flops() const { return factor_flops_; }
+
+   private:
+    int n_ = 0, np_ = 0;
+    bool fp32_ = false;
+    DArray<double> inv64_;
+    DArray<float> inv32_;
+    DArray<int64_t> map_;
+    double factor_seconds_ = 0.0, factor_flops_ = 0.0;
+};
+
+// The FP64 matrix-core tile product used by the factor, exposed for its unit test:
+// C = alpha C + beta A B^T with A (m x k), B (n x k), C (m x n), all row-major, m, n, k
+// multiples of 64.
+void dense_gemm_nt(int m, int n, int k, double alpha, const double* A, int lda, double beta, const double* B, int ldb,
+                   double* C, int ldc, hipStream_t s);
+
+}  // namespace fem

@@ -1,0 +1,484 @@
+// Kernels of the solve: operator applies fused with the Chebyshev-Jacobi smoother, residuals,
+// grid transfers, FP64 PCG vector updates with block-partial dot products, and conversions.
+// Every reduction is two-stage with a fixed order (block partials, then one block sums them), so
+// a result is bitwise the same from run to run and alone or next to other solves.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ops.hpp"
+
+namespace fem {
+
+constexpr int BLOCK = 256;
+
+// Operators with a uniform interface: C components per node, apply and diagonal of a node row.
+template <class Op>
+struct ScalarOp {
+    static constexpr int C = 1;
+    Op op;
+    __device__ __forceinline__ GridDims dims() const { return op.g; }
+    template <class T>
+    __device__ __forceinline__ void apply(const T* x, int i, int j, int k, T* out) const {
+        out[0] = op.apply(x, i, j, k);
+    }
+    template <class T>
+    __device__ __forceinline__ void diag(int i, int j, int k, T* out) const {
+        out[0] = op.diag(i, j, k);
+    }
+};
+
+template <class Op>
+struct VectorOp {
+    static constexpr int C = 3;
+    Op op;
+    __device__ __forceinline__ GridDims dims() const { return op.g; }
+    template <class T>
+    __device__ __forceinline__ void apply(const T* x, int i, int j, int k, T* out) const {
+        op.apply3(x, i, j, k, out);
+    }
+    template <class T>
+    __device__ __forceinline__ void diag(int i, int j, int k, T* out) const {
+        op.diag3(i, j, k, out);
+    }
+};
+
+__device__ __forceinline__ void node_ijk(const GridDims& g, size_t node, int& i, int& j, int& k) {
+    i = (int)(node % (size_t)g.nx);
+    const size_t r = node / (size_t)g.nx;
+    j = (int)(r % (size_t)g.ny);
+    k = (int)(r / (size_t)g.ny);
+}
+
+// Sum of v over the block, deterministic; the result is valid in thread 0.
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double part[BLOCK / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) part[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < BLOCK / 64; ++w) s += part[w];
+    __syncthreads();  // `part` may be written again by a following call
+    return s;
+}
+
+// ------------------------------------------------------------------ operator kernels
+// y = A x on the unknowns; 0 on fixed nodes.
+template <class OP, class T>
+__global__ void __launch_bounds__(BLOCK) k_apply(OP op, const T* __restrict__ x, T* __restrict__ y) {
+    const GridDims g = op.dims();
+    const size_t node = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    if (node >= g.nodes()) return;
+    int i, j, k;
+    node_ijk(g, node, i, j, k);
+    T out[OP::C];
+    if (is_fixed(g, i, j, k)) {
+#pragma unroll
+        for (int c = 0; c < OP::C; ++c) out[c] = T(0);
+    } else {
+        op.apply(x, i, j, k, out);
+    }
+#pragma unroll
+    for (int c = 0; c < OP::C; ++c) y[OP::C * node + c] = out[c];
+}
+
+// r = b - A x.
+template <class OP, class T>
+__global__ void __launch_bounds__(BLOCK) k_residual(OP op, const T* __restrict__ x, const T* __restrict__ b,
+                                                    T* __restrict__ r) {
+    const GridDims g = op.dims();
+    const size_t node = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    if (node >= g.nodes()) return;
+    int i, j, k;
+    node_ijk(g, node, i, j, k);
+    T out[OP::C];
+    if (is_fixed(g, i, j, k)) {
+#pragma unroll
+        for (int c = 0; c < OP::C; ++c) r[OP::C * node + c] = T(0);
+        return;
+    }
+    op.apply(x, i, j, k, out);
+#pragma unroll
+    for (int c = 0; c < OP::C; ++c) r[OP::C * node + c] = b[OP::C * node + c] - out[c];
+}
+
+// One Chebyshev step from a nonzero x: r = b - A x_in, d = c1 d + c2 D^-1 r, x_out = x_in + d.
+template <class OP, class T>
+__global__ void __launch_bounds__(BLOCK) k_cheb(OP op, const T* __restrict__ xin, T* __restrict__ xout,
+                                                T* __restrict__ d, const T* __restrict__ b,
+                                                const T* __restrict__ invd, T c1, T c2) {
+    const GridDims g = op.dims();
+    const size_t node = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    if (node >= g.nodes()) return;
+    int i, j, k;
+    node_ijk(g, node, i, j, k);
+    if (is_fixed(g, i, j, k)) {
+#pragma unroll
+        for (int c = 0; c < OP::C; ++c) {
+            xout[OP::C * node + c] = T(0);
+            d[OP::C * node + c] = T(0);
+        }
+        return;
+    }
+    T out[OP::C];
+    op.apply(xin, i, j, k, out);
+#pragma unroll
+    for (int c = 0; c < OP::C; ++c) {
+        const size_t u = OP::C * node + c;
+        const T r = b[u] - out[c];
+        const T dn = c1 * d[u] + c2 * invd[u] * r;
+        d[u] = dn;
+        xout[u] = xin[u] + dn;
+    }
+}
+
+// The first Chebyshev step from x = 0: d = c0 D^-1 b, x = d.
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_cheb_first(size_t n, const T* __restrict__ b, const T* __restrict__ invd,
+                                                      T* __restrict__ x, T* __restrict__ d, T c0) {
+    const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    if (u >= n) return;
+    const T v = c0 * invd[u] * b[u];
+    d[u] = v;
+    x[u] = v;
+}
+
+// invd = 1 / diag(A); 0 on fixed nodes.
+template <class OP, class T>
+__global__ void __launch_bounds__(BLOCK) k_inv_diag(OP op, T* __restrict__ invd) {
+    const GridDims g = op.dims();
+    const size_t node = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    if (node >= g.nodes()) return;
+    int i, j, k;
+    node_ijk(g, node, i, j, k);
+    T dg[OP::C];
+    op.diag(i, j, k, dg);
+    const bool fx = is_fixed(g, i, j, k);
+#pragma unroll
+    for (int c = 0; c < OP::C; ++c) invd[OP::C * node + c] = fx ? T(0) : T(1) / dg[c];
+}
+
+// ------------------------------------------------------------------ grid transfers
+// The fine grid has 2 n + 1 nodes where the coarse one has n + 1, on each axis.
+// Trilinear (hex8): fine node f gets coarse f/2 (even) or the mean of floor and ceil (odd),
+// as a tensor product. Kuhn P1 (tet4, and tet10 -> tet4 on the same cells): fine node f is the
+// midpoint of the Kuhn edge from floor(f/2) to ceil(f/2), componentwise.
+template <class T, int C, bool KUHN_P1>
+__global__ void __launch_bounds__(BLOCK) k_prolong_add(GridDims fg, GridDims cg, const T* __restrict__ ec,
+                                                       T* __restrict__ xf) {
+    const size_t node = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    if (node >= fg.nodes()) return;
+    int i, j, k;
+    node_ijk(fg, node, i, j, k);
+    if (is_fixed(fg, i, j, k)) return;
+    T acc[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) acc[c] = T(0);
+    const int lo[3] = {i >> 1, j >> 1, k >> 1};
+    const int hi[3] = {(i + 1) >> 1, (j + 1) >> 1, (k + 1) >> 1};
+    if (KUHN_P1) {
+        const size_t a = lo[0] + (size_t)cg.nx * (lo[1] + (size_t)cg.ny * lo[2]);
+        const size_t b = hi[0] + (size_t)cg.nx * (hi[1] + (size_t)cg.ny * hi[2]);
+        const bool same = a == b;
+#pragma unroll
+        for (int c = 0; c < C; ++c) acc[c] = same ? ec[C * a + c] : T(0.5) * (ec[C * a + c] + ec[C * b + c]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int sx = q & 1, sy = (q >> 1) & 1, sz = q >> 2;
+            const int cx = sx ? hi[0] : lo[0], cy = sy ? hi[1] : lo[1], cz = sz ? hi[2] : lo[2];
+            // weight: 1 for an even coordinate on its one coarse node (counted once), 1/2 for odd
+            const bool dx = (i & 1), dy = (j & 1), dz = (k & 1);
+            if ((!dx && sx) || (!dy && sy) || (!dz && sz)) continue;
+            const T w = T(dx ? 0.5 : 1.0) * T(dy ? 0.5 : 1.0) * T(dz ? 0.5 : 1.0);
+            const size_t cn = cx + (size_t)cg.nx * (cy + (size_t)cg.ny * cz);
+#pragma unroll
+            for (int c = 0; c < C; ++c) acc[c] += w * ec[C * cn + c];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) xf[C * node + c] += acc[c];
+}
+
+// The transpose: b_c = P^T r_f, gathered per coarse node; 0 on fixed coarse nodes.
+template <class T, int C, bool KUHN_P1>
+__global__ void __launch_bounds__(BLOCK) k_restrict(GridDims fg, GridDims cg, const T* __restrict__ rf,
+                                                    T* __restrict__ bc) {
+    const size_t node = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    if (node >= cg.nodes()) return;
+    int I, J, K;
+    node_ijk(cg, node, I, J, K);
+    T acc[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) acc[c] = T(0);
+    if (!is_fixed(cg, I, J, K)) {
+        const int fi = 2 * I, fj = 2 * J, fk = 2 * K;
+        if (KUHN_P1) {
+            // The coarse node itself, and the 14 midpoints of the Kuhn edges at it: directions
+            // +-(1,0,0), +-(0,1,0), +-(0,0,1), +-(1,1,0), +-(1,0,1), +-(0,1,1), +-(1,1,1).
+            const size_t me = fi + (size_t)fg.nx * (fj + (size_t)fg.ny * fk);
+#pragma unroll
+            for (int c = 0; c < C; ++c) acc[c] = rf[C * me + c];
+            const int DIRS[7][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+#pragma unroll
+            for (int s = 0; s < 7; ++s)
+#pragma unroll
+                for (int sign = -1; sign <= 1; sign += 2) {
+                    const int a = fi + sign * DIRS[s][0], b = fj + sign * DIRS[s][1], cz = fk + sign * DIRS[s][2];
+                    if (a < 0 || b < 0 || cz < 0 || a >= fg.nx || b >= fg.ny || cz >= fg.nz) continue;
+                    const size_t f = a + (size_t)fg.nx * (b + (size_t)fg.ny * cz);
+#pragma unroll
+                    for (int c = 0; c < C; ++c) acc[c] += T(0.5) * rf[C * f + c];
+                }
+        } else {
+#pragma unroll
+            for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+                for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int a = fi + dx, b = fj + dy, cz = fk + dz;
+                        if (a < 0 || b < 0 || cz < 0 || a >= fg.nx || b >= fg.ny || cz >= fg.nz) continue;
+                        const T w = T(dx ? 0.5 : 1.0) * T(dy ? 0.5 : 1.0) * T(dz ? 0.5 : 1.0);
+                        const size_t f = a + (size_t)fg.nx * (b + (size_t)fg.ny * cz);
+#pragma unroll
+                        for (int c = 0; c < C; ++c) acc[c] += w * rf[C * f + c];
+                    }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) bc[C * node + c] = acc[c];
+}
+
+// ------------------------------------------------------------------ FP64 PCG pieces
+// Scalars of one PCG solve, in device memory.
+struct PcgScalars {
+    double rz, pq, rr, bb, alpha, beta;
+};
+
+// r = b - A x (FP64 operator); partial sums of b.b and r.r.
+template <class OP>
+__global__ void __launch_bounds__(BLOCK) k_pcg_start(OP op, const double* __restrict__ x, const double* __restrict__ b,
+                                                     double* __restrict__ r, double* __restrict__ part_bb,
+                                                     double* __restrict__ part_rr) {
+    const GridDims g = op.dims();
+    const size_t node = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    double bb = 0.0, rr = 0.0;
+    if (node < g.nodes()) {
+        int i, j, k;
+        node_ijk(g, node, i, j, k);
+        double out[OP::C];
+        const bool fx = is_fixed(g, i, j, k);
+        if (!fx) op.apply(x, i, j, k, out);
+#pragma unroll
+        for (int c = 0; c < OP::C; ++c) {
+            const size_t u = OP::C * node + c;
+            const double rv = fx ? 0.0 : b[u] - out[c];
+            r[u] = rv;
+            bb += b[u] * b[u];
+            rr += rv * rv;
+        }
+    }
+    bb = block_sum(bb);
+    rr = block_sum(rr);
+    if (threadIdx.x == 0) {
+        part_bb[blockIdx.x] = bb;
+        part_rr[blockIdx.x] = rr;
+    }
+}
+
+// q = A p; partial sums of p.q.
+template <class OP>
+__global__ void __launch_bounds__(BLOCK) k_pcg_q(OP op, const double* __restrict__ p, double* __restrict__ q,
+                                                 double* __restrict__ part) {
+    const GridDims g = op.dims();
+    const size_t node = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    double pq = 0.0;
+    if (node < g.nodes()) {
+        int i, j, k;
+        node_ijk(g, node, i, j, k);
+        double out[OP::C];
+        const bool fx = is_fixed(g, i, j, k);
+        if (!fx) op.apply(p, i, j, k, out);
+#pragma unroll
+        for (int c = 0; c < OP::C; ++c) {
+            const size_t u = OP::C * node + c;
+            const double qv = fx ? 0.0 : out[c];
+            q[u] = qv;
+            pq += p[u] * qv;
+        }
+    }
+    pq = block_sum(pq);
+    if (threadIdx.x == 0) part[blockIdx.x] = pq;
+}
+
+// x += alpha p, r -= alpha q, rv = (V) r for the V-cycle; partial sums of r.r.
+template <class V>
+__global__ void __launch_bounds__(BLOCK) k_pcg_update(size_t n, const PcgScalars* __restrict__ s, double* __restrict__ x,
+                                                      double* __restrict__ r, const double* __restrict__ p,
+                                                      const double* __restrict__ q, V* __restrict__ rv,
+                                                      double* __restrict__ part) {
+    const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    double rr = 0.0;
+    if (u < n) {
+        const double alpha = s->alpha;
+        x[u] += alpha * p[u];
+        const double ru = r[u] - alpha * q[u];
+        r[u] = ru;
+        if (rv) rv[u] = (V)ru;
+        rr = ru * ru;
+    }
+    rr = block_sum(rr);
+    if (threadIdx.x == 0) part[blockIdx.x] = rr;
+}
+
+// z = (double) zv (if zv); partial sums of r.z.
+template <class V>
+__global__ void __launch_bounds__(BLOCK) k_pcg_rz(size_t n, const V* __restrict__ zv, double* __restrict__ z,
+                                                  const double* __restrict__ r, double* __restrict__ part) {
+    const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    double rz = 0.0;
+    if (u < n) {
+        double zu;
+        if (zv) {
+            zu = (double)zv[u];
+            z[u] = zu;
+        } else {
+            zu = z[u];
+        }
+        rz = r[u] * zu;
+    }
+    rz = block_sum(rz);
+    if (threadIdx.x == 0) part[blockIdx.x] = rz;
+}
+
+// p = z + beta p (beta = 0 on the first iteration gives p = z).
+__global__ void __launch_bounds__(BLOCK) k_pcg_dir(size_t n, const PcgScalars* __restrict__ s, const double* __restrict__ z,
+                                                   double* __restrict__ p, int first) {
+    const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    if (u >= n) return;
+    p[u] = first ? z[u] : z[u] + s->beta * p[u];
+}
+
+// Sum of `count` partials in a fixed order, by one block of BLOCK threads.
+__device__ __forceinline__ double sum_partials(const double* part, int count) {
+    double v = 0.0;
+    for (int t = threadIdx.x; t < count; t += BLOCK) v += part[t];
+    return block_sum(v);
+}
+
+// what: 0 start (bb, rr), 1 alpha = rz / pq, 2 rr, 3 rz (first: rz = rz'), 4 beta = rz'/rz then rz = rz'
+__global__ void __launch_bounds__(BLOCK) k_pcg_scalars(PcgScalars* s, const double* part_a, const double* part_b,
+                                                       int count, int what) {
+    const double a = sum_partials(part_a, count);
+    double b = 0.0;
+    if (what == 0) b = sum_partials(part_b, count);
+    if (threadIdx.x != 0) return;
+    switch (what) {
+        case 0:
+            s->bb = a;
+            s->rr = b;
+            break;
+        case 1:
+            s->pq = a;
+            s->alpha = s->rz / a;
+            break;
+        case 2:
+            s->rr = a;
+            break;
+        case 3:
+            s->rz = a;
+            break;
+        case 4:
+            s->beta = a / s->rz;
+            s->rz = a;
+            break;
+    }
+}
+
+// ------------------------------------------------------------------ small vector kernels
+template <class S, class D>
+__global__ void __launch_bounds__(BLOCK) k_convert(size_t n, const S* __restrict__ a, D* __restrict__ b) {
+    const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    if (u < n) b[u] = (D)a[u];
+}
+
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_fill(size_t n, T* __restrict__ a, T v) {
+    const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    if (u < n) a[u] = v;
+}
+
+// y = a x + b y.
+__global__ void __launch_bounds__(BLOCK) k_axpby(size_t n, double a, const double* __restrict__ x, double b,
+                                                 double* __restrict__ y) {
+    const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    if (u < n) y[u] = a * x[u] + b * y[u];
+}
+
+// grid[map[k]] = src[k] (the rest of grid is left as it is).
+__global__ void __launch_bounds__(BLOCK) k_scatter(size_t n, const int64_t* __restrict__ map, const double* __restrict__ src,
+                                                   double* __restrict__ grid) {
+    const size_t k = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    if (k < n) grid[map[k]] = src[k];
+}
+
+// dst[k] = grid[map[k]].
+__global__ void __launch_bounds__(BLOCK) k_gather(size_t n, const int64_t* __restrict__ map, const double* __restrict__ grid,
+                                                  double* __restrict__ dst) {
+    const size_t k = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    if (k < n) dst[k] = grid[map[k]];
+}
+
+// Partial sums of a.b (FP64).
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_dot(size_t n, const T* __restrict__ a, const T* __restrict__ b,
+                                               double* __restrict__ part) {
+    double v = 0.0;
+    for (size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x; u < n; u += (size_t)gridDim.x * BLOCK)
+        v += (double)a[u] * (double)b[u];
+    v = block_sum(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// out[0] = sum of the partials.
+__global__ void __launch_bounds__(BLOCK) k_sum(const double* part, int count, double* out) {
+    const double v = sum_partials(part, count);
+    if (threadIdx.x == 0) out[0] = v;
+}
+
+// a = s * a with s = 1 / sqrt(out[0]) (power iteration normalisation).
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_normalise(size_t n, T* __restrict__ a, const double* __restrict__ norm2) {
+    const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    if (u >= n) return;
+    const double s = norm2[0] > 0.0 ? 1.0 / sqrt(norm2[0]) : 0.0;
+    a[u] = (T)(s * (double)a[u]);
+}
+
+// y = D^-1 y (elementwise).
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_scale(size_t n, T* __restrict__ y, const T* __restrict__ invd) {
+    const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    if (u < n) y[u] *= invd[u];
+}
+
+// A deterministic pseudo-random start vector on the unknowns (0 on fixed nodes via invd = 0).
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_random(size_t n, T* __restrict__ a, const T* __restrict__ invd) {
+    const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    if (u >= n) return;
+    uint64_t h = (u + 1) * 0x9E3779B97F4A7C15ull;
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 29;
+    const double v = 0.5 + (double)(h >> 11) * (1.0 / 9007199254740992.0);
+    a[u] = invd[u] != T(0) ? (T)v : T(0);
+}
+
+}  // namespace fem

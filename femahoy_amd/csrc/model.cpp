@@ -1,0 +1,328 @@
+// The structured box model: numbering, free unknowns, materials in grid layout, loads.
+//
+// Numbering rules kept alike with the reference so that the solution vector has the reference's
+// layout:
+//   hex8 points: first seen over the cells (x fastest), corners in VTK order
+//                (crates/fem/src/heat.rs:1165-1208, grid.rs:201-213)
+//   tet points:  the extrusion of a footprint of quads cut at their lowest corner, triangles in
+//                footprint order and layers inside, points first seen (prism.rs:244-284), each
+//                prism split by the smallest-index rule (prism.rs:131-154) and a tetrahedron of
+//                negative volume reordered by swapping its corners 2 and 3 (prism.rs:278-280)
+//   tet10 edges: first seen over the tetrahedra in the order of tet10.rs:38 EDGES, numbered after
+//                the corners (tet10.rs:244-272)
+//   unknowns:    3 node + component (elastic_tet.rs:178-183); the free ones increasing
+//                (heat.rs:503, elastic_tet.rs:291)
+#include "model.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+#include "common.hpp"
+
+namespace fem {
+
+std::array<std::array<int, 4>, 3> prism_split(const std::array<uint64_t, 6>& g) {
+    // Rotations that bring each corner to corner 0 (bottom 0 1 2, top 3 4 5 with 3 above 0).
+    static const int ROT[6][6] = {{0, 1, 2, 3, 4, 5}, {1, 2, 0, 4, 5, 3}, {2, 0, 1, 5, 3, 4},
+                                  {3, 5, 4, 0, 2, 1}, {4, 3, 5, 1, 0, 2}, {5, 4, 3, 2, 1, 0}};
+    static const int CUT_A[3][4] = {{0, 1, 2, 5}, {0, 1, 5, 4}, {0, 4, 5, 3}};
+    static const int CUT_B[3][4] = {{0, 1, 2, 4}, {0, 4, 2, 5}, {0, 4, 5, 3}};
+    int m = 0;
+    for (int k = 1; k < 6; ++k)
+        if (g[k] < g[m]) m = k;
+    const int* r = ROT[m];
+    const bool a = std::min(g[r[1]], g[r[5]]) < std::min(g[r[2]], g[r[4]]);
+    const auto& cut = a ? CUT_A : CUT_B;
+    std::array<std::array<int, 4>, 3> out{};
+    for (int t = 0; t < 3; ++t)
+        for (int v = 0; v < 4; ++v) out[t][v] = r[cut[t][v]];
+    return out;
+}
+
+namespace {
+
+const int VTK_BITS[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+const int TET_EDGES[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+// Kuhn types by their axis order (the same table as the device code).
+const int KUHN_ORDER[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+
+int kuhn_type(int a, int b) {
+    for (int t = 0; t < 6; ++t)
+        if (KUHN_ORDER[t][0] == a && KUHN_ORDER[t][1] == b) return t;
+    return -1;
+}
+
+bool on_face(const uint32_t q[3], const int top[3], int f) {
+    const int d = f / 2;
+    return (f % 2 == 0) ? q[d] == 0 : (int)q[d] == top[d];
+}
+
+}  // namespace
+
+BoxModel build_model(const BoxInput& in) {
+    for (int d = 0; d < 3; ++d) {
+        if (in.n[d] < 1) throw Error("each axis needs 1 cell or more");
+        if (!(in.len[d] > 0.0 && std::isfinite(in.len[d]))) throw Error("the box sizes must be finite numbers above 0");
+    }
+    BoxModel m;
+    m.in = in;
+    const int nx = in.n[0], ny = in.n[1], nz = in.n[2];
+    const size_t cells = (size_t)nx * ny * nz;
+    const bool tets = in.kind != Kind::HeatHex8;
+    m.comps = (in.kind == Kind::ElasticTet4 || in.kind == Kind::ElasticTet10) ? 3 : 1;
+    m.quadratic = in.kind == Kind::ElasticTet10;
+    const int refine = m.quadratic ? 2 : 1;
+    for (int d = 0; d < 3; ++d) {
+        m.gn[d] = refine * in.n[d] + 1;
+        m.h[d] = in.len[d] / (double)in.n[d];
+    }
+    for (int f = 0; f < 6; ++f)
+        if (in.fixed[f]) m.fix |= 1u << f;
+    const size_t need = tets ? 6 * cells : cells;
+    if (in.coef0.size() != need) throw Error(tets ? "one material value for each tetrahedron" : "one conductivity for each cell");
+    if (m.comps == 3 && in.coef1.size() != need) throw Error("one Poisson ratio for each tetrahedron");
+
+    const size_t gnodes = m.grid_nodes();
+    auto gidx = [&](uint32_t i, uint32_t j, uint32_t k) { return i + (size_t)m.gn[0] * (j + (size_t)m.gn[1] * k); };
+
+    // ---------------------------------------------------------------- numbering
+    std::vector<uint32_t> gijk;  // grid point of each reference point, 3 per point
+    // tet index (reference order) -> cell and Kuhn type
+    std::vector<uint32_t> tet_cell;
+    std::vector<uint8_t> tet_type;
+    if (!tets) {
+        std::vector<uint32_t> number(gnodes, UINT32_MAX);
+        for (size_t cell = 0; cell < cells; ++cell) {
+            const uint32_t i = cell % nx, j = (cell / nx) % ny, k = cell / ((size_t)nx * ny);
+            for (const auto& b : VTK_BITS) {
+                const size_t gi = gidx(i + b[0], j + b[1], k + b[2]);
+                if (number[gi] == UINT32_MAX) {
+                    number[gi] = (uint32_t)m.point_grid.size();
+                    m.point_grid.push_back((uint32_t)gi);
+                }
+            }
+        }
+    } else {
+        const size_t nv = (size_t)(nx + 1) * (ny + 1);
+        std::vector<uint32_t> number(nv * (nz + 1), UINT32_MAX);
+        std::vector<std::array<uint32_t, 3>> corner_ijk;
+        std::vector<std::array<uint32_t, 4>> tet_nodes;
+        tet_nodes.reserve(6 * cells);
+        tet_cell.reserve(6 * cells);
+        tet_type.reserve(6 * cells);
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const uint32_t a = i + (nx + 1) * j, b = a + 1, c = a + 1 + (nx + 1), d = a + (nx + 1);
+                const uint32_t tris[2][3] = {{a, b, c}, {a, c, d}};
+                for (const auto& tri : tris)
+                    for (int k = 0; k < nz; ++k) {
+                        std::array<uint64_t, 6> g{};
+                        uint32_t node[6];
+                        for (int cc = 0; cc < 6; ++cc) {
+                            const uint64_t plane = (uint64_t)k + cc / 3;
+                            g[cc] = plane * nv + tri[cc % 3];
+                            uint32_t& slot = number[g[cc]];
+                            if (slot == UINT32_MAX) {
+                                slot = (uint32_t)corner_ijk.size();
+                                const uint32_t v2 = tri[cc % 3];
+                                corner_ijk.push_back({v2 % (uint32_t)(nx + 1), v2 / (uint32_t)(nx + 1), (uint32_t)plane});
+                            }
+                            node[cc] = slot;
+                        }
+                        for (const auto& local : prism_split(g)) {
+                            std::array<uint32_t, 4> t = {node[local[0]], node[local[1]], node[local[2]], node[local[3]]};
+                            // The sign of the volume from the integer grid points (the cell sizes are positive).
+                            long e[3][3];
+                            for (int v = 0; v < 3; ++v)
+                                for (int dd = 0; dd < 3; ++dd)
+                                    e[v][dd] = (long)corner_ijk[t[v + 1]][dd] - (long)corner_ijk[t[0]][dd];
+                            const long det = e[0][0] * (e[1][1] * e[2][2] - e[1][2] * e[2][1]) -
+                                             e[0][1] * (e[1][0] * e[2][2] - e[1][2] * e[2][0]) +
+                                             e[0][2] * (e[1][0] * e[2][1] - e[1][1] * e[2][0]);
+                            if (det < 0) std::swap(t[2], t[3]);
+                            tet_nodes.push_back(t);
+                            // The cell (lowest corner) and the Kuhn type (axis order of the chain).
+                            uint32_t lo[3] = {UINT32_MAX, UINT32_MAX, UINT32_MAX};
+                            for (int v = 0; v < 4; ++v)
+                                for (int dd = 0; dd < 3; ++dd) lo[dd] = std::min(lo[dd], corner_ijk[t[v]][dd]);
+                            int order[4];
+                            int sum[4];
+                            for (int v = 0; v < 4; ++v) {
+                                order[v] = v;
+                                sum[v] = 0;
+                                for (int dd = 0; dd < 3; ++dd) sum[v] += (int)(corner_ijk[t[v]][dd] - lo[dd]);
+                            }
+                            std::sort(order, order + 4, [&](int x, int y) { return sum[x] < sum[y]; });
+                            int axis[2];
+                            for (int s = 0; s < 2; ++s)
+                                for (int dd = 0; dd < 3; ++dd)
+                                    if (corner_ijk[t[order[s + 1]]][dd] != corner_ijk[t[order[s]]][dd]) axis[s] = dd;
+                            const int type = kuhn_type(axis[0], axis[1]);
+                            if (type < 0) throw Error("the box split is not the Kuhn split");
+                            tet_cell.push_back(lo[0] + (uint32_t)nx * (lo[1] + (uint32_t)ny * lo[2]));
+                            tet_type.push_back((uint8_t)type);
+                        }
+                    }
+            }
+        const uint32_t scale = m.quadratic ? 2 : 1;
+        for (const auto& c : corner_ijk) m.point_grid.push_back((uint32_t)gidx(scale * c[0], scale * c[1], scale * c[2]));
+        if (m.quadratic) {
+            std::vector<uint8_t> seen(gnodes, 0);
+            for (const auto& t : tet_nodes)
+                for (const auto& e : TET_EDGES) {
+                    const auto& a = corner_ijk[t[e[0]]];
+                    const auto& b = corner_ijk[t[e[1]]];
+                    const size_t gi = gidx(a[0] + b[0], a[1] + b[1], a[2] + b[2]);
+                    if (!seen[gi]) {
+                        seen[gi] = 1;
+                        m.point_grid.push_back((uint32_t)gi);
+                    }
+                }
+        }
+    }
+    m.points = m.point_grid.size();
+    if (m.points != gnodes) throw Error("the numbering does not cover the grid");
+
+    // ---------------------------------------------------------------- materials
+    if (!tets) {
+        m.cell_k = in.coef0;
+        for (double k : m.cell_k)
+            if (!(k > 0.0 && std::isfinite(k))) throw Error("the conductivity must be a finite number above 0");
+    } else {
+        const int q = m.comps == 3 ? 2 : 1;
+        m.tet_coef.assign((size_t)q * 6 * cells, 0.0);
+        for (size_t t = 0; t < 6 * cells; ++t) {
+            const size_t cell = tet_cell[t], type = tet_type[t];
+            if (m.comps == 1) {
+                const double k = in.coef0[t];
+                if (!(k > 0.0 && std::isfinite(k))) throw Error("the conductivity must be a finite number above 0");
+                m.tet_coef[type * cells + cell] = k;
+            } else {
+                const double e = in.coef0[t], nu = in.coef1[t];
+                if (!(std::isfinite(e) && e > 0.0) || !std::isfinite(nu))
+                    throw Error("the moduli must be above 0 and the Poisson ratios finite");
+                if (!(nu > -1.0 && nu < 0.5))
+                    throw Error(
+                        "the Poisson ratios give no stable material (for an isotropic material, nu must be between "
+                        "-1 and 0.5)");
+                const double mu = e / (2.0 * (1.0 + nu));
+                const double lam = e * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+                m.tet_coef[(0 * 6 + type) * cells + cell] = lam;
+                m.tet_coef[(1 * 6 + type) * cells + cell] = mu;
+            }
+        }
+    }
+
+    // ---------------------------------------------------------------- free unknowns
+    const int top[3] = {m.gn[0] - 1, m.gn[1] - 1, m.gn[2] - 1};
+    std::vector<uint8_t> node_fixed(gnodes, 0);
+    m.fixed_value.assign(gnodes * m.comps, 0.0);
+    for (size_t gi = 0; gi < gnodes; ++gi) {
+        const uint32_t q[3] = {(uint32_t)(gi % m.gn[0]), (uint32_t)((gi / m.gn[0]) % m.gn[1]),
+                               (uint32_t)(gi / ((size_t)m.gn[0] * m.gn[1]))};
+        for (int f = 0; f < 6; ++f)
+            if (in.fixed[f] && on_face(q, top, f)) {
+                node_fixed[gi] = 1;
+                if (m.comps == 1) m.fixed_value[gi] = in.value[f];
+            }
+    }
+    for (size_t p = 0; p < m.points; ++p) {
+        const uint32_t gi = m.point_grid[p];
+        if (node_fixed[gi]) continue;
+        for (int c = 0; c < m.comps; ++c) {
+            m.free.push_back((int64_t)(m.comps * p + c));
+            m.free_grid.push_back((int64_t)(m.comps * (size_t)gi + c));
+        }
+    }
+
+    // ---------------------------------------------------------------- loads
+    m.load.assign(gnodes * m.comps, 0.0);
+    const double hx = m.h[0], hy = m.h[1], hz = m.h[2];
+    if (in.kind == Kind::HeatHex8 && in.source != 0.0) {
+        const double v = hx * hy * hz;
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i)
+                    for (const auto& b : VTK_BITS) m.load[gidx(i + b[0], j + b[1], k + b[2])] += in.source * v / 8.0;
+    }
+    if (in.kind == Kind::HeatTet4 && in.source != 0.0) {
+        const double v = hx * hy * hz / 6.0;
+        // Each cell has 6 tetrahedra; corner 0 and corner 7 are in all 6, the others in 2.
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i)
+                    for (int t = 0; t < 6; ++t) {
+                        int bits[3] = {0, 0, 0};
+                        for (int vv = 0; vv < 4; ++vv) {
+                            if (vv > 0) bits[KUHN_ORDER[t][vv - 1]] = 1;
+                            m.load[gidx(i + bits[0], j + bits[1], k + bits[2])] += in.source * v / 4.0;
+                        }
+                    }
+    }
+    if (m.comps == 3) {
+        const bool body = in.body[0] != 0.0 || in.body[1] != 0.0 || in.body[2] != 0.0;
+        const double v = hx * hy * hz / 6.0;
+        const uint32_t s = m.quadratic ? 2 : 1;
+        if (body)
+            for (int k = 0; k < nz; ++k)
+                for (int j = 0; j < ny; ++j)
+                    for (int i = 0; i < nx; ++i)
+                        for (int t = 0; t < 6; ++t) {
+                            uint32_t vx[4][3];
+                            int bits[3] = {0, 0, 0};
+                            for (int vv = 0; vv < 4; ++vv) {
+                                if (vv > 0) bits[KUHN_ORDER[t][vv - 1]] = 1;
+                                for (int dd = 0; dd < 3; ++dd) vx[vv][dd] = (uint32_t)(dd == 0 ? i : dd == 1 ? j : k) + bits[dd];
+                            }
+                            for (int vv = 0; vv < 4; ++vv) {
+                                const size_t gi = gidx(s * vx[vv][0], s * vx[vv][1], s * vx[vv][2]);
+                                const double w = m.quadratic ? -v / 20.0 : v / 4.0;
+                                for (int dd = 0; dd < 3; ++dd) m.load[3 * gi + dd] += in.body[dd] * w;
+                            }
+                            if (m.quadratic)
+                                for (const auto& e : TET_EDGES) {
+                                    const size_t gi = gidx(vx[e[0]][0] + vx[e[1]][0], vx[e[0]][1] + vx[e[1]][1],
+                                                           vx[e[0]][2] + vx[e[1]][2]);
+                                    for (int dd = 0; dd < 3; ++dd) m.load[3 * gi + dd] += in.body[dd] * v / 5.0;
+                                }
+                        }
+        // Face tractions: each boundary quad is 2 triangles cut along the diagonal from its lowest
+        // corner; each triangle puts A/3 on its corners (tet4) or its edge points (tet10).
+        for (int f = 0; f < 6; ++f) {
+            const double* t = in.traction[f];
+            if (t[0] == 0.0 && t[1] == 0.0 && t[2] == 0.0) continue;
+            const int d = f / 2, u = d == 0 ? 1 : 0, w = d == 2 ? 1 : 2;
+            const uint32_t plane = (f % 2 == 0) ? 0 : (uint32_t)in.n[d];
+            const double area = 0.5 * m.h[u] * m.h[w];
+            for (int b = 0; b < in.n[w]; ++b)
+                for (int a = 0; a < in.n[u]; ++a) {
+                    // corners of the quad in (u, w): p00, p10, p11, p01; triangles (p00,p10,p11), (p00,p11,p01)
+                    auto corner = [&](int du, int dw) {
+                        std::array<uint32_t, 3> q{};
+                        q[d] = plane;
+                        q[u] = a + du;
+                        q[w] = b + dw;
+                        return q;
+                    };
+                    const std::array<uint32_t, 3> tri[2][3] = {{corner(0, 0), corner(1, 0), corner(1, 1)},
+                                                               {corner(0, 0), corner(1, 1), corner(0, 1)}};
+                    for (const auto& tr : tri) {
+                        for (int s2 = 0; s2 < 3; ++s2) {
+                            size_t gi;
+                            if (m.quadratic) {
+                                const auto& p = tr[s2];
+                                const auto& q2 = tr[(s2 + 1) % 3];
+                                gi = gidx(p[0] + q2[0], p[1] + q2[1], p[2] + q2[2]);
+                            } else {
+                                gi = gidx(tr[s2][0], tr[s2][1], tr[s2][2]);
+                            }
+                            for (int dd = 0; dd < 3; ++dd) m.load[3 * gi + dd] += t[dd] * area / 3.0;
+                        }
+                    }
+                }
+        }
+    }
+    return m;
+}
+
+}  // namespace fem

@@ -1,0 +1,564 @@
+// Device-side operators of the structured box models, node-centric: each thread computes the
+// whole row of one grid node, so no output is shared between threads (no atomics, bitwise
+// reproducible results).
+//
+// Grid conventions: node (i, j, k) of a grid with nx x ny x nz nodes has index i + nx (j + ny k).
+// A vector field stores the 3 components of a node side by side (unknown 3 node + c). A face of
+// the box that is fixed (bit f of `fix`: 0 x=0, 1 x=max, 2 y=0, 3 y=max, 4 z=0, 5 z=max) holds
+// no unknowns: every kernel writes 0 there, so the vectors are 0 on it and the operators read
+// such neighbours as 0 without a test.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace fem {
+
+struct GridDims {
+    int nx, ny, nz;  // nodes per axis
+    uint32_t fix;    // fixed faces
+    __host__ __device__ size_t nodes() const { return (size_t)nx * ny * nz; }
+};
+
+__device__ __forceinline__ bool is_fixed(const GridDims& g, int i, int j, int k) {
+    return ((g.fix & 1u) && i == 0) || ((g.fix & 2u) && i == g.nx - 1) || ((g.fix & 4u) && j == 0) ||
+           ((g.fix & 8u) && j == g.ny - 1) || ((g.fix & 16u) && k == 0) || ((g.fix & 32u) && k == g.nz - 1);
+}
+
+// ------------------------------------------------------------------ hex8 heat, scalar k per cell
+// The 8 x 8 conduction matrix of a box cell (heat.rs:1246-1278) depends only on which axes the
+// two corners differ along: entry (p, q) = k * c[(p xor q) as bits x|y<<1|z<<2]. The corner bits
+// here are (x, y, z) in {0,1}^3, not the VTK order; the matrix is the same operator.
+template <class T>
+struct Hex8K {
+    GridDims g;
+    const T* k;  // per cell, (nx-1)(ny-1)(nz-1), x fastest
+    T c[8];
+
+    __device__ __forceinline__ T cell_k(int ex, int ey, int ez) const {
+        if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx - 1 || ey >= g.ny - 1 || ez >= g.nz - 1) return T(0);
+        return k[ex + (size_t)(g.nx - 1) * (ey + (size_t)(g.ny - 1) * ez)];
+    }
+
+    __device__ __forceinline__ T apply(const T* x, int i, int j, int kk) const {
+        T xv[27];
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx) {
+                    const int a = i + dx - 1, b = j + dy - 1, c2 = kk + dz - 1;
+                    const bool in = a >= 0 && b >= 0 && c2 >= 0 && a < g.nx && b < g.ny && c2 < g.nz;
+                    xv[dx + 3 * dy + 9 * dz] = in ? x[a + (size_t)g.nx * (b + (size_t)g.ny * c2)] : T(0);
+                }
+        T acc = T(0);
+#pragma unroll
+        for (int cz = 0; cz < 2; ++cz)
+#pragma unroll
+            for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+                for (int cx = 0; cx < 2; ++cx) {
+                    const T ke = cell_k(i - 1 + cx, j - 1 + cy, kk - 1 + cz);
+                    const int lx = 1 - cx, ly = 1 - cy, lz = 1 - cz;
+                    T s = T(0);
+#pragma unroll
+                    for (int qz = 0; qz < 2; ++qz)
+#pragma unroll
+                        for (int qy = 0; qy < 2; ++qy)
+#pragma unroll
+                            for (int qx = 0; qx < 2; ++qx)
+                                s += c[(qx ^ lx) | ((qy ^ ly) << 1) | ((qz ^ lz) << 2)] *
+                                     xv[(cx + qx) + 3 * (cy + qy) + 9 * (cz + qz)];
+                    acc += ke * s;
+                }
+        return acc;
+    }
+
+    __device__ __forceinline__ T diag(int i, int j, int kk) const {
+        T s = T(0);
+#pragma unroll
+        for (int cz = 0; cz < 2; ++cz)
+#pragma unroll
+            for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+                for (int cx = 0; cx < 2; ++cx) s += cell_k(i - 1 + cx, j - 1 + cy, kk - 1 + cz);
+        return s * c[0];
+    }
+};
+
+// ------------------------------------------------------------------ 27-point symmetric stencil
+// The Galerkin coarse operators of the hex8 hierarchy. Coefficient s of node i couples i with
+// i + OFF[s] (s = 0 the diagonal, s = 1..13 the forward half); the backward half is read from
+// the neighbour's forward coefficient. Storage is s-major: a[s * nodes + i].
+__constant__ static const int STENCIL_OFF[14][3] = {
+    {0, 0, 0},  {1, 0, 0},  {-1, 1, 0}, {0, 1, 0}, {1, 1, 0}, {-1, -1, 1}, {0, -1, 1},
+    {1, -1, 1}, {-1, 0, 1}, {0, 0, 1},  {1, 0, 1}, {-1, 1, 1}, {0, 1, 1},  {1, 1, 1}};
+
+template <class T>
+struct Stencil27 {
+    GridDims g;
+    const T* a;
+
+    __device__ __forceinline__ T apply(const T* x, int i, int j, int k) const {
+        const size_t n = g.nodes();
+        const size_t me = i + (size_t)g.nx * (j + (size_t)g.ny * k);
+        T acc = a[me] * x[me];
+#pragma unroll
+        for (int s = 1; s < 14; ++s) {
+            const int dx = STENCIL_OFF[s][0], dy = STENCIL_OFF[s][1], dz = STENCIL_OFF[s][2];
+            const long off = dx + (long)g.nx * (dy + (long)g.ny * dz);
+            const int fi = i + dx, fj = j + dy, fk = k + dz;
+            if (fi >= 0 && fj >= 0 && fk >= 0 && fi < g.nx && fj < g.ny && fk < g.nz)
+                acc += a[s * n + me] * x[me + off];
+            const int bi = i - dx, bj = j - dy, bk = k - dz;
+            if (bi >= 0 && bj >= 0 && bk >= 0 && bi < g.nx && bj < g.ny && bk < g.nz)
+                acc += a[s * n + me - off] * x[me - off];
+        }
+        return acc;
+    }
+
+    __device__ __forceinline__ T diag(int i, int j, int k) const {
+        return a[i + (size_t)g.nx * (j + (size_t)g.ny * k)];
+    }
+};
+
+// ------------------------------------------------------------------ Kuhn tetrahedra
+// The reference's prism split of a structured box (prism.rs:144-154 on a footprint of quads cut
+// at their lowest corner) gives, in each cell, the 6 tetrahedra of the Kuhn split: the chains
+// 0 -> e_a -> e_a + e_b -> (1,1,1) for the 6 orders (a, b, c) of the axes. Type t has the axis
+// order KUHN[t]. Its barycentric gradients are -e_a/h_a, e_a/h_a - e_b/h_b, e_b/h_b - e_c/h_c,
+// e_c/h_c, and its volume hx hy hz / 6.
+__constant__ static const int KUHN[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+
+// The vertex of a Kuhn tetrahedron of type t at cell corner (bx, by, bz), or -1.
+__host__ __device__ constexpr int kuhn_vertex(int t, int bx, int by, int bz) {
+    const int a = t / 2 == 0 ? 0 : (t / 2 == 1 ? 1 : 2);
+    const int b = t == 0 ? 1 : t == 1 ? 2 : t == 2 ? 0 : t == 3 ? 2 : t == 4 ? 0 : 1;
+    const int bits[3] = {bx, by, bz};
+    const int n = bx + by + bz;
+    if (n == 0) return 0;
+    if (n == 3) return 3;
+    if (n == 1) return bits[a] ? 1 : -1;
+    return (bits[a] && bits[b]) ? 2 : -1;
+}
+
+// The gradient of barycentric coordinate v of a Kuhn tetrahedron of type t, with inverse cell
+// sizes ih.
+template <class T>
+__device__ __forceinline__ void kuhn_grad(int t, int v, const T ih[3], T g[3]) {
+    const int a = KUHN[t][0], b = KUHN[t][1], c = KUHN[t][2];
+    g[0] = g[1] = g[2] = T(0);
+    if (v == 0) {
+        g[a] = -ih[a];
+    } else if (v == 1) {
+        g[a] = ih[a];
+        g[b] = -ih[b];
+    } else if (v == 2) {
+        g[b] = ih[b];
+        g[c] = -ih[c];
+    } else {
+        g[c] = ih[c];
+    }
+}
+
+// Scalar P1 conduction on Kuhn tetrahedra (heat.rs:972-1000): k per tetrahedron, stored
+// coef[t * cells + cell] for type t.
+template <class T>
+struct Tet4Heat {
+    GridDims g;
+    const T* coef;
+    T ih[3];
+    T vol;
+
+    __device__ __forceinline__ T apply(const T* x, int i, int j, int kk) const {
+        const size_t cells = (size_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1);
+        T acc = T(0);
+#pragma unroll
+        for (int cz = 0; cz < 2; ++cz)
+#pragma unroll
+            for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+                for (int cx = 0; cx < 2; ++cx) {
+                    const int ex = i - 1 + cx, ey = j - 1 + cy, ez = kk - 1 + cz;
+                    if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx - 1 || ey >= g.ny - 1 || ez >= g.nz - 1) continue;
+                    const size_t cell = ex + (size_t)(g.nx - 1) * (ey + (size_t)(g.ny - 1) * ez);
+                    T xc[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        xc[q] = x[(ex + (q & 1)) + (size_t)g.nx * ((ey + ((q >> 1) & 1)) + (size_t)g.ny * (ez + (q >> 2)))];
+                    const int lx = 1 - cx, ly = 1 - cy, lz = 1 - cz;
+#pragma unroll
+                    for (int t = 0; t < 6; ++t) {
+                        const int me = kuhn_vertex(t, lx, ly, lz);
+                        if (me < 0) continue;
+                        // grad u = sum over the 4 vertices of u_v g_v
+                        T gu[3] = {T(0), T(0), T(0)};
+                        int bits[3] = {0, 0, 0};
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            if (v > 0) bits[KUHN[t][v - 1]] = 1;
+                            const T uv = xc[bits[0] | (bits[1] << 1) | (bits[2] << 2)];
+                            T gv[3];
+                            kuhn_grad<T>(t, v, ih, gv);
+                            gu[0] += uv * gv[0];
+                            gu[1] += uv * gv[1];
+                            gu[2] += uv * gv[2];
+                        }
+                        T gm[3];
+                        kuhn_grad<T>(t, me, ih, gm);
+                        acc += coef[t * cells + cell] * vol * (gm[0] * gu[0] + gm[1] * gu[1] + gm[2] * gu[2]);
+                    }
+                }
+        return acc;
+    }
+
+    __device__ __forceinline__ T diag(int i, int j, int kk) const {
+        const size_t cells = (size_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1);
+        T acc = T(0);
+        for (int cz = 0; cz < 2; ++cz)
+            for (int cy = 0; cy < 2; ++cy)
+                for (int cx = 0; cx < 2; ++cx) {
+                    const int ex = i - 1 + cx, ey = j - 1 + cy, ez = kk - 1 + cz;
+                    if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx - 1 || ey >= g.ny - 1 || ez >= g.nz - 1) continue;
+                    const size_t cell = ex + (size_t)(g.nx - 1) * (ey + (size_t)(g.ny - 1) * ez);
+                    for (int t = 0; t < 6; ++t) {
+                        const int me = kuhn_vertex(t, 1 - cx, 1 - cy, 1 - cz);
+                        if (me < 0) continue;
+                        T gm[3];
+                        kuhn_grad<T>(t, me, ih, gm);
+                        acc += coef[t * cells + cell] * vol * (gm[0] * gm[0] + gm[1] * gm[1] + gm[2] * gm[2]);
+                    }
+                }
+        return acc;
+    }
+};
+
+// Isotropic linear elasticity on Kuhn tetrahedra, constant strain (the operator of
+// tet10.rs:90-120 with constant gradients): lambda and mu per tetrahedron, stored
+// coef[(0 * 6 + t) * cells + cell] (lambda) and coef[(1 * 6 + t) * cells + cell] (mu).
+template <class T>
+struct Tet4Elastic {
+    GridDims g;
+    const T* coef;
+    T ih[3];
+    T vol;
+
+    __device__ __forceinline__ void apply3(const T* x, int i, int j, int kk, T out[3]) const {
+        const size_t cells = (size_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1);
+        out[0] = out[1] = out[2] = T(0);
+#pragma unroll
+        for (int cz = 0; cz < 2; ++cz)
+#pragma unroll
+            for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+                for (int cx = 0; cx < 2; ++cx) {
+                    const int ex = i - 1 + cx, ey = j - 1 + cy, ez = kk - 1 + cz;
+                    if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx - 1 || ey >= g.ny - 1 || ez >= g.nz - 1) continue;
+                    const size_t cell = ex + (size_t)(g.nx - 1) * (ey + (size_t)(g.ny - 1) * ez);
+                    const int lx = 1 - cx, ly = 1 - cy, lz = 1 - cz;
+#pragma unroll
+                    for (int t = 0; t < 6; ++t) {
+                        const int me = kuhn_vertex(t, lx, ly, lz);
+                        if (me < 0) continue;
+                        // G[a][b] = d u_a / d x_b = sum_v u_va g_v[b]
+                        T G[3][3] = {{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, {T(0), T(0), T(0)}};
+                        int bits[3] = {0, 0, 0};
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            if (v > 0) bits[KUHN[t][v - 1]] = 1;
+                            const size_t node =
+                                (ex + bits[0]) + (size_t)g.nx * ((ey + bits[1]) + (size_t)g.ny * (ez + bits[2]));
+                            T gv[3];
+                            kuhn_grad<T>(t, v, ih, gv);
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) {
+                                const T u = x[3 * node + a];
+#pragma unroll
+                                for (int b = 0; b < 3; ++b) G[a][b] += u * gv[b];
+                            }
+                        }
+                        const T lam = coef[(0 * 6 + t) * cells + cell];
+                        const T mu = coef[(1 * 6 + t) * cells + cell];
+                        const T tr = G[0][0] + G[1][1] + G[2][2];
+                        T gm[3];
+                        kuhn_grad<T>(t, me, ih, gm);
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            T s = T(0);
+#pragma unroll
+                            for (int b = 0; b < 3; ++b) {
+                                const T sig = (a == b ? lam * tr : T(0)) + mu * (G[a][b] + G[b][a]);
+                                s += sig * gm[b];
+                            }
+                            out[a] += vol * s;
+                        }
+                    }
+                }
+    }
+
+    __device__ __forceinline__ void diag3(int i, int j, int kk, T out[3]) const {
+        const size_t cells = (size_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1);
+        out[0] = out[1] = out[2] = T(0);
+        for (int cz = 0; cz < 2; ++cz)
+            for (int cy = 0; cy < 2; ++cy)
+                for (int cx = 0; cx < 2; ++cx) {
+                    const int ex = i - 1 + cx, ey = j - 1 + cy, ez = kk - 1 + cz;
+                    if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx - 1 || ey >= g.ny - 1 || ez >= g.nz - 1) continue;
+                    const size_t cell = ex + (size_t)(g.nx - 1) * (ey + (size_t)(g.ny - 1) * ez);
+                    for (int t = 0; t < 6; ++t) {
+                        const int me = kuhn_vertex(t, 1 - cx, 1 - cy, 1 - cz);
+                        if (me < 0) continue;
+                        T gm[3];
+                        kuhn_grad<T>(t, me, ih, gm);
+                        const T lam = coef[(0 * 6 + t) * cells + cell];
+                        const T mu = coef[(1 * 6 + t) * cells + cell];
+                        const T g2 = gm[0] * gm[0] + gm[1] * gm[1] + gm[2] * gm[2];
+                        for (int a = 0; a < 3; ++a) out[a] += vol * ((lam + mu) * gm[a] * gm[a] + mu * g2);
+                    }
+                }
+    }
+};
+
+// ------------------------------------------------------------------ tet10 on Kuhn cells
+// The 10 nodes of a Kuhn tetrahedron sit on the grid refined twice: its corners at 2 v and its
+// edge points at v_a + v_b, with the chain vertices v_0..v_3 (in cell bits). Local nodes 0..3
+// are the corners, 4..9 the edges (0,1), (1,2), (0,2), (0,3), (1,3), (2,3) (tet10.rs:38).
+//
+// The operator runs in 2 phases. Phase A, one thread per tetrahedron: the element product of
+// tet10.rs:90-120 (the 4-point rule of degree 2, exact for affine cells) written to an element
+// buffer E, 30 values per tetrahedron. Phase B, one thread per node: the sum of the entries of E
+// at that node, over the tetrahedra that hold it, in a fixed order. No atomics, so the sums are
+// bitwise reproducible.
+__constant__ static const int TET10_EDGE[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+
+// Chain vertex v (0..3) of Kuhn type t, as cell bits.
+__host__ __device__ inline void kuhn_chain(int t, int v, int bits[3]) {
+    const int order[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    bits[0] = bits[1] = bits[2] = 0;
+    for (int s = 0; s < v; ++s) bits[order[t][s]] = 1;
+}
+
+// The local node (0..9) of Kuhn type t at refined cell position (px, py, pz) in {0,1,2}^3, or -1.
+__host__ __device__ inline int tet10_local(int t, int px, int py, int pz) {
+    int vb[4][3];
+    for (int v = 0; v < 4; ++v) kuhn_chain(t, v, vb[v]);
+    for (int v = 0; v < 4; ++v)
+        if (2 * vb[v][0] == px && 2 * vb[v][1] == py && 2 * vb[v][2] == pz) return v;
+    const int E[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+    for (int e = 0; e < 6; ++e) {
+        const int* a = vb[E[e][0]];
+        const int* b = vb[E[e][1]];
+        if (a[0] + b[0] == px && a[1] + b[1] == py && a[2] + b[2] == pz) return 4 + e;
+    }
+    return -1;
+}
+
+// The element product y = K u of one tet10 with isotropic lambda, mu (tet10.rs:90-120), through
+// the vertex gradients: grad u is linear in the barycentric coordinates L, grad u(L) = sum_k L_k
+// G_k with G_k = 4 u_k g_k^T + 4 sum_{j != k} u_kj g_j^T - sum_i u_i g_i^T, so each quadrature
+// point costs one 3 x 3 combination, and the adjoint collects sum_q w L_k(q) sigma(q).
+template <class T>
+__device__ __forceinline__ void tet10_element(const T u[30], const T g[4][3], T lam, T mu, T w, T y[30]) {
+    const T qa = T(0.5854101966249685), qb = T(0.1381966011250105);  // tet10.rs:45-46
+    T Cm[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) Cm[a][b] = u[a] * g[0][b] + u[3 + a] * g[1][b] + u[6 + a] * g[2][b] + u[9 + a] * g[3][b];
+    // edge index of (k, j), j != k
+    const int EI[4][4] = {{-1, 4, 6, 7}, {4, -1, 5, 8}, {6, 5, -1, 9}, {7, 8, 9, -1}};
+    T G[4][3][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                T s = u[3 * k + a] * g[k][b];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (j != k) s += u[3 * EI[k][j] + a] * g[j][b];
+                G[k][a][b] = T(4) * s - Cm[a][b];
+            }
+    T S[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) S[a][b] = G[0][a][b] + G[1][a][b] + G[2][a][b] + G[3][a][b];
+    // sigma at each quadrature point q (L_q = qa, others qb), symmetric: 6 entries
+    T sig[4][6];
+    T ssum[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        T D[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) D[a][b] = qb * S[a][b] + (qa - qb) * G[q][a][b];
+        const T tr = lam * (D[0][0] + D[1][1] + D[2][2]);
+        sig[q][0] = tr + T(2) * mu * D[0][0];
+        sig[q][1] = tr + T(2) * mu * D[1][1];
+        sig[q][2] = tr + T(2) * mu * D[2][2];
+        sig[q][3] = mu * (D[1][2] + D[2][1]);  // yz
+        sig[q][4] = mu * (D[0][2] + D[2][0]);  // xz
+        sig[q][5] = mu * (D[0][1] + D[1][0]);  // xy
+#pragma unroll
+        for (int s = 0; s < 6; ++s) ssum[s] += sig[q][s];
+    }
+    // Sigma'_k = w (qb sum_q sigma + (qa - qb) sigma_k); S' = w sum_q sigma
+    T Sp[4][6];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int s = 0; s < 6; ++s) Sp[k][s] = w * (qb * ssum[s] + (qa - qb) * sig[k][s]);
+    T S0[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) S0[s] = w * ssum[s];
+    // sym 3x3 (xx, yy, zz, yz, xz, xy) times a vector
+    auto mv = [](const T* m, const T* v, T* o) {
+        o[0] = m[0] * v[0] + m[5] * v[1] + m[4] * v[2];
+        o[1] = m[5] * v[0] + m[1] * v[1] + m[3] * v[2];
+        o[2] = m[4] * v[0] + m[3] * v[1] + m[2] * v[2];
+    };
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        T m[6];
+#pragma unroll
+        for (int s = 0; s < 6; ++s) m[s] = T(4) * Sp[k][s] - S0[s];
+        mv(m, g[k], &y[3 * k]);
+    }
+#pragma unroll
+    for (int e = 0; e < 6; ++e) {
+        const int k = TET10_EDGE[e][0], j = TET10_EDGE[e][1];
+        T o1[3], o2[3];
+        mv(Sp[k], g[j], o1);
+        mv(Sp[j], g[k], o2);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) y[3 * (4 + e) + a] = T(4) * (o1[a] + o2[a]);
+    }
+}
+
+// Phase B of the tet10 operator, node-centric: the sum of the element buffer at a node.
+// coef: lambda at [t * cells + cell], mu at [(6 + t) * cells + cell] (cells of the coarse grid);
+// E: [(t * 30 + 3 * local + c) * cells + cell].
+template <class T>
+struct Tet10Op {
+    GridDims g;  // refined grid: 2 n + 1 nodes per axis
+    const T* coef;
+    const T* E;
+    T ih[3];
+    T vol;
+
+    __device__ __forceinline__ void apply3(const T* /*x, read by phase A*/, int i, int j, int k, T out[3]) const {
+        const int cx = (g.nx - 1) / 2, cy = (g.ny - 1) / 2, cz = (g.nz - 1) / 2;
+        const size_t cells = (size_t)cx * cy * cz;
+        out[0] = out[1] = out[2] = T(0);
+        const int pos[3] = {i, j, k};
+        const int ncell[3] = {cx, cy, cz};
+        int lo[3], hi[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (pos[d] & 1) {
+                lo[d] = hi[d] = (pos[d] - 1) / 2;
+            } else {
+                lo[d] = pos[d] / 2 - 1;
+                hi[d] = pos[d] / 2;
+                if (lo[d] < 0) lo[d] = 0;
+                if (hi[d] > ncell[d] - 1) hi[d] = ncell[d] - 1;
+            }
+        }
+        for (int ez = lo[2]; ez <= hi[2]; ++ez)
+            for (int ey = lo[1]; ey <= hi[1]; ++ey)
+                for (int ex = lo[0]; ex <= hi[0]; ++ex) {
+                    const size_t cell = ex + (size_t)cx * (ey + (size_t)cy * ez);
+                    const int px = i - 2 * ex, py = j - 2 * ey, pz = k - 2 * ez;
+#pragma unroll
+                    for (int t = 0; t < 6; ++t) {
+                        const int loc = tet10_local(t, px, py, pz);
+                        if (loc < 0) continue;
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) out[c] += E[((size_t)(t * 30 + 3 * loc + c)) * cells + cell];
+                    }
+                }
+    }
+
+    // The diagonal, recomputed from the element rule (setup only).
+    __device__ void diag3(int i, int j, int k, T out[3]) const {
+        const int cx = (g.nx - 1) / 2, cy = (g.ny - 1) / 2, cz = (g.nz - 1) / 2;
+        const size_t cells = (size_t)cx * cy * cz;
+        out[0] = out[1] = out[2] = T(0);
+        const T qa = T(0.5854101966249685), qb = T(0.1381966011250105);
+        for (int ez = 0; ez < cz; ++ez) {
+            if (k < 2 * ez || k > 2 * ez + 2) continue;
+            for (int ey = 0; ey < cy; ++ey) {
+                if (j < 2 * ey || j > 2 * ey + 2) continue;
+                for (int ex = 0; ex < cx; ++ex) {
+                    if (i < 2 * ex || i > 2 * ex + 2) continue;
+                    const size_t cell = ex + (size_t)cx * (ey + (size_t)cy * ez);
+                    for (int t = 0; t < 6; ++t) {
+                        const int loc = tet10_local(t, i - 2 * ex, j - 2 * ey, k - 2 * ez);
+                        if (loc < 0) continue;
+                        T gg[4][3];
+                        for (int v = 0; v < 4; ++v) kuhn_grad<T>(t, v, ih, gg[v]);
+                        const T lam = coef[t * cells + cell], mu = coef[(6 + t) * cells + cell];
+                        for (int q = 0; q < 4; ++q) {
+                            T L[4];
+                            for (int v = 0; v < 4; ++v) L[v] = v == q ? qa : qb;
+                            T dn[3];
+                            if (loc < 4) {
+                                for (int a = 0; a < 3; ++a) dn[a] = (T(4) * L[loc] - T(1)) * gg[loc][a];
+                            } else {
+                                const int p = TET10_EDGE[loc - 4][0], r = TET10_EDGE[loc - 4][1];
+                                for (int a = 0; a < 3; ++a) dn[a] = T(4) * (L[p] * gg[r][a] + L[r] * gg[p][a]);
+                            }
+                            const T n2 = dn[0] * dn[0] + dn[1] * dn[1] + dn[2] * dn[2];
+                            for (int a = 0; a < 3; ++a) out[a] += vol / T(4) * ((lam + mu) * dn[a] * dn[a] + mu * n2);
+                        }
+                    }
+                }
+            }
+        }
+    }
+};
+
+// Phase A: one thread per (cell, Kuhn type), cell fastest.
+template <class T>
+__global__ void __launch_bounds__(256) k_tet10_elements(GridDims g, const T* __restrict__ coef, const T* __restrict__ x,
+                                                        T* __restrict__ E, T ih0, T ih1, T ih2, T vol) {
+    const int cx = (g.nx - 1) / 2, cy = (g.ny - 1) / 2, cz = (g.nz - 1) / 2;
+    const size_t cells = (size_t)cx * cy * cz;
+    const size_t tid = blockIdx.x * (size_t)256 + threadIdx.x;
+    if (tid >= 6 * cells) return;
+    const int t = (int)(tid / cells);
+    const size_t cell = tid % cells;
+    const int ex = (int)(cell % cx), ey = (int)((cell / cx) % cy), ez = (int)(cell / ((size_t)cx * cy));
+    int vb[4][3];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) kuhn_chain(t, v, vb[v]);
+    T u[30];
+#pragma unroll
+    for (int n = 0; n < 10; ++n) {
+        int p[3];
+        if (n < 4) {
+            for (int d = 0; d < 3; ++d) p[d] = 2 * vb[n][d];
+        } else {
+            const int a = TET10_EDGE[n - 4][0], b = TET10_EDGE[n - 4][1];
+            for (int d = 0; d < 3; ++d) p[d] = vb[a][d] + vb[b][d];
+        }
+        const size_t node = (2 * ex + p[0]) + (size_t)g.nx * ((2 * ey + p[1]) + (size_t)g.ny * (2 * ez + p[2]));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) u[3 * n + c] = x[3 * node + c];
+    }
+    const T ih[3] = {ih0, ih1, ih2};
+    T gg[4][3];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) kuhn_grad<T>(t, v, ih, gg[v]);
+    T y[30];
+    tet10_element<T>(u, gg, coef[t * cells + cell], coef[(6 + t) * cells + cell], vol / T(4), y);
+#pragma unroll
+    for (int s = 0; s < 30; ++s) E[((size_t)(t * 30 + s)) * cells + cell] = y[s];
+}
+
+}  // namespace fem

@@ -1,0 +1,1245 @@
+// The solver: the multigrid hierarchy on the device, the V-cycle, the FP64 PCG of the reference
+// (crates/fem/src/multigrid.rs:692-753) and the C ABI of include/fem_solver.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstddef>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/fem_solver.h"
+#include "common.hpp"
+#include "dense.hpp"
+#include "kernels.hpp"
+#include "model.hpp"
+#include "setup_kernels.hpp"
+
+namespace fem {
+
+namespace {
+
+inline unsigned nblocks(size_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+
+// Launches `kernel` on `blocks` blocks of BLOCK threads.
+template <class... KArgs, class... Args>
+void launch_blocks(void (*kernel)(KArgs...), unsigned blocks, hipStream_t s, Args&&... args) {
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(BLOCK), 0, s, static_cast<KArgs>(args)...);
+}
+
+// Launches `kernel` with one thread per item of n items.
+template <class... KArgs, class... Args>
+void launch(void (*kernel)(KArgs...), size_t n, hipStream_t s, Args&&... args) {
+    if (n == 0) return;
+    launch_blocks(kernel, nblocks(n), s, std::forward<Args>(args)...);
+}
+
+enum class OpKind { Hex8K, Stencil27, Tet4Heat, Tet4Elastic, Tet10 };
+
+// One level of the hierarchy in storage type T.
+template <class T>
+struct Level {
+    OpKind op = OpKind::Hex8K;
+    int comps = 1;
+    GridDims g{};
+    int cells[3] = {0, 0, 0};  // cells of the level (tet10: the cells of its grid, nodes 2 n + 1)
+    double h[3] = {0, 0, 0};
+    bool kuhn = false;         // transfer to the next level is Kuhn P1 (else trilinear)
+    DArray<T> coef, E;
+    DArray<T> invd, b, xa, xb, d, r;
+    int cur = 0;
+    double lmax = 0.0;
+    std::vector<T> c1, c2;     // Chebyshev step coefficients
+    size_t nodes() const { return g.nodes(); }
+    size_t dofs() const { return g.nodes() * comps; }
+    size_t ncell() const { return (size_t)cells[0] * cells[1] * cells[2]; }
+    T* x() { return cur ? xb.data() : xa.data(); }
+    T* x2() { return cur ? xa.data() : xb.data(); }
+};
+
+template <class T>
+void hex_c(const double h[3], T c[8]) {
+    for (int p = 0; p < 8; ++p) {
+        double v = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            double term = 1.0;
+            for (int e = 0; e < 3; ++e) {
+                const bool differ = (p >> e) & 1;
+                if (e == d) term *= differ ? -1.0 / h[e] : 1.0 / h[e];
+                else term *= differ ? h[e] / 6.0 : h[e] / 3.0;
+            }
+            v += term;
+        }
+        c[p] = (T)v;
+    }
+}
+
+template <class T, class F>
+void with_op(Level<T>& L, F&& f) {
+    const T ih[3] = {T(1.0 / L.h[0]), T(1.0 / L.h[1]), T(1.0 / L.h[2])};
+    const T vol = T(L.h[0] * L.h[1] * L.h[2] / 6.0);
+    switch (L.op) {
+        case OpKind::Hex8K: {
+            Hex8K<T> o{L.g, L.coef.data(), {}};
+            hex_c<T>(L.h, o.c);
+            f(ScalarOp<Hex8K<T>>{o});
+            break;
+        }
+        case OpKind::Stencil27:
+            f(ScalarOp<Stencil27<T>>{Stencil27<T>{L.g, L.coef.data()}});
+            break;
+        case OpKind::Tet4Heat:
+            f(ScalarOp<Tet4Heat<T>>{Tet4Heat<T>{L.g, L.coef.data(), {ih[0], ih[1], ih[2]}, vol}});
+            break;
+        case OpKind::Tet4Elastic:
+            f(VectorOp<Tet4Elastic<T>>{Tet4Elastic<T>{L.g, L.coef.data(), {ih[0], ih[1], ih[2]}, vol}});
+            break;
+        case OpKind::Tet10:
+            f(VectorOp<Tet10Op<T>>{Tet10Op<T>{L.g, L.coef.data(), L.E.data(), {ih[0], ih[1], ih[2]}, vol}});
+            break;
+    }
+}
+
+// Phase A of an operator whose apply reads an element buffer (tet10); nothing for the others.
+template <class T>
+void prepare(Level<T>& L, const T* x, hipStream_t s) {
+    if (L.op != OpKind::Tet10) return;
+    const T ih[3] = {T(1.0 / L.h[0]), T(1.0 / L.h[1]), T(1.0 / L.h[2])};
+    const T vol = T(L.h[0] * L.h[1] * L.h[2] / 6.0);
+    const size_t n = 6 * L.ncell();
+    launch_blocks(k_tet10_elements<T>, nblocks(n), s, L.g, (const T*)L.coef.data(), x,
+                       L.E.data(), ih[0], ih[1], ih[2], vol);
+}
+
+// Conversion of the FP64 coefficients of a level to T.
+template <class T>
+void convert_level(const Level<double>& src, Level<T>& dst, hipStream_t s) {
+    dst.op = src.op;
+    dst.comps = src.comps;
+    dst.g = src.g;
+    for (int d = 0; d < 3; ++d) {
+        dst.cells[d] = src.cells[d];
+        dst.h[d] = src.h[d];
+    }
+    dst.kuhn = src.kuhn;
+    dst.coef.resize(src.coef.size());
+    launch((k_convert<double, T>), src.coef.size(), s, src.coef.size(), src.coef.data(), dst.coef.data());
+    if (src.op == OpKind::Tet10) dst.E.resize(30 * 6 * src.ncell());
+}
+
+// Device time of chosen launches by HIP events on the launching stream: begin/end pairs, not
+// nested, category 0 the level-0 smoother step, 1 the outer FP64 operator apply.
+struct Profiler {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<std::pair<int, size_t>> marks;
+
+    hipEvent_t next() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            FEM_HIP(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    void begin(int cat, hipStream_t s) {
+        if (!on) return;
+        marks.push_back({cat, used});
+        FEM_HIP(hipEventRecord(next(), s));
+    }
+    void end(hipStream_t s) {
+        if (!on) return;
+        FEM_HIP(hipEventRecord(next(), s));
+    }
+    void reset() {
+        used = 0;
+        marks.clear();
+    }
+    void collect(double secs[2], uint64_t counts[2]) {
+        secs[0] = secs[1] = 0.0;
+        counts[0] = counts[1] = 0;
+        for (const auto& m : marks) {
+            FEM_HIP(hipEventSynchronize(pool[m.second + 1]));
+            float ms = 0.f;
+            FEM_HIP(hipEventElapsedTime(&ms, pool[m.second], pool[m.second + 1]));
+            secs[m.first] += ms * 1e-3;
+            counts[m.first] += 1;
+        }
+    }
+    ~Profiler() {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// The hierarchy of one precision: levels, smoother, dense coarse level.
+template <class T>
+class Hierarchy {
+   public:
+    std::vector<Level<T>> lv;
+    DenseCoarse coarse;
+    int degree = 3;
+    Profiler* prof = nullptr;
+
+    void mark_begin(size_t l, hipStream_t s) {
+        if (l == 0 && prof) prof->begin(0, s);
+    }
+    void mark_end(size_t l, hipStream_t s) {
+        if (l == 0 && prof) prof->end(s);
+    }
+
+    void vcycle(hipStream_t s) { cycle(0, s); }
+
+    void cycle(size_t l, hipStream_t s) {
+        Level<T>& L = lv[l];
+        if (l + 1 == lv.size()) {
+            coarse.solve<T>(L.b.data(), L.x(), s);
+            return;
+        }
+        const size_t n = L.dofs();
+        // pre-smoothing from x = 0
+        launch(k_cheb_first<T>, n, s, n, (const T*)L.b.data(), (const T*)L.invd.data(), L.x(), L.d.data(), L.c2[0]);
+        for (int st = 1; st < degree; ++st) {
+            mark_begin(l, s);
+            prepare(L, L.x(), s);
+            with_op(L, [&](auto op) {
+                launch((k_cheb<decltype(op), T>), L.nodes(), s, op, (const T*)L.x(), L.x2(), L.d.data(),
+                       (const T*)L.b.data(), (const T*)L.invd.data(), L.c1[st], L.c2[st]);
+            });
+            mark_end(l, s);
+            L.cur ^= 1;
+        }
+        prepare(L, L.x(), s);
+        with_op(L, [&](auto op) {
+            launch((k_residual<decltype(op), T>), L.nodes(), s, op, (const T*)L.x(), (const T*)L.b.data(), L.r.data());
+        });
+        Level<T>& Cc = lv[l + 1];
+        transfer(L, Cc, true, s);
+        cycle(l + 1, s);
+        transfer(L, Cc, false, s);
+        for (int st = 0; st < degree; ++st) {
+            mark_begin(l, s);
+            prepare(L, L.x(), s);
+            with_op(L, [&](auto op) {
+                launch((k_cheb<decltype(op), T>), L.nodes(), s, op, (const T*)L.x(), L.x2(), L.d.data(),
+                       (const T*)L.b.data(), (const T*)L.invd.data(), st == 0 ? T(0) : L.c1[st], L.c2[st]);
+            });
+            mark_end(l, s);
+            L.cur ^= 1;
+        }
+    }
+
+    void transfer(Level<T>& F, Level<T>& Cc, bool restrict_, hipStream_t s) {
+        const int C = F.comps;
+        if (restrict_) {
+            const size_t n = Cc.nodes();
+            if (F.kuhn) {
+                if (C == 1) launch((k_restrict<T, 1, true>), n, s, F.g, Cc.g, (const T*)F.r.data(), Cc.b.data());
+                else launch((k_restrict<T, 3, true>), n, s, F.g, Cc.g, (const T*)F.r.data(), Cc.b.data());
+            } else {
+                if (C == 1) launch((k_restrict<T, 1, false>), n, s, F.g, Cc.g, (const T*)F.r.data(), Cc.b.data());
+                else launch((k_restrict<T, 3, false>), n, s, F.g, Cc.g, (const T*)F.r.data(), Cc.b.data());
+            }
+        } else {
+            const size_t n = F.nodes();
+            if (F.kuhn) {
+                if (C == 1) launch((k_prolong_add<T, 1, true>), n, s, F.g, Cc.g, (const T*)Cc.x(), F.x());
+                else launch((k_prolong_add<T, 3, true>), n, s, F.g, Cc.g, (const T*)Cc.x(), F.x());
+            } else {
+                if (C == 1) launch((k_prolong_add<T, 1, false>), n, s, F.g, Cc.g, (const T*)Cc.x(), F.x());
+                else launch((k_prolong_add<T, 3, false>), n, s, F.g, Cc.g, (const T*)Cc.x(), F.x());
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------------------------
+struct Options {
+    double tol = 1e-10;
+    uint64_t max_iter = 500;
+    int method = FEM_METHOD_AUTO;
+    int precision = FEM_PREC_FP32;
+    uint64_t crossover = 3000;
+    int levels = 0;
+    int degree = 3;
+    double cheb_lower = 0.1, cheb_upper = 1.1;
+    int power_steps = 20;
+    uint64_t coarse_max = 6000;
+    int refine = 1;
+    int device = 0;
+};
+
+class Solver {
+   public:
+    Solver(const BoxInput& in, const Options& o) : opt_(o) {
+        const auto t0 = std::chrono::steady_clock::now();
+        FEM_HIP(hipSetDevice(o.device));
+        hipDeviceProp_t prop;
+        FEM_HIP(hipGetDeviceProperties(&prop, o.device));
+        device_name_ = std::string(prop.name[0] ? prop.name : "AMD GPU") + " (" + prop.gcnArchName + ")";
+        FEM_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+        model_ = build_model(in);
+        const size_t nfree = model_.free.size();
+        if (nfree == 0) throw Error("the model has no free unknown");
+        build_levels64();
+        n0_ = lv64_[0].dofs();
+        x_.resize(n0_);
+        b_.resize(n0_);
+        r_.resize(n0_);
+        z_.resize(n0_);
+        p_.resize(n0_);
+        q_.resize(n0_);
+        tmp_.resize(n0_);
+        parts_ = nblocks(n0_) + 1;
+        partA_.resize(parts_);
+        partB_.resize(parts_);
+        scal_.resize(1);
+        red_.resize(4);
+        free_grid_.resize(nfree);
+        free_grid_.upload(model_.free_grid.data(), nfree, s_);
+        fb_.resize(nfree);
+        fx_.resize(nfree);
+        // direct or multigrid (solve.rs:92-183 with Crossover::Unknowns)
+        bool direct = false;
+        if (o.method == FEM_METHOD_DIRECT) direct = true;
+        else if (o.method == FEM_METHOD_AUTO) direct = nfree <= o.crossover;
+        if (direct) {
+            if (nfree > 20000)
+                throw Error("the direct solver on the device holds up to 20000 unknowns; this model has " +
+                            std::to_string(nfree));
+            direct_ = std::make_unique<DenseCoarse>();
+            assemble_and_factor(lv64_[0], *direct_, false);
+            method_ = FEM_METHOD_DIRECT;
+        } else {
+            method_ = FEM_METHOD_MULTIGRID;
+            if (o.precision == FEM_PREC_FP64) {
+                h64_ = std::make_unique<Hierarchy<double>>();
+                build_hierarchy(*h64_);
+            } else {
+                h32_ = std::make_unique<Hierarchy<float>>();
+                build_hierarchy(*h32_);
+                r32_.resize(n0_);
+            }
+        }
+        FEM_HIP(hipStreamSynchronize(s_));
+        setup_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    ~Solver() {
+        if (s_) (void)hipStreamDestroy(s_);
+    }
+
+    int method() const { return method_; }
+
+    void set_profile(bool on) {
+        prof_.on = on;
+        if (h32_) h32_->prof = on ? &prof_ : nullptr;
+        if (h64_) h64_->prof = on ? &prof_ : nullptr;
+    }
+
+    // Device time and algorithmic bytes of the profiled launches of the last solve.
+    void profile(fem_profile* p) {
+        double secs[2];
+        uint64_t counts[2];
+        prof_.collect(secs, counts);
+        p->smoother_launches = counts[0];
+        p->smoother_seconds = secs[0];
+        p->outer_launches = counts[1];
+        p->outer_seconds = secs[1];
+        p->dofs = n0_;
+        const Level<double>& L = lv64_[0];
+        p->outer_bytes = 16 * n0_ + L.coef.bytes();
+        if (h32_) p->smoother_bytes = 6 * n0_ * sizeof(float) + h32_->lv[0].coef.bytes();
+        else if (h64_) p->smoother_bytes = 6 * n0_ * sizeof(double) + h64_->lv[0].coef.bytes();
+        else p->smoother_bytes = 0;
+    }
+    const BoxModel& model() const { return model_; }
+    Options& options() { return opt_; }
+    const char* device_name() const { return device_name_.c_str(); }
+
+    // Reduced right side over the free unknowns in reference order: load - A u_fixed.
+    void rhs(double* out) {
+        Level<double>& L = lv64_[0];
+        const size_t n = n0_;
+        tmp_.upload(model_.fixed_value.data(), n, s_);
+        // y = A u with u the fixed values: the operator's rows read the fixed nodes' values.
+        prepare(L, tmp_.data(), s_);
+        with_op(L, [&](auto op) { launch((k_apply<decltype(op), double>), L.nodes(), s_, op, (const double*)tmp_.data(), r_.data()); });
+        std::vector<double> y(n);
+        r_.download(y.data(), n, s_);
+        FEM_HIP(hipStreamSynchronize(s_));
+        for (size_t k = 0; k < model_.free.size(); ++k) {
+            const size_t gi = model_.free_grid[k];
+            out[k] = model_.load[gi] - y[gi];
+        }
+    }
+
+    // y = A x over the free unknowns (reference order, host).
+    void apply_host(const double* xin, double* yout) {
+        const size_t nf = model_.free.size();
+        fb_.upload(xin, nf, s_);
+        launch(k_fill<double>, n0_, s_, n0_, tmp_.data(), 0.0);
+        launch(k_scatter, nf, s_, nf, (const int64_t*)free_grid_.data(), (const double*)fb_.data(), tmp_.data());
+        apply64(tmp_.data(), r_.data());
+        launch(k_gather, nf, s_, nf, (const int64_t*)free_grid_.data(), (const double*)r_.data(), fx_.data());
+        fx_.download(yout, nf, s_);
+        FEM_HIP(hipStreamSynchronize(s_));
+    }
+
+    void precondition_host(const double* rin, double* zout) {
+        if (method_ != FEM_METHOD_MULTIGRID) throw Error("the direct solver has no preconditioner");
+        const size_t nf = model_.free.size();
+        fb_.upload(rin, nf, s_);
+        launch(k_fill<double>, n0_, s_, n0_, r_.data(), 0.0);
+        launch(k_scatter, nf, s_, nf, (const int64_t*)free_grid_.data(), (const double*)fb_.data(), r_.data());
+        precondition();
+        launch(k_gather, nf, s_, nf, (const int64_t*)free_grid_.data(), (const double*)z_.data(), fx_.data());
+        fx_.download(zout, nf, s_);
+        FEM_HIP(hipStreamSynchronize(s_));
+    }
+
+    void level_info(int l, uint64_t* unknowns, double* lmax) const {
+        if (h32_) {
+            if (l < 0 || l >= (int)h32_->lv.size()) throw Error("no such level");
+            *unknowns = free_count(h32_->lv[l].g, h32_->lv[l].comps);
+            *lmax = h32_->lv[l].lmax;
+        } else if (h64_) {
+            if (l < 0 || l >= (int)h64_->lv.size()) throw Error("no such level");
+            *unknowns = free_count(h64_->lv[l].g, h64_->lv[l].comps);
+            *lmax = h64_->lv[l].lmax;
+        } else {
+            if (l != 0) throw Error("no such level");
+            *unknowns = model_.free.size();
+            *lmax = 0.0;
+        }
+    }
+
+    // Solver::solve with device vectors in reference order.
+    void solve_device(const double* d_b, double* d_x, int (*cancelled)(void*), void* ctx, fem_stats* st) {
+        const size_t nf = model_.free.size();
+        hipEvent_t e0, e1;
+        FEM_HIP(hipEventCreate(&e0));
+        FEM_HIP(hipEventCreate(&e1));
+        FEM_HIP(hipEventRecord(e0, s_));
+        launch(k_fill<double>, n0_, s_, n0_, b_.data(), 0.0);
+        launch(k_fill<double>, n0_, s_, n0_, x_.data(), 0.0);
+        launch(k_scatter, nf, s_, nf, (const int64_t*)free_grid_.data(), d_b, b_.data());
+        launch(k_scatter, nf, s_, nf, (const int64_t*)free_grid_.data(), (const double*)d_x, x_.data());
+        prof_.reset();
+        fem_stats out{};
+        out.method = method_;
+        out.precision = method_ == FEM_METHOD_DIRECT ? FEM_PREC_FP64 : opt_.precision;
+        out.setup_seconds = setup_seconds_;
+        std::string error;
+        try {
+            if (method_ == FEM_METHOD_DIRECT) {
+                // x = A^-1 b, then one step of refinement on the FP64 residual: x += A^-1 (b - A x)
+                direct_->solve<double>(b_.data(), x_.data(), s_);
+                apply64(x_.data(), tmp_.data());
+                launch(k_axpby, n0_, s_, n0_, 1.0, (const double*)b_.data(), -1.0, tmp_.data());
+                launch(k_fill<double>, n0_, s_, n0_, z_.data(), 0.0);  // the solve writes the free rows only
+                direct_->solve<double>(tmp_.data(), z_.data(), s_);
+                launch(k_axpby, n0_, s_, n0_, 1.0, (const double*)z_.data(), 1.0, x_.data());
+                const double res = true_residual();
+                out.iterations = 0;
+                out.residual = res;
+                out.true_residual = res;
+                out.levels = 1;
+                out.coarse_unknowns = nf;
+            } else {
+                pcg(cancelled, ctx, out);
+                out.levels = h32_ ? h32_->lv.size() : h64_->lv.size();
+                out.coarse_unknowns = h32_ ? h32_->coarse.n() : h64_->coarse.n();
+            }
+        } catch (const Error& e) {
+            error = e.what();
+        }
+        launch(k_gather, nf, s_, nf, (const int64_t*)free_grid_.data(), (const double*)x_.data(), d_x);
+        FEM_HIP(hipEventRecord(e1, s_));
+        FEM_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        FEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        out.seconds = ms * 1e-3;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (st) *st = out;
+        if (!error.empty()) throw Error(error);
+    }
+
+    void solve_host(const double* b, double* x, int (*cancelled)(void*), void* ctx, fem_stats* st) {
+        const size_t nf = model_.free.size();
+        fb_.upload(b, nf, s_);
+        fx_.upload(x, nf, s_);
+        std::string error;
+        try {
+            solve_device(fb_.data(), fx_.data(), cancelled, ctx, st);
+        } catch (const Error& e) {
+            error = e.what();
+        }
+        fx_.download(x, nf, s_);
+        FEM_HIP(hipStreamSynchronize(s_));
+        if (!error.empty()) throw Error(error);
+    }
+
+   private:
+    Options opt_;
+    BoxModel model_;
+    hipStream_t s_ = nullptr;
+    std::string device_name_;
+    int method_ = FEM_METHOD_MULTIGRID;
+    double setup_seconds_ = 0.0;
+    std::vector<Level<double>> lv64_;  // FP64 coefficients of every level (level 0: the outer operator)
+    std::unique_ptr<Hierarchy<float>> h32_;
+    std::unique_ptr<Hierarchy<double>> h64_;
+    std::unique_ptr<DenseCoarse> direct_;
+    Profiler prof_;
+    size_t n0_ = 0;
+    int parts_ = 0;
+    DArray<double> x_, b_, r_, z_, p_, q_, tmp_, partA_, partB_, red_, fb_, fx_;
+    DArray<float> r32_;
+    DArray<PcgScalars> scal_;
+    DArray<int64_t> free_grid_;
+
+    static uint64_t free_count(const GridDims& g, int comps) {
+        uint64_t n = 0;
+        const int top[3] = {g.nx - 1, g.ny - 1, g.nz - 1};
+        // nodes not on a fixed face
+        for (int k = 0; k < g.nz; ++k)
+            for (int j = 0; j < g.ny; ++j)
+                for (int i = 0; i < g.nx; ++i) {
+                    const int q[3] = {i, j, k};
+                    bool fx = false;
+                    for (int f = 0; f < 6; ++f)
+                        if (g.fix & (1u << f)) fx = fx || (f % 2 == 0 ? q[f / 2] == 0 : q[f / 2] == top[f / 2]);
+                    if (!fx) n += comps;
+                }
+        return n;
+    }
+
+    // ---------------------------------------------------------------- FP64 levels
+    void build_levels64() {
+        const BoxModel& m = model_;
+        const int quad = m.quadratic ? 1 : 0;
+        lv64_.clear();
+        lv64_.emplace_back();
+        Level<double>& L0 = lv64_[0];
+        L0.comps = m.comps;
+        for (int d = 0; d < 3; ++d) {
+            L0.cells[d] = m.in.n[d];
+            L0.h[d] = m.h[d];
+        }
+        L0.g = GridDims{m.gn[0], m.gn[1], m.gn[2], m.fix};
+        switch (m.in.kind) {
+            case Kind::HeatHex8:
+                L0.op = OpKind::Hex8K;
+                L0.coef.resize(m.cell_k.size());
+                L0.coef.upload(m.cell_k.data(), m.cell_k.size(), s_);
+                break;
+            case Kind::HeatTet4:
+                L0.op = OpKind::Tet4Heat;
+                L0.kuhn = true;
+                break;
+            case Kind::ElasticTet4:
+                L0.op = OpKind::Tet4Elastic;
+                L0.kuhn = true;
+                break;
+            case Kind::ElasticTet10:
+                L0.op = OpKind::Tet10;
+                L0.kuhn = true;
+                L0.E.resize(30 * 6 * L0.ncell());
+                break;
+        }
+        if (m.in.kind != Kind::HeatHex8) {
+            L0.coef.resize(m.tet_coef.size());
+            L0.coef.upload(m.tet_coef.data(), m.tet_coef.size(), s_);
+        }
+        (void)quad;
+    }
+
+    // The levels below level 0 for the multigrid, in FP64: Galerkin stencils (hex8) or averaged
+    // Kuhn coefficients (tets). Stops when a level cannot halve, at opt.levels, or when the level is
+    // small enough for the dense solve and automatic levels were asked for.
+    void build_coarse_levels64() {
+        const BoxModel& m = model_;
+        const int want = opt_.levels;
+        {
+            const std::vector<double> P = child_interp();
+            FEM_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(HEX_CHILD_P), P.data(), P.size() * sizeof(double), 0,
+                                           hipMemcpyHostToDevice, s_));
+            int kids[6][8];
+            kuhn_kids(kids);
+            FEM_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(KUHN_KIDS), kids, sizeof kids, 0, hipMemcpyHostToDevice, s_));
+        }
+        DArray<double> cellmat;  // hex8 cell matrices of the current level
+        // tet10: level 1 is tet4 on the same cells and coefficients
+        if (m.in.kind == Kind::ElasticTet10 && (want == 0 || want > 1)) {
+            Level<double> L1;
+            L1.op = OpKind::Tet4Elastic;
+            L1.comps = 3;
+            L1.kuhn = true;
+            for (int d = 0; d < 3; ++d) {
+                L1.cells[d] = m.in.n[d];
+                L1.h[d] = m.h[d];
+            }
+            L1.g = GridDims{m.in.n[0] + 1, m.in.n[1] + 1, m.in.n[2] + 1, m.fix};
+            L1.coef.resize(lv64_[0].coef.size());
+            FEM_HIP(hipMemcpyAsync(L1.coef.data(), lv64_[0].coef.data(), lv64_[0].coef.bytes(), hipMemcpyDeviceToDevice, s_));
+            lv64_.push_back(std::move(L1));
+        }
+        for (;;) {
+            Level<double>& F = lv64_.back();
+            const int l = (int)lv64_.size() - 1;
+            if (want > 0 && l + 1 >= want) break;
+            const uint64_t nfree = free_count(F.g, F.comps);
+            if (want == 0 && nfree <= 1000) break;
+            // tet10 level: its grid is refined; the next level has the same cells (handled above)
+            if (F.op == OpKind::Tet10) break;
+            bool halves = true;
+            for (int d = 0; d < 3; ++d) halves = halves && F.cells[d] % 2 == 0 && F.cells[d] >= 2;
+            if (!halves) break;
+            Level<double> Cl;
+            Cl.comps = F.comps;
+            Cl.kuhn = F.kuhn;
+            for (int d = 0; d < 3; ++d) {
+                Cl.cells[d] = F.cells[d] / 2;
+                Cl.h[d] = 2.0 * F.h[d];
+            }
+            Cl.g = GridDims{Cl.cells[0] + 1, Cl.cells[1] + 1, Cl.cells[2] + 1, m.fix};
+            const size_t nc = Cl.ncell();
+            if (F.op == OpKind::Hex8K || F.op == OpKind::Stencil27) {
+                Cl.op = OpKind::Stencil27;
+                DArray<double> cm(nc * 64);
+                if (F.op == OpKind::Hex8K) {
+                    // M_c = P_c^T K1 P_c for the fine cell matrix K1 of k = 1
+                    double c[8];
+                    hex_c<double>(F.h, c);
+                    std::vector<double> M(8 * 64, 0.0), P = child_interp();
+                    for (int ch = 0; ch < 8; ++ch)
+                        for (int a = 0; a < 8; ++a)
+                            for (int b = 0; b < 8; ++b) {
+                                double s = 0.0;
+                                for (int p = 0; p < 8; ++p)
+                                    for (int q = 0; q < 8; ++q) s += P[ch * 64 + p * 8 + a] * c[p ^ q] * P[ch * 64 + q * 8 + b];
+                                M[ch * 64 + a * 8 + b] = s;
+                            }
+                    DArray<double> dM(8 * 64);
+                    dM.upload(M.data(), M.size(), s_);
+                    launch(k_hex_cells_from_k, nc, s_, Cl.cells[0], Cl.cells[1], Cl.cells[2],
+                           (const double*)F.coef.data(), (const double*)dM.data(), cm.data());
+                    FEM_HIP(hipStreamSynchronize(s_));
+                } else {
+                    launch(k_hex_cells_from_cells, nc, s_, Cl.cells[0], Cl.cells[1], Cl.cells[2],
+                           (const double*)cellmat.data(), cm.data());
+                }
+                Cl.coef.resize(14 * Cl.nodes());
+                launch(k_stencil_from_cells, Cl.nodes(), s_, Cl.g, (const double*)cm.data(), Cl.coef.data());
+                FEM_HIP(hipStreamSynchronize(s_));
+                cellmat = std::move(cm);
+            } else {
+                Cl.op = F.op == OpKind::Tet4Heat ? OpKind::Tet4Heat : OpKind::Tet4Elastic;
+                const int q = Cl.comps == 3 ? 2 : 1;
+                Cl.coef.resize((size_t)q * 6 * nc);
+                launch(k_kuhn_average, (size_t)q * 6 * nc, s_, Cl.cells[0], Cl.cells[1], Cl.cells[2], q,
+                       (const double*)F.coef.data(), Cl.coef.data());
+            }
+            lv64_.push_back(std::move(Cl));
+        }
+    }
+
+    static std::vector<double> child_interp() {
+        std::vector<double> P(8 * 64, 0.0);
+        for (int ch = 0; ch < 8; ++ch)
+            for (int p = 0; p < 8; ++p)
+                for (int q = 0; q < 8; ++q) {
+                    double w = 1.0;
+                    for (int d = 0; d < 3; ++d) {
+                        const int pos = ((ch >> d) & 1) + ((p >> d) & 1);  // 0, 1, 2 in fine units
+                        const int qd = (q >> d) & 1;
+                        const double wd = pos == 1 ? 0.5 : (pos == 0 ? (qd == 0 ? 1.0 : 0.0) : (qd == 1 ? 1.0 : 0.0));
+                        w *= wd;
+                    }
+                    P[ch * 64 + p * 8 + q] = w;
+                }
+        return P;
+    }
+
+    static void kuhn_kids(int out[6][8]) {
+        const int order[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+        for (int t = 0; t < 6; ++t) {
+            int m = 0;
+            for (int c = 0; c < 8; ++c)
+                for (int ft = 0; ft < 6; ++ft) {
+                    double y[3] = {0, 0, 0};
+                    int bits[3] = {0, 0, 0};
+                    for (int v = 0; v < 4; ++v) {
+                        if (v > 0) bits[order[ft][v - 1]] = 1;
+                        for (int d = 0; d < 3; ++d) y[d] += 0.25 * bits[d];
+                    }
+                    double x[3];
+                    for (int d = 0; d < 3; ++d) x[d] = 0.5 * (((c >> d) & 1) + y[d]);
+                    const int a = order[t][0], b = order[t][1], cc = order[t][2];
+                    if (x[a] > x[b] && x[b] > x[cc]) {
+                        if (m >= 8) throw Error("the Kuhn refinement is not nested");
+                        out[t][m++] = c * 8 + ft;
+                    }
+                }
+            if (m != 8) throw Error("the Kuhn refinement is not nested");
+        }
+    }
+
+    // ---------------------------------------------------------------- dense assembly by probing
+    // dense_of: grid unknown -> dense index (free unknowns of the level in grid order), or -1.
+    std::vector<int64_t> dense_index(const GridDims& g, int C, std::vector<int64_t>& free_units) {
+        std::vector<int64_t> of(g.nodes() * C, -1);
+        free_units.clear();
+        const int top[3] = {g.nx - 1, g.ny - 1, g.nz - 1};
+        for (int k = 0; k < g.nz; ++k)
+            for (int j = 0; j < g.ny; ++j)
+                for (int i = 0; i < g.nx; ++i) {
+                    const int q[3] = {i, j, k};
+                    bool fx = false;
+                    for (int f = 0; f < 6; ++f)
+                        if (g.fix & (1u << f)) fx = fx || (f % 2 == 0 ? q[f / 2] == 0 : q[f / 2] == top[f / 2]);
+                    if (fx) continue;
+                    const size_t node = i + (size_t)g.nx * (j + (size_t)g.ny * k);
+                    for (int c = 0; c < C; ++c) {
+                        of[node * C + c] = (int64_t)free_units.size();
+                        free_units.push_back((int64_t)(node * C + c));
+                    }
+                }
+        return of;
+    }
+
+    void assemble_and_factor(Level<double>& L, DenseCoarse& dc, bool fp32) {
+        const int C = L.comps;
+        std::vector<int64_t> units;
+        const std::vector<int64_t> of = dense_index(L.g, C, units);
+        const int64_t n = (int64_t)units.size();
+        DArray<int64_t> dof(of.size());
+        dof.upload(of.data(), of.size(), s_);
+        DArray<double> A((size_t)n * n), px(L.dofs()), py(L.dofs());
+        launch(k_fill<double>, (size_t)n * n, s_, (size_t)n * n, A.data(), 0.0);
+        const int S = L.op == OpKind::Tet10 ? 5 : 3;
+        for (int c0 = 0; c0 < S; ++c0)
+            for (int c1 = 0; c1 < S; ++c1)
+                for (int c2 = 0; c2 < S; ++c2)
+                    for (int comp = 0; comp < C; ++comp) {
+                        launch(k_probe_fill, L.dofs(), s_, L.g, C, S, c0, c1, c2, comp, (const int64_t*)dof.data(), px.data());
+                        prepare(L, px.data(), s_);
+                        with_op(L, [&](auto op) {
+                            launch((k_apply<decltype(op), double>), L.nodes(), s_, op, (const double*)px.data(), py.data());
+                        });
+                        launch(k_probe_extract, L.dofs(), s_, L.g, C, S, c0, c1, c2, comp, (const int64_t*)dof.data(),
+                               (const double*)py.data(), A.data(), n);
+                    }
+        FEM_HIP(hipGetLastError());
+        dc.factor(A.data(), (int)n, fp32, s_);
+        dc.map().resize(units.size());
+        dc.map().upload(units.data(), units.size(), s_);
+        FEM_HIP(hipStreamSynchronize(s_));
+    }
+
+    // ---------------------------------------------------------------- hierarchy in T
+    template <class T>
+    void build_hierarchy(Hierarchy<T>& H) {
+        build_coarse_levels64();
+        if (lv64_.size() < 2 && free_count(lv64_[0].g, lv64_[0].comps) > opt_.coarse_max)
+            throw Error("the multigrid has a single level with more unknowns than the dense coarse solve holds");
+        Level<double>& last = lv64_.back();
+        const uint64_t ncoarse = free_count(last.g, last.comps);
+        if (ncoarse > opt_.coarse_max)
+            throw Error("the coarsest level has " + std::to_string(ncoarse) + " unknowns, more than the dense limit " +
+                        std::to_string(opt_.coarse_max) + ": use a box whose cell counts halve more times");
+        H.degree = opt_.degree;
+        H.lv.resize(lv64_.size());
+        for (size_t l = 0; l < lv64_.size(); ++l) {
+            Level<T>& L = H.lv[l];
+            convert_level(lv64_[l], L, s_);
+            const size_t n = L.dofs();
+            L.invd.resize(n);
+            L.b.resize(n);
+            L.xa.resize(n);
+            L.xb.resize(n);
+            L.d.resize(n);
+            L.r.resize(n);
+            for (T* v : {L.b.data(), L.xa.data(), L.xb.data(), L.d.data(), L.r.data()}) launch(k_fill<T>, n, s_, n, v, T(0));
+            if (l + 1 < lv64_.size()) {
+                with_op(L, [&](auto op) { launch((k_inv_diag<decltype(op), T>), L.nodes(), s_, op, L.invd.data()); });
+                L.lmax = estimate_lmax(L);
+                chebyshev(L);
+            }
+        }
+        assemble_and_factor(lv64_.back(), H.coarse, sizeof(T) == 4);
+    }
+
+    template <class T>
+    double estimate_lmax(Level<T>& L) {
+        const size_t n = L.dofs();
+        T* v = L.xa.data();
+        T* w = L.xb.data();
+        launch(k_random<T>, n, s_, n, v, (const T*)L.invd.data());
+        const unsigned pb = std::min<unsigned>(nblocks(n), 1024);
+        double lam = 0.0;
+        for (int it = 0; it < opt_.power_steps; ++it) {
+            launch_blocks(k_dot<T>, pb, s_, n, (const T*)v, (const T*)v, partA_.data());
+            launch_blocks(k_sum, 1, s_, (const double*)partA_.data(), (int)pb, red_.data());
+            launch(k_normalise<T>, n, s_, n, v, (const double*)red_.data());
+            prepare(L, v, s_);
+            with_op(L, [&](auto op) { launch((k_apply<decltype(op), T>), L.nodes(), s_, op, (const T*)v, w); });
+            launch(k_scale<T>, n, s_, n, w, (const T*)L.invd.data());
+            launch_blocks(k_dot<T>, pb, s_, n, (const T*)w, (const T*)w, partA_.data());
+            launch_blocks(k_sum, 1, s_, (const double*)partA_.data(), (int)pb, red_.data() + 1);
+            double nw = 0.0;
+            FEM_HIP(hipMemcpyAsync(&nw, red_.data() + 1, sizeof(double), hipMemcpyDeviceToHost, s_));
+            FEM_HIP(hipStreamSynchronize(s_));
+            lam = std::sqrt(nw);
+            std::swap(v, w);
+        }
+        launch(k_fill<T>, n, s_, n, L.xa.data(), T(0));
+        launch(k_fill<T>, n, s_, n, L.xb.data(), T(0));
+        return lam;
+    }
+
+    template <class T>
+    void chebyshev(Level<T>& L) {
+        const double hi = opt_.cheb_upper * L.lmax, lo = opt_.cheb_lower * L.lmax;
+        const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+        L.c1.assign(opt_.degree, T(0));
+        L.c2.assign(opt_.degree, T(0));
+        double rho = 1.0 / sigma;
+        L.c1[0] = T(0);
+        L.c2[0] = T(1.0 / theta);
+        for (int s = 1; s < opt_.degree; ++s) {
+            const double rn = 1.0 / (2.0 * sigma - rho);
+            L.c1[s] = T(rn * rho);
+            L.c2[s] = T(2.0 * rn / delta);
+            rho = rn;
+        }
+    }
+
+    // ---------------------------------------------------------------- FP64 outer iteration
+    void apply64(const double* in, double* out) {
+        Level<double>& L = lv64_[0];
+        prepare(L, in, s_);
+        with_op(L, [&](auto op) { launch((k_apply<decltype(op), double>), L.nodes(), s_, op, in, out); });
+    }
+
+    // z = M^-1 r (device, grid layout).
+    void precondition() {
+        if (h32_) {
+            Level<float>& L = h32_->lv[0];
+            launch((k_convert<double, float>), n0_, s_, n0_, (const double*)r_.data(), L.b.data());
+            h32_->vcycle(s_);
+            launch((k_convert<float, double>), n0_, s_, n0_, (const float*)L.x(), z_.data());
+        } else {
+            Level<double>& L = h64_->lv[0];
+            FEM_HIP(hipMemcpyAsync(L.b.data(), r_.data(), n0_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
+            h64_->vcycle(s_);
+            FEM_HIP(hipMemcpyAsync(z_.data(), L.x(), n0_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
+        }
+    }
+
+    double read_scalar(size_t offset_doubles) {
+        double v = 0.0;
+        FEM_HIP(hipMemcpyAsync(&v, reinterpret_cast<const double*>(scal_.data()) + offset_doubles, sizeof(double),
+                               hipMemcpyDeviceToHost, s_));
+        FEM_HIP(hipStreamSynchronize(s_));
+        return v;
+    }
+
+    double true_residual() {
+        // |b - A x| / |b| in FP64 over the free unknowns (fixed rows are 0 in both)
+        Level<double>& L = lv64_[0];
+        prepare(L, x_.data(), s_);
+        const unsigned nb = nblocks(L.nodes());
+        with_op(L, [&](auto op) {
+            launch_blocks((k_pcg_start<decltype(op)>), nb, s_, op, (const double*)x_.data(),
+                               (const double*)b_.data(), tmp_.data(), partA_.data(), partB_.data());
+        });
+        launch_blocks(k_sum, 1, s_, (const double*)partA_.data(), (int)nb, red_.data());
+        launch_blocks(k_sum, 1, s_, (const double*)partB_.data(), (int)nb, red_.data() + 1);
+        double v[2];
+        FEM_HIP(hipMemcpyAsync(v, red_.data(), 2 * sizeof(double), hipMemcpyDeviceToHost, s_));
+        FEM_HIP(hipStreamSynchronize(s_));
+        const double bn = std::sqrt(v[0]), rn = std::sqrt(v[1]);
+        return bn > 0.0 ? rn / bn : rn;
+    }
+
+    static std::string rust_e2(double v) {
+        char buf[64];
+        std::snprintf(buf, sizeof buf, "%.2e", v);
+        std::string t(buf);
+        const size_t e = t.find('e');
+        if (e == std::string::npos) return t;
+        return t.substr(0, e) + "e" + std::to_string(std::atoi(t.c_str() + e + 1));
+    }
+
+    // multigrid.rs:692-753, with FP64 vectors and scalars on the device.
+    void pcg(int (*cancelled)(void*), void* ctx, fem_stats& out) {
+        Level<double>& L0 = lv64_[0];
+        const size_t n = n0_;
+        const unsigned nbn = nblocks(L0.nodes());  // partials of node kernels
+        const unsigned nbv = nblocks(n);          // partials of vector kernels
+        PcgScalars* sc = scal_.data();
+        uint64_t total_it = 0;
+        int restarts = 0;
+        for (;;) {
+            prepare(L0, x_.data(), s_);
+            with_op(L0, [&](auto op) {
+                launch_blocks((k_pcg_start<decltype(op)>), nbn, s_, op,
+                                   (const double*)x_.data(), (const double*)b_.data(), r_.data(), partA_.data(),
+                                   partB_.data());
+            });
+            launch_blocks(k_pcg_scalars, 1, s_, sc, (const double*)partA_.data(),
+                               (const double*)partB_.data(), (int)nbn, 0);
+            const double bb = read_scalar(offsetof(PcgScalars, bb) / sizeof(double));
+            const double norm_b = std::sqrt(bb);
+            if (norm_b == 0.0) {
+                launch(k_fill<double>, n, s_, n, x_.data(), 0.0);
+                out.iterations = 0;
+                out.residual = 0.0;
+                out.true_residual = 0.0;
+                return;
+            }
+            precondition();
+            launch_blocks((k_pcg_rz<float>), nbv, s_, n, (const float*)nullptr, z_.data(),
+                               (const double*)r_.data(), partA_.data());
+            launch_blocks(k_pcg_scalars, 1, s_, sc, (const double*)partA_.data(),
+                               (const double*)nullptr, (int)nbv, 3);
+            launch(k_pcg_dir, n, s_, n, (const PcgScalars*)sc, (const double*)z_.data(), p_.data(), 1);
+            uint64_t it = 0;
+            double residual = 0.0;
+            for (;; ++it) {
+                residual = std::sqrt(read_scalar(offsetof(PcgScalars, rr) / sizeof(double))) / norm_b;
+                if (residual <= opt_.tol) break;
+                if (total_it + it == opt_.max_iter) {
+                    out.iterations = total_it + it;
+                    out.residual = residual;
+                    out.restarts = restarts;
+                    throw Error("conjugate gradients did not converge: the residual is " + rust_e2(residual) + " after " +
+                                std::to_string(opt_.max_iter) + " iterations");
+                }
+                if (cancelled && cancelled(ctx)) throw Error("cancelled");
+                prof_.begin(1, s_);
+                prepare(L0, p_.data(), s_);
+                with_op(L0, [&](auto op) {
+                    launch_blocks((k_pcg_q<decltype(op)>), nbn, s_, op, (const double*)p_.data(),
+                                       q_.data(), partA_.data());
+                });
+                prof_.end(s_);
+                launch_blocks(k_pcg_scalars, 1, s_, sc, (const double*)partA_.data(),
+                                   (const double*)nullptr, (int)nbn, 1);
+                if (h32_) {
+                    launch_blocks((k_pcg_update<float>), nbv, s_, n, (const PcgScalars*)sc,
+                                       x_.data(), r_.data(), (const double*)p_.data(), (const double*)q_.data(),
+                                       h32_->lv[0].b.data(), partA_.data());
+                    launch_blocks(k_pcg_scalars, 1, s_, sc, (const double*)partA_.data(),
+                                       (const double*)nullptr, (int)nbv, 2);
+                    h32_->vcycle(s_);
+                    launch_blocks((k_pcg_rz<float>), nbv, s_, n, (const float*)h32_->lv[0].x(),
+                                       z_.data(), (const double*)r_.data(), partA_.data());
+                } else {
+                    launch_blocks((k_pcg_update<double>), nbv, s_, n, (const PcgScalars*)sc,
+                                       x_.data(), r_.data(), (const double*)p_.data(), (const double*)q_.data(),
+                                       h64_->lv[0].b.data(), partA_.data());
+                    launch_blocks(k_pcg_scalars, 1, s_, sc, (const double*)partA_.data(),
+                                       (const double*)nullptr, (int)nbv, 2);
+                    h64_->vcycle(s_);
+                    launch_blocks((k_pcg_rz<double>), nbv, s_, n, (const double*)h64_->lv[0].x(),
+                                       z_.data(), (const double*)r_.data(), partA_.data());
+                }
+                launch_blocks(k_pcg_scalars, 1, s_, sc, (const double*)partA_.data(),
+                                   (const double*)nullptr, (int)nbv, 4);
+                launch(k_pcg_dir, n, s_, n, (const PcgScalars*)sc, (const double*)z_.data(), p_.data(), 0);
+            }
+            total_it += it;
+            const double tr = true_residual();
+            out.iterations = total_it;
+            out.residual = residual;
+            out.true_residual = tr;
+            out.restarts = restarts;
+            if (!opt_.refine || tr <= opt_.tol || restarts >= 5) return;
+            ++restarts;
+        }
+    }
+};
+
+}  // namespace fem
+
+// ---------------------------------------------------------------------------------------------
+// C ABI
+struct fem_solver {
+    std::unique_ptr<fem::Solver> impl;
+};
+
+namespace {
+void put_err(char* err, size_t len, const char* msg) {
+    if (err && len) {
+        std::strncpy(err, msg, len - 1);
+        err[len - 1] = 0;
+    }
+}
+
+fem::BoxInput box_input(const fem_box* box);
+
+template <class F>
+int guarded(char* err, size_t errlen, F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        put_err(err, errlen, e.what());
+        return 1;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+void fem_options_default(fem_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    fem::Options d;
+    o->tol = d.tol;
+    o->max_iter = d.max_iter;
+    o->method = d.method;
+    o->precision = d.precision;
+    o->crossover = d.crossover;
+    o->levels = d.levels;
+    o->cheb_degree = d.degree;
+    o->cheb_lower = d.cheb_lower;
+    o->cheb_upper = d.cheb_upper;
+    o->power_steps = d.power_steps;
+    o->coarse_max = d.coarse_max;
+    o->refine = d.refine;
+    o->device = d.device;
+}
+
+int fem_solver_new_box(const fem_box* box, const fem_options* opt, fem_solver** out, char* err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!out) throw fem::Error("an output handle is needed");
+        *out = nullptr;
+        const fem::BoxInput in = box_input(box);
+        fem::Options o;
+        if (opt) {
+            o.tol = opt->tol;
+            o.max_iter = opt->max_iter;
+            o.method = opt->method;
+            o.precision = opt->precision;
+            o.crossover = opt->crossover;
+            o.levels = opt->levels;
+            o.degree = opt->cheb_degree > 0 ? opt->cheb_degree : 3;
+            o.cheb_lower = opt->cheb_lower;
+            o.cheb_upper = opt->cheb_upper;
+            o.power_steps = opt->power_steps > 0 ? opt->power_steps : 20;
+            o.coarse_max = opt->coarse_max;
+            o.refine = opt->refine;
+            o.device = opt->device;
+        }
+        if (o.precision < 0 || o.precision > 2) throw fem::Error("unknown precision");
+        if (o.precision == FEM_PREC_BF16) o.precision = FEM_PREC_FP32;  // BF16 coefficients: not in this build
+        auto* s = new fem_solver;
+        try {
+            s->impl = std::make_unique<fem::Solver>(in, o);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int fem_solver_solve(fem_solver* s, const double* b, double* x, int (*cancelled)(void*), void* ctx, fem_stats* stats,
+                     char* err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!s) throw fem::Error("no solver");
+        s->impl->solve_host(b, x, cancelled, ctx, stats);
+    });
+}
+
+int fem_solver_solve_device(fem_solver* s, const double* d_b, double* d_x, int (*cancelled)(void*), void* ctx,
+                            fem_stats* stats, char* err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!s) throw fem::Error("no solver");
+        s->impl->solve_device(d_b, d_x, cancelled, ctx, stats);
+    });
+}
+
+int fem_solver_method(const fem_solver* s) { return s ? s->impl->method() : -1; }
+
+void fem_solver_set_max_iter(fem_solver* s, uint64_t max_iter) {
+    if (s) s->impl->options().max_iter = max_iter;
+}
+
+void fem_solver_set_tol(fem_solver* s, double tol) {
+    if (s) s->impl->options().tol = tol;
+}
+
+void fem_solver_free(fem_solver* s) { delete s; }
+
+uint64_t fem_solver_free_count(const fem_solver* s) { return s ? s->impl->model().free.size() : 0; }
+
+uint64_t fem_solver_unknown_count(const fem_solver* s) { return s ? s->impl->model().unknowns() : 0; }
+
+int fem_solver_free_list(const fem_solver* s, int64_t* out) {
+    if (!s || !out) return 1;
+    const auto& f = s->impl->model().free;
+    std::memcpy(out, f.data(), f.size() * sizeof(int64_t));
+    return 0;
+}
+
+int fem_solver_point_grid(const fem_solver* s, uint32_t* out) {
+    if (!s || !out) return 1;
+    const auto& m = s->impl->model();
+    for (size_t p = 0; p < m.points; ++p) {
+        const uint32_t g = m.point_grid[p];
+        out[3 * p] = g % m.gn[0];
+        out[3 * p + 1] = (g / m.gn[0]) % m.gn[1];
+        out[3 * p + 2] = g / (m.gn[0] * m.gn[1]);
+    }
+    return 0;
+}
+
+int fem_solver_rhs(fem_solver* s, double* out, char* err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!s) throw fem::Error("no solver");
+        s->impl->rhs(out);
+    });
+}
+
+int fem_solver_apply(fem_solver* s, const double* x, double* y, char* err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!s) throw fem::Error("no solver");
+        s->impl->apply_host(x, y);
+    });
+}
+
+int fem_solver_precondition(fem_solver* s, const double* r, double* z, char* err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!s) throw fem::Error("no solver");
+        s->impl->precondition_host(r, z);
+    });
+}
+
+int fem_solver_level_info(const fem_solver* s, int32_t level, uint64_t* unknowns, double* lmax) {
+    if (!s || !unknowns || !lmax) return 1;
+    try {
+        s->impl->level_info(level, unknowns, lmax);
+        return 0;
+    } catch (...) {
+        return 1;
+    }
+}
+
+const char* fem_solver_device_name(const fem_solver* s) { return s ? s->impl->device_name() : ""; }
+
+void fem_solver_set_profile(fem_solver* s, int on) {
+    if (s) s->impl->set_profile(on != 0);
+}
+
+int fem_solver_profile(fem_solver* s, fem_profile* out) {
+    if (!s || !out) return 1;
+    try {
+        s->impl->profile(out);
+        return 0;
+    } catch (...) {
+        return 1;
+    }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// The model alone (host only).
+struct fem_model {
+    fem::BoxModel m;
+};
+
+namespace {
+fem::BoxInput box_input(const fem_box* box) {
+    if (!box) throw fem::Error("a box is needed");
+    fem::BoxInput in;
+    if (box->kind < 0 || box->kind > 3) throw fem::Error("unknown model kind");
+    in.kind = (fem::Kind)box->kind;
+    for (int d = 0; d < 3; ++d) {
+        in.n[d] = box->n[d];
+        in.len[d] = box->length[d];
+    }
+    if (box->n[0] < 1 || box->n[1] < 1 || box->n[2] < 1) throw fem::Error("each axis needs 1 cell or more");
+    const size_t cells = (size_t)box->n[0] * box->n[1] * box->n[2];
+    const size_t count = in.kind == fem::Kind::HeatHex8 ? cells : 6 * cells;
+    if (!box->coef0) throw fem::Error("the material values are missing");
+    in.coef0.assign(box->coef0, box->coef0 + count);
+    if (in.kind == fem::Kind::ElasticTet4 || in.kind == fem::Kind::ElasticTet10) {
+        if (!box->coef1) throw fem::Error("the Poisson ratios are missing");
+        in.coef1.assign(box->coef1, box->coef1 + count);
+    }
+    for (int f = 0; f < 6; ++f) {
+        in.fixed[f] = box->fixed[f] != 0;
+        in.value[f] = box->value[f];
+        for (int d = 0; d < 3; ++d) in.traction[f][d] = box->traction[3 * f + d];
+    }
+    in.source = box->source;
+    for (int d = 0; d < 3; ++d) in.body[d] = box->body[d];
+    return in;
+}
+}  // namespace
+
+extern "C" {
+
+int fem_model_new_box(const fem_box* box, fem_model** out, char* err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!out) throw fem::Error("an output handle is needed");
+        *out = nullptr;
+        auto* h = new fem_model;
+        try {
+            h->m = fem::build_model(box_input(box));
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+uint64_t fem_model_point_count(const fem_model* h) { return h ? h->m.points : 0; }
+uint64_t fem_model_unknown_count(const fem_model* h) { return h ? h->m.unknowns() : 0; }
+uint64_t fem_model_free_count(const fem_model* h) { return h ? h->m.free.size() : 0; }
+
+int fem_model_free_list(const fem_model* h, int64_t* out) {
+    if (!h || !out) return 1;
+    std::memcpy(out, h->m.free.data(), h->m.free.size() * sizeof(int64_t));
+    return 0;
+}
+
+int fem_model_point_grid(const fem_model* h, uint32_t* out) {
+    if (!h || !out) return 1;
+    const auto& m = h->m;
+    for (size_t p = 0; p < m.points; ++p) {
+        const uint32_t g = m.point_grid[p];
+        out[3 * p] = g % m.gn[0];
+        out[3 * p + 1] = (g / m.gn[0]) % m.gn[1];
+        out[3 * p + 2] = g / (m.gn[0] * m.gn[1]);
+    }
+    return 0;
+}
+
+int fem_model_load(const fem_model* h, double* out) {
+    if (!h || !out) return 1;
+    const auto& m = h->m;
+    for (size_t p = 0; p < m.points; ++p)
+        for (int c = 0; c < m.comps; ++c) out[m.comps * p + c] = m.load[(size_t)m.comps * m.point_grid[p] + c];
+    return 0;
+}
+
+void fem_model_free(fem_model* h) { delete h; }
+
+}  // extern "C"

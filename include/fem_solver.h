@@ -1,0 +1,201 @@
+/*
+ * femahoy_amd: the iterative solve of femahoy's `fem` crate (PCG preconditioned by a geometric /
+ * p-multigrid V-cycle, for steady heat conduction and linear elasticity) on one AMD MI355X.
+ *
+ * A thin C ABI: plain pointers and sizes, no C++ or torch types. Each entry point names the
+ * reference interface it replaces (paths relative to the reference repository root).
+ *
+ * Conventions
+ *  - Return 0 on success. A nonzero return leaves a message in `err` (up to `errlen` bytes, NUL
+ *    terminated). Messages that have a counterpart in the reference use the reference's text.
+ *  - A model is a structured box: nx x ny x nz cells on [0,lx] x [0,ly] x [0,lz]. Heat on hex8
+ *    cells or on tet4 (each cell cut in the 6 tetrahedra of the reference's prism split), or
+ *    linear elasticity on tet4 / tet10. Unknowns are numbered as the reference numbers them
+ *    (crates/fem/src/heat.rs:1165-1208 for grid points, prism.rs:244-284 and tet10.rs:244-272 for
+ *    tetrahedra, unknown 3 node + component for a displacement, elastic_tet.rs:178-183), and the
+ *    vectors of a solve hold the free unknowns in increasing order (heat.rs:503,
+ *    elastic_tet.rs:291).
+ *  - The caller keeps its host arrays; the library copies what it needs into device memory it
+ *    owns. A solver handle is not safe for concurrent calls; distinct handles are.
+ */
+#ifndef FEM_SOLVER_H
+#define FEM_SOLVER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The element and physics of a box model. */
+enum fem_kind {
+    FEM_HEAT_HEX8 = 0,     /* heat.rs:1246 conduction() on box cells */
+    FEM_HEAT_TET4 = 1,     /* heat.rs:972 tet_conduction() */
+    FEM_ELASTIC_TET4 = 2,  /* constant-strain tetrahedron (the p-multigrid coarse operator, pmultigrid.rs:7) */
+    FEM_ELASTIC_TET10 = 3  /* tet10.rs:90 Tet10::stiffness() */
+};
+
+/* solve.rs:47-50 Method, plus "let the crossover choose". */
+enum fem_method { FEM_METHOD_AUTO = -1, FEM_METHOD_DIRECT = 0, FEM_METHOD_MULTIGRID = 1 };
+
+/* The arithmetic of the preconditioner. The outer PCG (residual recurrence, dots, search
+ * direction, operator of the outer iteration, true residual) is FP64 in every mode. */
+enum fem_precision {
+    FEM_PREC_FP64 = 0,  /* FP64 V-cycle */
+    FEM_PREC_FP32 = 1,  /* FP32 V-cycle (storage and arithmetic) */
+    FEM_PREC_BF16 = 2   /* FP32 V-cycle with BF16-stored coefficients, FP32 accumulation */
+};
+
+/* A structured box model. Faces are numbered 0: x=0, 1: x=lx, 2: y=0, 3: y=ly, 4: z=0, 5: z=lz. */
+typedef struct fem_box {
+    int32_t kind;         /* enum fem_kind */
+    int32_t n[3];         /* cells along x, y, z */
+    double length[3];     /* box size */
+    const double* coef0;  /* heat: conductivity of each cell (hex8) or tetrahedron (tet4);
+                             elasticity: Young's modulus of each tetrahedron. Reference order. */
+    const double* coef1;  /* elasticity: Poisson ratio of each tetrahedron; heat: unused */
+    int32_t fixed[6];     /* heat: a face at a fixed temperature; elasticity: a clamped face */
+    double value[6];      /* heat: the temperature of a fixed face */
+    double source;        /* heat: heat source per volume */
+    double body[3];       /* elasticity: force per volume */
+    double traction[18];  /* elasticity: force per area on each face, 3 per face */
+} fem_box;
+
+/* Options of a solver. `fem_options_default` fills the reference's values where it has them. */
+typedef struct fem_options {
+    double tol;             /* relative recurrence residual; solve.rs:42 PCG_TOL = 1e-10 */
+    uint64_t max_iter;      /* solve.rs:43 PCG_MAX_ITER = 500; Solver::with_max_iter (solve.rs:187) */
+    int32_t method;         /* enum fem_method */
+    int32_t precision;      /* enum fem_precision */
+    uint64_t crossover;     /* direct up to this many unknowns with FEM_METHOD_AUTO; solve.rs:18 (3000) */
+    int32_t levels;         /* V-cycle levels including the finest; 0 = as many as halve cleanly */
+    int32_t cheb_degree;    /* Chebyshev-Jacobi smoother degree (pre and post) */
+    double cheb_lower;      /* smoothed range [cheb_lower * lmax, cheb_upper * lmax] of D^-1 A */
+    double cheb_upper;
+    int32_t power_steps;    /* power iterations of the lmax estimate on each level */
+    uint64_t coarse_max;    /* largest dense coarse level (unknowns) */
+    int32_t refine;         /* 1: after convergence of the recurrence, recompute the FP64 true
+                               residual and restart PCG from x while it is above tol (at most 5) */
+    int32_t device;         /* HIP device ordinal */
+} fem_options;
+
+/* solve.rs:81-86 Stats first ({iterations, residual}: the recurrence residual of PCG, the true
+ * residual of the direct solver), then what the GPU solve adds. */
+typedef struct fem_stats {
+    uint64_t iterations;
+    double residual;
+    double true_residual;    /* |b - A x| / |b| in FP64 at the end */
+    double seconds;          /* device time of the solve: right side resident to solution resident */
+    double setup_seconds;    /* hierarchy, eigenvalue estimates, coarse factor */
+    uint64_t levels;
+    uint64_t coarse_unknowns;
+    uint64_t restarts;       /* refinement restarts */
+    int32_t method;          /* enum fem_method (never AUTO) */
+    int32_t precision;       /* enum fem_precision */
+} fem_stats;
+
+typedef struct fem_solver fem_solver;
+
+/* Fills the defaults: tol 1e-10, max_iter 500, AUTO, FP32 V-cycle, crossover 3000, automatic
+ * levels, Chebyshev degree 3 on [0.1, 1.1] lmax with 12 power steps, coarse_max 6000, refine 1. */
+void fem_options_default(fem_options* opt);
+
+/* Replaces Solver::new (crates/fem/src/solve.rs:92-183) for a box model: builds the model, its
+ * numbering and free unknowns, and then the direct factor (a dense FP64 LDL^T on the device,
+ * up to 20000 unknowns) or the multigrid hierarchy. With FEM_METHOD_AUTO the direct solver is
+ * chosen up to `crossover` unknowns, as Crossover::Unknowns (solve.rs:106, 158). Unlike the
+ * reference, multigrid also runs on tetrahedra (solve.rs:132-137 refuses it there). Material
+ * errors use the texts of hex.rs:79 and hex.rs:93. */
+int fem_solver_new_box(const fem_box* box, const fem_options* opt, fem_solver** out, char* err, size_t errlen);
+
+/* Replaces Solver::solve (solve.rs:199-250) with host vectors: `b` and `x` hold the free
+ * unknowns in the reference order; `x` is the start value of PCG and gets the solution.
+ * `cancelled` (may be NULL) is polled once per PCG iteration, as the reference's closure
+ * (multigrid.rs:735-737). Errors: "conjugate gradients did not converge: the residual is {:.2e}
+ * after {} iterations" (multigrid.rs:730-733) and "cancelled" (multigrid.rs:736). */
+int fem_solver_solve(fem_solver* s, const double* b, double* x, int (*cancelled)(void*), void* ctx, fem_stats* stats,
+                     char* err, size_t errlen);
+
+/* The same solve with `b` and `x` already in device memory (HIP pointers, free unknowns in the
+ * reference order): no host transfer inside. */
+int fem_solver_solve_device(fem_solver* s, const double* d_b, double* d_x, int (*cancelled)(void*), void* ctx,
+                            fem_stats* stats, char* err, size_t errlen);
+
+/* Solver::method (solve.rs:191). */
+int fem_solver_method(const fem_solver* s);
+
+/* Solver::with_max_iter (solve.rs:187), in place. */
+void fem_solver_set_max_iter(fem_solver* s, uint64_t max_iter);
+
+/* Sets the relative tolerance of later solves (the reference fixes PCG_TOL). */
+void fem_solver_set_tol(fem_solver* s, double tol);
+
+void fem_solver_free(fem_solver* s);
+
+/* ---- the model's integer structures and right side, for the caller and for parity checks ---- */
+
+/* The number of free unknowns (the length of b and x), and of all unknowns. */
+uint64_t fem_solver_free_count(const fem_solver* s);
+uint64_t fem_solver_unknown_count(const fem_solver* s);
+
+/* The free unknowns in increasing order (heat.rs:503, elastic_tet.rs:291): `out` holds
+ * fem_solver_free_count values. */
+int fem_solver_free_list(const fem_solver* s, int64_t* out);
+
+/* The structured grid point of each point in the reference numbering (3 per point; for tet10
+ * the point on the grid refined twice, corners at even coordinates). `out` holds 3 x points. */
+int fem_solver_point_grid(const fem_solver* s, uint32_t* out);
+
+/* The reduced right side of the model (heat.rs:1219-1242 reduced_rhs: load minus the fixed
+ * values' coupling; elastic_tet.rs:202-266 loads), over the free unknowns in reference order. */
+int fem_solver_rhs(fem_solver* s, double* out, char* err, size_t errlen);
+
+/* y = A x over the free unknowns (reference order, FP64, host vectors): the reduced matrix of
+ * Csr::submatrix(free) (sparse.rs:108) times x, as Csr::mul (sparse.rs:96). */
+int fem_solver_apply(fem_solver* s, const double* x, double* y, char* err, size_t errlen);
+
+/* Applies the V-cycle preconditioner once, z = M^-1 r (host vectors, reference order). */
+int fem_solver_precondition(fem_solver* s, const double* r, double* z, char* err, size_t errlen);
+
+/* Hierarchy facts: the free unknowns of level l, and the Chebyshev bound lmax used there. */
+int fem_solver_level_info(const fem_solver* s, int32_t level, uint64_t* unknowns, double* lmax);
+
+/* The name of the HIP device the solver runs on. */
+const char* fem_solver_device_name(const fem_solver* s);
+
+/* Device time of the hot launches of the last solve, by HIP events on the solver's stream (no
+ * counterpart in the reference: measurement only). Profiling adds 2 events per launch. */
+typedef struct fem_profile {
+    uint64_t smoother_launches;  /* finest-level Chebyshev steps (operator fused with the update) */
+    double smoother_seconds;
+    uint64_t smoother_bytes;     /* algorithmic bytes of one step: x, b, D^-1, d in; d, x out; coefficients */
+    uint64_t outer_launches;     /* FP64 operator applies of the outer PCG (q = A p) */
+    double outer_seconds;
+    uint64_t outer_bytes;        /* p in, q out, FP64 coefficients */
+    uint64_t dofs;               /* unknowns of the finest grid, fixed ones included */
+} fem_profile;
+void fem_solver_set_profile(fem_solver* s, int on);
+int fem_solver_profile(fem_solver* s, fem_profile* out);
+
+/* ---- the model alone, on the host (no device is touched) ---- */
+
+typedef struct fem_model fem_model;
+
+/* The numbering, free unknowns and loads of a box model, as the reference's callers build them
+ * before Solver::new (heat.rs:755-877 and 503; elastic_tet.rs:88-291). */
+int fem_model_new_box(const fem_box* box, fem_model** out, char* err, size_t errlen);
+uint64_t fem_model_point_count(const fem_model* m);
+uint64_t fem_model_unknown_count(const fem_model* m);
+uint64_t fem_model_free_count(const fem_model* m);
+int fem_model_free_list(const fem_model* m, int64_t* out);
+int fem_model_point_grid(const fem_model* m, uint32_t* out);
+/* The load over all unknowns (reference order): heat sources, body forces, face tractions. */
+int fem_model_load(const fem_model* m, double* out);
+void fem_model_free(fem_model* m);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FEM_SOLVER_H */

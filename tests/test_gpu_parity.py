@@ -1,0 +1,134 @@
+"""GPU parity: the device solve against the oracle on the same seeded models (small sizes).
+
+Tolerance: displacement and temperature within 1e-8 relative L2 of the oracle's FP64 direct
+(multifrontal Cholesky) solution; the device solves to 1e-12 relative residual so that the
+solution error is well inside that bar. Integer structures (free unknowns) bit-exact.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL_X = 1e-8
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def heat_case(n, seed=1, tet4=False):
+    rng = np.random.default_rng(seed)
+    count = int(np.prod(n)) * (6 if tet4 else 1)
+    k = np.exp(rng.standard_normal(count))
+    return k, {"x0": 0.0, "x1": 1.0}
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("n", [[8, 8, 8], [16, 16, 16], [16, 8, 4]])
+def test_heat_hex8_multigrid_matches_oracle_direct(fem, oracle, n, precision):
+    k, fixed = heat_case(n)
+    pr = oracle.Problem.heat(n, [1, 1, 1], k, fixed=fixed)
+    xd, _, _ = oracle.Solver(pr, "direct").solve(pr.rhs())
+    s = fem.Solver(fem.Box("heat_hex8", n, [1, 1, 1], k, fixed=fixed), method="multigrid", precision=precision,
+                   tol=1e-12)
+    assert np.array_equal(s.free_list(), pr.free())
+    b = s.rhs()
+    assert rel(b, pr.rhs()) < 1e-12
+    x, st = s.solve(pr.rhs())
+    print(n, precision, st)
+    assert rel(x, xd) < TOL_X
+    assert st.true_residual < 1e-11
+
+
+def test_heat_hex8_apply_matches_oracle_matrix(fem, oracle):
+    n = [6, 5, 4]
+    k, fixed = heat_case(n, seed=3)
+    pr = oracle.Problem.heat(n, [2, 1, 1], k, fixed=fixed)
+    a = pr.reduced()
+    s = fem.Solver(fem.Box("heat_hex8", n, [2, 1, 1], k, fixed=fixed), method="multigrid", precision="fp64")
+    x = np.random.default_rng(5).standard_normal(s.n)
+    assert rel(s.apply(x), a.mul(x)) < 1e-13
+
+
+@pytest.mark.parametrize("kind,n", [("heat_tet4", [8, 8, 8]), ("elastic_tet4", [8, 4, 4]), ("elastic_tet10", [4, 4, 4])])
+def test_tets_apply_matches_oracle_matrix(fem, oracle, kind, n):
+    rng = np.random.default_rng(11)
+    nt = 6 * int(np.prod(n))
+    length = [1.0, 0.8, 1.2]
+    if kind == "heat_tet4":
+        k = rng.uniform(0.5, 2.0, nt)
+        pr = oracle.Problem.heat(n, length, k, fixed={"z0": 1.0, "z1": 0.0}, tet4=True)
+        box = fem.Box(kind, n, length, k, fixed={"z0": 1.0, "z1": 0.0})
+    else:
+        order = 4 if kind == "elastic_tet4" else 10
+        e = rng.uniform(1.0, 3.0, nt)
+        nu = rng.uniform(0.2, 0.4, nt)
+        pr = oracle.Problem.elastic(order, n, length, e, nu, clamped=["x0"], body=(0, 0, -1),
+                                    traction={"x1": (0.3, 0, 0)})
+        box = fem.Box(kind, n, length, e, nu, fixed={"x0": True}, body=(0, 0, -1), traction={"x1": (0.3, 0, 0)})
+    s = fem.Solver(box, method="multigrid", precision="fp64")
+    assert np.array_equal(s.free_list(), pr.free())
+    x = rng.standard_normal(s.n)
+    assert rel(s.apply(x), pr.reduced().mul(x)) < 1e-12
+    assert rel(s.rhs(), pr.rhs()) < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("kind,n", [("heat_tet4", [8, 8, 8]), ("elastic_tet4", [8, 4, 4]), ("elastic_tet10", [8, 4, 4])])
+def test_tets_multigrid_matches_oracle_direct(fem, oracle, kind, n, precision):
+    rng = np.random.default_rng(2)
+    nt = 6 * int(np.prod(n))
+    length = [2.0, 1.0, 1.0]
+    if kind == "heat_tet4":
+        k = np.exp(rng.standard_normal(nt))
+        pr = oracle.Problem.heat(n, length, k, fixed={"x0": 0.0, "x1": 1.0}, tet4=True)
+        box = fem.Box(kind, n, length, k, fixed={"x0": 0.0, "x1": 1.0})
+    else:
+        order = 4 if kind == "elastic_tet4" else 10
+        e = np.ones(nt)
+        nu = np.full(nt, 0.3)
+        pr = oracle.Problem.elastic(order, n, length, e, nu, clamped=["x0"], body=(0, 0, -1))
+        box = fem.Box(kind, n, length, e, nu, fixed={"x0": True}, body=(0, 0, -1))
+    xd, _, _ = oracle.Solver(pr, "direct").solve(pr.rhs())
+    s = fem.Solver(box, method="multigrid", precision=precision, tol=1e-12)
+    x, st = s.solve(pr.rhs())
+    print(kind, n, precision, st)
+    assert rel(x, xd) < TOL_X
+
+
+def test_direct_cantilever_tet4_matches_oracle(fem, oracle):
+    # 40 x 4 x 4 steel cantilever, tet4, clamped at x = 0, 1 MPa on the top face: 3000 free unknowns.
+    n, length = [40, 4, 4], [100.0, 10.0, 10.0]
+    nt = 6 * int(np.prod(n))
+    pr = oracle.Problem.elastic(4, n, length, np.full(nt, 210e3), np.full(nt, 0.3), clamped=["x0"],
+                                traction={"z1": (0, 0, -1.0)})
+    assert pr.sizes.n_free == 3000
+    xd, it, res = oracle.Solver(pr).solve(pr.rhs())
+    s = fem.Solver(fem.Box("elastic_tet4", n, length, np.full(nt, 210e3), np.full(nt, 0.3), fixed={"x0": True},
+                           traction={"z1": (0, 0, -1.0)}))
+    assert s.method() == "direct"
+    x, st = s.solve(s.rhs())
+    assert st.iterations == 0 and st.residual < 1e-10, st
+    assert rel(x, xd) < 1e-9
+
+
+def test_nonconvergence_and_cancel_messages(fem):
+    n = [16, 16, 16]
+    k, fixed = heat_case(n)
+    s = fem.Solver(fem.Box("heat_hex8", n, [1, 1, 1], k, fixed=fixed), method="multigrid").with_max_iter(1)
+    with pytest.raises(fem.FemError, match=r"conjugate gradients did not converge: the residual is .* after 1 iterations"):
+        s.solve(s.rhs())
+    s.with_max_iter(500)
+    with pytest.raises(fem.FemError, match="^cancelled$"):
+        s.solve(s.rhs(), cancelled=lambda: True)
+
+
+def test_solve_is_bitwise_reproducible(fem):
+    n = [16, 16, 16]
+    k, fixed = heat_case(n, seed=9)
+    s = fem.Solver(fem.Box("heat_hex8", n, [1, 1, 1], k, fixed=fixed), method="multigrid")
+    b = s.rhs()
+    x1, s1 = s.solve(b)
+    x2, s2 = s.solve(b)
+    assert s1.iterations == s2.iterations
+    assert np.array_equal(x1, x2)
