@@ -173,6 +173,8 @@ def main():
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--cpu-n", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--degree", type=int, default=3)
+    ap.add_argument("--coarse", choices=["galerkin", "mean"], default="galerkin")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (every box here has one GPU; "
@@ -186,9 +188,11 @@ def main():
     box, _ = heat_box(F, n)
     log(f"setup: heat hex8 {n}^3, mixed and FP64 hierarchies ({args.levels} levels)")
     t0 = time.perf_counter()
-    mixed = F.Solver(box, method="multigrid", precision="fp32", tol=1e-8, levels=args.levels)
+    opts = dict(method="multigrid", tol=1e-8, levels=args.levels, cheb_degree=args.degree,
+                coarse_galerkin=args.coarse == "galerkin")
+    mixed = F.Solver(box, precision="fp32", **opts)
     t1 = time.perf_counter()
-    fp64 = F.Solver(box, method="multigrid", precision="fp64", tol=1e-8, levels=args.levels)
+    fp64 = F.Solver(box, precision="fp64", **opts)
     t2 = time.perf_counter()
     nfree = mixed.n
     log(f"  {nfree} free unknowns on {mixed.device_name()}; setup mixed {t1 - t0:.2f} s, fp64 {t2 - t1:.2f} s")
@@ -200,10 +204,12 @@ def main():
     d_x = hip.malloc(b.nbytes)
     hip.h2d(d_b, b)
 
-    log("mixed (FP32 V-cycle, FP64 PCG): timed solves")
-    lat_m, st_m, prof, tot_m = timed_solves(F, hip, mixed, d_b, d_x, x0, args.steps, args.warmup, profile=True)
+    log("mixed (FP32 V-cycle, FP64 PCG): timed solves (replayed HIP graphs)")
+    lat_m, st_m, _, tot_m = timed_solves(F, hip, mixed, d_b, d_x, x0, args.steps, args.warmup, profile=False)
     x_mixed = np.zeros(nfree)
     hip.d2h(x_mixed, d_x)
+    log("mixed: the same solves again with HIP events around the finest-level kernels (eager launches)")
+    _, _, prof, _ = timed_solves(F, hip, mixed, d_b, d_x, x0, args.steps, 0, profile=True)
     log("FP64 MG-PCG: timed solves")
     lat_d, st_d, _, tot_d = timed_solves(F, hip, fp64, d_b, d_x, x0, args.steps, args.warmup, profile=False)
     x_fp64 = np.zeros(nfree)
@@ -231,6 +237,8 @@ def main():
                     "kernel": "k_cheb (finest-level Chebyshev step, hex8 operator fused with the update, FP32)",
                     "bytes_per_launch": int(prof["smoother_bytes"]),
                     "us_per_launch": round(prof["smoother_s"] * 1e6, 2), "launches": int(prof["smoother_n"]),
+                    "measured": "HIP events on the solver stream around each launch, over a pass of the timed "
+                                "solves with eager launches (the timed pass replays graphs)",
                     "outer_apply": {"kernel": "k_pcg_q (FP64 operator, q = A p, p.q partials)",
                                     "bytes_per_launch": int(prof["outer_bytes"]),
                                     "us_per_launch": round(prof["outer_s"] * 1e6, 2),
@@ -256,7 +264,8 @@ def main():
         "data": "synthetic (seeded lognormal conductivity)",
         "config": {
             "workload": (f"heat hex8 {n}^3 unit cube, k = exp(N(0,1)) per cell (seed 1234), T=0 at x=0 and T=1 at "
-                         f"x=1, {args.levels}-level GMG, Chebyshev(3)-Jacobi, dense FP64 LDLT coarse, tol 1e-8"),
+                         f"x=1, {args.levels}-level GMG ({args.coarse} coarse operators), Chebyshev({args.degree})-Jacobi, dense FP64 LDLT "
+                         "coarse, tol 1e-8"),
             "free_dof": nfree,
             "levels": [lv[0] for lv in levels],
             "mixed": {"mdof_s": round(value, 2), "ms_mean": round(ms_m, 3), "p50_ms": round(1e3 * float(np.percentile(lat_m, 50)), 3),

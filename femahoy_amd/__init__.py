@@ -71,6 +71,7 @@ class _Options(C.Structure):
         ("coarse_max", C.c_uint64),
         ("refine", C.c_int32),
         ("device", C.c_int32),
+        ("coarse_galerkin", C.c_int32),
     ]
 
 
@@ -264,13 +265,14 @@ class Solver:
     def __init__(self, box: Box, method: Optional[str] = None, precision: str = "fp32", tol: float = PCG_TOL,
                  max_iter: int = PCG_MAX_ITER, crossover: int = STEADY_DIRECT_LIMIT, levels: int = 0,
                  cheb_degree: int = 3, cheb_lower: float = 0.1, cheb_upper: float = 1.1, power_steps: int = 20,
-                 coarse_max: int = 6000, refine: bool = True, device: int = 0):
+                 coarse_max: int = 6000, refine: bool = True, device: int = 0, coarse_galerkin: bool = True):
         o = _Options()
         lib().fem_options_default(C.byref(o))
         o.tol, o.max_iter, o.method, o.precision = tol, max_iter, METHODS[method], PRECISIONS[precision]
         o.crossover, o.levels, o.cheb_degree = crossover, levels, cheb_degree
         o.cheb_lower, o.cheb_upper, o.power_steps = cheb_lower, cheb_upper, power_steps
         o.coarse_max, o.refine, o.device = coarse_max, int(refine), device
+        o.coarse_galerkin = int(coarse_galerkin)
         b, keep = box._struct()
         h = C.c_void_p()
         err = _err()

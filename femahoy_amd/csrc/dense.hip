@@ -175,6 +175,12 @@ __global__ void k_diag_of(const double* A, int np, double* dg) {
     if (r < np) dg[r] = A[(size_t)r * np + r];
 }
 
+template <class T>
+__global__ void k_zero(T* a, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) a[i] = T(0);
+}
+
 __global__ void k_identity(double* U, int np) {
     const size_t e = blockIdx.x * (size_t)256 + threadIdx.x;
     if (e >= (size_t)np * np) return;
@@ -188,30 +194,49 @@ __global__ void k_scale_cols(double* U, int np, const double* d) {
     U[e] *= 1.0 / sqrt(d[e % np]);
 }
 
+// out (n rows, leading dimension ld, zero padded) = the leading n x n block of Ainv.
 template <class T>
-__global__ void k_extract(const double* Ainv, int np, int n, T* out) {
+__global__ void k_extract(const double* Ainv, int np, int n, int ld, T* out) {
     const size_t e = blockIdx.x * (size_t)256 + threadIdx.x;
-    if (e >= (size_t)n * n) return;
-    out[e] = (T)Ainv[(e / n) * np + e % n];
+    if (e >= (size_t)n * ld) return;
+    const size_t r = e / ld, c = e % ld;
+    out[e] = c < (size_t)n ? (T)Ainv[r * np + c] : T(0);
 }
 
-// x[map[i]] = sum_j inv[i][j] b[map[j]]: one wave per row, 4 rows per block, b staged in LDS
-// (STAGE) or, for a large direct solve, read through the caches.
-template <class T, bool STAGE>
-__global__ void __launch_bounds__(256) k_coarse_gemv(const T* __restrict__ inv, int n, const int64_t* __restrict__ map,
-                                                     const T* __restrict__ b, T* __restrict__ x) {
-    extern __shared__ unsigned char smem[];
-    T* sb = reinterpret_cast<T*>(smem);
-    if (STAGE) {
-        for (int j = threadIdx.x; j < n; j += 256) sb[j] = b[map[j]];
-        __syncthreads();
-    }
+// bc[j] = b[map[j]] for the n dense unknowns (bc is zero beyond n).
+template <class T>
+__global__ void k_coarse_gather(int n, const int64_t* __restrict__ map, const T* __restrict__ b, T* __restrict__ bc) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j < n) bc[j] = b[map[j]];
+}
+
+// x[map[i]] = sum_j inv[i][j] bc[j]: one wave per row with 16-byte loads, 4 rows per block. Rows
+// and bc are zero padded to ld (a multiple of 64), so whole vectors are read.
+template <class T>
+__global__ void __launch_bounds__(256) k_coarse_gemv(const T* __restrict__ inv, int n, int ld, const T* __restrict__ bc,
+                                                     const int64_t* __restrict__ map, T* __restrict__ x) {
+    constexpr int W = 16 / sizeof(T);
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= n) return;
-    const T* a = inv + (size_t)row * n;
+    const T* a = inv + (size_t)row * ld;
     T s = T(0);
-    for (int j = lane; j < n; j += 64) s += a[j] * (STAGE ? sb[j] : b[map[j]]);
+    for (int j = lane * W; j < n; j += 64 * W) {
+        T av[W], bv[W];
+        if constexpr (W == 4) {
+            const float4 A4 = *reinterpret_cast<const float4*>(a + j);
+            const float4 B4 = *reinterpret_cast<const float4*>(bc + j);
+            av[0] = A4.x; av[1] = A4.y; av[2] = A4.z; av[3] = A4.w;
+            bv[0] = B4.x; bv[1] = B4.y; bv[2] = B4.z; bv[3] = B4.w;
+        } else {
+            const double2 A2 = *reinterpret_cast<const double2*>(a + j);
+            const double2 B2 = *reinterpret_cast<const double2*>(bc + j);
+            av[0] = A2.x; av[1] = A2.y;
+            bv[0] = B2.x; bv[1] = B2.y;
+        }
+#pragma unroll
+        for (int w = 0; w < W; ++w) s += av[w] * bv[w];
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if (lane == 0) x[map[row]] = s;
@@ -297,13 +322,19 @@ void DenseCoarse::factor(double* a, int n, bool fp32, hipStream_t s) {
     DArray<double> inv(nn);
     GemmArgs fg{U.data(), np, U.data(), np, inv.data(), np, np, 0.0, 1.0, 2, 0, 0, 0, nullptr, nullptr};
     launch_gemm(fg, dim3(nb, nb), s);
-    const unsigned eb = (unsigned)(((size_t)n * n + 255) / 256);
+    ld_ = np;  // rows zero padded to a multiple of 64
+    const unsigned eb = (unsigned)(((size_t)n * ld_ + 255) / 256);
+    const unsigned zb = (unsigned)((ld_ + 255) / 256);
     if (fp32) {
-        inv32_.resize((size_t)n * n);
-        hipLaunchKernelGGL(k_extract<float>, dim3(eb), dim3(256), 0, s, inv.data(), np, n, inv32_.data());
+        inv32_.resize((size_t)n * ld_);
+        bc32_.resize(ld_);
+        hipLaunchKernelGGL(k_extract<float>, dim3(eb), dim3(256), 0, s, inv.data(), np, n, ld_, inv32_.data());
+        hipLaunchKernelGGL(k_zero<float>, dim3(zb), dim3(256), 0, s, bc32_.data(), ld_);
     } else {
-        inv64_.resize((size_t)n * n);
-        hipLaunchKernelGGL(k_extract<double>, dim3(eb), dim3(256), 0, s, inv.data(), np, n, inv64_.data());
+        inv64_.resize((size_t)n * ld_);
+        bc64_.resize(ld_);
+        hipLaunchKernelGGL(k_extract<double>, dim3(eb), dim3(256), 0, s, inv.data(), np, n, ld_, inv64_.data());
+        hipLaunchKernelGGL(k_zero<double>, dim3(zb), dim3(256), 0, s, bc64_.data(), ld_);
     }
     FEM_HIP(hipGetLastError());
     FEM_HIP(hipStreamSynchronize(s));
@@ -315,18 +346,18 @@ void DenseCoarse::factor(double* a, int n, bool fp32, hipStream_t s) {
 template <class T>
 void DenseCoarse::solve(const T* b, T* x, hipStream_t s) const {
     const T* inv;
+    T* bc;
     if constexpr (sizeof(T) == 4) {
         if (!fp32_) throw Error("the coarse inverse is FP64");
         inv = inv32_.data();
+        bc = bc32_.data();
     } else {
         if (fp32_) throw Error("the coarse inverse is FP32");
         inv = inv64_.data();
+        bc = bc64_.data();
     }
-    const size_t shm = (size_t)n_ * sizeof(T);
-    if (shm <= 48 * 1024)
-        hipLaunchKernelGGL((k_coarse_gemv<T, true>), dim3((n_ + 3) / 4), dim3(256), shm, s, inv, n_, map_.data(), b, x);
-    else
-        hipLaunchKernelGGL((k_coarse_gemv<T, false>), dim3((n_ + 3) / 4), dim3(256), 0, s, inv, n_, map_.data(), b, x);
+    hipLaunchKernelGGL(k_coarse_gather<T>, dim3((n_ + 255) / 256), dim3(256), 0, s, n_, map_.data(), b, bc);
+    hipLaunchKernelGGL(k_coarse_gemv<T>, dim3((n_ + 3) / 4), dim3(256), 0, s, inv, n_, ld_, (const T*)bc, map_.data(), x);
 }
 
 template void DenseCoarse::solve<float>(const float*, float*, hipStream_t) const;

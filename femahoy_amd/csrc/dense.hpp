@@ -31,10 +31,10 @@ class DenseCoarse {
     double factor_flops() const { return factor_flops_; }
 
    private:
-    int n_ = 0, np_ = 0;
+    int n_ = 0, np_ = 0, ld_ = 0;
     bool fp32_ = false;
-    DArray<double> inv64_;
-    DArray<float> inv32_;
+    DArray<double> inv64_, bc64_;
+    DArray<float> inv32_, bc32_;
     DArray<int64_t> map_;
     double factor_seconds_ = 0.0, factor_flops_ = 0.0;
 };

@@ -52,9 +52,10 @@ __device__ __forceinline__ void node_ijk(const GridDims& g, size_t node, int& i,
     k = (int)(r / (size_t)g.ny);
 }
 
-// Sum of v over the block, deterministic; the result is valid in thread 0.
-__device__ __forceinline__ double block_sum(double v) {
-    __shared__ double part[BLOCK / 64];
+// Sum of v over a block of NT threads, deterministic; the result is valid in thread 0.
+template <int NT>
+__device__ __forceinline__ double block_sum_t(double v) {
+    __shared__ double part[NT / 64];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -62,10 +63,15 @@ __device__ __forceinline__ double block_sum(double v) {
     __syncthreads();
     double s = 0.0;
     if (threadIdx.x == 0)
-        for (int w = 0; w < BLOCK / 64; ++w) s += part[w];
+        for (int w = 0; w < NT / 64; ++w) s += part[w];
     __syncthreads();  // `part` may be written again by a following call
     return s;
 }
+
+__device__ __forceinline__ double block_sum(double v) { return block_sum_t<BLOCK>(v); }
+
+// The block size of the kernels that sum block partials.
+constexpr int RBLOCK = 1024;
 
 // ------------------------------------------------------------------ operator kernels
 // y = A x on the unknowns; 0 on fixed nodes.
@@ -243,9 +249,10 @@ __global__ void __launch_bounds__(BLOCK) k_restrict(GridDims fg, GridDims cg, co
 #pragma unroll
                     for (int dx = -1; dx <= 1; ++dx) {
                         const int a = fi + dx, b = fj + dy, cz = fk + dz;
-                        if (a < 0 || b < 0 || cz < 0 || a >= fg.nx || b >= fg.ny || cz >= fg.nz) continue;
-                        const T w = T(dx ? 0.5 : 1.0) * T(dy ? 0.5 : 1.0) * T(dz ? 0.5 : 1.0);
-                        const size_t f = a + (size_t)fg.nx * (b + (size_t)fg.ny * cz);
+                        const bool v = a >= 0 && b >= 0 && cz >= 0 && a < fg.nx && b < fg.ny && cz < fg.nz;
+                        const T w = v ? T(dx ? 0.5 : 1.0) * T(dy ? 0.5 : 1.0) * T(dz ? 0.5 : 1.0) : T(0);
+                        const size_t f = min(max(a, 0), fg.nx - 1) +
+                                         (size_t)fg.nx * (min(max(b, 0), fg.ny - 1) + (size_t)fg.ny * min(max(cz, 0), fg.nz - 1));
 #pragma unroll
                         for (int c = 0; c < C; ++c) acc[c] += w * rf[C * f + c];
                     }
@@ -365,15 +372,15 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_dir(size_t n, const PcgScalars* _
     p[u] = first ? z[u] : z[u] + s->beta * p[u];
 }
 
-// Sum of `count` partials in a fixed order, by one block of BLOCK threads.
+// Sum of `count` partials in a fixed order, by one block of RBLOCK threads.
 __device__ __forceinline__ double sum_partials(const double* part, int count) {
     double v = 0.0;
-    for (int t = threadIdx.x; t < count; t += BLOCK) v += part[t];
-    return block_sum(v);
+    for (int t = threadIdx.x; t < count; t += RBLOCK) v += part[t];
+    return block_sum_t<RBLOCK>(v);
 }
 
 // what: 0 start (bb, rr), 1 alpha = rz / pq, 2 rr, 3 rz (first: rz = rz'), 4 beta = rz'/rz then rz = rz'
-__global__ void __launch_bounds__(BLOCK) k_pcg_scalars(PcgScalars* s, const double* part_a, const double* part_b,
+__global__ void __launch_bounds__(RBLOCK) k_pcg_scalars(PcgScalars* s, const double* part_a, const double* part_b,
                                                        int count, int what) {
     const double a = sum_partials(part_a, count);
     double b = 0.0;
@@ -447,7 +454,7 @@ __global__ void __launch_bounds__(BLOCK) k_dot(size_t n, const T* __restrict__ a
 }
 
 // out[0] = sum of the partials.
-__global__ void __launch_bounds__(BLOCK) k_sum(const double* part, int count, double* out) {
+__global__ void __launch_bounds__(RBLOCK) k_sum(const double* part, int count, double* out) {
     const double v = sum_partials(part, count);
     if (threadIdx.x == 0) out[0] = v;
 }

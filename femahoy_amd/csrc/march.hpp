@@ -1,0 +1,279 @@
+// Column-marching kernels of the hex8 operator with a scalar coefficient per cell (heat.rs:1246
+// conduction matrices, node-centric). One thread owns a column (i, j) of nodes and walks a chunk of
+// march_kz<T> planes along z, keeping the 3 x 3 x 3 window of x and the 2 x 2 x 2 cells of k in
+// registers, so each step loads one new plane instead of 27 x values and 8 cells per node.
+//
+// Two load shapes, chosen by the arithmetic type from timings on a 129^3 grid:
+//  - FP32: every thread loads the 9 x values and 4 cells of its new plane itself (clamped
+//    addresses, out-of-grid values selected to 0), 4 planes per thread.
+//  - FP64: a wave owns 62 consecutive columns plus a halo lane on each side; each lane loads the
+//    3 x values (rows j-1, j, j+1) and 2 cells of its own column and takes the x-neighbours from
+//    the lanes beside it with ds_bpermute, 8 planes per thread. This cuts the vector loads of a
+//    plane from 13 to 5 per lane.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace fem {
+
+template <class T>
+struct MarchShape;
+template <>
+struct MarchShape<float> {
+    static constexpr bool shuffle = false;
+    static constexpr int kz = 4;
+};
+template <>
+struct MarchShape<double> {
+    static constexpr bool shuffle = true;
+    static constexpr int kz = 8;
+};
+
+// Columns a block of BLOCK threads owns.
+template <class T>
+constexpr int march_cols_per_block() {
+    return MarchShape<T>::shuffle ? (BLOCK / 64) * 62 : BLOCK;
+}
+
+enum MarchMode { MARCH_APPLY = 0, MARCH_RESIDUAL = 1, MARCH_CHEB = 2, MARCH_PCG_Q = 3, MARCH_PCG_START = 4 };
+
+template <class T>
+struct MarchArgs {
+    GridDims g;
+    const T* k;     // per cell
+    T c[8];         // conduction pattern coefficients for k = 1
+    const T* x;     // input field (p for MARCH_PCG_Q)
+    T* y;           // output: y, r, x_out, q
+    const T* b;     // right side (residual, cheb, pcg_start)
+    T* d;           // Chebyshev direction
+    const T* invd;  // D^-1
+    T c1, c2;       // Chebyshev coefficients
+    double* part;   // block partials (pcg_q: p.q; pcg_start: b.b)
+    double* part2;  // pcg_start: r.r
+};
+
+// The value of v held by lane src of the wave.
+template <class T>
+__device__ __forceinline__ T lane_value(T v, int src) {
+    if constexpr (sizeof(T) == 4) {
+        return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
+    } else {
+        const long long bits = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_ds_bpermute(src << 2, (int)(bits & 0xffffffffll));
+        const int hi = __builtin_amdgcn_ds_bpermute(src << 2, (int)(bits >> 32));
+        return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+    }
+}
+
+// A raw buffer resource over n values of T (gfx9 descriptor word 3; n * sizeof(T) < 2^31).
+template <class T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const T* p, size_t n) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)(n * sizeof(T)), 0x00020000);
+}
+
+// The value at element `idx` of a buffer, or 0 when `valid` is false (the offset then lies past
+// the buffer's end and the load returns 0).
+template <class T>
+__device__ __forceinline__ T buffer_value(__amdgpu_buffer_rsrc_t r, bool valid, unsigned idx) {
+    const int off = valid ? (int)(idx * (unsigned)sizeof(T)) : (int)0x80000000u;
+    if constexpr (sizeof(T) == 4) {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    } else {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+        return __longlong_as_double((long long)(((unsigned long long)v[1] << 32) | v[0]));
+    }
+}
+
+// The largest grid (nodes) the marching kernels take: byte offsets of 31 bits.
+constexpr size_t MARCH_MAX_BYTES = size_t(1) << 31;
+
+template <class T, int MODE>
+__global__ void __launch_bounds__(BLOCK) k_march_hex8(MarchArgs<T> a) {
+    constexpr bool SHFL = MarchShape<T>::shuffle;
+    constexpr int KZ = MarchShape<T>::kz;
+    const GridDims g = a.g;
+    const long long cols = (long long)g.nx * g.ny;
+    const int lane = threadIdx.x & 63;
+    long long col;
+    bool own;
+    if constexpr (SHFL) {
+        const long long wave = blockIdx.x * (long long)(BLOCK / 64) + (threadIdx.x >> 6);
+        col = wave * 62 + lane - 1;
+        own = lane >= 1 && lane <= 62 && col < cols;
+    } else {
+        col = blockIdx.x * (long long)BLOCK + threadIdx.x;
+        own = col < cols;
+    }
+    const bool incol = col >= 0 && col < cols;
+    const long long cc = incol ? col : 0;
+    const int i = (int)(cc % g.nx), j = (int)(cc / g.nx);
+    const int k0 = blockIdx.y * KZ;
+    const int k1 = min(k0 + KZ, g.nz);
+    const size_t plane = (size_t)cols;
+    const int ncx = g.nx - 1, ncy = g.ny - 1, ncz = g.nz - 1;
+    const size_t cplane = (size_t)ncx * ncy;
+    // clamped in-plane indices and their validity
+    int jj[3];
+    bool vj[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        vj[t] = incol && j + t - 1 >= 0 && j + t - 1 < g.ny;
+        jj[t] = min(max(j + t - 1, 0), g.ny - 1);
+    }
+    int cj_[2];
+    bool cj[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        cj[t] = incol && j - 1 + t >= 0 && j - 1 + t < ncy;
+        cj_[t] = min(max(j - 1 + t, 0), max(ncy - 1, 0));
+    }
+    const int srcl = max(lane - 1, 0), srcr = min(lane + 1, 63);
+    const bool hasl = i >= 1, hasr = i + 1 < g.nx;
+    // Loads through buffer resources: an out-of-grid neighbour gets an offset past the end of
+    // the buffer, which the hardware range check turns into 0 without touching memory, so a plane
+    // is loaded without branches or selects.
+    const __amdgpu_buffer_rsrc_t rx = make_rsrc(a.x, (size_t)g.nx * g.ny * g.nz);
+    const __amdgpu_buffer_rsrc_t rk = make_rsrc(a.k, (size_t)ncx * ncy * ncz);
+    auto load_plane = [&](int kk, T out[9]) {
+        const bool vk = kk >= 0 && kk < g.nz;
+        const unsigned base = (unsigned)kk * (unsigned)plane;
+        if constexpr (SHFL) {
+            T mid[3];
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+                mid[dy] = buffer_value<T>(rx, vk && vj[dy], base + (unsigned)jj[dy] * g.nx + i);
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy) {
+                const T l = lane_value(mid[dy], srcl), r = lane_value(mid[dy], srcr);
+                out[0 + 3 * dy] = hasl ? l : T(0);
+                out[1 + 3 * dy] = mid[dy];
+                out[2 + 3 * dy] = hasr ? r : T(0);
+            }
+        } else {
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx) {
+                    const int p = i + dx - 1;
+                    out[dx + 3 * dy] = buffer_value<T>(rx, vk && vj[dy] && p >= 0 && p < g.nx,
+                                                       base + (unsigned)jj[dy] * g.nx + p);
+                }
+        }
+    };
+    auto load_cells = [&](int kc, T out[4]) {
+        const bool vk = kc >= 0 && kc < ncz;
+        const unsigned base = (unsigned)kc * (unsigned)cplane;
+        if constexpr (SHFL) {
+            T mid[2];
+#pragma unroll
+            for (int cy = 0; cy < 2; ++cy)
+                mid[cy] = buffer_value<T>(rk, vk && cj[cy] && i < ncx, base + (unsigned)cj_[cy] * ncx + i);
+#pragma unroll
+            for (int cy = 0; cy < 2; ++cy) {
+                const T l = lane_value(mid[cy], srcl);
+                out[0 + 2 * cy] = hasl ? l : T(0);
+                out[1 + 2 * cy] = mid[cy];
+            }
+        } else {
+#pragma unroll
+            for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+                for (int cx = 0; cx < 2; ++cx) {
+                    const int p = i - 1 + cx;
+                    out[cx + 2 * cy] = buffer_value<T>(rk, vk && cj[cy] && p >= 0 && p < ncx,
+                                                       base + (unsigned)cj_[cy] * ncx + p);
+                }
+        }
+    };
+    T xv[27], kv[8];  // x planes k-1, k, k+1 (9 each); cells of planes k-1, k (4 each)
+    load_plane(k0 - 1, xv);
+    load_plane(k0, xv + 9);
+    load_cells(k0 - 1, kv);
+    const bool fxi = ((g.fix & 1u) && i == 0) || ((g.fix & 2u) && i == g.nx - 1);
+    const bool fxj = ((g.fix & 4u) && j == 0) || ((g.fix & 8u) && j == g.ny - 1);
+    double acc1 = 0.0, acc2 = 0.0;
+    for (int kk = k0; kk < k1; ++kk) {
+        load_plane(kk + 1, xv + 18);
+        load_cells(kk, kv + 4);
+        T acc = T(0);
+#pragma unroll
+        for (int cz = 0; cz < 2; ++cz)
+#pragma unroll
+            for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+                for (int cx = 0; cx < 2; ++cx) {
+                    const int lx = 1 - cx, ly = 1 - cy, lz = 1 - cz;
+                    T s = T(0);
+#pragma unroll
+                    for (int qz = 0; qz < 2; ++qz)
+#pragma unroll
+                        for (int qy = 0; qy < 2; ++qy)
+#pragma unroll
+                            for (int qx = 0; qx < 2; ++qx)
+                                s += a.c[(qx ^ lx) | ((qy ^ ly) << 1) | ((qz ^ lz) << 2)] *
+                                     xv[(cx + qx) + 3 * (cy + qy) + 9 * (cz + qz)];
+                    acc += kv[cx + 2 * cy + 4 * cz] * s;
+                }
+        if (own) {
+            const bool fixed = fxi || fxj || ((g.fix & 16u) && kk == 0) || ((g.fix & 32u) && kk == g.nz - 1);
+            const size_t u = (size_t)kk * plane + (size_t)col;
+            const T xc = xv[13];
+            if (MODE == MARCH_APPLY) {
+                a.y[u] = fixed ? T(0) : acc;
+            } else if (MODE == MARCH_RESIDUAL) {
+                a.y[u] = fixed ? T(0) : a.b[u] - acc;
+            } else if (MODE == MARCH_CHEB) {
+                if (fixed) {
+                    a.d[u] = T(0);
+                    a.y[u] = T(0);
+                } else {
+                    const T r = a.b[u] - acc;
+                    const T dn = a.c1 * a.d[u] + a.c2 * a.invd[u] * r;
+                    a.d[u] = dn;
+                    a.y[u] = xc + dn;
+                }
+            } else if (MODE == MARCH_PCG_Q) {
+                const T q = fixed ? T(0) : acc;
+                a.y[u] = q;
+                acc1 += (double)xc * (double)q;
+            } else if (MODE == MARCH_PCG_START) {
+                const T bu = a.b[u];
+                const T r = fixed ? T(0) : bu - acc;
+                a.y[u] = r;
+                acc1 += (double)bu * (double)bu;
+                acc2 += (double)r * (double)r;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 18; ++t) xv[t] = xv[t + 9];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) kv[t] = kv[t + 4];
+    }
+    if (MODE == MARCH_PCG_Q || MODE == MARCH_PCG_START) {
+        acc1 = block_sum(acc1);
+        if (MODE == MARCH_PCG_START) acc2 = block_sum(acc2);
+        if (threadIdx.x == 0) {
+            const size_t blk = blockIdx.y * (size_t)gridDim.x + blockIdx.x;
+            a.part[blk] = acc1;
+            if (MODE == MARCH_PCG_START) a.part2[blk] = acc2;
+        }
+    }
+}
+
+// The launch grid of the column-marching kernels.
+template <class T>
+inline dim3 march_grid(const GridDims& g) {
+    const size_t cols = (size_t)g.nx * g.ny;
+    const size_t per = march_cols_per_block<T>();
+    constexpr int KZ = MarchShape<T>::kz;
+    return dim3((unsigned)((cols + per - 1) / per), (unsigned)((g.nz + KZ - 1) / KZ));
+}
+
+// Blocks of a march launch, the partials a reducing march kernel writes.
+template <class T>
+inline size_t march_blocks(const GridDims& g) {
+    const dim3 d = march_grid<T>(g);
+    return (size_t)d.x * d.y;
+}
+
+}  // namespace fem

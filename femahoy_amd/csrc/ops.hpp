@@ -41,7 +41,19 @@ struct Hex8K {
         return k[ex + (size_t)(g.nx - 1) * (ey + (size_t)(g.ny - 1) * ez)];
     }
 
+    // Branch-free: addresses clamped into the grid, out-of-grid values selected to 0.
     __device__ __forceinline__ T apply(const T* x, int i, int j, int kk) const {
+        int ia[3], ja[3], ka[3];
+        bool va[3], vb[3], vc[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            va[t] = i + t - 1 >= 0 && i + t - 1 < g.nx;
+            vb[t] = j + t - 1 >= 0 && j + t - 1 < g.ny;
+            vc[t] = kk + t - 1 >= 0 && kk + t - 1 < g.nz;
+            ia[t] = min(max(i + t - 1, 0), g.nx - 1);
+            ja[t] = min(max(j + t - 1, 0), g.ny - 1);
+            ka[t] = min(max(kk + t - 1, 0), g.nz - 1);
+        }
         T xv[27];
 #pragma unroll
         for (int dz = 0; dz < 3; ++dz)
@@ -49,9 +61,23 @@ struct Hex8K {
             for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
                 for (int dx = 0; dx < 3; ++dx) {
-                    const int a = i + dx - 1, b = j + dy - 1, c2 = kk + dz - 1;
-                    const bool in = a >= 0 && b >= 0 && c2 >= 0 && a < g.nx && b < g.ny && c2 < g.nz;
-                    xv[dx + 3 * dy + 9 * dz] = in ? x[a + (size_t)g.nx * (b + (size_t)g.ny * c2)] : T(0);
+                    const T v = x[ia[dx] + (size_t)g.nx * (ja[dy] + (size_t)g.ny * ka[dz])];
+                    xv[dx + 3 * dy + 9 * dz] = (va[dx] && vb[dy] && vc[dz]) ? v : T(0);
+                }
+        const int ncx = g.nx - 1, ncy = g.ny - 1, ncz = g.nz - 1;
+        T kv[8];
+#pragma unroll
+        for (int cz = 0; cz < 2; ++cz)
+#pragma unroll
+            for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+                for (int cx = 0; cx < 2; ++cx) {
+                    const int ex = i - 1 + cx, ey = j - 1 + cy, ez = kk - 1 + cz;
+                    const bool v = ex >= 0 && ey >= 0 && ez >= 0 && ex < ncx && ey < ncy && ez < ncz;
+                    const int cxx = min(max(ex, 0), max(ncx - 1, 0)), cyy = min(max(ey, 0), max(ncy - 1, 0)),
+                              czz = min(max(ez, 0), max(ncz - 1, 0));
+                    const T kval = k[cxx + (size_t)ncx * (cyy + (size_t)ncy * czz)];
+                    kv[cx + 2 * cy + 4 * cz] = v ? kval : T(0);
                 }
         T acc = T(0);
 #pragma unroll
@@ -60,7 +86,7 @@ struct Hex8K {
             for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
                 for (int cx = 0; cx < 2; ++cx) {
-                    const T ke = cell_k(i - 1 + cx, j - 1 + cy, kk - 1 + cz);
+                    const T ke = kv[cx + 2 * cy + 4 * cz];
                     const int lx = 1 - cx, ly = 1 - cy, lz = 1 - cz;
                     T s = T(0);
 #pragma unroll
@@ -101,6 +127,7 @@ struct Stencil27 {
     GridDims g;
     const T* a;
 
+    // Branch-free: a neighbour outside the grid is read at the node itself and selected to 0.
     __device__ __forceinline__ T apply(const T* x, int i, int j, int k) const {
         const size_t n = g.nodes();
         const size_t me = i + (size_t)g.nx * (j + (size_t)g.ny * k);
@@ -110,11 +137,14 @@ struct Stencil27 {
             const int dx = STENCIL_OFF[s][0], dy = STENCIL_OFF[s][1], dz = STENCIL_OFF[s][2];
             const long off = dx + (long)g.nx * (dy + (long)g.ny * dz);
             const int fi = i + dx, fj = j + dy, fk = k + dz;
-            if (fi >= 0 && fj >= 0 && fk >= 0 && fi < g.nx && fj < g.ny && fk < g.nz)
-                acc += a[s * n + me] * x[me + off];
+            const bool vf = fi >= 0 && fj >= 0 && fk >= 0 && fi < g.nx && fj < g.ny && fk < g.nz;
             const int bi = i - dx, bj = j - dy, bk = k - dz;
-            if (bi >= 0 && bj >= 0 && bk >= 0 && bi < g.nx && bj < g.ny && bk < g.nz)
-                acc += a[s * n + me - off] * x[me - off];
+            const bool vb = bi >= 0 && bj >= 0 && bk >= 0 && bi < g.nx && bj < g.ny && bk < g.nz;
+            const size_t f = vf ? me + off : me;
+            const size_t b = vb ? me - off : me;
+            const T af = a[s * n + me], xf = x[f];
+            const T ab = a[s * n + b], xb = x[b];
+            acc += (vf ? af * xf : T(0)) + (vb ? ab * xb : T(0));
         }
         return acc;
     }

@@ -34,6 +34,21 @@ __global__ void k_hex_cells_from_k(int ncx, int ncy, int ncz, const double* __re
     for (int e = 0; e < 64; ++e) Kc[cell * 64 + e] = acc[e];
 }
 
+// A coarse hex8 level rediscretized: the mean conductivity of the 8 children of each coarse cell.
+__global__ void k_hex_average(int ncx, int ncy, int ncz, const double* __restrict__ kf, double* __restrict__ kc) {
+    const size_t cell = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    const size_t nc = (size_t)ncx * ncy * ncz;
+    if (cell >= nc) return;
+    const int I = (int)(cell % ncx), J = (int)((cell / ncx) % ncy), K = (int)(cell / ((size_t)ncx * ncy));
+    const int nfx = 2 * ncx, nfy = 2 * ncy;
+    double s = 0.0;
+    for (int c = 0; c < 8; ++c) {
+        const int fx = 2 * I + (c & 1), fy = 2 * J + ((c >> 1) & 1), fz = 2 * K + (c >> 2);
+        s += kf[fx + (size_t)nfx * (fy + (size_t)nfy * fz)];
+    }
+    kc[cell] = s / 8.0;
+}
+
 // Level l+1 from level l cell matrices: K_C = sum_c P_c^T K_c P_c.
 __global__ void k_hex_cells_from_cells(int ncx, int ncy, int ncz, const double* __restrict__ Kf,
                                        double* __restrict__ Kc) {

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -17,6 +18,7 @@
 #include "dense.hpp"
 #include "kernels.hpp"
 #include "model.hpp"
+#include "march.hpp"
 #include "setup_kernels.hpp"
 
 namespace fem {
@@ -30,6 +32,12 @@ template <class... KArgs, class... Args>
 void launch_blocks(void (*kernel)(KArgs...), unsigned blocks, hipStream_t s, Args&&... args) {
     if (blocks == 0) return;
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(BLOCK), 0, s, static_cast<KArgs>(args)...);
+}
+
+// Launches a partial-sum kernel: one block of RBLOCK threads.
+template <class... KArgs, class... Args>
+void launch_reduce(void (*kernel)(KArgs...), hipStream_t s, Args&&... args) {
+    hipLaunchKernelGGL(kernel, dim3(1), dim3(RBLOCK), 0, s, static_cast<KArgs>(args)...);
 }
 
 // Launches `kernel` with one thread per item of n items.
@@ -56,6 +64,8 @@ struct Level {
     double lmax = 0.0;
     std::vector<T> c1, c2;     // Chebyshev step coefficients
     size_t nodes() const { return g.nodes(); }
+    // the hex8 operator through the column-marching kernels (31-bit byte offsets)
+    bool march() const { return op == OpKind::Hex8K && nodes() * sizeof(T) < MARCH_MAX_BYTES; }
     size_t dofs() const { return g.nodes() * comps; }
     size_t ncell() const { return (size_t)cells[0] * cells[1] * cells[2]; }
     T* x() { return cur ? xb.data() : xa.data(); }
@@ -105,6 +115,21 @@ void with_op(Level<T>& L, F&& f) {
     }
 }
 
+// The column-marching arguments of a hex8 level with a coefficient per cell.
+template <class T>
+MarchArgs<T> march_args(Level<T>& L) {
+    MarchArgs<T> m{};
+    m.g = L.g;
+    m.k = L.coef.data();
+    hex_c<T>(L.h, m.c);
+    return m;
+}
+
+template <class T, int MODE>
+void launch_march(const MarchArgs<T>& m, hipStream_t s) {
+    hipLaunchKernelGGL((k_march_hex8<T, MODE>), march_grid<T>(m.g), dim3(BLOCK), 0, s, m);
+}
+
 // Phase A of an operator whose apply reads an element buffer (tet10); nothing for the others.
 template <class T>
 void prepare(Level<T>& L, const T* x, hipStream_t s) {
@@ -133,12 +158,15 @@ void convert_level(const Level<double>& src, Level<T>& dst, hipStream_t s) {
 }
 
 // Device time of chosen launches by HIP events on the launching stream: begin/end pairs, not
-// nested, category 0 the level-0 smoother step, 1 the outer FP64 operator apply.
+// nested, category 0 the finest-level smoother step, 1 the outer FP64 operator apply. A profiled
+// solve launches its iterations one by one (HIP cannot time events that graph nodes record).
 struct Profiler {
     bool on = false;
     std::vector<hipEvent_t> pool;
     size_t used = 0;
     std::vector<std::pair<int, size_t>> marks;
+    double secs[2] = {0.0, 0.0};
+    uint64_t counts[2] = {0, 0};
 
     hipEvent_t next() {
         if (used == pool.size()) {
@@ -157,20 +185,24 @@ struct Profiler {
         if (!on) return;
         FEM_HIP(hipEventRecord(next(), s));
     }
-    void reset() {
-        used = 0;
-        marks.clear();
+    // Adds the times of `m` (whose events have completed) to the totals.
+    void flush(const std::vector<std::pair<int, size_t>>& m) {
+        for (const auto& k : m) {
+            FEM_HIP(hipEventSynchronize(pool[k.second + 1]));
+            float ms = 0.f;
+            FEM_HIP(hipEventElapsedTime(&ms, pool[k.second], pool[k.second + 1]));
+            secs[k.first] += ms * 1e-3;
+            counts[k.first] += 1;
+        }
     }
-    void collect(double secs[2], uint64_t counts[2]) {
+    void flush_eager() {
+        flush(marks);
+        marks.clear();
+        used = 0;
+    }
+    void reset_totals() {
         secs[0] = secs[1] = 0.0;
         counts[0] = counts[1] = 0;
-        for (const auto& m : marks) {
-            FEM_HIP(hipEventSynchronize(pool[m.second + 1]));
-            float ms = 0.f;
-            FEM_HIP(hipEventElapsedTime(&ms, pool[m.second], pool[m.second + 1]));
-            secs[m.first] += ms * 1e-3;
-            counts[m.first] += 1;
-        }
     }
     ~Profiler() {
         for (hipEvent_t e : pool) (void)hipEventDestroy(e);
@@ -200,6 +232,7 @@ class Hierarchy {
 
     void cycle(size_t l, hipStream_t s) {
         Level<T>& L = lv[l];
+        L.cur = 0;  // every cycle uses the same buffers in the same order (graph capture)
         if (l + 1 == lv.size()) {
             coarse.solve<T>(L.b.data(), L.x(), s);
             return;
@@ -209,32 +242,53 @@ class Hierarchy {
         launch(k_cheb_first<T>, n, s, n, (const T*)L.b.data(), (const T*)L.invd.data(), L.x(), L.d.data(), L.c2[0]);
         for (int st = 1; st < degree; ++st) {
             mark_begin(l, s);
-            prepare(L, L.x(), s);
-            with_op(L, [&](auto op) {
-                launch((k_cheb<decltype(op), T>), L.nodes(), s, op, (const T*)L.x(), L.x2(), L.d.data(),
-                       (const T*)L.b.data(), (const T*)L.invd.data(), L.c1[st], L.c2[st]);
-            });
+            cheb_step(L, L.c1[st], L.c2[st], s);
             mark_end(l, s);
             L.cur ^= 1;
         }
-        prepare(L, L.x(), s);
-        with_op(L, [&](auto op) {
-            launch((k_residual<decltype(op), T>), L.nodes(), s, op, (const T*)L.x(), (const T*)L.b.data(), L.r.data());
-        });
+        if (L.march()) {
+            MarchArgs<T> m = march_args(L);
+            m.x = L.x();
+            m.b = L.b.data();
+            m.y = L.r.data();
+            launch_march<T, MARCH_RESIDUAL>(m, s);
+        } else {
+            prepare(L, L.x(), s);
+            with_op(L, [&](auto op) {
+                launch((k_residual<decltype(op), T>), L.nodes(), s, op, (const T*)L.x(), (const T*)L.b.data(), L.r.data());
+            });
+        }
         Level<T>& Cc = lv[l + 1];
         transfer(L, Cc, true, s);
         cycle(l + 1, s);
         transfer(L, Cc, false, s);
         for (int st = 0; st < degree; ++st) {
             mark_begin(l, s);
-            prepare(L, L.x(), s);
-            with_op(L, [&](auto op) {
-                launch((k_cheb<decltype(op), T>), L.nodes(), s, op, (const T*)L.x(), L.x2(), L.d.data(),
-                       (const T*)L.b.data(), (const T*)L.invd.data(), st == 0 ? T(0) : L.c1[st], L.c2[st]);
-            });
+            cheb_step(L, st == 0 ? T(0) : L.c1[st], L.c2[st], s);
             mark_end(l, s);
             L.cur ^= 1;
         }
+    }
+
+    // d = c1 d + c2 D^-1 (b - A x), x2 = x + d.
+    void cheb_step(Level<T>& L, T c1, T c2, hipStream_t s) {
+        if (L.march()) {
+            MarchArgs<T> m = march_args(L);
+            m.x = L.x();
+            m.y = L.x2();
+            m.d = L.d.data();
+            m.b = L.b.data();
+            m.invd = L.invd.data();
+            m.c1 = c1;
+            m.c2 = c2;
+            launch_march<T, MARCH_CHEB>(m, s);
+            return;
+        }
+        prepare(L, L.x(), s);
+        with_op(L, [&](auto op) {
+            launch((k_cheb<decltype(op), T>), L.nodes(), s, op, (const T*)L.x(), L.x2(), L.d.data(), (const T*)L.b.data(),
+                   (const T*)L.invd.data(), c1, c2);
+        });
     }
 
     void transfer(Level<T>& F, Level<T>& Cc, bool restrict_, hipStream_t s) {
@@ -275,6 +329,7 @@ struct Options {
     uint64_t coarse_max = 6000;
     int refine = 1;
     int device = 0;
+    int coarse_galerkin = 1;
 };
 
 class Solver {
@@ -286,6 +341,8 @@ class Solver {
         FEM_HIP(hipGetDeviceProperties(&prop, o.device));
         device_name_ = std::string(prop.name[0] ? prop.name : "AMD GPU") + " (" + prop.gcnArchName + ")";
         FEM_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+        FEM_HIP(hipHostMalloc((void**)&pinned_, 8 * sizeof(double), hipHostMallocDefault));
+        use_graph_ = std::getenv("FEM_NO_GRAPH") == nullptr;  // eager launches for debugging
         model_ = build_model(in);
         const size_t nfree = model_.free.size();
         if (nfree == 0) throw Error("the model has no free unknown");
@@ -298,7 +355,8 @@ class Solver {
         p_.resize(n0_);
         q_.resize(n0_);
         tmp_.resize(n0_);
-        parts_ = nblocks(n0_) + 1;
+        // partials of the reducing kernels: one per block of a node, vector or march launch
+        parts_ = std::max<size_t>(nblocks(n0_), std::max<size_t>(nblocks(lv64_[0].nodes()), march_blocks<double>(lv64_[0].g))) + 1;
         partA_.resize(parts_);
         partB_.resize(parts_);
         scal_.resize(1);
@@ -334,6 +392,8 @@ class Solver {
     }
 
     ~Solver() {
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        if (pinned_) (void)hipHostFree(pinned_);
         if (s_) (void)hipStreamDestroy(s_);
     }
 
@@ -347,13 +407,11 @@ class Solver {
 
     // Device time and algorithmic bytes of the profiled launches of the last solve.
     void profile(fem_profile* p) {
-        double secs[2];
-        uint64_t counts[2];
-        prof_.collect(secs, counts);
-        p->smoother_launches = counts[0];
-        p->smoother_seconds = secs[0];
-        p->outer_launches = counts[1];
-        p->outer_seconds = secs[1];
+        prof_.flush_eager();
+        p->smoother_launches = prof_.counts[0];
+        p->smoother_seconds = prof_.secs[0];
+        p->outer_launches = prof_.counts[1];
+        p->outer_seconds = prof_.secs[1];
         p->dofs = n0_;
         const Level<double>& L = lv64_[0];
         p->outer_bytes = 16 * n0_ + L.coef.bytes();
@@ -433,7 +491,8 @@ class Solver {
         launch(k_fill<double>, n0_, s_, n0_, x_.data(), 0.0);
         launch(k_scatter, nf, s_, nf, (const int64_t*)free_grid_.data(), d_b, b_.data());
         launch(k_scatter, nf, s_, nf, (const int64_t*)free_grid_.data(), (const double*)d_x, x_.data());
-        prof_.reset();
+        prof_.flush_eager();
+        prof_.reset_totals();
         fem_stats out{};
         out.method = method_;
         out.precision = method_ == FEM_METHOD_DIRECT ? FEM_PREC_FP64 : opt_.precision;
@@ -501,6 +560,9 @@ class Solver {
     std::unique_ptr<Hierarchy<double>> h64_;
     std::unique_ptr<DenseCoarse> direct_;
     Profiler prof_;
+    bool use_graph_ = true;
+    hipGraphExec_t graph_exec_ = nullptr;
+    double* pinned_ = nullptr;  // host-visible scalars read back each iteration
     size_t n0_ = 0;
     int parts_ = 0;
     DArray<double> x_, b_, r_, z_, p_, q_, tmp_, partA_, partB_, red_, fb_, fx_;
@@ -614,7 +676,13 @@ class Solver {
             }
             Cl.g = GridDims{Cl.cells[0] + 1, Cl.cells[1] + 1, Cl.cells[2] + 1, m.fix};
             const size_t nc = Cl.ncell();
-            if (F.op == OpKind::Hex8K || F.op == OpKind::Stencil27) {
+            if (F.op == OpKind::Hex8K && !opt_.coarse_galerkin) {
+                // rediscretized: the same hex8 operator with the mean conductivity of the children
+                Cl.op = OpKind::Hex8K;
+                Cl.coef.resize(nc);
+                launch(k_hex_average, nc, s_, Cl.cells[0], Cl.cells[1], Cl.cells[2], (const double*)F.coef.data(),
+                       Cl.coef.data());
+            } else if (F.op == OpKind::Hex8K || F.op == OpKind::Stencil27) {
                 Cl.op = OpKind::Stencil27;
                 DArray<double> cm(nc * 64);
                 if (F.op == OpKind::Hex8K) {
@@ -790,13 +858,13 @@ class Solver {
         double lam = 0.0;
         for (int it = 0; it < opt_.power_steps; ++it) {
             launch_blocks(k_dot<T>, pb, s_, n, (const T*)v, (const T*)v, partA_.data());
-            launch_blocks(k_sum, 1, s_, (const double*)partA_.data(), (int)pb, red_.data());
+            launch_reduce(k_sum, s_, (const double*)partA_.data(), (int)pb, red_.data());
             launch(k_normalise<T>, n, s_, n, v, (const double*)red_.data());
             prepare(L, v, s_);
             with_op(L, [&](auto op) { launch((k_apply<decltype(op), T>), L.nodes(), s_, op, (const T*)v, w); });
             launch(k_scale<T>, n, s_, n, w, (const T*)L.invd.data());
             launch_blocks(k_dot<T>, pb, s_, n, (const T*)w, (const T*)w, partA_.data());
-            launch_blocks(k_sum, 1, s_, (const double*)partA_.data(), (int)pb, red_.data() + 1);
+            launch_reduce(k_sum, s_, (const double*)partA_.data(), (int)pb, red_.data() + 1);
             double nw = 0.0;
             FEM_HIP(hipMemcpyAsync(&nw, red_.data() + 1, sizeof(double), hipMemcpyDeviceToHost, s_));
             FEM_HIP(hipStreamSynchronize(s_));
@@ -848,11 +916,10 @@ class Solver {
     }
 
     double read_scalar(size_t offset_doubles) {
-        double v = 0.0;
-        FEM_HIP(hipMemcpyAsync(&v, reinterpret_cast<const double*>(scal_.data()) + offset_doubles, sizeof(double),
+        FEM_HIP(hipMemcpyAsync(pinned_, reinterpret_cast<const double*>(scal_.data()) + offset_doubles, sizeof(double),
                                hipMemcpyDeviceToHost, s_));
         FEM_HIP(hipStreamSynchronize(s_));
-        return v;
+        return pinned_[0];
     }
 
     double true_residual() {
@@ -864,8 +931,8 @@ class Solver {
             launch_blocks((k_pcg_start<decltype(op)>), nb, s_, op, (const double*)x_.data(),
                                (const double*)b_.data(), tmp_.data(), partA_.data(), partB_.data());
         });
-        launch_blocks(k_sum, 1, s_, (const double*)partA_.data(), (int)nb, red_.data());
-        launch_blocks(k_sum, 1, s_, (const double*)partB_.data(), (int)nb, red_.data() + 1);
+        launch_reduce(k_sum, s_, (const double*)partA_.data(), (int)nb, red_.data());
+        launch_reduce(k_sum, s_, (const double*)partB_.data(), (int)nb, red_.data() + 1);
         double v[2];
         FEM_HIP(hipMemcpyAsync(v, red_.data(), 2 * sizeof(double), hipMemcpyDeviceToHost, s_));
         FEM_HIP(hipStreamSynchronize(s_));
@@ -880,6 +947,71 @@ class Solver {
         const size_t e = t.find('e');
         if (e == std::string::npos) return t;
         return t.substr(0, e) + "e" + std::to_string(std::atoi(t.c_str() + e + 1));
+    }
+
+
+    // One PCG iteration after the residual check (multigrid.rs:738-750): q = A p, alpha, the x and r
+    // updates, z = M^-1 r, rz, beta, p = z + beta p; the next residual is in the scalars. Captured
+    // once in a HIP graph and replayed (all buffers are fixed per solver); run eagerly when the
+    // launches are profiled.
+    void iteration_body() {
+        Level<double>& L0 = lv64_[0];
+        const size_t n = n0_;
+        const unsigned nbn = nblocks(L0.nodes());
+        const unsigned nbv = nblocks(n);
+        PcgScalars* sc = scal_.data();
+        prof_.begin(1, s_);
+        unsigned nq = nbn;
+        if (L0.march()) {
+            MarchArgs<double> m = march_args(L0);
+            m.x = p_.data();
+            m.y = q_.data();
+            m.part = partA_.data();
+            launch_march<double, MARCH_PCG_Q>(m, s_);
+            const dim3 gm = march_grid<double>(L0.g);
+            nq = gm.x * gm.y;
+        } else {
+            prepare(L0, p_.data(), s_);
+            with_op(L0, [&](auto op) {
+                launch_blocks((k_pcg_q<decltype(op)>), nbn, s_, op, (const double*)p_.data(), q_.data(), partA_.data());
+            });
+        }
+        prof_.end(s_);
+        launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr, (int)nq, 1);
+        if (h32_) {
+            launch_blocks((k_pcg_update<float>), nbv, s_, n, (const PcgScalars*)sc, x_.data(), r_.data(),
+                          (const double*)p_.data(), (const double*)q_.data(), h32_->lv[0].b.data(), partA_.data());
+            launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr, (int)nbv, 2);
+            h32_->vcycle(s_);
+            launch_blocks((k_pcg_rz<float>), nbv, s_, n, (const float*)h32_->lv[0].x(), z_.data(),
+                          (const double*)r_.data(), partA_.data());
+        } else {
+            launch_blocks((k_pcg_update<double>), nbv, s_, n, (const PcgScalars*)sc, x_.data(), r_.data(),
+                          (const double*)p_.data(), (const double*)q_.data(), h64_->lv[0].b.data(), partA_.data());
+            launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr, (int)nbv, 2);
+            h64_->vcycle(s_);
+            launch_blocks((k_pcg_rz<double>), nbv, s_, n, (const double*)h64_->lv[0].x(), z_.data(),
+                          (const double*)r_.data(), partA_.data());
+        }
+        launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr, (int)nbv, 4);
+        launch(k_pcg_dir, n, s_, n, (const PcgScalars*)sc, (const double*)z_.data(), p_.data(), 0);
+    }
+
+    void iterate() {
+        if (!use_graph_ || prof_.on) {  // HIP cannot time events recorded by graph nodes
+            iteration_body();
+            prof_.flush_eager();
+            return;
+        }
+        if (!graph_exec_) {
+            hipGraph_t graph = nullptr;
+            FEM_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+            iteration_body();
+            FEM_HIP(hipStreamEndCapture(s_, &graph));
+            FEM_HIP(hipGraphInstantiate(&graph_exec_, graph, nullptr, nullptr, 0));
+            FEM_HIP(hipGraphDestroy(graph));
+        }
+        FEM_HIP(hipGraphLaunch(graph_exec_, s_));
     }
 
     // multigrid.rs:692-753, with FP64 vectors and scalars on the device.
@@ -898,7 +1030,7 @@ class Solver {
                                    (const double*)x_.data(), (const double*)b_.data(), r_.data(), partA_.data(),
                                    partB_.data());
             });
-            launch_blocks(k_pcg_scalars, 1, s_, sc, (const double*)partA_.data(),
+            launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(),
                                (const double*)partB_.data(), (int)nbn, 0);
             const double bb = read_scalar(offsetof(PcgScalars, bb) / sizeof(double));
             const double norm_b = std::sqrt(bb);
@@ -912,7 +1044,7 @@ class Solver {
             precondition();
             launch_blocks((k_pcg_rz<float>), nbv, s_, n, (const float*)nullptr, z_.data(),
                                (const double*)r_.data(), partA_.data());
-            launch_blocks(k_pcg_scalars, 1, s_, sc, (const double*)partA_.data(),
+            launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(),
                                (const double*)nullptr, (int)nbv, 3);
             launch(k_pcg_dir, n, s_, n, (const PcgScalars*)sc, (const double*)z_.data(), p_.data(), 1);
             uint64_t it = 0;
@@ -928,37 +1060,7 @@ class Solver {
                                 std::to_string(opt_.max_iter) + " iterations");
                 }
                 if (cancelled && cancelled(ctx)) throw Error("cancelled");
-                prof_.begin(1, s_);
-                prepare(L0, p_.data(), s_);
-                with_op(L0, [&](auto op) {
-                    launch_blocks((k_pcg_q<decltype(op)>), nbn, s_, op, (const double*)p_.data(),
-                                       q_.data(), partA_.data());
-                });
-                prof_.end(s_);
-                launch_blocks(k_pcg_scalars, 1, s_, sc, (const double*)partA_.data(),
-                                   (const double*)nullptr, (int)nbn, 1);
-                if (h32_) {
-                    launch_blocks((k_pcg_update<float>), nbv, s_, n, (const PcgScalars*)sc,
-                                       x_.data(), r_.data(), (const double*)p_.data(), (const double*)q_.data(),
-                                       h32_->lv[0].b.data(), partA_.data());
-                    launch_blocks(k_pcg_scalars, 1, s_, sc, (const double*)partA_.data(),
-                                       (const double*)nullptr, (int)nbv, 2);
-                    h32_->vcycle(s_);
-                    launch_blocks((k_pcg_rz<float>), nbv, s_, n, (const float*)h32_->lv[0].x(),
-                                       z_.data(), (const double*)r_.data(), partA_.data());
-                } else {
-                    launch_blocks((k_pcg_update<double>), nbv, s_, n, (const PcgScalars*)sc,
-                                       x_.data(), r_.data(), (const double*)p_.data(), (const double*)q_.data(),
-                                       h64_->lv[0].b.data(), partA_.data());
-                    launch_blocks(k_pcg_scalars, 1, s_, sc, (const double*)partA_.data(),
-                                       (const double*)nullptr, (int)nbv, 2);
-                    h64_->vcycle(s_);
-                    launch_blocks((k_pcg_rz<double>), nbv, s_, n, (const double*)h64_->lv[0].x(),
-                                       z_.data(), (const double*)r_.data(), partA_.data());
-                }
-                launch_blocks(k_pcg_scalars, 1, s_, sc, (const double*)partA_.data(),
-                                   (const double*)nullptr, (int)nbv, 4);
-                launch(k_pcg_dir, n, s_, n, (const PcgScalars*)sc, (const double*)z_.data(), p_.data(), 0);
+                iterate();
             }
             total_it += it;
             const double tr = true_residual();
@@ -1021,6 +1123,7 @@ void fem_options_default(fem_options* o) {
     o->coarse_max = d.coarse_max;
     o->refine = d.refine;
     o->device = d.device;
+    o->coarse_galerkin = d.coarse_galerkin;
 }
 
 int fem_solver_new_box(const fem_box* box, const fem_options* opt, fem_solver** out, char* err, size_t errlen) {
@@ -1043,6 +1146,7 @@ int fem_solver_new_box(const fem_box* box, const fem_options* opt, fem_solver** 
             o.coarse_max = opt->coarse_max;
             o.refine = opt->refine;
             o.device = opt->device;
+            o.coarse_galerkin = opt->coarse_galerkin;
         }
         if (o.precision < 0 || o.precision > 2) throw fem::Error("unknown precision");
         if (o.precision == FEM_PREC_BF16) o.precision = FEM_PREC_FP32;  // BF16 coefficients: not in this build

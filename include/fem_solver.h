@@ -78,6 +78,9 @@ typedef struct fem_options {
     int32_t refine;         /* 1: after convergence of the recurrence, recompute the FP64 true
                                residual and restart PCG from x while it is above tol (at most 5) */
     int32_t device;         /* HIP device ordinal */
+    int32_t coarse_galerkin; /* hex8: 1 Galerkin coarse operators (27-point stencils, as the reference's
+                                P^T A P, multigrid.rs:109); 0 the hex8 operator rediscretized with the
+                                mean conductivity of the 8 children */
 } fem_options;
 
 /* solve.rs:81-86 Stats first ({iterations, residual}: the recurrence residual of PCG, the true
@@ -98,7 +101,8 @@ typedef struct fem_stats {
 typedef struct fem_solver fem_solver;
 
 /* Fills the defaults: tol 1e-10, max_iter 500, AUTO, FP32 V-cycle, crossover 3000, automatic
- * levels, Chebyshev degree 3 on [0.1, 1.1] lmax with 12 power steps, coarse_max 6000, refine 1. */
+ * levels, Chebyshev degree 3 on [0.1, 1.1] lmax with 20 power steps, coarse_max 6000, refine 1,
+ * Galerkin coarse operators. */
 void fem_options_default(fem_options* opt);
 
 /* Replaces Solver::new (crates/fem/src/solve.rs:92-183) for a box model: builds the model, its

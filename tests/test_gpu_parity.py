@@ -132,3 +132,21 @@ def test_solve_is_bitwise_reproducible(fem):
     x2, s2 = s.solve(b)
     assert s1.iterations == s2.iterations
     assert np.array_equal(x1, x2)
+
+
+def test_graph_replays_match_eager_launches(fem, monkeypatch):
+    # The PCG iteration runs as a replayed HIP graph; every replay must give what eager launches give.
+    n = [16, 16, 16]
+    k, fixed = heat_case(n, seed=4)
+    box = fem.Box("heat_hex8", n, [1, 1, 1], k, fixed=fixed)
+    graph = fem.Solver(box, method="multigrid", tol=1e-11)
+    monkeypatch.setenv("FEM_NO_GRAPH", "1")
+    eager = fem.Solver(box, method="multigrid", tol=1e-11)
+    b = graph.rhs()
+    xe, se = eager.solve(b)
+    for _ in range(3):
+        xg, sg = graph.solve(b)
+        assert sg.iterations == se.iterations
+        assert np.array_equal(xg, xe)
+        xe2, _ = eager.solve(b)
+        assert np.array_equal(xe2, xe)
