@@ -72,25 +72,12 @@ class Hip:
         self.lib.hipFree(C.c_void_p(p))
 
 
-class Profile(C.Structure):
-    _fields_ = [("smoother_launches", C.c_uint64), ("smoother_seconds", C.c_double), ("smoother_bytes", C.c_uint64),
-                ("outer_launches", C.c_uint64), ("outer_seconds", C.c_double), ("outer_bytes", C.c_uint64),
-                ("dofs", C.c_uint64)]
-
-
 def get_profile(F, solver):
-    L = F.lib()
-    L.fem_solver_profile.argtypes = [C.c_void_p, C.POINTER(Profile)]
-    L.fem_solver_set_profile.argtypes = [C.c_void_p, C.c_int]
-    p = Profile()
-    L.fem_solver_profile(solver.h, C.byref(p))
-    return p
+    return solver.profile()
 
 
 def set_profile(F, solver, on):
-    L = F.lib()
-    L.fem_solver_set_profile.argtypes = [C.c_void_p, C.c_int]
-    L.fem_solver_set_profile(solver.h, 1 if on else 0)
+    solver.set_profile(on)
 
 
 def heat_box(F, n):

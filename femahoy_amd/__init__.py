@@ -92,6 +92,14 @@ class _Stats(C.Structure):
 
 CANCEL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)
 
+class Profile(C.Structure):
+    """fem_profile: device time of the hot launches of the last solve (measurement only)."""
+
+    _fields_ = [("smoother_launches", C.c_uint64), ("smoother_seconds", C.c_double), ("smoother_bytes", C.c_uint64),
+                ("outer_launches", C.c_uint64), ("outer_seconds", C.c_double), ("outer_bytes", C.c_uint64),
+                ("dofs", C.c_uint64)]
+
+
 EXPORTS = (
     "fem_options_default", "fem_solver_new_box", "fem_solver_solve", "fem_solver_solve_device",
     "fem_solver_method", "fem_solver_set_max_iter", "fem_solver_set_tol", "fem_solver_free",
@@ -99,6 +107,7 @@ EXPORTS = (
     "fem_solver_rhs", "fem_solver_apply", "fem_solver_precondition", "fem_solver_level_info",
     "fem_solver_device_name", "fem_model_new_box", "fem_model_point_count", "fem_model_unknown_count",
     "fem_model_free_count", "fem_model_free_list", "fem_model_point_grid", "fem_model_load", "fem_model_free",
+    "fem_solver_set_profile", "fem_solver_profile",
 )
 
 _lib = None
@@ -148,6 +157,8 @@ def lib() -> C.CDLL:
         "fem_model_point_grid": (C.c_int, [P, U32P]),
         "fem_model_load": (C.c_int, [P, F64P]),
         "fem_model_free": (None, [P]),
+        "fem_solver_set_profile": (None, [P, C.c_int]),
+        "fem_solver_profile": (C.c_int, [P, C.POINTER(Profile)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -335,6 +346,17 @@ class Solver:
 
     def device_name(self) -> str:
         return lib().fem_solver_device_name(self.h).decode()
+
+    def set_profile(self, on: bool) -> None:
+        """HIP events around the finest-level smoother and outer operator launches of later
+        solves (eager launches while on)."""
+        lib().fem_solver_set_profile(self.h, 1 if on else 0)
+
+    def profile(self) -> Profile:
+        p = Profile()
+        if lib().fem_solver_profile(self.h, C.byref(p)):
+            raise FemError("no profile")
+        return p
 
     def solve(self, b, x=None, cancelled: Optional[Callable[[], bool]] = None):
         """Solve A x = b; `x` is the start value (zeros if None). Returns (x, Stats)."""
