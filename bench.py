@@ -122,6 +122,17 @@ def timed_solves(F, hip, solver, d_b, d_x, x0, steps, warmup, profile=False):
     return lat, st, prof, total
 
 
+def measured_traffic(n):
+    """HBM bytes per launch of the dominant kernel from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    kept under profiles/ (measured at this workload only), or None."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_heat_hex8_{n}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return {"bytes_per_launch": t["bytes_per_launch"], "source": os.path.relpath(path, ROOT)}
+
+
 def cpu_baseline(n_cpu):
     """The oracle (the reference method restated in C++: assembled CSR, line Gauss-Seidel GMG with
     Galerkin coarse operators and a sparse Cholesky coarsest level, FP64 PCG), one thread, on a
@@ -162,6 +173,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--degree", type=int, default=3)
     ap.add_argument("--coarse", choices=["galerkin", "mean"], default="galerkin")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run one plain 256 MiB device copy first (the yardstick of HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (every box here has one GPU; "
@@ -190,6 +203,16 @@ def main():
     d_b = hip.malloc(b.nbytes)
     d_x = hip.malloc(b.nbytes)
     hip.h2d(d_b, b)
+    if args.calibrate:
+        cal = 256 << 20
+        d_c0, d_c1 = hip.malloc(cal), hip.malloc(cal)
+        err = C.create_string_buffer(256)
+        F.lib().fem_test_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_char_p, C.c_size_t]
+        if F.lib().fem_test_copy(C.c_void_p(d_c0), C.c_void_p(d_c1), cal, err, 256):
+            raise RuntimeError(err.value.decode())
+        log(f"calibration: one copy kernel of {cal} bytes read + {cal} bytes written")
+        hip.free(d_c0)
+        hip.free(d_c1)
 
     log("mixed (FP32 V-cycle, FP64 PCG): timed solves (replayed HIP graphs)")
     lat_m, st_m, _, tot_m = timed_solves(F, hip, mixed, d_b, d_x, x0, args.steps, args.warmup, profile=False)
@@ -217,16 +240,20 @@ def main():
     fp64_value = nfree / float(np.mean(lat_d)) / 1e6
 
     roofline = None
+    traffic = measured_traffic(n)
     if prof:
         achieved = prof["smoother_bytes"] / prof["smoother_s"] / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                    "kernel": "k_cheb (finest-level Chebyshev step, hex8 operator fused with the update, FP32)",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": traffic["bytes_per_launch"] if traffic else None,
+                    "kernel": "k_march_hex8<float, MARCH_CHEB> (finest-level Chebyshev step: hex8 operator fused "
+                              "with the update, FP32)",
                     "bytes_per_launch": int(prof["smoother_bytes"]),
                     "us_per_launch": round(prof["smoother_s"] * 1e6, 2), "launches": int(prof["smoother_n"]),
                     "measured": "HIP events on the solver stream around each launch, over a pass of the timed "
                                 "solves with eager launches (the timed pass replays graphs)",
-                    "outer_apply": {"kernel": "k_pcg_q (FP64 operator, q = A p, p.q partials)",
+                    "traffic_source": traffic["source"] if traffic else None,
+                    "outer_apply": {"kernel": "k_march_hex8<double, MARCH_PCG_Q> (FP64 operator, q = A p, p.q partials)",
                                     "bytes_per_launch": int(prof["outer_bytes"]),
                                     "us_per_launch": round(prof["outer_s"] * 1e6, 2),
                                     "achieved": round(prof["outer_bytes"] / prof["outer_s"] / 1e9, 1)}}

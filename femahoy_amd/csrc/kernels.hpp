@@ -74,11 +74,19 @@ __device__ __forceinline__ double block_sum(double v) { return block_sum_t<BLOCK
 constexpr int RBLOCK = 1024;
 
 // ------------------------------------------------------------------ operator kernels
+// The logical block of a 1-D launch, XCD-aware: workgroups go to the 8 XCDs round-robin by id
+// and each XCD has its own L2, so id p takes block (p % 8) * (G / 8) + p / 8 and neighbouring
+// blocks, which read each other's halo, share an L2. The tail beyond a multiple of 8 keeps its id.
+__device__ __forceinline__ size_t xcd_block() {
+    const unsigned G = gridDim.x, p = blockIdx.x, per = G / 8;
+    return p < per * 8 ? (size_t)(p % 8) * per + p / 8 : (size_t)p;
+}
+
 // y = A x on the unknowns; 0 on fixed nodes.
 template <class OP, class T>
 __global__ void __launch_bounds__(BLOCK) k_apply(OP op, const T* __restrict__ x, T* __restrict__ y) {
     const GridDims g = op.dims();
-    const size_t node = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    const size_t node = xcd_block() * BLOCK + threadIdx.x;
     if (node >= g.nodes()) return;
     int i, j, k;
     node_ijk(g, node, i, j, k);
@@ -98,7 +106,7 @@ template <class OP, class T>
 __global__ void __launch_bounds__(BLOCK) k_residual(OP op, const T* __restrict__ x, const T* __restrict__ b,
                                                     T* __restrict__ r) {
     const GridDims g = op.dims();
-    const size_t node = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    const size_t node = xcd_block() * BLOCK + threadIdx.x;
     if (node >= g.nodes()) return;
     int i, j, k;
     node_ijk(g, node, i, j, k);
@@ -119,7 +127,7 @@ __global__ void __launch_bounds__(BLOCK) k_cheb(OP op, const T* __restrict__ xin
                                                 T* __restrict__ d, const T* __restrict__ b,
                                                 const T* __restrict__ invd, T c1, T c2) {
     const GridDims g = op.dims();
-    const size_t node = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    const size_t node = xcd_block() * BLOCK + threadIdx.x;
     if (node >= g.nodes()) return;
     int i, j, k;
     node_ijk(g, node, i, j, k);
@@ -158,7 +166,7 @@ __global__ void __launch_bounds__(BLOCK) k_cheb_first(size_t n, const T* __restr
 template <class OP, class T>
 __global__ void __launch_bounds__(BLOCK) k_inv_diag(OP op, T* __restrict__ invd) {
     const GridDims g = op.dims();
-    const size_t node = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    const size_t node = xcd_block() * BLOCK + threadIdx.x;
     if (node >= g.nodes()) return;
     int i, j, k;
     node_ijk(g, node, i, j, k);
@@ -177,7 +185,7 @@ __global__ void __launch_bounds__(BLOCK) k_inv_diag(OP op, T* __restrict__ invd)
 template <class T, int C, bool KUHN_P1>
 __global__ void __launch_bounds__(BLOCK) k_prolong_add(GridDims fg, GridDims cg, const T* __restrict__ ec,
                                                        T* __restrict__ xf) {
-    const size_t node = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    const size_t node = xcd_block() * BLOCK + threadIdx.x;
     if (node >= fg.nodes()) return;
     int i, j, k;
     node_ijk(fg, node, i, j, k);
@@ -215,7 +223,7 @@ __global__ void __launch_bounds__(BLOCK) k_prolong_add(GridDims fg, GridDims cg,
 template <class T, int C, bool KUHN_P1>
 __global__ void __launch_bounds__(BLOCK) k_restrict(GridDims fg, GridDims cg, const T* __restrict__ rf,
                                                     T* __restrict__ bc) {
-    const size_t node = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    const size_t node = xcd_block() * BLOCK + threadIdx.x;
     if (node >= cg.nodes()) return;
     int I, J, K;
     node_ijk(cg, node, I, J, K);

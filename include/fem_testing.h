@@ -21,6 +21,11 @@ int fem_test_gemm_nt(int m, int n, int k, const double* A, const double* B, doub
  * blocked LDL^T and triangular inverse, in FP64. */
 int fem_test_dense_inverse(int n, const double* A, double* Ainv, char* err, size_t errlen);
 
+/* A plain copy of `bytes` (a multiple of 16) between two device buffers on the null stream, then
+ * a wait: the calibration of HBM byte counters (FETCH_SIZE / WRITE_SIZE) against a kernel whose
+ * traffic is known. */
+int fem_test_copy(const void* d_src, void* d_dst, size_t bytes, char* err, size_t errlen);
+
 #ifdef __cplusplus
 }
 #endif
