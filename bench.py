@@ -86,6 +86,13 @@ def heat_box(F, n):
     return F.Box("heat_hex8", [n, n, n], [1.0, 1.0, 1.0], k, fixed={"x0": 0.0, "x1": 1.0}), k
 
 
+def elastic_box(F, n):
+    """Linear elasticity, tet10 on an n x n/2 x n/2 box of unit cells split into 6 tetrahedra per
+    cell, E = 1, nu = 0.3, clamped at x = 0, unit body force in -z."""
+    cells = [n, n // 2, n // 2]
+    return F.Box("elastic_tet10", cells, [float(c) for c in cells], 1.0, 0.3, fixed={"x0": True}, body=(0.0, 0.0, -1.0))
+
+
 def timed_solves(F, hip, solver, d_b, d_x, x0, steps, warmup, profile=False):
     """W untimed then K timed solves from x = 0; returns (latencies s, stats of the last, profile)."""
     for _ in range(warmup):
@@ -162,6 +169,66 @@ def cpu_baseline(n_cpu):
     }
 
 
+def run_workload(F, hip, box, levels, args, label):
+    """Mixed and FP64 MG-PCG on one box: timed solves, a profiled eager pass, the error against an
+    FP64 solve to 1e-13. Returns the result dict."""
+    log(f"setup: {label}, mixed and FP64 hierarchies ({levels or 'automatic'} levels)")
+    t0 = time.perf_counter()
+    opts = dict(method="multigrid", tol=1e-8, levels=levels, cheb_degree=args.degree,
+                coarse_galerkin=args.coarse == "galerkin")
+    mixed = F.Solver(box, precision="fp32", **opts)
+    t1 = time.perf_counter()
+    fp64 = F.Solver(box, precision="fp64", **opts)
+    t2 = time.perf_counter()
+    nfree = mixed.n
+    log(f"  {nfree} free unknowns on {mixed.device_name()}; setup mixed {t1 - t0:.2f} s, fp64 {t2 - t1:.2f} s")
+    lv = []
+    for l in range(32):
+        try:
+            lv.append(mixed.level_info(l))
+        except F.FemError:
+            break
+    log("  levels (unknowns, lmax):", lv)
+    b = mixed.rhs()
+    x0 = np.zeros(nfree)
+    d_b = hip.malloc(b.nbytes)
+    d_x = hip.malloc(b.nbytes)
+    hip.h2d(d_b, b)
+    log("mixed (FP32 V-cycle, FP64 PCG): timed solves (replayed HIP graphs)")
+    lat_m, st_m, _, _ = timed_solves(F, hip, mixed, d_b, d_x, x0, args.steps, args.warmup, profile=False)
+    x_mixed = np.zeros(nfree)
+    hip.d2h(x_mixed, d_x)
+    log("mixed: the same solves again with HIP events around the finest-level kernels (eager launches)")
+    _, _, prof, _ = timed_solves(F, hip, mixed, d_b, d_x, x0, max(2, args.steps // 2), 0, profile=True)
+    log("FP64 MG-PCG: timed solves")
+    lat_d, st_d, _, _ = timed_solves(F, hip, fp64, d_b, d_x, x0, args.steps, args.warmup, profile=False)
+    x_fp64 = np.zeros(nfree)
+    hip.d2h(x_fp64, d_x)
+    fp64.set_tol(1e-13)
+    hip.h2d(d_x, x0)
+    st_ref = fp64.solve_device(d_b, d_x)
+    x_ref = np.zeros(nfree)
+    hip.d2h(x_ref, d_x)
+    hip.free(d_b)
+    hip.free(d_x)
+    err_mixed = float(np.linalg.norm(x_mixed - x_ref) / np.linalg.norm(x_ref))
+    err_fp64 = float(np.linalg.norm(x_fp64 - x_ref) / np.linalg.norm(x_ref))
+    log(f"  error vs FP64 1e-13 solve: mixed {err_mixed:.2e}, fp64 {err_fp64:.2e}; reference took {st_ref.iterations} it")
+
+    def leg(lat, st, err, setup):
+        return {"mdof_s": round(nfree / float(np.mean(lat)) / 1e6, 2), "ms_mean": round(1e3 * float(np.mean(lat)), 3),
+                "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3), "iterations": st.iterations,
+                "residual": st.residual, "true_residual": st.true_residual, "error_vs_fp64": err,
+                "setup_s": round(setup, 3)}
+
+    res = {"free_dof": nfree, "levels": [v[0] for v in lv], "mixed": leg(lat_m, st_m, err_mixed, t1 - t0),
+           "fp64": leg(lat_d, st_d, err_fp64, t2 - t1),
+           "fp64_reference": f"FP64 MG-PCG to 1e-13 relative residual ({st_ref.iterations} iterations)"}
+    res["speedup_mixed_vs_fp64"] = round(res["mixed"]["mdof_s"] / res["fp64"]["mdof_s"], 3)
+    return res, prof, lat_m
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +240,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--degree", type=int, default=3)
     ap.add_argument("--coarse", choices=["galerkin", "mean"], default="galerkin")
+    ap.add_argument("--elastic-n", type=int, default=72,
+                    help="cells along x of the tet10 elasticity box reported beside the headline (0: skip)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run one plain 256 MiB device copy first (the yardstick of HBM byte counters)")
     args = ap.parse_args()
@@ -184,25 +253,6 @@ def main():
     import femahoy_amd as F
 
     hip = Hip()
-    n = args.n
-    box, _ = heat_box(F, n)
-    log(f"setup: heat hex8 {n}^3, mixed and FP64 hierarchies ({args.levels} levels)")
-    t0 = time.perf_counter()
-    opts = dict(method="multigrid", tol=1e-8, levels=args.levels, cheb_degree=args.degree,
-                coarse_galerkin=args.coarse == "galerkin")
-    mixed = F.Solver(box, precision="fp32", **opts)
-    t1 = time.perf_counter()
-    fp64 = F.Solver(box, precision="fp64", **opts)
-    t2 = time.perf_counter()
-    nfree = mixed.n
-    log(f"  {nfree} free unknowns on {mixed.device_name()}; setup mixed {t1 - t0:.2f} s, fp64 {t2 - t1:.2f} s")
-    levels = [mixed.level_info(l) for l in range(args.levels)]
-    log("  levels (unknowns, lmax):", levels)
-    b = mixed.rhs()
-    x0 = np.zeros(nfree)
-    d_b = hip.malloc(b.nbytes)
-    d_x = hip.malloc(b.nbytes)
-    hip.h2d(d_b, b)
     if args.calibrate:
         cal = 256 << 20
         d_c0, d_c1 = hip.malloc(cal), hip.malloc(cal)
@@ -214,30 +264,10 @@ def main():
         hip.free(d_c0)
         hip.free(d_c1)
 
-    log("mixed (FP32 V-cycle, FP64 PCG): timed solves (replayed HIP graphs)")
-    lat_m, st_m, _, tot_m = timed_solves(F, hip, mixed, d_b, d_x, x0, args.steps, args.warmup, profile=False)
-    x_mixed = np.zeros(nfree)
-    hip.d2h(x_mixed, d_x)
-    log("mixed: the same solves again with HIP events around the finest-level kernels (eager launches)")
-    _, _, prof, _ = timed_solves(F, hip, mixed, d_b, d_x, x0, args.steps, 0, profile=True)
-    log("FP64 MG-PCG: timed solves")
-    lat_d, st_d, _, tot_d = timed_solves(F, hip, fp64, d_b, d_x, x0, args.steps, args.warmup, profile=False)
-    x_fp64 = np.zeros(nfree)
-    hip.d2h(x_fp64, d_x)
-    # FP64 reference solution: FP64 MG-PCG to 1e-13
-    fp64.set_tol(1e-13)
-    hip.h2d(d_x, x0)
-    st_ref = fp64.solve_device(d_b, d_x)
-    x_ref = np.zeros(nfree)
-    hip.d2h(x_ref, d_x)
-    err_mixed = float(np.linalg.norm(x_mixed - x_ref) / np.linalg.norm(x_ref))
-    err_fp64 = float(np.linalg.norm(x_fp64 - x_ref) / np.linalg.norm(x_ref))
-    log(f"  error vs FP64 1e-13 solve: mixed {err_mixed:.2e}, fp64 {err_fp64:.2e}; reference took {st_ref.iterations} it")
-
-    ms_m = 1e3 * float(np.mean(lat_m))
-    ms_d = 1e3 * float(np.mean(lat_d))
-    value = nfree / float(np.mean(lat_m)) / 1e6
-    fp64_value = nfree / float(np.mean(lat_d)) / 1e6
+    n = args.n
+    box, _ = heat_box(F, n)
+    heat, prof, lat_m = run_workload(F, hip, box, args.levels, args, f"heat hex8 {n}^3")
+    value = heat["mixed"]["mdof_s"]
 
     roofline = None
     traffic = measured_traffic(n)
@@ -257,48 +287,51 @@ def main():
                                     "bytes_per_launch": int(prof["outer_bytes"]),
                                     "us_per_launch": round(prof["outer_s"] * 1e6, 2),
                                     "achieved": round(prof["outer_bytes"] / prof["outer_s"] / 1e9, 1)}}
+
+    elastic = None
+    if args.elastic_n > 0:
+        en = args.elastic_n
+        ebox = elastic_box(F, en)
+        elastic, eprof, _ = run_workload(F, hip, ebox, 0, args, f"elasticity tet10 {en}x{en // 2}x{en // 2}")
+        elastic["workload"] = (f"linear elasticity, tet10 on {en}x{en // 2}x{en // 2} unit cells (6 Kuhn tets per cell), "
+                               "E=1, nu=0.3, clamped at x=0, body force (0,0,-1); p-MG tet10->tet4 then geometric tet4 "
+                               "levels, dense FP64 LDLT coarse, Chebyshev(3)-Jacobi, tol 1e-8")
+        if eprof:
+            elastic["finest_smoother"] = {"kernel": "k_cheb<VectorOp<Tet10Op<float>>> (+ its element pass)",
+                                          "bytes_per_launch": int(eprof["smoother_bytes"]),
+                                          "us_per_launch": round(eprof["smoother_s"] * 1e6, 2),
+                                          "achieved_gbs": round(eprof["smoother_bytes"] / eprof["smoother_s"] / 1e9, 1)}
+
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline: oracle on heat hex8 {args.cpu_n}^3, 1 thread")
         cpu = cpu_baseline(args.cpu_n)
         log("  " + cpu["sample"])
 
+    config = {"workload": (f"heat hex8 {n}^3 unit cube, k = exp(N(0,1)) per cell (seed 1234), T=0 at x=0 and T=1 at "
+                           f"x=1, {args.levels}-level GMG ({args.coarse} coarse operators), Chebyshev({args.degree})-Jacobi, "
+                           "dense FP64 LDLT coarse, tol 1e-8")}
+    config.update(heat)
+    if elastic:
+        config["elasticity_tet10"] = elastic
     out = {
         "metric": METRIC,
-        "value": round(value, 2),
+        "value": value,
         "unit": "MDOF/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_m, 3),
+        "ms_per_step": heat["mixed"]["ms_mean"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64 PCG + f32 V-cycle",
-        "data": "synthetic (seeded lognormal conductivity)",
-        "config": {
-            "workload": (f"heat hex8 {n}^3 unit cube, k = exp(N(0,1)) per cell (seed 1234), T=0 at x=0 and T=1 at "
-                         f"x=1, {args.levels}-level GMG ({args.coarse} coarse operators), Chebyshev({args.degree})-Jacobi, dense FP64 LDLT "
-                         "coarse, tol 1e-8"),
-            "free_dof": nfree,
-            "levels": [lv[0] for lv in levels],
-            "mixed": {"mdof_s": round(value, 2), "ms_mean": round(ms_m, 3), "p50_ms": round(1e3 * float(np.percentile(lat_m, 50)), 3),
-                      "p99_ms": round(1e3 * float(np.percentile(lat_m, 99)), 3), "iterations": st_m.iterations,
-                      "residual": st_m.residual, "true_residual": st_m.true_residual,
-                      "error_vs_fp64": err_mixed, "setup_s": round(t1 - t0, 3)},
-            "fp64": {"mdof_s": round(fp64_value, 2), "ms_mean": round(ms_d, 3), "p50_ms": round(1e3 * float(np.percentile(lat_d, 50)), 3),
-                     "p99_ms": round(1e3 * float(np.percentile(lat_d, 99)), 3), "iterations": st_d.iterations,
-                     "residual": st_d.residual, "true_residual": st_d.true_residual,
-                     "error_vs_fp64": err_fp64, "setup_s": round(t2 - t1, 3)},
-            "speedup_mixed_vs_fp64": round(value / fp64_value, 3),
-            "fp64_reference": f"FP64 MG-PCG to 1e-13 relative residual ({st_ref.iterations} iterations)",
-        },
+        "data": "synthetic (seeded lognormal conductivity; uniform elastic material)",
+        "config": config,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    hip.free(d_b)
-    hip.free(d_x)
 
 
 if __name__ == "__main__":
