@@ -107,7 +107,7 @@ EXPORTS = (
     "fem_solver_rhs", "fem_solver_apply", "fem_solver_precondition", "fem_solver_level_info",
     "fem_solver_device_name", "fem_model_new_box", "fem_model_point_count", "fem_model_unknown_count",
     "fem_model_free_count", "fem_model_free_list", "fem_model_point_grid", "fem_model_load", "fem_model_free",
-    "fem_solver_set_profile", "fem_solver_profile",
+    "fem_solver_set_profile", "fem_solver_profile", "fem_solver_time_kernels",
 )
 
 _lib = None
@@ -159,6 +159,7 @@ def lib() -> C.CDLL:
         "fem_model_free": (None, [P]),
         "fem_solver_set_profile": (None, [P, C.c_int]),
         "fem_solver_profile": (C.c_int, [P, C.POINTER(Profile)]),
+        "fem_solver_time_kernels": (C.c_int, [P, C.c_int, C.POINTER(Profile)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -356,6 +357,14 @@ class Solver:
         p = Profile()
         if lib().fem_solver_profile(self.h, C.byref(p)):
             raise FemError("no profile")
+        return p
+
+    def time_kernels(self, reps: int = 100) -> Profile:
+        """Mean device time of the finest smoother step and of the outer q = A p, each repeated
+        `reps` times back to back between one pair of HIP events on the solver's stream."""
+        p = Profile()
+        if lib().fem_solver_time_kernels(self.h, int(reps), C.byref(p)):
+            raise FemError("kernel timing failed")
         return p
 
     def solve(self, b, x=None, cancelled: Optional[Callable[[], bool]] = None):

@@ -954,14 +954,9 @@ class Solver {
     // updates, z = M^-1 r, rz, beta, p = z + beta p; the next residual is in the scalars. Captured
     // once in a HIP graph and replayed (all buffers are fixed per solver); run eagerly when the
     // launches are profiled.
-    void iteration_body() {
+    // q = A p in FP64 with the block partials of p.q; returns the number of partials.
+    unsigned outer_apply() {
         Level<double>& L0 = lv64_[0];
-        const size_t n = n0_;
-        const unsigned nbn = nblocks(L0.nodes());
-        const unsigned nbv = nblocks(n);
-        PcgScalars* sc = scal_.data();
-        prof_.begin(1, s_);
-        unsigned nq = nbn;
         if (L0.march()) {
             MarchArgs<double> m = march_args(L0);
             m.x = p_.data();
@@ -969,13 +964,67 @@ class Solver {
             m.part = partA_.data();
             launch_march<double, MARCH_PCG_Q>(m, s_);
             const dim3 gm = march_grid<double>(L0.g);
-            nq = gm.x * gm.y;
-        } else {
-            prepare(L0, p_.data(), s_);
-            with_op(L0, [&](auto op) {
-                launch_blocks((k_pcg_q<decltype(op)>), nbn, s_, op, (const double*)p_.data(), q_.data(), partA_.data());
-            });
+            return gm.x * gm.y;
         }
+        prepare(L0, p_.data(), s_);
+        with_op(L0, [&](auto op) {
+            launch_blocks((k_pcg_q<decltype(op)>), nblocks(L0.nodes()), s_, op, (const double*)p_.data(), q_.data(),
+                          partA_.data());
+        });
+        return nblocks(L0.nodes());
+    }
+
+   public:
+    // Mean device time of the two hot launches, each repeated `reps` times back to back on the
+    // solver's stream between one pair of HIP events: the finest-level Chebyshev step of the
+    // V-cycle and the outer FP64 q = A p. They run on the solver's own work vectors, whose contents
+    // every solve sets afresh, so later solves are unaffected.
+    void time_kernels(int reps, fem_profile* p) {
+        if (reps < 1) reps = 1;
+        hipEvent_t e0, e1;
+        FEM_HIP(hipEventCreate(&e0));
+        FEM_HIP(hipEventCreate(&e1));
+        auto timed = [&](auto body) {
+            body();  // warm
+            FEM_HIP(hipEventRecord(e0, s_));
+            for (int r = 0; r < reps; ++r) body();
+            FEM_HIP(hipEventRecord(e1, s_));
+            FEM_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            FEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            return ms * 1e-3;
+        };
+        fem_profile out{};
+        out.dofs = n0_;
+        if (h32_ && !h32_->lv[0].c1.empty()) {
+            auto& H = *h32_;
+            Level<float>& L = H.lv[0];
+            const size_t st = L.c1.size() > 1 ? 1 : 0;
+            out.smoother_seconds = timed([&] { H.cheb_step(L, L.c1[st], L.c2[st], s_); L.cur ^= 1; });
+            out.smoother_bytes = 6 * n0_ * sizeof(float) + L.coef.bytes();
+        } else if (h64_ && !h64_->lv[0].c1.empty()) {
+            auto& H = *h64_;
+            Level<double>& L = H.lv[0];
+            const size_t st = L.c1.size() > 1 ? 1 : 0;
+            out.smoother_seconds = timed([&] { H.cheb_step(L, L.c1[st], L.c2[st], s_); L.cur ^= 1; });
+            out.smoother_bytes = 6 * n0_ * sizeof(double) + L.coef.bytes();
+        }
+        out.smoother_launches = out.smoother_seconds > 0 ? (uint64_t)reps : 0;
+        out.outer_seconds = timed([&] { (void)outer_apply(); });
+        out.outer_launches = reps;
+        out.outer_bytes = 16 * n0_ + lv64_[0].coef.bytes();
+        FEM_HIP(hipEventDestroy(e0));
+        FEM_HIP(hipEventDestroy(e1));
+        *p = out;
+    }
+
+   private:
+    void iteration_body() {
+        const size_t n = n0_;
+        const unsigned nbv = nblocks(n);
+        PcgScalars* sc = scal_.data();
+        prof_.begin(1, s_);
+        const unsigned nq = outer_apply();
         prof_.end(s_);
         launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr, (int)nq, 1);
         if (h32_) {
@@ -1247,6 +1296,16 @@ const char* fem_solver_device_name(const fem_solver* s) { return s ? s->impl->de
 
 void fem_solver_set_profile(fem_solver* s, int on) {
     if (s) s->impl->set_profile(on != 0);
+}
+
+int fem_solver_time_kernels(fem_solver* s, int reps, fem_profile* out) {
+    if (!s || !out) return 1;
+    try {
+        s->impl->time_kernels(reps, out);
+        return 0;
+    } catch (...) {
+        return 1;
+    }
 }
 
 int fem_solver_profile(fem_solver* s, fem_profile* out) {

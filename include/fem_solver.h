@@ -182,6 +182,11 @@ typedef struct fem_profile {
 void fem_solver_set_profile(fem_solver* s, int on);
 int fem_solver_profile(fem_solver* s, fem_profile* out);
 
+/* The mean device time of the same two launches, each repeated `reps` times back to back on the
+ * solver's stream between one pair of HIP events (no launch gaps inside): the finest-level
+ * smoother step and the outer q = A p. Measurement only; later solves are unaffected. */
+int fem_solver_time_kernels(fem_solver* s, int reps, fem_profile* out);
+
 /* ---- the model alone, on the host (no device is touched) ---- */
 
 typedef struct fem_model fem_model;

@@ -18,6 +18,30 @@
 //   crates/fem/src/hex.rs        Orthotropic::isotropic / stiffness
 //   crates/fem/src/tet10.rs      Tet10, Quadratic
 //   crates/fem/src/prism.rs      split_prism and the extrusion's tet orientation rule
+//
+// How each part is pinned (tests/test_oracle_kats.py):
+//   Csr::pattern, Rect::galerkin   the reference's known answers (sparse.rs:300, 308) + scipy P^T A P
+//   nested_dissection              order.rs:186 known answer (line of 200: permutation, last 99)
+//   Symbolic                       entries == nonzeros of a dense Cholesky factor (grid Laplacian)
+//   Cholesky                       cholesky.rs:244 and 268 known answers + scipy spsolve
+//   pcg                            solutions equal scipy spsolve; message text of multigrid.rs:730
+//   Multigrid / PMultigrid         solutions equal scipy spsolve; every coarse matrix is P^T A P;
+//                                  the p-prolongation carries linear fields exactly and P^T K10 P = K4
+//   conduction, tet_conduction,    Gauss integration / numpy (V G^T K G) / matrix inverse
+//   tet_gradients
+//   Orthotropic, Tet10, tet4       hex.rs compliance-inverse test; tet10.rs:300 exact mass; energy of
+//                                  uniform strain and of a quadratic field; rigid motions
+//   split_prism, Quadratic         tests/tet.rs:92 (720 orders); one node per distinct edge
+//   box problems (loads, rhs)      closed forms solved through the direct solver: layered wall
+//                                  (tests/heat.rs:112), uniform strain with face tractions, a hanging
+//                                  bar under body force (quadratic, exact on tet10)
+// Parity unpinned (no reference vector exists and none of the checks above fixes it):
+//   - the first-seen order of points and edge nodes of the box problems (heat.rs:1165-1208,
+//     tet10.rs:244-272) beyond being a permutation that covers each point and edge once;
+//   - the nested-dissection order on meshes with ties (order.rs:85-101), the supernode partition,
+//     the operator-dependent coarsen weights and first-seen coarse numbering (multigrid.rs:329-452),
+//     and therefore the reference's iteration counts;
+//   - bitwise equality with the Rust binary.
 #pragma once
 
 #include <array>

@@ -150,3 +150,18 @@ def test_graph_replays_match_eager_launches(fem, monkeypatch):
         assert np.array_equal(xg, xe)
         xe2, _ = eager.solve(b)
         assert np.array_equal(xe2, xe)
+
+
+def test_kernel_timing_leaves_later_solves_unchanged(fem, oracle):
+    """fem_solver_time_kernels runs on the solver's work vectors; a solve after it is bitwise the
+    solve before it."""
+    n = [16, 8, 8]
+    k, fixed = heat_case(n, seed=4)
+    s = fem.Solver(fem.Box("heat_hex8", n, [1, 1, 1], k, fixed=fixed), method="multigrid", precision="fp32",
+                   tol=1e-10)
+    b = s.rhs()
+    x1, _ = s.solve(b)
+    p = s.time_kernels(5)
+    assert p.smoother_launches == 5 and p.smoother_seconds > 0 and p.outer_seconds > 0
+    x2, _ = s.solve(b)
+    assert np.array_equal(x1, x2)

@@ -158,6 +158,17 @@ def test_tet_gradients_invert_the_corner_matrix(oracle):
     assert abs(v - np.linalg.det(M) / 6.0) < 1e-14
 
 
+def test_tet_conduction_is_v_gt_k_g(oracle):
+    """heat.rs:972 tet_conduction(): V G^T diag(k) G, against numpy."""
+    p = np.array([[0.1, 0.0, 0.2], [1.1, 0.2, 0.0], [0.3, 0.9, 0.1], [0.2, 0.3, 1.4]])
+    g, v = oracle.tet_gradients(p)
+    k = np.array([2.0, 0.5, 1.5])
+    got = oracle.tet_conduction(g, v, k)
+    want = v * (g * k) @ g.T
+    assert np.abs(got - want).max() < 1e-14 * np.abs(want).max()
+    assert np.abs(got.sum(axis=1)).max() < 1e-14 * np.abs(want).max()
+
+
 def test_orthotropic_stiffness_inverts_the_compliance(oracle):
     """crates/fem/src/hex.rs tests: the_orthotropic_stiffness_inverts_the_compliance."""
     c, shear = oracle.orthotropic_stiffness([140.0, 10.0, 10.0], [0.3, 0.3, 0.45], [5.0, 5.0, 3.5])
