@@ -356,11 +356,11 @@ struct Tet4Elastic {
 // edge points at v_a + v_b, with the chain vertices v_0..v_3 (in cell bits). Local nodes 0..3
 // are the corners, 4..9 the edges (0,1), (1,2), (0,2), (0,3), (1,3), (2,3) (tet10.rs:38).
 //
-// The operator runs in 2 phases. Phase A, one thread per tetrahedron: the element product of
-// tet10.rs:90-120 (the 4-point rule of degree 2, exact for affine cells) written to an element
-// buffer E, 30 values per tetrahedron. Phase B, one thread per node: the sum of the entries of E
-// at that node, over the tetrahedra that hold it, in a fixed order. No atomics, so the sums are
-// bitwise reproducible.
+// The operator runs in 2 phases. Phase A (k_tet10_elems, tet10_kuhn.hpp), one thread per
+// tetrahedron: the element product of tet10.rs:90-120 (the 4-point rule of degree 2, exact for
+// affine cells) written to an element buffer E, 30 values per tetrahedron. Phase B (Tet10Op
+// below), one thread per node: the sum of the entries of E at that node, over the tetrahedra that
+// hold it, in a fixed order. No atomics, so the sums are bitwise reproducible.
 __constant__ static const int TET10_EDGE[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
 
 // Chain vertex v (0..3) of Kuhn type t, as cell bits.
@@ -383,91 +383,6 @@ __host__ __device__ inline int tet10_local(int t, int px, int py, int pz) {
         if (a[0] + b[0] == px && a[1] + b[1] == py && a[2] + b[2] == pz) return 4 + e;
     }
     return -1;
-}
-
-// The element product y = K u of one tet10 with isotropic lambda, mu (tet10.rs:90-120), through
-// the vertex gradients: grad u is linear in the barycentric coordinates L, grad u(L) = sum_k L_k
-// G_k with G_k = 4 u_k g_k^T + 4 sum_{j != k} u_kj g_j^T - sum_i u_i g_i^T, so each quadrature
-// point costs one 3 x 3 combination, and the adjoint collects sum_q w L_k(q) sigma(q).
-template <class T>
-__device__ __forceinline__ void tet10_element(const T u[30], const T g[4][3], T lam, T mu, T w, T y[30]) {
-    const T qa = T(0.5854101966249685), qb = T(0.1381966011250105);  // tet10.rs:45-46
-    T Cm[3][3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) Cm[a][b] = u[a] * g[0][b] + u[3 + a] * g[1][b] + u[6 + a] * g[2][b] + u[9 + a] * g[3][b];
-    // edge index of (k, j), j != k
-    const int EI[4][4] = {{-1, 4, 6, 7}, {4, -1, 5, 8}, {6, 5, -1, 9}, {7, 8, 9, -1}};
-    T G[4][3][3];
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int b = 0; b < 3; ++b) {
-                T s = u[3 * k + a] * g[k][b];
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    if (j != k) s += u[3 * EI[k][j] + a] * g[j][b];
-                G[k][a][b] = T(4) * s - Cm[a][b];
-            }
-    T S[3][3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) S[a][b] = G[0][a][b] + G[1][a][b] + G[2][a][b] + G[3][a][b];
-    // sigma at each quadrature point q (L_q = qa, others qb), symmetric: 6 entries
-    T sig[4][6];
-    T ssum[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        T D[3][3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int b = 0; b < 3; ++b) D[a][b] = qb * S[a][b] + (qa - qb) * G[q][a][b];
-        const T tr = lam * (D[0][0] + D[1][1] + D[2][2]);
-        sig[q][0] = tr + T(2) * mu * D[0][0];
-        sig[q][1] = tr + T(2) * mu * D[1][1];
-        sig[q][2] = tr + T(2) * mu * D[2][2];
-        sig[q][3] = mu * (D[1][2] + D[2][1]);  // yz
-        sig[q][4] = mu * (D[0][2] + D[2][0]);  // xz
-        sig[q][5] = mu * (D[0][1] + D[1][0]);  // xy
-#pragma unroll
-        for (int s = 0; s < 6; ++s) ssum[s] += sig[q][s];
-    }
-    // Sigma'_k = w (qb sum_q sigma + (qa - qb) sigma_k); S' = w sum_q sigma
-    T Sp[4][6];
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-#pragma unroll
-        for (int s = 0; s < 6; ++s) Sp[k][s] = w * (qb * ssum[s] + (qa - qb) * sig[k][s]);
-    T S0[6];
-#pragma unroll
-    for (int s = 0; s < 6; ++s) S0[s] = w * ssum[s];
-    // sym 3x3 (xx, yy, zz, yz, xz, xy) times a vector
-    auto mv = [](const T* m, const T* v, T* o) {
-        o[0] = m[0] * v[0] + m[5] * v[1] + m[4] * v[2];
-        o[1] = m[5] * v[0] + m[1] * v[1] + m[3] * v[2];
-        o[2] = m[4] * v[0] + m[3] * v[1] + m[2] * v[2];
-    };
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        T m[6];
-#pragma unroll
-        for (int s = 0; s < 6; ++s) m[s] = T(4) * Sp[k][s] - S0[s];
-        mv(m, g[k], &y[3 * k]);
-    }
-#pragma unroll
-    for (int e = 0; e < 6; ++e) {
-        const int k = TET10_EDGE[e][0], j = TET10_EDGE[e][1];
-        T o1[3], o2[3];
-        mv(Sp[k], g[j], o1);
-        mv(Sp[j], g[k], o2);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) y[3 * (4 + e) + a] = T(4) * (o1[a] + o2[a]);
-    }
 }
 
 // Phase B of the tet10 operator, node-centric: the sum of the element buffer at a node.
@@ -552,43 +467,5 @@ struct Tet10Op {
         }
     }
 };
-
-// Phase A: one thread per (cell, Kuhn type), cell fastest.
-template <class T>
-__global__ void __launch_bounds__(256) k_tet10_elements(GridDims g, const T* __restrict__ coef, const T* __restrict__ x,
-                                                        T* __restrict__ E, T ih0, T ih1, T ih2, T vol) {
-    const int cx = (g.nx - 1) / 2, cy = (g.ny - 1) / 2, cz = (g.nz - 1) / 2;
-    const size_t cells = (size_t)cx * cy * cz;
-    const size_t tid = blockIdx.x * (size_t)256 + threadIdx.x;
-    if (tid >= 6 * cells) return;
-    const int t = (int)(tid / cells);
-    const size_t cell = tid % cells;
-    const int ex = (int)(cell % cx), ey = (int)((cell / cx) % cy), ez = (int)(cell / ((size_t)cx * cy));
-    int vb[4][3];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) kuhn_chain(t, v, vb[v]);
-    T u[30];
-#pragma unroll
-    for (int n = 0; n < 10; ++n) {
-        int p[3];
-        if (n < 4) {
-            for (int d = 0; d < 3; ++d) p[d] = 2 * vb[n][d];
-        } else {
-            const int a = TET10_EDGE[n - 4][0], b = TET10_EDGE[n - 4][1];
-            for (int d = 0; d < 3; ++d) p[d] = vb[a][d] + vb[b][d];
-        }
-        const size_t node = (2 * ex + p[0]) + (size_t)g.nx * ((2 * ey + p[1]) + (size_t)g.ny * (2 * ez + p[2]));
-#pragma unroll
-        for (int c = 0; c < 3; ++c) u[3 * n + c] = x[3 * node + c];
-    }
-    const T ih[3] = {ih0, ih1, ih2};
-    T gg[4][3];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) kuhn_grad<T>(t, v, ih, gg[v]);
-    T y[30];
-    tet10_element<T>(u, gg, coef[t * cells + cell], coef[(6 + t) * cells + cell], vol / T(4), y);
-#pragma unroll
-    for (int s = 0; s < 30; ++s) E[((size_t)(t * 30 + s)) * cells + cell] = y[s];
-}
 
 }  // namespace fem

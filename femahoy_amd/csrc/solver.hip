@@ -19,6 +19,7 @@
 #include "kernels.hpp"
 #include "model.hpp"
 #include "march.hpp"
+#include "tet10_kuhn.hpp"
 #include "setup_kernels.hpp"
 
 namespace fem {
@@ -130,15 +131,29 @@ void launch_march(const MarchArgs<T>& m, hipStream_t s) {
     hipLaunchKernelGGL((k_march_hex8<T, MODE>), march_grid<T>(m.g), dim3(BLOCK), 0, s, m);
 }
 
+// The arguments of the fused tet10 kernels of a level.
+template <class T>
+Tet10Args<T> tet10_args(const Level<T>& L) {
+    Tet10Args<T> a{};
+    a.g = L.g;
+    a.cx = L.cells[0];
+    a.cy = L.cells[1];
+    a.cz = L.cells[2];
+    a.coef = L.coef.data();
+    a.ih0 = T(1.0 / L.h[0]);
+    a.ih1 = T(1.0 / L.h[1]);
+    a.ih2 = T(1.0 / L.h[2]);
+    a.w = T(L.h[0] * L.h[1] * L.h[2] / 6.0 / 4.0);
+    return a;
+}
+
 // Phase A of an operator whose apply reads an element buffer (tet10); nothing for the others.
 template <class T>
 void prepare(Level<T>& L, const T* x, hipStream_t s) {
     if (L.op != OpKind::Tet10) return;
-    const T ih[3] = {T(1.0 / L.h[0]), T(1.0 / L.h[1]), T(1.0 / L.h[2])};
-    const T vol = T(L.h[0] * L.h[1] * L.h[2] / 6.0);
-    const size_t n = 6 * L.ncell();
-    launch_blocks(k_tet10_elements<T>, nblocks(n), s, L.g, (const T*)L.coef.data(), x,
-                       L.E.data(), ih[0], ih[1], ih[2], vol);
+    Tet10Args<T> a = tet10_args(L);
+    a.x = x;
+    hipLaunchKernelGGL((k_tet10_elems<T>), tet10_elems_grid(a.cx, a.cy, a.cz), dim3(256), 0, s, a, L.E.data());
 }
 
 // Conversion of the FP64 coefficients of a level to T.
@@ -425,12 +440,10 @@ class Solver {
 
     // Reduced right side over the free unknowns in reference order: load - A u_fixed.
     void rhs(double* out) {
-        Level<double>& L = lv64_[0];
         const size_t n = n0_;
         tmp_.upload(model_.fixed_value.data(), n, s_);
         // y = A u with u the fixed values: the operator's rows read the fixed nodes' values.
-        prepare(L, tmp_.data(), s_);
-        with_op(L, [&](auto op) { launch((k_apply<decltype(op), double>), L.nodes(), s_, op, (const double*)tmp_.data(), r_.data()); });
+        apply64(tmp_.data(), r_.data());
         std::vector<double> y(n);
         r_.download(y.data(), n, s_);
         FEM_HIP(hipStreamSynchronize(s_));
