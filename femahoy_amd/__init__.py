@@ -92,6 +92,17 @@ class _Stats(C.Structure):
 
 CANCEL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)
 
+class _ServerOptions(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_active", C.c_int32), ("device", C.c_int32)]
+
+
+class RequestInfo(C.Structure):
+    """fem_request_info: the timeline (seconds since the server started) and footprint of a solve."""
+
+    _fields_ = [("submit_time", C.c_double), ("admit_time", C.c_double), ("start_time", C.c_double),
+                ("finish_time", C.c_double), ("footprint", C.c_uint64), ("device_bytes", C.c_int64)]
+
+
 class Profile(C.Structure):
     """fem_profile: device time of the hot launches of the last solve (measurement only)."""
 
@@ -108,6 +119,7 @@ EXPORTS = (
     "fem_solver_device_name", "fem_model_new_box", "fem_model_point_count", "fem_model_unknown_count",
     "fem_model_free_count", "fem_model_free_list", "fem_model_point_grid", "fem_model_load", "fem_model_free",
     "fem_solver_set_profile", "fem_solver_profile", "fem_solver_time_kernels",
+    "fem_server_new", "fem_server_submit", "fem_server_wait", "fem_server_cancel", "fem_server_free",
 )
 
 _lib = None
@@ -160,6 +172,12 @@ def lib() -> C.CDLL:
         "fem_solver_set_profile": (None, [P, C.c_int]),
         "fem_solver_profile": (C.c_int, [P, C.POINTER(Profile)]),
         "fem_solver_time_kernels": (C.c_int, [P, C.c_int, C.POINTER(Profile)]),
+        "fem_server_new": (C.c_int, [C.POINTER(_ServerOptions), C.POINTER(P), ERR, SZ]),
+        "fem_server_submit": (C.c_int, [P, C.POINTER(_Box), C.POINTER(_Options), P, C.POINTER(C.c_uint64), ERR, SZ]),
+        "fem_server_wait": (C.c_int, [P, C.c_uint64, C.c_double, P, C.c_uint64, C.POINTER(_Stats),
+                                      C.POINTER(RequestInfo), ERR, SZ]),
+        "fem_server_cancel": (C.c_int, [P, C.c_uint64]),
+        "fem_server_free": (None, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -271,6 +289,19 @@ class Model:
         return out
 
 
+def _options(method=None, precision="fp32", tol=PCG_TOL, max_iter=PCG_MAX_ITER, crossover=STEADY_DIRECT_LIMIT,
+             levels=0, cheb_degree=3, cheb_lower=0.1, cheb_upper=1.1, power_steps=20, coarse_max=6000, refine=True,
+             device=0, coarse_galerkin=True) -> "_Options":
+    o = _Options()
+    lib().fem_options_default(C.byref(o))
+    o.tol, o.max_iter, o.method, o.precision = tol, max_iter, METHODS[method], PRECISIONS[precision]
+    o.crossover, o.levels, o.cheb_degree = crossover, levels, cheb_degree
+    o.cheb_lower, o.cheb_upper, o.power_steps = cheb_lower, cheb_upper, power_steps
+    o.coarse_max, o.refine, o.device = coarse_max, int(refine), device
+    o.coarse_galerkin = int(coarse_galerkin)
+    return o
+
+
 class Solver:
     """Solver::new / solve / method / with_max_iter over the device library."""
 
@@ -278,13 +309,8 @@ class Solver:
                  max_iter: int = PCG_MAX_ITER, crossover: int = STEADY_DIRECT_LIMIT, levels: int = 0,
                  cheb_degree: int = 3, cheb_lower: float = 0.1, cheb_upper: float = 1.1, power_steps: int = 20,
                  coarse_max: int = 6000, refine: bool = True, device: int = 0, coarse_galerkin: bool = True):
-        o = _Options()
-        lib().fem_options_default(C.byref(o))
-        o.tol, o.max_iter, o.method, o.precision = tol, max_iter, METHODS[method], PRECISIONS[precision]
-        o.crossover, o.levels, o.cheb_degree = crossover, levels, cheb_degree
-        o.cheb_lower, o.cheb_upper, o.power_steps = cheb_lower, cheb_upper, power_steps
-        o.coarse_max, o.refine, o.device = coarse_max, int(refine), device
-        o.coarse_galerkin = int(coarse_galerkin)
+        o = _options(method, precision, tol, max_iter, crossover, levels, cheb_degree, cheb_lower, cheb_upper,
+                     power_steps, coarse_max, refine, device, coarse_galerkin)
         b, keep = box._struct()
         h = C.c_void_p()
         err = _err()
@@ -396,3 +422,58 @@ def _stats(st: _Stats) -> Stats:
                  float(st.setup_seconds), int(st.levels), int(st.coarse_unknowns), int(st.restarts),
                  ("direct", "multigrid")[st.method] if st.method in (0, 1) else "?",
                  {v: k for k, v in PRECISIONS.items()}.get(st.precision, "?"))
+
+
+class Server:
+    """The serving loop (fem_server_*): solves of uneven sizes admitted by device footprint, set up
+    on a thread of their own and advanced together, one PCG iteration per solve at a time."""
+
+    def __init__(self, hbm_budget: int, max_active: int = 8, device: int = 0):
+        o = _ServerOptions(int(hbm_budget), int(max_active), int(device))
+        h = C.c_void_p()
+        err = _err()
+        if lib().fem_server_new(C.byref(o), C.byref(h), err, 2048):
+            raise FemError(err.value.decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fem_server_free(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def submit(self, box: Box, b: Optional[np.ndarray] = None, **options) -> int:
+        """Queues a solve of `box` (right side `b` over the free unknowns, or the model's own);
+        options as for Solver. Returns the request id."""
+        o = _options(**options)
+        bs, keep = box._struct()
+        ptr = None
+        if b is not None:
+            b = np.ascontiguousarray(b, np.float64)
+            ptr = b.ctypes.data_as(C.c_void_p)
+        rid = C.c_uint64()
+        err = _err()
+        if lib().fem_server_submit(self.h, C.byref(bs), C.byref(o), ptr, C.byref(rid), err, 2048):
+            raise FemError(err.value.decode())
+        del keep
+        return int(rid.value)
+
+    def wait(self, rid: int, n: int, timeout: float = -1.0):
+        """The result of request `rid` with `n` free unknowns: (x, Stats, RequestInfo). Raises
+        FemError with the solve's error, TimeoutError on timeout."""
+        x = np.zeros(n)
+        st = _Stats()
+        info = RequestInfo()
+        err = _err()
+        rc = lib().fem_server_wait(self.h, rid, float(timeout), x.ctypes.data_as(C.c_void_p), n, C.byref(st),
+                                   C.byref(info), err, 2048)
+        if rc == 2:
+            raise TimeoutError(f"request {rid} is not done")
+        if rc:
+            raise FemError(err.value.decode())
+        return x, _stats(st), info
+
+    def cancel(self, rid: int) -> bool:
+        return lib().fem_server_cancel(self.h, rid) == 0
+

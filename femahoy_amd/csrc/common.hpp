@@ -24,6 +24,13 @@ struct Error : std::runtime_error {
     } while (0)
 
 // An owned device allocation.
+// Device bytes of the Buffers allocated on this thread while a meter is set: live bytes and their
+// peak (the footprint of a solver's construction).
+struct AllocMeter {
+    int64_t live = 0, peak = 0;
+};
+inline thread_local AllocMeter* alloc_meter = nullptr;
+
 class Buffer {
    public:
     Buffer() = default;
@@ -49,9 +56,14 @@ class Buffer {
         release();
         if (bytes) FEM_HIP(hipMalloc(&p_, bytes));
         n_ = bytes;
+        if (alloc_meter) {
+            alloc_meter->live += (int64_t)bytes;
+            if (alloc_meter->live > alloc_meter->peak) alloc_meter->peak = alloc_meter->live;
+        }
     }
     void release() {
         if (p_) (void)hipFree(p_);
+        if (alloc_meter) alloc_meter->live -= (int64_t)n_;
         p_ = nullptr;
         n_ = 0;
     }

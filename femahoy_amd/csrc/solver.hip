@@ -3,14 +3,20 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fem_solver.h"
@@ -357,6 +363,16 @@ class Solver {
         device_name_ = std::string(prop.name[0] ? prop.name : "AMD GPU") + " (" + prop.gcnArchName + ")";
         FEM_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
         FEM_HIP(hipHostMalloc((void**)&pinned_, 8 * sizeof(double), hipHostMallocDefault));
+        FEM_HIP(hipEventCreateWithFlags(&ev_, hipEventDisableTiming));
+        FEM_HIP(hipEventCreate(&ev_t0_));
+        FEM_HIP(hipEventCreate(&ev_t1_));
+        AllocMeter meter;
+        AllocMeter* const outer_meter = alloc_meter;
+        alloc_meter = &meter;
+        struct Restore {
+            AllocMeter* m;
+            ~Restore() { alloc_meter = m; }
+        } restore{outer_meter};
         use_graph_ = std::getenv("FEM_NO_GRAPH") == nullptr;  // eager launches for debugging
         model_ = build_model(in);
         const size_t nfree = model_.free.size();
@@ -404,11 +420,16 @@ class Solver {
         }
         FEM_HIP(hipStreamSynchronize(s_));
         setup_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        device_bytes_ = meter.peak;
     }
 
     ~Solver() {
+        if (s_) (void)hipStreamSynchronize(s_);  // nothing of this solver may still run on the device
         if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
         if (pinned_) (void)hipHostFree(pinned_);
+        if (ev_) (void)hipEventDestroy(ev_);
+        if (ev_t0_) (void)hipEventDestroy(ev_t0_);
+        if (ev_t1_) (void)hipEventDestroy(ev_t1_);
         if (s_) (void)hipStreamDestroy(s_);
     }
 
@@ -495,24 +516,35 @@ class Solver {
 
     // Solver::solve with device vectors in reference order.
     void solve_device(const double* d_b, double* d_x, int (*cancelled)(void*), void* ctx, fem_stats* st) {
+        start_run(d_b, d_x, cancelled, ctx);
+        while (!advance(true)) {
+        }
+        if (st) *st = run_.out;
+        if (!run_.error.empty()) throw Error(run_.error);
+    }
+
+    // ---- one solve as a resumable state machine: start_run, then advance until it reports done ----
+    // The same launches in the same order whether advance waits (a single solve) or polls (the
+    // serving loop interleaving many solves on their own streams), so a solve's result does not
+    // depend on what else runs.
+    void start_run(const double* d_b, double* d_x, int (*cancelled)(void*), void* ctx) {
         const size_t nf = model_.free.size();
-        hipEvent_t e0, e1;
-        FEM_HIP(hipEventCreate(&e0));
-        FEM_HIP(hipEventCreate(&e1));
-        FEM_HIP(hipEventRecord(e0, s_));
+        run_ = Run{};
+        run_.cancelled = cancelled;
+        run_.ctx = ctx;
+        run_.d_out = d_x;
+        FEM_HIP(hipEventRecord(ev_t0_, s_));
         launch(k_fill<double>, n0_, s_, n0_, b_.data(), 0.0);
         launch(k_fill<double>, n0_, s_, n0_, x_.data(), 0.0);
         launch(k_scatter, nf, s_, nf, (const int64_t*)free_grid_.data(), d_b, b_.data());
         launch(k_scatter, nf, s_, nf, (const int64_t*)free_grid_.data(), (const double*)d_x, x_.data());
         prof_.flush_eager();
         prof_.reset_totals();
-        fem_stats out{};
-        out.method = method_;
-        out.precision = method_ == FEM_METHOD_DIRECT ? FEM_PREC_FP64 : opt_.precision;
-        out.setup_seconds = setup_seconds_;
-        std::string error;
-        try {
-            if (method_ == FEM_METHOD_DIRECT) {
+        run_.out.method = method_;
+        run_.out.precision = method_ == FEM_METHOD_DIRECT ? FEM_PREC_FP64 : opt_.precision;
+        run_.out.setup_seconds = setup_seconds_;
+        if (method_ == FEM_METHOD_DIRECT) {
+            try {
                 // x = A^-1 b, then one step of refinement on the FP64 residual: x += A^-1 (b - A x)
                 direct_->solve<double>(b_.data(), x_.data(), s_);
                 apply64(x_.data(), tmp_.data());
@@ -521,30 +553,146 @@ class Solver {
                 direct_->solve<double>(tmp_.data(), z_.data(), s_);
                 launch(k_axpby, n0_, s_, n0_, 1.0, (const double*)z_.data(), 1.0, x_.data());
                 const double res = true_residual();
-                out.iterations = 0;
-                out.residual = res;
-                out.true_residual = res;
-                out.levels = 1;
-                out.coarse_unknowns = nf;
-            } else {
-                pcg(cancelled, ctx, out);
-                out.levels = h32_ ? h32_->lv.size() : h64_->lv.size();
-                out.coarse_unknowns = h32_ ? h32_->coarse.n() : h64_->coarse.n();
+                run_.out.iterations = 0;
+                run_.out.residual = res;
+                run_.out.true_residual = res;
+                run_.out.levels = 1;
+                run_.out.coarse_unknowns = nf;
+            } catch (const Error& e) {
+                run_.error = e.what();
             }
-        } catch (const Error& e) {
-            error = e.what();
+            finish_run();
+            return;
         }
-        launch(k_gather, nf, s_, nf, (const int64_t*)free_grid_.data(), (const double*)x_.data(), d_x);
-        FEM_HIP(hipEventRecord(e1, s_));
-        FEM_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        FEM_HIP(hipEventElapsedTime(&ms, e0, e1));
-        out.seconds = ms * 1e-3;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        if (st) *st = out;
-        if (!error.empty()) throw Error(error);
+        launch_start();
     }
+
+    // Advances the running solve; returns true once it has finished (solution and stats out, or an
+    // error). With wait = false it returns false at once while the device is still working.
+    bool advance(bool wait) {
+        for (;;) {
+            if (run_.phase == Phase::Finished) return true;
+            if (run_.pending) {
+                if (wait) {
+                    FEM_HIP(hipEventSynchronize(ev_));
+                } else {
+                    const hipError_t q = hipEventQuery(ev_);
+                    if (q == hipErrorNotReady) return false;
+                    FEM_HIP(q);
+                }
+                run_.pending = false;
+            }
+            try {
+                if (run_.phase == Phase::Start) {
+                    const double rr = pinned_[0], bb = pinned_[1];
+                    run_.norm_b = std::sqrt(bb);
+                    if (run_.norm_b == 0.0) {
+                        launch(k_fill<double>, n0_, s_, n0_, x_.data(), 0.0);
+                        run_.out.iterations = 0;
+                        run_.out.residual = 0.0;
+                        run_.out.true_residual = 0.0;
+                        finish_run();
+                        continue;
+                    }
+                    const size_t n = n0_;
+                    const unsigned nbv = nblocks(n);
+                    PcgScalars* sc = scal_.data();
+                    precondition();
+                    launch_blocks((k_pcg_rz<float>), nbv, s_, n, (const float*)nullptr, z_.data(),
+                                  (const double*)r_.data(), partA_.data());
+                    launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr,
+                                  (int)nbv, 3);
+                    launch(k_pcg_dir, n, s_, n, (const PcgScalars*)sc, (const double*)z_.data(), p_.data(), 1);
+                    run_.it = 0;
+                    run_.residual = std::sqrt(rr) / run_.norm_b;
+                } else {  // Iterating: one more iteration has completed
+                    ++run_.it;
+                    run_.residual = std::sqrt(pinned_[0]) / run_.norm_b;
+                }
+                next_step();
+            } catch (const Error& e) {
+                run_.error = e.what();
+                finish_run();
+            }
+        }
+    }
+
+    // An upper estimate of the device bytes a solver of this model holds (admission by footprint in
+    // the serving loop): the level-0 FP64 vectors and coefficients, every level of the V-cycle in its
+    // precision, the element buffers of tet10, the dense coarse inverse or direct factor with its
+    // workspace, and a quarter more for partials and allocation granularity.
+    static uint64_t estimate_bytes(const BoxModel& m, const Options& o) {
+        const uint64_t nfree = m.free.size();
+        const uint64_t cells = (uint64_t)m.in.n[0] * m.in.n[1] * m.in.n[2];
+        const uint64_t nodes0 = (uint64_t)m.gn[0] * m.gn[1] * m.gn[2];
+        const uint64_t dofs0 = nodes0 * m.comps;
+        const uint64_t vb = o.precision == FEM_PREC_FP64 ? 8 : 4;
+        uint64_t bytes = 7 * dofs0 * 8 + 3 * nfree * 8 + 2 * (dofs0 / 256 + 64) * 8;
+        auto coef64 = [&](Kind k, uint64_t c) -> uint64_t {
+            switch (k) {
+                case Kind::HeatHex8: return c * 8;
+                case Kind::HeatTet4: return 6 * c * 8;
+                default: return 12 * c * 8;
+            }
+        };
+        // the probed matrix, the factor's A, U and inverse, its panels, and the kept inverse
+        auto dense = [&](uint64_t n) -> uint64_t {
+            const uint64_t np = (n + 63) / 64 * 64;
+            return np * np * 8 * 4 + np * 64 * 8 * 2 + np * np * vb + (1u << 18);
+        };
+        const bool direct = o.method == FEM_METHOD_DIRECT || (o.method == FEM_METHOD_AUTO && nfree <= o.crossover);
+        bytes += coef64(m.in.kind, cells);
+        if (m.in.kind == Kind::ElasticTet10) bytes += 180 * cells * 8;
+        if (direct) return (bytes + dense(nfree)) * 5 / 4;
+        // V-cycle levels: level 0, then (tet10) tet4 on the same cells, then halvings
+        int c[3] = {m.in.n[0], m.in.n[1], m.in.n[2]};
+        uint64_t dofs = dofs0;
+        Kind kind = m.in.kind;
+        int lev = 0;
+        uint64_t last_free = nfree;
+        for (;;) {
+            const uint64_t lc = (uint64_t)c[0] * c[1] * c[2];
+            bytes += 6 * dofs * vb;
+            if (lev > 0) {
+                const uint64_t nodes = (uint64_t)(c[0] + 1) * (c[1] + 1) * (c[2] + 1);
+                const uint64_t cf = kind == Kind::HeatHex8 && o.coarse_galerkin ? 14 * nodes * 8 + 64 * lc * 8 : coef64(kind, lc);
+                bytes += cf + cf * vb / 8;
+            } else {
+                bytes += coef64(kind, lc) * vb / 8;
+                if (kind == Kind::ElasticTet10) bytes += 180 * lc * vb;
+            }
+            ++lev;
+            if (o.levels > 0 && lev >= o.levels) break;
+            Kind next = kind;
+            int nc[3] = {c[0], c[1], c[2]};
+            if (kind == Kind::ElasticTet10) {
+                next = Kind::ElasticTet4;
+            } else {
+                if (o.levels == 0 && last_free <= 1000) break;
+                bool halves = true;
+                for (int d = 0; d < 3; ++d) halves = halves && c[d] % 2 == 0 && c[d] >= 2;
+                if (!halves) break;
+                for (int d = 0; d < 3; ++d) nc[d] = c[d] / 2;
+            }
+            kind = next;
+            for (int d = 0; d < 3; ++d) c[d] = nc[d];
+            const uint64_t nodes = (uint64_t)(c[0] + 1) * (c[1] + 1) * (c[2] + 1);
+            dofs = nodes * m.comps;
+            last_free = dofs;
+        }
+        return (bytes + dense(std::min<uint64_t>(last_free, o.coarse_max))) * 5 / 4;
+    }
+
+    // The state of the running solve.
+    bool finished() const { return run_.phase == Phase::Finished; }
+    const fem_stats& run_stats() const { return run_.out; }
+    const std::string& run_error() const { return run_.error; }
+    // Host staging of the free-unknown vectors of a solve (the serving loop's requests).
+    double* stage_b() { return fb_.data(); }
+    double* stage_x() { return fx_.data(); }
+    hipStream_t stream() const { return s_; }
+    size_t free_count_host() const { return model_.free.size(); }
+    int64_t device_bytes() const { return device_bytes_; }
 
     void solve_host(const double* b, double* x, int (*cancelled)(void*), void* ctx, fem_stats* st) {
         const size_t nf = model_.free.size();
@@ -576,6 +724,21 @@ class Solver {
     bool use_graph_ = true;
     hipGraphExec_t graph_exec_ = nullptr;
     double* pinned_ = nullptr;  // host-visible scalars read back each iteration
+    hipEvent_t ev_ = nullptr, ev_t0_ = nullptr, ev_t1_ = nullptr;  // readback; device time of a solve
+    enum class Phase { Idle, Start, Iterating, Finished };
+    struct Run {
+        Phase phase = Phase::Idle;
+        bool pending = false;  // a readback into pinned_ is in flight (ev_)
+        uint64_t total_it = 0, it = 0;
+        int restarts = 0;
+        double norm_b = 0.0, residual = 0.0;
+        int (*cancelled)(void*) = nullptr;
+        void* ctx = nullptr;
+        double* d_out = nullptr;
+        fem_stats out{};
+        std::string error;
+    } run_;
+    int64_t device_bytes_ = 0;  // peak device bytes of the construction
     size_t n0_ = 0;
     int parts_ = 0;
     DArray<double> x_, b_, r_, z_, p_, q_, tmp_, partA_, partB_, red_, fb_, fx_;
@@ -1076,66 +1239,361 @@ class Solver {
         FEM_HIP(hipGraphLaunch(graph_exec_, s_));
     }
 
-    // multigrid.rs:692-753, with FP64 vectors and scalars on the device.
-    void pcg(int (*cancelled)(void*), void* ctx, fem_stats& out) {
+    // multigrid.rs:692-753 as steps: r = b - A x and the scalars, then one readback per iteration.
+    void launch_start() {
         Level<double>& L0 = lv64_[0];
-        const size_t n = n0_;
-        const unsigned nbn = nblocks(L0.nodes());  // partials of node kernels
-        const unsigned nbv = nblocks(n);          // partials of vector kernels
-        PcgScalars* sc = scal_.data();
-        uint64_t total_it = 0;
-        int restarts = 0;
-        for (;;) {
-            prepare(L0, x_.data(), s_);
-            with_op(L0, [&](auto op) {
-                launch_blocks((k_pcg_start<decltype(op)>), nbn, s_, op,
-                                   (const double*)x_.data(), (const double*)b_.data(), r_.data(), partA_.data(),
-                                   partB_.data());
-            });
-            launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(),
-                               (const double*)partB_.data(), (int)nbn, 0);
-            const double bb = read_scalar(offsetof(PcgScalars, bb) / sizeof(double));
-            const double norm_b = std::sqrt(bb);
-            if (norm_b == 0.0) {
-                launch(k_fill<double>, n, s_, n, x_.data(), 0.0);
-                out.iterations = 0;
-                out.residual = 0.0;
-                out.true_residual = 0.0;
+        const unsigned nbn = nblocks(L0.nodes());
+        prepare(L0, x_.data(), s_);
+        with_op(L0, [&](auto op) {
+            launch_blocks((k_pcg_start<decltype(op)>), nbn, s_, op, (const double*)x_.data(), (const double*)b_.data(),
+                          r_.data(), partA_.data(), partB_.data());
+        });
+        launch_reduce(k_pcg_scalars, s_, scal_.data(), (const double*)partA_.data(), (const double*)partB_.data(),
+                      (int)nbn, 0);
+        // rr and bb are adjacent in PcgScalars
+        FEM_HIP(hipMemcpyAsync(pinned_, reinterpret_cast<const double*>(scal_.data()) + offsetof(PcgScalars, rr) / sizeof(double),
+                               2 * sizeof(double), hipMemcpyDeviceToHost, s_));
+        FEM_HIP(hipEventRecord(ev_, s_));
+        run_.phase = Phase::Start;
+        run_.pending = true;
+    }
+
+    // The test of the loop head (multigrid.rs:722-737) on the current residual, then one more
+    // iteration with its readback, or the end of this PCG pass.
+    void next_step() {
+        if (run_.residual <= opt_.tol) {
+            run_.total_it += run_.it;
+            const double tr = true_residual();
+            run_.out.iterations = run_.total_it;
+            run_.out.residual = run_.residual;
+            run_.out.true_residual = tr;
+            run_.out.restarts = run_.restarts;
+            if (!opt_.refine || tr <= opt_.tol || run_.restarts >= 5) {
+                finish_run();
                 return;
             }
-            precondition();
-            launch_blocks((k_pcg_rz<float>), nbv, s_, n, (const float*)nullptr, z_.data(),
-                               (const double*)r_.data(), partA_.data());
-            launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(),
-                               (const double*)nullptr, (int)nbv, 3);
-            launch(k_pcg_dir, n, s_, n, (const PcgScalars*)sc, (const double*)z_.data(), p_.data(), 1);
-            uint64_t it = 0;
-            double residual = 0.0;
-            for (;; ++it) {
-                residual = std::sqrt(read_scalar(offsetof(PcgScalars, rr) / sizeof(double))) / norm_b;
-                if (residual <= opt_.tol) break;
-                if (total_it + it == opt_.max_iter) {
-                    out.iterations = total_it + it;
-                    out.residual = residual;
-                    out.restarts = restarts;
-                    throw Error("conjugate gradients did not converge: the residual is " + rust_e2(residual) + " after " +
-                                std::to_string(opt_.max_iter) + " iterations");
-                }
-                if (cancelled && cancelled(ctx)) throw Error("cancelled");
-                iterate();
-            }
-            total_it += it;
-            const double tr = true_residual();
-            out.iterations = total_it;
-            out.residual = residual;
-            out.true_residual = tr;
-            out.restarts = restarts;
-            if (!opt_.refine || tr <= opt_.tol || restarts >= 5) return;
-            ++restarts;
+            ++run_.restarts;
+            launch_start();
+            return;
         }
+        if (run_.total_it + run_.it == opt_.max_iter) {
+            run_.out.iterations = run_.total_it + run_.it;
+            run_.out.residual = run_.residual;
+            run_.out.restarts = run_.restarts;
+            throw Error("conjugate gradients did not converge: the residual is " + rust_e2(run_.residual) + " after " +
+                        std::to_string(opt_.max_iter) + " iterations");
+        }
+        if (run_.cancelled && run_.cancelled(run_.ctx)) throw Error("cancelled");
+        iterate();
+        FEM_HIP(hipMemcpyAsync(pinned_, reinterpret_cast<const double*>(scal_.data()) + offsetof(PcgScalars, rr) / sizeof(double),
+                               sizeof(double), hipMemcpyDeviceToHost, s_));
+        FEM_HIP(hipEventRecord(ev_, s_));
+        run_.phase = Phase::Iterating;
+        run_.pending = true;
+    }
+
+    // The solution to the caller's vector, the device time, the hierarchy facts.
+    void finish_run() {
+        const size_t nf = model_.free.size();
+        if (method_ != FEM_METHOD_DIRECT) {
+            run_.out.levels = h32_ ? h32_->lv.size() : h64_->lv.size();
+            run_.out.coarse_unknowns = h32_ ? h32_->coarse.n() : h64_->coarse.n();
+        }
+        launch(k_gather, nf, s_, nf, (const int64_t*)free_grid_.data(), (const double*)x_.data(), run_.d_out);
+        FEM_HIP(hipEventRecord(ev_t1_, s_));
+        FEM_HIP(hipEventSynchronize(ev_t1_));
+        float ms = 0.f;
+        FEM_HIP(hipEventElapsedTime(&ms, ev_t0_, ev_t1_));
+        run_.out.seconds = ms * 1e-3;
+        run_.phase = Phase::Finished;
+        run_.pending = false;
     }
 };
 
+
+// ---------------------------------------------------------------------------------------------
+// The serving loop: a persistent request loop over Solver runs (no counterpart in the reference;
+// its analogues are Solver reuse across right sides, heat.rs:697-735, the cancel closure of
+// multigrid.rs:735 and refusal by size before allocation, heat.rs:1180-1186).
+//
+//  - submit() queues a solve with its model, options and right side (copied).
+//  - Admission in arrival order by device footprint: a solve is admitted while the admitted ones
+//    hold less than the budget (estimated bytes) and fewer than max_active are on the device.
+//  - A setup thread builds each admitted solve (model, hierarchy, factors: host work and device
+//    allocation on the solve's own stream) and starts its run; the loop thread then advances every
+//    running solve without waiting: each advance launches one PCG iteration (a replayed hipGraph)
+//    on that solve's stream and returns, so the iterations of all running solves are in flight on
+//    the device together, and a solve that finishes frees its slot and memory for the next one.
+//  - Each run is the same sequence of launches as a solve on its own, on its own stream and
+//    buffers, so its result is bitwise the same alone and in any mix.
+struct ServeRequest {
+    uint64_t id = 0;
+    BoxInput in;
+    Options opt;
+    std::vector<double> b;  // empty: the model's own right side
+    std::atomic<bool> cancel{false};
+    std::unique_ptr<Solver> solver;
+    std::vector<double> x;
+    fem_stats st{};
+    std::string err;
+    bool done = false;
+    uint64_t footprint = 0;
+    int64_t device_bytes = 0;
+    double t_submit = 0, t_admit = 0, t_start = 0, t_finish = 0;
+};
+
+class Server {
+   public:
+    Server(uint64_t budget, int max_active, int device)
+        : budget_(budget), max_active_(max_active > 0 ? max_active : 1), device_(device), t0_(clock::now()) {
+        loop_ = std::thread([this] { loop(); });
+        setup_ = std::thread([this] { setup_loop(); });
+    }
+    ~Server() {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        setup_cv_.notify_all();
+        loop_.join();
+        setup_.join();
+    }
+
+    uint64_t submit(const BoxInput& in, const Options& o, const double* b) {
+        auto r = std::make_shared<ServeRequest>();
+        r->in = in;
+        r->opt = o;
+        r->opt.device = device_;
+        const BoxModel m = build_model(in);
+        if (b) r->b.assign(b, b + m.free.size());
+        r->footprint = Solver::estimate_bytes(m, r->opt);
+        if (r->footprint > budget_)
+            throw Error("the solve needs about " + std::to_string(r->footprint) + " device bytes, more than the server's budget of " +
+                        std::to_string(budget_));
+        std::lock_guard<std::mutex> g(m_);
+        if (!fatal_.empty()) throw Error(fatal_);
+        r->id = ++next_id_;
+        r->t_submit = now();
+        all_[r->id] = r;
+        queue_.push_back(r);
+        cv_.notify_all();
+        return r->id;
+    }
+
+    // 0 done (results out), 1 the solve failed (message out), 2 timeout; the request is forgotten
+    // once its result has been taken.
+    int wait(uint64_t id, double timeout_s, double* x, uint64_t n, fem_stats* st, fem_request_info* info,
+             std::string& err) {
+        std::unique_lock<std::mutex> g(m_);
+        auto it = all_.find(id);
+        if (it == all_.end()) {
+            err = "no such request";
+            return 1;
+        }
+        std::shared_ptr<ServeRequest> r = it->second;
+        auto ready = [&] { return r->done; };
+        if (timeout_s < 0) {
+            done_cv_.wait(g, ready);
+        } else if (!done_cv_.wait_for(g, std::chrono::duration<double>(timeout_s), ready)) {
+            return 2;
+        }
+        all_.erase(id);
+        if (st) *st = r->st;
+        if (info) {
+            info->submit_time = r->t_submit;
+            info->admit_time = r->t_admit;
+            info->start_time = r->t_start;
+            info->finish_time = r->t_finish;
+            info->footprint = r->footprint;
+            info->device_bytes = r->device_bytes;
+        }
+        if (x && n == r->x.size()) std::copy(r->x.begin(), r->x.end(), x);
+        if (!r->err.empty()) {
+            err = r->err;
+            return 1;
+        }
+        if (x && n != r->x.size()) {
+            err = "x needs " + std::to_string(r->x.size()) + " values";
+            return 1;
+        }
+        return 0;
+    }
+
+    bool cancel(uint64_t id) {
+        std::lock_guard<std::mutex> g(m_);
+        auto it = all_.find(id);
+        if (it == all_.end()) return false;
+        it->second->cancel = true;
+        return true;
+    }
+
+   private:
+    using clock = std::chrono::steady_clock;
+    double now() const { return std::chrono::duration<double>(clock::now() - t0_).count(); }
+
+    static int cancel_fn(void* ctx) { return static_cast<ServeRequest*>(ctx)->cancel.load() ? 1 : 0; }
+
+    // Admission (under m_): FIFO while the footprint fits and a slot is free.
+    void admit() {
+        while (!queue_.empty() && admitted_ < max_active_ && used_ + queue_.front()->footprint <= budget_) {
+            auto r = queue_.front();
+            queue_.pop_front();
+            used_ += r->footprint;
+            ++admitted_;
+            r->t_admit = now();
+            to_setup_.push_back(r);
+        }
+        if (!to_setup_.empty()) setup_cv_.notify_one();
+    }
+
+    void complete(const std::shared_ptr<ServeRequest>& r) {
+        // the result leaves the device before the solver (and its memory) goes
+        if (r->solver) {
+            r->st = r->solver->run_stats();
+            if (r->err.empty()) r->err = r->solver->run_error();
+            const size_t nf = r->solver->free_count_host();
+            r->x.resize(nf);
+            FEM_HIP(hipMemcpyAsync(r->x.data(), r->solver->stage_x(), nf * sizeof(double), hipMemcpyDeviceToHost,
+                                   r->solver->stream()));
+            FEM_HIP(hipStreamSynchronize(r->solver->stream()));
+            r->solver.reset();
+        }
+        std::lock_guard<std::mutex> g(m_);
+        r->t_finish = now();
+        r->done = true;
+        used_ -= r->footprint;
+        --admitted_;
+        admit();
+        done_cv_.notify_all();
+    }
+
+    // A failure outside any one solve (the device itself) ends the server: every waiter gets it.
+    void fail_all(const std::string& what) {
+        std::lock_guard<std::mutex> g(m_);
+        fatal_ = what;
+        for (auto& kv : all_) {
+            if (!kv.second->done) {
+                kv.second->err = what;
+                kv.second->done = true;
+            }
+        }
+        stop_ = true;
+        cv_.notify_all();
+        setup_cv_.notify_all();
+        done_cv_.notify_all();
+    }
+
+    void setup_loop() {
+        try {
+            setup_body();
+        } catch (const std::exception& e) {
+            fail_all(e.what());
+        }
+    }
+
+    void setup_body() {
+        FEM_HIP(hipSetDevice(device_));
+        for (;;) {
+            std::shared_ptr<ServeRequest> r;
+            {
+                std::unique_lock<std::mutex> g(m_);
+                setup_cv_.wait(g, [&] { return stop_ || !to_setup_.empty(); });
+                if (stop_) return;
+                r = to_setup_.front();
+                to_setup_.pop_front();
+            }
+            try {
+                r->solver = std::make_unique<Solver>(r->in, r->opt);
+                r->device_bytes = r->solver->device_bytes();
+                Solver& s = *r->solver;
+                const size_t nf = s.free_count_host();
+                if (r->b.empty()) {
+                    r->b.resize(nf);
+                    s.rhs(r->b.data());
+                }
+                std::vector<double> zero(nf, 0.0);
+                FEM_HIP(hipMemcpyAsync(s.stage_b(), r->b.data(), nf * sizeof(double), hipMemcpyHostToDevice, s.stream()));
+                FEM_HIP(hipMemcpyAsync(s.stage_x(), zero.data(), nf * sizeof(double), hipMemcpyHostToDevice, s.stream()));
+                FEM_HIP(hipStreamSynchronize(s.stream()));
+                r->t_start = now();
+                s.start_run(s.stage_b(), s.stage_x(), cancel_fn, r.get());
+            } catch (const std::exception& e) {
+                r->err = e.what();
+                r->solver.reset();
+            }
+            std::lock_guard<std::mutex> g(m_);
+            ready_.push_back(r);
+            cv_.notify_all();
+        }
+    }
+
+    void loop() {
+        try {
+            loop_body();
+        } catch (const std::exception& e) {
+            fail_all(e.what());
+        }
+    }
+
+    void loop_body() {
+        FEM_HIP(hipSetDevice(device_));
+        std::vector<std::shared_ptr<ServeRequest>> active;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> g(m_);
+                if (stop_) return;
+                admit();
+                for (auto& r : ready_) active.push_back(r);
+                ready_.clear();
+                if (active.empty()) {
+                    cv_.wait(g, [&] {
+                        return stop_ || !ready_.empty() ||
+                               (!queue_.empty() && admitted_ < max_active_ && used_ + queue_.front()->footprint <= budget_);
+                    });
+                    continue;
+                }
+            }
+            bool progressed = false;
+            for (size_t k = 0; k < active.size();) {
+                auto& r = active[k];
+                bool finished = !r->solver;  // setup failed
+                if (!finished) {
+                    try {
+                        finished = r->solver->advance(false);
+                    } catch (const std::exception& e) {
+                        r->err = e.what();
+                        finished = true;
+                    }
+                }
+                if (finished) {
+                    complete(r);
+                    active[k] = active.back();
+                    active.pop_back();
+                    progressed = true;
+                } else {
+                    ++k;
+                }
+            }
+            if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
+        }
+    }
+
+    const uint64_t budget_;
+    const int max_active_;
+    const int device_;
+    const clock::time_point t0_;
+    std::mutex m_;
+    std::condition_variable cv_, setup_cv_, done_cv_;
+    bool stop_ = false;
+    std::string fatal_;
+    uint64_t next_id_ = 0, used_ = 0;
+    int admitted_ = 0;
+    std::deque<std::shared_ptr<ServeRequest>> queue_, to_setup_;
+    std::vector<std::shared_ptr<ServeRequest>> ready_;
+    std::map<uint64_t, std::shared_ptr<ServeRequest>> all_;
+    std::thread loop_, setup_;
+};
 }  // namespace fem
 
 // ---------------------------------------------------------------------------------------------
@@ -1153,6 +1611,28 @@ void put_err(char* err, size_t len, const char* msg) {
 }
 
 fem::BoxInput box_input(const fem_box* box);
+
+// The solver options of a fem_options.
+fem::Options options_from(const fem_options& opt) {
+    fem::Options o;
+    o.tol = opt.tol;
+    o.max_iter = opt.max_iter;
+    o.method = opt.method;
+    o.precision = opt.precision;
+    o.crossover = opt.crossover;
+    o.levels = opt.levels;
+    o.degree = opt.cheb_degree > 0 ? opt.cheb_degree : 3;
+    o.cheb_lower = opt.cheb_lower;
+    o.cheb_upper = opt.cheb_upper;
+    o.power_steps = opt.power_steps > 0 ? opt.power_steps : 20;
+    o.coarse_max = opt.coarse_max;
+    o.refine = opt.refine;
+    o.device = opt.device;
+    o.coarse_galerkin = opt.coarse_galerkin;
+    if (o.precision < 0 || o.precision > 2) throw fem::Error("unknown precision");
+    if (o.precision == FEM_PREC_BF16) o.precision = FEM_PREC_FP32;  // BF16 coefficients: not in this build
+    return o;
+}
 
 template <class F>
 int guarded(char* err, size_t errlen, F&& f) {
@@ -1193,25 +1673,9 @@ int fem_solver_new_box(const fem_box* box, const fem_options* opt, fem_solver** 
         if (!out) throw fem::Error("an output handle is needed");
         *out = nullptr;
         const fem::BoxInput in = box_input(box);
-        fem::Options o;
-        if (opt) {
-            o.tol = opt->tol;
-            o.max_iter = opt->max_iter;
-            o.method = opt->method;
-            o.precision = opt->precision;
-            o.crossover = opt->crossover;
-            o.levels = opt->levels;
-            o.degree = opt->cheb_degree > 0 ? opt->cheb_degree : 3;
-            o.cheb_lower = opt->cheb_lower;
-            o.cheb_upper = opt->cheb_upper;
-            o.power_steps = opt->power_steps > 0 ? opt->power_steps : 20;
-            o.coarse_max = opt->coarse_max;
-            o.refine = opt->refine;
-            o.device = opt->device;
-            o.coarse_galerkin = opt->coarse_galerkin;
-        }
-        if (o.precision < 0 || o.precision > 2) throw fem::Error("unknown precision");
-        if (o.precision == FEM_PREC_BF16) o.precision = FEM_PREC_FP32;  // BF16 coefficients: not in this build
+        fem_options def;
+        fem_options_default(&def);
+        const fem::Options o = options_from(opt ? *opt : def);
         auto* s = new fem_solver;
         try {
             s->impl = std::make_unique<fem::Solver>(in, o);
@@ -1330,6 +1794,45 @@ int fem_solver_profile(fem_solver* s, fem_profile* out) {
         return 1;
     }
 }
+
+struct fem_server {
+    std::unique_ptr<fem::Server> impl;
+};
+
+int fem_server_new(const fem_server_options* opt, fem_server** out, char* err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!opt || !out) throw fem::Error("fem_server_new needs options and an output handle");
+        auto* srv = new fem_server;
+        srv->impl = std::make_unique<fem::Server>(opt->hbm_budget, opt->max_active, opt->device);
+        *out = srv;
+    });
+}
+
+int fem_server_submit(fem_server* srv, const fem_box* box, const fem_options* opt, const double* b, uint64_t* id,
+                      char* err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!srv || !box || !id) throw fem::Error("fem_server_submit needs a server, a box and an id output");
+        fem_options def;
+        fem_options_default(&def);
+        *id = srv->impl->submit(box_input(box), options_from(opt ? *opt : def), b);
+    });
+}
+
+int fem_server_wait(fem_server* srv, uint64_t id, double timeout_s, double* x, uint64_t n, fem_stats* st,
+                    fem_request_info* info, char* err, size_t errlen) {
+    if (!srv) {
+        put_err(err, errlen, "no server");
+        return 1;
+    }
+    std::string e;
+    const int rc = srv->impl->wait(id, timeout_s, x, n, st, info, e);
+    if (rc == 1) put_err(err, errlen, e.c_str());
+    return rc;
+}
+
+int fem_server_cancel(fem_server* srv, uint64_t id) { return srv && srv->impl->cancel(id) ? 0 : 1; }
+
+void fem_server_free(fem_server* srv) { delete srv; }
 
 }  // extern "C"
 

@@ -187,6 +187,55 @@ int fem_solver_profile(fem_solver* s, fem_profile* out);
  * smoother step and the outer q = A p. Measurement only; later solves are unaffected. */
 int fem_solver_time_kernels(fem_solver* s, int reps, fem_profile* out);
 
+/* ---- the serving loop (no counterpart in the reference) ----
+ * A persistent request loop over solves of uneven sizes: submit queues a solve (model, options,
+ * right side), which is admitted in arrival order while the admitted solves' estimated device
+ * footprint stays within hbm_budget and fewer than max_active are on the device. A setup thread
+ * builds each admitted solve (model, hierarchy, coarse factor) on its own stream, and the loop
+ * thread advances every running solve by one PCG iteration at a time without waiting (a replayed
+ * hipGraph per iteration on the solve's own stream), so the work of all running solves overlaps
+ * on the device; a finished solve frees its slot and memory for the next. Each solve runs the
+ * same launches as fem_solver_solve on its own stream and buffers, so its result is bitwise the
+ * same alone and in any mix. Reference analogues: Solver reuse across right sides
+ * (heat.rs:697-735), the cancel closure (multigrid.rs:735), refusal by size before allocation
+ * (heat.rs:1180-1186). */
+typedef struct fem_server fem_server;
+
+typedef struct fem_server_options {
+    uint64_t hbm_budget; /* device bytes the admitted solves may hold together (estimated footprints) */
+    int32_t max_active;  /* solves admitted at once */
+    int32_t device;      /* HIP device ordinal of every solve */
+} fem_server_options;
+
+typedef struct fem_request_info {
+    double submit_time;   /* seconds since the server started */
+    double admit_time;    /* admitted (slot and memory reserved) */
+    double start_time;    /* set up (hierarchy built), first PCG launch */
+    double finish_time;   /* solution on the host */
+    uint64_t footprint;   /* estimated device bytes reserved at admission */
+    int64_t device_bytes; /* measured peak device bytes of the solve's construction */
+} fem_request_info;
+
+int fem_server_new(const fem_server_options* opt, fem_server** out, char* err, size_t errlen);
+
+/* Queues a solve. `b` holds the free unknowns' right side (copied), or NULL for the model's own
+ * (fem_solver_rhs); `opt` NULL takes the defaults. Refuses a solve whose estimated footprint
+ * exceeds the budget. */
+int fem_server_submit(fem_server* srv, const fem_box* box, const fem_options* opt, const double* b, uint64_t* id,
+                      char* err, size_t errlen);
+
+/* Waits for solve `id` (timeout_s < 0: no limit) and takes its result: x (n free unknowns, as
+ * fem_model_free_count), stats and timings. Returns 0, 1 when the solve failed (its error in
+ * err), 2 on timeout (the request stays). */
+int fem_server_wait(fem_server* srv, uint64_t id, double timeout_s, double* x, uint64_t n, fem_stats* st,
+                    fem_request_info* info, char* err, size_t errlen);
+
+/* Asks a queued or running solve to stop; it ends with the error "cancelled". */
+int fem_server_cancel(fem_server* srv, uint64_t id);
+
+/* Stops the loop (after the running solves' current step) and frees every solve. */
+void fem_server_free(fem_server* srv);
+
 /* ---- the model alone, on the host (no device is touched) ---- */
 
 typedef struct fem_model fem_model;

@@ -1,0 +1,118 @@
+"""The serving loop (fem_server_*): each solve's result is the same alone and in a mix, admission
+respects the device-memory budget, and cancellation and refusal behave as documented.
+
+Tolerance: a solve in the mix must equal the same solve alone to 1e-12 relative L2; the loop runs
+the same launches on the solve's own stream and buffers, so the test also expects bitwise
+equality, and checks each against the oracle's direct solution to the 1e-8 parity bar.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def specs():
+    """Refinement pairs (h, h/2) of heat tet4 and elasticity tet10 boxes, seeded materials."""
+    rng = np.random.default_rng(17)
+    out = []
+    for n in ([4, 2, 2], [8, 4, 4]):
+        nt = 6 * int(np.prod(n))
+        out.append(("heat_tet4", n, dict(coef0=rng.uniform(0.5, 2.0, nt), fixed={"x0": 1.0, "x1": 0.0})))
+        out.append(("elastic_tet10", n, dict(coef0=rng.uniform(1.0, 2.0, nt), coef1=rng.uniform(0.2, 0.35, nt),
+                                             fixed={"x0": True}, body=(0, 0, -1))))
+    for n in ([8, 4, 4], [16, 8, 8]):
+        nt = 6 * int(np.prod(n))
+        out.append(("heat_tet4", n, dict(coef0=rng.uniform(0.5, 2.0, nt), fixed={"y0": 0.0, "z1": 2.0}, source=1.0)))
+    return out
+
+
+def make_box(fem, kind, n, kw):
+    length = [float(v) / max(n) for v in n]
+    return fem.Box(kind, n, length, kw["coef0"], kw.get("coef1"), fixed=kw["fixed"], source=kw.get("source", 0.0),
+                   body=kw.get("body", (0.0, 0.0, 0.0)))
+
+
+OPTS = dict(method="multigrid", precision="fp32", tol=1e-10)
+
+
+def test_each_result_is_the_same_alone_and_in_the_mix(fem, oracle):
+    cases = specs()
+    alone = []
+    for kind, n, kw in cases:
+        s = fem.Solver(make_box(fem, kind, n, kw), **OPTS)
+        x, st = s.solve(s.rhs())
+        alone.append(x)
+    srv = fem.Server(hbm_budget=8 << 30, max_active=3)
+    try:
+        ids = [srv.submit(make_box(fem, kind, n, kw), **OPTS) for kind, n, kw in cases]
+        for (kind, n, kw), rid, xa in zip(cases, ids, alone):
+            x, st, info = srv.wait(rid, len(xa), timeout=120)
+            print(kind, n, st.iterations, f"{info.admit_time:.3f} {info.start_time:.3f} {info.finish_time:.3f}",
+                  info.footprint, info.device_bytes)
+            assert np.linalg.norm(x - xa) <= 1e-12 * np.linalg.norm(xa)
+            assert np.array_equal(x, xa)
+            assert info.device_bytes <= info.footprint
+    finally:
+        srv.close()
+    # and each agrees with the oracle's direct solve
+    for (kind, n, kw), xa in zip(cases, alone):
+        length = [float(v) / max(n) for v in n]
+        if kind == "heat_tet4":
+            pr = oracle.Problem.heat(n, length, kw["coef0"], fixed=kw["fixed"], source=kw.get("source", 0.0), tet4=True)
+        else:
+            pr = oracle.Problem.elastic(10, n, length, kw["coef0"], kw["coef1"], clamped=list(kw["fixed"]),
+                                        body=kw["body"])
+        xd, _, _ = oracle.Solver(pr, "direct").solve(pr.rhs())
+        assert np.linalg.norm(xa - xd) <= 1e-8 * np.linalg.norm(xd)
+
+
+def test_admission_keeps_the_footprint_within_the_budget(fem):
+    cases = [c for c in specs() if c[0] == "heat_tet4"]
+    boxes = [make_box(fem, kind, n, kw) for kind, n, kw in cases]
+    # a budget that holds the largest solve but not two of them
+    probe = fem.Server(hbm_budget=8 << 30, max_active=1)
+    try:
+        fps = []
+        for b in boxes:
+            rid = probe.submit(b, **OPTS)
+            _, _, info = probe.wait(rid, fem.Model(b).free_list().size, timeout=120)
+            fps.append(info.footprint)
+    finally:
+        probe.close()
+    budget = max(fps) + min(fps) - 1
+    srv = fem.Server(hbm_budget=budget, max_active=8)
+    try:
+        ids = [srv.submit(b, **OPTS) for b in boxes]
+        spans = []
+        for b, rid, fp in zip(boxes, ids, fps):
+            _, _, info = srv.wait(rid, fem.Model(b).free_list().size, timeout=120)
+            spans.append((info.admit_time, info.finish_time, fp))
+        # at any admission instant, the footprints of the solves then admitted fit the budget
+        for t, _, _ in spans:
+            held = sum(fp for a, f, fp in spans if a <= t < f)
+            assert held <= budget
+    finally:
+        srv.close()
+
+
+def test_a_cancelled_solve_ends_with_the_reference_text(fem):
+    kind, n, kw = specs()[-1]
+    box = make_box(fem, kind, n, kw)
+    srv = fem.Server(hbm_budget=8 << 30, max_active=2)
+    try:
+        rid = srv.submit(box, method="multigrid", precision="fp32", tol=1e-300, max_iter=100000)
+        assert srv.cancel(rid)
+        with pytest.raises(fem.FemError, match="^cancelled$"):
+            srv.wait(rid, fem.Model(box).free_list().size, timeout=120)
+    finally:
+        srv.close()
+
+
+def test_a_solve_beyond_the_budget_is_refused(fem):
+    kind, n, kw = specs()[-1]
+    srv = fem.Server(hbm_budget=1 << 16, max_active=2)
+    try:
+        with pytest.raises(fem.FemError, match="more than the server's budget"):
+            srv.submit(make_box(fem, kind, n, kw), **OPTS)
+    finally:
+        srv.close()
