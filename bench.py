@@ -129,6 +129,93 @@ def timed_solves(F, hip, solver, d_b, d_x, x0, steps, warmup, profile=False):
     return lat, st, prof, total
 
 
+def serve_requests(F, n_solves, seed):
+    """The serving stream: refinement pairs (h, h/2) of heat tet4 or elasticity tet10 boxes (50/50),
+    the coarse member's size log-uniform so that both members lie in about 5k-500k DOF. Boxes of
+    2m x m x m cells (tet4 heat: m in 12..24, fine 2m; tet10: m in 4..10, fine 2m) so that every
+    level halves down to a small dense coarse level. Materials seeded."""
+    rng = np.random.default_rng(seed)
+    heat_m = [12, 16, 20, 24]
+    elas_m = [4, 6, 8, 10]
+
+    def dofs(kind, m):
+        if kind == "heat_tet4":
+            return (2 * m + 1) * (m + 1) ** 2
+        return 3 * (4 * m + 1) * (2 * m + 1) ** 2
+
+    reqs = []
+    for p in range(n_solves // 2):
+        kind = "heat_tet4" if rng.random() < 0.5 else "elastic_tet10"
+        ms = heat_m if kind == "heat_tet4" else elas_m
+        target = np.exp(rng.uniform(np.log(5e3), np.log(5e5 / 8)))
+        m = min(ms, key=lambda v: abs(np.log(dofs(kind, v) / target)))
+        for mm in (m, 2 * m):
+            cells = [2 * mm, mm, mm]
+            nt = 6 * int(np.prod(cells))
+            length = [2.0, 1.0, 1.0]
+            if kind == "heat_tet4":
+                box = F.Box(kind, cells, length, np.exp(0.5 * rng.standard_normal(nt)), fixed={"x0": 0.0, "x1": 1.0})
+            else:
+                box = F.Box(kind, cells, length, rng.uniform(1.0, 2.0, nt), np.full(nt, 0.3), fixed={"x0": True},
+                            body=(0.0, 0.0, -1.0))
+            reqs.append((p, kind, mm, box, int(F.Model(box).free_list().size)))
+    return reqs
+
+
+def run_stream(F, reqs, arrivals, budget, max_active):
+    """Submits reqs at their arrival times (seconds from the start; None = all at once) and waits
+    for all; returns (makespan s, latencies s, per-request stats)."""
+    srv = F.Server(hbm_budget=budget, max_active=max_active, setup_threads=4)
+    t0 = time.perf_counter()
+    ids = []
+    opts = dict(method="multigrid", precision="fp32", tol=1e-8)
+    for k, (_, kind, m, box, n) in enumerate(reqs):
+        if arrivals is not None:
+            dt = arrivals[k] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+        ids.append(srv.submit(box, **opts))
+    lat, its = [], []
+    for rid, (_, kind, m, box, n) in zip(ids, reqs):
+        _, st, info = srv.wait(rid, n, timeout=600)
+        lat.append(info.finish_time - info.submit_time)
+        its.append(st.iterations)
+    makespan = time.perf_counter() - t0
+    srv.close()
+    return makespan, np.array(lat), its
+
+
+def serving(F, n_solves, seed):
+    reqs = serve_requests(F, n_solves, seed)
+    total_dof = sum(r[4] for r in reqs)
+    budget, max_active = 64 << 30, 16
+    log(f"serving: {len(reqs)} solves ({total_dof} DOF), capacity run (all submitted at once)")
+    cap_span, cap_lat, _ = run_stream(F, reqs, None, budget, max_active)
+    capacity = len(reqs) / cap_span
+    rate = 0.8 * capacity
+    rng = np.random.default_rng(seed + 1)
+    pair_t = np.cumsum(rng.exponential(2.0 / rate, len(reqs) // 2))  # pairs arrive together
+    arrivals = [pair_t[r[0]] for r in reqs]
+    log(f"  capacity {capacity:.1f} solves/s ({total_dof / cap_span / 1e6:.1f} MDOF/s); Poisson arrivals at "
+        f"{rate:.1f} solves/s")
+    span, lat, its = run_stream(F, reqs, arrivals, budget, max_active)
+    out = {
+        "workload": (f"{len(reqs)} solves in refinement pairs (h, h/2), heat tet4 or elasticity tet10 (50/50), "
+                     f"{min(r[4] for r in reqs)}-{max(r[4] for r in reqs)} DOF, Poisson pair arrivals at 80% of the "
+                     f"measured capacity (seed {seed}), mixed precision, tol 1e-8, HBM budget {budget >> 30} GiB, "
+                     f"{max_active} solves admitted at once"),
+        "solves": len(reqs), "total_dof": total_dof,
+        "capacity_solves_s": round(capacity, 2), "capacity_mdof_s": round(total_dof / cap_span / 1e6, 2),
+        "arrival_rate_solves_s": round(rate, 2),
+        "mdof_s": round(total_dof / span / 1e6, 2), "makespan_s": round(span, 3),
+        "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 2), "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 2),
+        "mean_ms": round(1e3 * float(np.mean(lat)), 2), "max_iterations": int(max(its)),
+        "latency": "submit to solution on the host, including setup (model, hierarchy, coarse factor)",
+    }
+    log(f"  stream: {out['mdof_s']} MDOF/s, latency p50 {out['p50_ms']} ms, p99 {out['p99_ms']} ms")
+    return out
+
+
 def measured_traffic(n):
     """HBM bytes per launch of the dominant kernel from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
     kept under profiles/ (measured at this workload only), or None."""
@@ -246,6 +333,7 @@ def main():
     ap.add_argument("--coarse", choices=["galerkin", "mean"], default="galerkin")
     ap.add_argument("--elastic-n", type=int, default=72,
                     help="cells along x of the tet10 elasticity box reported beside the headline (0: skip)")
+    ap.add_argument("--serve-n", type=int, default=256, help="solves of the serving stream (0: skip)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run one plain 256 MiB device copy first (the yardstick of HBM byte counters)")
     args = ap.parse_args()
@@ -306,6 +394,8 @@ def main():
                                           "us_per_launch": round(eprof["smoother_s"] * 1e6, 2),
                                           "achieved_gbs": round(eprof["smoother_bytes"] / eprof["smoother_s"] / 1e9, 1)}
 
+    serve = serving(F, args.serve_n, 4321) if args.serve_n > 0 else None
+
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline: oracle on heat hex8 {args.cpu_n}^3, 1 thread")
@@ -318,6 +408,8 @@ def main():
     config.update(heat)
     if elastic:
         config["elasticity_tet10"] = elastic
+    if serve:
+        config["serving"] = serve
     out = {
         "metric": METRIC,
         "value": value,

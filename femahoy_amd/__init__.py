@@ -93,7 +93,8 @@ class _Stats(C.Structure):
 CANCEL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)
 
 class _ServerOptions(C.Structure):
-    _fields_ = [("hbm_budget", C.c_uint64), ("max_active", C.c_int32), ("device", C.c_int32)]
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_active", C.c_int32), ("device", C.c_int32),
+                ("setup_threads", C.c_int32)]
 
 
 class RequestInfo(C.Structure):
@@ -428,8 +429,8 @@ class Server:
     """The serving loop (fem_server_*): solves of uneven sizes admitted by device footprint, set up
     on a thread of their own and advanced together, one PCG iteration per solve at a time."""
 
-    def __init__(self, hbm_budget: int, max_active: int = 8, device: int = 0):
-        o = _ServerOptions(int(hbm_budget), int(max_active), int(device))
+    def __init__(self, hbm_budget: int, max_active: int = 8, device: int = 0, setup_threads: int = 2):
+        o = _ServerOptions(int(hbm_budget), int(max_active), int(device), int(setup_threads))
         h = C.c_void_p()
         err = _err()
         if lib().fem_server_new(C.byref(o), C.byref(h), err, 2048):

@@ -1319,7 +1319,7 @@ class Solver {
 //  - submit() queues a solve with its model, options and right side (copied).
 //  - Admission in arrival order by device footprint: a solve is admitted while the admitted ones
 //    hold less than the budget (estimated bytes) and fewer than max_active are on the device.
-//  - A setup thread builds each admitted solve (model, hierarchy, factors: host work and device
+//  - Setup threads build each admitted solve (model, hierarchy, factors: host work and device
 //    allocation on the solve's own stream) and starts its run; the loop thread then advances every
 //    running solve without waiting: each advance launches one PCG iteration (a replayed hipGraph)
 //    on that solve's stream and returns, so the iterations of all running solves are in flight on
@@ -1344,10 +1344,10 @@ struct ServeRequest {
 
 class Server {
    public:
-    Server(uint64_t budget, int max_active, int device)
+    Server(uint64_t budget, int max_active, int device, int setup_threads)
         : budget_(budget), max_active_(max_active > 0 ? max_active : 1), device_(device), t0_(clock::now()) {
         loop_ = std::thread([this] { loop(); });
-        setup_ = std::thread([this] { setup_loop(); });
+        for (int k = 0; k < std::max(1, setup_threads); ++k) setup_.emplace_back([this] { setup_loop(); });
     }
     ~Server() {
         {
@@ -1357,7 +1357,7 @@ class Server {
         cv_.notify_all();
         setup_cv_.notify_all();
         loop_.join();
-        setup_.join();
+        for (auto& t : setup_) t.join();
     }
 
     uint64_t submit(const BoxInput& in, const Options& o, const double* b) {
@@ -1592,7 +1592,8 @@ class Server {
     std::deque<std::shared_ptr<ServeRequest>> queue_, to_setup_;
     std::vector<std::shared_ptr<ServeRequest>> ready_;
     std::map<uint64_t, std::shared_ptr<ServeRequest>> all_;
-    std::thread loop_, setup_;
+    std::thread loop_;
+    std::vector<std::thread> setup_;
 };
 }  // namespace fem
 
@@ -1803,7 +1804,7 @@ int fem_server_new(const fem_server_options* opt, fem_server** out, char* err, s
     return guarded(err, errlen, [&] {
         if (!opt || !out) throw fem::Error("fem_server_new needs options and an output handle");
         auto* srv = new fem_server;
-        srv->impl = std::make_unique<fem::Server>(opt->hbm_budget, opt->max_active, opt->device);
+        srv->impl = std::make_unique<fem::Server>(opt->hbm_budget, opt->max_active, opt->device, opt->setup_threads);
         *out = srv;
     });
 }

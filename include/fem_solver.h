@@ -190,8 +190,8 @@ int fem_solver_time_kernels(fem_solver* s, int reps, fem_profile* out);
 /* ---- the serving loop (no counterpart in the reference) ----
  * A persistent request loop over solves of uneven sizes: submit queues a solve (model, options,
  * right side), which is admitted in arrival order while the admitted solves' estimated device
- * footprint stays within hbm_budget and fewer than max_active are on the device. A setup thread
- * builds each admitted solve (model, hierarchy, coarse factor) on its own stream, and the loop
+ * footprint stays within hbm_budget and fewer than max_active are on the device. Setup threads
+ * build each admitted solve (model, hierarchy, coarse factor) on its own stream, and the loop
  * thread advances every running solve by one PCG iteration at a time without waiting (a replayed
  * hipGraph per iteration on the solve's own stream), so the work of all running solves overlaps
  * on the device; a finished solve frees its slot and memory for the next. Each solve runs the
@@ -205,6 +205,7 @@ typedef struct fem_server_options {
     uint64_t hbm_budget; /* device bytes the admitted solves may hold together (estimated footprints) */
     int32_t max_active;  /* solves admitted at once */
     int32_t device;      /* HIP device ordinal of every solve */
+    int32_t setup_threads; /* host threads building admitted solves (model, hierarchy, factor); 0: 1 */
 } fem_server_options;
 
 typedef struct fem_request_info {
