@@ -356,11 +356,8 @@ struct Tet4Elastic {
 // edge points at v_a + v_b, with the chain vertices v_0..v_3 (in cell bits). Local nodes 0..3
 // are the corners, 4..9 the edges (0,1), (1,2), (0,2), (0,3), (1,3), (2,3) (tet10.rs:38).
 //
-// The operator runs in 2 phases. Phase A (k_tet10_elems, tet10_kuhn.hpp), one thread per
-// tetrahedron: the element product of tet10.rs:90-120 (the 4-point rule of degree 2, exact for
-// affine cells) written to an element buffer E, 30 values per tetrahedron. Phase B (Tet10Op
-// below), one thread per node: the sum of the entries of E at that node, over the tetrahedra that
-// hold it, in a fixed order. No atomics, so the sums are bitwise reproducible.
+// The operator itself runs in the fused kernels of tet10_fused.hpp; Tet10Op below gives the
+// diagonal of its rows (setup).
 __constant__ static const int TET10_EDGE[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
 
 // Chain vertex v (0..3) of Kuhn type t, as cell bits.
@@ -385,49 +382,14 @@ __host__ __device__ inline int tet10_local(int t, int px, int py, int pz) {
     return -1;
 }
 
-// Phase B of the tet10 operator, node-centric: the sum of the element buffer at a node.
-// coef: lambda at [t * cells + cell], mu at [(6 + t) * cells + cell] (cells of the coarse grid);
-// E: [(t * 30 + 3 * local + c) * cells + cell].
+// The diagonal of the tet10 operator's rows, from the element rule of tet10.rs:90-120 (setup).
+// coef: lambda at [t * cells + cell], mu at [(6 + t) * cells + cell] (cells of the coarse grid).
 template <class T>
 struct Tet10Op {
     GridDims g;  // refined grid: 2 n + 1 nodes per axis
     const T* coef;
-    const T* E;
     T ih[3];
     T vol;
-
-    __device__ __forceinline__ void apply3(const T* /*x, read by phase A*/, int i, int j, int k, T out[3]) const {
-        const int cx = (g.nx - 1) / 2, cy = (g.ny - 1) / 2, cz = (g.nz - 1) / 2;
-        const size_t cells = (size_t)cx * cy * cz;
-        out[0] = out[1] = out[2] = T(0);
-        const int pos[3] = {i, j, k};
-        const int ncell[3] = {cx, cy, cz};
-        int lo[3], hi[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            if (pos[d] & 1) {
-                lo[d] = hi[d] = (pos[d] - 1) / 2;
-            } else {
-                lo[d] = pos[d] / 2 - 1;
-                hi[d] = pos[d] / 2;
-                if (lo[d] < 0) lo[d] = 0;
-                if (hi[d] > ncell[d] - 1) hi[d] = ncell[d] - 1;
-            }
-        }
-        for (int ez = lo[2]; ez <= hi[2]; ++ez)
-            for (int ey = lo[1]; ey <= hi[1]; ++ey)
-                for (int ex = lo[0]; ex <= hi[0]; ++ex) {
-                    const size_t cell = ex + (size_t)cx * (ey + (size_t)cy * ez);
-                    const int px = i - 2 * ex, py = j - 2 * ey, pz = k - 2 * ez;
-#pragma unroll
-                    for (int t = 0; t < 6; ++t) {
-                        const int loc = tet10_local(t, px, py, pz);
-                        if (loc < 0) continue;
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) out[c] += E[((size_t)(t * 30 + 3 * loc + c)) * cells + cell];
-                    }
-                }
-    }
 
     // The diagonal, recomputed from the element rule (setup only).
     __device__ void diag3(int i, int j, int k, T out[3]) const {
@@ -435,11 +397,12 @@ struct Tet10Op {
         const size_t cells = (size_t)cx * cy * cz;
         out[0] = out[1] = out[2] = T(0);
         const T qa = T(0.5854101966249685), qb = T(0.1381966011250105);
-        for (int ez = 0; ez < cz; ++ez) {
+        // the cells whose refined positions 0..2 hold the node along each axis
+        for (int ez = max(0, (k - 1) / 2); ez <= min(cz - 1, k / 2); ++ez) {
             if (k < 2 * ez || k > 2 * ez + 2) continue;
-            for (int ey = 0; ey < cy; ++ey) {
+            for (int ey = max(0, (j - 1) / 2); ey <= min(cy - 1, j / 2); ++ey) {
                 if (j < 2 * ey || j > 2 * ey + 2) continue;
-                for (int ex = 0; ex < cx; ++ex) {
+                for (int ex = max(0, (i - 1) / 2); ex <= min(cx - 1, i / 2); ++ex) {
                     if (i < 2 * ex || i > 2 * ex + 2) continue;
                     const size_t cell = ex + (size_t)cx * (ey + (size_t)cy * ez);
                     for (int t = 0; t < 6; ++t) {

@@ -25,6 +25,7 @@
 #include "kernels.hpp"
 #include "model.hpp"
 #include "march.hpp"
+#include "tet10_fused.hpp"
 #include "tet10_kuhn.hpp"
 #include "setup_kernels.hpp"
 
@@ -65,8 +66,13 @@ struct Level {
     int cells[3] = {0, 0, 0};  // cells of the level (tet10: the cells of its grid, nodes 2 n + 1)
     double h[3] = {0, 0, 0};
     bool kuhn = false;         // transfer to the next level is Kuhn P1 (else trilinear)
-    DArray<T> coef, E;
+    DArray<T> coef;
     DArray<T> invd, b, xa, xb, d, r;
+    // tet10: the partial sums of the fused kernels (tet10_fused.hpp, 8 slots per unknown) and the
+    // nodes their boundary launch finishes, for the tile of T
+    DArray<T> P;
+    DArray<int> blist;
+    int nbound = 0;
     int cur = 0;
     double lmax = 0.0;
     std::vector<T> c1, c2;     // Chebyshev step coefficients
@@ -117,8 +123,7 @@ void with_op(Level<T>& L, F&& f) {
             f(VectorOp<Tet4Elastic<T>>{Tet4Elastic<T>{L.g, L.coef.data(), {ih[0], ih[1], ih[2]}, vol}});
             break;
         case OpKind::Tet10:
-            f(VectorOp<Tet10Op<T>>{Tet10Op<T>{L.g, L.coef.data(), L.E.data(), {ih[0], ih[1], ih[2]}, vol}});
-            break;
+            throw Error("internal: the tet10 operator runs through its fused kernels");
     }
 }
 
@@ -137,29 +142,96 @@ void launch_march(const MarchArgs<T>& m, hipStream_t s) {
     hipLaunchKernelGGL((k_march_hex8<T, MODE>), march_grid<T>(m.g), dim3(BLOCK), 0, s, m);
 }
 
-// The arguments of the fused tet10 kernels of a level.
+// The arguments of the fused tet10 kernels of a level (tet10_fused.hpp).
 template <class T>
-Tet10Args<T> tet10_args(const Level<T>& L) {
-    Tet10Args<T> a{};
+T10Args<T> t10_args(const Level<T>& L) {
+    using TL = T10Tile<T>;
+    T10Args<T> a{};
     a.g = L.g;
     a.cx = L.cells[0];
     a.cy = L.cells[1];
     a.cz = L.cells[2];
+    a.tx = (a.cx + TL::X - 1) / TL::X;
+    a.ty = (a.cy + TL::Y - 1) / TL::Y;
+    a.tz = (a.cz + TL::Z - 1) / TL::Z;
     a.coef = L.coef.data();
     a.ih0 = T(1.0 / L.h[0]);
     a.ih1 = T(1.0 / L.h[1]);
     a.ih2 = T(1.0 / L.h[2]);
     a.w = T(L.h[0] * L.h[1] * L.h[2] / 6.0 / 4.0);
+    a.P = L.P.data();
+    a.blist = L.blist.data();
+    a.nb = L.nbound;
     return a;
 }
 
-// Phase A of an operator whose apply reads an element buffer (tet10); nothing for the others.
+// The partial sums of a tet10 level's fused kernels and the nodes that their boundary launch
+// finishes, for the tile of T.
 template <class T>
-void prepare(Level<T>& L, const T* x, hipStream_t s) {
-    if (L.op != OpKind::Tet10) return;
-    Tet10Args<T> a = tet10_args(L);
-    a.x = x;
-    hipLaunchKernelGGL((k_tet10_elems<T>), tet10_elems_grid(a.cx, a.cy, a.cz), dim3(256), 0, s, a, L.E.data());
+void t10_setup(Level<T>& L, hipStream_t s) {
+    using TL = T10Tile<T>;
+    if (L.dofs() * sizeof(T) >= MARCH_MAX_BYTES || 12 * L.ncell() * sizeof(T) >= MARCH_MAX_BYTES)
+        throw Error("the tet10 grid is too large for the fused operator kernels (byte offsets of 31 bits)");
+    const int tn[3] = {TL::X, TL::Y, TL::Z};
+    std::vector<int> list;
+    t10_boundary(L.cells, tn, list);
+    L.nbound = (int)list.size();
+    L.blist.resize(list.size());
+    if (!list.empty()) L.blist.upload(list.data(), list.size(), s);
+    L.P.resize(8 * L.dofs());
+    FEM_HIP(hipStreamSynchronize(s));  // the host list ends with this scope
+}
+
+// The two launches of a fused tet10 mode; returns the number of block partials (PCGQ).
+template <int MODE, class T>
+unsigned t10_run(T10Args<T> a, hipStream_t s) {
+    using TL = T10Tile<T>;
+    const unsigned n1 = (unsigned)a.tx * a.ty * a.tz;
+    const unsigned n2 = (unsigned)((a.nb + BLOCK - 1) / BLOCK);
+    a.nb1 = (int)n1;
+    hipLaunchKernelGGL((k_t10_tile<T, TL::X, TL::Y, TL::Z, MODE>), dim3(n1), dim3(TL::X * TL::Y * TL::Z), 0, s, a);
+    if (n2) hipLaunchKernelGGL((k_t10_bound<T, TL::X, TL::Y, TL::Z, MODE>), dim3(n2), dim3(BLOCK), 0, s, a);
+    return n1 + n2;
+}
+
+// y = A x on a level (0 on fixed nodes).
+template <class T>
+void op_apply(Level<T>& L, const T* x, T* y, hipStream_t s) {
+    if (L.op == OpKind::Tet10) {
+        T10Args<T> a = t10_args(L);
+        a.x = x;
+        a.y = y;
+        t10_run<T10_APPLY>(a, s);
+        return;
+    }
+    with_op(L, [&](auto op) { launch((k_apply<decltype(op), T>), L.nodes(), s, op, x, y); });
+}
+
+// r = b - A x on a level (0 on fixed nodes).
+template <class T>
+void op_residual(Level<T>& L, const T* x, const T* b, T* r, hipStream_t s) {
+    if (L.op == OpKind::Tet10) {
+        T10Args<T> a = t10_args(L);
+        a.x = x;
+        a.b = b;
+        a.y = r;
+        t10_run<T10_RESID>(a, s);
+        return;
+    }
+    with_op(L, [&](auto op) { launch((k_residual<decltype(op), T>), L.nodes(), s, op, x, b, r); });
+}
+
+// invd = 1 / diag(A) on a level (0 on fixed nodes).
+template <class T>
+void op_inv_diag(Level<T>& L, hipStream_t s) {
+    if (L.op == OpKind::Tet10) {
+        const T ih[3] = {T(1.0 / L.h[0]), T(1.0 / L.h[1]), T(1.0 / L.h[2])};
+        const T vol = T(L.h[0] * L.h[1] * L.h[2] / 6.0);
+        const VectorOp<Tet10Op<T>> op{Tet10Op<T>{L.g, L.coef.data(), {ih[0], ih[1], ih[2]}, vol}};
+        launch((k_inv_diag<VectorOp<Tet10Op<T>>, T>), L.nodes(), s, op, L.invd.data());
+        return;
+    }
+    with_op(L, [&](auto op) { launch((k_inv_diag<decltype(op), T>), L.nodes(), s, op, L.invd.data()); });
 }
 
 // Conversion of the FP64 coefficients of a level to T.
@@ -175,7 +247,7 @@ void convert_level(const Level<double>& src, Level<T>& dst, hipStream_t s) {
     dst.kuhn = src.kuhn;
     dst.coef.resize(src.coef.size());
     launch((k_convert<double, T>), src.coef.size(), s, src.coef.size(), src.coef.data(), dst.coef.data());
-    if (src.op == OpKind::Tet10) dst.E.resize(30 * 6 * src.ncell());
+    if (src.op == OpKind::Tet10) t10_setup(dst, s);
 }
 
 // Device time of chosen launches by HIP events on the launching stream: begin/end pairs, not
@@ -274,10 +346,7 @@ class Hierarchy {
             m.y = L.r.data();
             launch_march<T, MARCH_RESIDUAL>(m, s);
         } else {
-            prepare(L, L.x(), s);
-            with_op(L, [&](auto op) {
-                launch((k_residual<decltype(op), T>), L.nodes(), s, op, (const T*)L.x(), (const T*)L.b.data(), L.r.data());
-            });
+            op_residual(L, (const T*)L.x(), (const T*)L.b.data(), L.r.data(), s);
         }
         Level<T>& Cc = lv[l + 1];
         transfer(L, Cc, true, s);
@@ -305,7 +374,18 @@ class Hierarchy {
             launch_march<T, MARCH_CHEB>(m, s);
             return;
         }
-        prepare(L, L.x(), s);
+        if (L.op == OpKind::Tet10) {
+            T10Args<T> a = t10_args(L);
+            a.x = L.x();
+            a.y = L.x2();
+            a.d = L.d.data();
+            a.b = L.b.data();
+            a.invd = L.invd.data();
+            a.c1 = c1;
+            a.c2 = c2;
+            t10_run<T10_CHEB>(a, s);
+            return;
+        }
         with_op(L, [&](auto op) {
             launch((k_cheb<decltype(op), T>), L.nodes(), s, op, (const T*)L.x(), L.x2(), L.d.data(), (const T*)L.b.data(),
                    (const T*)L.invd.data(), c1, c2);
@@ -617,10 +697,18 @@ class Solver {
         }
     }
 
+    // The device bytes of the fused tet10 kernels' partial sums and node list on a level of c
+    // cells (at most every node in the list).
+    template <class T>
+    static uint64_t t10_record_bytes(const int c[3]) {
+        const uint64_t nodes = (2 * (uint64_t)c[0] + 1) * (2 * (uint64_t)c[1] + 1) * (2 * (uint64_t)c[2] + 1);
+        return nodes * 24 * sizeof(T) + nodes * 4;
+    }
+
     // An upper estimate of the device bytes a solver of this model holds (admission by footprint in
     // the serving loop): the level-0 FP64 vectors and coefficients, every level of the V-cycle in its
-    // precision, the element buffers of tet10, the dense coarse inverse or direct factor with its
-    // workspace, and a quarter more for partials and allocation granularity.
+    // precision, the boundary records of the fused tet10 kernels, the dense coarse inverse or direct
+    // factor with its workspace, and a quarter more for partials and allocation granularity.
     static uint64_t estimate_bytes(const BoxModel& m, const Options& o) {
         const uint64_t nfree = m.free.size();
         const uint64_t cells = (uint64_t)m.in.n[0] * m.in.n[1] * m.in.n[2];
@@ -642,7 +730,7 @@ class Solver {
         };
         const bool direct = o.method == FEM_METHOD_DIRECT || (o.method == FEM_METHOD_AUTO && nfree <= o.crossover);
         bytes += coef64(m.in.kind, cells);
-        if (m.in.kind == Kind::ElasticTet10) bytes += 180 * cells * 8;
+        if (m.in.kind == Kind::ElasticTet10) bytes += t10_record_bytes<double>(m.in.n);
         if (direct) return (bytes + dense(nfree)) * 5 / 4;
         // V-cycle levels: level 0, then (tet10) tet4 on the same cells, then halvings
         int c[3] = {m.in.n[0], m.in.n[1], m.in.n[2]};
@@ -659,7 +747,8 @@ class Solver {
                 bytes += cf + cf * vb / 8;
             } else {
                 bytes += coef64(kind, lc) * vb / 8;
-                if (kind == Kind::ElasticTet10) bytes += 180 * lc * vb;
+                if (kind == Kind::ElasticTet10)
+                    bytes += vb == 8 ? t10_record_bytes<double>(c) : t10_record_bytes<float>(c);
             }
             ++lev;
             if (o.levels > 0 && lev >= o.levels) break;
@@ -792,7 +881,7 @@ class Solver {
             case Kind::ElasticTet10:
                 L0.op = OpKind::Tet10;
                 L0.kuhn = true;
-                L0.E.resize(30 * 6 * L0.ncell());
+                t10_setup(L0, s_);
                 break;
         }
         if (m.in.kind != Kind::HeatHex8) {
@@ -977,10 +1066,7 @@ class Solver {
                 for (int c2 = 0; c2 < S; ++c2)
                     for (int comp = 0; comp < C; ++comp) {
                         launch(k_probe_fill, L.dofs(), s_, L.g, C, S, c0, c1, c2, comp, (const int64_t*)dof.data(), px.data());
-                        prepare(L, px.data(), s_);
-                        with_op(L, [&](auto op) {
-                            launch((k_apply<decltype(op), double>), L.nodes(), s_, op, (const double*)px.data(), py.data());
-                        });
+                        op_apply(L, (const double*)px.data(), py.data(), s_);
                         launch(k_probe_extract, L.dofs(), s_, L.g, C, S, c0, c1, c2, comp, (const int64_t*)dof.data(),
                                (const double*)py.data(), A.data(), n);
                     }
@@ -1016,7 +1102,7 @@ class Solver {
             L.r.resize(n);
             for (T* v : {L.b.data(), L.xa.data(), L.xb.data(), L.d.data(), L.r.data()}) launch(k_fill<T>, n, s_, n, v, T(0));
             if (l + 1 < lv64_.size()) {
-                with_op(L, [&](auto op) { launch((k_inv_diag<decltype(op), T>), L.nodes(), s_, op, L.invd.data()); });
+                op_inv_diag(L, s_);
                 L.lmax = estimate_lmax(L);
                 chebyshev(L);
             }
@@ -1036,8 +1122,7 @@ class Solver {
             launch_blocks(k_dot<T>, pb, s_, n, (const T*)v, (const T*)v, partA_.data());
             launch_reduce(k_sum, s_, (const double*)partA_.data(), (int)pb, red_.data());
             launch(k_normalise<T>, n, s_, n, v, (const double*)red_.data());
-            prepare(L, v, s_);
-            with_op(L, [&](auto op) { launch((k_apply<decltype(op), T>), L.nodes(), s_, op, (const T*)v, w); });
+            op_apply(L, (const T*)v, w, s_);
             launch(k_scale<T>, n, s_, n, w, (const T*)L.invd.data());
             launch_blocks(k_dot<T>, pb, s_, n, (const T*)w, (const T*)w, partA_.data());
             launch_reduce(k_sum, s_, (const double*)partA_.data(), (int)pb, red_.data() + 1);
@@ -1072,8 +1157,7 @@ class Solver {
     // ---------------------------------------------------------------- FP64 outer iteration
     void apply64(const double* in, double* out) {
         Level<double>& L = lv64_[0];
-        prepare(L, in, s_);
-        with_op(L, [&](auto op) { launch((k_apply<decltype(op), double>), L.nodes(), s_, op, in, out); });
+        op_apply(L, in, out, s_);
     }
 
     // z = M^-1 r (device, grid layout).
@@ -1098,15 +1182,28 @@ class Solver {
         return pinned_[0];
     }
 
-    double true_residual() {
-        // |b - A x| / |b| in FP64 over the free unknowns (fixed rows are 0 in both)
+    // r = b - A x in FP64 on level 0 with the block partials of b.b (partA_) and r.r (partB_);
+    // returns their count.
+    unsigned residual64(const double* x, double* r) {
         Level<double>& L = lv64_[0];
-        prepare(L, x_.data(), s_);
+        if (L.op == OpKind::Tet10) {
+            op_residual(L, x, (const double*)b_.data(), r, s_);
+            const unsigned pb = std::min<unsigned>(nblocks(n0_), 1024);
+            launch_blocks(k_dot<double>, pb, s_, n0_, (const double*)b_.data(), (const double*)b_.data(), partA_.data());
+            launch_blocks(k_dot<double>, pb, s_, n0_, (const double*)r, (const double*)r, partB_.data());
+            return pb;
+        }
         const unsigned nb = nblocks(L.nodes());
         with_op(L, [&](auto op) {
-            launch_blocks((k_pcg_start<decltype(op)>), nb, s_, op, (const double*)x_.data(),
-                               (const double*)b_.data(), tmp_.data(), partA_.data(), partB_.data());
+            launch_blocks((k_pcg_start<decltype(op)>), nb, s_, op, x, (const double*)b_.data(), r, partA_.data(),
+                          partB_.data());
         });
+        return nb;
+    }
+
+    double true_residual() {
+        // |b - A x| / |b| in FP64 over the free unknowns (fixed rows are 0 in both)
+        const unsigned nb = residual64(x_.data(), tmp_.data());
         launch_reduce(k_sum, s_, (const double*)partA_.data(), (int)nb, red_.data());
         launch_reduce(k_sum, s_, (const double*)partB_.data(), (int)nb, red_.data() + 1);
         double v[2];
@@ -1142,7 +1239,13 @@ class Solver {
             const dim3 gm = march_grid<double>(L0.g);
             return gm.x * gm.y;
         }
-        prepare(L0, p_.data(), s_);
+        if (L0.op == OpKind::Tet10) {
+            T10Args<double> a = t10_args(L0);
+            a.x = p_.data();
+            a.y = q_.data();
+            a.part = partA_.data();
+            return t10_run<T10_PCGQ>(a, s_);
+        }
         with_op(L0, [&](auto op) {
             launch_blocks((k_pcg_q<decltype(op)>), nblocks(L0.nodes()), s_, op, (const double*)p_.data(), q_.data(),
                           partA_.data());
@@ -1241,13 +1344,7 @@ class Solver {
 
     // multigrid.rs:692-753 as steps: r = b - A x and the scalars, then one readback per iteration.
     void launch_start() {
-        Level<double>& L0 = lv64_[0];
-        const unsigned nbn = nblocks(L0.nodes());
-        prepare(L0, x_.data(), s_);
-        with_op(L0, [&](auto op) {
-            launch_blocks((k_pcg_start<decltype(op)>), nbn, s_, op, (const double*)x_.data(), (const double*)b_.data(),
-                          r_.data(), partA_.data(), partB_.data());
-        });
+        const unsigned nbn = residual64(x_.data(), r_.data());
         launch_reduce(k_pcg_scalars, s_, scal_.data(), (const double*)partA_.data(), (const double*)partB_.data(),
                       (int)nbn, 0);
         // rr and bb are adjacent in PcgScalars

@@ -1,8 +1,7 @@
-// The element pass of the tet10 elasticity operator on Kuhn cells (tet10.rs:90-120): one thread
-// per (cell, Kuhn type), the type fixed per block so that the element product runs with the
-// type's gradient pattern written out at compile time. The 30 element forces go to the element
-// buffer E[(t * 30 + 3 local + c) * cells + cell], which the node-centric gather of Tet10Op (ops.hpp)
-// sums, in a fixed order, into the operator and its fused updates.
+// The tet10 elasticity element product on Kuhn cells (tet10.rs:90-120), with the Kuhn type fixed
+// at compile time so that the element product runs with the type's gradient pattern written out,
+// and the refined-grid offsets of the 10 nodes of each type. The fused operator kernels
+// (tet10_fused.hpp) call it once per tetrahedron.
 //
 #pragma once
 
@@ -138,62 +137,6 @@ __host__ __device__ __forceinline__ void tet10_element_kuhn(const T u[30], const
             y[3 * (4 + e) + a] = T(4) * acc;
         }
     }
-}
-
-template <class T>
-struct Tet10Args {
-    GridDims g;         // the refined grid (2 c + 1 nodes per axis) and its fixed faces
-    int cx, cy, cz;     // cells
-    const T* coef;      // lambda at [t * cells + cell], mu at [(6 + t) * cells + cell]
-    T ih0, ih1, ih2;    // inverse cell sizes
-    T w;                // quadrature weight: cell volume / 6 / 4
-    const T* x;         // input (p for PCG_Q)
-    T* y;               // output (y, r, x_out or q)
-    const T* b;
-    T* d;
-    const T* invd;
-    T c1, c2;
-    double* part;       // block partials of p.q (PCG_Q)
-};
-
-// Pass 1: the element forces of every (cell, type); block (x, type) holds 256 cells of one type.
-template <class T, int TT>
-__device__ __forceinline__ void t10_element_at(const Tet10Args<T>& a, T* __restrict__ E, size_t cell, size_t cells) {
-    const int ex = (int)(cell % a.cx), ey = (int)((cell / a.cx) % a.cy), ez = (int)(cell / ((size_t)a.cx * a.cy));
-    const int NX = a.g.nx, NY = a.g.ny;
-    T u[30];
-#pragma unroll
-    for (int n = 0; n < 10; ++n) {
-        const int I = 2 * ex + t10_offset(TT, n, 0), J = 2 * ey + t10_offset(TT, n, 1), K = 2 * ez + t10_offset(TT, n, 2);
-        const size_t node = I + (size_t)NX * (J + (size_t)NY * K);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) u[3 * n + c] = a.x[3 * node + c];
-    }
-    const T ih[3] = {a.ih0, a.ih1, a.ih2};
-    T y[30];
-    tet10_element_kuhn<T, TT>(u, ih, a.coef[TT * cells + cell], a.coef[(6 + TT) * cells + cell], a.w, y);
-#pragma unroll
-    for (int s = 0; s < 30; ++s) E[(size_t)(TT * 30 + s) * cells + cell] = y[s];
-}
-
-template <class T>
-__global__ void __launch_bounds__(256) k_tet10_elems(Tet10Args<T> a, T* __restrict__ E) {
-    const size_t cells = (size_t)a.cx * a.cy * a.cz;
-    const size_t cell = blockIdx.x * (size_t)256 + threadIdx.x;
-    if (cell >= cells) return;
-    switch (blockIdx.y) {
-        case 0: t10_element_at<T, 0>(a, E, cell, cells); break;
-        case 1: t10_element_at<T, 1>(a, E, cell, cells); break;
-        case 2: t10_element_at<T, 2>(a, E, cell, cells); break;
-        case 3: t10_element_at<T, 3>(a, E, cell, cells); break;
-        case 4: t10_element_at<T, 4>(a, E, cell, cells); break;
-        default: t10_element_at<T, 5>(a, E, cell, cells); break;
-    }
-}
-
-// The launch shape of the element pass: cells x 6 types.
-inline dim3 tet10_elems_grid(int cx, int cy, int cz) {
-    return dim3((unsigned)(((size_t)cx * cy * cz + 255) / 256), 6);
 }
 
 }  // namespace fem

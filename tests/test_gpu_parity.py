@@ -50,7 +50,9 @@ def test_heat_hex8_apply_matches_oracle_matrix(fem, oracle):
     assert rel(s.apply(x), a.mul(x)) < 1e-13
 
 
-@pytest.mark.parametrize("kind,n", [("heat_tet4", [8, 8, 8]), ("elastic_tet4", [8, 4, 4]), ("elastic_tet10", [4, 4, 4])])
+# tet10 [10, 9, 6]: several tiles of the fused operator kernels along each axis, the last ones partial.
+@pytest.mark.parametrize("kind,n", [("heat_tet4", [8, 8, 8]), ("elastic_tet4", [8, 4, 4]), ("elastic_tet10", [4, 4, 4]),
+                                    ("elastic_tet10", [10, 9, 6])])
 def test_tets_apply_matches_oracle_matrix(fem, oracle, kind, n):
     rng = np.random.default_rng(11)
     nt = 6 * int(np.prod(n))
@@ -74,7 +76,8 @@ def test_tets_apply_matches_oracle_matrix(fem, oracle, kind, n):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
-@pytest.mark.parametrize("kind,n", [("heat_tet4", [8, 8, 8]), ("elastic_tet4", [8, 4, 4]), ("elastic_tet10", [8, 4, 4])])
+@pytest.mark.parametrize("kind,n", [("heat_tet4", [8, 8, 8]), ("elastic_tet4", [8, 4, 4]), ("elastic_tet10", [8, 4, 4]),
+                                    ("elastic_tet10", [16, 12, 4])])
 def test_tets_multigrid_matches_oracle_direct(fem, oracle, kind, n, precision):
     rng = np.random.default_rng(2)
     nt = 6 * int(np.prod(n))
@@ -123,10 +126,34 @@ def test_nonconvergence_and_cancel_messages(fem):
         s.solve(s.rhs(), cancelled=lambda: True)
 
 
-def test_solve_is_bitwise_reproducible(fem):
-    n = [16, 16, 16]
-    k, fixed = heat_case(n, seed=9)
-    s = fem.Solver(fem.Box("heat_hex8", n, [1, 1, 1], k, fixed=fixed), method="multigrid")
+def test_fused_tet10_v_cycle_fp32_agrees_with_fp64(fem):
+    """The FP32 V-cycle (fused tet10 kernels on 8 x 8 x 4 tiles) against the FP64 one (8 x 4 x 4
+    tiles) on a box of several partial tiles: the same preconditioner up to FP32 rounding."""
+    n = [18, 10, 6]
+    rng = np.random.default_rng(21)
+    nt = 6 * int(np.prod(n))
+    box = fem.Box("elastic_tet10", n, [1.8, 1.0, 0.6], rng.uniform(1.0, 2.0, nt), rng.uniform(0.2, 0.4, nt),
+                  fixed={"x0": True}, body=(0, 0, -1))
+    s32 = fem.Solver(box, method="multigrid", precision="fp32", levels=3)
+    s64 = fem.Solver(box, method="multigrid", precision="fp64", levels=3)
+    r = rng.standard_normal(s32.n)
+    z32, z64 = s32.precondition(r), s64.precondition(r)
+    assert rel(z32, z64) < 1e-5
+    assert rel(s32.apply(r), s64.apply(r)) == 0.0  # both apply the FP64 level-0 operator
+
+
+@pytest.mark.parametrize("kind", ["heat_hex8", "elastic_tet10"])
+def test_solve_is_bitwise_reproducible(fem, kind):
+    if kind == "heat_hex8":
+        n = [16, 16, 16]
+        k, fixed = heat_case(n, seed=9)
+        box = fem.Box("heat_hex8", n, [1, 1, 1], k, fixed=fixed)
+    else:
+        n = [16, 8, 8]
+        nt = 6 * int(np.prod(n))
+        box = fem.Box(kind, n, [2.0, 1.0, 1.0], np.random.default_rng(9).uniform(1.0, 2.0, nt), np.full(nt, 0.3),
+                      fixed={"x0": True}, body=(0, 0, -1))
+    s = fem.Solver(box, method="multigrid")
     b = s.rhs()
     x1, s1 = s.solve(b)
     x2, s2 = s.solve(b)
