@@ -170,7 +170,7 @@ T10Args<T> t10_args(const Level<T>& L) {
 template <class T>
 void t10_setup(Level<T>& L, hipStream_t s) {
     using TL = T10Tile<T>;
-    if (L.dofs() * sizeof(T) >= MARCH_MAX_BYTES || 12 * L.ncell() * sizeof(T) >= MARCH_MAX_BYTES)
+    if (8 * L.dofs() * sizeof(T) >= MARCH_MAX_BYTES || 12 * L.ncell() * sizeof(T) >= MARCH_MAX_BYTES)
         throw Error("the tet10 grid is too large for the fused operator kernels (byte offsets of 31 bits)");
     const int tn[3] = {TL::X, TL::Y, TL::Z};
     std::vector<int> list;

@@ -29,16 +29,19 @@ namespace fem {
 enum { T10_APPLY = 0, T10_RESID = 1, T10_CHEB = 2, T10_PCGQ = 3 };
 
 // The tile of the fused kernels per storage type: 256 threads in FP32, 128 in FP64 (whose cell
-// state takes twice the registers).
+// state takes twice the registers). Long in x: a wave then spans whole runs of a grid row, so its
+// node loads and stores touch fewer cache lines, and fewer tile faces cut across rows (measured on
+// 72 x 36 x 36 cells against 8 x 8 x 4, 16 x 4 x 4 and 16 x 8 x 2: 25 us per FP32 Chebyshev step
+// against 31, 27 and 27).
 template <class T>
 struct T10Tile;
 template <>
 struct T10Tile<float> {
-    static constexpr int X = 8, Y = 8, Z = 4;
+    static constexpr int X = 32, Y = 4, Z = 2;
 };
 template <>
 struct T10Tile<double> {
-    static constexpr int X = 8, Y = 4, Z = 4;
+    static constexpr int X = 16, Y = 2, Z = 4;
 };
 
 template <class T>
@@ -80,27 +83,54 @@ __host__ __device__ inline T10Role t10_role(int p, int l, int c0, int tn, int cn
 // A refined coordinate on a face shared by two tiles of extent tn cells, in a grid of n nodes.
 __host__ __device__ inline bool t10_on_face(int i, int tn, int n) { return i > 0 && i < n - 1 && i % (2 * tn) == 0; }
 
-// The update of one node of the mode from its operator value yv, its operand xin and (CHEB,
-// RESID) its b, d and D^-1 already loaded; pq gathers x.q (PCGQ).
+// The value at element idx of p when valid, else 0, without a branch (a load of element 0 in its
+// place).
+template <class T>
+__device__ __forceinline__ T t10_load(const T* p, bool valid, unsigned idx) {
+    const T v = p[valid ? idx : 0u];
+    return valid ? v : T(0);
+}
+
+// The 3 components of a node (unknowns u0 .. u0 + 2) when valid, else 0, branch-free; one base for
+// the 3, so that they load as one vector access.
+template <class T>
+__device__ __forceinline__ void t10_load3(const T* p, bool valid, unsigned u0, T v[3]) {
+    const T* q = p + (valid ? u0 : 0u);
+    const T a = q[0], b = q[1], c = q[2];
+    v[0] = valid ? a : T(0);
+    v[1] = valid ? b : T(0);
+    v[2] = valid ? c : T(0);
+}
+
+// The update of one node (unknowns u0 .. u0 + 2) of the mode from its operator value yv, its
+// operand xin and (CHEB, RESID) its b, d and D^-1 already loaded; pq gathers x.q (PCGQ).
 template <class T, int MODE>
-__device__ __forceinline__ void t10_update(const T10Args<T>& a, size_t node, bool fixed, const T yv[3], const T xin[3],
+__device__ __forceinline__ void t10_update(const T10Args<T>& a, unsigned u0, bool fixed, const T yv[3], const T xin[3],
                                            const T bv[3], const T dv[3], const T iv[3], double& pq) {
+    T out[3], dn[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const size_t u = 3 * node + c;
         if (MODE == T10_APPLY) {
-            a.y[u] = fixed ? T(0) : yv[c];
+            out[c] = fixed ? T(0) : yv[c];
         } else if (MODE == T10_RESID) {
-            a.y[u] = fixed ? T(0) : bv[c] - yv[c];
+            out[c] = fixed ? T(0) : bv[c] - yv[c];
         } else if (MODE == T10_CHEB) {
-            const T dn = fixed ? T(0) : a.c1 * dv[c] + a.c2 * iv[c] * (bv[c] - yv[c]);
-            a.d[u] = dn;
-            a.y[u] = fixed ? T(0) : xin[c] + dn;
+            dn[c] = fixed ? T(0) : a.c1 * dv[c] + a.c2 * iv[c] * (bv[c] - yv[c]);
+            out[c] = fixed ? T(0) : xin[c] + dn[c];
         } else {
-            const T q = fixed ? T(0) : yv[c];
-            a.y[u] = q;
-            pq += (double)xin[c] * (double)q;
+            out[c] = fixed ? T(0) : yv[c];
+            pq += (double)xin[c] * (double)out[c];
         }
+    }
+    T* y = a.y + u0;
+    y[0] = out[0];
+    y[1] = out[1];
+    y[2] = out[2];
+    if (MODE == T10_CHEB) {
+        T* d = a.d + u0;
+        d[0] = dn[0];
+        d[1] = dn[1];
+        d[2] = dn[2];
     }
 }
 
@@ -150,10 +180,10 @@ __device__ __forceinline__ void t10_exchange(T F[27][3], T* xch, int tid, bool s
 }
 
 // FP32 asks for 2 waves per SIMD (at most 256 registers per lane), which two tiles of LDS per CU
-// allow; FP64 takes 1.
+// allow; FP64 takes 1. Index arithmetic is 32-bit: the host keeps every byte offset below 2^31.
 template <class T, int TX, int TY, int TZ, int MODE>
 __global__ void __launch_bounds__(TX * TY * TZ, sizeof(T) == 4 ? 2 : 1) k_t10_tile(T10Args<T> a) {
-    constexpr int NT = TX * TY * TZ;
+    constexpr int NT = TX * TY * TZ, NW = NT / 64;
     constexpr int LX = 2 * TX + 1, LY = 2 * TY + 1, LZ = 2 * TZ + 1;
     constexpr int RS = 3 * LX;  // values per box row
     __shared__ T xs[LZ * LY * RS];
@@ -161,43 +191,54 @@ __global__ void __launch_bounds__(TX * TY * TZ, sizeof(T) == 4 ? 2 : 1) k_t10_ti
     const int tid = threadIdx.x;
     const int tile = (int)xcd_block();
     const int X0 = (tile % a.tx) * TX, Y0 = ((tile / a.tx) % a.ty) * TY, Z0 = (tile / (a.tx * a.ty)) * TZ;
-    const int NX = a.g.nx, NY = a.g.ny, NZ = a.g.nz;
+    const unsigned NX = a.g.nx, NY = a.g.ny, NXY = NX * NY;
+    const unsigned N3 = 3 * NXY * a.g.nz;
     const int lx = tid % TX, ly = (tid / TX) % TY, lz = tid / (TX * TY);
     const int ex = X0 + lx, ey = Y0 + ly, ez = Z0 + lz;
     const bool valid = ex < a.cx && ey < a.cy && ez < a.cz;
     // Every load of a phase is issued before the first of them is used, so that their latencies
-    // overlap: the cell's 12 Lame coefficients, then the box. The loads go through buffer
-    // resources, branch-free: an offset past the end reads 0 (byte offsets below 2^31, checked by
-    // the host).
+    // overlap: the cell's 12 Lame coefficients, then the box. The loads are branch-free.
     const unsigned cells = (unsigned)a.cx * a.cy * a.cz;
-    const __amdgpu_buffer_rsrc_t rx = make_rsrc(a.x, 3 * (size_t)NX * NY * NZ);
     T lam[6], mu[6];
     {
-        const __amdgpu_buffer_rsrc_t rc = make_rsrc(a.coef, 12 * (size_t)cells);
         const unsigned cell = ex + (unsigned)a.cx * (ey + (unsigned)a.cy * ez);
 #pragma unroll
         for (int t = 0; t < 6; ++t) {
-            lam[t] = buffer_value<T>(rc, valid, t * cells + cell);
-            mu[t] = buffer_value<T>(rc, valid, (6 + t) * cells + cell);
+            lam[t] = t10_load(a.coef, valid, t * cells + cell);
+            mu[t] = t10_load(a.coef, valid, (6 + t) * cells + cell);
         }
     }
-    // 1. the box of x, zero beyond the grid
+    // 1. the box of x, zero beyond the grid: wave w takes rows w, w + NW, ...; a row is RS
+    // contiguous values of the grid, in chunks of 64 lanes, read through a buffer resource (an
+    // offset past the end reads 0).
     {
-        constexpr int NE = LZ * LY * RS, NL = (NE + NT - 1) / NT;
+        const __amdgpu_buffer_rsrc_t rx = make_rsrc(a.x, N3);
+        constexpr int ROWS = LY * LZ, RPW = (ROWS + NW - 1) / NW, CH = (RS + 63) / 64;
+        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
         const int I0 = 2 * X0, J0 = 2 * Y0, K0 = 2 * Z0;
-        const int nI = min(LX, NX - I0), nJ = min(LY, NY - J0), nK = min(LZ, NZ - K0);
-        T v[NL];
+        const int nI = min(LX, (int)NX - I0), nJ = min(LY, (int)NY - J0), nK = min(LZ, a.g.nz - K0);
+        T v[RPW][CH];
 #pragma unroll
-        for (int i = 0; i < NL; ++i) {
-            const int e = tid + i * NT;
-            const int q = e % RS, r = e / RS;
+        for (int i = 0; i < RPW; ++i) {
+            const int r = wave + i * NW;  // wave-uniform
             const int by = r % LY, bz = r / LY;
-            v[i] = buffer_value<T>(rx, e < NE && q < 3 * nI && by < nJ && bz < nK,
-                                   3 * (I0 + (unsigned)NX * ((J0 + by) + (unsigned)NY * (K0 + bz))) + q);
+            const bool row = r < ROWS && by < nJ && bz < nK;
+            const unsigned base = 3 * (I0 + NX * (J0 + by) + NXY * (K0 + bz)) + lane;
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                const int q = lane + 64 * c;
+                v[i][c] = buffer_value<T>(rx, row && q < 3 * nI, base + 64 * c);
+            }
         }
 #pragma unroll
-        for (int i = 0; i < NL; ++i)
-            if (tid + i * NT < NE) xs[tid + i * NT] = v[i];
+        for (int i = 0; i < RPW; ++i) {
+            const int r = wave + i * NW;
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                const int q = lane + 64 * c;
+                if (r < ROWS && q < RS) xs[r * RS + q] = v[i][c];
+            }
+        }
     }
     __syncthreads();
     // 2. the 6 tetrahedra of the cell
@@ -225,8 +266,8 @@ __global__ void __launch_bounds__(TX * TY * TZ, sizeof(T) == 4 ? 2 : 1) k_t10_ti
     // last cells of the tile only), which frees most of F, then the 8 nodes p in {0, 1}^3, with
     // the loads of all 8 issued before the first update.
     double pq = 0.0;
-    const size_t N3 = 3 * (size_t)NX * NY * NZ;
     if (valid) {
+        const unsigned node0 = 2 * ex + NX * (2 * ey) + NXY * (2 * ez);
 #pragma unroll
         for (int pos = 0; pos < 27; ++pos) {
             const int px = pos % 3, py = (pos / 3) % 3, pz = pos / 9;
@@ -234,42 +275,40 @@ __global__ void __launch_bounds__(TX * TY * TZ, sizeof(T) == 4 ? 2 : 1) k_t10_ti
             const T10Role rx = t10_role(px, lx, X0, TX, a.cx), ry = t10_role(py, ly, Y0, TY, a.cy),
                           rz = t10_role(pz, lz, Z0, TZ, a.cz);
             if (!(rx.held && ry.held && rz.held)) continue;
-            const int slot = (rx.high ? 1 : 0) | (ry.high ? 2 : 0) | (rz.high ? 4 : 0);
-            const size_t node = (2 * ex + px) + (size_t)NX * ((2 * ey + py) + (size_t)NY * (2 * ez + pz));
-#pragma unroll
-            for (int c = 0; c < 3; ++c) a.P[slot * N3 + 3 * node + c] = F[pos][c];
+            const unsigned slot = (rx.high ? 1 : 0) | (ry.high ? 2 : 0) | (rz.high ? 4 : 0);
+            T* q = a.P + (slot * N3 + 3 * (node0 + px + NX * py + NXY * pz));
+            q[0] = F[pos][0];
+            q[1] = F[pos][1];
+            q[2] = F[pos][2];
         }
         constexpr int NV = MODE == T10_CHEB ? 3 : MODE == T10_RESID ? 1 : 0;
-        const __amdgpu_buffer_rsrc_t rb = make_rsrc(a.b, N3), rd = make_rsrc(a.d, N3), ri = make_rsrc(a.invd, N3);
         T nv[3][8][3];
         bool fin[8];
-        size_t nodes[8];
 #pragma unroll
         for (int p = 0; p < 8; ++p) {
             const int px = p & 1, py = (p >> 1) & 1, pz = p >> 2;
             fin[p] = !(t10_role(px, lx, X0, TX, a.cx).low || t10_role(py, ly, Y0, TY, a.cy).low ||
                        t10_role(pz, lz, Z0, TZ, a.cz).low);
-            nodes[p] = (2 * ex + px) + (size_t)NX * ((2 * ey + py) + (size_t)NY * (2 * ez + pz));
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const unsigned u = 3 * (unsigned)nodes[p] + c;
-                nv[0][p][c] = NV > 0 ? buffer_value<T>(rb, fin[p], u) : T(0);
-                nv[1][p][c] = NV > 1 ? buffer_value<T>(rd, fin[p], u) : T(0);
-                nv[2][p][c] = NV > 2 ? buffer_value<T>(ri, fin[p], u) : T(0);
-            }
+            const unsigned u0 = 3 * (node0 + px + NX * py + NXY * pz);
+            if (NV > 0) t10_load3(a.b, fin[p], u0, nv[0][p]);
+            if (NV > 1) t10_load3(a.d, fin[p], u0, nv[1][p]);
+            if (NV > 2) t10_load3(a.invd, fin[p], u0, nv[2][p]);
         }
 #pragma unroll
         for (int p = 0; p < 8; ++p) {
             const int px = p & 1, py = (p >> 1) & 1, pz = p >> 2;
             const int pos = px + 3 * py + 9 * pz;
+            const unsigned u0 = 3 * (node0 + px + NX * py + NXY * pz);
             if (!fin[p]) {  // on the tile's low face: slot 0
-#pragma unroll
-                for (int c = 0; c < 3; ++c) a.P[3 * nodes[p] + c] = F[pos][c];
+                T* q = a.P + u0;
+                q[0] = F[pos][0];
+                q[1] = F[pos][1];
+                q[2] = F[pos][2];
                 continue;
             }
             const T* xl = base + (pz * LY + py) * RS + 3 * px;
             const T xin[3] = {xl[0], xl[1], xl[2]};
-            t10_update<T, MODE>(a, nodes[p], is_fixed(a.g, 2 * ex + px, 2 * ey + py, 2 * ez + pz), F[pos], xin, nv[0][p],
+            t10_update<T, MODE>(a, u0, is_fixed(a.g, 2 * ex + px, 2 * ey + py, 2 * ez + pz), F[pos], xin, nv[0][p],
                                 nv[1][p], nv[2][p], pq);
         }
     }
@@ -285,27 +324,29 @@ __global__ void __launch_bounds__(BLOCK) k_t10_bound(T10Args<T> a) {
     const int e = blockIdx.x * BLOCK + threadIdx.x;
     double pq = 0.0;
     if (e < a.nb) {
-        const int NX = a.g.nx, NY = a.g.ny, NZ = a.g.nz;
-        const size_t node = (size_t)a.blist[e];
-        const int I = (int)(node % NX), J = (int)((node / NX) % NY), K = (int)(node / ((size_t)NX * NY));
-        const int B = (t10_on_face(I, TX, NX) ? 1 : 0) | (t10_on_face(J, TY, NY) ? 2 : 0) | (t10_on_face(K, TZ, NZ) ? 4 : 0);
-        const size_t N3 = 3 * (size_t)NX * NY * NZ;
+        const unsigned NX = a.g.nx, NY = a.g.ny, NXY = NX * NY;
+        const unsigned N3 = 3 * NXY * a.g.nz;
+        const unsigned node = (unsigned)a.blist[e];
+        const int I = (int)(node % NX), J = (int)((node / NX) % NY), K = (int)(node / NXY);
+        const int B = (t10_on_face(I, TX, NX) ? 1 : 0) | (t10_on_face(J, TY, NY) ? 2 : 0) | (t10_on_face(K, TZ, a.g.nz) ? 4 : 0);
+        const unsigned u0 = 3 * node;
+        T P[8][3], xin[3], bv[3] = {T(0), T(0), T(0)}, dv[3] = {T(0), T(0), T(0)}, iv[3] = {T(0), T(0), T(0)};
+#pragma unroll
+        for (int s = 0; s < 8; ++s) t10_load3(a.P, (s & ~B) == 0, s * N3 + u0, P[s]);
+        t10_load3(a.x, true, u0, xin);
+        if (MODE == T10_RESID || MODE == T10_CHEB) t10_load3(a.b, true, u0, bv);
+        if (MODE == T10_CHEB) {
+            t10_load3(a.d, true, u0, dv);
+            t10_load3(a.invd, true, u0, iv);
+        }
         T yv[3] = {T(0), T(0), T(0)};
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             if (s & ~B) continue;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) yv[c] += a.P[s * N3 + 3 * node + c];
+            for (int c = 0; c < 3; ++c) yv[c] += P[s][c];
         }
-        T xin[3], bv[3], dv[3], iv[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            xin[c] = a.x[3 * node + c];
-            bv[c] = MODE == T10_RESID || MODE == T10_CHEB ? a.b[3 * node + c] : T(0);
-            dv[c] = MODE == T10_CHEB ? a.d[3 * node + c] : T(0);
-            iv[c] = MODE == T10_CHEB ? a.invd[3 * node + c] : T(0);
-        }
-        t10_update<T, MODE>(a, node, is_fixed(a.g, I, J, K), yv, xin, bv, dv, iv, pq);
+        t10_update<T, MODE>(a, u0, is_fixed(a.g, I, J, K), yv, xin, bv, dv, iv, pq);
     }
     if (MODE == T10_PCGQ) {
         pq = block_sum(pq);

@@ -43,67 +43,65 @@ __host__ __device__ constexpr int t10_local_at(int t, int px, int py, int pz) {
         if (t10_offset(t, n, 0) == px && t10_offset(t, n, 1) == py && t10_offset(t, n, 2) == pz) r = n;
     return r;
 }
-// The element product y = K u of one tet10 of Kuhn type TT with isotropic lambda, mu: the
-// formulation of tet10_element with the barycentric gradients of the type written out. Along axis
-// b, the gradients are -1/h_b at chain vertex m = rank(b), +1/h_b at m + 1 and 0 elsewhere, so
-// every gradient product is a difference of two nodal values.
+// The element product y = K u of one tet10 of Kuhn type TT with isotropic lambda, mu (the matrix
+// of tet10.rs:90-120: the 4-point rule of degree 2 is exact here), in vertex form. On an affine
+// tetrahedron grad u is linear, so the stress is linear with vertex values s_m = sigma(G_m), G_m
+// the gradient at vertex m; grad N_i is linear too, and int L_m L_k = V (1 + [m = k]) / 20 gives
+// y_i = sum_m grad N_i(v_m) : t_m with t_m = V / 20 (s_m + sum_k s_k). With the barycentric
+// gradients of a Kuhn chain (along axis b: -1/h_b at vertex m = rank(b), +1/h_b at m + 1, 0
+// elsewhere), every gradient product is a difference of two values:
+//  - G_m[a][b] = (4 v - c) / h_b, v and c differences of nodal values of component a;
+//  - corner k: y_k = (4 t_k - sum_m t_m) grad L_k, with sum_m t_m = 5 V / 20 sum_k s_k;
+//  - edge (i, j): y_ij = 4 (t_i grad L_j + t_j grad L_i).
+// w is the cell volume / 24 (V / 4 of each of the 6 tetrahedra).
 template <class T, int TT>
 __host__ __device__ __forceinline__ void tet10_element_kuhn(const T u[30], const T ih[3], T lam, T mu, T w, T y[30]) {
     constexpr int NE[4][4] = {{-1, 4, 6, 7}, {4, -1, 5, 8}, {6, 5, -1, 9}, {7, 8, 9, -1}};
     constexpr int EDGE[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
-    const T qa = T(0.5854101966249685), qb = T(0.1381966011250105);  // tet10.rs:45-46
+    // the vertex gradients G[k][a][b] = v * (4 / h_b) - c / h_b
     T G[4][3][3];
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
         const int m = kuhn_rank(TT, b);
         const int mm = NE[m][m + 1];
+        const T i4 = T(4) * ih[b];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const T cm = u[3 * (m + 1) + a] - u[3 * m + a];
+            const T ci = (u[3 * (m + 1) + a] - u[3 * m + a]) * ih[b];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 T v;
                 if (k == m) v = u[3 * mm + a] - u[3 * m + a];
                 else if (k == m + 1) v = u[3 * (m + 1) + a] - u[3 * mm + a];
                 else v = u[3 * NE[k][m + 1] + a] - u[3 * NE[k][m] + a];
-                G[k][a][b] = ih[b] * (T(4) * v - cm);
+                G[k][a][b] = v * i4 - ci;
             }
         }
     }
-    T S[3][3];
+    // vertex stresses with lambda, mu scaled by V / 20, in (xx, yy, zz, yz, xz, xy)
+    const T v20 = w * T(0.2);
+    const T l2 = lam * v20, m2 = mu * v20, mm2 = T(2) * m2;
+    T t[4][6], ss[6];
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) S[a][b] = G[0][a][b] + G[1][a][b] + G[2][a][b] + G[3][a][b];
-    T sig[4][6];
-    T ssum[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        T D[3][3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int b = 0; b < 3; ++b) D[a][b] = qb * S[a][b] + (qa - qb) * G[q][a][b];
-        const T tr = lam * (D[0][0] + D[1][1] + D[2][2]);
-        sig[q][0] = tr + T(2) * mu * D[0][0];
-        sig[q][1] = tr + T(2) * mu * D[1][1];
-        sig[q][2] = tr + T(2) * mu * D[2][2];
-        sig[q][3] = mu * (D[1][2] + D[2][1]);  // yz
-        sig[q][4] = mu * (D[0][2] + D[2][0]);  // xz
-        sig[q][5] = mu * (D[0][1] + D[1][0]);  // xy
-#pragma unroll
-        for (int s = 0; s < 6; ++s) ssum[s] += sig[q][s];
+    for (int k = 0; k < 4; ++k) {
+        const T tr = l2 * (G[k][0][0] + G[k][1][1] + G[k][2][2]);
+        t[k][0] = mm2 * G[k][0][0] + tr;
+        t[k][1] = mm2 * G[k][1][1] + tr;
+        t[k][2] = mm2 * G[k][2][2] + tr;
+        t[k][3] = m2 * (G[k][1][2] + G[k][2][1]);
+        t[k][4] = m2 * (G[k][0][2] + G[k][2][0]);
+        t[k][5] = m2 * (G[k][0][1] + G[k][1][0]);
     }
-    T Sp[4][6], S0[6];
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int s = 0; s < 6; ++s) {
+        ss[s] = (t[0][s] + t[1][s]) + (t[2][s] + t[3][s]);
 #pragma unroll
-        for (int s = 0; s < 6; ++s) Sp[k][s] = w * (qb * ssum[s] + (qa - qb) * sig[k][s]);
-#pragma unroll
-    for (int s = 0; s < 6; ++s) S0[s] = w * ssum[s];
+        for (int k = 0; k < 4; ++k) t[k][s] += ss[s];
+        ss[s] *= T(5);  // sum_m t_m
+    }
     // the symmetric entry (a, b) of (xx, yy, zz, yz, xz, xy)
     auto sym = [](int a, int b) { return a == b ? a : 6 - a - b; };
-    // corners: y_k = (4 Sp_k - S0) g_k
+    // corners: y_k = (4 t_k - sum_m t_m) grad L_k
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
@@ -113,12 +111,12 @@ __host__ __device__ __forceinline__ void tet10_element_kuhn(const T u[30], const
             for (int b = 0; b < 3; ++b) {
                 const int m = kuhn_rank(TT, b);
                 const int s = sym(a, b);
-                if (k == m + 1) acc += (T(4) * Sp[k][s] - S0[s]) * ih[b];
-                else if (k == m) acc -= (T(4) * Sp[k][s] - S0[s]) * ih[b];
+                if (k == m + 1) acc += (T(4) * t[k][s] - ss[s]) * ih[b];
+                else if (k == m) acc -= (T(4) * t[k][s] - ss[s]) * ih[b];
             }
             y[3 * k + a] = acc;
         }
-    // edges (k, j): y = 4 (Sp_k g_j + Sp_j g_k)
+    // edges (k, j): y = 4 (t_k grad L_j + t_j grad L_k)
 #pragma unroll
     for (int e = 0; e < 6; ++e) {
         const int k = EDGE[e][0], j = EDGE[e][1];
@@ -129,10 +127,10 @@ __host__ __device__ __forceinline__ void tet10_element_kuhn(const T u[30], const
             for (int b = 0; b < 3; ++b) {
                 const int m = kuhn_rank(TT, b);
                 const int s = sym(a, b);
-                if (j == m + 1) acc += Sp[k][s] * ih[b];
-                else if (j == m) acc -= Sp[k][s] * ih[b];
-                if (k == m + 1) acc += Sp[j][s] * ih[b];
-                else if (k == m) acc -= Sp[j][s] * ih[b];
+                if (j == m + 1) acc += t[k][s] * ih[b];
+                else if (j == m) acc -= t[k][s] * ih[b];
+                if (k == m + 1) acc += t[j][s] * ih[b];
+                else if (k == m) acc -= t[j][s] * ih[b];
             }
             y[3 * (4 + e) + a] = T(4) * acc;
         }

@@ -50,9 +50,10 @@ def test_heat_hex8_apply_matches_oracle_matrix(fem, oracle):
     assert rel(s.apply(x), a.mul(x)) < 1e-13
 
 
-# tet10 [10, 9, 6]: several tiles of the fused operator kernels along each axis, the last ones partial.
+# tet10 [36, 5, 6]: several tiles of the fused operator kernels along each axis (FP64 tiles of
+# 16 x 2 x 4 cells), the last ones partial.
 @pytest.mark.parametrize("kind,n", [("heat_tet4", [8, 8, 8]), ("elastic_tet4", [8, 4, 4]), ("elastic_tet10", [4, 4, 4]),
-                                    ("elastic_tet10", [10, 9, 6])])
+                                    ("elastic_tet10", [36, 5, 6])])
 def test_tets_apply_matches_oracle_matrix(fem, oracle, kind, n):
     rng = np.random.default_rng(11)
     nt = 6 * int(np.prod(n))
@@ -77,7 +78,7 @@ def test_tets_apply_matches_oracle_matrix(fem, oracle, kind, n):
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
 @pytest.mark.parametrize("kind,n", [("heat_tet4", [8, 8, 8]), ("elastic_tet4", [8, 4, 4]), ("elastic_tet10", [8, 4, 4]),
-                                    ("elastic_tet10", [16, 12, 4])])
+                                    ("elastic_tet10", [16, 12, 4]), ("elastic_tet10", [34, 6, 4])])
 def test_tets_multigrid_matches_oracle_direct(fem, oracle, kind, n, precision):
     rng = np.random.default_rng(2)
     nt = 6 * int(np.prod(n))
@@ -127,12 +128,13 @@ def test_nonconvergence_and_cancel_messages(fem):
 
 
 def test_fused_tet10_v_cycle_fp32_agrees_with_fp64(fem):
-    """The FP32 V-cycle (fused tet10 kernels on 8 x 8 x 4 tiles) against the FP64 one (8 x 4 x 4
-    tiles) on a box of several partial tiles: the same preconditioner up to FP32 rounding."""
-    n = [18, 10, 6]
+    """The FP32 V-cycle against the FP64 one, whose fused tet10 kernels run on other tiles, on a box
+    of several tiles along each axis, the last ones partial: the same preconditioner up to FP32
+    rounding."""
+    n = [36, 10, 6]
     rng = np.random.default_rng(21)
     nt = 6 * int(np.prod(n))
-    box = fem.Box("elastic_tet10", n, [1.8, 1.0, 0.6], rng.uniform(1.0, 2.0, nt), rng.uniform(0.2, 0.4, nt),
+    box = fem.Box("elastic_tet10", n, [3.6, 1.0, 0.6], rng.uniform(1.0, 2.0, nt), rng.uniform(0.2, 0.4, nt),
                   fixed={"x0": True}, body=(0, 0, -1))
     s32 = fem.Solver(box, method="multigrid", precision="fp32", levels=3)
     s64 = fem.Solver(box, method="multigrid", precision="fp64", levels=3)
