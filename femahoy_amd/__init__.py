@@ -31,6 +31,8 @@ KINDS = {"heat_hex8": HEAT_HEX8, "heat_tet4": HEAT_TET4, "elastic_tet4": ELASTIC
 METHODS = {None: -1, "direct": 0, "multigrid": 1}
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2}
 FACES = ("x0", "x1", "y0", "y1", "z0", "z1")
+# The chain axis order of each Kuhn type, as the device code and fem_model_tet_cells number them.
+KUHN_AXES = ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0))
 
 PCG_TOL = 1e-10        # solve.rs:42
 PCG_MAX_ITER = 500     # solve.rs:43
@@ -119,6 +121,7 @@ EXPORTS = (
     "fem_solver_rhs", "fem_solver_apply", "fem_solver_precondition", "fem_solver_level_info",
     "fem_solver_device_name", "fem_model_new_box", "fem_model_point_count", "fem_model_unknown_count",
     "fem_model_free_count", "fem_model_free_list", "fem_model_point_grid", "fem_model_load", "fem_model_free",
+    "fem_model_tet_cells",
     "fem_solver_set_profile", "fem_solver_profile", "fem_solver_time_kernels",
     "fem_server_new", "fem_server_submit", "fem_server_wait", "fem_server_cancel", "fem_server_free",
 )
@@ -169,6 +172,7 @@ def lib() -> C.CDLL:
         "fem_model_free_list": (C.c_int, [P, I64P]),
         "fem_model_point_grid": (C.c_int, [P, U32P]),
         "fem_model_load": (C.c_int, [P, F64P]),
+        "fem_model_tet_cells": (C.c_int, [P, U32P, np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")]),
         "fem_model_free": (None, [P]),
         "fem_solver_set_profile": (None, [P, C.c_int]),
         "fem_solver_profile": (C.c_int, [P, C.POINTER(Profile)]),
@@ -264,6 +268,7 @@ class Model:
         if lib().fem_model_new_box(C.byref(b), C.byref(h), err, 2048):
             raise FemError(err.value.decode())
         self.h = h
+        self._n, self._length = list(box.n), list(box.length)
         del keep
 
     def __del__(self):
@@ -288,6 +293,29 @@ class Model:
         out = np.zeros(lib().fem_model_unknown_count(self.h))
         lib().fem_model_load(self.h, out)
         return out
+
+    def tet_cells(self):
+        """The cell (x fastest) and Kuhn type of each tetrahedron, in the reference's numbering."""
+        ntet = 6 * int(np.prod(self._n))
+        cell = np.zeros(ntet, np.uint32)
+        kind = np.zeros(ntet, np.uint8)
+        if lib().fem_model_tet_cells(self.h, cell, kind):
+            raise FemError("a hex8 model has no tetrahedra")
+        return cell, kind
+
+    def tet_centroids(self) -> np.ndarray:
+        """The centroid of each tetrahedron (reference numbering), shape (tets, 3). A Kuhn chain
+        of axes (a, b, c) from the cell's lowest corner has its centroid at (3/4, 2/4, 1/4) of the
+        cell along a, b, c."""
+        cell, kind = self.tet_cells()
+        n = np.asarray(self._n, np.int64)
+        h = np.asarray(self._length, np.float64) / n
+        ijk = np.stack([cell % n[0], (cell // n[0]) % n[1], cell // (n[0] * n[1])], axis=1).astype(np.float64)
+        frac = np.zeros((6, 3))
+        for t, axes in enumerate(KUHN_AXES):
+            for r, a in enumerate(axes):
+                frac[t, a] = (3 - r) / 4.0
+        return (ijk + frac[kind]) * h
 
 
 def _options(method=None, precision="fp32", tol=PCG_TOL, max_iter=PCG_MAX_ITER, crossover=STEADY_DIRECT_LIMIT,
@@ -478,3 +506,26 @@ class Server:
     def cancel(self, rid: int) -> bool:
         return lib().fem_server_cancel(self.h, rid) == 0
 
+
+
+def inclusion_box(n: int, count: int, seed: int, e_ratio: float = 1e3, nu_matrix: float = 0.49,
+                  nu_inclusion: float = 0.3) -> "tuple[Box, np.ndarray]":
+    """Nearly incompressible heterogeneous elasticity: tet10 on n x n/2 x n/2 unit cells (6 Kuhn
+    tetrahedra per cell), a matrix of E = 1 and nu = nu_matrix holding `count` spheres of E =
+    e_ratio and nu = nu_inclusion, centres uniform in the box and radii uniform in 0.02-0.05 of
+    the box length, from numpy's default_rng(seed); a tetrahedron is in a sphere when its centroid
+    is. Clamped at x = 0, traction (1, 0, 0) on x = 1. Returns the box and the inclusion mask
+    (reference numbering of tetrahedra)."""
+    cells = [n, n // 2, n // 2]
+    length = [float(c) for c in cells]
+    c = Model(Box("elastic_tet10", cells, length, 1.0, 0.3, fixed={"x0": True})).tet_centroids()
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(0.0, 1.0, (count, 3)) * np.asarray(length)
+    radii = rng.uniform(0.02, 0.05, count) * max(length)
+    inside = np.zeros(len(c), bool)
+    for p, r in zip(centres, radii):
+        inside |= ((c - p) ** 2).sum(axis=1) <= r * r
+    e = np.where(inside, e_ratio, 1.0)
+    nu = np.where(inside, nu_inclusion, nu_matrix)
+    box = Box("elastic_tet10", cells, length, e, nu, fixed={"x0": True}, traction={"x1": (1.0, 0.0, 0.0)})
+    return box, inside

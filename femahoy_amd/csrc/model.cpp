@@ -88,8 +88,8 @@ BoxModel build_model(const BoxInput& in) {
     // ---------------------------------------------------------------- numbering
     std::vector<uint32_t> gijk;  // grid point of each reference point, 3 per point
     // tet index (reference order) -> cell and Kuhn type
-    std::vector<uint32_t> tet_cell;
-    std::vector<uint8_t> tet_type;
+    std::vector<uint32_t>& tet_cell = m.tet_cell;
+    std::vector<uint8_t>& tet_type = m.tet_type;
     if (!tets) {
         std::vector<uint32_t> number(gnodes, UINT32_MAX);
         for (size_t cell = 0; cell < cells; ++cell) {

@@ -38,6 +38,8 @@ struct BoxModel {
     // materials in grid layout, FP64
     std::vector<double> cell_k;        // hex8: k per cell (x fastest)
     std::vector<double> tet_coef;      // Kuhn tets: [(q * 6 + type) * cells + cell], q = 0 (k or lambda), 1 (mu)
+    std::vector<uint32_t> tet_cell;    // tetrahedron (reference order) -> cell (x fastest)
+    std::vector<uint8_t> tet_type;     // tetrahedron (reference order) -> Kuhn type (chain axis order)
     // loads over all level-0 grid unknowns, and the fixed values
     std::vector<double> load;
     std::vector<double> fixed_value;   // over level-0 grid unknowns; nonzero only on fixed faces

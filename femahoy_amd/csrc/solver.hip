@@ -2017,6 +2017,13 @@ int fem_model_load(const fem_model* h, double* out) {
     return 0;
 }
 
+int fem_model_tet_cells(const fem_model* h, uint32_t* cell, uint8_t* type) {
+    if (!h || !cell || !type || h->m.tet_cell.empty()) return 1;
+    std::copy(h->m.tet_cell.begin(), h->m.tet_cell.end(), cell);
+    std::copy(h->m.tet_type.begin(), h->m.tet_type.end(), type);
+    return 0;
+}
+
 void fem_model_free(fem_model* h) { delete h; }
 
 }  // extern "C"

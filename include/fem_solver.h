@@ -251,6 +251,11 @@ int fem_model_free_list(const fem_model* m, int64_t* out);
 int fem_model_point_grid(const fem_model* m, uint32_t* out);
 /* The load over all unknowns (reference order): heat sources, body forces, face tractions. */
 int fem_model_load(const fem_model* m, double* out);
+/* The cell (x fastest) and the Kuhn type (0..5: chain axis orders xyz, xzy, yxz, yzx, zxy, zyx)
+ * of each tetrahedron, in the reference's numbering of tetrahedra (the extrusion of
+ * crates/fem/src/prism.rs:244-284 cut by split_prism, prism.rs:144-154), so that a caller can
+ * give materials by position. 6 per cell; returns 1 for a hex8 model. */
+int fem_model_tet_cells(const fem_model* m, uint32_t* cell, uint8_t* type);
 void fem_model_free(fem_model* m);
 
 #ifdef __cplusplus

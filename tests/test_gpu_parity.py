@@ -100,6 +100,24 @@ def test_tets_multigrid_matches_oracle_direct(fem, oracle, kind, n, precision):
     assert rel(x, xd) < TOL_X
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_stiff_inclusions_in_a_nearly_incompressible_matrix_match_oracle_direct(fem, oracle, precision):
+    """The heterogeneous workload (nu = 0.49 matrix, 10 spheres of 1000 times its modulus, traction
+    on x = 1) on 16 x 8 x 8 tet10 cells: the device solve to 1e-10 against the oracle's direct
+    solve, within 1e-8 relative L2."""
+    box, inside = fem.inclusion_box(16, 10, 7)
+    assert inside.any() and not inside.all()
+    pr = oracle.Problem.elastic(10, box.n, box.length, np.asarray(box.coef0), np.asarray(box.coef1), clamped=["x0"],
+                                traction={"x1": (1.0, 0.0, 0.0)})
+    xd, _, _ = oracle.Solver(pr, "direct").solve(pr.rhs())
+    s = fem.Solver(box, method="multigrid", precision=precision, tol=1e-10, max_iter=2000)
+    assert np.array_equal(s.free_list(), pr.free())
+    x, st = s.solve(pr.rhs())
+    print(precision, st)
+    assert st.true_residual <= 1e-10
+    assert rel(x, xd) < TOL_X
+
+
 def test_direct_cantilever_tet4_matches_oracle(fem, oracle):
     # 40 x 4 x 4 steel cantilever, tet4, clamped at x = 0, 1 MPa on the top face: 3000 free unknowns.
     n, length = [40, 4, 4], [100.0, 10.0, 10.0]
@@ -114,6 +132,22 @@ def test_direct_cantilever_tet4_matches_oracle(fem, oracle):
     x, st = s.solve(s.rhs())
     assert st.iterations == 0 and st.residual < 1e-10, st
     assert rel(x, xd) < 1e-9
+
+
+def test_cantilever_tet4_fp64_multigrid_matches_oracle(fem, oracle):
+    """The same cantilever by plain FP64 MG-PCG to the reference tolerance 1e-10 (solve.rs:42):
+    within 1e-8 of the oracle's direct solve."""
+    n, length = [40, 4, 4], [100.0, 10.0, 10.0]
+    nt = 6 * int(np.prod(n))
+    pr = oracle.Problem.elastic(4, n, length, np.full(nt, 210e3), np.full(nt, 0.3), clamped=["x0"],
+                                traction={"z1": (0, 0, -1.0)})
+    xd, _, _ = oracle.Solver(pr, "direct").solve(pr.rhs())
+    s = fem.Solver(fem.Box("elastic_tet4", n, length, np.full(nt, 210e3), np.full(nt, 0.3), fixed={"x0": True},
+                           traction={"z1": (0, 0, -1.0)}), method="multigrid", precision="fp64", tol=1e-10)
+    x, st = s.solve(s.rhs())
+    print(st)
+    assert st.residual <= 1e-10 and st.true_residual <= 1e-10
+    assert rel(x, xd) < TOL_X
 
 
 def test_nonconvergence_and_cancel_messages(fem):

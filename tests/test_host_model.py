@@ -67,3 +67,23 @@ def test_material_errors_use_the_reference_texts(fem):
     with pytest.raises(fem.FemError) as e:
         fem.Model(fem.Box("elastic_tet4", [1, 1, 1], [1, 1, 1], -1.0, 0.3, fixed={"x0": True}))
     assert "the moduli must be above 0 and the Poisson ratios finite" in str(e.value)  # hex.rs:79
+
+
+@pytest.mark.parametrize("kind,n", [("heat_tet4", [3, 4, 2]), ("elastic_tet10", [2, 5, 1]), ("elastic_tet4", [4, 2, 3])])
+def test_tet_centroids_follow_the_reference_numbering(fem, oracle, kind, n):
+    """Model.tet_centroids (from fem_model_tet_cells) against the centroids of the oracle's own
+    tetrahedra, which it numbers by the reference's extrusion and prism split."""
+    length = [1.3, 0.7, 2.1]
+    nt = 6 * int(np.prod(n))
+    if kind == "heat_tet4":
+        pr = oracle.Problem.heat(n, length, np.ones(nt), fixed={"x0": 0.0}, tet4=True)
+        m = fem.Model(fem.Box(kind, n, length, np.ones(nt), fixed={"x0": 0.0}))
+    else:
+        order = 4 if kind == "elastic_tet4" else 10
+        pr = oracle.Problem.elastic(order, n, length, np.ones(nt), np.full(nt, 0.3), clamped=["x0"])
+        m = fem.Model(fem.Box(kind, n, length, 1.0, 0.3, fixed={"x0": True}))
+    want = pr.points()[pr.cells()[:, :4]].mean(axis=1)
+    assert np.abs(m.tet_centroids() - want).max() <= 1e-14
+    cell, kind_ = m.tet_cells()
+    assert np.array_equal(np.bincount(cell, minlength=int(np.prod(n))), np.full(int(np.prod(n)), 6))
+    assert set(np.unique(kind_).tolist()) == set(range(6))
