@@ -175,14 +175,16 @@ def run_stream(F, reqs, arrivals, budget, max_active):
             if dt > 0:
                 time.sleep(dt)
         ids.append(srv.submit(box, **opts))
-    lat, its = [], []
+    lat, its, phases = [], [], []
     for rid, (_, kind, m, box, n) in zip(ids, reqs):
         _, st, info = srv.wait(rid, n, timeout=600)
         lat.append(info.finish_time - info.submit_time)
         its.append(st.iterations)
+        phases.append((info.admit_time - info.submit_time, info.start_time - info.admit_time,
+                       info.finish_time - info.start_time))
     makespan = time.perf_counter() - t0
     srv.close()
-    return makespan, np.array(lat), its
+    return makespan, np.array(lat), its, np.array(phases)
 
 
 def serving(F, n_solves, seed):
@@ -190,7 +192,7 @@ def serving(F, n_solves, seed):
     total_dof = sum(r[4] for r in reqs)
     budget, max_active = 64 << 30, 16
     log(f"serving: {len(reqs)} solves ({total_dof} DOF), capacity run (all submitted at once)")
-    cap_span, cap_lat, _ = run_stream(F, reqs, None, budget, max_active)
+    cap_span, cap_lat, _, _ = run_stream(F, reqs, None, budget, max_active)
     capacity = len(reqs) / cap_span
     rate = 0.8 * capacity
     rng = np.random.default_rng(seed + 1)
@@ -198,7 +200,7 @@ def serving(F, n_solves, seed):
     arrivals = [pair_t[r[0]] for r in reqs]
     log(f"  capacity {capacity:.1f} solves/s ({total_dof / cap_span / 1e6:.1f} MDOF/s); Poisson arrivals at "
         f"{rate:.1f} solves/s")
-    span, lat, its = run_stream(F, reqs, arrivals, budget, max_active)
+    span, lat, its, ph = run_stream(F, reqs, arrivals, budget, max_active)
     out = {
         "workload": (f"{len(reqs)} solves in refinement pairs (h, h/2), heat tet4 or elasticity tet10 (50/50), "
                      f"{min(r[4] for r in reqs)}-{max(r[4] for r in reqs)} DOF, Poisson pair arrivals at 80% of the "
@@ -210,6 +212,8 @@ def serving(F, n_solves, seed):
         "mdof_s": round(total_dof / span / 1e6, 2), "makespan_s": round(span, 3),
         "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 2), "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 2),
         "mean_ms": round(1e3 * float(np.mean(lat)), 2), "max_iterations": int(max(its)),
+        "mean_queue_ms": round(1e3 * float(ph[:, 0].mean()), 2), "mean_setup_ms": round(1e3 * float(ph[:, 1].mean()), 2),
+        "mean_solve_ms": round(1e3 * float(ph[:, 2].mean()), 2),
         "latency": "submit to solution on the host, including setup (model, hierarchy, coarse factor)",
     }
     log(f"  stream: {out['mdof_s']} MDOF/s, latency p50 {out['p50_ms']} ms, p99 {out['p99_ms']} ms")
@@ -267,6 +271,8 @@ def run_workload(F, hip, box, levels, args, label):
     t1 = time.perf_counter()
     fp64 = F.Solver(box, precision="fp64", **opts)
     t2 = time.perf_counter()
+    bf16 = F.Solver(box, precision="bf16", **opts)
+    t3 = time.perf_counter()
     nfree = mixed.n
     log(f"  {nfree} free unknowns on {mixed.device_name()}; setup mixed {t1 - t0:.2f} s, fp64 {t2 - t1:.2f} s")
     lv = []
@@ -291,6 +297,13 @@ def run_workload(F, hip, box, levels, args, label):
                 smoother_n=tk.smoother_launches, outer_s=tk.outer_seconds / tk.outer_launches,
                 outer_bytes=tk.outer_bytes, outer_n=tk.outer_launches)
     log(f"  smoother {prof['smoother_s'] * 1e6:.2f} us, outer apply {prof['outer_s'] * 1e6:.2f} us per launch")
+    log("mixed with BF16-stored coefficients and D^-1 (FP32 V-cycle arithmetic, FP64 PCG): timed solves")
+    lat_h, st_h, _, _ = timed_solves(F, hip, bf16, d_b, d_x, x0, args.steps, args.warmup, profile=False)
+    x_bf16 = np.zeros(nfree)
+    hip.d2h(x_bf16, d_x)
+    tkh = bf16.time_kernels(100)
+    prof_h = dict(smoother_s=tkh.smoother_seconds / tkh.smoother_launches, smoother_bytes=tkh.smoother_bytes)
+    log(f"  bf16 smoother {prof_h['smoother_s'] * 1e6:.2f} us per launch ({prof_h['smoother_bytes']} bytes)")
     log("FP64 MG-PCG: timed solves")
     lat_d, st_d, _, _ = timed_solves(F, hip, fp64, d_b, d_x, x0, args.steps, args.warmup, profile=False)
     x_fp64 = np.zeros(nfree)
@@ -304,7 +317,9 @@ def run_workload(F, hip, box, levels, args, label):
     hip.free(d_x)
     err_mixed = float(np.linalg.norm(x_mixed - x_ref) / np.linalg.norm(x_ref))
     err_fp64 = float(np.linalg.norm(x_fp64 - x_ref) / np.linalg.norm(x_ref))
-    log(f"  error vs FP64 1e-13 solve: mixed {err_mixed:.2e}, fp64 {err_fp64:.2e}; reference took {st_ref.iterations} it")
+    err_bf16 = float(np.linalg.norm(x_bf16 - x_ref) / np.linalg.norm(x_ref))
+    log(f"  error vs FP64 1e-13 solve: mixed {err_mixed:.2e}, bf16 {err_bf16:.2e}, fp64 {err_fp64:.2e}; reference took "
+        f"{st_ref.iterations} it")
 
     def leg(lat, st, err, setup):
         return {"mdof_s": round(nfree / float(np.mean(lat)) / 1e6, 2), "ms_mean": round(1e3 * float(np.mean(lat)), 3),
@@ -314,9 +329,14 @@ def run_workload(F, hip, box, levels, args, label):
                 "error_vs_fp64": err, "setup_s": round(setup, 3)}
 
     res = {"free_dof": nfree, "levels": [v[0] for v in lv], "mixed": leg(lat_m, st_m, err_mixed, t1 - t0),
+           "mixed_bf16": leg(lat_h, st_h, err_bf16, t3 - t2),
            "fp64": leg(lat_d, st_d, err_fp64, t2 - t1),
            "fp64_reference": f"FP64 MG-PCG to 1e-13 relative residual ({st_ref.iterations} iterations)"}
     res["speedup_mixed_vs_fp64"] = round(res["mixed"]["mdof_s"] / res["fp64"]["mdof_s"], 3)
+    res["speedup_bf16_vs_fp64"] = round(res["mixed_bf16"]["mdof_s"] / res["fp64"]["mdof_s"], 3)
+    res["mixed_bf16"]["finest_smoother"] = {"bytes_per_launch": int(prof_h["smoother_bytes"]),
+                                            "us_per_launch": round(prof_h["smoother_s"] * 1e6, 2),
+                                            "achieved_gbs": round(prof_h["smoother_bytes"] / prof_h["smoother_s"] / 1e9, 1)}
     return res, prof, lat_m
 
 

@@ -122,10 +122,11 @@ __global__ void __launch_bounds__(BLOCK) k_residual(OP op, const T* __restrict__
 }
 
 // One Chebyshev step from a nonzero x: r = b - A x_in, d = c1 d + c2 D^-1 r, x_out = x_in + d.
-template <class OP, class T>
+// D^-1 stored in S (T, or BF16).
+template <class OP, class T, class S = T>
 __global__ void __launch_bounds__(BLOCK) k_cheb(OP op, const T* __restrict__ xin, T* __restrict__ xout,
                                                 T* __restrict__ d, const T* __restrict__ b,
-                                                const T* __restrict__ invd, T c1, T c2) {
+                                                const S* __restrict__ invd, T c1, T c2) {
     const GridDims g = op.dims();
     const size_t node = xcd_block() * BLOCK + threadIdx.x;
     if (node >= g.nodes()) return;
@@ -145,19 +146,19 @@ __global__ void __launch_bounds__(BLOCK) k_cheb(OP op, const T* __restrict__ xin
     for (int c = 0; c < OP::C; ++c) {
         const size_t u = OP::C * node + c;
         const T r = b[u] - out[c];
-        const T dn = c1 * d[u] + c2 * invd[u] * r;
+        const T dn = c1 * d[u] + c2 * ld<T>(invd, u) * r;
         d[u] = dn;
         xout[u] = xin[u] + dn;
     }
 }
 
 // The first Chebyshev step from x = 0: d = c0 D^-1 b, x = d.
-template <class T>
-__global__ void __launch_bounds__(BLOCK) k_cheb_first(size_t n, const T* __restrict__ b, const T* __restrict__ invd,
+template <class T, class S = T>
+__global__ void __launch_bounds__(BLOCK) k_cheb_first(size_t n, const T* __restrict__ b, const S* __restrict__ invd,
                                                       T* __restrict__ x, T* __restrict__ d, T c0) {
     const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
     if (u >= n) return;
-    const T v = c0 * invd[u] * b[u];
+    const T v = c0 * ld<T>(invd, u) * b[u];
     d[u] = v;
     x[u] = v;
 }
@@ -474,6 +475,16 @@ __global__ void __launch_bounds__(BLOCK) k_normalise(size_t n, T* __restrict__ a
     if (u >= n) return;
     const double s = norm2[0] > 0.0 ? 1.0 / sqrt(norm2[0]) : 0.0;
     a[u] = (T)(s * (double)a[u]);
+}
+
+// Rounds an FP32 array to BF16: a keeps the rounded values (so that setup sees the operator the
+// smoother applies) and h their 16-bit storage.
+__global__ void __launch_bounds__(BLOCK) k_round_bf16(size_t n, float* __restrict__ a, bf16_t* __restrict__ h) {
+    const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    if (u >= n) return;
+    const bf16_t v = bf16_round(a[u]);
+    h[u] = v;
+    a[u] = bf16_value(v);
 }
 
 // y = D^-1 y (elementwise).

@@ -37,16 +37,16 @@ constexpr int march_cols_per_block() {
 
 enum MarchMode { MARCH_APPLY = 0, MARCH_RESIDUAL = 1, MARCH_CHEB = 2, MARCH_PCG_Q = 3, MARCH_PCG_START = 4 };
 
-template <class T>
+template <class T, class S = T>
 struct MarchArgs {
     GridDims g;
-    const T* k;     // per cell
+    const S* k;     // per cell, stored in S (T, or BF16)
     T c[8];         // conduction pattern coefficients for k = 1
     const T* x;     // input field (p for MARCH_PCG_Q)
     T* y;           // output: y, r, x_out, q
     const T* b;     // right side (residual, cheb, pcg_start)
     T* d;           // Chebyshev direction
-    const T* invd;  // D^-1
+    const S* invd;  // D^-1, stored in S
     T c1, c2;       // Chebyshev coefficients
     double* part;   // block partials (pcg_q: p.q; pcg_start: b.b)
     double* part2;  // pcg_start: r.r
@@ -84,6 +84,17 @@ __device__ __forceinline__ T buffer_value(__amdgpu_buffer_rsrc_t r, bool valid, 
     }
 }
 
+// The value at element idx of a buffer stored in S, as T, or 0 when valid is false.
+template <class T, class S>
+__device__ __forceinline__ T buffer_value_as(__amdgpu_buffer_rsrc_t r, bool valid, unsigned idx) {
+    if constexpr (sizeof(S) == 2) {
+        const int off = valid ? (int)(idx * 2u) : (int)0x80000000u;
+        return T(bf16_value((bf16_t)__builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0)));
+    } else {
+        return T(buffer_value<S>(r, valid, idx));
+    }
+}
+
 // The largest grid (nodes) the marching kernels take: byte offsets of 31 bits.
 constexpr size_t MARCH_MAX_BYTES = size_t(1) << 31;
 
@@ -100,8 +111,8 @@ __device__ __forceinline__ void xcd_tile(unsigned& bx, unsigned& by) {
     by = t / gridDim.x;
 }
 
-template <class T, int MODE, bool XCD = true, bool SHFL = MarchShape<T>::shuffle, int KZ = MarchShape<T>::kz>
-__global__ void __launch_bounds__(BLOCK) k_march_hex8(MarchArgs<T> a) {
+template <class T, int MODE, bool XCD = true, bool SHFL = MarchShape<T>::shuffle, int KZ = MarchShape<T>::kz, class S = T>
+__global__ void __launch_bounds__(BLOCK) k_march_hex8(MarchArgs<T, S> a) {
     const GridDims g = a.g;
     unsigned bx = blockIdx.x, by = blockIdx.y;
     if (XCD) xcd_tile(bx, by);
@@ -180,7 +191,7 @@ __global__ void __launch_bounds__(BLOCK) k_march_hex8(MarchArgs<T> a) {
             T mid[2];
 #pragma unroll
             for (int cy = 0; cy < 2; ++cy)
-                mid[cy] = buffer_value<T>(rk, vk && cj[cy] && i < ncx, base + (unsigned)cj_[cy] * ncx + i);
+                mid[cy] = buffer_value_as<T, S>(rk, vk && cj[cy] && i < ncx, base + (unsigned)cj_[cy] * ncx + i);
 #pragma unroll
             for (int cy = 0; cy < 2; ++cy) {
                 const T l = lane_value(mid[cy], srcl);
@@ -193,8 +204,8 @@ __global__ void __launch_bounds__(BLOCK) k_march_hex8(MarchArgs<T> a) {
 #pragma unroll
                 for (int cx = 0; cx < 2; ++cx) {
                     const int p = i - 1 + cx;
-                    out[cx + 2 * cy] = buffer_value<T>(rk, vk && cj[cy] && p >= 0 && p < ncx,
-                                                       base + (unsigned)cj_[cy] * ncx + p);
+                    out[cx + 2 * cy] = buffer_value_as<T, S>(rk, vk && cj[cy] && p >= 0 && p < ncx,
+                                                             base + (unsigned)cj_[cy] * ncx + p);
                 }
         }
     };
@@ -241,7 +252,7 @@ __global__ void __launch_bounds__(BLOCK) k_march_hex8(MarchArgs<T> a) {
                     a.y[u] = T(0);
                 } else {
                     const T r = a.b[u] - acc;
-                    const T dn = a.c1 * a.d[u] + a.c2 * a.invd[u] * r;
+                    const T dn = a.c1 * a.d[u] + a.c2 * ld<T>(a.invd, u) * r;
                     a.d[u] = dn;
                     a.y[u] = xc + dn;
                 }

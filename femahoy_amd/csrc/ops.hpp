@@ -21,6 +21,33 @@ struct GridDims {
     __host__ __device__ size_t nodes() const { return (size_t)nx * ny * nz; }
 };
 
+// BF16 storage: the upper 16 bits of an FP32 value, rounded to nearest even. Arrays of
+// coefficients and Jacobi diagonals may be stored so (uint16_t) and widened to FP32 on load.
+typedef uint16_t bf16_t;
+
+__host__ __device__ inline bf16_t bf16_round(float v) {
+    uint32_t u;
+    __builtin_memcpy(&u, &v, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (bf16_t)(u >> 16);
+}
+__host__ __device__ inline float bf16_value(bf16_t b) {
+    const uint32_t u = (uint32_t)b << 16;
+    float v;
+    __builtin_memcpy(&v, &u, 4);
+    return v;
+}
+
+// Element i of an array stored in S, as T.
+template <class T, class S>
+__device__ __forceinline__ T ld(const S* p, size_t i) {
+    if constexpr (sizeof(S) == 2) {
+        return T(bf16_value(p[i]));
+    } else {
+        return T(p[i]);
+    }
+}
+
 __device__ __forceinline__ bool is_fixed(const GridDims& g, int i, int j, int k) {
     return ((g.fix & 1u) && i == 0) || ((g.fix & 2u) && i == g.nx - 1) || ((g.fix & 4u) && j == 0) ||
            ((g.fix & 8u) && j == g.ny - 1) || ((g.fix & 16u) && k == 0) || ((g.fix & 32u) && k == g.nz - 1);
@@ -30,15 +57,15 @@ __device__ __forceinline__ bool is_fixed(const GridDims& g, int i, int j, int k)
 // The 8 x 8 conduction matrix of a box cell (heat.rs:1246-1278) depends only on which axes the
 // two corners differ along: entry (p, q) = k * c[(p xor q) as bits x|y<<1|z<<2]. The corner bits
 // here are (x, y, z) in {0,1}^3, not the VTK order; the matrix is the same operator.
-template <class T>
+template <class T, class S = T>
 struct Hex8K {
     GridDims g;
-    const T* k;  // per cell, (nx-1)(ny-1)(nz-1), x fastest
+    const S* k;  // per cell, (nx-1)(ny-1)(nz-1), x fastest
     T c[8];
 
     __device__ __forceinline__ T cell_k(int ex, int ey, int ez) const {
         if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx - 1 || ey >= g.ny - 1 || ez >= g.nz - 1) return T(0);
-        return k[ex + (size_t)(g.nx - 1) * (ey + (size_t)(g.ny - 1) * ez)];
+        return ld<T>(k, ex + (size_t)(g.nx - 1) * (ey + (size_t)(g.ny - 1) * ez));
     }
 
     // Branch-free: addresses clamped into the grid, out-of-grid values selected to 0.
@@ -76,7 +103,7 @@ struct Hex8K {
                     const bool v = ex >= 0 && ey >= 0 && ez >= 0 && ex < ncx && ey < ncy && ez < ncz;
                     const int cxx = min(max(ex, 0), max(ncx - 1, 0)), cyy = min(max(ey, 0), max(ncy - 1, 0)),
                               czz = min(max(ez, 0), max(ncz - 1, 0));
-                    const T kval = k[cxx + (size_t)ncx * (cyy + (size_t)ncy * czz)];
+                    const T kval = ld<T>(k, cxx + (size_t)ncx * (cyy + (size_t)ncy * czz));
                     kv[cx + 2 * cy + 4 * cz] = v ? kval : T(0);
                 }
         T acc = T(0);
@@ -195,10 +222,10 @@ __device__ __forceinline__ void kuhn_grad(int t, int v, const T ih[3], T g[3]) {
 
 // Scalar P1 conduction on Kuhn tetrahedra (heat.rs:972-1000): k per tetrahedron, stored
 // coef[t * cells + cell] for type t.
-template <class T>
+template <class T, class S = T>
 struct Tet4Heat {
     GridDims g;
-    const T* coef;
+    const S* coef;
     T ih[3];
     T vol;
 
@@ -238,7 +265,7 @@ struct Tet4Heat {
                         }
                         T gm[3];
                         kuhn_grad<T>(t, me, ih, gm);
-                        acc += coef[t * cells + cell] * vol * (gm[0] * gu[0] + gm[1] * gu[1] + gm[2] * gu[2]);
+                        acc += ld<T>(coef, t * cells + cell) * vol * (gm[0] * gu[0] + gm[1] * gu[1] + gm[2] * gu[2]);
                     }
                 }
         return acc;
@@ -258,7 +285,7 @@ struct Tet4Heat {
                         if (me < 0) continue;
                         T gm[3];
                         kuhn_grad<T>(t, me, ih, gm);
-                        acc += coef[t * cells + cell] * vol * (gm[0] * gm[0] + gm[1] * gm[1] + gm[2] * gm[2]);
+                        acc += ld<T>(coef, t * cells + cell) * vol * (gm[0] * gm[0] + gm[1] * gm[1] + gm[2] * gm[2]);
                     }
                 }
         return acc;
@@ -268,10 +295,10 @@ struct Tet4Heat {
 // Isotropic linear elasticity on Kuhn tetrahedra, constant strain (the operator of
 // tet10.rs:90-120 with constant gradients): lambda and mu per tetrahedron, stored
 // coef[(0 * 6 + t) * cells + cell] (lambda) and coef[(1 * 6 + t) * cells + cell] (mu).
-template <class T>
+template <class T, class S = T>
 struct Tet4Elastic {
     GridDims g;
-    const T* coef;
+    const S* coef;
     T ih[3];
     T vol;
 
@@ -309,8 +336,8 @@ struct Tet4Elastic {
                                 for (int b = 0; b < 3; ++b) G[a][b] += u * gv[b];
                             }
                         }
-                        const T lam = coef[(0 * 6 + t) * cells + cell];
-                        const T mu = coef[(1 * 6 + t) * cells + cell];
+                        const T lam = ld<T>(coef, (0 * 6 + t) * cells + cell);
+                        const T mu = ld<T>(coef, (1 * 6 + t) * cells + cell);
                         const T tr = G[0][0] + G[1][1] + G[2][2];
                         T gm[3];
                         kuhn_grad<T>(t, me, ih, gm);
@@ -342,8 +369,8 @@ struct Tet4Elastic {
                         if (me < 0) continue;
                         T gm[3];
                         kuhn_grad<T>(t, me, ih, gm);
-                        const T lam = coef[(0 * 6 + t) * cells + cell];
-                        const T mu = coef[(1 * 6 + t) * cells + cell];
+                        const T lam = ld<T>(coef, (0 * 6 + t) * cells + cell);
+                        const T mu = ld<T>(coef, (1 * 6 + t) * cells + cell);
                         const T g2 = gm[0] * gm[0] + gm[1] * gm[1] + gm[2] * gm[2];
                         for (int a = 0; a < 3; ++a) out[a] += vol * ((lam + mu) * gm[a] * gm[a] + mu * g2);
                     }

@@ -17,6 +17,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/fem_solver.h"
@@ -68,6 +69,10 @@ struct Level {
     bool kuhn = false;         // transfer to the next level is Kuhn P1 (else trilinear)
     DArray<T> coef;
     DArray<T> invd, b, xa, xb, d, r;
+    // BF16 storage (FP32 levels only): the per-cell coefficients (not the Galerkin stencils) and D^-1
+    // held as BF16 values, in 16 bits for the kernels and as their FP32 values in coef and invd
+    bool bf16 = false;
+    DArray<bf16_t> coef16, invd16;
     // tet10: the partial sums of the fused kernels (tet10_fused.hpp, 8 slots per unknown) and the
     // nodes their boundary launch finishes, for the tile of T
     DArray<T> P;
@@ -102,32 +107,81 @@ void hex_c(const double h[3], T c[8]) {
     }
 }
 
+// The element type of a pointer.
+template <class P>
+using elem_t = std::remove_const_t<std::remove_pointer_t<P>>;
+
+// Calls f with the level's operator, whose per-cell coefficients are read in their stored type:
+// T, or BF16 on a level of an FP32 hierarchy in BF16 storage (Galerkin stencils stay in T).
 template <class T, class F>
 void with_op(Level<T>& L, F&& f) {
     const T ih[3] = {T(1.0 / L.h[0]), T(1.0 / L.h[1]), T(1.0 / L.h[2])};
     const T vol = T(L.h[0] * L.h[1] * L.h[2] / 6.0);
-    switch (L.op) {
-        case OpKind::Hex8K: {
-            Hex8K<T> o{L.g, L.coef.data(), {}};
-            hex_c<T>(L.h, o.c);
-            f(ScalarOp<Hex8K<T>>{o});
-            break;
+    auto build = [&](auto cp) {
+        using S = elem_t<decltype(cp)>;
+        switch (L.op) {
+            case OpKind::Hex8K: {
+                Hex8K<T, S> o{L.g, cp, {}};
+                hex_c<T>(L.h, o.c);
+                f(ScalarOp<Hex8K<T, S>>{o});
+                break;
+            }
+            case OpKind::Stencil27:
+                f(ScalarOp<Stencil27<T>>{Stencil27<T>{L.g, L.coef.data()}});
+                break;
+            case OpKind::Tet4Heat:
+                f(ScalarOp<Tet4Heat<T, S>>{Tet4Heat<T, S>{L.g, cp, {ih[0], ih[1], ih[2]}, vol}});
+                break;
+            case OpKind::Tet4Elastic:
+                f(VectorOp<Tet4Elastic<T, S>>{Tet4Elastic<T, S>{L.g, cp, {ih[0], ih[1], ih[2]}, vol}});
+                break;
+            case OpKind::Tet10:
+                throw Error("internal: the tet10 operator runs through its fused kernels");
         }
-        case OpKind::Stencil27:
-            f(ScalarOp<Stencil27<T>>{Stencil27<T>{L.g, L.coef.data()}});
-            break;
-        case OpKind::Tet4Heat:
-            f(ScalarOp<Tet4Heat<T>>{Tet4Heat<T>{L.g, L.coef.data(), {ih[0], ih[1], ih[2]}, vol}});
-            break;
-        case OpKind::Tet4Elastic:
-            f(VectorOp<Tet4Elastic<T>>{Tet4Elastic<T>{L.g, L.coef.data(), {ih[0], ih[1], ih[2]}, vol}});
-            break;
-        case OpKind::Tet10:
-            throw Error("internal: the tet10 operator runs through its fused kernels");
+    };
+    if constexpr (std::is_same<T, float>::value) {
+        if (L.bf16) {
+            build((const bf16_t*)L.coef16.data());
+            return;
+        }
     }
+    build((const T*)L.coef.data());
 }
 
-// The column-marching arguments of a hex8 level with a coefficient per cell.
+// Calls f with the level's D^-1 in its stored type (T, or BF16).
+template <class T, class F>
+void with_invd(Level<T>& L, F&& f) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (L.bf16) {
+            f((const bf16_t*)L.invd16.data());
+            return;
+        }
+    }
+    f((const T*)L.invd.data());
+}
+
+// Calls f with the column-marching arguments of a hex8 level (a coefficient per cell), its
+// coefficients and D^-1 in their stored type.
+template <class T, class F>
+void with_march(Level<T>& L, F&& f) {
+    auto go = [&](auto kp, auto ip) {
+        MarchArgs<T, elem_t<decltype(kp)>> m{};
+        m.g = L.g;
+        m.k = kp;
+        m.invd = ip;
+        hex_c<T>(L.h, m.c);
+        f(m);
+    };
+    if constexpr (std::is_same<T, float>::value) {
+        if (L.bf16) {
+            go((const bf16_t*)L.coef16.data(), (const bf16_t*)L.invd16.data());
+            return;
+        }
+    }
+    go((const T*)L.coef.data(), (const T*)L.invd.data());
+}
+
+// The column-marching arguments of an FP64 hex8 level (the outer operator).
 template <class T>
 MarchArgs<T> march_args(Level<T>& L) {
     MarchArgs<T> m{};
@@ -137,16 +191,18 @@ MarchArgs<T> march_args(Level<T>& L) {
     return m;
 }
 
-template <class T, int MODE>
-void launch_march(const MarchArgs<T>& m, hipStream_t s) {
-    hipLaunchKernelGGL((k_march_hex8<T, MODE>), march_grid<T>(m.g), dim3(BLOCK), 0, s, m);
+template <class T, int MODE, class S>
+void launch_march(const MarchArgs<T, S>& m, hipStream_t s) {
+    hipLaunchKernelGGL((k_march_hex8<T, MODE, true, MarchShape<T>::shuffle, MarchShape<T>::kz, S>), march_grid<T>(m.g),
+                       dim3(BLOCK), 0, s, m);
 }
 
-// The arguments of the fused tet10 kernels of a level (tet10_fused.hpp).
-template <class T>
-T10Args<T> t10_args(const Level<T>& L) {
+// The arguments of the fused tet10 kernels of a level (tet10_fused.hpp), with its Lame
+// coefficients and D^-1 stored in S.
+template <class T, class S = T>
+T10Args<T, S> t10_args(const Level<T>& L) {
     using TL = T10Tile<T>;
-    T10Args<T> a{};
+    T10Args<T, S> a{};
     a.g = L.g;
     a.cx = L.cells[0];
     a.cy = L.cells[1];
@@ -154,7 +210,13 @@ T10Args<T> t10_args(const Level<T>& L) {
     a.tx = (a.cx + TL::X - 1) / TL::X;
     a.ty = (a.cy + TL::Y - 1) / TL::Y;
     a.tz = (a.cz + TL::Z - 1) / TL::Z;
-    a.coef = L.coef.data();
+    if constexpr (sizeof(S) == 2) {
+        a.coef = L.coef16.data();
+        a.invd = L.invd16.data();
+    } else {
+        a.coef = L.coef.data();
+        a.invd = L.invd.data();
+    }
     a.ih0 = T(1.0 / L.h[0]);
     a.ih1 = T(1.0 / L.h[1]);
     a.ih2 = T(1.0 / L.h[2]);
@@ -183,25 +245,38 @@ void t10_setup(Level<T>& L, hipStream_t s) {
 }
 
 // The two launches of a fused tet10 mode; returns the number of block partials (PCGQ).
-template <int MODE, class T>
-unsigned t10_run(T10Args<T> a, hipStream_t s) {
+template <int MODE, class T, class S>
+unsigned t10_run(T10Args<T, S> a, hipStream_t s) {
     using TL = T10Tile<T>;
     const unsigned n1 = (unsigned)a.tx * a.ty * a.tz;
     const unsigned n2 = (unsigned)((a.nb + BLOCK - 1) / BLOCK);
     a.nb1 = (int)n1;
-    hipLaunchKernelGGL((k_t10_tile<T, TL::X, TL::Y, TL::Z, MODE>), dim3(n1), dim3(TL::X * TL::Y * TL::Z), 0, s, a);
-    if (n2) hipLaunchKernelGGL((k_t10_bound<T, TL::X, TL::Y, TL::Z, MODE>), dim3(n2), dim3(BLOCK), 0, s, a);
+    hipLaunchKernelGGL((k_t10_tile<T, TL::X, TL::Y, TL::Z, MODE, S>), dim3(n1), dim3(TL::X * TL::Y * TL::Z), 0, s, a);
+    if (n2) hipLaunchKernelGGL((k_t10_bound<T, TL::X, TL::Y, TL::Z, MODE, S>), dim3(n2), dim3(BLOCK), 0, s, a);
     return n1 + n2;
+}
+
+// Calls f with the fused tet10 arguments of a level in its storage type.
+template <class T, class F>
+void with_t10(const Level<T>& L, F&& f) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (L.bf16) {
+            f(t10_args<T, bf16_t>(L));
+            return;
+        }
+    }
+    f(t10_args<T>(L));
 }
 
 // y = A x on a level (0 on fixed nodes).
 template <class T>
 void op_apply(Level<T>& L, const T* x, T* y, hipStream_t s) {
     if (L.op == OpKind::Tet10) {
-        T10Args<T> a = t10_args(L);
-        a.x = x;
-        a.y = y;
-        t10_run<T10_APPLY>(a, s);
+        with_t10(L, [&](auto a) {
+            a.x = x;
+            a.y = y;
+            t10_run<T10_APPLY>(a, s);
+        });
         return;
     }
     with_op(L, [&](auto op) { launch((k_apply<decltype(op), T>), L.nodes(), s, op, x, y); });
@@ -211,11 +286,12 @@ void op_apply(Level<T>& L, const T* x, T* y, hipStream_t s) {
 template <class T>
 void op_residual(Level<T>& L, const T* x, const T* b, T* r, hipStream_t s) {
     if (L.op == OpKind::Tet10) {
-        T10Args<T> a = t10_args(L);
-        a.x = x;
-        a.b = b;
-        a.y = r;
-        t10_run<T10_RESID>(a, s);
+        with_t10(L, [&](auto a) {
+            a.x = x;
+            a.b = b;
+            a.y = r;
+            t10_run<T10_RESID>(a, s);
+        });
         return;
     }
     with_op(L, [&](auto op) { launch((k_residual<decltype(op), T>), L.nodes(), s, op, x, b, r); });
@@ -312,6 +388,7 @@ class Hierarchy {
     std::vector<Level<T>> lv;
     DenseCoarse coarse;
     int degree = 3;
+    bool bf16 = false;  // BF16 storage of the levels' coefficients and D^-1 (FP32 hierarchies)
     Profiler* prof = nullptr;
 
     void mark_begin(size_t l, hipStream_t s) {
@@ -332,7 +409,9 @@ class Hierarchy {
         }
         const size_t n = L.dofs();
         // pre-smoothing from x = 0
-        launch(k_cheb_first<T>, n, s, n, (const T*)L.b.data(), (const T*)L.invd.data(), L.x(), L.d.data(), L.c2[0]);
+        with_invd(L, [&](auto iv) {
+            launch(k_cheb_first<T, elem_t<decltype(iv)>>, n, s, n, (const T*)L.b.data(), iv, L.x(), L.d.data(), L.c2[0]);
+        });
         for (int st = 1; st < degree; ++st) {
             mark_begin(l, s);
             cheb_step(L, L.c1[st], L.c2[st], s);
@@ -340,11 +419,12 @@ class Hierarchy {
             L.cur ^= 1;
         }
         if (L.march()) {
-            MarchArgs<T> m = march_args(L);
-            m.x = L.x();
-            m.b = L.b.data();
-            m.y = L.r.data();
-            launch_march<T, MARCH_RESIDUAL>(m, s);
+            with_march(L, [&](auto m) {
+                m.x = L.x();
+                m.b = L.b.data();
+                m.y = L.r.data();
+                launch_march<T, MARCH_RESIDUAL>(m, s);
+            });
         } else {
             op_residual(L, (const T*)L.x(), (const T*)L.b.data(), L.r.data(), s);
         }
@@ -363,32 +443,34 @@ class Hierarchy {
     // d = c1 d + c2 D^-1 (b - A x), x2 = x + d.
     void cheb_step(Level<T>& L, T c1, T c2, hipStream_t s) {
         if (L.march()) {
-            MarchArgs<T> m = march_args(L);
-            m.x = L.x();
-            m.y = L.x2();
-            m.d = L.d.data();
-            m.b = L.b.data();
-            m.invd = L.invd.data();
-            m.c1 = c1;
-            m.c2 = c2;
-            launch_march<T, MARCH_CHEB>(m, s);
+            with_march(L, [&](auto m) {
+                m.x = L.x();
+                m.y = L.x2();
+                m.d = L.d.data();
+                m.b = L.b.data();
+                m.c1 = c1;
+                m.c2 = c2;
+                launch_march<T, MARCH_CHEB>(m, s);
+            });
             return;
         }
         if (L.op == OpKind::Tet10) {
-            T10Args<T> a = t10_args(L);
-            a.x = L.x();
-            a.y = L.x2();
-            a.d = L.d.data();
-            a.b = L.b.data();
-            a.invd = L.invd.data();
-            a.c1 = c1;
-            a.c2 = c2;
-            t10_run<T10_CHEB>(a, s);
+            with_t10(L, [&](auto a) {
+                a.x = L.x();
+                a.y = L.x2();
+                a.d = L.d.data();
+                a.b = L.b.data();
+                a.c1 = c1;
+                a.c2 = c2;
+                t10_run<T10_CHEB>(a, s);
+            });
             return;
         }
         with_op(L, [&](auto op) {
-            launch((k_cheb<decltype(op), T>), L.nodes(), s, op, (const T*)L.x(), L.x2(), L.d.data(), (const T*)L.b.data(),
-                   (const T*)L.invd.data(), c1, c2);
+            with_invd(L, [&](auto iv) {
+                launch((k_cheb<decltype(op), T, elem_t<decltype(iv)>>), L.nodes(), s, op, (const T*)L.x(), L.x2(),
+                       L.d.data(), (const T*)L.b.data(), iv, c1, c2);
+            });
         });
     }
 
@@ -494,6 +576,7 @@ class Solver {
                 build_hierarchy(*h64_);
             } else {
                 h32_ = std::make_unique<Hierarchy<float>>();
+                h32_->bf16 = o.precision == FEM_PREC_BF16;
                 build_hierarchy(*h32_);
                 r32_.resize(n0_);
             }
@@ -531,10 +614,17 @@ class Solver {
         p->dofs = n0_;
         const Level<double>& L = lv64_[0];
         p->outer_bytes = 16 * n0_ + L.coef.bytes();
-        if (h32_) p->smoother_bytes = 6 * n0_ * sizeof(float) + h32_->lv[0].coef.bytes();
+        if (h32_) p->smoother_bytes = smoother_bytes32(h32_->lv[0]);
         else if (h64_) p->smoother_bytes = 6 * n0_ * sizeof(double) + h64_->lv[0].coef.bytes();
         else p->smoother_bytes = 0;
     }
+    // Algorithmic bytes of one FP32 Chebyshev step on level 0: x, b, d in and d, x out, D^-1 and the
+    // coefficients in their stored type.
+    uint64_t smoother_bytes32(const Level<float>& L) const {
+        if (L.bf16) return 5 * n0_ * sizeof(float) + n0_ * sizeof(bf16_t) + L.coef16.bytes();
+        return 6 * n0_ * sizeof(float) + L.coef.bytes();
+    }
+
     const BoxModel& model() const { return model_; }
     Options& options() { return opt_; }
     const char* device_name() const { return device_name_.c_str(); }
@@ -741,6 +831,7 @@ class Solver {
         for (;;) {
             const uint64_t lc = (uint64_t)c[0] * c[1] * c[2];
             bytes += 6 * dofs * vb;
+            if (o.precision == FEM_PREC_BF16) bytes += dofs * 2 + coef64(kind, lc) / 4;
             if (lev > 0) {
                 const uint64_t nodes = (uint64_t)(c[0] + 1) * (c[1] + 1) * (c[2] + 1);
                 const uint64_t cf = kind == Kind::HeatHex8 && o.coarse_galerkin ? 14 * nodes * 8 + 64 * lc * 8 : coef64(kind, lc);
@@ -1102,7 +1193,24 @@ class Solver {
             L.r.resize(n);
             for (T* v : {L.b.data(), L.xa.data(), L.xb.data(), L.d.data(), L.r.data()}) launch(k_fill<T>, n, s_, n, v, T(0));
             if (l + 1 < lv64_.size()) {
+                if constexpr (std::is_same<T, float>::value) {
+                    // BF16 storage: the coefficients are rounded before D^-1 and the spectrum
+                    // estimate are taken, so that both see the operator the smoother applies.
+                    if (H.bf16) {
+                        L.bf16 = true;
+                        if (L.op != OpKind::Stencil27) {
+                            L.coef16.resize(L.coef.size());
+                            launch(k_round_bf16, L.coef.size(), s_, L.coef.size(), L.coef.data(), L.coef16.data());
+                        }
+                    }
+                }
                 op_inv_diag(L, s_);
+                if constexpr (std::is_same<T, float>::value) {
+                    if (H.bf16) {
+                        L.invd16.resize(n);
+                        launch(k_round_bf16, n, s_, n, L.invd.data(), L.invd16.data());
+                    }
+                }
                 L.lmax = estimate_lmax(L);
                 chebyshev(L);
             }
@@ -1280,7 +1388,7 @@ class Solver {
             Level<float>& L = H.lv[0];
             const size_t st = L.c1.size() > 1 ? 1 : 0;
             out.smoother_seconds = timed([&] { H.cheb_step(L, L.c1[st], L.c2[st], s_); L.cur ^= 1; });
-            out.smoother_bytes = 6 * n0_ * sizeof(float) + L.coef.bytes();
+            out.smoother_bytes = smoother_bytes32(L);
         } else if (h64_ && !h64_->lv[0].c1.empty()) {
             auto& H = *h64_;
             Level<double>& L = H.lv[0];
@@ -1728,7 +1836,6 @@ fem::Options options_from(const fem_options& opt) {
     o.device = opt.device;
     o.coarse_galerkin = opt.coarse_galerkin;
     if (o.precision < 0 || o.precision > 2) throw fem::Error("unknown precision");
-    if (o.precision == FEM_PREC_BF16) o.precision = FEM_PREC_FP32;  // BF16 coefficients: not in this build
     return o;
 }
 

@@ -44,19 +44,19 @@ struct T10Tile<double> {
     static constexpr int X = 16, Y = 2, Z = 4;
 };
 
-template <class T>
+template <class T, class S = T>
 struct T10Args {
     GridDims g;          // the refined grid (2 c + 1 nodes per axis) and its fixed faces
     int cx, cy, cz;      // cells
     int tx, ty, tz;      // tiles per axis
-    const T* coef;       // lambda at [t * cells + cell], mu at [(6 + t) * cells + cell]
+    const S* coef;       // lambda at [t * cells + cell], mu at [(6 + t) * cells + cell], stored in S (T, or BF16)
     T ih0, ih1, ih2;     // inverse cell sizes
     T w;                 // quadrature weight: cell volume / 6 / 4
     const T* x;          // operand (p for PCGQ)
     T* y;                // y, r, x_out or q
     const T* b;
     T* d;
-    const T* invd;
+    const S* invd;       // D^-1, stored in S
     T c1, c2;
     double* part;        // PCGQ: block partials of x.y, the tile launch first, then the boundary launch
     T* P;                // partial sums: P[slot * 3 N + 3 node + c] over the N nodes, 8 slots
@@ -85,18 +85,18 @@ __host__ __device__ inline bool t10_on_face(int i, int tn, int n) { return i > 0
 
 // The value at element idx of p when valid, else 0, without a branch (a load of element 0 in its
 // place).
-template <class T>
-__device__ __forceinline__ T t10_load(const T* p, bool valid, unsigned idx) {
-    const T v = p[valid ? idx : 0u];
+template <class T, class S>
+__device__ __forceinline__ T t10_load(const S* p, bool valid, unsigned idx) {
+    const T v = ld<T>(p, valid ? idx : 0u);
     return valid ? v : T(0);
 }
 
 // The 3 components of a node (unknowns u0 .. u0 + 2) when valid, else 0, branch-free; one base for
 // the 3, so that they load as one vector access.
-template <class T>
-__device__ __forceinline__ void t10_load3(const T* p, bool valid, unsigned u0, T v[3]) {
-    const T* q = p + (valid ? u0 : 0u);
-    const T a = q[0], b = q[1], c = q[2];
+template <class T, class S>
+__device__ __forceinline__ void t10_load3(const S* p, bool valid, unsigned u0, T v[3]) {
+    const S* q = p + (valid ? u0 : 0u);
+    const T a = ld<T>(q, 0), b = ld<T>(q, 1), c = ld<T>(q, 2);
     v[0] = valid ? a : T(0);
     v[1] = valid ? b : T(0);
     v[2] = valid ? c : T(0);
@@ -104,8 +104,8 @@ __device__ __forceinline__ void t10_load3(const T* p, bool valid, unsigned u0, T
 
 // The update of one node (unknowns u0 .. u0 + 2) of the mode from its operator value yv, its
 // operand xin and (CHEB, RESID) its b, d and D^-1 already loaded; pq gathers x.q (PCGQ).
-template <class T, int MODE>
-__device__ __forceinline__ void t10_update(const T10Args<T>& a, unsigned u0, bool fixed, const T yv[3], const T xin[3],
+template <class T, int MODE, class S>
+__device__ __forceinline__ void t10_update(const T10Args<T, S>& a, unsigned u0, bool fixed, const T yv[3], const T xin[3],
                                            const T bv[3], const T dv[3], const T iv[3], double& pq) {
     T out[3], dn[3];
 #pragma unroll
@@ -181,8 +181,8 @@ __device__ __forceinline__ void t10_exchange(T F[27][3], T* xch, int tid, bool s
 
 // FP32 asks for 2 waves per SIMD (at most 256 registers per lane), which two tiles of LDS per CU
 // allow; FP64 takes 1. Index arithmetic is 32-bit: the host keeps every byte offset below 2^31.
-template <class T, int TX, int TY, int TZ, int MODE>
-__global__ void __launch_bounds__(TX * TY * TZ, sizeof(T) == 4 ? 2 : 1) k_t10_tile(T10Args<T> a) {
+template <class T, int TX, int TY, int TZ, int MODE, class S = T>
+__global__ void __launch_bounds__(TX * TY * TZ, sizeof(T) == 4 ? 2 : 1) k_t10_tile(T10Args<T, S> a) {
     constexpr int NT = TX * TY * TZ, NW = NT / 64;
     constexpr int LX = 2 * TX + 1, LY = 2 * TY + 1, LZ = 2 * TZ + 1;
     constexpr int RS = 3 * LX;  // values per box row
@@ -204,8 +204,8 @@ __global__ void __launch_bounds__(TX * TY * TZ, sizeof(T) == 4 ? 2 : 1) k_t10_ti
         const unsigned cell = ex + (unsigned)a.cx * (ey + (unsigned)a.cy * ez);
 #pragma unroll
         for (int t = 0; t < 6; ++t) {
-            lam[t] = t10_load(a.coef, valid, t * cells + cell);
-            mu[t] = t10_load(a.coef, valid, (6 + t) * cells + cell);
+            lam[t] = t10_load<T>(a.coef, valid, t * cells + cell);
+            mu[t] = t10_load<T>(a.coef, valid, (6 + t) * cells + cell);
         }
     }
     // 1. the box of x, zero beyond the grid: wave w takes rows w, w + NW, ...; a row is RS
@@ -319,8 +319,8 @@ __global__ void __launch_bounds__(TX * TY * TZ, sizeof(T) == 4 ? 2 : 1) k_t10_ti
 }
 
 // The boundary nodes: the sum of their slots in slot order, then the mode's update.
-template <class T, int TX, int TY, int TZ, int MODE>
-__global__ void __launch_bounds__(BLOCK) k_t10_bound(T10Args<T> a) {
+template <class T, int TX, int TY, int TZ, int MODE, class S = T>
+__global__ void __launch_bounds__(BLOCK) k_t10_bound(T10Args<T, S> a) {
     const int e = blockIdx.x * BLOCK + threadIdx.x;
     double pq = 0.0;
     if (e < a.nb) {

@@ -23,7 +23,7 @@ def heat_case(n, seed=1, tet4=False):
     return k, {"x0": 0.0, "x1": 1.0}
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("precision", ["fp32", "fp64", "bf16"])
 @pytest.mark.parametrize("n", [[8, 8, 8], [16, 16, 16], [16, 8, 4]])
 def test_heat_hex8_multigrid_matches_oracle_direct(fem, oracle, n, precision):
     k, fixed = heat_case(n)
@@ -76,7 +76,7 @@ def test_tets_apply_matches_oracle_matrix(fem, oracle, kind, n):
     assert rel(s.rhs(), pr.rhs()) < 1e-12
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("precision", ["fp32", "fp64", "bf16"])
 @pytest.mark.parametrize("kind,n", [("heat_tet4", [8, 8, 8]), ("elastic_tet4", [8, 4, 4]), ("elastic_tet10", [8, 4, 4]),
                                     ("elastic_tet10", [16, 12, 4]), ("elastic_tet10", [34, 6, 4])])
 def test_tets_multigrid_matches_oracle_direct(fem, oracle, kind, n, precision):
@@ -176,6 +176,37 @@ def test_fused_tet10_v_cycle_fp32_agrees_with_fp64(fem):
     z32, z64 = s32.precondition(r), s64.precondition(r)
     assert rel(z32, z64) < 1e-5
     assert rel(s32.apply(r), s64.apply(r)) == 0.0  # both apply the FP64 level-0 operator
+
+
+@pytest.mark.parametrize("kind", ["heat_hex8", "heat_tet4", "elastic_tet10"])
+def test_bf16_storage_is_the_fp32_v_cycle_with_rounded_coefficients(fem, kind):
+    """precision="bf16" stores the per-cell coefficients and D^-1 in 16 bits: its preconditioner
+    differs from the FP32 one by the rounding of those (relative 2^-9 each), not more, and the
+    outer FP64 PCG still reaches the tolerance; the stats report the storage."""
+    rng = np.random.default_rng(17)
+    if kind == "heat_hex8":
+        n = [16, 16, 16]
+        box = fem.Box(kind, n, [1, 1, 1], np.exp(rng.standard_normal(int(np.prod(n)))), fixed={"x0": 0.0, "x1": 1.0})
+    elif kind == "heat_tet4":
+        n = [16, 8, 8]
+        box = fem.Box(kind, n, [2, 1, 1], np.exp(rng.standard_normal(6 * int(np.prod(n)))), fixed={"x0": 0.0, "x1": 1.0})
+    else:
+        n = [36, 10, 6]
+        nt = 6 * int(np.prod(n))
+        box = fem.Box(kind, n, [3.6, 1.0, 0.6], rng.uniform(1.0, 2.0, nt), np.full(nt, 0.3), fixed={"x0": True},
+                      body=(0, 0, -1))
+    s32 = fem.Solver(box, method="multigrid", precision="fp32", tol=1e-10)
+    s16 = fem.Solver(box, method="multigrid", precision="bf16", tol=1e-10)
+    r = rng.standard_normal(s32.n)
+    z32, z16 = s32.precondition(r), s16.precondition(r)
+    d = rel(z16, z32)
+    assert 1e-6 < d < 3e-2, d
+    b = s32.rhs()
+    x32, st32 = s32.solve(b)
+    x16, st16 = s16.solve(b)
+    assert st16.precision == "bf16" and st32.precision == "fp32"
+    assert st16.true_residual <= 1e-10
+    assert rel(x16, x32) < 1e-8
 
 
 @pytest.mark.parametrize("kind", ["heat_hex8", "elastic_tet10"])
