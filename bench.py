@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 METRIC = "MDOF/s solved to 1e-8 rel. residual, mixed vs FP64 MG-PCG; p50/p99 latency"
+SETUP_THREADS = 8  # host threads of the serving loop that build admitted solves (half the box's 16-core share)
 
 
 def log(*a):
@@ -165,7 +166,7 @@ def serve_requests(F, n_solves, seed):
 def run_stream(F, reqs, arrivals, budget, max_active):
     """Submits reqs at their arrival times (seconds from the start; None = all at once) and waits
     for all; returns (makespan s, latencies s, per-request stats)."""
-    srv = F.Server(hbm_budget=budget, max_active=max_active, setup_threads=4)
+    srv = F.Server(hbm_budget=budget, max_active=max_active, setup_threads=SETUP_THREADS)
     t0 = time.perf_counter()
     ids = []
     opts = dict(method="multigrid", precision="fp32", tol=1e-8)
@@ -205,7 +206,7 @@ def serving(F, n_solves, seed):
         "workload": (f"{len(reqs)} solves in refinement pairs (h, h/2), heat tet4 or elasticity tet10 (50/50), "
                      f"{min(r[4] for r in reqs)}-{max(r[4] for r in reqs)} DOF, Poisson pair arrivals at 80% of the "
                      f"measured capacity (seed {seed}), mixed precision, tol 1e-8, HBM budget {budget >> 30} GiB, "
-                     f"{max_active} solves admitted at once"),
+                     f"{max_active} solves admitted at once, {SETUP_THREADS} setup threads"),
         "solves": len(reqs), "total_dof": total_dof,
         "capacity_solves_s": round(capacity, 2), "capacity_mdof_s": round(total_dof / cap_span / 1e6, 2),
         "arrival_rate_solves_s": round(rate, 2),

@@ -21,22 +21,38 @@
 
 namespace fem {
 
-std::array<std::array<int, 4>, 3> prism_split(const std::array<uint64_t, 6>& g) {
+// The case of the split of a prism: m, the corner of the smallest global index, and a, whether
+// the diagonal of the side opposite it starts at the corner next to m (prism.rs:131-154).
+static void prism_case(const std::array<uint64_t, 6>& g, int& m, bool& a) {
     // Rotations that bring each corner to corner 0 (bottom 0 1 2, top 3 4 5 with 3 above 0).
+    static const int ROT[6][6] = {{0, 1, 2, 3, 4, 5}, {1, 2, 0, 4, 5, 3}, {2, 0, 1, 5, 3, 4},
+                                  {3, 5, 4, 0, 2, 1}, {4, 3, 5, 1, 0, 2}, {5, 4, 3, 2, 1, 0}};
+    m = 0;
+    for (int k = 1; k < 6; ++k)
+        if (g[k] < g[m]) m = k;
+    const int* r = ROT[m];
+    a = std::min(g[r[1]], g[r[5]]) < std::min(g[r[2]], g[r[4]]);
+}
+
+// The 3 tetrahedra (local corners) of the split of case (m, a).
+static std::array<std::array<int, 4>, 3> prism_tets(int m, bool a) {
     static const int ROT[6][6] = {{0, 1, 2, 3, 4, 5}, {1, 2, 0, 4, 5, 3}, {2, 0, 1, 5, 3, 4},
                                   {3, 5, 4, 0, 2, 1}, {4, 3, 5, 1, 0, 2}, {5, 4, 3, 2, 1, 0}};
     static const int CUT_A[3][4] = {{0, 1, 2, 5}, {0, 1, 5, 4}, {0, 4, 5, 3}};
     static const int CUT_B[3][4] = {{0, 1, 2, 4}, {0, 4, 2, 5}, {0, 4, 5, 3}};
-    int m = 0;
-    for (int k = 1; k < 6; ++k)
-        if (g[k] < g[m]) m = k;
     const int* r = ROT[m];
-    const bool a = std::min(g[r[1]], g[r[5]]) < std::min(g[r[2]], g[r[4]]);
     const auto& cut = a ? CUT_A : CUT_B;
     std::array<std::array<int, 4>, 3> out{};
     for (int t = 0; t < 3; ++t)
         for (int v = 0; v < 4; ++v) out[t][v] = r[cut[t][v]];
     return out;
+}
+
+std::array<std::array<int, 4>, 3> prism_split(const std::array<uint64_t, 6>& g) {
+    int m;
+    bool a;
+    prism_case(g, m, a);
+    return prism_tets(m, a);
 }
 
 namespace {
@@ -52,6 +68,60 @@ int kuhn_type(int a, int b) {
     return -1;
 }
 
+// The tetrahedra of a prism of the box grid, for each footprint triangle (0: corners (0,0),
+// (1,0), (1,1) of the quad; 1: (0,0), (1,1), (0,1)) and each split case (m, a): their local corners
+// with a tetrahedron of negative volume reordered by swapping its corners 2 and 3
+// (prism.rs:278-280), and their Kuhn types. Every quantity follows from the corners' bits in the
+// cell, so the table is built once from the same rules as a tetrahedron by tetrahedron pass.
+struct PrismTable {
+    int local[2][6][2][3][4];
+    int type[2][6][2][3];
+};
+
+int kuhn_type(int a, int b);
+
+const PrismTable& prism_table() {
+    static const PrismTable table = [] {
+        PrismTable tb{};
+        const int QUAD[2][3][2] = {{{0, 0}, {1, 0}, {1, 1}}, {{0, 0}, {1, 1}, {0, 1}}};
+        for (int tri = 0; tri < 2; ++tri)
+            for (int m = 0; m < 6; ++m)
+                for (int a = 0; a < 2; ++a) {
+                    const auto tets = prism_tets(m, a != 0);
+                    for (int t = 0; t < 3; ++t) {
+                        int bits[4][3];
+                        std::array<int, 4> loc = tets[t];
+                        for (int v = 0; v < 4; ++v) {
+                            const int cc = loc[v];
+                            bits[v][0] = QUAD[tri][cc % 3][0];
+                            bits[v][1] = QUAD[tri][cc % 3][1];
+                            bits[v][2] = cc / 3;
+                        }
+                        long e[3][3];
+                        for (int v = 0; v < 3; ++v)
+                            for (int dd = 0; dd < 3; ++dd) e[v][dd] = bits[v + 1][dd] - bits[0][dd];
+                        const long det = e[0][0] * (e[1][1] * e[2][2] - e[1][2] * e[2][1]) -
+                                         e[0][1] * (e[1][0] * e[2][2] - e[1][2] * e[2][0]) +
+                                         e[0][2] * (e[1][0] * e[2][1] - e[1][1] * e[2][0]);
+                        if (det < 0) std::swap(loc[2], loc[3]);
+                        for (int v = 0; v < 4; ++v) tb.local[tri][m][a][t][v] = loc[v];
+                        // the chain order: vertices by the sum of their bits, the axis of each step
+                        int sum[4], order[4] = {0, 1, 2, 3};
+                        for (int v = 0; v < 4; ++v) sum[v] = bits[v][0] + bits[v][1] + bits[v][2];
+                        std::sort(order, order + 4, [&](int x, int y) { return sum[x] < sum[y]; });
+                        int axis[2] = {0, 0};
+                        for (int st = 0; st < 2; ++st)
+                            for (int dd = 0; dd < 3; ++dd)
+                                if (bits[order[st + 1]][dd] != bits[order[st]][dd]) axis[st] = dd;
+                        // -1 for a case whose tetrahedra are not Kuhn chains (no box ordering gives it)
+                        tb.type[tri][m][a][t] = kuhn_type(axis[0], axis[1]);
+                    }
+                }
+        return tb;
+    }();
+    return table;
+}
+
 bool on_face(const uint32_t q[3], const int top[3], int f) {
     const int d = f / 2;
     return (f % 2 == 0) ? q[d] == 0 : (int)q[d] == top[d];
@@ -59,15 +129,14 @@ bool on_face(const uint32_t q[3], const int top[3], int f) {
 
 }  // namespace
 
-BoxModel build_model(const BoxInput& in) {
+BoxModel box_shape(const BoxInput& in) {
     for (int d = 0; d < 3; ++d) {
         if (in.n[d] < 1) throw Error("each axis needs 1 cell or more");
         if (!(in.len[d] > 0.0 && std::isfinite(in.len[d]))) throw Error("the box sizes must be finite numbers above 0");
     }
     BoxModel m;
     m.in = in;
-    const int nx = in.n[0], ny = in.n[1], nz = in.n[2];
-    const size_t cells = (size_t)nx * ny * nz;
+    const size_t cells = (size_t)in.n[0] * in.n[1] * in.n[2];
     const bool tets = in.kind != Kind::HeatHex8;
     m.comps = (in.kind == Kind::ElasticTet4 || in.kind == Kind::ElasticTet10) ? 3 : 1;
     m.quadratic = in.kind == Kind::ElasticTet10;
@@ -81,6 +150,18 @@ BoxModel build_model(const BoxInput& in) {
     const size_t need = tets ? 6 * cells : cells;
     if (in.coef0.size() != need) throw Error(tets ? "one material value for each tetrahedron" : "one conductivity for each cell");
     if (m.comps == 3 && in.coef1.size() != need) throw Error("one Poisson ratio for each tetrahedron");
+    // a node is free when it lies on no fixed face, that is, off the fixed faces along every axis
+    size_t free_nodes = 1;
+    for (int d = 0; d < 3; ++d) free_nodes *= (size_t)(m.gn[d] - (in.fixed[2 * d] ? 1 : 0) - (in.fixed[2 * d + 1] ? 1 : 0));
+    m.free_count = free_nodes * m.comps;
+    return m;
+}
+
+BoxModel build_model(const BoxInput& in) {
+    BoxModel m = box_shape(in);
+    const int nx = in.n[0], ny = in.n[1], nz = in.n[2];
+    const size_t cells = (size_t)nx * ny * nz;
+    const bool tets = in.kind != Kind::HeatHex8;
 
     const size_t gnodes = m.grid_nodes();
     auto gidx = [&](uint32_t i, uint32_t j, uint32_t k) { return i + (size_t)m.gn[0] * (j + (size_t)m.gn[1] * k); };
@@ -110,11 +191,13 @@ BoxModel build_model(const BoxInput& in) {
         tet_nodes.reserve(6 * cells);
         tet_cell.reserve(6 * cells);
         tet_type.reserve(6 * cells);
+        const PrismTable& pt = prism_table();
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
                 const uint32_t a = i + (nx + 1) * j, b = a + 1, c = a + 1 + (nx + 1), d = a + (nx + 1);
                 const uint32_t tris[2][3] = {{a, b, c}, {a, c, d}};
-                for (const auto& tri : tris)
+                for (int trix = 0; trix < 2; ++trix) {
+                    const uint32_t* tri = tris[trix];
                     for (int k = 0; k < nz; ++k) {
                         std::array<uint64_t, 6> g{};
                         uint32_t node[6];
@@ -129,40 +212,20 @@ BoxModel build_model(const BoxInput& in) {
                             }
                             node[cc] = slot;
                         }
-                        for (const auto& local : prism_split(g)) {
-                            std::array<uint32_t, 4> t = {node[local[0]], node[local[1]], node[local[2]], node[local[3]]};
-                            // The sign of the volume from the integer grid points (the cell sizes are positive).
-                            long e[3][3];
-                            for (int v = 0; v < 3; ++v)
-                                for (int dd = 0; dd < 3; ++dd)
-                                    e[v][dd] = (long)corner_ijk[t[v + 1]][dd] - (long)corner_ijk[t[0]][dd];
-                            const long det = e[0][0] * (e[1][1] * e[2][2] - e[1][2] * e[2][1]) -
-                                             e[0][1] * (e[1][0] * e[2][2] - e[1][2] * e[2][0]) +
-                                             e[0][2] * (e[1][0] * e[2][1] - e[1][1] * e[2][0]);
-                            if (det < 0) std::swap(t[2], t[3]);
-                            tet_nodes.push_back(t);
-                            // The cell (lowest corner) and the Kuhn type (axis order of the chain).
-                            uint32_t lo[3] = {UINT32_MAX, UINT32_MAX, UINT32_MAX};
-                            for (int v = 0; v < 4; ++v)
-                                for (int dd = 0; dd < 3; ++dd) lo[dd] = std::min(lo[dd], corner_ijk[t[v]][dd]);
-                            int order[4];
-                            int sum[4];
-                            for (int v = 0; v < 4; ++v) {
-                                order[v] = v;
-                                sum[v] = 0;
-                                for (int dd = 0; dd < 3; ++dd) sum[v] += (int)(corner_ijk[t[v]][dd] - lo[dd]);
-                            }
-                            std::sort(order, order + 4, [&](int x, int y) { return sum[x] < sum[y]; });
-                            int axis[2];
-                            for (int s = 0; s < 2; ++s)
-                                for (int dd = 0; dd < 3; ++dd)
-                                    if (corner_ijk[t[order[s + 1]]][dd] != corner_ijk[t[order[s]]][dd]) axis[s] = dd;
-                            const int type = kuhn_type(axis[0], axis[1]);
-                            if (type < 0) throw Error("the box split is not the Kuhn split");
-                            tet_cell.push_back(lo[0] + (uint32_t)nx * (lo[1] + (uint32_t)ny * lo[2]));
-                            tet_type.push_back((uint8_t)type);
+                        int mcase;
+                        bool acase;
+                        prism_case(g, mcase, acase);
+                        const uint32_t cell = (uint32_t)i + (uint32_t)nx * ((uint32_t)j + (uint32_t)ny * (uint32_t)k);
+                        for (int t = 0; t < 3; ++t) {
+                            const int* loc = pt.local[trix][mcase][acase][t];
+                            if (pt.type[trix][mcase][acase][t] < 0) throw Error("the box split is not the Kuhn split");
+                            if (m.quadratic)
+                                tet_nodes.push_back({node[loc[0]], node[loc[1]], node[loc[2]], node[loc[3]]});
+                            tet_cell.push_back(cell);
+                            tet_type.push_back((uint8_t)pt.type[trix][mcase][acase][t]);
                         }
                     }
+                }
             }
         const uint32_t scale = m.quadratic ? 2 : 1;
         for (const auto& c : corner_ijk) m.point_grid.push_back((uint32_t)gidx(scale * c[0], scale * c[1], scale * c[2]));
@@ -226,6 +289,8 @@ BoxModel build_model(const BoxInput& in) {
                 if (m.comps == 1) m.fixed_value[gi] = in.value[f];
             }
     }
+    m.free.reserve(m.free_count);
+    m.free_grid.reserve(m.free_count);
     for (size_t p = 0; p < m.points; ++p) {
         const uint32_t gi = m.point_grid[p];
         if (node_fixed[gi]) continue;
@@ -234,6 +299,7 @@ BoxModel build_model(const BoxInput& in) {
             m.free_grid.push_back((int64_t)(m.comps * (size_t)gi + c));
         }
     }
+    if (m.free.size() != m.free_count) throw Error("internal: the free unknowns differ from the shape's count");
 
     // ---------------------------------------------------------------- loads
     m.load.assign(gnodes * m.comps, 0.0);

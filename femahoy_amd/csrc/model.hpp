@@ -32,6 +32,7 @@ struct BoxModel {
     double h[3] = {0, 0, 0};
     uint32_t fix = 0;       // face mask of the fixed faces
     size_t points = 0;      // points of the reference numbering (= grid nodes of level 0)
+    size_t free_count = 0;  // free unknowns (known from the shape alone)
     std::vector<uint32_t> point_grid;  // reference point -> level-0 grid node
     std::vector<int64_t> free;         // free unknowns, increasing (reference numbering)
     std::vector<int64_t> free_grid;    // free unknown k -> level-0 grid unknown (comps * node + c)
@@ -52,6 +53,10 @@ struct BoxModel {
 
 // Builds the model. Throws fem::Error with a message on bad input.
 BoxModel build_model(const BoxInput& in);
+
+// The shape of the model alone (grid, cell sizes, fixed faces, the count of free unknowns), with
+// the input checked, without the numbering: what admission by footprint needs, in O(1).
+BoxModel box_shape(const BoxInput& in);
 
 // The prism split of the reference (crates/fem/src/prism.rs:131-154): the 3 tetrahedra of a
 // prism with the global indices g of its 6 corners, as local corners.

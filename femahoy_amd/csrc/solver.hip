@@ -800,7 +800,7 @@ class Solver {
     // precision, the boundary records of the fused tet10 kernels, the dense coarse inverse or direct
     // factor with its workspace, and a quarter more for partials and allocation granularity.
     static uint64_t estimate_bytes(const BoxModel& m, const Options& o) {
-        const uint64_t nfree = m.free.size();
+        const uint64_t nfree = m.free_count;
         const uint64_t cells = (uint64_t)m.in.n[0] * m.in.n[1] * m.in.n[2];
         const uint64_t nodes0 = (uint64_t)m.gn[0] * m.gn[1] * m.gn[2];
         const uint64_t dofs0 = nodes0 * m.comps;
@@ -1225,21 +1225,26 @@ class Solver {
         T* w = L.xb.data();
         launch(k_random<T>, n, s_, n, v, (const T*)L.invd.data());
         const unsigned pb = std::min<unsigned>(nblocks(n), 1024);
-        double lam = 0.0;
+        // The iterate is normalised on the device; the host reads |D^-1 A v| once, after the last
+        // step.
         for (int it = 0; it < opt_.power_steps; ++it) {
             launch_blocks(k_dot<T>, pb, s_, n, (const T*)v, (const T*)v, partA_.data());
             launch_reduce(k_sum, s_, (const double*)partA_.data(), (int)pb, red_.data());
             launch(k_normalise<T>, n, s_, n, v, (const double*)red_.data());
             op_apply(L, (const T*)v, w, s_);
             launch(k_scale<T>, n, s_, n, w, (const T*)L.invd.data());
-            launch_blocks(k_dot<T>, pb, s_, n, (const T*)w, (const T*)w, partA_.data());
-            launch_reduce(k_sum, s_, (const double*)partA_.data(), (int)pb, red_.data() + 1);
-            double nw = 0.0;
-            FEM_HIP(hipMemcpyAsync(&nw, red_.data() + 1, sizeof(double), hipMemcpyDeviceToHost, s_));
-            FEM_HIP(hipStreamSynchronize(s_));
-            lam = std::sqrt(nw);
+            if (it + 1 == opt_.power_steps) {
+                launch_blocks(k_dot<T>, pb, s_, n, (const T*)w, (const T*)w, partA_.data());
+                launch_reduce(k_sum, s_, (const double*)partA_.data(), (int)pb, red_.data() + 1);
+            }
             std::swap(v, w);
         }
+        double nw = 0.0;
+        if (opt_.power_steps > 0) {
+            FEM_HIP(hipMemcpyAsync(&nw, red_.data() + 1, sizeof(double), hipMemcpyDeviceToHost, s_));
+            FEM_HIP(hipStreamSynchronize(s_));
+        }
+        const double lam = std::sqrt(nw);
         launch(k_fill<T>, n, s_, n, L.xa.data(), T(0));
         launch(k_fill<T>, n, s_, n, L.xb.data(), T(0));
         return lam;
@@ -1570,8 +1575,8 @@ class Server {
         r->in = in;
         r->opt = o;
         r->opt.device = device_;
-        const BoxModel m = build_model(in);
-        if (b) r->b.assign(b, b + m.free.size());
+        const BoxModel m = box_shape(in);
+        if (b) r->b.assign(b, b + m.free_count);
         r->footprint = Solver::estimate_bytes(m, r->opt);
         if (r->footprint > budget_)
             throw Error("the solve needs about " + std::to_string(r->footprint) + " device bytes, more than the server's budget of " +
