@@ -272,10 +272,31 @@ __global__ void __launch_bounds__(BLOCK) k_restrict(GridDims fg, GridDims cg, co
 }
 
 // ------------------------------------------------------------------ FP64 PCG pieces
-// Scalars of one PCG solve, in device memory.
+// Scalars of one PCG solve, in device memory. The stopping test of the loop head
+// (multigrid.rs:722-737) runs on the device: `active` says whether the next iteration runs (the
+// relative residual above tol and fewer than it_limit iterations done), `go` is that decision
+// taken at the head of the current iteration, and the kernels that change the solve's state (x,
+// r, p and these scalars) do nothing when it is 0, so iterations queued past the end leave the
+// solve as it was.
 struct PcgScalars {
     double rz, pq, rr, bb, alpha, beta;
+    double go, active, it, it_limit, tol;
 };
+
+// Configures a PCG pass: tolerance and the iterations it may take.
+__global__ void k_pcg_config(PcgScalars* s, double tol, double it_limit) {
+    if (threadIdx.x != 0) return;
+    s->tol = tol;
+    s->it_limit = it_limit;
+    s->it = 0.0;
+    s->go = 0.0;
+}
+
+// The stopping test of the loop head on the current r.r (the host takes the same quotient).
+__device__ __forceinline__ double pcg_next_active(const PcgScalars* s, double rr) {
+    const double res = s->bb > 0.0 ? sqrt(rr) / sqrt(s->bb) : 0.0;
+    return (res > s->tol && s->it < s->it_limit) ? 1.0 : 0.0;
+}
 
 // r = b - A x (FP64 operator); partial sums of b.b and r.r.
 template <class OP>
@@ -341,7 +362,7 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_update(size_t n, const PcgScalars
                                                       double* __restrict__ part) {
     const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
     double rr = 0.0;
-    if (u < n) {
+    if (u < n && s->go != 0.0) {
         const double alpha = s->alpha;
         x[u] += alpha * p[u];
         const double ru = r[u] - alpha * q[u];
@@ -378,7 +399,8 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_dir(size_t n, const PcgScalars* _
                                                    double* __restrict__ p, int first) {
     const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
     if (u >= n) return;
-    p[u] = first ? z[u] : z[u] + s->beta * p[u];
+    if (first) p[u] = z[u];
+    else if (s->go != 0.0) p[u] = z[u] + s->beta * p[u];
 }
 
 // Sum of `count` partials in a fixed order, by one block of RBLOCK threads.
@@ -388,17 +410,26 @@ __device__ __forceinline__ double sum_partials(const double* part, int count) {
     return block_sum_t<RBLOCK>(v);
 }
 
-// what: 0 start (bb, rr), 1 alpha = rz / pq, 2 rr, 3 rz (first: rz = rz'), 4 beta = rz'/rz then rz = rz'
+// what: 0 start (bb, rr, and the stopping test), 1 the head of an iteration (go = active, the
+// count) then alpha = rz / pq, 2 rr (and the stopping test for the next iteration), 3 rz (first:
+// rz = rz'), 4 beta = rz'/rz then rz = rz'. 1, 2 and 4 change nothing else in an iteration that
+// does not run (go = 0).
 __global__ void __launch_bounds__(RBLOCK) k_pcg_scalars(PcgScalars* s, const double* part_a, const double* part_b,
                                                        int count, int what) {
     const double a = sum_partials(part_a, count);
     double b = 0.0;
     if (what == 0) b = sum_partials(part_b, count);
     if (threadIdx.x != 0) return;
+    if (what == 1) {
+        s->go = s->active;
+        if (s->go != 0.0) s->it += 1.0;
+    }
+    if ((what == 1 || what == 2 || what == 4) && s->go == 0.0) return;
     switch (what) {
         case 0:
             s->bb = a;
             s->rr = b;
+            s->active = pcg_next_active(s, b);
             break;
         case 1:
             s->pq = a;
@@ -406,6 +437,7 @@ __global__ void __launch_bounds__(RBLOCK) k_pcg_scalars(PcgScalars* s, const dou
             break;
         case 2:
             s->rr = a;
+            s->active = pcg_next_active(s, a);
             break;
         case 3:
             s->rz = a;

@@ -524,7 +524,7 @@ class Solver {
         FEM_HIP(hipGetDeviceProperties(&prop, o.device));
         device_name_ = std::string(prop.name[0] ? prop.name : "AMD GPU") + " (" + prop.gcnArchName + ")";
         FEM_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
-        FEM_HIP(hipHostMalloc((void**)&pinned_, 8 * sizeof(double), hipHostMallocDefault));
+        FEM_HIP(hipHostMalloc((void**)&pinned_, 16 * sizeof(double), hipHostMallocDefault));
         FEM_HIP(hipEventCreateWithFlags(&ev_, hipEventDisableTiming));
         FEM_HIP(hipEventCreate(&ev_t0_));
         FEM_HIP(hipEventCreate(&ev_t1_));
@@ -753,8 +753,10 @@ class Solver {
                 run_.pending = false;
             }
             try {
+                const PcgScalars ps = *reinterpret_cast<const PcgScalars*>(pinned_);
+                run_.stopped = ps.active == 0.0;
                 if (run_.phase == Phase::Start) {
-                    const double rr = pinned_[0], bb = pinned_[1];
+                    const double rr = ps.rr, bb = ps.bb;
                     run_.norm_b = std::sqrt(bb);
                     if (run_.norm_b == 0.0) {
                         launch(k_fill<double>, n0_, s_, n0_, x_.data(), 0.0);
@@ -775,9 +777,14 @@ class Solver {
                     launch(k_pcg_dir, n, s_, n, (const PcgScalars*)sc, (const double*)z_.data(), p_.data(), 1);
                     run_.it = 0;
                     run_.residual = std::sqrt(rr) / run_.norm_b;
-                } else {  // Iterating: one more iteration has completed
-                    ++run_.it;
-                    run_.residual = std::sqrt(pinned_[0]) / run_.norm_b;
+                    run_.rate = 0.0;
+                } else {  // Iterating: a batch of iterations has completed (or stopped on the device)
+                    const uint64_t it = (uint64_t)ps.it;
+                    const double res = std::sqrt(ps.rr) / run_.norm_b;
+                    if (it > run_.it && run_.residual > 0.0 && res > 0.0)
+                        run_.rate = std::pow(res / run_.residual, 1.0 / (double)(it - run_.it));
+                    run_.it = it;
+                    run_.residual = res;
                 }
                 next_step();
             } catch (const Error& e) {
@@ -873,6 +880,9 @@ class Solver {
     hipStream_t stream() const { return s_; }
     size_t free_count_host() const { return model_.free.size(); }
     int64_t device_bytes() const { return device_bytes_; }
+    // The most PCG iterations queued between two readbacks. Solves that share the GPU keep it at
+    // 1: streams share the hardware queues, and a long batch on one holds up the others.
+    void set_max_batch(int k) { max_batch_ = std::max(1, k); }
 
     void solve_host(const double* b, double* x, int (*cancelled)(void*), void* ctx, fem_stats* st) {
         const size_t nf = model_.free.size();
@@ -902,6 +912,7 @@ class Solver {
     std::unique_ptr<DenseCoarse> direct_;
     Profiler prof_;
     bool use_graph_ = true;
+    int max_batch_ = 8;
     hipGraphExec_t graph_exec_ = nullptr;
     double* pinned_ = nullptr;  // host-visible scalars read back each iteration
     hipEvent_t ev_ = nullptr, ev_t0_ = nullptr, ev_t1_ = nullptr;  // readback; device time of a solve
@@ -912,6 +923,8 @@ class Solver {
         uint64_t total_it = 0, it = 0;
         int restarts = 0;
         double norm_b = 0.0, residual = 0.0;
+        double rate = 0.0;     // mean residual reduction per iteration of the last batch
+        bool stopped = false;  // the device-side stopping test has ended the pass
         int (*cancelled)(void*) = nullptr;
         void* ctx = nullptr;
         double* d_out = nullptr;
@@ -1288,13 +1301,6 @@ class Solver {
         }
     }
 
-    double read_scalar(size_t offset_doubles) {
-        FEM_HIP(hipMemcpyAsync(pinned_, reinterpret_cast<const double*>(scal_.data()) + offset_doubles, sizeof(double),
-                               hipMemcpyDeviceToHost, s_));
-        FEM_HIP(hipStreamSynchronize(s_));
-        return pinned_[0];
-    }
-
     // r = b - A x in FP64 on level 0 with the block partials of b.b (partA_) and r.r (partB_);
     // returns their count.
     unsigned residual64(const double* x, double* r) {
@@ -1455,23 +1461,45 @@ class Solver {
         FEM_HIP(hipGraphLaunch(graph_exec_, s_));
     }
 
-    // multigrid.rs:692-753 as steps: r = b - A x and the scalars, then one readback per iteration.
+    // The scalars to the host (pinned), then the event the host waits on.
+    void read_back() {
+        FEM_HIP(hipMemcpyAsync(pinned_, scal_.data(), sizeof(PcgScalars), hipMemcpyDeviceToHost, s_));
+        FEM_HIP(hipEventRecord(ev_, s_));
+        run_.pending = true;
+    }
+
+    // multigrid.rs:692-753 as steps: r = b - A x and the scalars (a pass may take the iterations
+    // left of max_iter), then one readback per batch of iterations.
     void launch_start() {
+        const double limit = (double)(opt_.max_iter - std::min<uint64_t>(opt_.max_iter, run_.total_it));
+        hipLaunchKernelGGL(k_pcg_config, dim3(1), dim3(64), 0, s_, scal_.data(), opt_.tol, limit);
         const unsigned nbn = residual64(x_.data(), r_.data());
         launch_reduce(k_pcg_scalars, s_, scal_.data(), (const double*)partA_.data(), (const double*)partB_.data(),
                       (int)nbn, 0);
-        // rr and bb are adjacent in PcgScalars
-        FEM_HIP(hipMemcpyAsync(pinned_, reinterpret_cast<const double*>(scal_.data()) + offsetof(PcgScalars, rr) / sizeof(double),
-                               2 * sizeof(double), hipMemcpyDeviceToHost, s_));
-        FEM_HIP(hipEventRecord(ev_, s_));
+        read_back();
         run_.phase = Phase::Start;
-        run_.pending = true;
+    }
+
+    // The iterations to queue before the next readback: half of those the convergence rate of the
+    // last batch says are still needed. An iteration past the end changes nothing (the stopping
+    // test is on the device) but costs a whole V-cycle, while a readback costs a few microseconds,
+    // and the rate of MG-PCG improves as it goes, so the prediction errs long: halving it keeps the
+    // queued iterations short of the end.
+    int batch_size() const {
+        if (max_batch_ <= 1 || !(run_.rate > 0.0 && run_.rate < 1.0) || !(run_.residual > opt_.tol)) return 1;
+        const double need = std::ceil(std::log(opt_.tol / run_.residual) / std::log(run_.rate));
+        const uint64_t left = opt_.max_iter - std::min<uint64_t>(opt_.max_iter, run_.total_it + run_.it);
+        const double k = std::min(std::floor(0.5 * need), (double)max_batch_);
+        return (int)std::max(1.0, std::min(k, (double)std::max<uint64_t>(left, 1)));
     }
 
     // The test of the loop head (multigrid.rs:722-737) on the current residual, then one more
     // iteration with its readback, or the end of this PCG pass.
     void next_step() {
-        if (run_.residual <= opt_.tol) {
+        // The device decides when a pass ends (the same test on the same doubles); the host
+        // follows it, so that a pass can never spin on iterations the device skips.
+        const bool at_limit = run_.total_it + run_.it >= opt_.max_iter;
+        if (run_.residual <= opt_.tol || (run_.stopped && !at_limit)) {
             run_.total_it += run_.it;
             const double tr = true_residual();
             run_.out.iterations = run_.total_it;
@@ -1486,7 +1514,7 @@ class Solver {
             launch_start();
             return;
         }
-        if (run_.total_it + run_.it == opt_.max_iter) {
+        if (at_limit) {
             run_.out.iterations = run_.total_it + run_.it;
             run_.out.residual = run_.residual;
             run_.out.restarts = run_.restarts;
@@ -1494,12 +1522,10 @@ class Solver {
                         std::to_string(opt_.max_iter) + " iterations");
         }
         if (run_.cancelled && run_.cancelled(run_.ctx)) throw Error("cancelled");
-        iterate();
-        FEM_HIP(hipMemcpyAsync(pinned_, reinterpret_cast<const double*>(scal_.data()) + offsetof(PcgScalars, rr) / sizeof(double),
-                               sizeof(double), hipMemcpyDeviceToHost, s_));
-        FEM_HIP(hipEventRecord(ev_, s_));
+        const int k = batch_size();
+        for (int q = 0; q < k; ++q) iterate();
+        read_back();
         run_.phase = Phase::Iterating;
-        run_.pending = true;
     }
 
     // The solution to the caller's vector, the device time, the hierarchy facts.
@@ -1717,6 +1743,7 @@ class Server {
                 r->solver = std::make_unique<Solver>(r->in, r->opt);
                 r->device_bytes = r->solver->device_bytes();
                 Solver& s = *r->solver;
+                s.set_max_batch(1);
                 const size_t nf = s.free_count_host();
                 if (r->b.empty()) {
                     r->b.resize(nf);
