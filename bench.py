@@ -133,11 +133,11 @@ def timed_solves(F, hip, solver, d_b, d_x, x0, steps, warmup, profile=False):
 def serve_requests(F, n_solves, seed):
     """The serving stream: refinement pairs (h, h/2) of heat tet4 or elasticity tet10 boxes (50/50),
     the coarse member's size log-uniform so that both members lie in about 5k-500k DOF. Boxes of
-    2m x m x m cells (tet4 heat: m in 12..24, fine 2m; tet10: m in 4..10, fine 2m) so that every
-    level halves down to a small dense coarse level. Materials seeded."""
+    2m x m x m cells, m the integer whose size is nearest the drawn one (tet4 heat: m in 13..31,
+    fine 2m; tet10: m in 4..10, fine 2m). Materials seeded."""
     rng = np.random.default_rng(seed)
-    heat_m = [12, 16, 20, 24]
-    elas_m = [4, 6, 8, 10]
+    heat_m = list(range(13, 32))
+    elas_m = list(range(4, 11))
 
     def dofs(kind, m):
         if kind == "heat_tet4":
@@ -352,7 +352,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--degree", type=int, default=3)
     ap.add_argument("--coarse", choices=["galerkin", "mean"], default="galerkin")
-    ap.add_argument("--elastic-n", type=int, default=72,
+    ap.add_argument("--elastic-n", type=int, default=60,
                     help="cells along x of the tet10 elasticity box reported beside the headline (0: skip)")
     ap.add_argument("--inclusions-n", type=int, default=80,
                     help="cells along x of the nearly incompressible inclusion box (0: skip)")

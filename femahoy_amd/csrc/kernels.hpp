@@ -179,7 +179,8 @@ __global__ void __launch_bounds__(BLOCK) k_inv_diag(OP op, T* __restrict__ invd)
 }
 
 // ------------------------------------------------------------------ grid transfers
-// The fine grid has 2 n + 1 nodes where the coarse one has n + 1, on each axis.
+// On each axis the fine grid has 2 n + 1 nodes (n coarse cells), or 2 n on an axis of an odd
+// number of fine cells, whose last coarse node lies one fine cell past the box.
 // Trilinear (hex8): fine node f gets coarse f/2 (even) or the mean of floor and ceil (odd),
 // as a tensor product. Kuhn P1 (tet4, and tet10 -> tet4 on the same cells): fine node f is the
 // midpoint of the Kuhn edge from floor(f/2) to ceil(f/2), componentwise.
@@ -236,9 +237,11 @@ __global__ void __launch_bounds__(BLOCK) k_restrict(GridDims fg, GridDims cg, co
         if (KUHN_P1) {
             // The coarse node itself, and the 14 midpoints of the Kuhn edges at it: directions
             // +-(1,0,0), +-(0,1,0), +-(0,0,1), +-(1,1,0), +-(1,0,1), +-(0,1,1), +-(1,1,1).
-            const size_t me = fi + (size_t)fg.nx * (fj + (size_t)fg.ny * fk);
+            if (fi < fg.nx && fj < fg.ny && fk < fg.nz) {
+                const size_t me = fi + (size_t)fg.nx * (fj + (size_t)fg.ny * fk);
 #pragma unroll
-            for (int c = 0; c < C; ++c) acc[c] = rf[C * me + c];
+                for (int c = 0; c < C; ++c) acc[c] = rf[C * me + c];
+            }
             const int DIRS[7][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
 #pragma unroll
             for (int s = 0; s < 7; ++s)

@@ -1,5 +1,11 @@
 // Setup kernels (FP64): Galerkin coarse operators of the hex8 hierarchy, coefficient averaging of
 // the nested Kuhn hierarchy, and the probing that assembles a dense coarse matrix.
+//
+// A coarse level has ceil(n / 2) cells on an axis of n fine cells. On an odd axis the last coarse
+// cell reaches one fine cell past the box: its children outside the box are void (no material),
+// so they add nothing to a Galerkin product or a coefficient sum. The coarse functions stay
+// nested (a fine cell is a child of the 2x refinement of its coarse cell), and every coarse cell
+// keeps at least its corner child, so the coarse operators stay positive definite.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,17 +23,17 @@ __constant__ static double HEX_CHILD_P[8][64];
 // Level 1 of the hex8 hierarchy: K_C = sum over the 8 children c of k_c M_c, M_c = P_c^T K1 P_c
 // (K1 the fine cell matrix for k = 1), the Galerkin product restricted to the coarse cell
 // (exact: a child's nodes interpolate from its coarse cell's corners only).
-__global__ void k_hex_cells_from_k(int ncx, int ncy, int ncz, const double* __restrict__ kf,
+__global__ void k_hex_cells_from_k(int ncx, int ncy, int ncz, int nfx, int nfy, int nfz, const double* __restrict__ kf,
                                    const double* __restrict__ M, double* __restrict__ Kc) {
     const size_t cell = blockIdx.x * (size_t)BLOCK + threadIdx.x;
     const size_t nc = (size_t)ncx * ncy * ncz;
     if (cell >= nc) return;
     const int I = (int)(cell % ncx), J = (int)((cell / ncx) % ncy), K = (int)(cell / ((size_t)ncx * ncy));
-    const int nfx = 2 * ncx, nfy = 2 * ncy;
     double acc[64];
     for (int e = 0; e < 64; ++e) acc[e] = 0.0;
     for (int c = 0; c < 8; ++c) {
         const int fx = 2 * I + (c & 1), fy = 2 * J + ((c >> 1) & 1), fz = 2 * K + (c >> 2);
+        if (fx >= nfx || fy >= nfy || fz >= nfz) continue;
         const double k = kf[fx + (size_t)nfx * (fy + (size_t)nfy * fz)];
         for (int e = 0; e < 64; ++e) acc[e] += k * M[c * 64 + e];
     }
@@ -35,32 +41,33 @@ __global__ void k_hex_cells_from_k(int ncx, int ncy, int ncz, const double* __re
 }
 
 // A coarse hex8 level rediscretized: the mean conductivity of the 8 children of each coarse cell.
-__global__ void k_hex_average(int ncx, int ncy, int ncz, const double* __restrict__ kf, double* __restrict__ kc) {
+__global__ void k_hex_average(int ncx, int ncy, int ncz, int nfx, int nfy, int nfz, const double* __restrict__ kf,
+                              double* __restrict__ kc) {
     const size_t cell = blockIdx.x * (size_t)BLOCK + threadIdx.x;
     const size_t nc = (size_t)ncx * ncy * ncz;
     if (cell >= nc) return;
     const int I = (int)(cell % ncx), J = (int)((cell / ncx) % ncy), K = (int)(cell / ((size_t)ncx * ncy));
-    const int nfx = 2 * ncx, nfy = 2 * ncy;
     double s = 0.0;
     for (int c = 0; c < 8; ++c) {
         const int fx = 2 * I + (c & 1), fy = 2 * J + ((c >> 1) & 1), fz = 2 * K + (c >> 2);
+        if (fx >= nfx || fy >= nfy || fz >= nfz) continue;
         s += kf[fx + (size_t)nfx * (fy + (size_t)nfy * fz)];
     }
     kc[cell] = s / 8.0;
 }
 
 // Level l+1 from level l cell matrices: K_C = sum_c P_c^T K_c P_c.
-__global__ void k_hex_cells_from_cells(int ncx, int ncy, int ncz, const double* __restrict__ Kf,
-                                       double* __restrict__ Kc) {
+__global__ void k_hex_cells_from_cells(int ncx, int ncy, int ncz, int nfx, int nfy, int nfz,
+                                       const double* __restrict__ Kf, double* __restrict__ Kc) {
     const size_t cell = blockIdx.x * (size_t)BLOCK + threadIdx.x;
     const size_t nc = (size_t)ncx * ncy * ncz;
     if (cell >= nc) return;
     const int I = (int)(cell % ncx), J = (int)((cell / ncx) % ncy), K = (int)(cell / ((size_t)ncx * ncy));
-    const int nfx = 2 * ncx, nfy = 2 * ncy;
     double acc[64];
     for (int e = 0; e < 64; ++e) acc[e] = 0.0;
     for (int c = 0; c < 8; ++c) {
         const int fx = 2 * I + (c & 1), fy = 2 * J + ((c >> 1) & 1), fz = 2 * K + (c >> 2);
+        if (fx >= nfx || fy >= nfy || fz >= nfz) continue;
         const double* A = Kf + (fx + (size_t)nfx * (fy + (size_t)nfy * fz)) * 64;
         const double* P = HEX_CHILD_P[c];
         // T = A P (8 x 8), then acc += P^T T
@@ -116,21 +123,21 @@ __global__ void k_stencil_from_cells(GridDims g, const double* __restrict__ Kc, 
 // so P^T A P integrates the fine material). kids[t][m] = (child cell bits) * 8 + fine type.
 __constant__ static int KUHN_KIDS[6][8];
 
-__global__ void k_kuhn_average(int ncx, int ncy, int ncz, int q, const double* __restrict__ fine,
-                               double* __restrict__ coarse) {
+__global__ void k_kuhn_average(int ncx, int ncy, int ncz, int nfx, int nfy, int nfz, int q,
+                               const double* __restrict__ fine, double* __restrict__ coarse) {
     const size_t nc = (size_t)ncx * ncy * ncz;
     const size_t tid = blockIdx.x * (size_t)BLOCK + threadIdx.x;
     if (tid >= (size_t)q * 6 * nc) return;
     const size_t cell = tid % nc;
     const int t = (int)((tid / nc) % 6), comp = (int)(tid / (6 * nc));
     const int I = (int)(cell % ncx), J = (int)((cell / ncx) % ncy), K = (int)(cell / ((size_t)ncx * ncy));
-    const size_t nf = 8 * nc;
-    const int nfx = 2 * ncx, nfy = 2 * ncy;
+    const size_t nf = (size_t)nfx * nfy * nfz;
     double s = 0.0;
     for (int m = 0; m < 8; ++m) {
         const int code = KUHN_KIDS[t][m];
         const int c = code / 8, ft = code % 8;
         const int fx = 2 * I + (c & 1), fy = 2 * J + ((c >> 1) & 1), fz = 2 * K + (c >> 2);
+        if (fx >= nfx || fy >= nfy || fz >= nfz) continue;
         s += fine[((size_t)comp * 6 + ft) * nf + fx + (size_t)nfx * (fy + (size_t)nfy * fz)];
     }
     coarse[((size_t)comp * 6 + t) * nc + cell] = s / 8.0;

@@ -855,11 +855,9 @@ class Solver {
             if (kind == Kind::ElasticTet10) {
                 next = Kind::ElasticTet4;
             } else {
-                if (o.levels == 0 && last_free <= 1000) break;
-                bool halves = true;
-                for (int d = 0; d < 3; ++d) halves = halves && c[d] % 2 == 0 && c[d] >= 2;
-                if (!halves) break;
-                for (int d = 0; d < 3; ++d) nc[d] = c[d] / 2;
+                if (o.levels == 0 && last_free <= auto_coarse(nfree, o)) break;
+                if (!coarsens(c)) break;
+                for (int d = 0; d < 3; ++d) nc[d] = (c[d] + 1) / 2;
             }
             kind = next;
             for (int d = 0; d < 3; ++d) c[d] = nc[d];
@@ -939,6 +937,18 @@ class Solver {
     DArray<PcgScalars> scal_;
     DArray<int64_t> free_grid_;
 
+    // Automatic levels stop at the first level of at most this many unknowns. The dense coarse
+    // solve costs n^2 per cycle and n^3 at setup; n0 / 256 keeps its per-cycle bytes within about
+    // the finest level's (measured on MI355X: a 3,645 or 5,400 unknown coarse level beats one more
+    // level of a few hundred, whose launches cost about 50 us per cycle), and 1000 keeps the setup
+    // of small solves short.
+    static uint64_t auto_coarse(uint64_t n0, const Options& o) {
+        return std::min<uint64_t>(o.coarse_max, std::max<uint64_t>(1000, n0 / 256));
+    }
+
+    // Whether a level of c cells coarsens: ceil(c / 2) cells per axis, every axis at least 2 cells.
+    static bool coarsens(const int c[3]) { return c[0] >= 2 && c[1] >= 2 && c[2] >= 2; }
+
     static uint64_t free_count(const GridDims& g, int comps) {
         uint64_t n = 0;
         const int top[3] = {g.nx - 1, g.ny - 1, g.nz - 1};
@@ -996,11 +1006,12 @@ class Solver {
     }
 
     // The levels below level 0 for the multigrid, in FP64: Galerkin stencils (hex8) or averaged
-    // Kuhn coefficients (tets). Stops when a level cannot halve, at opt.levels, or when the level is
+    // Kuhn coefficients (tets). Stops when an axis has fewer than 2 cells, at opt.levels, or when the level is
     // small enough for the dense solve and automatic levels were asked for.
     void build_coarse_levels64() {
         const BoxModel& m = model_;
         const int want = opt_.levels;
+        const uint64_t n0_free = m.free_count;
         {
             const std::vector<double> P = child_interp();
             FEM_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(HEX_CHILD_P), P.data(), P.size() * sizeof(double), 0,
@@ -1030,27 +1041,26 @@ class Solver {
             const int l = (int)lv64_.size() - 1;
             if (want > 0 && l + 1 >= want) break;
             const uint64_t nfree = free_count(F.g, F.comps);
-            if (want == 0 && nfree <= 1000) break;
+            if (want == 0 && nfree <= auto_coarse(n0_free, opt_)) break;
             // tet10 level: its grid is refined; the next level has the same cells (handled above)
             if (F.op == OpKind::Tet10) break;
-            bool halves = true;
-            for (int d = 0; d < 3; ++d) halves = halves && F.cells[d] % 2 == 0 && F.cells[d] >= 2;
-            if (!halves) break;
+            if (!coarsens(F.cells)) break;
             Level<double> Cl;
             Cl.comps = F.comps;
             Cl.kuhn = F.kuhn;
             for (int d = 0; d < 3; ++d) {
-                Cl.cells[d] = F.cells[d] / 2;
+                Cl.cells[d] = (F.cells[d] + 1) / 2;
                 Cl.h[d] = 2.0 * F.h[d];
             }
+            const int* fc = F.cells;
             Cl.g = GridDims{Cl.cells[0] + 1, Cl.cells[1] + 1, Cl.cells[2] + 1, m.fix};
             const size_t nc = Cl.ncell();
             if (F.op == OpKind::Hex8K && !opt_.coarse_galerkin) {
                 // rediscretized: the same hex8 operator with the mean conductivity of the children
                 Cl.op = OpKind::Hex8K;
                 Cl.coef.resize(nc);
-                launch(k_hex_average, nc, s_, Cl.cells[0], Cl.cells[1], Cl.cells[2], (const double*)F.coef.data(),
-                       Cl.coef.data());
+                launch(k_hex_average, nc, s_, Cl.cells[0], Cl.cells[1], Cl.cells[2], fc[0], fc[1], fc[2],
+                       (const double*)F.coef.data(), Cl.coef.data());
             } else if (F.op == OpKind::Hex8K || F.op == OpKind::Stencil27) {
                 Cl.op = OpKind::Stencil27;
                 DArray<double> cm(nc * 64);
@@ -1069,11 +1079,11 @@ class Solver {
                             }
                     DArray<double> dM(8 * 64);
                     dM.upload(M.data(), M.size(), s_);
-                    launch(k_hex_cells_from_k, nc, s_, Cl.cells[0], Cl.cells[1], Cl.cells[2],
+                    launch(k_hex_cells_from_k, nc, s_, Cl.cells[0], Cl.cells[1], Cl.cells[2], fc[0], fc[1], fc[2],
                            (const double*)F.coef.data(), (const double*)dM.data(), cm.data());
                     FEM_HIP(hipStreamSynchronize(s_));
                 } else {
-                    launch(k_hex_cells_from_cells, nc, s_, Cl.cells[0], Cl.cells[1], Cl.cells[2],
+                    launch(k_hex_cells_from_cells, nc, s_, Cl.cells[0], Cl.cells[1], Cl.cells[2], fc[0], fc[1], fc[2],
                            (const double*)cellmat.data(), cm.data());
                 }
                 Cl.coef.resize(14 * Cl.nodes());
@@ -1084,8 +1094,8 @@ class Solver {
                 Cl.op = F.op == OpKind::Tet4Heat ? OpKind::Tet4Heat : OpKind::Tet4Elastic;
                 const int q = Cl.comps == 3 ? 2 : 1;
                 Cl.coef.resize((size_t)q * 6 * nc);
-                launch(k_kuhn_average, (size_t)q * 6 * nc, s_, Cl.cells[0], Cl.cells[1], Cl.cells[2], q,
-                       (const double*)F.coef.data(), Cl.coef.data());
+                launch(k_kuhn_average, (size_t)q * 6 * nc, s_, Cl.cells[0], Cl.cells[1], Cl.cells[2], fc[0], fc[1], fc[2],
+                       q, (const double*)F.coef.data(), Cl.coef.data());
             }
             lv64_.push_back(std::move(Cl));
         }
@@ -1191,7 +1201,7 @@ class Solver {
         const uint64_t ncoarse = free_count(last.g, last.comps);
         if (ncoarse > opt_.coarse_max)
             throw Error("the coarsest level has " + std::to_string(ncoarse) + " unknowns, more than the dense limit " +
-                        std::to_string(opt_.coarse_max) + ": use a box whose cell counts halve more times");
+                        std::to_string(opt_.coarse_max) + ": ask for more levels");
         H.degree = opt_.degree;
         H.lv.resize(lv64_.size());
         for (size_t l = 0; l < lv64_.size(); ++l) {

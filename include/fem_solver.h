@@ -69,7 +69,7 @@ typedef struct fem_options {
     int32_t method;         /* enum fem_method */
     int32_t precision;      /* enum fem_precision */
     uint64_t crossover;     /* direct up to this many unknowns with FEM_METHOD_AUTO; solve.rs:18 (3000) */
-    int32_t levels;         /* V-cycle levels including the finest; 0 = as many as halve cleanly */
+    int32_t levels;         /* V-cycle levels including the finest; 0 = automatic: down to max(1000, free/256) unknowns, at most coarse_max */
     int32_t cheb_degree;    /* Chebyshev-Jacobi smoother degree (pre and post) */
     double cheb_lower;      /* smoothed range [cheb_lower * lmax, cheb_upper * lmax] of D^-1 A */
     double cheb_upper;

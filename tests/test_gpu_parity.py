@@ -30,7 +30,7 @@ def test_heat_hex8_multigrid_matches_oracle_direct(fem, oracle, n, precision):
     pr = oracle.Problem.heat(n, [1, 1, 1], k, fixed=fixed)
     xd, _, _ = oracle.Solver(pr, "direct").solve(pr.rhs())
     s = fem.Solver(fem.Box("heat_hex8", n, [1, 1, 1], k, fixed=fixed), method="multigrid", precision=precision,
-                   tol=1e-12)
+                   tol=1e-12, levels=3)
     assert np.array_equal(s.free_list(), pr.free())
     b = s.rhs()
     assert rel(b, pr.rhs()) < 1e-12
@@ -77,9 +77,10 @@ def test_tets_apply_matches_oracle_matrix(fem, oracle, kind, n):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64", "bf16"])
-@pytest.mark.parametrize("kind,n", [("heat_tet4", [8, 8, 8]), ("elastic_tet4", [8, 4, 4]), ("elastic_tet10", [8, 4, 4]),
-                                    ("elastic_tet10", [16, 12, 4]), ("elastic_tet10", [34, 6, 4])])
-def test_tets_multigrid_matches_oracle_direct(fem, oracle, kind, n, precision):
+@pytest.mark.parametrize("kind,n,levels", [("heat_tet4", [8, 8, 8], 3), ("elastic_tet4", [8, 4, 4], 3),
+                                           ("elastic_tet10", [8, 4, 4], 3), ("elastic_tet10", [16, 12, 4], 4),
+                                           ("elastic_tet10", [34, 6, 4], 4), ("elastic_tet10", [16, 12, 4], 0)])
+def test_tets_multigrid_matches_oracle_direct(fem, oracle, kind, n, levels, precision):
     rng = np.random.default_rng(2)
     nt = 6 * int(np.prod(n))
     length = [2.0, 1.0, 1.0]
@@ -94,9 +95,69 @@ def test_tets_multigrid_matches_oracle_direct(fem, oracle, kind, n, precision):
         pr = oracle.Problem.elastic(order, n, length, e, nu, clamped=["x0"], body=(0, 0, -1))
         box = fem.Box(kind, n, length, e, nu, fixed={"x0": True}, body=(0, 0, -1))
     xd, _, _ = oracle.Solver(pr, "direct").solve(pr.rhs())
-    s = fem.Solver(box, method="multigrid", precision=precision, tol=1e-12)
+    s = fem.Solver(box, method="multigrid", precision=precision, tol=1e-12, levels=levels)
     x, st = s.solve(pr.rhs())
     print(kind, n, precision, st)
+    if levels:
+        assert st.levels == levels
+    else:  # automatic: down to the first level that fits the dense coarse solve (6000 unknowns)
+        assert st.coarse_unknowns <= 6000 and st.levels >= 2
+    assert rel(x, xd) < TOL_X
+
+
+def padded_coarse_unknowns(n, levels, comps, fixed_axes):
+    """Free unknowns of the coarsest level when each level has ceil(c / 2) cells per axis: a fixed
+    face fixes its node plane on every level (on an odd axis the top plane lies past the box)."""
+    c = list(n)
+    for _ in range(levels):
+        c = [(v + 1) // 2 for v in c]
+    nodes = [v + 1 for v in c]
+    free = [nodes[d] - fixed_axes.get(d, 0) for d in range(3)]
+    return comps * int(np.prod(free))
+
+
+# Odd cell counts: the coarse levels take ceil(n / 2) cells per axis, the last coarse cell half
+# outside the box (void). Cases with a fixed face at the odd end (x1) and rediscretized hex levels.
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("case", ["heat_hex8", "heat_hex8_rediscretized", "heat_tet4", "elastic_tet4_x1",
+                                  "elastic_tet10"])
+def test_odd_cell_counts_coarsen_by_padding_and_match_oracle_direct(fem, oracle, case, precision):
+    rng = np.random.default_rng(17)
+    length = [1.5, 1.0, 0.8]
+    galerkin = True
+    if case.startswith("heat_hex8"):
+        n, levels, comps, fixed_axes = [13, 9, 7], 4, 1, {0: 2}
+        k = np.exp(rng.standard_normal(int(np.prod(n))))
+        pr = oracle.Problem.heat(n, length, k, fixed={"x0": 0.0, "x1": 1.0})
+        box = fem.Box("heat_hex8", n, length, k, fixed={"x0": 0.0, "x1": 1.0})
+        galerkin = case == "heat_hex8"
+    elif case == "heat_tet4":
+        n, levels, comps, fixed_axes = [15, 9, 7], 3, 1, {0: 2}
+        k = np.exp(rng.standard_normal(6 * int(np.prod(n))))
+        pr = oracle.Problem.heat(n, length, k, fixed={"x0": 0.0, "x1": 1.0}, tet4=True)
+        box = fem.Box("heat_tet4", n, length, k, fixed={"x0": 0.0, "x1": 1.0})
+    else:
+        tet10 = case == "elastic_tet10"
+        n = [15, 7, 5] if tet10 else [9, 5, 5]
+        face = "x0" if tet10 else "x1"
+        levels, comps, fixed_axes = (4 if tet10 else 3), 3, {0: 1}
+        nt = 6 * int(np.prod(n))
+        e = rng.uniform(1.0, 3.0, nt)
+        nu = rng.uniform(0.2, 0.4, nt)
+        pr = oracle.Problem.elastic(10 if tet10 else 4, n, length, e, nu, clamped=[face], body=(0, 0, -1),
+                                    traction={"z1": (0.2, 0, 0)})
+        box = fem.Box("elastic_tet10" if tet10 else "elastic_tet4", n, length, e, nu, fixed={face: True},
+                      body=(0, 0, -1), traction={"z1": (0.2, 0, 0)})
+    xd, _, _ = oracle.Solver(pr, "direct").solve(pr.rhs())
+    s = fem.Solver(box, method="multigrid", precision=precision, tol=1e-12, levels=levels,
+                   coarse_galerkin=galerkin)
+    assert np.array_equal(s.free_list(), pr.free())
+    x, st = s.solve(pr.rhs())
+    print(case, n, precision, st)
+    assert st.levels == levels
+    halvings = levels - 1 - (1 if case == "elastic_tet10" else 0)
+    assert st.coarse_unknowns == padded_coarse_unknowns(n, halvings, comps, fixed_axes)
+    assert st.true_residual < 1e-11
     assert rel(x, xd) < TOL_X
 
 
@@ -153,7 +214,7 @@ def test_cantilever_tet4_fp64_multigrid_matches_oracle(fem, oracle):
 def test_nonconvergence_and_cancel_messages(fem):
     n = [16, 16, 16]
     k, fixed = heat_case(n)
-    s = fem.Solver(fem.Box("heat_hex8", n, [1, 1, 1], k, fixed=fixed), method="multigrid").with_max_iter(1)
+    s = fem.Solver(fem.Box("heat_hex8", n, [1, 1, 1], k, fixed=fixed), method="multigrid", levels=3).with_max_iter(1)
     with pytest.raises(fem.FemError, match=r"conjugate gradients did not converge: the residual is .* after 1 iterations"):
         s.solve(s.rhs())
     s.with_max_iter(500)
@@ -195,8 +256,9 @@ def test_bf16_storage_is_the_fp32_v_cycle_with_rounded_coefficients(fem, kind):
         nt = 6 * int(np.prod(n))
         box = fem.Box(kind, n, [3.6, 1.0, 0.6], rng.uniform(1.0, 2.0, nt), np.full(nt, 0.3), fixed={"x0": True},
                       body=(0, 0, -1))
-    s32 = fem.Solver(box, method="multigrid", precision="fp32", tol=1e-10)
-    s16 = fem.Solver(box, method="multigrid", precision="bf16", tol=1e-10)
+    lv = 0 if kind == "elastic_tet10" else 3
+    s32 = fem.Solver(box, method="multigrid", precision="fp32", tol=1e-10, levels=lv)
+    s16 = fem.Solver(box, method="multigrid", precision="bf16", tol=1e-10, levels=lv)
     r = rng.standard_normal(s32.n)
     z32, z16 = s32.precondition(r), s16.precondition(r)
     d = rel(z16, z32)
@@ -220,7 +282,7 @@ def test_solve_is_bitwise_reproducible(fem, kind):
         nt = 6 * int(np.prod(n))
         box = fem.Box(kind, n, [2.0, 1.0, 1.0], np.random.default_rng(9).uniform(1.0, 2.0, nt), np.full(nt, 0.3),
                       fixed={"x0": True}, body=(0, 0, -1))
-    s = fem.Solver(box, method="multigrid")
+    s = fem.Solver(box, method="multigrid", levels=3)
     b = s.rhs()
     x1, s1 = s.solve(b)
     x2, s2 = s.solve(b)
@@ -233,9 +295,9 @@ def test_graph_replays_match_eager_launches(fem, monkeypatch):
     n = [16, 16, 16]
     k, fixed = heat_case(n, seed=4)
     box = fem.Box("heat_hex8", n, [1, 1, 1], k, fixed=fixed)
-    graph = fem.Solver(box, method="multigrid", tol=1e-11)
+    graph = fem.Solver(box, method="multigrid", tol=1e-11, levels=3)
     monkeypatch.setenv("FEM_NO_GRAPH", "1")
-    eager = fem.Solver(box, method="multigrid", tol=1e-11)
+    eager = fem.Solver(box, method="multigrid", tol=1e-11, levels=3)
     b = graph.rhs()
     xe, se = eager.solve(b)
     for _ in range(3):
@@ -252,7 +314,7 @@ def test_kernel_timing_leaves_later_solves_unchanged(fem, oracle):
     n = [16, 8, 8]
     k, fixed = heat_case(n, seed=4)
     s = fem.Solver(fem.Box("heat_hex8", n, [1, 1, 1], k, fixed=fixed), method="multigrid", precision="fp32",
-                   tol=1e-10)
+                   tol=1e-10, levels=3)
     b = s.rhs()
     x1, _ = s.solve(b)
     p = s.time_kernels(5)
