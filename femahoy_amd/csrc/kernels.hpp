@@ -204,27 +204,36 @@ __global__ void __launch_bounds__(BLOCK) k_prolong_add(GridDims fg, GridDims cg,
 #pragma unroll
         for (int c = 0; c < C; ++c) acc[c] = same ? ec[C * a + c] : T(0.5) * (ec[C * a + c] + ec[C * b + c]);
     } else {
+        // Per axis: the lower coarse node with weight 1 (even coordinate) or 1/2 (odd), the upper
+        // one with 0 or 1/2 (for an even coordinate it is the same node). All 8 corners are loaded
+        // unconditionally, so that the loads are in flight together.
+        const T w1[3] = {(i & 1) ? T(0.5) : T(0), (j & 1) ? T(0.5) : T(0), (k & 1) ? T(0.5) : T(0)};
+        T v[8][C];
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            const int sx = q & 1, sy = (q >> 1) & 1, sz = q >> 2;
-            const int cx = sx ? hi[0] : lo[0], cy = sy ? hi[1] : lo[1], cz = sz ? hi[2] : lo[2];
-            // weight: 1 for an even coordinate on its one coarse node (counted once), 1/2 for odd
-            const bool dx = (i & 1), dy = (j & 1), dz = (k & 1);
-            if ((!dx && sx) || (!dy && sy) || (!dz && sz)) continue;
-            const T w = T(dx ? 0.5 : 1.0) * T(dy ? 0.5 : 1.0) * T(dz ? 0.5 : 1.0);
+            const int cx = (q & 1) ? hi[0] : lo[0], cy = ((q >> 1) & 1) ? hi[1] : lo[1], cz = (q >> 2) ? hi[2] : lo[2];
             const size_t cn = cx + (size_t)cg.nx * (cy + (size_t)cg.ny * cz);
 #pragma unroll
-            for (int c = 0; c < C; ++c) acc[c] += w * ec[C * cn + c];
+            for (int c = 0; c < C; ++c) v[q][c] = ec[C * cn + c];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const T w = ((q & 1) ? w1[0] : T(1) - w1[0]) * (((q >> 1) & 1) ? w1[1] : T(1) - w1[1]) *
+                        ((q >> 2) ? w1[2] : T(1) - w1[2]);
+#pragma unroll
+            for (int c = 0; c < C; ++c) acc[c] += w * v[q][c];
         }
     }
 #pragma unroll
     for (int c = 0; c < C; ++c) xf[C * node + c] += acc[c];
 }
 
-// The transpose: b_c = P^T r_f, gathered per coarse node; 0 on fixed coarse nodes.
-template <class T, int C, bool KUHN_P1>
+// The transpose: b_c = P^T r_f, gathered per coarse node; 0 on fixed coarse nodes. With invd,
+// also the coarse level's first Chebyshev step from x = 0: d_c = x_c = c0 D^-1 b_c.
+template <class T, int C, bool KUHN_P1, class S>
 __global__ void __launch_bounds__(BLOCK) k_restrict(GridDims fg, GridDims cg, const T* __restrict__ rf,
-                                                    T* __restrict__ bc) {
+                                                    T* __restrict__ bc, const S* __restrict__ invd, T c0,
+                                                    T* __restrict__ xc, T* __restrict__ dc) {
     const size_t node = xcd_block() * BLOCK + threadIdx.x;
     if (node >= cg.nodes()) return;
     int I, J, K;
@@ -271,7 +280,15 @@ __global__ void __launch_bounds__(BLOCK) k_restrict(GridDims fg, GridDims cg, co
         }
     }
 #pragma unroll
-    for (int c = 0; c < C; ++c) bc[C * node + c] = acc[c];
+    for (int c = 0; c < C; ++c) {
+        const size_t u = C * node + c;
+        bc[u] = acc[c];
+        if (invd) {  // the coarse level's first Chebyshev step (k_cheb_first, same arithmetic)
+            const T v = c0 * ld<T>(invd, u) * acc[c];
+            dc[u] = v;
+            xc[u] = v;
+        }
+    }
 }
 
 // ------------------------------------------------------------------ FP64 PCG pieces
@@ -296,6 +313,42 @@ __global__ void k_pcg_config(PcgScalars* s, double tol, double it_limit) {
 }
 
 // The stopping test of the loop head on the current r.r (the host takes the same quotient).
+__device__ __forceinline__ double pcg_next_active(const PcgScalars* s, double rr);
+
+// what: 0 start (bb, rr, and the stopping test), 1 the head of an iteration (go = active, the
+// count) then alpha = rz / pq, 2 rr (and the stopping test for the next iteration), 3 rz (first:
+// rz = rz'), 4 beta = rz'/rz then rz = rz'. 1, 2 and 4 change nothing else in an iteration that
+// does not run (go = 0). Run by one thread with the sums a (and b for what 0).
+__device__ __forceinline__ void pcg_scalars_apply(PcgScalars* s, double a, double b, int what) {
+    if (what == 1) {
+        s->go = s->active;
+        if (s->go != 0.0) s->it += 1.0;
+    }
+    if ((what == 1 || what == 2 || what == 4) && s->go == 0.0) return;
+    switch (what) {
+        case 0:
+            s->bb = a;
+            s->rr = b;
+            s->active = pcg_next_active(s, b);
+            break;
+        case 1:
+            s->pq = a;
+            s->alpha = s->rz / a;
+            break;
+        case 2:
+            s->rr = a;
+            s->active = pcg_next_active(s, a);
+            break;
+        case 3:
+            s->rz = a;
+            break;
+        case 4:
+            s->beta = a / s->rz;
+            s->rz = a;
+            break;
+    }
+}
+
 __device__ __forceinline__ double pcg_next_active(const PcgScalars* s, double rr) {
     const double res = s->bb > 0.0 ? sqrt(rr) / sqrt(s->bb) : 0.0;
     return (res > s->tol && s->it < s->it_limit) ? 1.0 : 0.0;
@@ -357,12 +410,15 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_q(OP op, const double* __restrict
     if (threadIdx.x == 0) part[blockIdx.x] = pq;
 }
 
-// x += alpha p, r -= alpha q, rv = (V) r for the V-cycle; partial sums of r.r.
-template <class V>
+// x += alpha p, r -= alpha q, rv = (V) r for the V-cycle; partial sums of r.r. With invd, also
+// the first Chebyshev step of the V-cycle's level 0 from x = 0: d0 = x0 = c0 D^-1 rv (the step
+// k_cheb_first takes, in the same arithmetic).
+template <class V, class S>
 __global__ void __launch_bounds__(BLOCK) k_pcg_update(size_t n, const PcgScalars* __restrict__ s, double* __restrict__ x,
                                                       double* __restrict__ r, const double* __restrict__ p,
                                                       const double* __restrict__ q, V* __restrict__ rv,
-                                                      double* __restrict__ part) {
+                                                      double* __restrict__ part, const S* __restrict__ invd, V c0,
+                                                      V* __restrict__ x0, V* __restrict__ d0) {
     const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
     double rr = 0.0;
     if (u < n && s->go != 0.0) {
@@ -370,14 +426,20 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_update(size_t n, const PcgScalars
         x[u] += alpha * p[u];
         const double ru = r[u] - alpha * q[u];
         r[u] = ru;
-        if (rv) rv[u] = (V)ru;
+        const V bu = (V)ru;
+        rv[u] = bu;
+        if (invd) {
+            const V v = c0 * ld<V>(invd, u) * bu;
+            d0[u] = v;
+            x0[u] = v;
+        }
         rr = ru * ru;
     }
     rr = block_sum(rr);
     if (threadIdx.x == 0) part[blockIdx.x] = rr;
 }
 
-// z = (double) zv (if zv); partial sums of r.z.
+// Partial sums of r.z, z read from zv (if zv; also stored to z as FP64 if z) or from z.
 template <class V>
 __global__ void __launch_bounds__(BLOCK) k_pcg_rz(size_t n, const V* __restrict__ zv, double* __restrict__ z,
                                                   const double* __restrict__ r, double* __restrict__ part) {
@@ -387,7 +449,7 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_rz(size_t n, const V* __restrict_
         double zu;
         if (zv) {
             zu = (double)zv[u];
-            z[u] = zu;
+            if (z) z[u] = zu;
         } else {
             zu = z[u];
         }
@@ -397,13 +459,14 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_rz(size_t n, const V* __restrict_
     if (threadIdx.x == 0) part[blockIdx.x] = rz;
 }
 
-// p = z + beta p (beta = 0 on the first iteration gives p = z).
-__global__ void __launch_bounds__(BLOCK) k_pcg_dir(size_t n, const PcgScalars* __restrict__ s, const double* __restrict__ z,
+// p = z + beta p (first: p = z), z in the preconditioner's precision.
+template <class V>
+__global__ void __launch_bounds__(BLOCK) k_pcg_dir(size_t n, const PcgScalars* __restrict__ s, const V* __restrict__ z,
                                                    double* __restrict__ p, int first) {
     const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
     if (u >= n) return;
-    if (first) p[u] = z[u];
-    else if (s->go != 0.0) p[u] = z[u] + s->beta * p[u];
+    if (first) p[u] = (double)z[u];
+    else if (s->go != 0.0) p[u] = (double)z[u] + s->beta * p[u];
 }
 
 // Sum of `count` partials in a fixed order, by one block of RBLOCK threads.
@@ -413,43 +476,13 @@ __device__ __forceinline__ double sum_partials(const double* part, int count) {
     return block_sum_t<RBLOCK>(v);
 }
 
-// what: 0 start (bb, rr, and the stopping test), 1 the head of an iteration (go = active, the
-// count) then alpha = rz / pq, 2 rr (and the stopping test for the next iteration), 3 rz (first:
-// rz = rz'), 4 beta = rz'/rz then rz = rz'. 1, 2 and 4 change nothing else in an iteration that
-// does not run (go = 0).
+// The scalars from block partials (pcg_scalars_apply).
 __global__ void __launch_bounds__(RBLOCK) k_pcg_scalars(PcgScalars* s, const double* part_a, const double* part_b,
                                                        int count, int what) {
     const double a = sum_partials(part_a, count);
     double b = 0.0;
     if (what == 0) b = sum_partials(part_b, count);
-    if (threadIdx.x != 0) return;
-    if (what == 1) {
-        s->go = s->active;
-        if (s->go != 0.0) s->it += 1.0;
-    }
-    if ((what == 1 || what == 2 || what == 4) && s->go == 0.0) return;
-    switch (what) {
-        case 0:
-            s->bb = a;
-            s->rr = b;
-            s->active = pcg_next_active(s, b);
-            break;
-        case 1:
-            s->pq = a;
-            s->alpha = s->rz / a;
-            break;
-        case 2:
-            s->rr = a;
-            s->active = pcg_next_active(s, a);
-            break;
-        case 3:
-            s->rz = a;
-            break;
-        case 4:
-            s->beta = a / s->rz;
-            s->rz = a;
-            break;
-    }
+    if (threadIdx.x == 0) pcg_scalars_apply(s, a, b, what);
 }
 
 // ------------------------------------------------------------------ small vector kernels

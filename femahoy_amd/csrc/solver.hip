@@ -398,9 +398,11 @@ class Hierarchy {
         if (l == 0 && prof) prof->end(s);
     }
 
-    void vcycle(hipStream_t s) { cycle(0, s); }
+    // first_done: the caller's last kernel has taken level 0's first Chebyshev step (x and d of
+    // buffer 0, as k_cheb_first does).
+    void vcycle(hipStream_t s, bool first_done = false) { cycle(0, s, first_done); }
 
-    void cycle(size_t l, hipStream_t s) {
+    void cycle(size_t l, hipStream_t s, bool first_done) {
         Level<T>& L = lv[l];
         L.cur = 0;  // every cycle uses the same buffers in the same order (graph capture)
         if (l + 1 == lv.size()) {
@@ -409,9 +411,12 @@ class Hierarchy {
         }
         const size_t n = L.dofs();
         // pre-smoothing from x = 0
-        with_invd(L, [&](auto iv) {
-            launch(k_cheb_first<T, elem_t<decltype(iv)>>, n, s, n, (const T*)L.b.data(), iv, L.x(), L.d.data(), L.c2[0]);
-        });
+        if (!first_done) {
+            with_invd(L, [&](auto iv) {
+                launch(k_cheb_first<T, elem_t<decltype(iv)>>, n, s, n, (const T*)L.b.data(), iv, L.x(), L.d.data(),
+                       L.c2[0]);
+            });
+        }
         for (int st = 1; st < degree; ++st) {
             mark_begin(l, s);
             cheb_step(L, L.c1[st], L.c2[st], s);
@@ -429,8 +434,9 @@ class Hierarchy {
             op_residual(L, (const T*)L.x(), (const T*)L.b.data(), L.r.data(), s);
         }
         Level<T>& Cc = lv[l + 1];
-        transfer(L, Cc, true, s);
-        cycle(l + 1, s);
+        const bool smoothed = l + 2 < lv.size();  // the next level smooths (it is not the dense one)
+        transfer(L, Cc, true, s, smoothed);
+        cycle(l + 1, s, smoothed);
         transfer(L, Cc, false, s);
         for (int st = 0; st < degree; ++st) {
             mark_begin(l, s);
@@ -474,17 +480,27 @@ class Hierarchy {
         });
     }
 
-    void transfer(Level<T>& F, Level<T>& Cc, bool restrict_, hipStream_t s) {
+    // restrict_: b_c = P^T r_f, and with first_step the coarse level's first Chebyshev step into
+    // its buffer 0 (the buffer its cycle starts from); else x_f += P x_c.
+    void transfer(Level<T>& F, Level<T>& Cc, bool restrict_, hipStream_t s, bool first_step = false) {
         const int C = F.comps;
         if (restrict_) {
             const size_t n = Cc.nodes();
-            if (F.kuhn) {
-                if (C == 1) launch((k_restrict<T, 1, true>), n, s, F.g, Cc.g, (const T*)F.r.data(), Cc.b.data());
-                else launch((k_restrict<T, 3, true>), n, s, F.g, Cc.g, (const T*)F.r.data(), Cc.b.data());
-            } else {
-                if (C == 1) launch((k_restrict<T, 1, false>), n, s, F.g, Cc.g, (const T*)F.r.data(), Cc.b.data());
-                else launch((k_restrict<T, 3, false>), n, s, F.g, Cc.g, (const T*)F.r.data(), Cc.b.data());
-            }
+            with_invd(Cc, [&](auto iv) {
+                using S = elem_t<decltype(iv)>;
+                const S* inv = first_step ? iv : nullptr;
+                const T c0 = first_step ? Cc.c2[0] : T(0);
+                T* xc = Cc.xa.data();
+                T* dc = Cc.d.data();
+                const T* rf = F.r.data();
+                if (F.kuhn) {
+                    if (C == 1) launch((k_restrict<T, 1, true, S>), n, s, F.g, Cc.g, rf, Cc.b.data(), inv, c0, xc, dc);
+                    else launch((k_restrict<T, 3, true, S>), n, s, F.g, Cc.g, rf, Cc.b.data(), inv, c0, xc, dc);
+                } else {
+                    if (C == 1) launch((k_restrict<T, 1, false, S>), n, s, F.g, Cc.g, rf, Cc.b.data(), inv, c0, xc, dc);
+                    else launch((k_restrict<T, 3, false, S>), n, s, F.g, Cc.g, rf, Cc.b.data(), inv, c0, xc, dc);
+                }
+            });
         } else {
             const size_t n = F.nodes();
             if (F.kuhn) {
@@ -774,7 +790,7 @@ class Solver {
                                   (const double*)r_.data(), partA_.data());
                     launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr,
                                   (int)nbv, 3);
-                    launch(k_pcg_dir, n, s_, n, (const PcgScalars*)sc, (const double*)z_.data(), p_.data(), 1);
+                    launch((k_pcg_dir<double>), n, s_, n, (const PcgScalars*)sc, (const double*)z_.data(), p_.data(), 1);
                     run_.it = 0;
                     run_.residual = std::sqrt(rr) / run_.norm_b;
                     run_.rate = 0.0;
@@ -1427,6 +1443,22 @@ class Solver {
     }
 
    private:
+    template <class T>
+    void update_and_vcycle(Hierarchy<T>& H) {
+        const size_t n = n0_;
+        Level<T>& L = H.lv[0];
+        const bool smooths = H.lv.size() > 1;
+        with_invd(L, [&](auto iv) {
+            using S = elem_t<decltype(iv)>;
+            launch_blocks((k_pcg_update<T, S>), nblocks(n), s_, n, (const PcgScalars*)scal_.data(), x_.data(),
+                          r_.data(), (const double*)p_.data(), (const double*)q_.data(), L.b.data(), partA_.data(),
+                          smooths ? iv : (const S*)nullptr, smooths ? L.c2[0] : T(0), L.xa.data(), L.d.data());
+        });
+        launch_reduce(k_pcg_scalars, s_, scal_.data(), (const double*)partA_.data(), (const double*)nullptr,
+                      (int)nblocks(n), 2);
+        H.vcycle(s_, smooths);
+    }
+
     void iteration_body() {
         const size_t n = n0_;
         const unsigned nbv = nblocks(n);
@@ -1435,23 +1467,21 @@ class Solver {
         const unsigned nq = outer_apply();
         prof_.end(s_);
         launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr, (int)nq, 1);
+        // the update, r.r and the stopping test, and the V-cycle's first step on level 0 (when
+        // level 0 smooths); then the V-cycle, then r.z and beta
         if (h32_) {
-            launch_blocks((k_pcg_update<float>), nbv, s_, n, (const PcgScalars*)sc, x_.data(), r_.data(),
-                          (const double*)p_.data(), (const double*)q_.data(), h32_->lv[0].b.data(), partA_.data());
-            launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr, (int)nbv, 2);
-            h32_->vcycle(s_);
-            launch_blocks((k_pcg_rz<float>), nbv, s_, n, (const float*)h32_->lv[0].x(), z_.data(),
+            update_and_vcycle(*h32_);
+            launch_blocks((k_pcg_rz<float>), nbv, s_, n, (const float*)h32_->lv[0].x(), (double*)nullptr,
                           (const double*)r_.data(), partA_.data());
         } else {
-            launch_blocks((k_pcg_update<double>), nbv, s_, n, (const PcgScalars*)sc, x_.data(), r_.data(),
-                          (const double*)p_.data(), (const double*)q_.data(), h64_->lv[0].b.data(), partA_.data());
-            launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr, (int)nbv, 2);
-            h64_->vcycle(s_);
-            launch_blocks((k_pcg_rz<double>), nbv, s_, n, (const double*)h64_->lv[0].x(), z_.data(),
+            update_and_vcycle(*h64_);
+            launch_blocks((k_pcg_rz<double>), nbv, s_, n, (const double*)h64_->lv[0].x(), (double*)nullptr,
                           (const double*)r_.data(), partA_.data());
         }
         launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr, (int)nbv, 4);
-        launch(k_pcg_dir, n, s_, n, (const PcgScalars*)sc, (const double*)z_.data(), p_.data(), 0);
+        // z straight from the V-cycle's level-0 iterate (kept until the next V-cycle)
+        if (h32_) launch((k_pcg_dir<float>), n, s_, n, (const PcgScalars*)sc, (const float*)h32_->lv[0].x(), p_.data(), 0);
+        else launch((k_pcg_dir<double>), n, s_, n, (const PcgScalars*)sc, (const double*)h64_->lv[0].x(), p_.data(), 0);
     }
 
     void iterate() {
