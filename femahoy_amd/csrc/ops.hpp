@@ -189,6 +189,15 @@ struct Stencil27 {
 // e_c/h_c, and its volume hx hy hz / 6.
 __constant__ static const int KUHN[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
 
+// Bit d of chain vertex v (0..3) of Kuhn type t: set when axis d is among the first v axes of
+// the chain KUHN[t].
+__host__ __device__ constexpr int kuhn_chain_bit(int t, int v, int d) {
+    const int a = t / 2 == 0 ? 0 : (t / 2 == 1 ? 1 : 2);
+    const int b = t == 0 ? 1 : t == 1 ? 2 : t == 2 ? 0 : t == 3 ? 2 : t == 4 ? 0 : 1;
+    const int rank = d == a ? 0 : (d == b ? 1 : 2);
+    return rank < v ? 1 : 0;
+}
+
 // The vertex of a Kuhn tetrahedron of type t at cell corner (bx, by, bz), or -1.
 __host__ __device__ constexpr int kuhn_vertex(int t, int bx, int by, int bz) {
     const int a = t / 2 == 0 ? 0 : (t / 2 == 1 ? 1 : 2);
@@ -229,8 +238,32 @@ struct Tet4Heat {
     T ih[3];
     T vol;
 
+    // Node-centred, as Tet4Elastic::apply3: the 3 x 3 x 3 neighbourhood and the 8 cells'
+    // coefficients first (clamped indices, zero coefficients outside the grid), then the 24
+    // tetrahedra at the node in a fixed order.
     __device__ __forceinline__ T apply(const T* x, int i, int j, int kk) const {
-        const size_t cells = (size_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1);
+        const int ncx = g.nx - 1, ncy = g.ny - 1, ncz = g.nz - 1;
+        const size_t cells = (size_t)ncx * ncy * ncz;
+        T u[27];
+#pragma unroll
+        for (int q = 0; q < 27; ++q) {
+            const int ni = min(max(i + q % 3 - 1, 0), g.nx - 1), nj = min(max(j + (q / 3) % 3 - 1, 0), g.ny - 1),
+                      nk = min(max(kk + q / 9 - 1, 0), g.nz - 1);
+            u[q] = x[ni + (size_t)g.nx * (nj + (size_t)g.ny * nk)];
+        }
+        T kc[8][6];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int ex = i - 1 + (c & 1), ey = j - 1 + ((c >> 1) & 1), ez = kk - 1 + (c >> 2);
+            const bool in = ex >= 0 && ey >= 0 && ez >= 0 && ex < ncx && ey < ncy && ez < ncz;
+            const size_t cell = min(max(ex, 0), ncx - 1) + (size_t)ncx * (min(max(ey, 0), ncy - 1) +
+                                                                           (size_t)ncy * min(max(ez, 0), ncz - 1));
+#pragma unroll
+            for (int t = 0; t < 6; ++t) {
+                const T k = ld<T>(coef, t * cells + cell);
+                kc[c][t] = in ? k : T(0);
+            }
+        }
         T acc = T(0);
 #pragma unroll
         for (int cz = 0; cz < 2; ++cz)
@@ -238,13 +271,7 @@ struct Tet4Heat {
             for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
                 for (int cx = 0; cx < 2; ++cx) {
-                    const int ex = i - 1 + cx, ey = j - 1 + cy, ez = kk - 1 + cz;
-                    if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx - 1 || ey >= g.ny - 1 || ez >= g.nz - 1) continue;
-                    const size_t cell = ex + (size_t)(g.nx - 1) * (ey + (size_t)(g.ny - 1) * ez);
-                    T xc[8];
-#pragma unroll
-                    for (int q = 0; q < 8; ++q)
-                        xc[q] = x[(ex + (q & 1)) + (size_t)g.nx * ((ey + ((q >> 1) & 1)) + (size_t)g.ny * (ez + (q >> 2)))];
+                    const int c = cx | (cy << 1) | (cz << 2);
                     const int lx = 1 - cx, ly = 1 - cy, lz = 1 - cz;
 #pragma unroll
                     for (int t = 0; t < 6; ++t) {
@@ -252,11 +279,11 @@ struct Tet4Heat {
                         if (me < 0) continue;
                         // grad u = sum over the 4 vertices of u_v g_v
                         T gu[3] = {T(0), T(0), T(0)};
-                        int bits[3] = {0, 0, 0};
 #pragma unroll
                         for (int v = 0; v < 4; ++v) {
-                            if (v > 0) bits[KUHN[t][v - 1]] = 1;
-                            const T uv = xc[bits[0] | (bits[1] << 1) | (bits[2] << 2)];
+                            const int q = (cx + kuhn_chain_bit(t, v, 0)) +
+                                          3 * ((cy + kuhn_chain_bit(t, v, 1)) + 3 * (cz + kuhn_chain_bit(t, v, 2)));
+                            const T uv = u[q];
                             T gv[3];
                             kuhn_grad<T>(t, v, ih, gv);
                             gu[0] += uv * gv[0];
@@ -265,7 +292,7 @@ struct Tet4Heat {
                         }
                         T gm[3];
                         kuhn_grad<T>(t, me, ih, gm);
-                        acc += ld<T>(coef, t * cells + cell) * vol * (gm[0] * gu[0] + gm[1] * gu[1] + gm[2] * gu[2]);
+                        acc += kc[c][t] * vol * (gm[0] * gu[0] + gm[1] * gu[1] + gm[2] * gu[2]);
                     }
                 }
         return acc;
@@ -302,8 +329,35 @@ struct Tet4Elastic {
     T ih[3];
     T vol;
 
+    // Node-centred: the 3 x 3 x 3 node neighbourhood and the coefficients of the 8 cells around
+    // the node are loaded first, all at once (indices clamped to the grid; a cell outside it gets
+    // zero coefficients), then the 24 tetrahedra at the node are summed in a fixed order.
     __device__ __forceinline__ void apply3(const T* x, int i, int j, int kk, T out[3]) const {
-        const size_t cells = (size_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1);
+        const int ncx = g.nx - 1, ncy = g.ny - 1, ncz = g.nz - 1;
+        const size_t cells = (size_t)ncx * ncy * ncz;
+        T u[27][3];
+#pragma unroll
+        for (int q = 0; q < 27; ++q) {
+            const int ni = min(max(i + q % 3 - 1, 0), g.nx - 1), nj = min(max(j + (q / 3) % 3 - 1, 0), g.ny - 1),
+                      nk = min(max(kk + q / 9 - 1, 0), g.nz - 1);
+            const size_t node = ni + (size_t)g.nx * (nj + (size_t)g.ny * nk);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) u[q][a] = x[3 * node + a];
+        }
+        T lm[8][6][2];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int ex = i - 1 + (c & 1), ey = j - 1 + ((c >> 1) & 1), ez = kk - 1 + (c >> 2);
+            const bool in = ex >= 0 && ey >= 0 && ez >= 0 && ex < ncx && ey < ncy && ez < ncz;
+            const size_t cell = min(max(ex, 0), ncx - 1) + (size_t)ncx * (min(max(ey, 0), ncy - 1) +
+                                                                           (size_t)ncy * min(max(ez, 0), ncz - 1));
+#pragma unroll
+            for (int t = 0; t < 6; ++t) {
+                const T lam = ld<T>(coef, (0 * 6 + t) * cells + cell), mu = ld<T>(coef, (1 * 6 + t) * cells + cell);
+                lm[c][t][0] = in ? lam : T(0);
+                lm[c][t][1] = in ? mu : T(0);
+            }
+        }
         out[0] = out[1] = out[2] = T(0);
 #pragma unroll
         for (int cz = 0; cz < 2; ++cz)
@@ -311,9 +365,7 @@ struct Tet4Elastic {
             for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
                 for (int cx = 0; cx < 2; ++cx) {
-                    const int ex = i - 1 + cx, ey = j - 1 + cy, ez = kk - 1 + cz;
-                    if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx - 1 || ey >= g.ny - 1 || ez >= g.nz - 1) continue;
-                    const size_t cell = ex + (size_t)(g.nx - 1) * (ey + (size_t)(g.ny - 1) * ez);
+                    const int c = cx | (cy << 1) | (cz << 2);
                     const int lx = 1 - cx, ly = 1 - cy, lz = 1 - cz;
 #pragma unroll
                     for (int t = 0; t < 6; ++t) {
@@ -321,23 +373,19 @@ struct Tet4Elastic {
                         if (me < 0) continue;
                         // G[a][b] = d u_a / d x_b = sum_v u_va g_v[b]
                         T G[3][3] = {{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, {T(0), T(0), T(0)}};
-                        int bits[3] = {0, 0, 0};
 #pragma unroll
                         for (int v = 0; v < 4; ++v) {
-                            if (v > 0) bits[KUHN[t][v - 1]] = 1;
-                            const size_t node =
-                                (ex + bits[0]) + (size_t)g.nx * ((ey + bits[1]) + (size_t)g.ny * (ez + bits[2]));
+                            // chain vertex v of type t: the first v axes of the chain set
+                            const int bx = kuhn_chain_bit(t, v, 0), by = kuhn_chain_bit(t, v, 1), bz = kuhn_chain_bit(t, v, 2);
+                            const int q = (cx + bx) + 3 * ((cy + by) + 3 * (cz + bz));
                             T gv[3];
                             kuhn_grad<T>(t, v, ih, gv);
 #pragma unroll
-                            for (int a = 0; a < 3; ++a) {
-                                const T u = x[3 * node + a];
+                            for (int a = 0; a < 3; ++a)
 #pragma unroll
-                                for (int b = 0; b < 3; ++b) G[a][b] += u * gv[b];
-                            }
+                                for (int b = 0; b < 3; ++b) G[a][b] += u[q][a] * gv[b];
                         }
-                        const T lam = ld<T>(coef, (0 * 6 + t) * cells + cell);
-                        const T mu = ld<T>(coef, (1 * 6 + t) * cells + cell);
+                        const T lam = lm[c][t][0], mu = lm[c][t][1];
                         const T tr = G[0][0] + G[1][1] + G[2][2];
                         T gm[3];
                         kuhn_grad<T>(t, me, ih, gm);
