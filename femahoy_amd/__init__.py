@@ -94,6 +94,23 @@ class _Stats(C.Structure):
 
 CANCEL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)
 
+UNKNOWNS = {"grid": 0, "points": 1, "quadratic": 2}
+
+
+class _Csr(C.Structure):
+    _fields_ = [("rows", C.c_uint64), ("cols", C.c_uint64), ("ptr", C.POINTER(C.c_uint64)),
+                ("idx", C.POINTER(C.c_uint32)), ("val", C.POINTER(C.c_double))]
+
+
+class _GridLevel(C.Structure):
+    _fields_ = [("planes", C.POINTER(C.c_double) * 3), ("nplanes", C.c_uint64 * 3),
+                ("points", C.POINTER(C.c_uint32)), ("comp", C.POINTER(C.c_uint8))]
+
+
+class _Unknowns(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("grid", _GridLevel), ("p", _Csr), ("coarse_has_grid", C.c_int32),
+                ("coarse_grid", _GridLevel)]
+
 class _ServerOptions(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_active", C.c_int32), ("device", C.c_int32),
                 ("setup_threads", C.c_int32)]
@@ -124,6 +141,7 @@ EXPORTS = (
     "fem_model_tet_cells",
     "fem_solver_set_profile", "fem_solver_profile", "fem_solver_time_kernels",
     "fem_server_new", "fem_server_submit", "fem_server_wait", "fem_server_cancel", "fem_server_free",
+    "fem_solver_new_csr",
 )
 
 _lib = None
@@ -183,6 +201,14 @@ def lib() -> C.CDLL:
                                       C.POINTER(RequestInfo), ERR, SZ]),
         "fem_server_cancel": (C.c_int, [P, C.c_uint64]),
         "fem_server_free": (None, [P]),
+        "fem_solver_new_csr": (C.c_int, [C.POINTER(_Csr), C.POINTER(_Unknowns), C.POINTER(_Options), C.POINTER(P),
+                                         ERR, SZ]),
+        "fem_test_csr_levels_new": (C.c_int, [C.POINTER(_Csr), C.POINTER(_GridLevel), C.c_uint64, C.c_int,
+                                              C.POINTER(P), ERR, SZ]),
+        "fem_test_csr_levels_count": (C.c_int, [P]),
+        "fem_test_csr_level_sizes": (C.c_int, [P, C.c_int] + [C.POINTER(C.c_uint64)] * 4),
+        "fem_test_csr_level_arrays": (C.c_int, [P, C.c_int, P, P, P, P, P, P, P, P]),
+        "fem_test_csr_levels_free": (None, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -444,6 +470,120 @@ class Solver:
         if lib().fem_solver_solve_device(self.h, C.c_void_p(d_b), C.c_void_p(d_x), cb, None, C.byref(st), err, 2048):
             raise FemError(err.value.decode())
         return _stats(st)
+
+
+@dataclass
+class Grid:
+    """The unknowns of a grid level (multigrid.rs:55-59 Level): the plane coordinates of each axis,
+    and the grid point (plane indices) and component of each unknown."""
+
+    planes: Sequence[np.ndarray]
+    points: np.ndarray  # (n, 3) plane indices
+    comp: np.ndarray    # (n,) 0 for a scalar, 0..2 for a displacement
+
+    def _struct(self, n: int):
+        g = _GridLevel()
+        keep = []
+        for d in range(3):
+            pl = np.ascontiguousarray(self.planes[d], np.float64)
+            keep.append(pl)
+            g.planes[d] = pl.ctypes.data_as(C.POINTER(C.c_double))
+            g.nplanes[d] = pl.size
+        pts = np.ascontiguousarray(self.points, np.uint32).reshape(-1)
+        cmp = np.ascontiguousarray(self.comp, np.uint8).reshape(-1)
+        if pts.size != 3 * n or cmp.size != n:
+            raise FemError(f"the grid level needs a point and a component for each of {n} unknowns")
+        keep += [pts, cmp]
+        g.points = pts.ctypes.data_as(C.POINTER(C.c_uint32))
+        g.comp = cmp.ctypes.data_as(C.POINTER(C.c_uint8))
+        return g, keep
+
+
+def _csr_struct(ptr, idx, val, rows: int, cols: int):
+    ptr = np.ascontiguousarray(ptr, np.uint64)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    val = np.ascontiguousarray(val, np.float64)
+    if ptr.size != rows + 1 or idx.size != int(ptr[-1]) or val.size != int(ptr[-1]):
+        raise FemError("a CSR matrix needs rows + 1 offsets and ptr[rows] columns and values")
+    m = _Csr()
+    m.rows, m.cols = rows, cols
+    m.ptr = ptr.ctypes.data_as(C.POINTER(C.c_uint64))
+    m.idx = idx.ctypes.data_as(C.POINTER(C.c_uint32))
+    m.val = val.ctypes.data_as(C.POINTER(C.c_double))
+    return m, [ptr, idx, val]
+
+
+class CsrSolver(Solver):
+    """Solver::new(a, unknowns, method, crossover, None) (solve.rs:92-183) for an assembled matrix
+    of the free unknowns (CSR, both triangles, sorted rows): `unknowns` is "grid" (with `grid`),
+    "points", or "quadratic" (with the prolongation `p` = (ptr, idx, val, cols) and, optionally,
+    `coarse_grid`, the grid level of the linear unknowns). b and x are in the matrix's order."""
+
+    def __init__(self, ptr, idx, val, unknowns: str = "grid", grid: Optional[Grid] = None, p=None,
+                 coarse_grid: Optional[Grid] = None, method: Optional[str] = None, precision: str = "fp32",
+                 tol: float = PCG_TOL, max_iter: int = PCG_MAX_ITER, crossover: int = STEADY_DIRECT_LIMIT,
+                 levels: int = 0, cheb_degree: int = 3, cheb_lower: float = 0.1, cheb_upper: float = 1.1,
+                 power_steps: int = 20, coarse_max: int = 6000, refine: bool = True, device: int = 0):
+        n = len(ptr) - 1
+        o = _options(method, precision, tol, max_iter, crossover, levels, cheb_degree, cheb_lower, cheb_upper,
+                     power_steps, coarse_max, refine, device, True)
+        a, keep = _csr_struct(ptr, idx, val, n, n)
+        u = _Unknowns()
+        u.kind = UNKNOWNS[unknowns]
+        if unknowns == "grid":
+            if grid is None:
+                raise FemError("grid unknowns need their grid level")
+            u.grid, k = grid._struct(n)
+            keep += k
+        elif unknowns == "quadratic":
+            if p is None:
+                raise FemError("quadratic unknowns need the prolongation")
+            pptr, pidx, pval, pcols = p
+            u.p, k = _csr_struct(pptr, pidx, pval, n, int(pcols))
+            keep += k
+            if coarse_grid is not None:
+                u.coarse_has_grid = 1
+                u.coarse_grid, k = coarse_grid._struct(int(pcols))
+                keep += k
+        h = C.c_void_p()
+        err = _err()
+        if lib().fem_solver_new_csr(C.byref(a), C.byref(u), C.byref(o), C.byref(h), err, 2048):
+            raise FemError(err.value.decode())
+        del keep
+        self.h = h
+        self.n = int(lib().fem_solver_free_count(self.h))
+
+
+def csr_levels(ptr, idx, val, grid: Grid, coarsest: int = 3000, max_levels: int = 0):
+    """The host setup of the assembled grid multigrid alone (no device): the reference's level loop
+    with coarsen and Galerkin products. Returns a list of dicts per level: n, a = (ptr, idx, val),
+    p = (ptr, idx, val, cols) from the next level (None on the last), points, comp."""
+    n = len(ptr) - 1
+    a, keep = _csr_struct(ptr, idx, val, n, n)
+    g, k = grid._struct(n)
+    keep += k
+    h = C.c_void_p()
+    err = _err()
+    L = lib()
+    if L.fem_test_csr_levels_new(C.byref(a), C.byref(g), coarsest, max_levels, C.byref(h), err, 2048):
+        raise FemError(err.value.decode())
+    try:
+        out = []
+        for l in range(L.fem_test_csr_levels_count(h)):
+            sz = [C.c_uint64() for _ in range(4)]
+            L.fem_test_csr_level_sizes(h, l, *[C.byref(v) for v in sz])
+            nl, annz, pcols, pnnz = [int(v.value) for v in sz]
+            ap, ai, av = np.zeros(nl + 1, np.uint64), np.zeros(annz, np.uint32), np.zeros(annz)
+            last = pcols == 0 and pnnz == 0 and l == L.fem_test_csr_levels_count(h) - 1
+            pp, pi, pv = np.zeros(nl + 1, np.uint64), np.zeros(pnnz, np.uint32), np.zeros(pnnz)
+            pts, cmp = np.zeros(3 * nl, np.uint32), np.zeros(nl, np.uint8)
+            vp = lambda x: C.c_void_p(x.ctypes.data)
+            L.fem_test_csr_level_arrays(h, l, vp(ap), vp(ai), vp(av), vp(pp), vp(pi), vp(pv), vp(pts), vp(cmp))
+            out.append(dict(n=nl, a=(ap, ai, av), p=None if last else (pp, pi, pv, pcols), points=pts.reshape(-1, 3),
+                            comp=cmp))
+        return out
+    finally:
+        L.fem_test_csr_levels_free(h)
 
 
 def _stats(st: _Stats) -> Stats:

@@ -1,0 +1,52 @@
+// The solver of the assembled boundary: Solver::new(a, unknowns, method, crossover, None) of
+// crates/fem/src/solve.rs:92-183 for a CSR matrix of the free unknowns, on the device.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+
+#include "../../include/fem_solver.h"
+#include "csr_host.hpp"
+#include "solver_common.hpp"
+
+namespace fem {
+
+// The unknowns of the boundary (solve.rs:64-77) on the host.
+struct CsrUnknowns {
+    int kind = FEM_UNKNOWNS_GRID;
+    csr::GridLevel grid;                // GRID
+    csr::Rect p;                        // QUADRATIC: the prolongation from the linear unknowns
+    bool coarse_grid = false;           // QUADRATIC: the linear unknowns' grid level is given
+    csr::GridLevel coarse;              // ... and here it is
+};
+
+class CsrSolverImpl;
+
+class CsrSolver {
+   public:
+    CsrSolver(csr::Csr a, CsrUnknowns u, const Options& o);
+    ~CsrSolver();
+    int method() const;
+    uint64_t n() const;
+    Options& options();
+    void solve_device(const double* d_b, double* d_x, int (*cancelled)(void*), void* ctx, fem_stats* st);
+    void solve_host(const double* b, double* x, int (*cancelled)(void*), void* ctx, fem_stats* st);
+    void apply_host(const double* x, double* y);
+    void precondition_host(const double* r, double* z);
+    void level_info(int level, uint64_t* unknowns, double* lmax) const;
+    const char* device_name() const;
+    // The host hierarchy (levels' matrices and prolongations) the device levels were made from.
+    const csr::GridHierarchy& host_levels() const;
+
+   private:
+    std::unique_ptr<CsrSolverImpl> impl_;
+};
+
+// The boundary's C structs on the host (validated by the solver).
+csr::Csr csr_from(const fem_csr* a);
+csr::Rect rect_from(const fem_csr* p);
+csr::GridLevel grid_from(const fem_grid_level* g, uint64_t n);
+CsrUnknowns unknowns_from(const fem_unknowns* u, uint64_t n);
+
+}  // namespace fem

@@ -304,7 +304,7 @@ struct PcgScalars {
 };
 
 // Configures a PCG pass: tolerance and the iterations it may take.
-__global__ void k_pcg_config(PcgScalars* s, double tol, double it_limit) {
+static __global__ void k_pcg_config(PcgScalars* s, double tol, double it_limit) {
     if (threadIdx.x != 0) return;
     s->tol = tol;
     s->it_limit = it_limit;
@@ -477,7 +477,7 @@ __device__ __forceinline__ double sum_partials(const double* part, int count) {
 }
 
 // The scalars from block partials (pcg_scalars_apply).
-__global__ void __launch_bounds__(RBLOCK) k_pcg_scalars(PcgScalars* s, const double* part_a, const double* part_b,
+static __global__ void __launch_bounds__(RBLOCK) k_pcg_scalars(PcgScalars* s, const double* part_a, const double* part_b,
                                                        int count, int what) {
     const double a = sum_partials(part_a, count);
     double b = 0.0;
@@ -499,21 +499,21 @@ __global__ void __launch_bounds__(BLOCK) k_fill(size_t n, T* __restrict__ a, T v
 }
 
 // y = a x + b y.
-__global__ void __launch_bounds__(BLOCK) k_axpby(size_t n, double a, const double* __restrict__ x, double b,
+static __global__ void __launch_bounds__(BLOCK) k_axpby(size_t n, double a, const double* __restrict__ x, double b,
                                                  double* __restrict__ y) {
     const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
     if (u < n) y[u] = a * x[u] + b * y[u];
 }
 
 // grid[map[k]] = src[k] (the rest of grid is left as it is).
-__global__ void __launch_bounds__(BLOCK) k_scatter(size_t n, const int64_t* __restrict__ map, const double* __restrict__ src,
+static __global__ void __launch_bounds__(BLOCK) k_scatter(size_t n, const int64_t* __restrict__ map, const double* __restrict__ src,
                                                    double* __restrict__ grid) {
     const size_t k = blockIdx.x * (size_t)BLOCK + threadIdx.x;
     if (k < n) grid[map[k]] = src[k];
 }
 
 // dst[k] = grid[map[k]].
-__global__ void __launch_bounds__(BLOCK) k_gather(size_t n, const int64_t* __restrict__ map, const double* __restrict__ grid,
+static __global__ void __launch_bounds__(BLOCK) k_gather(size_t n, const int64_t* __restrict__ map, const double* __restrict__ grid,
                                                   double* __restrict__ dst) {
     const size_t k = blockIdx.x * (size_t)BLOCK + threadIdx.x;
     if (k < n) dst[k] = grid[map[k]];
@@ -531,7 +531,7 @@ __global__ void __launch_bounds__(BLOCK) k_dot(size_t n, const T* __restrict__ a
 }
 
 // out[0] = sum of the partials.
-__global__ void __launch_bounds__(RBLOCK) k_sum(const double* part, int count, double* out) {
+static __global__ void __launch_bounds__(RBLOCK) k_sum(const double* part, int count, double* out) {
     const double v = sum_partials(part, count);
     if (threadIdx.x == 0) out[0] = v;
 }
@@ -547,7 +547,7 @@ __global__ void __launch_bounds__(BLOCK) k_normalise(size_t n, T* __restrict__ a
 
 // Rounds an FP32 array to BF16: a keeps the rounded values (so that setup sees the operator the
 // smoother applies) and h their 16-bit storage.
-__global__ void __launch_bounds__(BLOCK) k_round_bf16(size_t n, float* __restrict__ a, bf16_t* __restrict__ h) {
+static __global__ void __launch_bounds__(BLOCK) k_round_bf16(size_t n, float* __restrict__ a, bf16_t* __restrict__ h) {
     const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
     if (u >= n) return;
     const bf16_t v = bf16_round(a[u]);

@@ -29,32 +29,12 @@
 #include "tet10_fused.hpp"
 #include "tet10_kuhn.hpp"
 #include "setup_kernels.hpp"
+#include "solver_common.hpp"
+#include "csr_solver.hpp"
 
 namespace fem {
 
 namespace {
-
-inline unsigned nblocks(size_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
-
-// Launches `kernel` on `blocks` blocks of BLOCK threads.
-template <class... KArgs, class... Args>
-void launch_blocks(void (*kernel)(KArgs...), unsigned blocks, hipStream_t s, Args&&... args) {
-    if (blocks == 0) return;
-    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(BLOCK), 0, s, static_cast<KArgs>(args)...);
-}
-
-// Launches a partial-sum kernel: one block of RBLOCK threads.
-template <class... KArgs, class... Args>
-void launch_reduce(void (*kernel)(KArgs...), hipStream_t s, Args&&... args) {
-    hipLaunchKernelGGL(kernel, dim3(1), dim3(RBLOCK), 0, s, static_cast<KArgs>(args)...);
-}
-
-// Launches `kernel` with one thread per item of n items.
-template <class... KArgs, class... Args>
-void launch(void (*kernel)(KArgs...), size_t n, hipStream_t s, Args&&... args) {
-    if (n == 0) return;
-    launch_blocks(kernel, nblocks(n), s, std::forward<Args>(args)...);
-}
 
 enum class OpKind { Hex8K, Stencil27, Tet4Heat, Tet4Elastic, Tet10 };
 
@@ -515,22 +495,6 @@ class Hierarchy {
 };
 
 // ---------------------------------------------------------------------------------------------
-struct Options {
-    double tol = 1e-10;
-    uint64_t max_iter = 500;
-    int method = FEM_METHOD_AUTO;
-    int precision = FEM_PREC_FP32;
-    uint64_t crossover = 3000;
-    int levels = 0;
-    int degree = 3;
-    double cheb_lower = 0.1, cheb_upper = 1.1;
-    int power_steps = 20;
-    uint64_t coarse_max = 6000;
-    int refine = 1;
-    int device = 0;
-    int coarse_galerkin = 1;
-};
-
 class Solver {
    public:
     Solver(const BoxInput& in, const Options& o) : opt_(o) {
@@ -1358,15 +1322,6 @@ class Solver {
         return bn > 0.0 ? rn / bn : rn;
     }
 
-    static std::string rust_e2(double v) {
-        char buf[64];
-        std::snprintf(buf, sizeof buf, "%.2e", v);
-        std::string t(buf);
-        const size_t e = t.find('e');
-        if (e == std::string::npos) return t;
-        return t.substr(0, e) + "e" + std::to_string(std::atoi(t.c_str() + e + 1));
-    }
-
 
     // One PCG iteration after the residual check (multigrid.rs:738-750): q = A p, alpha, the x and r
     // updates, z = M^-1 r, rz, beta, p = z + beta p; the next residual is in the scalars. Captured
@@ -1876,8 +1831,10 @@ class Server {
 
 // ---------------------------------------------------------------------------------------------
 // C ABI
+// A box solver (impl) or an assembled one (csr).
 struct fem_solver {
     std::unique_ptr<fem::Solver> impl;
+    std::unique_ptr<fem::CsrSolver> csr;
 };
 
 namespace {
@@ -1964,11 +1921,33 @@ int fem_solver_new_box(const fem_box* box, const fem_options* opt, fem_solver** 
     });
 }
 
+int fem_solver_new_csr(const fem_csr* a, const fem_unknowns* u, const fem_options* opt, fem_solver** out, char* err,
+                       size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!out) throw fem::Error("an output handle is needed");
+        *out = nullptr;
+        fem::csr::Csr m = fem::csr_from(a);
+        fem::CsrUnknowns un = fem::unknowns_from(u, m.n);
+        fem_options def;
+        fem_options_default(&def);
+        const fem::Options o = options_from(opt ? *opt : def);
+        auto* s = new fem_solver;
+        try {
+            s->csr = std::make_unique<fem::CsrSolver>(std::move(m), std::move(un), o);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
 int fem_solver_solve(fem_solver* s, const double* b, double* x, int (*cancelled)(void*), void* ctx, fem_stats* stats,
                      char* err, size_t errlen) {
     return guarded(err, errlen, [&] {
         if (!s) throw fem::Error("no solver");
-        s->impl->solve_host(b, x, cancelled, ctx, stats);
+        if (s->csr) s->csr->solve_host(b, x, cancelled, ctx, stats);
+        else s->impl->solve_host(b, x, cancelled, ctx, stats);
     });
 }
 
@@ -1976,35 +1955,49 @@ int fem_solver_solve_device(fem_solver* s, const double* d_b, double* d_x, int (
                             fem_stats* stats, char* err, size_t errlen) {
     return guarded(err, errlen, [&] {
         if (!s) throw fem::Error("no solver");
-        s->impl->solve_device(d_b, d_x, cancelled, ctx, stats);
+        if (s->csr) s->csr->solve_device(d_b, d_x, cancelled, ctx, stats);
+        else s->impl->solve_device(d_b, d_x, cancelled, ctx, stats);
     });
 }
 
-int fem_solver_method(const fem_solver* s) { return s ? s->impl->method() : -1; }
+int fem_solver_method(const fem_solver* s) {
+    if (!s) return -1;
+    return s->csr ? s->csr->method() : s->impl->method();
+}
 
 void fem_solver_set_max_iter(fem_solver* s, uint64_t max_iter) {
-    if (s) s->impl->options().max_iter = max_iter;
+    if (s) (s->csr ? s->csr->options() : s->impl->options()).max_iter = max_iter;
 }
 
 void fem_solver_set_tol(fem_solver* s, double tol) {
-    if (s) s->impl->options().tol = tol;
+    if (s) (s->csr ? s->csr->options() : s->impl->options()).tol = tol;
 }
 
 void fem_solver_free(fem_solver* s) { delete s; }
 
-uint64_t fem_solver_free_count(const fem_solver* s) { return s ? s->impl->model().free.size() : 0; }
+uint64_t fem_solver_free_count(const fem_solver* s) {
+    if (!s) return 0;
+    return s->csr ? s->csr->n() : s->impl->model().free.size();
+}
 
-uint64_t fem_solver_unknown_count(const fem_solver* s) { return s ? s->impl->model().unknowns() : 0; }
+uint64_t fem_solver_unknown_count(const fem_solver* s) {
+    if (!s) return 0;
+    return s->csr ? s->csr->n() : s->impl->model().unknowns();
+}
 
 int fem_solver_free_list(const fem_solver* s, int64_t* out) {
     if (!s || !out) return 1;
+    if (s->csr) {  // the matrix's unknowns are all free, in its order
+        for (uint64_t i = 0; i < s->csr->n(); ++i) out[i] = (int64_t)i;
+        return 0;
+    }
     const auto& f = s->impl->model().free;
     std::memcpy(out, f.data(), f.size() * sizeof(int64_t));
     return 0;
 }
 
 int fem_solver_point_grid(const fem_solver* s, uint32_t* out) {
-    if (!s || !out) return 1;
+    if (!s || !out || s->csr) return 1;
     const auto& m = s->impl->model();
     for (size_t p = 0; p < m.points; ++p) {
         const uint32_t g = m.point_grid[p];
@@ -2018,6 +2011,7 @@ int fem_solver_point_grid(const fem_solver* s, uint32_t* out) {
 int fem_solver_rhs(fem_solver* s, double* out, char* err, size_t errlen) {
     return guarded(err, errlen, [&] {
         if (!s) throw fem::Error("no solver");
+        if (s->csr) throw fem::Error("an assembled solver has no model: its right side is the caller's");
         s->impl->rhs(out);
     });
 }
@@ -2025,35 +2019,41 @@ int fem_solver_rhs(fem_solver* s, double* out, char* err, size_t errlen) {
 int fem_solver_apply(fem_solver* s, const double* x, double* y, char* err, size_t errlen) {
     return guarded(err, errlen, [&] {
         if (!s) throw fem::Error("no solver");
-        s->impl->apply_host(x, y);
+        if (s->csr) s->csr->apply_host(x, y);
+        else s->impl->apply_host(x, y);
     });
 }
 
 int fem_solver_precondition(fem_solver* s, const double* r, double* z, char* err, size_t errlen) {
     return guarded(err, errlen, [&] {
         if (!s) throw fem::Error("no solver");
-        s->impl->precondition_host(r, z);
+        if (s->csr) s->csr->precondition_host(r, z);
+        else s->impl->precondition_host(r, z);
     });
 }
 
 int fem_solver_level_info(const fem_solver* s, int32_t level, uint64_t* unknowns, double* lmax) {
     if (!s || !unknowns || !lmax) return 1;
     try {
-        s->impl->level_info(level, unknowns, lmax);
+        if (s->csr) s->csr->level_info(level, unknowns, lmax);
+        else s->impl->level_info(level, unknowns, lmax);
         return 0;
     } catch (...) {
         return 1;
     }
 }
 
-const char* fem_solver_device_name(const fem_solver* s) { return s ? s->impl->device_name() : ""; }
+const char* fem_solver_device_name(const fem_solver* s) {
+    if (!s) return "";
+    return s->csr ? s->csr->device_name() : s->impl->device_name();
+}
 
 void fem_solver_set_profile(fem_solver* s, int on) {
-    if (s) s->impl->set_profile(on != 0);
+    if (s && s->impl) s->impl->set_profile(on != 0);
 }
 
 int fem_solver_time_kernels(fem_solver* s, int reps, fem_profile* out) {
-    if (!s || !out) return 1;
+    if (!s || !out || !s->impl) return 1;
     try {
         s->impl->time_kernels(reps, out);
         return 0;
@@ -2063,7 +2063,7 @@ int fem_solver_time_kernels(fem_solver* s, int reps, fem_profile* out) {
 }
 
 int fem_solver_profile(fem_solver* s, fem_profile* out) {
-    if (!s || !out) return 1;
+    if (!s || !out || !s->impl) return 1;
     try {
         s->impl->profile(out);
         return 0;

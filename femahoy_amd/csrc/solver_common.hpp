@@ -1,0 +1,67 @@
+// What the box solver and the assembled (CSR) solver share on the host: kernel launch helpers,
+// the solver options and the reference's number format in error texts.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <utility>
+
+#include "../../include/fem_solver.h"
+#include "kernels.hpp"
+
+namespace fem {
+
+inline unsigned nblocks(size_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+
+// Launches `kernel` on `blocks` blocks of BLOCK threads.
+template <class... KArgs, class... Args>
+void launch_blocks(void (*kernel)(KArgs...), unsigned blocks, hipStream_t s, Args&&... args) {
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(BLOCK), 0, s, static_cast<KArgs>(args)...);
+}
+
+// Launches a partial-sum kernel: one block of RBLOCK threads.
+template <class... KArgs, class... Args>
+void launch_reduce(void (*kernel)(KArgs...), hipStream_t s, Args&&... args) {
+    hipLaunchKernelGGL(kernel, dim3(1), dim3(RBLOCK), 0, s, static_cast<KArgs>(args)...);
+}
+
+// Launches `kernel` with one thread per item of n items.
+template <class... KArgs, class... Args>
+void launch(void (*kernel)(KArgs...), size_t n, hipStream_t s, Args&&... args) {
+    if (n == 0) return;
+    launch_blocks(kernel, nblocks(n), s, std::forward<Args>(args)...);
+}
+
+// The options of a solver (fem_options after defaults).
+struct Options {
+    double tol = 1e-10;
+    uint64_t max_iter = 500;
+    int method = FEM_METHOD_AUTO;
+    int precision = FEM_PREC_FP32;
+    uint64_t crossover = 3000;
+    int levels = 0;
+    int degree = 3;
+    double cheb_lower = 0.1, cheb_upper = 1.1;
+    int power_steps = 20;
+    uint64_t coarse_max = 6000;
+    int refine = 1;
+    int device = 0;
+    int coarse_galerkin = 1;
+};
+
+// Rust's {:.2e} (multigrid.rs:730-733): 2 decimals, the exponent without sign padding or zeros.
+inline std::string rust_e2(double v) {
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%.2e", v);
+    std::string t(buf);
+    const size_t e = t.find('e');
+    if (e == std::string::npos) return t;
+    return t.substr(0, e) + "e" + std::to_string(std::atoi(t.c_str() + e + 1));
+}
+
+}  // namespace fem

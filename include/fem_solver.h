@@ -137,6 +137,58 @@ void fem_solver_set_tol(fem_solver* s, double tol);
 
 void fem_solver_free(fem_solver* s);
 
+/* ---- the assembled boundary: a CSR matrix and its unknowns, as Solver::new takes them ---- */
+
+/* Where the unknowns are (solve.rs:64-77 Unknowns). */
+enum fem_unknowns_kind {
+    FEM_UNKNOWNS_GRID = 0,     /* Unknowns::Grid(Level): grid multigrid (multigrid.rs:84-139) or direct */
+    FEM_UNKNOWNS_POINTS = 1,   /* Unknowns::Points: direct only (solve.rs:132-137) */
+    FEM_UNKNOWNS_QUADRATIC = 2 /* Unknowns::Quadratic: p-multigrid (pmultigrid.rs:46-60) or direct */
+};
+
+/* A sparse matrix in CSR form (sparse.rs:12-17, and Rect, sparse.rs:202-208): row i holds the
+ * columns idx[ptr[i] .. ptr[i+1]), increasing, with the values val at the same places. */
+typedef struct fem_csr {
+    uint64_t rows, cols;   /* a square matrix has cols == rows */
+    const uint64_t* ptr;   /* rows + 1 offsets */
+    const uint32_t* idx;   /* ptr[rows] columns */
+    const double* val;     /* ptr[rows] values */
+} fem_csr;
+
+/* A grid level (multigrid.rs:55-59 Level): the plane coordinates of each axis, increasing, and the
+ * grid point (3 plane indices) and the component (0 for a scalar, 0..2 for a displacement) of
+ * each unknown. */
+typedef struct fem_grid_level {
+    const double* planes[3];
+    uint64_t nplanes[3];
+    const uint32_t* points;  /* 3 per unknown */
+    const uint8_t* comp;     /* 1 per unknown */
+} fem_grid_level;
+
+typedef struct fem_unknowns {
+    int32_t kind;              /* enum fem_unknowns_kind */
+    fem_grid_level grid;       /* GRID: the level of the matrix's unknowns */
+    fem_csr p;                 /* QUADRATIC: the prolongation from the linear unknowns (rows = unknowns,
+                                  cols = linear unknowns; pmultigrid.rs:104-151 makes it) */
+    int32_t coarse_has_grid;   /* QUADRATIC: 1 when coarse_grid gives the linear unknowns' grid level, so
+                                  that grid multigrid continues below the linear level (not in the
+                                  reference, which factors that level: pmultigrid.rs:46-60) */
+    fem_grid_level coarse_grid;
+} fem_unknowns;
+
+/* Replaces Solver::new(a, unknowns, method, crossover, None) (crates/fem/src/solve.rs:92-183)
+ * for an assembled matrix of the free unknowns (symmetric positive definite, both triangles).
+ * The method follows the reference: direct with FEM_METHOD_DIRECT, or with FEM_METHOD_AUTO up to
+ * `crossover` unknowns (Crossover::Unknowns); else grid multigrid (GRID) or p-multigrid
+ * (QUADRATIC); POINTS with FEM_METHOD_MULTIGRID fails with the text of solve.rs:135. The grid
+ * hierarchy is the reference's (multigrid.rs:84-139: coarsen, 329-452, and Galerkin products,
+ * sparse.rs:233-292, down to 3000 unknowns; `levels` > 0 stops earlier); its smoother is
+ * Chebyshev-Jacobi and its coarsest level the dense LDL^T (up to coarse_max unknowns). The direct
+ * solver is the dense LDL^T (up to 20000 unknowns). b and x of fem_solver_solve are the matrix's
+ * unknowns in its order; fem_solver_rhs and fem_solver_point_grid do not apply. */
+int fem_solver_new_csr(const fem_csr* a, const fem_unknowns* u, const fem_options* opt, fem_solver** out, char* err,
+                       size_t errlen);
+
 /* ---- the model's integer structures and right side, for the caller and for parity checks ---- */
 
 /* The number of free unknowns (the length of b and x), and of all unknowns. */

@@ -9,6 +9,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "fem_solver.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -25,6 +27,25 @@ int fem_test_dense_inverse(int n, const double* A, double* Ainv, char* err, size
  * a wait: the calibration of HBM byte counters (FETCH_SIZE / WRITE_SIZE) against a kernel whose
  * traffic is known. */
 int fem_test_copy(const void* d_src, void* d_dst, size_t bytes, char* err, size_t errlen);
+
+/* The host setup of the grid multigrid of the assembled boundary (fem_solver_new_csr), alone: the
+ * reference's level loop (multigrid.rs:84-139) with coarsen (329-452) and Galerkin products
+ * (sparse.rs:233-292), down to `coarsest` unknowns (max_levels > 0: at most that many levels).
+ * No device is touched. */
+typedef struct fem_csr_levels fem_csr_levels;
+int fem_test_csr_levels_new(const fem_csr* a, const fem_grid_level* g, uint64_t coarsest, int max_levels,
+                            fem_csr_levels** out, char* err, size_t errlen);
+int fem_test_csr_levels_count(const fem_csr_levels* h);
+/* Level l: its unknowns n, the entries of its matrix, and the columns and entries of the
+ * prolongation from level l + 1 (0 and 0 on the last level). */
+int fem_test_csr_level_sizes(const fem_csr_levels* h, int l, uint64_t* n, uint64_t* a_nnz, uint64_t* p_cols,
+                             uint64_t* p_nnz);
+/* Copies level l: the matrix (n + 1, a_nnz, a_nnz), the prolongation from level l + 1 (n + 1,
+ * p_nnz, p_nnz; skipped on the last level) and the grid point (3 per unknown) and component of
+ * each unknown. */
+int fem_test_csr_level_arrays(const fem_csr_levels* h, int l, uint64_t* a_ptr, uint32_t* a_idx, double* a_val,
+                              uint64_t* p_ptr, uint32_t* p_idx, double* p_val, uint32_t* points, uint8_t* comp);
+void fem_test_csr_levels_free(fem_csr_levels* h);
 
 #ifdef __cplusplus
 }

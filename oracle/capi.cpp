@@ -346,6 +346,32 @@ void* orc_solver_new(void* h, int method, size_t crossover_unknowns, char* err, 
         return nullptr;
     }
 }
+// A solver of an assembled matrix (a Csr handle) on grid unknowns given as arrays: the plane
+// coordinates of each axis, the grid point (3 plane indices) and the component of each unknown.
+void* orc_grid_solver_new(void* csr, const double* px, size_t nx, const double* py, size_t ny, const double* pz,
+                          size_t nz, const uint32_t* points, const uint8_t* comp, int method,
+                          size_t crossover_unknowns, char* err, size_t errlen) {
+    try {
+        const Csr& a = *static_cast<Csr*>(csr);
+        Unknowns u;
+        u.kind = Unknowns::Grid;
+        u.level.planes = {std::vector<double>(px, px + nx), std::vector<double>(py, py + ny),
+                          std::vector<double>(pz, pz + nz)};
+        for (usize i = 0; i < a.n; ++i) {
+            u.level.points.push_back({points[3 * i], points[3 * i + 1], points[3 * i + 2]});
+            u.level.comp.push_back(comp[i]);
+        }
+        std::optional<Method> m;
+        if (method == 0) m = Method::Direct;
+        if (method == 1) m = Method::Multigrid;
+        Crossover c;
+        c.unknowns = crossover_unknowns;
+        return new Solver(Solver::make(a, u, m, c));
+    } catch (const std::exception& e) {
+        set_err(err, errlen, e.what());
+        return nullptr;
+    }
+}
 int orc_solver_method(void* h) { return (int)static_cast<Solver*>(h)->method(); }
 size_t orc_solver_levels(void* h) {
     const Solver* s = static_cast<Solver*>(h);

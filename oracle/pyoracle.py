@@ -101,6 +101,9 @@ def lib() -> C.CDLL:
         "orc_problem_coarse_xyz": (_SZ, [_P, C.c_void_p]),
         "orc_problem_free_handle": (None, [_P]),
         "orc_solver_new": (_P, [_P, C.c_int, _SZ, C.c_char_p, _SZ]),
+        "orc_grid_solver_new": (_P, [_P, _F64P, _SZ, _F64P, _SZ, _F64P, _SZ, _U32P,
+                                     np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS"), C.c_int, _SZ,
+                                     C.c_char_p, _SZ]),
         "orc_solver_method": (C.c_int, [_P]),
         "orc_solver_levels": (_SZ, [_P]),
         "orc_solver_level_n": (_SZ, [_P, _SZ]),
@@ -494,6 +497,21 @@ class Solver:
         if not self.h:
             raise OracleError(err.value.decode())
         self.n = problem.sizes.n_free
+
+    @classmethod
+    def grid(cls, a: "Csr", planes, points, comp, method=None, crossover=3000) -> "Solver":
+        """solve.rs Solver::new(a, Unknowns::Grid(level), ...) on an assembled matrix."""
+        m = {None: -1, "direct": 0, "multigrid": 1}[method]
+        pl = [np.ascontiguousarray(p, np.float64) for p in planes]
+        err = _err()
+        self = cls.__new__(cls)
+        self.h = lib().orc_grid_solver_new(a.h, pl[0], pl[0].size, pl[1], pl[1].size, pl[2], pl[2].size,
+                                           np.ascontiguousarray(points, np.uint32).reshape(-1),
+                                           np.ascontiguousarray(comp, np.uint8), m, crossover, err, 1024)
+        if not self.h:
+            raise OracleError(err.value.decode())
+        self.n = a.n
+        return self
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -1,0 +1,147 @@
+"""The assembled boundary on the device (fem_solver_new_csr): Solver::new(a, unknowns, ...) of
+solve.rs:92-183 for the oracle's assembled matrices, each solve against the oracle's direct
+(multifrontal Cholesky) solution within 1e-8 relative L2, the method chosen as the reference
+chooses it, and the hierarchy's level sizes equal to the oracle's (the levels of the same
+coarsen and Galerkin products)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL_X = 1e-8
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def heat(oracle, n, length, seed):
+    k = np.exp(np.random.default_rng(seed).standard_normal(int(np.prod(n))))
+    pr = oracle.Problem.heat(n, length, k, fixed={"x0": 0.0, "x1": 1.0})
+    planes = [(length[d] / n[d]) * np.arange(n[d] + 1, dtype=np.float64) for d in range(3)]
+    pts = pr.ijk()[pr.free()]
+    return pr, planes, pts
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_grid_multigrid_on_an_assembled_heat_matrix_matches_oracle_direct(fem, oracle, precision):
+    n, length = [24, 18, 14], [1.0, 0.8, 0.7]
+    pr, planes, pts = heat(oracle, n, length, 5)
+    a = pr.reduced()
+    ptr, idx, val = a.arrays()
+    comp = np.zeros(len(pts), np.uint8)
+    s = fem.CsrSolver(ptr, idx, val, "grid", fem.Grid(planes, pts, comp), method="multigrid", precision=precision,
+                      tol=1e-12)
+    assert s.method() == "multigrid"
+    ref = oracle.Solver.grid(a, planes, pts, comp, method="multigrid")
+    for l in range(ref.levels()):
+        assert s.level_info(l)[0] == ref.level_n(l)
+    xd, _, _ = oracle.Solver(pr, "direct").solve(pr.rhs())
+    x, st = s.solve(pr.rhs())
+    print(precision, st)
+    assert st.levels == ref.levels()
+    assert st.true_residual < 1e-11
+    assert rel(x, xd) < TOL_X
+    # the FP64 operator is the matrix itself
+    v = np.random.default_rng(1).standard_normal(s.n)
+    assert rel(s.apply(v), a.mul(v)) < 1e-14
+
+
+def test_small_assembled_matrices_take_the_direct_solver(fem, oracle):
+    n, length = [10, 8, 6], [1.0, 1.0, 1.0]
+    pr, planes, pts = heat(oracle, n, length, 2)
+    ptr, idx, val = pr.reduced().arrays()
+    g = fem.Grid(planes, pts, np.zeros(len(pts), np.uint8))
+    s = fem.CsrSolver(ptr, idx, val, "grid", g)  # 567 unknowns <= crossover 3000 (solve.rs:158)
+    assert s.method() == "direct"
+    xd, _, _ = oracle.Solver(pr, "direct").solve(pr.rhs())
+    x, st = s.solve(pr.rhs())
+    assert st.iterations == 0 and st.residual < 1e-12
+    assert rel(x, xd) < 1e-10
+
+
+def test_points_unknowns_are_direct_only(fem, oracle):
+    """A mesh of tetrahedra given as points (solve.rs:132-137): direct, and multigrid refused with
+    the reference's text."""
+    n, length = [10, 3, 3], [100.0, 10.0, 10.0]
+    nt = 6 * int(np.prod(n))
+    pr = oracle.Problem.elastic(4, n, length, np.full(nt, 210e3), np.full(nt, 0.3), clamped=["x0"],
+                                traction={"z1": (0, 0, -1.0)})
+    ptr, idx, val = pr.reduced().arrays()
+    with pytest.raises(fem.FemError, match="^multigrid needs the box grid: a mesh of tetrahedra has the direct solver$"):
+        fem.CsrSolver(ptr, idx, val, "points", method="multigrid")
+    s = fem.CsrSolver(ptr, idx, val, "points")
+    assert s.method() == "direct"
+    xd, _, _ = oracle.Solver(pr, "direct").solve(pr.rhs())
+    x, _ = s.solve(pr.rhs())
+    assert rel(x, xd) < 1e-10
+
+
+def quadratic(oracle, n, length, seed):
+    nt = 6 * int(np.prod(n))
+    rng = np.random.default_rng(seed)
+    pr = oracle.Problem.elastic(10, n, length, rng.uniform(1.0, 2.0, nt), rng.uniform(0.2, 0.35, nt),
+                                clamped=["x0"], body=(0, 0, -1))
+    p = pr.prolongation()
+    pp, pi, pv = p.arrays()
+    return pr, (pp, pi, pv, p.cols)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_p_multigrid_on_an_assembled_tet10_matrix_matches_oracle_direct(fem, oracle, precision):
+    """Unknowns::Quadratic: the caller's prolongation (pmultigrid.rs:104-151, made by the oracle),
+    its Galerkin product as the linear level, the dense LDL^T there."""
+    n, length = [8, 4, 4], [2.0, 1.0, 1.0]
+    pr, p = quadratic(oracle, n, length, 3)
+    ptr, idx, val = pr.reduced().arrays()
+    s = fem.CsrSolver(ptr, idx, val, "quadratic", p=p, method="multigrid", precision=precision, tol=1e-12)
+    assert s.method() == "multigrid"
+    assert s.level_info(1)[0] == p[3]
+    xd, _, _ = oracle.Solver(pr, "direct").solve(pr.rhs())
+    x, st = s.solve(pr.rhs())
+    print(precision, st)
+    assert st.levels == 2
+    assert rel(x, xd) < TOL_X
+
+
+def test_p_multigrid_continues_with_grid_levels_below_the_linear_level(fem, oracle):
+    """With the linear unknowns' grid given, the reference's grid coarsening continues below the
+    linear level (here forced by a small dense limit)."""
+    n, length = [16, 8, 8], [2.0, 1.0, 1.0]
+    pr, p = quadratic(oracle, n, length, 4)
+    ptr, idx, val = pr.reduced().arrays()
+    # the linear unknowns: the free corner unknowns 3 v + c (v < corners) in free order
+    corners = pr.sizes.corners
+    free = pr.free()
+    lin = free[free < 3 * corners]
+    assert len(lin) == p[3]
+    h = [length[d] / n[d] for d in range(3)]
+    xyz = pr.points()[lin // 3]
+    pts = np.stack([np.rint(xyz[:, d] / h[d]) for d in range(3)], axis=1).astype(np.uint32)
+    planes = [h[d] * np.arange(n[d] + 1, dtype=np.float64) for d in range(3)]
+    cg = fem.Grid(planes, pts, (lin % 3).astype(np.uint8))
+    s = fem.CsrSolver(ptr, idx, val, "quadratic", p=p, coarse_grid=cg, method="multigrid", tol=1e-12,
+                      coarse_max=1500)
+    x, st = s.solve(pr.rhs())
+    print(st)
+    assert st.levels >= 3 and st.coarse_unknowns <= 1500
+    xd, _, _ = oracle.Solver(pr, "direct").solve(pr.rhs())
+    assert rel(x, xd) < TOL_X
+
+
+def test_assembled_solves_are_bitwise_reproducible_and_report_the_reference_errors(fem, oracle):
+    n, length = [24, 18, 14], [1.0, 1.0, 1.0]
+    pr, planes, pts = heat(oracle, n, length, 7)
+    ptr, idx, val = pr.reduced().arrays()
+    g = fem.Grid(planes, pts, np.zeros(len(pts), np.uint8))
+    s = fem.CsrSolver(ptr, idx, val, "grid", g, method="multigrid", tol=1e-10)
+    b = pr.rhs()
+    x1, s1 = s.solve(b)
+    x2, s2 = s.solve(b)
+    assert s1.iterations == s2.iterations and np.array_equal(x1, x2)
+    s.with_max_iter(1)
+    with pytest.raises(fem.FemError, match=r"^conjugate gradients did not converge: the residual is .* after 1 iterations$"):
+        s.solve(b)
+    s.with_max_iter(500)
+    with pytest.raises(fem.FemError, match="^cancelled$"):
+        s.solve(b, cancelled=lambda: True)
