@@ -154,25 +154,34 @@ struct Stencil27 {
     GridDims g;
     const T* a;
 
-    // Branch-free: a neighbour outside the grid is read at the node itself and selected to 0.
+    // Branch-free: a neighbour outside the grid is read at the node itself and selected to 0. All
+    // 27 coefficients and 27 values are loaded before the sum, so that the loads are in flight
+    // together (the kernels on these levels are latency-bound).
     __device__ __forceinline__ T apply(const T* x, int i, int j, int k) const {
         const size_t n = g.nodes();
         const size_t me = i + (size_t)g.nx * (j + (size_t)g.ny * k);
-        T acc = a[me] * x[me];
+        T af[14], xf[14], ab[14], xb[14];
+        bool vf[14], vb[14];
+        af[0] = a[me];
+        xf[0] = x[me];
 #pragma unroll
         for (int s = 1; s < 14; ++s) {
             const int dx = STENCIL_OFF[s][0], dy = STENCIL_OFF[s][1], dz = STENCIL_OFF[s][2];
             const long off = dx + (long)g.nx * (dy + (long)g.ny * dz);
             const int fi = i + dx, fj = j + dy, fk = k + dz;
-            const bool vf = fi >= 0 && fj >= 0 && fk >= 0 && fi < g.nx && fj < g.ny && fk < g.nz;
+            vf[s] = fi >= 0 && fj >= 0 && fk >= 0 && fi < g.nx && fj < g.ny && fk < g.nz;
             const int bi = i - dx, bj = j - dy, bk = k - dz;
-            const bool vb = bi >= 0 && bj >= 0 && bk >= 0 && bi < g.nx && bj < g.ny && bk < g.nz;
-            const size_t f = vf ? me + off : me;
-            const size_t b = vb ? me - off : me;
-            const T af = a[s * n + me], xf = x[f];
-            const T ab = a[s * n + b], xb = x[b];
-            acc += (vf ? af * xf : T(0)) + (vb ? ab * xb : T(0));
+            vb[s] = bi >= 0 && bj >= 0 && bk >= 0 && bi < g.nx && bj < g.ny && bk < g.nz;
+            const size_t f = vf[s] ? me + off : me;
+            const size_t b = vb[s] ? me - off : me;
+            af[s] = a[s * n + me];
+            xf[s] = x[f];
+            ab[s] = a[s * n + b];
+            xb[s] = x[b];
         }
+        T acc = af[0] * xf[0];
+#pragma unroll
+        for (int s = 1; s < 14; ++s) acc += (vf[s] ? af[s] * xf[s] : T(0)) + (vb[s] ? ab[s] * xb[s] : T(0));
         return acc;
     }
 
