@@ -201,8 +201,14 @@ __global__ void __launch_bounds__(BLOCK) k_prolong_add(GridDims fg, GridDims cg,
         const size_t a = lo[0] + (size_t)cg.nx * (lo[1] + (size_t)cg.ny * lo[2]);
         const size_t b = hi[0] + (size_t)cg.nx * (hi[1] + (size_t)cg.ny * hi[2]);
         const bool same = a == b;
+        T va[C], vb[C];  // both ends loaded unconditionally (b = a for a coarse node)
 #pragma unroll
-        for (int c = 0; c < C; ++c) acc[c] = same ? ec[C * a + c] : T(0.5) * (ec[C * a + c] + ec[C * b + c]);
+        for (int c = 0; c < C; ++c) {
+            va[c] = ec[C * a + c];
+            vb[c] = ec[C * b + c];
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) acc[c] = same ? va[c] : T(0.5) * (va[c] + vb[c]);
     } else {
         // Per axis: the lower coarse node with weight 1 (even coordinate) or 1/2 (odd), the upper
         // one with 0 or 1/2 (for an even coordinate it is the same node). All 8 corners are loaded
@@ -245,23 +251,30 @@ __global__ void __launch_bounds__(BLOCK) k_restrict(GridDims fg, GridDims cg, co
         const int fi = 2 * I, fj = 2 * J, fk = 2 * K;
         if (KUHN_P1) {
             // The coarse node itself, and the 14 midpoints of the Kuhn edges at it: directions
-            // +-(1,0,0), +-(0,1,0), +-(0,0,1), +-(1,1,0), +-(1,0,1), +-(0,1,1), +-(1,1,1).
-            if (fi < fg.nx && fj < fg.ny && fk < fg.nz) {
-                const size_t me = fi + (size_t)fg.nx * (fj + (size_t)fg.ny * fk);
-#pragma unroll
-                for (int c = 0; c < C; ++c) acc[c] = rf[C * me + c];
-            }
+            // +-(1,0,0), +-(0,1,0), +-(0,0,1), +-(1,1,0), +-(1,0,1), +-(0,1,1), +-(1,1,1). All 15
+            // values are loaded first (indices clamped to the grid, weight 0 outside), then summed
+            // in that order.
             const int DIRS[7][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+            T v[15][C];
+            bool in[15];
 #pragma unroll
-            for (int s = 0; s < 7; ++s)
+            for (int q = 0; q < 15; ++q) {
+                const int s = (q - 1) / 2, sign = (q & 1) ? -1 : 1;
+                const int a = q == 0 ? fi : fi + sign * DIRS[s][0], b = q == 0 ? fj : fj + sign * DIRS[s][1],
+                          cz = q == 0 ? fk : fk + sign * DIRS[s][2];
+                in[q] = a >= 0 && b >= 0 && cz >= 0 && a < fg.nx && b < fg.ny && cz < fg.nz;
+                const size_t f = min(max(a, 0), fg.nx - 1) +
+                                 (size_t)fg.nx * (min(max(b, 0), fg.ny - 1) + (size_t)fg.ny * min(max(cz, 0), fg.nz - 1));
 #pragma unroll
-                for (int sign = -1; sign <= 1; sign += 2) {
-                    const int a = fi + sign * DIRS[s][0], b = fj + sign * DIRS[s][1], cz = fk + sign * DIRS[s][2];
-                    if (a < 0 || b < 0 || cz < 0 || a >= fg.nx || b >= fg.ny || cz >= fg.nz) continue;
-                    const size_t f = a + (size_t)fg.nx * (b + (size_t)fg.ny * cz);
+                for (int c = 0; c < C; ++c) v[q][c] = rf[C * f + c];
+            }
 #pragma unroll
-                    for (int c = 0; c < C; ++c) acc[c] += T(0.5) * rf[C * f + c];
-                }
+            for (int c = 0; c < C; ++c) acc[c] = in[0] ? v[0][c] : T(0);
+#pragma unroll
+            for (int q = 1; q < 15; ++q)
+#pragma unroll
+                for (int c = 0; c < C; ++c)
+                    if (in[q]) acc[c] += T(0.5) * v[q][c];
         } else {
 #pragma unroll
             for (int dz = -1; dz <= 1; ++dz)
