@@ -502,12 +502,11 @@ class CsrSolverImpl {
         CLevel<T>& L = H.lv[0];
         const bool smooths = H.lv.size() > 1;
         const unsigned nbv = (unsigned)((n_ + BLOCK - 1) / BLOCK);
-        launch_blocks((k_pcg_update<T, T>), nbv, s_, n_, (const PcgScalars*)scal_.data(), x_.data(), r_.data(),
-                      (const double*)p_.data(), (const double*)q_.data(), L.b.data(), partA_.data(),
+        launch_blocks((k_pcg_update<T, T>), nbv, s_, n_, (const PcgScalars*)scal_.data(), r_.data(),
+                      (const double*)q_.data(), L.b.data(), partB_.data(),
                       smooths ? (const T*)L.invd.data() : (const T*)nullptr, smooths ? L.c2[0] : T(0), L.xa.data(),
                       L.d.data());
-        launch_reduce(k_pcg_scalars, s_, scal_.data(), (const double*)partA_.data(), (const double*)nullptr, (int)nbv, 2);
-        H.cycle(0, s_, smooths);
+        H.cycle(0, s_, smooths);  // r.r (partB_) is summed with r.z after it
     }
 
     // One PCG iteration after the head (multigrid.rs:738-750), gated on the device (PcgScalars).
@@ -531,9 +530,13 @@ class CsrSolverImpl {
             launch_blocks((k_pcg_rz<double>), nbv, s_, n_, (const double*)h64_->lv[0].x(), (double*)nullptr,
                           (const double*)r_.data(), partA_.data());
         }
-        launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr, (int)nbv, 4);
-        if (h32_) launch((k_pcg_dir<float>), n_, s_, n_, (const PcgScalars*)sc, (const float*)h32_->lv[0].x(), p_.data(), 0);
-        else launch((k_pcg_dir<double>), n_, s_, n_, (const PcgScalars*)sc, (const double*)h64_->lv[0].x(), p_.data(), 0);
+        launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)partB_.data(), (int)nbv, 5);
+        if (h32_)
+            launch((k_pcg_dir<float>), n_, s_, n_, (const PcgScalars*)sc, (const float*)h32_->lv[0].x(), p_.data(),
+                   x_.data(), 0);
+        else
+            launch((k_pcg_dir<double>), n_, s_, n_, (const PcgScalars*)sc, (const double*)h64_->lv[0].x(), p_.data(),
+                   x_.data(), 0);
     }
 
     void iterate() {
@@ -582,7 +585,8 @@ class CsrSolverImpl {
                           partA_.data());
             launch_reduce(k_pcg_scalars, s_, scal_.data(), (const double*)partA_.data(), (const double*)nullptr,
                           (int)nbv, 3);
-            launch((k_pcg_dir<double>), n_, s_, n_, (const PcgScalars*)scal_.data(), (const double*)z_.data(), p_.data(), 1);
+            launch((k_pcg_dir<double>), n_, s_, n_, (const PcgScalars*)scal_.data(), (const double*)z_.data(), p_.data(),
+                   x_.data(), 1);
             uint64_t it = 0;
             double residual = std::sqrt(ps.rr) / norm_b, rate = 0.0;
             bool stopped = ps.active == 0.0;

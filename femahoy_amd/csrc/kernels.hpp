@@ -330,14 +330,15 @@ __device__ __forceinline__ double pcg_next_active(const PcgScalars* s, double rr
 
 // what: 0 start (bb, rr, and the stopping test), 1 the head of an iteration (go = active, the
 // count) then alpha = rz / pq, 2 rr (and the stopping test for the next iteration), 3 rz (first:
-// rz = rz'), 4 beta = rz'/rz then rz = rz'. 1, 2 and 4 change nothing else in an iteration that
-// does not run (go = 0). Run by one thread with the sums a (and b for what 0).
+// rz = rz'), 4 beta = rz'/rz then rz = rz', 5 both 2 (from b) and 4 (from a). 1, 2, 4 and 5
+// change nothing else in an iteration that does not run (go = 0). Run by one thread with the
+// sums a (and b for what 0 and 5).
 __device__ __forceinline__ void pcg_scalars_apply(PcgScalars* s, double a, double b, int what) {
     if (what == 1) {
         s->go = s->active;
         if (s->go != 0.0) s->it += 1.0;
     }
-    if ((what == 1 || what == 2 || what == 4) && s->go == 0.0) return;
+    if ((what == 1 || what == 2 || what == 4 || what == 5) && s->go == 0.0) return;
     switch (what) {
         case 0:
             s->bb = a;
@@ -356,6 +357,12 @@ __device__ __forceinline__ void pcg_scalars_apply(PcgScalars* s, double a, doubl
             s->rz = a;
             break;
         case 4:
+            s->beta = a / s->rz;
+            s->rz = a;
+            break;
+        case 5:
+            s->rr = b;
+            s->active = pcg_next_active(s, b);
             s->beta = a / s->rz;
             s->rz = a;
             break;
@@ -423,20 +430,19 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_q(OP op, const double* __restrict
     if (threadIdx.x == 0) part[blockIdx.x] = pq;
 }
 
-// x += alpha p, r -= alpha q, rv = (V) r for the V-cycle; partial sums of r.r. With invd, also
-// the first Chebyshev step of the V-cycle's level 0 from x = 0: d0 = x0 = c0 D^-1 rv (the step
-// k_cheb_first takes, in the same arithmetic).
+// r -= alpha q, rv = (V) r for the V-cycle; partial sums of r.r. With invd, also the first
+// Chebyshev step of the V-cycle's level 0 from x = 0: d0 = x0 = c0 D^-1 rv (the step k_cheb_first
+// takes, in the same arithmetic). x += alpha p is left to k_pcg_dir, which reads p anyway.
 template <class V, class S>
-__global__ void __launch_bounds__(BLOCK) k_pcg_update(size_t n, const PcgScalars* __restrict__ s, double* __restrict__ x,
-                                                      double* __restrict__ r, const double* __restrict__ p,
-                                                      const double* __restrict__ q, V* __restrict__ rv,
-                                                      double* __restrict__ part, const S* __restrict__ invd, V c0,
-                                                      V* __restrict__ x0, V* __restrict__ d0) {
+__global__ void __launch_bounds__(BLOCK) k_pcg_update(size_t n, const PcgScalars* __restrict__ s,
+                                                      double* __restrict__ r, const double* __restrict__ q,
+                                                      V* __restrict__ rv, double* __restrict__ part,
+                                                      const S* __restrict__ invd, V c0, V* __restrict__ x0,
+                                                      V* __restrict__ d0) {
     const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
     double rr = 0.0;
     if (u < n && s->go != 0.0) {
         const double alpha = s->alpha;
-        x[u] += alpha * p[u];
         const double ru = r[u] - alpha * q[u];
         r[u] = ru;
         const V bu = (V)ru;
@@ -472,14 +478,20 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_rz(size_t n, const V* __restrict_
     if (threadIdx.x == 0) part[blockIdx.x] = rz;
 }
 
-// p = z + beta p (first: p = z), z in the preconditioner's precision.
+// x += alpha p (the iteration's update of x, multigrid.rs:741), then p = z + beta p; first: p = z
+// alone. z in the preconditioner's precision.
 template <class V>
 __global__ void __launch_bounds__(BLOCK) k_pcg_dir(size_t n, const PcgScalars* __restrict__ s, const V* __restrict__ z,
-                                                   double* __restrict__ p, int first) {
+                                                   double* __restrict__ p, double* __restrict__ x, int first) {
     const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
     if (u >= n) return;
-    if (first) p[u] = (double)z[u];
-    else if (s->go != 0.0) p[u] = (double)z[u] + s->beta * p[u];
+    if (first) {
+        p[u] = (double)z[u];
+    } else if (s->go != 0.0) {
+        const double pu = p[u];
+        x[u] += s->alpha * pu;
+        p[u] = (double)z[u] + s->beta * pu;
+    }
 }
 
 // Sum of `count` partials in a fixed order, by one block of RBLOCK threads.
@@ -494,7 +506,7 @@ static __global__ void __launch_bounds__(RBLOCK) k_pcg_scalars(PcgScalars* s, co
                                                        int count, int what) {
     const double a = sum_partials(part_a, count);
     double b = 0.0;
-    if (what == 0) b = sum_partials(part_b, count);
+    if (what == 0 || what == 5) b = sum_partials(part_b, count);
     if (threadIdx.x == 0) pcg_scalars_apply(s, a, b, what);
 }
 

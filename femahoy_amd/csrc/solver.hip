@@ -754,7 +754,8 @@ class Solver {
                                   (const double*)r_.data(), partA_.data());
                     launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr,
                                   (int)nbv, 3);
-                    launch((k_pcg_dir<double>), n, s_, n, (const PcgScalars*)sc, (const double*)z_.data(), p_.data(), 1);
+                    launch((k_pcg_dir<double>), n, s_, n, (const PcgScalars*)sc, (const double*)z_.data(), p_.data(),
+                           x_.data(), 1);
                     run_.it = 0;
                     run_.residual = std::sqrt(rr) / run_.norm_b;
                     run_.rate = 0.0;
@@ -1405,12 +1406,12 @@ class Solver {
         const bool smooths = H.lv.size() > 1;
         with_invd(L, [&](auto iv) {
             using S = elem_t<decltype(iv)>;
-            launch_blocks((k_pcg_update<T, S>), nblocks(n), s_, n, (const PcgScalars*)scal_.data(), x_.data(),
-                          r_.data(), (const double*)p_.data(), (const double*)q_.data(), L.b.data(), partA_.data(),
-                          smooths ? iv : (const S*)nullptr, smooths ? L.c2[0] : T(0), L.xa.data(), L.d.data());
+            launch_blocks((k_pcg_update<T, S>), nblocks(n), s_, n, (const PcgScalars*)scal_.data(), r_.data(),
+                          (const double*)q_.data(), L.b.data(), partB_.data(), smooths ? iv : (const S*)nullptr,
+                          smooths ? L.c2[0] : T(0), L.xa.data(), L.d.data());
         });
-        launch_reduce(k_pcg_scalars, s_, scal_.data(), (const double*)partA_.data(), (const double*)nullptr,
-                      (int)nblocks(n), 2);
+        // r.r (partB_) waits for the scalars after r.z (what 5): the stopping test is read at the
+        // next iteration's head
         H.vcycle(s_, smooths);
     }
 
@@ -1433,10 +1434,14 @@ class Solver {
             launch_blocks((k_pcg_rz<double>), nbv, s_, n, (const double*)h64_->lv[0].x(), (double*)nullptr,
                           (const double*)r_.data(), partA_.data());
         }
-        launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr, (int)nbv, 4);
+        launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)partB_.data(), (int)nbv, 5);
         // z straight from the V-cycle's level-0 iterate (kept until the next V-cycle)
-        if (h32_) launch((k_pcg_dir<float>), n, s_, n, (const PcgScalars*)sc, (const float*)h32_->lv[0].x(), p_.data(), 0);
-        else launch((k_pcg_dir<double>), n, s_, n, (const PcgScalars*)sc, (const double*)h64_->lv[0].x(), p_.data(), 0);
+        if (h32_)
+            launch((k_pcg_dir<float>), n, s_, n, (const PcgScalars*)sc, (const float*)h32_->lv[0].x(), p_.data(),
+                   x_.data(), 0);
+        else
+            launch((k_pcg_dir<double>), n, s_, n, (const PcgScalars*)sc, (const double*)h64_->lv[0].x(), p_.data(),
+                   x_.data(), 0);
     }
 
     void iterate() {
