@@ -501,7 +501,7 @@ class CsrSolverImpl {
     void update_and_vcycle(CHierarchy<T>& H) {
         CLevel<T>& L = H.lv[0];
         const bool smooths = H.lv.size() > 1;
-        const unsigned nbv = (unsigned)((n_ + BLOCK - 1) / BLOCK);
+        const unsigned nbv = rblocks(n_);
         launch_blocks((k_pcg_update<T, T>), nbv, s_, n_, (const PcgScalars*)scal_.data(), r_.data(),
                       (const double*)q_.data(), L.b.data(), partB_.data(),
                       smooths ? (const T*)L.invd.data() : (const T*)nullptr, smooths ? L.c2[0] : T(0), L.xa.data(),
@@ -511,7 +511,7 @@ class CsrSolverImpl {
 
     // One PCG iteration after the head (multigrid.rs:738-750), gated on the device (PcgScalars).
     void iteration_body() {
-        const unsigned nbv = (unsigned)((n_ + BLOCK - 1) / BLOCK);
+        const unsigned nbv = rblocks(n_);
         PcgScalars* sc = scal_.data();
         const unsigned nq = row_blocks(n_, a64_.lanes);
         with_lanes(a64_.lanes, [&](auto lc) {
@@ -560,7 +560,7 @@ class CsrSolverImpl {
     // multigrid.rs:692-753 with the refinement restarts of the box solver: passes of PCG from the
     // current x, each ended by the device's stopping test, queued in batches between readbacks.
     void pcg(int (*cancelled)(void*), void* ctx, fem_stats& out) {
-        const unsigned nbv = (unsigned)((n_ + BLOCK - 1) / BLOCK);
+        const unsigned nbv = rblocks(n_);
         uint64_t total = 0;
         int restarts = 0;
         for (;;) {

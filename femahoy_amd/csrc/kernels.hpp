@@ -430,7 +430,7 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_q(OP op, const double* __restrict
     if (threadIdx.x == 0) part[blockIdx.x] = pq;
 }
 
-// r -= alpha q, rv = (V) r for the V-cycle; partial sums of r.r. With invd, also the first
+// r -= alpha q, rv = (V) r for the V-cycle; partial sums of r.r (grid-stride: any grid size). With invd, also the first
 // Chebyshev step of the V-cycle's level 0 from x = 0: d0 = x0 = c0 D^-1 rv (the step k_cheb_first
 // takes, in the same arithmetic). x += alpha p is left to k_pcg_dir, which reads p anyway.
 template <class V, class S>
@@ -439,32 +439,33 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_update(size_t n, const PcgScalars
                                                       V* __restrict__ rv, double* __restrict__ part,
                                                       const S* __restrict__ invd, V c0, V* __restrict__ x0,
                                                       V* __restrict__ d0) {
-    const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
     double rr = 0.0;
-    if (u < n && s->go != 0.0) {
+    if (s->go != 0.0) {
         const double alpha = s->alpha;
-        const double ru = r[u] - alpha * q[u];
-        r[u] = ru;
-        const V bu = (V)ru;
-        rv[u] = bu;
-        if (invd) {
-            const V v = c0 * ld<V>(invd, u) * bu;
-            d0[u] = v;
-            x0[u] = v;
+        for (size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x; u < n; u += (size_t)gridDim.x * BLOCK) {
+            const double ru = r[u] - alpha * q[u];
+            r[u] = ru;
+            const V bu = (V)ru;
+            rv[u] = bu;
+            if (invd) {
+                const V v = c0 * ld<V>(invd, u) * bu;
+                d0[u] = v;
+                x0[u] = v;
+            }
+            rr += ru * ru;
         }
-        rr = ru * ru;
     }
     rr = block_sum(rr);
     if (threadIdx.x == 0) part[blockIdx.x] = rr;
 }
 
-// Partial sums of r.z, z read from zv (if zv; also stored to z as FP64 if z) or from z.
+// Partial sums of r.z (grid-stride), z read from zv (if zv; also stored to z as FP64 if z) or
+// from z.
 template <class V>
 __global__ void __launch_bounds__(BLOCK) k_pcg_rz(size_t n, const V* __restrict__ zv, double* __restrict__ z,
                                                   const double* __restrict__ r, double* __restrict__ part) {
-    const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
     double rz = 0.0;
-    if (u < n) {
+    for (size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x; u < n; u += (size_t)gridDim.x * BLOCK) {
         double zu;
         if (zv) {
             zu = (double)zv[u];
@@ -472,7 +473,7 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_rz(size_t n, const V* __restrict_
         } else {
             zu = z[u];
         }
-        rz = r[u] * zu;
+        rz += r[u] * zu;
     }
     rz = block_sum(rz);
     if (threadIdx.x == 0) part[blockIdx.x] = rz;

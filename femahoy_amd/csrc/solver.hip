@@ -747,7 +747,7 @@ class Solver {
                         continue;
                     }
                     const size_t n = n0_;
-                    const unsigned nbv = nblocks(n);
+                    const unsigned nbv = rblocks(n);
                     PcgScalars* sc = scal_.data();
                     precondition();
                     launch_blocks((k_pcg_rz<float>), nbv, s_, n, (const float*)nullptr, z_.data(),
@@ -1406,7 +1406,7 @@ class Solver {
         const bool smooths = H.lv.size() > 1;
         with_invd(L, [&](auto iv) {
             using S = elem_t<decltype(iv)>;
-            launch_blocks((k_pcg_update<T, S>), nblocks(n), s_, n, (const PcgScalars*)scal_.data(), r_.data(),
+            launch_blocks((k_pcg_update<T, S>), rblocks(n), s_, n, (const PcgScalars*)scal_.data(), r_.data(),
                           (const double*)q_.data(), L.b.data(), partB_.data(), smooths ? iv : (const S*)nullptr,
                           smooths ? L.c2[0] : T(0), L.xa.data(), L.d.data());
         });
@@ -1417,7 +1417,7 @@ class Solver {
 
     void iteration_body() {
         const size_t n = n0_;
-        const unsigned nbv = nblocks(n);
+        const unsigned nbv = rblocks(n);
         PcgScalars* sc = scal_.data();
         prof_.begin(1, s_);
         const unsigned nq = outer_apply();

@@ -17,6 +17,9 @@ namespace fem {
 
 inline unsigned nblocks(size_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
 
+// Blocks of a grid-stride reduction over n items (a few items per thread; fewer partials).
+inline unsigned rblocks(size_t n) { return nblocks(n) < 2048u ? nblocks(n) : 2048u; }
+
 // Launches `kernel` on `blocks` blocks of BLOCK threads.
 template <class... KArgs, class... Args>
 void launch_blocks(void (*kernel)(KArgs...), unsigned blocks, hipStream_t s, Args&&... args) {
