@@ -120,7 +120,9 @@ struct CHierarchy {
         CLevel<T>& L = lv[l];
         L.cur = 0;
         if (l + 1 == lv.size()) {
-            coarse.solve<T>(L.b.data(), L.x(), s);
+            // first_done on the dense level: the restriction wrote its dense right side
+            if (first_done) coarse.solve_rhs<T>(L.x(), s);
+            else coarse.solve<T>(L.b.data(), L.x(), s);
             return;
         }
         const uint64_t n = L.n;
@@ -138,14 +140,16 @@ struct CHierarchy {
         });
         CLevel<T>& C = lv[l + 1];
         const bool smoothed = l + 2 < lv.size();
+        // into the dense level: straight to its dense right side (its unknowns are in dense order)
+        T* bc = smoothed ? C.b.data() : coarse.rhs<T>();
         with_lanes(L.r.lanes, [&](auto lc) {
             constexpr int LN = decltype(lc)::value;
             launch_blocks((k_csr_restrict<LN, T, T, T>), row_blocks(C.n, LN), s, C.n, (const uint32_t*)L.r.ptr.data(),
-                          (const uint32_t*)L.r.idx.data(), (const T*)L.r.val.data(), (const T*)L.res.data(),
-                          C.b.data(), smoothed ? (const T*)C.invd.data() : (const T*)nullptr,
-                          smoothed ? C.c2[0] : T(0), C.xa.data(), C.d.data());
+                          (const uint32_t*)L.r.idx.data(), (const T*)L.r.val.data(), (const T*)L.res.data(), bc,
+                          smoothed ? (const T*)C.invd.data() : (const T*)nullptr, smoothed ? C.c2[0] : T(0),
+                          C.xa.data(), C.d.data());
         });
-        cycle(l + 1, s, smoothed);
+        cycle(l + 1, s, true);
         with_lanes(L.p.lanes, [&](auto lc) {
             constexpr int LN = decltype(lc)::value;
             launch_blocks((k_csr_prolong_add<LN, T, T>), row_blocks(n, LN), s, n, (const uint32_t*)L.p.ptr.data(),

@@ -360,8 +360,32 @@ void DenseCoarse::solve(const T* b, T* x, hipStream_t s) const {
     hipLaunchKernelGGL(k_coarse_gemv<T>, dim3((n_ + 3) / 4), dim3(256), 0, s, inv, n_, ld_, (const T*)bc, map_.data(), x);
 }
 
+template <class T>
+T* DenseCoarse::rhs() const {
+    if constexpr (sizeof(T) == 4) {
+        if (!fp32_) throw Error("the coarse inverse is FP64");
+        return bc32_.data();
+    } else {
+        if (fp32_) throw Error("the coarse inverse is FP32");
+        return bc64_.data();
+    }
+}
+
+template <class T>
+void DenseCoarse::solve_rhs(T* x, hipStream_t s) const {
+    const T* inv;
+    if constexpr (sizeof(T) == 4) inv = inv32_.data();
+    else inv = inv64_.data();
+    hipLaunchKernelGGL(k_coarse_gemv<T>, dim3((n_ + 3) / 4), dim3(256), 0, s, inv, n_, ld_, (const T*)rhs<T>(),
+                       map_.data(), x);
+}
+
 template void DenseCoarse::solve<float>(const float*, float*, hipStream_t) const;
 template void DenseCoarse::solve<double>(const double*, double*, hipStream_t) const;
+template float* DenseCoarse::rhs<float>() const;
+template double* DenseCoarse::rhs<double>() const;
+template void DenseCoarse::solve_rhs<float>(float*, hipStream_t) const;
+template void DenseCoarse::solve_rhs<double>(double*, hipStream_t) const;
 
 }  // namespace fem
 

@@ -24,6 +24,17 @@ class DenseCoarse {
     template <class T>
     void solve(const T* b, T* x, hipStream_t s) const;
 
+    // The dense right side (n values, zero padded) that solve() gathers b into: a producer may
+    // write it directly (rhs()[dense index] = value) and call solve_rhs(), skipping the gather.
+    template <class T>
+    T* rhs() const;
+    template <class T>
+    void solve_rhs(T* x, hipStream_t s) const;
+    // The dense index of each unknown of the level, or -1 (the inverse of map), for producers of
+    // rhs(); empty unless set by the owner.
+    DArray<int32_t>& dense_of() { return dense_of_; }
+    const DArray<int32_t>& dense_of() const { return dense_of_; }
+
     int n() const { return n_; }
     const DArray<int64_t>& map() const { return map_; }
     DArray<int64_t>& map() { return map_; }
@@ -36,6 +47,7 @@ class DenseCoarse {
     DArray<double> inv64_, bc64_;
     DArray<float> inv32_, bc32_;
     DArray<int64_t> map_;
+    DArray<int32_t> dense_of_;
     double factor_seconds_ = 0.0, factor_flops_ = 0.0;
 };
 

@@ -235,11 +235,13 @@ __global__ void __launch_bounds__(BLOCK) k_prolong_add(GridDims fg, GridDims cg,
 }
 
 // The transpose: b_c = P^T r_f, gathered per coarse node; 0 on fixed coarse nodes. With invd,
-// also the coarse level's first Chebyshev step from x = 0: d_c = x_c = c0 D^-1 b_c.
+// also the coarse level's first Chebyshev step from x = 0: d_c = x_c = c0 D^-1 b_c. With dense_of
+// (the coarse level is the dense one), b_c goes to its dense index in dense_b instead of bc.
 template <class T, int C, bool KUHN_P1, class S>
 __global__ void __launch_bounds__(BLOCK) k_restrict(GridDims fg, GridDims cg, const T* __restrict__ rf,
                                                     T* __restrict__ bc, const S* __restrict__ invd, T c0,
-                                                    T* __restrict__ xc, T* __restrict__ dc) {
+                                                    T* __restrict__ xc, T* __restrict__ dc,
+                                                    const int32_t* __restrict__ dense_of, T* __restrict__ dense_b) {
     const size_t node = xcd_block() * BLOCK + threadIdx.x;
     if (node >= cg.nodes()) return;
     int I, J, K;
@@ -295,6 +297,11 @@ __global__ void __launch_bounds__(BLOCK) k_restrict(GridDims fg, GridDims cg, co
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         const size_t u = C * node + c;
+        if (dense_of) {
+            const int32_t j = dense_of[u];
+            if (j >= 0) dense_b[j] = acc[c];
+            continue;
+        }
         bc[u] = acc[c];
         if (invd) {  // the coarse level's first Chebyshev step (k_cheb_first, same arithmetic)
             const T v = c0 * ld<T>(invd, u) * acc[c];

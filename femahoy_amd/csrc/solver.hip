@@ -386,7 +386,9 @@ class Hierarchy {
         Level<T>& L = lv[l];
         L.cur = 0;  // every cycle uses the same buffers in the same order (graph capture)
         if (l + 1 == lv.size()) {
-            coarse.solve<T>(L.b.data(), L.x(), s);
+            // first_done on the dense level: the restriction wrote its dense right side
+            if (first_done) coarse.solve_rhs<T>(L.x(), s);
+            else coarse.solve<T>(L.b.data(), L.x(), s);
             return;
         }
         const size_t n = L.dofs();
@@ -416,7 +418,7 @@ class Hierarchy {
         Level<T>& Cc = lv[l + 1];
         const bool smoothed = l + 2 < lv.size();  // the next level smooths (it is not the dense one)
         transfer(L, Cc, true, s, smoothed);
-        cycle(l + 1, s, smoothed);
+        cycle(l + 1, s, true);
         transfer(L, Cc, false, s);
         for (int st = 0; st < degree; ++st) {
             mark_begin(l, s);
@@ -466,6 +468,10 @@ class Hierarchy {
         const int C = F.comps;
         if (restrict_) {
             const size_t n = Cc.nodes();
+            // into the dense level (not smoothed): straight to its dense right side
+            const bool dense = !first_step;
+            const int32_t* of = dense ? coarse.dense_of().data() : nullptr;
+            T* db = dense ? coarse.rhs<T>() : nullptr;
             with_invd(Cc, [&](auto iv) {
                 using S = elem_t<decltype(iv)>;
                 const S* inv = first_step ? iv : nullptr;
@@ -474,11 +480,11 @@ class Hierarchy {
                 T* dc = Cc.d.data();
                 const T* rf = F.r.data();
                 if (F.kuhn) {
-                    if (C == 1) launch((k_restrict<T, 1, true, S>), n, s, F.g, Cc.g, rf, Cc.b.data(), inv, c0, xc, dc);
-                    else launch((k_restrict<T, 3, true, S>), n, s, F.g, Cc.g, rf, Cc.b.data(), inv, c0, xc, dc);
+                    if (C == 1) launch((k_restrict<T, 1, true, S>), n, s, F.g, Cc.g, rf, Cc.b.data(), inv, c0, xc, dc, of, db);
+                    else launch((k_restrict<T, 3, true, S>), n, s, F.g, Cc.g, rf, Cc.b.data(), inv, c0, xc, dc, of, db);
                 } else {
-                    if (C == 1) launch((k_restrict<T, 1, false, S>), n, s, F.g, Cc.g, rf, Cc.b.data(), inv, c0, xc, dc);
-                    else launch((k_restrict<T, 3, false, S>), n, s, F.g, Cc.g, rf, Cc.b.data(), inv, c0, xc, dc);
+                    if (C == 1) launch((k_restrict<T, 1, false, S>), n, s, F.g, Cc.g, rf, Cc.b.data(), inv, c0, xc, dc, of, db);
+                    else launch((k_restrict<T, 3, false, S>), n, s, F.g, Cc.g, rf, Cc.b.data(), inv, c0, xc, dc, of, db);
                 }
             });
         } else {
@@ -1169,6 +1175,9 @@ class Solver {
         dc.factor(A.data(), (int)n, fp32, s_);
         dc.map().resize(units.size());
         dc.map().upload(units.data(), units.size(), s_);
+        std::vector<int32_t> of32(of.begin(), of.end());
+        dc.dense_of().resize(of32.size());
+        dc.dense_of().upload(of32.data(), of32.size(), s_);
         FEM_HIP(hipStreamSynchronize(s_));
     }
 
