@@ -117,6 +117,8 @@ __global__ void __launch_bounds__(256) k_gemm_nt(GemmArgs g) {
 
 // Factors the diagonal block kb in LDS: A_kk = L D L^T (L unit lower). Writes L into the block
 // (strict lower part), D into d[kb*64 ..], L^-1 into linv[kb] and D^-1 L^-1 into bk.
+// The trailing update takes 4 lanes per row with the pivot's reciprocal; L^-1 is formed one column
+// per thread in registers (fully unrolled forward substitution on the unit lower factor in LDS).
 __global__ void __launch_bounds__(256) k_ldlt_diag(double* A, int lda, int kb, const double* orig_diag, double* d,
                                                    double* linv, double* bk, int* status) {
     __shared__ double S[NB][NB + 1];
@@ -127,34 +129,41 @@ __global__ void __launch_bounds__(256) k_ldlt_diag(double* A, int lda, int kb, c
         S[r][c] = r >= c ? blk[(size_t)r * lda + c] : 0.0;
     }
     __syncthreads();
+    const int row = tid >> 2, lane = tid & 3;
     for (int j = 0; j < NB; ++j) {
         double dj = S[j][j];
         if (!(dj > PIVOT * fabs(orig_diag[kb * NB + j]))) {
             if (tid == 0 && *status == 0) *status = kb * NB + j + 1;
             dj = 1.0;  // keep going with a harmless pivot; the host reports the failure
         }
-        const int m = NB - 1 - j;
+        const double inv = 1.0 / dj;
         // trailing update of the lower triangle: S[r][c] -= S[r][j] S[c][j] / dj, j < c <= r
-        for (int e = tid; e < m * m; e += 256) {
-            const int r = j + 1 + e / m, c = j + 1 + e % m;
-            if (c <= r) S[r][c] -= S[r][j] * S[c][j] / dj;
+        if (row > j) {
+            const double lr = S[row][j] * inv;
+            for (int c = j + 1 + lane; c <= row; c += 4) S[row][c] -= lr * S[c][j];
         }
         __syncthreads();
-        if (tid > j && tid < NB) S[tid][j] /= dj;
+        if (tid > j && tid < NB) S[tid][j] *= inv;
         if (tid == 0) S[j][j] = dj;
         __syncthreads();
     }
-    // L^-1 (unit lower), one column per thread; each thread reads back only its own column.
-    double* li = linv + (size_t)kb * NB * NB;
+    // L^-1 (unit lower): thread c forms column c by forward substitution, in registers.
     if (tid < NB) {
         const int c = tid;
-        for (int r = 0; r < NB; ++r) li[r * NB + c] = r == c ? 1.0 : 0.0;
-        for (int r = c + 1; r < NB; ++r) {
+        double li[NB];
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
             double s = 0.0;
-            for (int k = c; k < r; ++k) s += S[r][k] * li[k * NB + c];
-            li[r * NB + c] = -s;
+#pragma unroll
+            for (int k = 0; k < r; ++k) s += S[r][k] * li[k];
+            li[r] = r == c ? 1.0 : (r < c ? 0.0 : -s);
         }
-        for (int r = 0; r < NB; ++r) bk[r * NB + c] = li[r * NB + c] / S[r][r];
+        double* lc = linv + (size_t)kb * NB * NB;
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+            lc[r * NB + c] = li[r];
+            bk[r * NB + c] = li[r] / S[r][r];
+        }
     }
     for (int e = tid; e < NB * NB; e += 256) {
         const int r = e / NB, c = e % NB;
