@@ -1,0 +1,103 @@
+"""Edge cases of the solve, as the reference's pcg defines them (multigrid.rs:692-753): a zero right
+side gives x = 0 after 0 iterations (706-710); the residual test comes before the iteration limit
+(722-733), so a start value that already meets the tolerance takes 0 iterations and max_iter = 0
+fails only when it does not; a start value is where PCG begins. Thin boxes of one cell across, on
+both boundaries (box and assembled)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def heat_box(fem, n, seed=1):
+    k = np.exp(np.random.default_rng(seed).standard_normal(int(np.prod(n))))
+    return fem.Box("heat_hex8", n, [1.0, 1.0, 1.0], k, fixed={"x0": 0.0, "x1": 1.0}), k
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_a_zero_right_side_gives_zero_after_no_iteration(fem, precision):
+    box, _ = heat_box(fem, [16, 16, 16])
+    s = fem.Solver(box, method="multigrid", precision=precision, levels=3)
+    x, st = s.solve(np.zeros(s.n), x=np.random.default_rng(2).standard_normal(s.n))
+    assert st.iterations == 0 and st.residual == 0.0
+    assert not x.any()
+
+
+def test_a_start_value_that_meets_the_tolerance_takes_no_iteration(fem):
+    box, _ = heat_box(fem, [16, 16, 16])
+    s = fem.Solver(box, method="multigrid", levels=3, tol=1e-10)
+    b = s.rhs()
+    x, st = s.solve(b)
+    assert st.iterations > 0
+    x2, st2 = s.solve(b, x=x)
+    assert st2.iterations == 0 and st2.residual <= 1e-10
+    assert np.array_equal(x2, x)
+    # max_iter = 0: the residual test comes first, so this start value still succeeds ...
+    s.with_max_iter(0)
+    _, st3 = s.solve(b, x=x)
+    assert st3.iterations == 0
+    # ... and a start value that does not meet it fails with the reference's text
+    with pytest.raises(fem.FemError, match=r"^conjugate gradients did not converge: the residual is .* after 0 iterations$"):
+        s.solve(b)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_pcg_starts_from_the_given_value(fem, oracle, precision):
+    n = [16, 12, 8]
+    box, k = heat_box(fem, n, 3)
+    pr = oracle.Problem.heat(n, [1, 1, 1], k, fixed={"x0": 0.0, "x1": 1.0})
+    xd, _, _ = oracle.Solver(pr, "direct").solve(pr.rhs())
+    s = fem.Solver(box, method="multigrid", precision=precision, levels=3, tol=1e-12)
+    x0 = 10.0 * np.random.default_rng(4).standard_normal(s.n)
+    x, st = s.solve(pr.rhs(), x=x0)
+    assert st.iterations > 0
+    assert rel(x, xd) < 1e-8
+
+
+@pytest.mark.parametrize("kind,n", [("heat_hex8", [24, 1, 1]), ("heat_tet4", [12, 1, 2]), ("elastic_tet10", [10, 1, 1]),
+                                    ("elastic_tet4", [20, 2, 1])])
+def test_thin_boxes_of_one_cell_across_match_oracle_direct(fem, oracle, kind, n):
+    rng = np.random.default_rng(5)
+    length = [4.0, 0.5, 0.7]
+    if kind.startswith("heat"):
+        count = int(np.prod(n)) * (1 if kind == "heat_hex8" else 6)
+        k = rng.uniform(0.5, 2.0, count)
+        pr = oracle.Problem.heat(n, length, k, fixed={"x0": 1.0, "x1": 0.0}, source=0.3, tet4=kind == "heat_tet4")
+        box = fem.Box(kind, n, length, k, fixed={"x0": 1.0, "x1": 0.0}, source=0.3)
+    else:
+        nt = 6 * int(np.prod(n))
+        e, nu = rng.uniform(1.0, 2.0, nt), rng.uniform(0.2, 0.35, nt)
+        order = 10 if kind == "elastic_tet10" else 4
+        pr = oracle.Problem.elastic(order, n, length, e, nu, clamped=["x0"], body=(0, 0, -1))
+        box = fem.Box(kind, n, length, e, nu, fixed={"x0": True}, body=(0, 0, -1))
+    xd, _, _ = oracle.Solver(pr, "direct").solve(pr.rhs())
+    for method in ("direct", "multigrid"):
+        s = fem.Solver(box, method=method, tol=1e-12)
+        assert np.array_equal(s.free_list(), pr.free())
+        x, st = s.solve(pr.rhs())
+        print(kind, n, method, st)
+        assert rel(x, xd) < 1e-8
+
+
+def test_the_assembled_boundary_keeps_the_same_edge_semantics(fem, oracle):
+    n, length = [24, 18, 14], [1.0, 1.0, 1.0]
+    k = np.exp(np.random.default_rng(6).standard_normal(int(np.prod(n))))
+    pr = oracle.Problem.heat(n, length, k, fixed={"x0": 0.0, "x1": 1.0})
+    ptr, idx, val = pr.reduced().arrays()
+    planes = [(length[d] / n[d]) * np.arange(n[d] + 1, dtype=np.float64) for d in range(3)]
+    pts = pr.ijk()[pr.free()]
+    s = fem.CsrSolver(ptr, idx, val, "grid", fem.Grid(planes, pts, np.zeros(len(pts), np.uint8)), method="multigrid",
+                      tol=1e-10)
+    x, st = s.solve(np.zeros(s.n), x=np.ones(s.n))
+    assert st.iterations == 0 and not x.any()
+    b = pr.rhs()
+    x, st = s.solve(b)
+    x2, st2 = s.solve(b, x=x)
+    assert st2.iterations == 0 and np.array_equal(x2, x)
+    s.with_max_iter(0)
+    with pytest.raises(fem.FemError, match=r"after 0 iterations$"):
+        s.solve(b)
