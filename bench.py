@@ -338,7 +338,50 @@ def run_workload(F, hip, box, levels, args, label):
     res["mixed_bf16"]["finest_smoother"] = {"bytes_per_launch": int(prof_h["smoother_bytes"]),
                                             "us_per_launch": round(prof_h["smoother_s"] * 1e6, 2),
                                             "achieved_gbs": round(prof_h["smoother_bytes"] / prof_h["smoother_s"] / 1e9, 1)}
+    res["_x_ref"] = x_ref
     return res, prof, lat_m
+
+
+def assembled_leg(F, hip, solver, box, x_ref, args):
+    """The assembled boundary on the same heat box: the model's reduced CSR (fem_solver_matrix, as
+    the reference's caller assembles it), its grid unknowns, and fem_solver_new_csr with the
+    reference's grid hierarchy (coarsen and Galerkin products down to 3000 unknowns), mixed
+    precision, solved on the device like the box solver."""
+    log("assembled boundary: the model's reduced CSR and grid unknowns into fem_solver_new_csr")
+    t0 = time.perf_counter()
+    ptr, idx, val = solver.matrix()
+    n = list(box.n)
+    planes = [(box.length[d] / n[d]) * np.arange(n[d] + 1, dtype=np.float64) for d in range(3)]
+    pts = F.Model(box).point_grid()[solver.free_list()]
+    t1 = time.perf_counter()
+    csr = F.CsrSolver(ptr, idx, val, "grid", F.Grid(planes, pts, np.zeros(solver.n, np.uint8)), method="multigrid",
+                      precision="fp32", tol=1e-8, max_iter=2000)
+    t2 = time.perf_counter()
+    lv = []
+    for l in range(32):
+        try:
+            lv.append(csr.level_info(l)[0])
+        except F.FemError:
+            break
+    log(f"  {int(ptr[-1])} entries; export {t1 - t0:.2f} s, setup {t2 - t1:.2f} s; levels {lv}")
+    b = solver.rhs()
+    d_b = hip.malloc(b.nbytes)
+    d_x = hip.malloc(b.nbytes)
+    hip.h2d(d_b, b)
+    x0 = np.zeros(solver.n)
+    lat, st, _, _ = timed_solves(F, hip, csr, d_b, d_x, x0, args.steps, args.warmup, profile=False)
+    x = np.zeros(solver.n)
+    hip.d2h(x, d_x)
+    hip.free(d_b)
+    hip.free(d_x)
+    err = float(np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref))
+    return {"mdof_s": round(solver.n / float(np.mean(lat)) / 1e6, 2), "ms_mean": round(1e3 * float(np.mean(lat)), 3),
+            "iterations": st.iterations, "residual": st.residual, "true_residual": st.true_residual,
+            "error_vs_fp64": err, "entries": int(ptr[-1]), "levels": lv, "export_s": round(t1 - t0, 3),
+            "setup_s": round(t2 - t1, 3),
+            "workload": "the same heat box through fem_solver_new_csr: CSR SpMV operators on every level (FP32 "
+                        "V-cycle, FP64 PCG), the reference's coarsen and Galerkin levels down to 3000 unknowns, "
+                        "Chebyshev(3)-Jacobi, dense LDLT coarse, tol 1e-8"}
 
 
 def main():
@@ -358,6 +401,7 @@ def main():
                     help="cells along x of the nearly incompressible inclusion box (0: skip)")
     ap.add_argument("--inclusions", type=int, default=200, help="spheres in the inclusion box")
     ap.add_argument("--serve-n", type=int, default=256, help="solves of the serving stream (0: skip)")
+    ap.add_argument("--no-assembled", action="store_true", help="skip the assembled (CSR) boundary leg of the heat box")
     ap.add_argument("--calibrate", action="store_true",
                     help="run one plain 256 MiB device copy first (the yardstick of HBM byte counters)")
     args = ap.parse_args()
@@ -384,6 +428,10 @@ def main():
     box, _ = heat_box(F, n)
     heat, prof, lat_m = run_workload(F, hip, box, args.levels, args, f"heat hex8 {n}^3")
     value = heat["mixed"]["mdof_s"]
+    x_ref = heat.pop("_x_ref")
+    if not args.no_assembled:
+        heat["assembled_csr"] = assembled_leg(F, hip, F.Solver(box, method="multigrid", levels=args.levels), box,
+                                              x_ref, args)
 
     roofline = None
     traffic = measured_traffic(n)
@@ -409,6 +457,7 @@ def main():
         en = args.elastic_n
         ebox = elastic_box(F, en)
         elastic, eprof, _ = run_workload(F, hip, ebox, 0, args, f"elasticity tet10 {en}x{en // 2}x{en // 2}")
+        elastic.pop("_x_ref")
         elastic["workload"] = (f"linear elasticity, tet10 on {en}x{en // 2}x{en // 2} unit cells (6 Kuhn tets per cell), "
                                "E=1, nu=0.3, clamped at x=0, body force (0,0,-1); p-MG tet10->tet4 then geometric tet4 "
                                "levels, dense FP64 LDLT coarse, Chebyshev(3)-Jacobi, tol 1e-8")
@@ -423,6 +472,7 @@ def main():
         ni = args.inclusions_n
         ibox, inside = F.inclusion_box(ni, args.inclusions, 7)
         inclusions, _, _ = run_workload(F, hip, ibox, 0, args, f"inclusions tet10 {ni}x{ni // 2}x{ni // 2}")
+        inclusions.pop("_x_ref")
         inclusions["workload"] = (f"nearly incompressible heterogeneous elasticity, tet10 on {ni}x{ni // 2}x{ni // 2} unit "
                                   f"cells, nu=0.49 matrix (E=1) with {args.inclusions} seeded spheres (radius 0.02-0.05 "
                                   "of the box length, E=1e3, nu=0.3; a tet is inside when its centroid is), clamped at "

@@ -141,7 +141,7 @@ EXPORTS = (
     "fem_model_tet_cells",
     "fem_solver_set_profile", "fem_solver_profile", "fem_solver_time_kernels",
     "fem_server_new", "fem_server_submit", "fem_server_wait", "fem_server_cancel", "fem_server_free",
-    "fem_solver_new_csr",
+    "fem_solver_new_csr", "fem_solver_matrix",
 )
 
 _lib = None
@@ -201,6 +201,7 @@ def lib() -> C.CDLL:
                                       C.POINTER(RequestInfo), ERR, SZ]),
         "fem_server_cancel": (C.c_int, [P, C.c_uint64]),
         "fem_server_free": (None, [P]),
+        "fem_solver_matrix": (C.c_int, [P, C.POINTER(C.c_uint64), P, P, P, ERR, SZ]),
         "fem_solver_new_csr": (C.c_int, [C.POINTER(_Csr), C.POINTER(_Unknowns), C.POINTER(_Options), C.POINTER(P),
                                          ERR, SZ]),
         "fem_test_csr_levels_new": (C.c_int, [C.POINTER(_Csr), C.POINTER(_GridLevel), C.c_uint64, C.c_int,
@@ -411,6 +412,22 @@ class Solver:
         if lib().fem_solver_apply(self.h, np.ascontiguousarray(x, np.float64), y, err, 2048):
             raise FemError(err.value.decode())
         return y
+
+    def matrix(self):
+        """The reduced matrix of the model (fem_solver_matrix): (ptr, idx, val) of a CSR over the
+        free unknowns in reference order, as the reference's callers assemble it."""
+        nnz = C.c_uint64()
+        err = _err()
+        L = lib()
+        if L.fem_solver_matrix(self.h, C.byref(nnz), None, None, None, err, 2048):
+            raise FemError(err.value.decode())
+        ptr = np.zeros(self.n + 1, np.uint64)
+        idx = np.zeros(nnz.value, np.uint32)
+        val = np.zeros(nnz.value)
+        vp = lambda a: C.c_void_p(a.ctypes.data)
+        if L.fem_solver_matrix(self.h, C.byref(nnz), vp(ptr), vp(idx), vp(val), err, 2048):
+            raise FemError(err.value.decode())
+        return ptr, idx, val
 
     def precondition(self, r) -> np.ndarray:
         z = np.zeros(self.n)

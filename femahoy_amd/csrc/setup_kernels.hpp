@@ -180,4 +180,24 @@ __global__ void k_probe_extract(GridDims g, int C, int S, int c0, int c1, int c2
     A[row * n + col] = y[u];
 }
 
+// The probe's values into a CSR of the free unknowns (row r: grid unknown free_grid[r]): every
+// entry whose column (a free unknown) has the probe's colour (residues c0, c1, c2 mod S of its
+// node, component comp) takes y at the row's grid unknown. Within the coupling radius (S = 2R + 1)
+// a row meets one node of each colour, so each entry is written by exactly one probe.
+__global__ void k_probe_csr(uint64_t rows, GridDims g, int C, int S, int c0, int c1, int c2, int comp,
+                            const uint64_t* __restrict__ ptr, const uint32_t* __restrict__ idx,
+                            const int64_t* __restrict__ free_grid, const double* __restrict__ y,
+                            double* __restrict__ val) {
+    const uint64_t r = blockIdx.x * (uint64_t)BLOCK + threadIdx.x;
+    if (r >= rows) return;
+    const double yr = y[free_grid[r]];
+    for (uint64_t k = ptr[r]; k < ptr[r + 1]; ++k) {
+        const int64_t gu = free_grid[idx[k]];
+        const int64_t node = gu / C;
+        int i, j, kk;
+        node_ijk(g, (size_t)node, i, j, kk);
+        if ((int)(gu % C) == comp && i % S == c0 && j % S == c1 && kk % S == c2) val[k] = yr;
+    }
+}
+
 }  // namespace fem

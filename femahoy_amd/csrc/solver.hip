@@ -642,6 +642,86 @@ class Solver {
         FEM_HIP(hipStreamSynchronize(s_));
     }
 
+    // The reduced matrix of the model (Csr::submatrix(free), sparse.rs:108-135, of the matrix the
+    // reference's callers assemble): its pattern is every pair of free unknowns whose nodes share
+    // an element (Csr::pattern, sparse.rs:22-65: 27 neighbours on hex8 cells, the 15 of the Kuhn
+    // edges on tetrahedra), columns in increasing order; its values come from probing the FP64
+    // operator with one unit vector per colour of nodes 3 apart. Returns the entry count; fills
+    // the arrays when they are given. Not for tet10.
+    uint64_t export_matrix(uint64_t* out_ptr, uint32_t* out_idx, double* out_val) {
+        const BoxModel& m = model_;
+        if (m.in.kind == Kind::ElasticTet10) throw Error("the matrix export covers hex8 and tet4 models");
+        const int C = m.comps;
+        const int nx = m.gn[0], ny = m.gn[1], nz = m.gn[2];
+        const uint64_t nf = m.free.size();
+        std::vector<int64_t> fidx(m.grid_unknowns(), -1);
+        for (uint64_t r = 0; r < nf; ++r) fidx[m.free_grid[r]] = (int64_t)r;
+        // neighbour offsets of the pattern
+        std::vector<std::array<int, 3>> offs;
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    if (m.in.kind != Kind::HeatHex8) {  // a Kuhn edge: all components >= 0 or all <= 0
+                        const bool up = dx >= 0 && dy >= 0 && dz >= 0, down = dx <= 0 && dy <= 0 && dz <= 0;
+                        if (!up && !down) continue;
+                    }
+                    offs.push_back({dx, dy, dz});
+                }
+        std::vector<uint64_t> ptr(nf + 1, 0);
+        std::vector<uint32_t> idx;
+        auto row_cols = [&](uint64_t r, std::vector<uint32_t>& cols) {
+            cols.clear();
+            const int64_t gu = m.free_grid[r];
+            const int64_t node = gu / C;
+            const int i = (int)(node % nx), j = (int)((node / nx) % ny), k = (int)(node / ((int64_t)nx * ny));
+            for (const auto& o : offs) {
+                const int a = i + o[0], b = j + o[1], c = k + o[2];
+                if (a < 0 || b < 0 || c < 0 || a >= nx || b >= ny || c >= nz) continue;
+                const int64_t nb = a + (int64_t)nx * (b + (int64_t)ny * c);
+                for (int q = 0; q < C; ++q) {
+                    const int64_t f = fidx[nb * C + q];
+                    if (f >= 0) cols.push_back((uint32_t)f);
+                }
+            }
+            std::sort(cols.begin(), cols.end());
+        };
+        std::vector<uint32_t> cols;
+        for (uint64_t r = 0; r < nf; ++r) {
+            row_cols(r, cols);
+            ptr[r + 1] = ptr[r] + cols.size();
+        }
+        const uint64_t nnz = ptr[nf];
+        if (!out_ptr) return nnz;
+        idx.resize(nnz);
+        for (uint64_t r = 0; r < nf; ++r) {
+            row_cols(r, cols);
+            std::copy(cols.begin(), cols.end(), idx.begin() + ptr[r]);
+        }
+        DArray<uint64_t> dptr(nf + 1);
+        DArray<uint32_t> didx(std::max<uint64_t>(nnz, 1));
+        DArray<double> dval(std::max<uint64_t>(nnz, 1)), px(n0_), py(n0_);
+        DArray<int64_t> dof(fidx.size());
+        dptr.upload(ptr.data(), nf + 1, s_);
+        if (nnz) didx.upload(idx.data(), nnz, s_);
+        dof.upload(fidx.data(), fidx.size(), s_);
+        Level<double>& L = lv64_[0];
+        for (int c0 = 0; c0 < 3; ++c0)
+            for (int c1 = 0; c1 < 3; ++c1)
+                for (int c2 = 0; c2 < 3; ++c2)
+                    for (int comp = 0; comp < C; ++comp) {
+                        launch(k_probe_fill, L.dofs(), s_, L.g, C, 3, c0, c1, c2, comp, (const int64_t*)dof.data(), px.data());
+                        op_apply(L, (const double*)px.data(), py.data(), s_);
+                        launch(k_probe_csr, nf, s_, nf, L.g, C, 3, c0, c1, c2, comp, (const uint64_t*)dptr.data(),
+                               (const uint32_t*)didx.data(), (const int64_t*)free_grid_.data(), (const double*)py.data(),
+                               dval.data());
+                    }
+        std::copy(ptr.begin(), ptr.end(), out_ptr);
+        if (out_idx) std::copy(idx.begin(), idx.end(), out_idx);
+        if (out_val && nnz) dval.download(out_val, nnz, s_);
+        FEM_HIP(hipStreamSynchronize(s_));
+        return nnz;
+    }
+
     void precondition_host(const double* rin, double* zout) {
         if (method_ != FEM_METHOD_MULTIGRID) throw Error("the direct solver has no preconditioner");
         const size_t nf = model_.free.size();
@@ -2043,6 +2123,16 @@ int fem_solver_precondition(fem_solver* s, const double* r, double* z, char* err
         if (!s) throw fem::Error("no solver");
         if (s->csr) s->csr->precondition_host(r, z);
         else s->impl->precondition_host(r, z);
+    });
+}
+
+int fem_solver_matrix(fem_solver* s, uint64_t* nnz, uint64_t* ptr, uint32_t* idx, double* val, char* err,
+                      size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!s) throw fem::Error("no solver");
+        if (s->csr) throw fem::Error("an assembled solver holds the caller's matrix already");
+        const uint64_t n = s->impl->export_matrix(ptr, idx, val);
+        if (nnz) *nnz = n;
     });
 }
 

@@ -211,6 +211,14 @@ int fem_solver_rhs(fem_solver* s, double* out, char* err, size_t errlen);
  * Csr::submatrix(free) (sparse.rs:108) times x, as Csr::mul (sparse.rs:96). */
 int fem_solver_apply(fem_solver* s, const double* x, double* y, char* err, size_t errlen);
 
+/* The reduced matrix of a box model, as the reference's callers assemble it before Solver::new:
+ * Csr::submatrix(free) (sparse.rs:108-135) of the Csr::pattern of the elements (sparse.rs:22-65),
+ * over the free unknowns in reference order, columns increasing, FP64 values of the device's
+ * operator. Call with ptr NULL for the entry count (*nnz); then with ptr (free count + 1), idx and
+ * val (nnz each). hex8 and tet4 models. */
+int fem_solver_matrix(fem_solver* s, uint64_t* nnz, uint64_t* ptr, uint32_t* idx, double* val, char* err,
+                      size_t errlen);
+
 /* Applies the V-cycle preconditioner once, z = M^-1 r (host vectors, reference order). */
 int fem_solver_precondition(fem_solver* s, const double* r, double* z, char* err, size_t errlen);
 
