@@ -34,40 +34,17 @@ namespace {
 constexpr uint64_t DIRECT_LIMIT = 20000;  // the dense direct solve (as the box solver's)
 constexpr uint64_t COARSEST = 3000;       // multigrid.rs:50
 
-// Lanes per row for a matrix of this mean row length: the largest power of 2 up to half of it,
-// between 1 and 32.
-int lanes_for(uint64_t rows, uint64_t nnz) {
-    const double mean = rows ? (double)nnz / (double)rows : 1.0;
-    int l = 1;
-    while (l < 32 && 2.0 * (2 * l) <= mean) l *= 2;
-    return l;
-}
-
-template <class F>
-void with_lanes(int l, F&& f) {
-    switch (l) {
-        case 1: f(std::integral_constant<int, 1>()); break;
-        case 2: f(std::integral_constant<int, 2>()); break;
-        case 4: f(std::integral_constant<int, 4>()); break;
-        case 8: f(std::integral_constant<int, 8>()); break;
-        case 16: f(std::integral_constant<int, 16>()); break;
-        default: f(std::integral_constant<int, 32>()); break;
-    }
-}
-
-unsigned row_blocks(uint64_t rows, int lanes) { return (unsigned)((rows * (uint64_t)lanes + BLOCK - 1) / BLOCK); }
-
 int threads_for_setup() {
     const unsigned hc = std::thread::hardware_concurrency();
     return (int)std::max(1u, std::min(16u, hc ? hc : 1u));
 }
 
-// A CSR matrix on the device: 32-bit offsets and columns, values in V.
+// A matrix on the device in sliced ELL form (csr_kernels.hpp): slices of SELL_C rows, each
+// padded to its longest row and stored column-major; base[s] is slice s's first entry.
 template <class V>
 struct DevCsr {
-    uint64_t rows = 0, cols = 0, nnz = 0;
-    int lanes = 1;
-    DArray<uint32_t> ptr, idx;
+    uint64_t rows = 0, cols = 0, nnz = 0, padded = 0;
+    DArray<uint32_t> base, col;
     DArray<V> val;
 
     void upload(uint64_t r, uint64_t c, const std::vector<uint64_t>& p, const std::vector<uint32_t>& ix,
@@ -75,24 +52,40 @@ struct DevCsr {
         rows = r;
         cols = c;
         nnz = p[r];
-        if (nnz > 0xFFFFFFFFull) throw Error("a matrix of more than 2^32 - 1 entries is beyond the device path");
-        lanes = lanes_for(rows, nnz);
-        std::vector<uint32_t> p32(r + 1);
-        for (uint64_t i = 0; i <= r; ++i) p32[i] = (uint32_t)p[i];
-        ptr.resize(r + 1);
-        ptr.upload(p32.data(), r + 1, s);
-        idx.resize(std::max<uint64_t>(nnz, 1));
-        if (nnz) idx.upload(ix.data(), nnz, s);
-        val.resize(std::max<uint64_t>(nnz, 1));
-        if (nnz) {
-            if constexpr (std::is_same<V, double>::value) {
-                val.upload(v.data(), nnz, s);
-            } else {
-                std::vector<V> vv(nnz);
-                for (uint64_t k = 0; k < nnz; ++k) vv[k] = (V)v[k];
-                val.upload(vv.data(), nnz, s);
+        const uint64_t slices = (r + SELL_C - 1) / SELL_C;
+        std::vector<uint32_t> b(slices + 1, 0);
+        uint64_t total = 0;
+        for (uint64_t sl = 0; sl < slices; ++sl) {
+            uint64_t w = 0;
+            for (uint64_t q = sl * SELL_C; q < std::min<uint64_t>(r, (sl + 1) * SELL_C); ++q) w = std::max(w, p[q + 1] - p[q]);
+            total += w * SELL_C;
+            if (total > 0xFFFFFFFFull) throw Error("a matrix of more than 2^32 - 1 entries is beyond the device path");
+            b[sl + 1] = (uint32_t)total;
+        }
+        padded = total;
+        std::vector<uint32_t> cc(std::max<uint64_t>(total, 1), 0);
+        std::vector<V> vv(std::max<uint64_t>(total, 1), V(0));
+        for (uint64_t sl = 0; sl < slices; ++sl) {
+            const uint64_t w = (b[sl + 1] - b[sl]) / SELL_C;
+            for (int lane = 0; lane < SELL_C; ++lane) {
+                const uint64_t q = sl * SELL_C + lane;
+                for (uint64_t jj = 0; jj < w; ++jj) {
+                    const uint64_t at = b[sl] + jj * SELL_C + lane;
+                    if (q < r && p[q] + jj < p[q + 1]) {
+                        cc[at] = ix[p[q] + jj];
+                        vv[at] = (V)v[p[q] + jj];
+                    } else {
+                        cc[at] = q < r && q < c ? (uint32_t)q : 0;  // padding: value 0 at a valid column
+                    }
+                }
             }
         }
+        base.resize(slices + 1);
+        base.upload(b.data(), slices + 1, s);
+        col.resize(cc.size());
+        col.upload(cc.data(), cc.size(), s);
+        val.resize(vv.size());
+        val.upload(vv.data(), vv.size(), s);
         FEM_HIP(hipStreamSynchronize(s));  // the host staging goes out of scope
     }
 };
@@ -132,29 +125,20 @@ struct CHierarchy {
             step(L, L.c1[st], L.c2[st], s);
             L.cur ^= 1;
         }
-        with_lanes(L.a.lanes, [&](auto lc) {
-            constexpr int LN = decltype(lc)::value;
-            launch_blocks((k_csr_residual<LN, T, T>), row_blocks(n, LN), s, n, (const uint32_t*)L.a.ptr.data(),
-                          (const uint32_t*)L.a.idx.data(), (const T*)L.a.val.data(), (const T*)L.x(),
-                          (const T*)L.b.data(), L.res.data());
-        });
+        launch_blocks((k_csr_residual<T, T>), nblocks(n), s, n, (const uint32_t*)L.a.base.data(),
+            (const uint32_t*)L.a.col.data(), (const T*)L.a.val.data(), (const T*)L.x(),
+            (const T*)L.b.data(), L.res.data());
         CLevel<T>& C = lv[l + 1];
         const bool smoothed = l + 2 < lv.size();
         // into the dense level: straight to its dense right side (its unknowns are in dense order)
         T* bc = smoothed ? C.b.data() : coarse.rhs<T>();
-        with_lanes(L.r.lanes, [&](auto lc) {
-            constexpr int LN = decltype(lc)::value;
-            launch_blocks((k_csr_restrict<LN, T, T, T>), row_blocks(C.n, LN), s, C.n, (const uint32_t*)L.r.ptr.data(),
-                          (const uint32_t*)L.r.idx.data(), (const T*)L.r.val.data(), (const T*)L.res.data(), bc,
-                          smoothed ? (const T*)C.invd.data() : (const T*)nullptr, smoothed ? C.c2[0] : T(0),
-                          C.xa.data(), C.d.data());
-        });
+        launch_blocks((k_csr_restrict<T, T, T>), nblocks(C.n), s, C.n, (const uint32_t*)L.r.base.data(),
+            (const uint32_t*)L.r.col.data(), (const T*)L.r.val.data(), (const T*)L.res.data(), bc,
+            smoothed ? (const T*)C.invd.data() : (const T*)nullptr, smoothed ? C.c2[0] : T(0),
+            C.xa.data(), C.d.data());
         cycle(l + 1, s, true);
-        with_lanes(L.p.lanes, [&](auto lc) {
-            constexpr int LN = decltype(lc)::value;
-            launch_blocks((k_csr_prolong_add<LN, T, T>), row_blocks(n, LN), s, n, (const uint32_t*)L.p.ptr.data(),
-                          (const uint32_t*)L.p.idx.data(), (const T*)L.p.val.data(), (const T*)C.x(), L.x());
-        });
+        launch_blocks((k_csr_prolong_add<T, T>), nblocks(n), s, n, (const uint32_t*)L.p.base.data(),
+            (const uint32_t*)L.p.col.data(), (const T*)L.p.val.data(), (const T*)C.x(), L.x());
         for (int st = 0; st < degree; ++st) {
             step(L, st == 0 ? T(0) : L.c1[st], L.c2[st], s);
             L.cur ^= 1;
@@ -162,12 +146,9 @@ struct CHierarchy {
     }
 
     void step(CLevel<T>& L, T c1, T c2, hipStream_t s) {
-        with_lanes(L.a.lanes, [&](auto lc) {
-            constexpr int LN = decltype(lc)::value;
-            launch_blocks((k_csr_cheb<LN, T, T, T>), row_blocks(L.n, LN), s, L.n, (const uint32_t*)L.a.ptr.data(),
-                          (const uint32_t*)L.a.idx.data(), (const T*)L.a.val.data(), (const T*)L.x(), L.x2(),
-                          L.d.data(), (const T*)L.b.data(), (const T*)L.invd.data(), c1, c2);
-        });
+        launch_blocks((k_csr_cheb<T, T, T>), nblocks(L.n), s, L.n, (const uint32_t*)L.a.base.data(),
+            (const uint32_t*)L.a.col.data(), (const T*)L.a.val.data(), (const T*)L.x(), L.x2(),
+            L.d.data(), (const T*)L.b.data(), (const T*)L.invd.data(), c1, c2);
     }
 };
 
@@ -200,7 +181,7 @@ class CsrSolverImpl {
         n_ = n;
         a64_.upload(n, n, a.ptr, a.idx, a.val, s_);
         for (DArray<double>* v : {&x_, &b_, &r_, &p_, &q_, &z_}) v->resize(n);
-        parts_ = std::max<uint64_t>(row_blocks(n, a64_.lanes), (n + BLOCK - 1) / BLOCK) + 1;
+        parts_ = (n + BLOCK - 1) / BLOCK + 1;
         partA_.resize(parts_);
         partB_.resize(parts_);
         scal_.resize(1);
@@ -413,11 +394,8 @@ class CsrSolverImpl {
             launch_blocks(k_dot<T>, pb, s_, n, (const T*)v, (const T*)v, partA_.data());
             launch_reduce(k_sum, s_, (const double*)partA_.data(), (int)pb, red_.data());
             launch(k_normalise<T>, n, s_, n, v, (const double*)red_.data());
-            with_lanes(L.a.lanes, [&](auto lc) {
-                constexpr int LN = decltype(lc)::value;
-                launch_blocks((k_csr_apply<LN, T, T>), row_blocks(n, LN), s_, n, (const uint32_t*)L.a.ptr.data(),
-                              (const uint32_t*)L.a.idx.data(), (const T*)L.a.val.data(), (const T*)v, w);
-            });
+            launch_blocks((k_csr_apply<T, T>), nblocks(n), s_, n, (const uint32_t*)L.a.base.data(),
+                (const uint32_t*)L.a.col.data(), (const T*)L.a.val.data(), (const T*)v, w);
             launch(k_scale<T>, n, s_, n, w, (const T*)L.invd.data());
             if (it + 1 == opt_.power_steps) {
                 launch_blocks(k_dot<T>, pb, s_, n, (const T*)w, (const T*)w, partA_.data());
@@ -454,12 +432,9 @@ class CsrSolverImpl {
 
     // ---------------------------------------------------------------- the operator and PCG
     void apply64(const double* x, double* y) {
-        with_lanes(a64_.lanes, [&](auto lc) {
-            constexpr int LN = decltype(lc)::value;
-            launch_blocks((k_csr_apply<LN, double, double>), row_blocks(n_, LN), s_, n_,
-                          (const uint32_t*)a64_.ptr.data(), (const uint32_t*)a64_.idx.data(),
-                          (const double*)a64_.val.data(), x, y);
-        });
+        launch_blocks((k_csr_apply<double, double>), nblocks(n_), s_, n_,
+            (const uint32_t*)a64_.base.data(), (const uint32_t*)a64_.col.data(),
+            (const double*)a64_.val.data(), x, y);
     }
 
     // |b - A x| / |b| in FP64.
@@ -476,13 +451,10 @@ class CsrSolverImpl {
 
     // r = b - A x with the partials of b.b (partA_) and r.r (partB_); returns their count.
     unsigned start_residual() {
-        const unsigned nb = row_blocks(n_, a64_.lanes);
-        with_lanes(a64_.lanes, [&](auto lc) {
-            constexpr int LN = decltype(lc)::value;
-            launch_blocks((k_csr_pcg_start<LN>), nb, s_, n_, (const uint32_t*)a64_.ptr.data(),
-                          (const uint32_t*)a64_.idx.data(), (const double*)a64_.val.data(), (const double*)x_.data(),
-                          (const double*)b_.data(), r_.data(), partA_.data(), partB_.data());
-        });
+        const unsigned nb = nblocks(n_);
+        launch_blocks((k_csr_pcg_start), nb, s_, n_, (const uint32_t*)a64_.base.data(),
+            (const uint32_t*)a64_.col.data(), (const double*)a64_.val.data(), (const double*)x_.data(),
+            (const double*)b_.data(), r_.data(), partA_.data(), partB_.data());
         return nb;
     }
 
@@ -517,13 +489,10 @@ class CsrSolverImpl {
     void iteration_body() {
         const unsigned nbv = rblocks(n_);
         PcgScalars* sc = scal_.data();
-        const unsigned nq = row_blocks(n_, a64_.lanes);
-        with_lanes(a64_.lanes, [&](auto lc) {
-            constexpr int LN = decltype(lc)::value;
-            launch_blocks((k_csr_pcg_q<LN>), nq, s_, n_, (const uint32_t*)a64_.ptr.data(),
-                          (const uint32_t*)a64_.idx.data(), (const double*)a64_.val.data(), (const double*)p_.data(),
-                          q_.data(), partA_.data());
-        });
+        const unsigned nq = nblocks(n_);
+        launch_blocks((k_csr_pcg_q), nq, s_, n_, (const uint32_t*)a64_.base.data(),
+            (const uint32_t*)a64_.col.data(), (const double*)a64_.val.data(), (const double*)p_.data(),
+            q_.data(), partA_.data());
         launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr, (int)nq, 1);
         if (h32_) {
             update_and_vcycle(*h32_);
