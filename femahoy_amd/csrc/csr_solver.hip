@@ -107,7 +107,7 @@ template <class T>
 struct CHierarchy {
     std::vector<CLevel<T>> lv;
     DenseCoarse coarse;
-    int degree = 3;
+    int degree = 2;
 
     void cycle(size_t l, hipStream_t s, bool first_done) {
         CLevel<T>& L = lv[l];

@@ -367,7 +367,7 @@ class Hierarchy {
    public:
     std::vector<Level<T>> lv;
     DenseCoarse coarse;
-    int degree = 3;
+    int degree = 2;
     bool bf16 = false;  // BF16 storage of the levels' coefficients and D^-1 (FP32 hierarchies)
     Profiler* prof = nullptr;
 
@@ -1950,7 +1950,7 @@ fem::Options options_from(const fem_options& opt) {
     o.precision = opt.precision;
     o.crossover = opt.crossover;
     o.levels = opt.levels;
-    o.degree = opt.cheb_degree > 0 ? opt.cheb_degree : 3;
+    o.degree = opt.cheb_degree > 0 ? opt.cheb_degree : 2;
     o.cheb_lower = opt.cheb_lower;
     o.cheb_upper = opt.cheb_upper;
     o.power_steps = opt.power_steps > 0 ? opt.power_steps : 20;

@@ -48,7 +48,7 @@ struct Options {
     int precision = FEM_PREC_FP32;
     uint64_t crossover = 3000;
     int levels = 0;
-    int degree = 3;
+    int degree = 2;
     double cheb_lower = 0.1, cheb_upper = 1.1;
     int power_steps = 20;
     uint64_t coarse_max = 6000;

@@ -101,7 +101,7 @@ typedef struct fem_stats {
 typedef struct fem_solver fem_solver;
 
 /* Fills the defaults: tol 1e-10, max_iter 500, AUTO, FP32 V-cycle, crossover 3000, automatic
- * levels, Chebyshev degree 3 on [0.1, 1.1] lmax with 20 power steps, coarse_max 6000, refine 1,
+ * levels, Chebyshev degree 2 on [0.1, 1.1] lmax with 20 power steps, coarse_max 6000, refine 1,
  * Galerkin coarse operators. */
 void fem_options_default(fem_options* opt);
 
