@@ -16,21 +16,25 @@
 
 namespace fem {
 
-// kz: the planes a thread walks. The walk is software pipelined (the loads of the next plane are
-// in flight while a plane is computed), and at 129^3 every wave of the launch is resident at once,
-// so the launch time is the walk's latency: 6 planes per thread measured best (FP32 Chebyshev
-// step 12.9 us, against 14.7 at 4 planes and 13.8 at 8; FP64 19.6 us).
+// kz: the planes a thread walks; pf: how many planes ahead of the one computed its loads are
+// issued. At 129^3 every wave of the launch is resident at once. Measured at 129^3 (FP32
+// Chebyshev step, FP64 q = A p): kz 6, pf 1: 12.4 and 18.3 us; pf 2: 12.4 and 17.9; pf 3: 12.8 and
+// 18.5; kz 8, pf 2: 13.2 and 18.1; kz 4, pf 2: 13.5 and 19.1. Non-temporal node loads or stores
+// cost 2.5-3.5 us (the level's vectors live in the last-level cache between launches), and
+// capping the FP64 kernel at 80 VGPRs for 6 waves per SIMD makes it spill (41 us).
 template <class T>
 struct MarchShape;
 template <>
 struct MarchShape<float> {
     static constexpr bool shuffle = true;
     static constexpr int kz = 6;
+    static constexpr int pf = 1;
 };
 template <>
 struct MarchShape<double> {
     static constexpr bool shuffle = true;
     static constexpr int kz = 6;
+    static constexpr int pf = 1;
 };
 
 // Columns a block of BLOCK threads owns.
@@ -115,7 +119,8 @@ __device__ __forceinline__ void xcd_tile(unsigned& bx, unsigned& by) {
     by = t / gridDim.x;
 }
 
-template <class T, int MODE, bool XCD = true, bool SHFL = MarchShape<T>::shuffle, int KZ = MarchShape<T>::kz, class S = T>
+template <class T, int MODE, bool XCD = true, bool SHFL = MarchShape<T>::shuffle, int KZ = MarchShape<T>::kz, class S = T,
+          int PF = MarchShape<T>::pf>
 __global__ void __launch_bounds__(BLOCK) k_march_hex8(MarchArgs<T, S> a) {
     const GridDims g = a.g;
     unsigned bx = blockIdx.x, by = blockIdx.y;
@@ -240,35 +245,38 @@ __global__ void __launch_bounds__(BLOCK) k_march_hex8(MarchArgs<T, S> a) {
         out[1] = NV > 1 ? buffer_value<T>(rd, v, u) : T(0);
         out[2] = NV > 2 ? buffer_value_as<T, S>(ri, v, u) : T(0);
     };
+    // Raw loads by plane: x planes k0 - 1 .. k0 + KZ, cell planes k0 - 1 .. k0 + KZ - 1, node data
+    // of planes k0 .. k0 + KZ - 1. Fully unrolled, so each entry is a register; the loads of step
+    // st + PF are issued at step st, which keeps PF planes in flight behind the one computed.
+    T xr[KZ + 2][RX], kr[KZ + 1][RC], nr[KZ][3];
     T xv[27], kv[8];  // x planes k-1, k, k+1 (9 each); cells of planes k-1, k (4 each)
-    T xr[RX], kr[RC], nv[3];
-    load_plane(k0 - 1, xr);
-    expand_plane(xr, xv);
-    load_plane(k0, xr);
-    expand_plane(xr, xv + 9);
-    load_cells(k0 - 1, kr);
-    expand_cells(kr, kv);
-    load_plane(k0 + 1, xr);
-    load_cells(k0, kr);
-    load_node(k0, nv);
-    expand_plane(xr, xv + 18);
-    expand_cells(kr, kv + 4);
+    load_plane(k0 - 1, xr[0]);
+    load_plane(k0, xr[1]);
+    load_cells(k0 - 1, kr[0]);
+#pragma unroll
+    for (int st = 0; st < PF && st < KZ; ++st) {
+        load_plane(k0 + st + 1, xr[st + 2]);
+        load_cells(k0 + st, kr[st + 1]);
+        load_node(k0 + st, nr[st]);
+    }
+    expand_plane(xr[0], xv);
+    expand_plane(xr[1], xv + 9);
+    expand_cells(kr[0], kv);
     const bool fxi = ((g.fix & 1u) && i == 0) || ((g.fix & 2u) && i == g.nx - 1);
     const bool fxj = ((g.fix & 4u) && j == 0) || ((g.fix & 8u) && j == g.ny - 1);
     double acc1 = 0.0, acc2 = 0.0;
-    // Software pipelined over the KZ planes: the loads of plane kk + 1 are in flight while plane kk
-    // is computed and written.
 #pragma unroll
     for (int st = 0; st < KZ; ++st) {
         const int kk = k0 + st;
         if (kk >= k1) break;  // uniform over the block
-        T xn[RX], kn[RC], nn[3];
-        const bool more = st + 1 < KZ;
-        if (more) {
-            load_plane(kk + 2, xn);
-            load_cells(kk + 1, kn);
-            load_node(kk + 1, nn);
+        if (st + PF < KZ) {
+            load_plane(kk + PF + 1, xr[st + PF + 2]);
+            load_cells(kk + PF, kr[st + PF + 1]);
+            load_node(kk + PF, nr[st + PF]);
         }
+        expand_plane(xr[st + 2], xv + 18);
+        expand_cells(kr[st + 1], kv + 4);
+        const T* nv = nr[st];
         T acc = T(0);
 #pragma unroll
         for (int cz = 0; cz < 2; ++cz)
@@ -312,16 +320,10 @@ __global__ void __launch_bounds__(BLOCK) k_march_hex8(MarchArgs<T, S> a) {
                 acc2 += (double)r * (double)r;
             }
         }
-        if (more) {
 #pragma unroll
-            for (int t = 0; t < 18; ++t) xv[t] = xv[t + 9];
+        for (int t = 0; t < 18; ++t) xv[t] = xv[t + 9];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) kv[t] = kv[t + 4];
-            expand_plane(xn, xv + 18);
-            expand_cells(kn, kv + 4);
-#pragma unroll
-            for (int t = 0; t < 3; ++t) nv[t] = nn[t];
-        }
+        for (int t = 0; t < 4; ++t) kv[t] = kv[t + 4];
     }
     if (MODE == MARCH_PCG_Q || MODE == MARCH_PCG_START) {
         acc1 = block_sum(acc1);
