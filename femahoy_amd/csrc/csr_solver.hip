@@ -493,7 +493,7 @@ class CsrSolverImpl {
         launch_blocks((k_csr_pcg_q), nq, s_, n_, (const uint32_t*)a64_.base.data(),
             (const uint32_t*)a64_.col.data(), (const double*)a64_.val.data(), (const double*)p_.data(),
             q_.data(), partA_.data());
-        launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr, (int)nq, 1);
+        launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr, (int)nq, (int)nq, 1);
         if (h32_) {
             update_and_vcycle(*h32_);
             launch_blocks((k_pcg_rz<float>), nbv, s_, n_, (const float*)h32_->lv[0].x(), (double*)nullptr,
@@ -503,7 +503,7 @@ class CsrSolverImpl {
             launch_blocks((k_pcg_rz<double>), nbv, s_, n_, (const double*)h64_->lv[0].x(), (double*)nullptr,
                           (const double*)r_.data(), partA_.data());
         }
-        launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)partB_.data(), (int)nbv, 5);
+        launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)partB_.data(), (int)nbv, (int)nbv, 5);
         if (h32_)
             launch((k_pcg_dir<float>), n_, s_, n_, (const PcgScalars*)sc, (const float*)h32_->lv[0].x(), p_.data(),
                    x_.data(), 0);
@@ -541,7 +541,7 @@ class CsrSolverImpl {
             hipLaunchKernelGGL(k_pcg_config, dim3(1), dim3(64), 0, s_, scal_.data(), opt_.tol, limit);
             const unsigned nb = start_residual();
             launch_reduce(k_pcg_scalars, s_, scal_.data(), (const double*)partA_.data(), (const double*)partB_.data(),
-                          (int)nb, 0);
+                          (int)nb, (int)nb, 0);
             PcgScalars ps = read_scalars();
             const double norm_b = std::sqrt(ps.bb);
             if (norm_b == 0.0) {  // multigrid.rs:706-710
@@ -557,7 +557,7 @@ class CsrSolverImpl {
             launch_blocks((k_pcg_rz<float>), nbv, s_, n_, (const float*)nullptr, z_.data(), (const double*)r_.data(),
                           partA_.data());
             launch_reduce(k_pcg_scalars, s_, scal_.data(), (const double*)partA_.data(), (const double*)nullptr,
-                          (int)nbv, 3);
+                          (int)nbv, (int)nbv, 3);
             launch((k_pcg_dir<double>), n_, s_, n_, (const PcgScalars*)scal_.data(), (const double*)z_.data(), p_.data(),
                    x_.data(), 1);
             uint64_t it = 0;

@@ -509,12 +509,12 @@ __device__ __forceinline__ double sum_partials(const double* part, int count) {
     return block_sum_t<RBLOCK>(v);
 }
 
-// The scalars from block partials (pcg_scalars_apply).
+// The scalars from block partials (pcg_scalars_apply): count_a of part_a, count_b of part_b.
 static __global__ void __launch_bounds__(RBLOCK) k_pcg_scalars(PcgScalars* s, const double* part_a, const double* part_b,
-                                                       int count, int what) {
-    const double a = sum_partials(part_a, count);
+                                                       int count_a, int count_b, int what) {
+    const double a = sum_partials(part_a, count_a);
     double b = 0.0;
-    if (what == 0 || what == 5) b = sum_partials(part_b, count);
+    if (what == 0 || what == 5) b = sum_partials(part_b, count_b);
     if (threadIdx.x == 0) pcg_scalars_apply(s, a, b, what);
 }
 

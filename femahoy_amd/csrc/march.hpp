@@ -43,7 +43,9 @@ constexpr int march_cols_per_block() {
     return SHFL ? (BLOCK / 64) * 62 : BLOCK;
 }
 
-enum MarchMode { MARCH_APPLY = 0, MARCH_RESIDUAL = 1, MARCH_CHEB = 2, MARCH_PCG_Q = 3, MARCH_PCG_START = 4 };
+// MARCH_CHEB_RZ: a Chebyshev step that also sums r.z over its output z (the preconditioned residual
+// of the outer PCG, r in FP64) into block partials.
+enum MarchMode { MARCH_APPLY = 0, MARCH_RESIDUAL = 1, MARCH_CHEB = 2, MARCH_PCG_Q = 3, MARCH_PCG_START = 4, MARCH_CHEB_RZ = 5 };
 
 template <class T, class S = T>
 struct MarchArgs {
@@ -56,8 +58,9 @@ struct MarchArgs {
     T* d;           // Chebyshev direction
     const S* invd;  // D^-1, stored in S
     T c1, c2;       // Chebyshev coefficients
-    double* part;   // block partials (pcg_q: p.q; pcg_start: b.b)
+    double* part;   // block partials (pcg_q: p.q; pcg_start: b.b; cheb_rz: r.z)
     double* part2;  // pcg_start: r.r
+    const double* r;  // cheb_rz: the outer residual
 };
 
 // The value of v held by lane src of the wave.
@@ -235,20 +238,24 @@ __global__ void __launch_bounds__(BLOCK) k_march_hex8(MarchArgs<T, S> a) {
         }
     };
     // The node data of the update (b; d and D^-1 for a Chebyshev step) of the column at plane kk.
-    constexpr int NV = MODE == MARCH_CHEB ? 3 : (MODE == MARCH_RESIDUAL || MODE == MARCH_PCG_START) ? 1 : 0;
+    constexpr bool CHEB = MODE == MARCH_CHEB || MODE == MARCH_CHEB_RZ;
+    constexpr int NV = CHEB ? 3 : (MODE == MARCH_RESIDUAL || MODE == MARCH_PCG_START) ? 1 : 0;
     const size_t nodes = (size_t)g.nx * g.ny * g.nz;
     const __amdgpu_buffer_rsrc_t rb = make_rsrc(a.b, nodes), rd = make_rsrc(a.d, nodes), ri = make_rsrc(a.invd, nodes);
-    auto load_node = [&](int kk, T out[3]) {
+    const __amdgpu_buffer_rsrc_t rr = make_rsrc(a.r, nodes);
+    auto load_node = [&](int kk, T out[3], double& rv) {
         const bool v = own && kk < k1;
         const unsigned u = (unsigned)kk * (unsigned)plane + (unsigned)col;
         out[0] = NV > 0 ? buffer_value<T>(rb, v, u) : T(0);
         out[1] = NV > 1 ? buffer_value<T>(rd, v, u) : T(0);
         out[2] = NV > 2 ? buffer_value_as<T, S>(ri, v, u) : T(0);
+        rv = MODE == MARCH_CHEB_RZ ? buffer_value<double>(rr, v, u) : 0.0;
     };
     // Raw loads by plane: x planes k0 - 1 .. k0 + KZ, cell planes k0 - 1 .. k0 + KZ - 1, node data
     // of planes k0 .. k0 + KZ - 1. Fully unrolled, so each entry is a register; the loads of step
     // st + PF are issued at step st, which keeps PF planes in flight behind the one computed.
     T xr[KZ + 2][RX], kr[KZ + 1][RC], nr[KZ][3];
+    double rn[KZ];
     T xv[27], kv[8];  // x planes k-1, k, k+1 (9 each); cells of planes k-1, k (4 each)
     load_plane(k0 - 1, xr[0]);
     load_plane(k0, xr[1]);
@@ -257,7 +264,7 @@ __global__ void __launch_bounds__(BLOCK) k_march_hex8(MarchArgs<T, S> a) {
     for (int st = 0; st < PF && st < KZ; ++st) {
         load_plane(k0 + st + 1, xr[st + 2]);
         load_cells(k0 + st, kr[st + 1]);
-        load_node(k0 + st, nr[st]);
+        load_node(k0 + st, nr[st], rn[st]);
     }
     expand_plane(xr[0], xv);
     expand_plane(xr[1], xv + 9);
@@ -272,7 +279,7 @@ __global__ void __launch_bounds__(BLOCK) k_march_hex8(MarchArgs<T, S> a) {
         if (st + PF < KZ) {
             load_plane(kk + PF + 1, xr[st + PF + 2]);
             load_cells(kk + PF, kr[st + PF + 1]);
-            load_node(kk + PF, nr[st + PF]);
+            load_node(kk + PF, nr[st + PF], rn[st + PF]);
         }
         expand_plane(xr[st + 2], xv + 18);
         expand_cells(kr[st + 1], kv + 4);
@@ -304,10 +311,12 @@ __global__ void __launch_bounds__(BLOCK) k_march_hex8(MarchArgs<T, S> a) {
                 a.y[u] = fixed ? T(0) : acc;
             } else if (MODE == MARCH_RESIDUAL) {
                 a.y[u] = fixed ? T(0) : nv[0] - acc;
-            } else if (MODE == MARCH_CHEB) {
+            } else if (CHEB) {
                 const T dn = fixed ? T(0) : a.c1 * nv[1] + a.c2 * nv[2] * (nv[0] - acc);
+                const T z = fixed ? T(0) : xc + dn;
                 a.d[u] = dn;
-                a.y[u] = fixed ? T(0) : xc + dn;
+                a.y[u] = z;
+                if (MODE == MARCH_CHEB_RZ) acc1 += rn[st] * (double)z;
             } else if (MODE == MARCH_PCG_Q) {
                 const T q = fixed ? T(0) : acc;
                 a.y[u] = q;
@@ -325,7 +334,7 @@ __global__ void __launch_bounds__(BLOCK) k_march_hex8(MarchArgs<T, S> a) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) kv[t] = kv[t + 4];
     }
-    if (MODE == MARCH_PCG_Q || MODE == MARCH_PCG_START) {
+    if (MODE == MARCH_PCG_Q || MODE == MARCH_PCG_START || MODE == MARCH_CHEB_RZ) {
         acc1 = block_sum(acc1);
         if (MODE == MARCH_PCG_START) acc2 = block_sum(acc2);
         if (threadIdx.x == 0) {

@@ -370,6 +370,14 @@ class Hierarchy {
     int degree = 2;
     bool bf16 = false;  // BF16 storage of the levels' coefficients and D^-1 (FP32 hierarchies)
     Profiler* prof = nullptr;
+    // With rz_part set, level 0's last smoothing step also sums r.z of the outer PCG (r = rz_r, z
+    // its output) into march_blocks partials at rz_part, when fuses_rz().
+    const double* rz_r = nullptr;
+    double* rz_part = nullptr;
+
+    bool fuses_rz() const {
+        return rz_part && lv.size() > 1 && lv[0].march() && lv[0].nodes() * sizeof(double) < MARCH_MAX_BYTES;
+    }
 
     void mark_begin(size_t l, hipStream_t s) {
         if (l == 0 && prof) prof->begin(0, s);
@@ -421,15 +429,17 @@ class Hierarchy {
         cycle(l + 1, s, true);
         transfer(L, Cc, false, s);
         for (int st = 0; st < degree; ++st) {
-            mark_begin(l, s);
-            cheb_step(L, st == 0 ? T(0) : L.c1[st], L.c2[st], s);
-            mark_end(l, s);
+            // the profile times plain steps only: the fused r.z step moves 8 more bytes a node
+            const bool rz = l == 0 && st + 1 == degree && fuses_rz();
+            if (!rz) mark_begin(l, s);
+            cheb_step(L, st == 0 ? T(0) : L.c1[st], L.c2[st], s, rz);
+            if (!rz) mark_end(l, s);
             L.cur ^= 1;
         }
     }
 
-    // d = c1 d + c2 D^-1 (b - A x), x2 = x + d.
-    void cheb_step(Level<T>& L, T c1, T c2, hipStream_t s) {
+    // d = c1 d + c2 D^-1 (b - A x), x2 = x + d; with rz (a march level), also the r.z partials.
+    void cheb_step(Level<T>& L, T c1, T c2, hipStream_t s, bool rz = false) {
         if (L.march()) {
             with_march(L, [&](auto m) {
                 m.x = L.x();
@@ -438,7 +448,13 @@ class Hierarchy {
                 m.b = L.b.data();
                 m.c1 = c1;
                 m.c2 = c2;
-                launch_march<T, MARCH_CHEB>(m, s);
+                if (rz) {
+                    m.r = rz_r;
+                    m.part = rz_part;
+                    launch_march<T, MARCH_CHEB_RZ>(m, s);
+                } else {
+                    launch_march<T, MARCH_CHEB>(m, s);
+                }
             });
             return;
         }
@@ -535,7 +551,8 @@ class Solver {
         q_.resize(n0_);
         tmp_.resize(n0_);
         // partials of the reducing kernels: one per block of a node, vector or march launch
-        parts_ = std::max<size_t>(nblocks(n0_), std::max<size_t>(nblocks(lv64_[0].nodes()), march_blocks<double>(lv64_[0].g))) + 1;
+        parts_ = std::max<size_t>({nblocks(n0_), nblocks(lv64_[0].nodes()), march_blocks<double>(lv64_[0].g),
+                                   march_blocks<float>(lv64_[0].g)}) + 1;
         partA_.resize(parts_);
         partB_.resize(parts_);
         scal_.resize(1);
@@ -560,10 +577,14 @@ class Solver {
             if (o.precision == FEM_PREC_FP64) {
                 h64_ = std::make_unique<Hierarchy<double>>();
                 build_hierarchy(*h64_);
+                h64_->rz_r = r_.data();
+                h64_->rz_part = partA_.data();
             } else {
                 h32_ = std::make_unique<Hierarchy<float>>();
                 h32_->bf16 = o.precision == FEM_PREC_BF16;
                 build_hierarchy(*h32_);
+                h32_->rz_r = r_.data();
+                h32_->rz_part = partA_.data();
                 r32_.resize(n0_);
             }
         }
@@ -839,7 +860,7 @@ class Solver {
                     launch_blocks((k_pcg_rz<float>), nbv, s_, n, (const float*)nullptr, z_.data(),
                                   (const double*)r_.data(), partA_.data());
                     launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr,
-                                  (int)nbv, 3);
+                                  (int)nbv, (int)nbv, 3);
                     launch((k_pcg_dir<double>), n, s_, n, (const PcgScalars*)sc, (const double*)z_.data(), p_.data(),
                            x_.data(), 1);
                     run_.it = 0;
@@ -1511,19 +1532,28 @@ class Solver {
         prof_.begin(1, s_);
         const unsigned nq = outer_apply();
         prof_.end(s_);
-        launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr, (int)nq, 1);
+        launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr, (int)nq, (int)nq, 1);
         // the update, r.r and the stopping test, and the V-cycle's first step on level 0 (when
         // level 0 smooths); then the V-cycle, then r.z and beta
+        // r.z: summed by level 0's last smoothing step where it fuses it, else here
+        unsigned nrz = nbv;
         if (h32_) {
             update_and_vcycle(*h32_);
-            launch_blocks((k_pcg_rz<float>), nbv, s_, n, (const float*)h32_->lv[0].x(), (double*)nullptr,
-                          (const double*)r_.data(), partA_.data());
+            if (h32_->fuses_rz())
+                nrz = (unsigned)march_blocks<float>(h32_->lv[0].g);
+            else
+                launch_blocks((k_pcg_rz<float>), nbv, s_, n, (const float*)h32_->lv[0].x(), (double*)nullptr,
+                              (const double*)r_.data(), partA_.data());
         } else {
             update_and_vcycle(*h64_);
-            launch_blocks((k_pcg_rz<double>), nbv, s_, n, (const double*)h64_->lv[0].x(), (double*)nullptr,
-                          (const double*)r_.data(), partA_.data());
+            if (h64_->fuses_rz())
+                nrz = (unsigned)march_blocks<double>(h64_->lv[0].g);
+            else
+                launch_blocks((k_pcg_rz<double>), nbv, s_, n, (const double*)h64_->lv[0].x(), (double*)nullptr,
+                              (const double*)r_.data(), partA_.data());
         }
-        launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)partB_.data(), (int)nbv, 5);
+        launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)partB_.data(), (int)nrz, (int)nbv,
+                      5);
         // z straight from the V-cycle's level-0 iterate (kept until the next V-cycle)
         if (h32_)
             launch((k_pcg_dir<float>), n, s_, n, (const PcgScalars*)sc, (const float*)h32_->lv[0].x(), p_.data(),
@@ -1564,7 +1594,7 @@ class Solver {
         hipLaunchKernelGGL(k_pcg_config, dim3(1), dim3(64), 0, s_, scal_.data(), opt_.tol, limit);
         const unsigned nbn = residual64(x_.data(), r_.data());
         launch_reduce(k_pcg_scalars, s_, scal_.data(), (const double*)partA_.data(), (const double*)partB_.data(),
-                      (int)nbn, 0);
+                      (int)nbn, (int)nbn, 0);
         read_back();
         run_.phase = Phase::Start;
     }
