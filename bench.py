@@ -5,9 +5,9 @@ FP64 MG-PCG, with p50/p99 latency.
 Workload (one step = one solve, right side resident in HBM to solution resident): 3D steady heat
 conduction, hex8 on a structured 128^3 unit cube (2,113,407 free unknowns), per-cell conductivity
 exp(N(0, 1)) from numpy's default_rng(1234), T = 0 on x = 0 and T = 1 on x = 1, insulated
-elsewhere; 4-level geometric multigrid with Chebyshev(2)-Jacobi smoothing and a dense FP64 LDL^T
-coarse level. `value` is the mixed solve (FP32 V-cycle inside the FP64 PCG); the plain FP64
-MG-PCG is run and reported beside it. Usage:
+elsewhere; 4-level geometric multigrid with Chebyshev-Jacobi smoothing (degree 3 on the finest
+level, 1 below) and a dense FP64 LDL^T coarse level. `value` is the mixed solve (FP32 V-cycle inside
+the FP64 PCG); the plain FP64 MG-PCG is run and reported beside it. Usage:
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--no-cpu]
 """
@@ -79,6 +79,14 @@ def get_profile(F, solver):
 
 def set_profile(F, solver, on):
     solver.set_profile(on)
+
+
+def cheb_desc(degree, vector):
+    """The smoother of the solves: an explicit degree, or the solver's automatic choice (degree 2
+    on every level for vector problems and the assembled boundary)."""
+    if degree > 0:
+        return f"Chebyshev({degree})-Jacobi"
+    return "Chebyshev(2)-Jacobi" if vector else "Chebyshev-Jacobi of degree 3 on the finest level, 1 below"
 
 
 def heat_box(F, n):
@@ -381,7 +389,7 @@ def assembled_leg(F, hip, solver, box, x_ref, args):
             "setup_s": round(t2 - t1, 3),
             "workload": "the same heat box through fem_solver_new_csr: CSR SpMV operators on every level (FP32 "
                         "V-cycle, FP64 PCG), the reference's coarsen and Galerkin levels down to 3000 unknowns, "
-                        f"Chebyshev({args.degree})-Jacobi, dense LDLT coarse, tol 1e-8"}
+                        f"{cheb_desc(args.degree, True)}, dense LDLT coarse, tol 1e-8"}
 
 
 def main():
@@ -393,7 +401,7 @@ def main():
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--cpu-n", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--degree", type=int, default=2)
+    ap.add_argument("--degree", type=int, default=0, help="Chebyshev degree on every level (0: automatic)")
     ap.add_argument("--coarse", choices=["galerkin", "mean"], default="galerkin")
     ap.add_argument("--elastic-n", type=int, default=60,
                     help="cells along x of the tet10 elasticity box reported beside the headline (0: skip)")
@@ -460,7 +468,7 @@ def main():
         elastic.pop("_x_ref")
         elastic["workload"] = (f"linear elasticity, tet10 on {en}x{en // 2}x{en // 2} unit cells (6 Kuhn tets per cell), "
                                "E=1, nu=0.3, clamped at x=0, body force (0,0,-1); p-MG tet10->tet4 then geometric tet4 "
-                               f"levels, dense FP64 LDLT coarse, Chebyshev({args.degree})-Jacobi, tol 1e-8")
+                               f"levels, dense FP64 LDLT coarse, {cheb_desc(args.degree, True)}, tol 1e-8")
         if eprof:
             elastic["finest_smoother"] = {"kernel": "k_cheb<VectorOp<Tet10Op<float>>> (+ its element pass)",
                                           "bytes_per_launch": int(eprof["smoother_bytes"]),
@@ -477,7 +485,7 @@ def main():
                                   f"cells, nu=0.49 matrix (E=1) with {args.inclusions} seeded spheres (radius 0.02-0.05 "
                                   "of the box length, E=1e3, nu=0.3; a tet is inside when its centroid is), clamped at "
                                   "x=0, traction (1,0,0) on x=1; p-MG tet10->tet4 then geometric tet4 levels, dense FP64 "
-                                  f"LDLT coarse, Chebyshev({args.degree})-Jacobi, tol 1e-8 with refinement against the FP64 true "
+                                  f"LDLT coarse, {cheb_desc(args.degree, True)}, tol 1e-8 with refinement against the FP64 true "
                                   "residual")
         inclusions["stiff_fraction"] = round(float(inside.mean()), 4)
 
@@ -490,7 +498,7 @@ def main():
         log("  " + cpu["sample"])
 
     config = {"workload": (f"heat hex8 {n}^3 unit cube, k = exp(N(0,1)) per cell (seed 1234), T=0 at x=0 and T=1 at "
-                           f"x=1, {args.levels}-level GMG ({args.coarse} coarse operators), Chebyshev({args.degree})-Jacobi, "
+                           f"x=1, {args.levels}-level GMG ({args.coarse} coarse operators), {cheb_desc(args.degree, False)}, "
                            "dense FP64 LDLT coarse, tol 1e-8")}
     config.update(heat)
     if elastic:

@@ -346,7 +346,7 @@ class Model:
 
 
 def _options(method=None, precision="fp32", tol=PCG_TOL, max_iter=PCG_MAX_ITER, crossover=STEADY_DIRECT_LIMIT,
-             levels=0, cheb_degree=2, cheb_lower=0.1, cheb_upper=1.1, power_steps=20, coarse_max=6000, refine=True,
+             levels=0, cheb_degree=0, cheb_lower=0.1, cheb_upper=1.1, power_steps=20, coarse_max=6000, refine=True,
              device=0, coarse_galerkin=True) -> "_Options":
     o = _Options()
     lib().fem_options_default(C.byref(o))
@@ -363,7 +363,7 @@ class Solver:
 
     def __init__(self, box: Box, method: Optional[str] = None, precision: str = "fp32", tol: float = PCG_TOL,
                  max_iter: int = PCG_MAX_ITER, crossover: int = STEADY_DIRECT_LIMIT, levels: int = 0,
-                 cheb_degree: int = 2, cheb_lower: float = 0.1, cheb_upper: float = 1.1, power_steps: int = 20,
+                 cheb_degree: int = 0, cheb_lower: float = 0.1, cheb_upper: float = 1.1, power_steps: int = 20,
                  coarse_max: int = 6000, refine: bool = True, device: int = 0, coarse_galerkin: bool = True):
         o = _options(method, precision, tol, max_iter, crossover, levels, cheb_degree, cheb_lower, cheb_upper,
                      power_steps, coarse_max, refine, device, coarse_galerkin)
@@ -539,7 +539,7 @@ class CsrSolver(Solver):
     def __init__(self, ptr, idx, val, unknowns: str = "grid", grid: Optional[Grid] = None, p=None,
                  coarse_grid: Optional[Grid] = None, method: Optional[str] = None, precision: str = "fp32",
                  tol: float = PCG_TOL, max_iter: int = PCG_MAX_ITER, crossover: int = STEADY_DIRECT_LIMIT,
-                 levels: int = 0, cheb_degree: int = 2, cheb_lower: float = 0.1, cheb_upper: float = 1.1,
+                 levels: int = 0, cheb_degree: int = 0, cheb_lower: float = 0.1, cheb_upper: float = 1.1,
                  power_steps: int = 20, coarse_max: int = 6000, refine: bool = True, device: int = 0):
         n = len(ptr) - 1
         o = _options(method, precision, tol, max_iter, crossover, levels, cheb_degree, cheb_lower, cheb_upper,

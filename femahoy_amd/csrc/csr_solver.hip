@@ -107,7 +107,6 @@ template <class T>
 struct CHierarchy {
     std::vector<CLevel<T>> lv;
     DenseCoarse coarse;
-    int degree = 2;
 
     void cycle(size_t l, hipStream_t s, bool first_done) {
         CLevel<T>& L = lv[l];
@@ -121,7 +120,8 @@ struct CHierarchy {
         const uint64_t n = L.n;
         if (!first_done) launch(k_cheb_first<T, T>, n, s, n, (const T*)L.b.data(), (const T*)L.invd.data(), L.x(),
                                 L.d.data(), L.c2[0]);
-        for (int st = 1; st < degree; ++st) {
+        const int deg = (int)L.c1.size();  // the level's smoothing steps
+        for (int st = 1; st < deg; ++st) {
             step(L, L.c1[st], L.c2[st], s);
             L.cur ^= 1;
         }
@@ -139,7 +139,7 @@ struct CHierarchy {
         cycle(l + 1, s, true);
         launch_blocks((k_csr_prolong_add<T, T>), nblocks(n), s, n, (const uint32_t*)L.p.base.data(),
             (const uint32_t*)L.p.col.data(), (const T*)L.p.val.data(), (const T*)C.x(), L.x());
-        for (int st = 0; st < degree; ++st) {
+        for (int st = 0; st < deg; ++st) {
             step(L, st == 0 ? T(0) : L.c1[st], L.c2[st], s);
             L.cur ^= 1;
         }
@@ -356,7 +356,6 @@ class CsrSolverImpl {
     template <class T>
     void make_levels(CHierarchy<T>& H) {
         const size_t count = host_.a.size();
-        H.degree = opt_.degree;
         H.lv.resize(count);
         for (size_t l = 0; l < count; ++l) {
             CLevel<T>& L = H.lv[l];
@@ -378,7 +377,10 @@ class CsrSolverImpl {
             L.invd.upload(inv.data(), L.n, s_);
             FEM_HIP(hipStreamSynchronize(s_));
             L.lmax = estimate_lmax(L);
-            chebyshev(L);
+            // automatic: 2 on every level. A finest step moves the whole matrix (8 B per entry in
+            // FP32), so the matrix-free path's 3-then-1 choice costs more than the iteration it
+            // saves (heat 128^3: 168 against 197 MDOF/s).
+            chebyshev(L, opt_.degree > 0 ? opt_.degree : 2);
         }
         factor_dense(host_.a.back(), H.coarse, sizeof(T) == 4);
     }
@@ -415,14 +417,14 @@ class CsrSolverImpl {
 
     // The Chebyshev recurrence on [cheb_lower, cheb_upper] lmax (as the box solver's).
     template <class T>
-    void chebyshev(CLevel<T>& L) {
+    void chebyshev(CLevel<T>& L, int degree) {
         const double hi = opt_.cheb_upper * L.lmax, lo = opt_.cheb_lower * L.lmax;
         const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
-        L.c1.assign(opt_.degree, T(0));
-        L.c2.assign(opt_.degree, T(0));
+        L.c1.assign(degree, T(0));
+        L.c2.assign(degree, T(0));
         double rho = 1.0 / sigma;
         L.c2[0] = T(1.0 / theta);
-        for (int s = 1; s < opt_.degree; ++s) {
+        for (int s = 1; s < degree; ++s) {
             const double rn = 1.0 / (2.0 * sigma - rho);
             L.c1[s] = T(rn * rho);
             L.c2[s] = T(2.0 * rn / delta);

@@ -367,7 +367,6 @@ class Hierarchy {
    public:
     std::vector<Level<T>> lv;
     DenseCoarse coarse;
-    int degree = 2;
     bool bf16 = false;  // BF16 storage of the levels' coefficients and D^-1 (FP32 hierarchies)
     Profiler* prof = nullptr;
     // With rz_part set, level 0's last smoothing step also sums r.z of the outer PCG (r = rz_r, z
@@ -407,7 +406,8 @@ class Hierarchy {
                        L.c2[0]);
             });
         }
-        for (int st = 1; st < degree; ++st) {
+        const int deg = (int)L.c1.size();  // the level's smoothing steps
+        for (int st = 1; st < deg; ++st) {
             mark_begin(l, s);
             cheb_step(L, L.c1[st], L.c2[st], s);
             mark_end(l, s);
@@ -428,9 +428,9 @@ class Hierarchy {
         transfer(L, Cc, true, s, smoothed);
         cycle(l + 1, s, true);
         transfer(L, Cc, false, s);
-        for (int st = 0; st < degree; ++st) {
+        for (int st = 0; st < deg; ++st) {
             // the profile times plain steps only: the fused r.z step moves 8 more bytes a node
-            const bool rz = l == 0 && st + 1 == degree && fuses_rz();
+            const bool rz = l == 0 && st + 1 == deg && fuses_rz();
             if (!rz) mark_begin(l, s);
             cheb_step(L, st == 0 ? T(0) : L.c1[st], L.c2[st], s, rz);
             if (!rz) mark_end(l, s);
@@ -1293,7 +1293,6 @@ class Solver {
         if (ncoarse > opt_.coarse_max)
             throw Error("the coarsest level has " + std::to_string(ncoarse) + " unknowns, more than the dense limit " +
                         std::to_string(opt_.coarse_max) + ": ask for more levels");
-        H.degree = opt_.degree;
         H.lv.resize(lv64_.size());
         for (size_t l = 0; l < lv64_.size(); ++l) {
             Level<T>& L = H.lv[l];
@@ -1326,7 +1325,7 @@ class Solver {
                     }
                 }
                 L.lmax = estimate_lmax(L);
-                chebyshev(L);
+                chebyshev(L, level_degree(opt_.degree, l, lv64_[0].comps > 1));
             }
         }
         assemble_and_factor(lv64_.back(), H.coarse, sizeof(T) == 4);
@@ -1365,15 +1364,15 @@ class Solver {
     }
 
     template <class T>
-    void chebyshev(Level<T>& L) {
+    void chebyshev(Level<T>& L, int degree) {
         const double hi = opt_.cheb_upper * L.lmax, lo = opt_.cheb_lower * L.lmax;
         const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
-        L.c1.assign(opt_.degree, T(0));
-        L.c2.assign(opt_.degree, T(0));
+        L.c1.assign(degree, T(0));
+        L.c2.assign(degree, T(0));
         double rho = 1.0 / sigma;
         L.c1[0] = T(0);
         L.c2[0] = T(1.0 / theta);
-        for (int s = 1; s < opt_.degree; ++s) {
+        for (int s = 1; s < degree; ++s) {
             const double rn = 1.0 / (2.0 * sigma - rho);
             L.c1[s] = T(rn * rho);
             L.c2[s] = T(2.0 * rn / delta);
@@ -1980,7 +1979,7 @@ fem::Options options_from(const fem_options& opt) {
     o.precision = opt.precision;
     o.crossover = opt.crossover;
     o.levels = opt.levels;
-    o.degree = opt.cheb_degree > 0 ? opt.cheb_degree : 2;
+    o.degree = opt.cheb_degree > 0 ? opt.cheb_degree : 0;
     o.cheb_lower = opt.cheb_lower;
     o.cheb_upper = opt.cheb_upper;
     o.power_steps = opt.power_steps > 0 ? opt.power_steps : 20;

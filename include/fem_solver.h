@@ -70,7 +70,9 @@ typedef struct fem_options {
     int32_t precision;      /* enum fem_precision */
     uint64_t crossover;     /* direct up to this many unknowns with FEM_METHOD_AUTO; solve.rs:18 (3000) */
     int32_t levels;         /* V-cycle levels including the finest; 0 = automatic: down to max(1000, free/256) unknowns, at most coarse_max */
-    int32_t cheb_degree;    /* Chebyshev-Jacobi smoother degree (pre and post) */
+    int32_t cheb_degree;    /* Chebyshev-Jacobi smoother degree (pre and post) on every level; 0 =
+                               automatic: for a box, 3 on the finest level and 1 below for scalar
+                               problems, 2 on every level for vector ones; 2 for fem_solver_new_csr */
     double cheb_lower;      /* smoothed range [cheb_lower * lmax, cheb_upper * lmax] of D^-1 A */
     double cheb_upper;
     int32_t power_steps;    /* power iterations of the lmax estimate on each level */
@@ -101,7 +103,7 @@ typedef struct fem_stats {
 typedef struct fem_solver fem_solver;
 
 /* Fills the defaults: tol 1e-10, max_iter 500, AUTO, FP32 V-cycle, crossover 3000, automatic
- * levels, Chebyshev degree 2 on [0.1, 1.1] lmax with 20 power steps, coarse_max 6000, refine 1,
+ * levels, automatic Chebyshev degree on [0.1, 1.1] lmax with 20 power steps, coarse_max 6000, refine 1,
  * Galerkin coarse operators. */
 void fem_options_default(fem_options* opt);
 
