@@ -6,9 +6,9 @@
 // Two load shapes (MarchShape picks one per arithmetic type, from timings on a 129^3 grid):
 //  - shuffle: a wave owns 62 consecutive columns plus a halo lane on each side; each lane loads
 //    the 3 x values (rows j-1, j, j+1) and 2 cells of its own column and takes the x-neighbours
-//    from the lanes beside it with ds_bpermute. This cuts the vector loads of a plane from 13 to
-//    5 per lane. With the XCD-aware tile order below: FP32 16.2 us, FP64 23.8 us per Chebyshev
-//    step at 129^3 (read bytes within 1% of the algorithmic ones).
+//    from the lanes beside it with DPP wave shifts (wave_shr:1, wave_shl:1; ds_bpermute before,
+//    which made the FP64 product 1-5 % slower). This cuts the vector loads of a plane from 13 to
+//    5 per lane. At 129^3 the HBM traffic of a Chebyshev step is 1.04x its algorithmic bytes.
 //  - plain: every thread loads the 9 x values and 4 cells of its new plane itself.
 #pragma once
 
@@ -63,16 +63,19 @@ struct MarchArgs {
     const double* r;  // cheb_rz: the outer residual
 };
 
-// The value of v held by lane src of the wave.
-template <class T>
-__device__ __forceinline__ T lane_value(T v, int src) {
+// The value of v held by the lane below (CTRL 0x138, wave_shr:1) or above (0x130, wave_shl:1)
+// in the wave, by a DPP move; a lane without that neighbour gets its own value.
+template <class T, int CTRL>
+__device__ __forceinline__ T lane_shift(T v) {
     if constexpr (sizeof(T) == 4) {
-        return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
+        const int b = __float_as_int(v);
+        return __int_as_float(__builtin_amdgcn_update_dpp(b, b, CTRL, 0xf, 0xf, false));
     } else {
         const long long bits = __double_as_longlong(v);
-        const int lo = __builtin_amdgcn_ds_bpermute(src << 2, (int)(bits & 0xffffffffll));
-        const int hi = __builtin_amdgcn_ds_bpermute(src << 2, (int)(bits >> 32));
-        return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+        const int lo = (int)(bits & 0xffffffffll), hi = (int)(bits >> 32);
+        const int l2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
+        const int h2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+        return __longlong_as_double(((long long)h2 << 32) | (long long)(unsigned)l2);
     }
 }
 
@@ -163,7 +166,6 @@ __global__ void __launch_bounds__(BLOCK) k_march_hex8(MarchArgs<T, S> a) {
         cj[t] = incol && j - 1 + t >= 0 && j - 1 + t < ncy;
         cj_[t] = min(max(j - 1 + t, 0), max(ncy - 1, 0));
     }
-    const int srcl = max(lane - 1, 0), srcr = min(lane + 1, 63);
     const bool hasl = i >= 1, hasr = i + 1 < g.nx;
     // Loads through buffer resources: an out-of-grid neighbour gets an offset past the end of
     // the buffer, which the hardware range check turns into 0 without touching memory, so a plane
@@ -196,7 +198,7 @@ __global__ void __launch_bounds__(BLOCK) k_march_hex8(MarchArgs<T, S> a) {
         if constexpr (SHFL) {
 #pragma unroll
             for (int dy = 0; dy < 3; ++dy) {
-                const T l = lane_value(raw[dy], srcl), r = lane_value(raw[dy], srcr);
+                const T l = lane_shift<T, 0x138>(raw[dy]), r = lane_shift<T, 0x130>(raw[dy]);
                 out[0 + 3 * dy] = hasl ? l : T(0);
                 out[1 + 3 * dy] = raw[dy];
                 out[2 + 3 * dy] = hasr ? r : T(0);
@@ -228,7 +230,7 @@ __global__ void __launch_bounds__(BLOCK) k_march_hex8(MarchArgs<T, S> a) {
         if constexpr (SHFL) {
 #pragma unroll
             for (int cy = 0; cy < 2; ++cy) {
-                const T l = lane_value(raw[cy], srcl);
+                const T l = lane_shift<T, 0x138>(raw[cy]);
                 out[0 + 2 * cy] = hasl ? l : T(0);
                 out[1 + 2 * cy] = raw[cy];
             }
