@@ -19,13 +19,32 @@ namespace fem {
 
 constexpr int SELL_C = 64;  // rows per slice
 
-// The sum of row `row` (slice base `base`, `width` entries per row) times x.
+// The sum of row `row` (slice base `base`, `width` entries per row) times x, in the row's order.
+// The entries go in groups of SELL_U: all their columns and values are loaded, then all their x,
+// before the group is summed, so that a long row (a restriction's, or any row of a small level,
+// where no other wave hides the latency) waits for memory once per group, not once per entry.
+constexpr uint32_t SELL_U = 8;
 template <class T, class V>
 __device__ __forceinline__ T sell_row(const uint32_t* __restrict__ col, const V* __restrict__ val,
                                       const T* __restrict__ x, uint32_t base, uint32_t width, int lane) {
     T s = T(0);
     uint32_t k = base + lane;
-    for (uint32_t j = 0; j < width; ++j, k += SELL_C) s += (T)val[k] * x[col[k]];
+    uint32_t j = 0;
+    for (; j + SELL_U <= width; j += SELL_U, k += SELL_U * SELL_C) {
+        uint32_t c[SELL_U];
+        V v[SELL_U];
+        T xv[SELL_U];
+#pragma unroll
+        for (uint32_t u = 0; u < SELL_U; ++u) {
+            c[u] = col[k + u * SELL_C];
+            v[u] = val[k + u * SELL_C];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < SELL_U; ++u) xv[u] = x[c[u]];
+#pragma unroll
+        for (uint32_t u = 0; u < SELL_U; ++u) s += (T)v[u] * xv[u];
+    }
+    for (; j < width; ++j, k += SELL_C) s += (T)val[k] * x[col[k]];
     return s;
 }
 
