@@ -44,7 +44,9 @@ int threads_for_setup() {
 template <class V>
 struct DevCsr {
     uint64_t rows = 0, cols = 0, nnz = 0, padded = 0;
-    DArray<uint32_t> base, col;
+    bool narrow = false;  // 16-bit column offsets from each slice's smallest column
+    DArray<uint32_t> base, col, cfirst;
+    DArray<uint16_t> col16;
     DArray<V> val;
 
     void upload(uint64_t r, uint64_t c, const std::vector<uint64_t>& p, const std::vector<uint32_t>& ix,
@@ -53,16 +55,27 @@ struct DevCsr {
         cols = c;
         nnz = p[r];
         const uint64_t slices = (r + SELL_C - 1) / SELL_C;
-        std::vector<uint32_t> b(slices + 1, 0);
+        std::vector<uint32_t> b(slices + 1, 0), lo(slices, 0);
         uint64_t total = 0;
+        narrow = true;
         for (uint64_t sl = 0; sl < slices; ++sl) {
             uint64_t w = 0;
-            for (uint64_t q = sl * SELL_C; q < std::min<uint64_t>(r, (sl + 1) * SELL_C); ++q) w = std::max(w, p[q + 1] - p[q]);
+            uint32_t cmin = UINT32_MAX, cmax = 0;
+            for (uint64_t q = sl * SELL_C; q < std::min<uint64_t>(r, (sl + 1) * SELL_C); ++q) {
+                w = std::max(w, p[q + 1] - p[q]);
+                for (uint64_t k = p[q]; k < p[q + 1]; ++k) {
+                    cmin = std::min(cmin, ix[k]);
+                    cmax = std::max(cmax, ix[k]);
+                }
+            }
+            lo[sl] = cmin == UINT32_MAX ? 0u : cmin;  // a slice without entries: column 0
+            if (cmin != UINT32_MAX && cmax - cmin > 0xFFFFu) narrow = false;
             total += w * SELL_C;
             if (total > 0xFFFFFFFFull) throw Error("a matrix of more than 2^32 - 1 entries is beyond the device path");
             b[sl + 1] = (uint32_t)total;
         }
         padded = total;
+        // padding: value 0 at the slice's smallest column
         std::vector<uint32_t> cc(std::max<uint64_t>(total, 1), 0);
         std::vector<V> vv(std::max<uint64_t>(total, 1), V(0));
         for (uint64_t sl = 0; sl < slices; ++sl) {
@@ -75,18 +88,37 @@ struct DevCsr {
                         cc[at] = ix[p[q] + jj];
                         vv[at] = (V)v[p[q] + jj];
                     } else {
-                        cc[at] = q < r && q < c ? (uint32_t)q : 0;  // padding: value 0 at a valid column
+                        cc[at] = lo[sl];
                     }
                 }
             }
         }
         base.resize(slices + 1);
         base.upload(b.data(), slices + 1, s);
-        col.resize(cc.size());
-        col.upload(cc.data(), cc.size(), s);
+        if (narrow) {
+            std::vector<uint16_t> c16(cc.size());
+            for (uint64_t sl = 0; sl < slices; ++sl)
+                for (uint64_t at = b[sl]; at < b[sl + 1]; ++at) c16[at] = (uint16_t)(cc[at] - lo[sl]);
+            col16.resize(c16.size());
+            col16.upload(c16.data(), c16.size(), s);
+            cfirst.resize(slices);
+            cfirst.upload(lo.data(), slices, s);
+        } else {
+            col.resize(cc.size());
+            col.upload(cc.data(), cc.size(), s);
+        }
         val.resize(vv.size());
         val.upload(vv.data(), vv.size(), s);
         FEM_HIP(hipStreamSynchronize(s));  // the host staging goes out of scope
+    }
+
+    // f(SellMat<V, uint16_t>) or f(SellMat<V, uint32_t>), as the columns are stored.
+    template <class F>
+    void visit(F&& f) const {
+        if (narrow)
+            f(SellMat<V, uint16_t>{base.data(), cfirst.data(), col16.data(), val.data()});
+        else
+            f(SellMat<V, uint32_t>{base.data(), nullptr, col.data(), val.data()});
     }
 };
 
@@ -125,20 +157,24 @@ struct CHierarchy {
             step(L, L.c1[st], L.c2[st], s);
             L.cur ^= 1;
         }
-        launch_blocks((k_csr_residual<T, T>), nblocks(n), s, n, (const uint32_t*)L.a.base.data(),
-            (const uint32_t*)L.a.col.data(), (const T*)L.a.val.data(), (const T*)L.x(),
-            (const T*)L.b.data(), L.res.data());
+        L.a.visit([&](auto m) {
+            launch_blocks((k_csr_residual<T, T, typename decltype(m)::col_t>), nblocks(n), s, n, m, (const T*)L.x(),
+                          (const T*)L.b.data(), L.res.data());
+        });
         CLevel<T>& C = lv[l + 1];
         const bool smoothed = l + 2 < lv.size();
         // into the dense level: straight to its dense right side (its unknowns are in dense order)
         T* bc = smoothed ? C.b.data() : coarse.rhs<T>();
-        launch_blocks((k_csr_restrict<T, T, T>), nblocks(C.n), s, C.n, (const uint32_t*)L.r.base.data(),
-            (const uint32_t*)L.r.col.data(), (const T*)L.r.val.data(), (const T*)L.res.data(), bc,
-            smoothed ? (const T*)C.invd.data() : (const T*)nullptr, smoothed ? C.c2[0] : T(0),
-            C.xa.data(), C.d.data());
+        L.r.visit([&](auto m) {
+            launch_blocks((k_csr_restrict<T, T, typename decltype(m)::col_t, T>), nblocks(C.n), s, C.n, m,
+                          (const T*)L.res.data(), bc, smoothed ? (const T*)C.invd.data() : (const T*)nullptr,
+                          smoothed ? C.c2[0] : T(0), C.xa.data(), C.d.data());
+        });
         cycle(l + 1, s, true);
-        launch_blocks((k_csr_prolong_add<T, T>), nblocks(n), s, n, (const uint32_t*)L.p.base.data(),
-            (const uint32_t*)L.p.col.data(), (const T*)L.p.val.data(), (const T*)C.x(), L.x());
+        L.p.visit([&](auto m) {
+            launch_blocks((k_csr_prolong_add<T, T, typename decltype(m)::col_t>), nblocks(n), s, n, m, (const T*)C.x(),
+                          L.x());
+        });
         for (int st = 0; st < deg; ++st) {
             step(L, st == 0 ? T(0) : L.c1[st], L.c2[st], s);
             L.cur ^= 1;
@@ -146,9 +182,10 @@ struct CHierarchy {
     }
 
     void step(CLevel<T>& L, T c1, T c2, hipStream_t s) {
-        launch_blocks((k_csr_cheb<T, T, T>), nblocks(L.n), s, L.n, (const uint32_t*)L.a.base.data(),
-            (const uint32_t*)L.a.col.data(), (const T*)L.a.val.data(), (const T*)L.x(), L.x2(),
-            L.d.data(), (const T*)L.b.data(), (const T*)L.invd.data(), c1, c2);
+        L.a.visit([&](auto m) {
+            launch_blocks((k_csr_cheb<T, T, typename decltype(m)::col_t, T>), nblocks(L.n), s, L.n, m, (const T*)L.x(),
+                          L.x2(), L.d.data(), (const T*)L.b.data(), (const T*)L.invd.data(), c1, c2);
+        });
     }
 };
 
@@ -396,8 +433,9 @@ class CsrSolverImpl {
             launch_blocks(k_dot<T>, pb, s_, n, (const T*)v, (const T*)v, partA_.data());
             launch_reduce(k_sum, s_, (const double*)partA_.data(), (int)pb, red_.data());
             launch(k_normalise<T>, n, s_, n, v, (const double*)red_.data());
-            launch_blocks((k_csr_apply<T, T>), nblocks(n), s_, n, (const uint32_t*)L.a.base.data(),
-                (const uint32_t*)L.a.col.data(), (const T*)L.a.val.data(), (const T*)v, w);
+            L.a.visit([&](auto m) {
+                launch_blocks((k_csr_apply<T, T, typename decltype(m)::col_t>), nblocks(n), s_, n, m, (const T*)v, w);
+            });
             launch(k_scale<T>, n, s_, n, w, (const T*)L.invd.data());
             if (it + 1 == opt_.power_steps) {
                 launch_blocks(k_dot<T>, pb, s_, n, (const T*)w, (const T*)w, partA_.data());
@@ -434,9 +472,9 @@ class CsrSolverImpl {
 
     // ---------------------------------------------------------------- the operator and PCG
     void apply64(const double* x, double* y) {
-        launch_blocks((k_csr_apply<double, double>), nblocks(n_), s_, n_,
-            (const uint32_t*)a64_.base.data(), (const uint32_t*)a64_.col.data(),
-            (const double*)a64_.val.data(), x, y);
+        a64_.visit([&](auto m) {
+            launch_blocks((k_csr_apply<double, double, typename decltype(m)::col_t>), nblocks(n_), s_, n_, m, x, y);
+        });
     }
 
     // |b - A x| / |b| in FP64.
@@ -454,9 +492,10 @@ class CsrSolverImpl {
     // r = b - A x with the partials of b.b (partA_) and r.r (partB_); returns their count.
     unsigned start_residual() {
         const unsigned nb = nblocks(n_);
-        launch_blocks((k_csr_pcg_start), nb, s_, n_, (const uint32_t*)a64_.base.data(),
-            (const uint32_t*)a64_.col.data(), (const double*)a64_.val.data(), (const double*)x_.data(),
-            (const double*)b_.data(), r_.data(), partA_.data(), partB_.data());
+        a64_.visit([&](auto m) {
+            launch_blocks((k_csr_pcg_start<typename decltype(m)::col_t>), nb, s_, n_, m, (const double*)x_.data(),
+                          (const double*)b_.data(), r_.data(), partA_.data(), partB_.data());
+        });
         return nb;
     }
 
@@ -492,9 +531,10 @@ class CsrSolverImpl {
         const unsigned nbv = rblocks(n_);
         PcgScalars* sc = scal_.data();
         const unsigned nq = nblocks(n_);
-        launch_blocks((k_csr_pcg_q), nq, s_, n_, (const uint32_t*)a64_.base.data(),
-            (const uint32_t*)a64_.col.data(), (const double*)a64_.val.data(), (const double*)p_.data(),
-            q_.data(), partA_.data());
+        a64_.visit([&](auto m) {
+            launch_blocks((k_csr_pcg_q<typename decltype(m)::col_t>), nq, s_, n_, m, (const double*)p_.data(), q_.data(),
+                          partA_.data());
+        });
         launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr, (int)nq, (int)nq, 1);
         if (h32_) {
             update_and_vcycle(*h32_);

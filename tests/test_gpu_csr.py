@@ -177,3 +177,35 @@ def test_the_exported_matrix_is_the_reference_reduced_matrix(fem, oracle, kind, 
         x, _ = c.solve(s.rhs())
         xd, _, _ = oracle.Solver(pr, "direct").solve(pr.rhs())
         assert rel(x, xd) < 1e-10
+
+
+def test_unknowns_numbered_far_apart_solve_as_the_grid_numbering_does(fem, oracle):
+    """A random renumbering of 76,824 unknowns: every slice of 64 rows then reaches columns more than
+    2^16 apart, so the device keeps 32-bit columns (the grid numbering fits 16-bit offsets from each
+    slice's first column). Both numberings build the same levels up to the numbering and give the
+    same solution."""
+    import scipy.sparse as sp
+
+    n, length = [48, 48, 32], [1.0, 1.0, 0.7]
+    pr, planes, pts = heat(oracle, n, length, 11)
+    ptr, idx, val = pr.reduced().arrays()
+    m = len(ptr) - 1
+    assert m > 65536
+    a = sp.csr_matrix((val, idx, ptr), shape=(m, m))
+    perm = np.random.default_rng(12).permutation(m)  # new unknown k is old unknown perm[k]
+    ap = a[perm][:, perm].tocsr()
+    ap.sort_indices()
+    b = pr.rhs()
+    comp = np.zeros(m, np.uint8)
+    s0 = fem.CsrSolver(ptr, idx, val, "grid", fem.Grid(planes, pts, comp), method="multigrid", tol=1e-12)
+    s1 = fem.CsrSolver(ap.indptr.astype(np.int64), ap.indices.astype(np.uint32), ap.data, "grid",
+                       fem.Grid(planes, pts[perm], comp), method="multigrid", tol=1e-12)
+    x0, st0 = s0.solve(b)
+    x1, st1 = s1.solve(b[perm])
+    print(st0, st1)
+    assert abs(st0.iterations - st1.iterations) <= 1
+    back = np.empty(m)
+    back[perm] = x1
+    assert rel(back, x0) < 1e-9
+    v = np.random.default_rng(13).standard_normal(m)
+    assert rel(s1.apply(v[perm]), (a @ v)[perm]) < 1e-14
