@@ -831,7 +831,12 @@ class Solver {
             if (run_.phase == Phase::Finished) return true;
             if (run_.pending) {
                 if (wait) {
-                    FEM_HIP(hipEventSynchronize(ev_));
+                    // a blocking caller polls: the batch's readback reaches the host sooner than
+                    // through hipEventSynchronize (heat 128^3: 1.0-1.5 % faster solves)
+                    hipError_t q;
+                    while ((q = hipEventQuery(ev_)) == hipErrorNotReady) {
+                    }
+                    FEM_HIP(q);
                 } else {
                     const hipError_t q = hipEventQuery(ev_);
                     if (q == hipErrorNotReady) return false;
@@ -1411,6 +1416,16 @@ class Solver {
             launch_blocks(k_dot<double>, pb, s_, n0_, (const double*)b_.data(), (const double*)b_.data(), partA_.data());
             launch_blocks(k_dot<double>, pb, s_, n0_, (const double*)r, (const double*)r, partB_.data());
             return pb;
+        }
+        if (L.march()) {
+            MarchArgs<double> m = march_args(L);
+            m.x = x;
+            m.b = b_.data();
+            m.y = r;
+            m.part = partA_.data();
+            m.part2 = partB_.data();
+            launch_march<double, MARCH_PCG_START>(m, s_);
+            return (unsigned)march_blocks<double>(L.g);
         }
         const unsigned nb = nblocks(L.nodes());
         with_op(L, [&](auto op) {
