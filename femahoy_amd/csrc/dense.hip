@@ -432,6 +432,24 @@ extern "C" int fem_test_gemm_nt(int m, int n, int k, const double* A, const doub
     });
 }
 
+extern "C" int fem_test_dense_solve32(int n, const double* A, const float* b, float* x, char* err, size_t errlen) {
+    return hook(err, errlen, [&] {
+        fem::DArray<double> a((size_t)n * n);
+        a.upload(A, (size_t)n * n);
+        fem::DenseCoarse dc;
+        dc.factor(a.data(), n, true, nullptr);
+        std::vector<int64_t> id(n);
+        for (int i = 0; i < n; ++i) id[i] = i;
+        dc.map().resize(n);
+        dc.map().upload(id.data(), n);
+        fem::DArray<float> bd(n), xd(n);
+        bd.upload(b, n);
+        dc.solve<float>(bd.data(), xd.data(), nullptr);
+        xd.download(x, n);
+        FEM_HIP(hipDeviceSynchronize());
+    });
+}
+
 extern "C" int fem_test_dense_inverse(int n, const double* A, double* Ainv, char* err, size_t errlen) {
     return hook(err, errlen, [&] {
         fem::DArray<double> a((size_t)n * n);

@@ -23,6 +23,10 @@ int fem_test_gemm_nt(int m, int n, int k, const double* A, const double* B, doub
  * blocked LDL^T and triangular inverse, in FP64. */
 int fem_test_dense_inverse(int n, const double* A, double* Ainv, char* err, size_t errlen);
 
+/* x = A^-1 b by the coarse solve of an FP32 V-cycle: the FP64 factor and inverse, the inverse
+ * kept in FP32, the product summed in FP32. */
+int fem_test_dense_solve32(int n, const double* A, const float* b, float* x, char* err, size_t errlen);
+
 /* A plain copy of `bytes` (a multiple of 16) between two device buffers on the null stream, then
  * a wait: the calibration of HBM byte counters (FETCH_SIZE / WRITE_SIZE) against a kernel whose
  * traffic is known. */

@@ -1,5 +1,5 @@
 """GPU checks of the dense coarse level's building blocks against numpy: the FP64 matrix-core
-product (its lane layout) and the LDL^T inverse."""
+product (its lane layout), the LDL^T inverse, and the FP32 V-cycle's coarse solve."""
 import ctypes as C
 
 import numpy as np
@@ -15,6 +15,9 @@ def _hooks(fem):
     L.fem_test_gemm_nt.argtypes = [C.c_int, C.c_int, C.c_int, F64P, F64P, F64P, C.c_char_p, C.c_size_t]
     L.fem_test_dense_inverse.restype = C.c_int
     L.fem_test_dense_inverse.argtypes = [C.c_int, F64P, F64P, C.c_char_p, C.c_size_t]
+    F32P = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    L.fem_test_dense_solve32.restype = C.c_int
+    L.fem_test_dense_solve32.argtypes = [C.c_int, F64P, F32P, F32P, C.c_char_p, C.c_size_t]
     return L
 
 
@@ -57,3 +60,28 @@ def test_dense_inverse_refuses_a_singular_matrix(fem):
     err = C.create_string_buffer(512)
     assert L.fem_test_dense_inverse(n, A.ravel(), out.ravel(), err, 512) != 0
     assert b"not positive definite" in err.value
+
+
+@pytest.mark.parametrize("n", [5, 100, 517, 1500])
+def test_the_fp32_coarse_solve_matches_numpy(fem, n):
+    """A graded 1D Laplacian plus a random SPD part (entries of the inverse over several decades):
+    the solve from the FP32 inverse is within the FP32 rounding of the entries and of the sums."""
+    L = _hooks(fem)
+    rng = np.random.default_rng(n)
+    k = np.exp(rng.standard_normal(n + 1))
+    A = np.zeros((n, n))
+    for i in range(n):
+        A[i, i] = k[i] + k[i + 1]
+        if i + 1 < n:
+            A[i, i + 1] = A[i + 1, i] = -k[i + 1]
+    M = rng.standard_normal((n, 8))
+    A += 0.01 * M @ M.T
+    b = rng.standard_normal(n).astype(np.float32)
+    x = np.zeros(n, np.float32)
+    err = C.create_string_buffer(512)
+    assert L.fem_test_dense_solve32(n, np.ascontiguousarray(A).ravel(), b, x, err, 512) == 0, err.value
+    inv = np.linalg.inv(A)
+    ref = inv @ b.astype(np.float64)
+    # the FP32 rounding of each entry and FP32 sums of up to n / 64 + 6 terms per lane and wave
+    bound = (n / 64 + 8) * 2.0 ** -24 * (np.abs(inv) @ np.abs(b))
+    assert np.all(np.abs(x - ref) <= bound)
