@@ -414,9 +414,9 @@ class CsrSolverImpl {
             L.invd.upload(inv.data(), L.n, s_);
             FEM_HIP(hipStreamSynchronize(s_));
             L.lmax = estimate_lmax(L);
-            // automatic: 2 on every level. A finest step moves the whole matrix (8 B per entry in
-            // FP32), so the matrix-free path's 3-then-1 choice costs more than the iteration it
-            // saves (heat 128^3: 168 against 197 MDOF/s).
+            // automatic: 2 on every level. A finest step moves the whole matrix (6-8 B per FP32
+            // entry), so the matrix-free path's 3-then-1 choice costs more than the iteration it
+            // saves (heat 128^3 with 32-bit columns: 168 against 197 MDOF/s).
             chebyshev(L, opt_.degree > 0 ? opt_.degree : 2);
         }
         factor_dense(host_.a.back(), H.coarse, sizeof(T) == 4);
