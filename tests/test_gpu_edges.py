@@ -101,3 +101,31 @@ def test_the_assembled_boundary_keeps_the_same_edge_semantics(fem, oracle):
     s.with_max_iter(0)
     with pytest.raises(fem.FemError, match=r"after 0 iterations$"):
         s.solve(b)
+
+
+def test_the_automatic_smoother_degree_is_two_for_elasticity_and_three_then_one_for_heat(fem, oracle):
+    """cheb_degree 0 (the default): an elastic box smooths as with degree 2 on every level (the same
+    solution bit for bit); a heat box smooths with degree 3 on the finest level and 1 below, which
+    is neither 2 nor 3 everywhere. Both solve to the oracle's direct solution."""
+    n = [16, 8, 8]
+    nt = 6 * int(np.prod(n))
+    rng = np.random.default_rng(21)
+    e, nu = rng.uniform(1.0, 2.0, nt), rng.uniform(0.2, 0.35, nt)
+    ebox = fem.Box("elastic_tet4", n, [2.0, 1.0, 1.0], e, nu, fixed={"x0": True}, body=(0, 0, -1))
+    auto = fem.Solver(ebox, method="multigrid", levels=3, tol=1e-10)
+    two = fem.Solver(ebox, method="multigrid", levels=3, tol=1e-10, cheb_degree=2)
+    b = auto.rhs()
+    xa, sa = auto.solve(b)
+    x2, s2 = two.solve(b)
+    assert sa.iterations == s2.iterations and np.array_equal(xa, x2)
+
+    box, k = heat_box(fem, [24, 24, 24], seed=22)
+    pr = oracle.Problem.heat([24, 24, 24], [1, 1, 1], k, fixed={"x0": 0.0, "x1": 1.0})
+    xd, _, _ = oracle.Solver(pr, "direct").solve(pr.rhs())
+    xs = {}
+    for deg in (0, 2, 3):
+        s = fem.Solver(box, method="multigrid", levels=3, tol=1e-12, cheb_degree=deg)
+        xs[deg], st = s.solve(pr.rhs())
+        print(deg, st)
+        assert rel(xs[deg], xd) < 1e-8
+    assert not np.array_equal(xs[0], xs[2]) and not np.array_equal(xs[0], xs[3])
