@@ -5,9 +5,10 @@ FP64 MG-PCG, with p50/p99 latency.
 Workload (one step = one solve, right side resident in HBM to solution resident): 3D steady heat
 conduction, hex8 on a structured 128^3 unit cube (2,113,407 free unknowns), per-cell conductivity
 exp(N(0, 1)) from numpy's default_rng(1234), T = 0 on x = 0 and T = 1 on x = 1, insulated
-elsewhere; 4-level geometric multigrid with Chebyshev-Jacobi smoothing (degree 3 on the finest
-level, 1 below) and a dense FP64 LDL^T coarse level. `value` is the mixed solve (FP32 V-cycle inside
-the FP64 PCG); the plain FP64 MG-PCG is run and reported beside it. Usage:
+elsewhere; 4-level geometric multigrid with Chebyshev(3)-Jacobi smoothing and a dense FP64 LDL^T
+coarse level. `value` is the mixed solve (FP32 V-cycle inside the FP64 PCG); the plain FP64 MG-PCG
+is run and reported beside it, and so is the mixed solve with the solver's automatic smoother degree
+(3 on the finest level, 1 below). Usage:
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--no-cpu]
 """
@@ -269,12 +270,13 @@ def cpu_baseline(n_cpu):
     }
 
 
-def run_workload(F, hip, box, levels, args, label):
-    """Mixed and FP64 MG-PCG on one box: timed solves, a profiled eager pass, the error against an
-    FP64 solve to 1e-13. Returns the result dict."""
+def run_workload(F, hip, box, levels, args, label, degree, auto_leg=False):
+    """Mixed and FP64 MG-PCG on one box with Chebyshev degree `degree` (0: the solver's automatic
+    choice): timed solves, the kernel timings, the error against an FP64 solve to 1e-13; with
+    auto_leg, also the mixed solve with the automatic degree. Returns the result dict."""
     log(f"setup: {label}, mixed and FP64 hierarchies ({levels or 'automatic'} levels)")
     t0 = time.perf_counter()
-    opts = dict(method="multigrid", tol=1e-8, levels=levels, cheb_degree=args.degree,
+    opts = dict(method="multigrid", tol=1e-8, levels=levels, cheb_degree=degree,
                 coarse_galerkin=args.coarse == "galerkin", max_iter=2000)
     mixed = F.Solver(box, precision="fp32", **opts)
     t1 = time.perf_counter()
@@ -313,6 +315,17 @@ def run_workload(F, hip, box, levels, args, label):
     tkh = bf16.time_kernels(100)
     prof_h = dict(smoother_s=tkh.smoother_seconds / tkh.smoother_launches, smoother_bytes=tkh.smoother_bytes)
     log(f"  bf16 smoother {prof_h['smoother_s'] * 1e6:.2f} us per launch ({prof_h['smoother_bytes']} bytes)")
+    auto = None
+    if auto_leg:
+        log("mixed with the automatic smoother degree (3 on the finest level, 1 below): timed solves")
+        ta = time.perf_counter()
+        mixed_a = F.Solver(box, precision="fp32", **dict(opts, cheb_degree=0))
+        ta = time.perf_counter() - ta
+        lat_a, st_a, _, _ = timed_solves(F, hip, mixed_a, d_b, d_x, x0, args.steps, args.warmup, profile=False)
+        x_auto = np.zeros(nfree)
+        hip.d2h(x_auto, d_x)
+        del mixed_a
+        auto = (lat_a, st_a, x_auto, ta)
     log("FP64 MG-PCG: timed solves")
     lat_d, st_d, _, _ = timed_solves(F, hip, fp64, d_b, d_x, x0, args.steps, args.warmup, profile=False)
     x_fp64 = np.zeros(nfree)
@@ -341,6 +354,10 @@ def run_workload(F, hip, box, levels, args, label):
            "mixed_bf16": leg(lat_h, st_h, err_bf16, t3 - t2),
            "fp64": leg(lat_d, st_d, err_fp64, t2 - t1),
            "fp64_reference": f"FP64 MG-PCG to 1e-13 relative residual ({st_ref.iterations} iterations)"}
+    if auto is not None:
+        lat_a, st_a, x_auto, ta = auto
+        res["mixed_automatic_degree"] = leg(lat_a, st_a, float(np.linalg.norm(x_auto - x_ref) / np.linalg.norm(x_ref)), ta)
+        res["mixed_automatic_degree"]["smoother"] = cheb_desc(0, False)
     res["speedup_mixed_vs_fp64"] = round(res["mixed"]["mdof_s"] / res["fp64"]["mdof_s"], 3)
     res["speedup_bf16_vs_fp64"] = round(res["mixed_bf16"]["mdof_s"] / res["fp64"]["mdof_s"], 3)
     res["mixed_bf16"]["finest_smoother"] = {"bytes_per_launch": int(prof_h["smoother_bytes"]),
@@ -350,7 +367,7 @@ def run_workload(F, hip, box, levels, args, label):
     return res, prof, lat_m
 
 
-def assembled_leg(F, hip, solver, box, x_ref, args):
+def assembled_leg(F, hip, solver, box, x_ref, args, degree):
     """The assembled boundary on the same heat box: the model's reduced CSR (fem_solver_matrix, as
     the reference's caller assembles it), its grid unknowns, and fem_solver_new_csr with the
     reference's grid hierarchy (coarsen and Galerkin products down to 3000 unknowns), mixed
@@ -363,7 +380,7 @@ def assembled_leg(F, hip, solver, box, x_ref, args):
     pts = F.Model(box).point_grid()[solver.free_list()]
     t1 = time.perf_counter()
     csr = F.CsrSolver(ptr, idx, val, "grid", F.Grid(planes, pts, np.zeros(solver.n, np.uint8)), method="multigrid",
-                      precision="fp32", tol=1e-8, max_iter=2000, cheb_degree=args.degree)
+                      precision="fp32", tol=1e-8, max_iter=2000, cheb_degree=degree)
     t2 = time.perf_counter()
     lv = []
     for l in range(32):
@@ -389,7 +406,7 @@ def assembled_leg(F, hip, solver, box, x_ref, args):
             "setup_s": round(t2 - t1, 3),
             "workload": "the same heat box through fem_solver_new_csr: CSR SpMV operators on every level (FP32 "
                         "V-cycle, FP64 PCG), the reference's coarsen and Galerkin levels down to 3000 unknowns, "
-                        f"{cheb_desc(args.degree, True)}, dense LDLT coarse, tol 1e-8"}
+                        f"{cheb_desc(degree, True)}, dense LDLT coarse, tol 1e-8"}
 
 
 def main():
@@ -401,7 +418,9 @@ def main():
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--cpu-n", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--degree", type=int, default=0, help="Chebyshev degree on every level (0: automatic)")
+    ap.add_argument("--degree", type=int, default=None,
+                    help="Chebyshev degree on every level (0: automatic); default: 3 for the heat box as its "
+                         "configuration names, automatic for the elasticity boxes")
     ap.add_argument("--coarse", choices=["galerkin", "mean"], default="galerkin")
     ap.add_argument("--elastic-n", type=int, default=60,
                     help="cells along x of the tet10 elasticity box reported beside the headline (0: skip)")
@@ -432,14 +451,18 @@ def main():
         hip.free(d_c0)
         hip.free(d_c1)
 
+    # the heat configuration names Chebyshev(3)-Jacobi; the elasticity ones name no degree
+    hdeg = 3 if args.degree is None else args.degree
+    vdeg = 0 if args.degree is None else args.degree
     n = args.n
     box, _ = heat_box(F, n)
-    heat, prof, lat_m = run_workload(F, hip, box, args.levels, args, f"heat hex8 {n}^3")
+    heat, prof, lat_m = run_workload(F, hip, box, args.levels, args, f"heat hex8 {n}^3", hdeg,
+                                     auto_leg=args.degree is None)
     value = heat["mixed"]["mdof_s"]
     x_ref = heat.pop("_x_ref")
     if not args.no_assembled:
         heat["assembled_csr"] = assembled_leg(F, hip, F.Solver(box, method="multigrid", levels=args.levels), box,
-                                              x_ref, args)
+                                              x_ref, args, vdeg)
 
     roofline = None
     traffic = measured_traffic(n)
@@ -464,11 +487,11 @@ def main():
     if args.elastic_n > 0:
         en = args.elastic_n
         ebox = elastic_box(F, en)
-        elastic, eprof, _ = run_workload(F, hip, ebox, 0, args, f"elasticity tet10 {en}x{en // 2}x{en // 2}")
+        elastic, eprof, _ = run_workload(F, hip, ebox, 0, args, f"elasticity tet10 {en}x{en // 2}x{en // 2}", vdeg)
         elastic.pop("_x_ref")
         elastic["workload"] = (f"linear elasticity, tet10 on {en}x{en // 2}x{en // 2} unit cells (6 Kuhn tets per cell), "
                                "E=1, nu=0.3, clamped at x=0, body force (0,0,-1); p-MG tet10->tet4 then geometric tet4 "
-                               f"levels, dense FP64 LDLT coarse, {cheb_desc(args.degree, True)}, tol 1e-8")
+                               f"levels, dense FP64 LDLT coarse, {cheb_desc(vdeg, True)}, tol 1e-8")
         if eprof:
             elastic["finest_smoother"] = {"kernel": "k_cheb<VectorOp<Tet10Op<float>>> (+ its element pass)",
                                           "bytes_per_launch": int(eprof["smoother_bytes"]),
@@ -479,13 +502,13 @@ def main():
     if args.inclusions_n > 0:
         ni = args.inclusions_n
         ibox, inside = F.inclusion_box(ni, args.inclusions, 7)
-        inclusions, _, _ = run_workload(F, hip, ibox, 0, args, f"inclusions tet10 {ni}x{ni // 2}x{ni // 2}")
+        inclusions, _, _ = run_workload(F, hip, ibox, 0, args, f"inclusions tet10 {ni}x{ni // 2}x{ni // 2}", vdeg)
         inclusions.pop("_x_ref")
         inclusions["workload"] = (f"nearly incompressible heterogeneous elasticity, tet10 on {ni}x{ni // 2}x{ni // 2} unit "
                                   f"cells, nu=0.49 matrix (E=1) with {args.inclusions} seeded spheres (radius 0.02-0.05 "
                                   "of the box length, E=1e3, nu=0.3; a tet is inside when its centroid is), clamped at "
                                   "x=0, traction (1,0,0) on x=1; p-MG tet10->tet4 then geometric tet4 levels, dense FP64 "
-                                  f"LDLT coarse, {cheb_desc(args.degree, True)}, tol 1e-8 with refinement against the FP64 true "
+                                  f"LDLT coarse, {cheb_desc(vdeg, True)}, tol 1e-8 with refinement against the FP64 true "
                                   "residual")
         inclusions["stiff_fraction"] = round(float(inside.mean()), 4)
 
@@ -498,7 +521,7 @@ def main():
         log("  " + cpu["sample"])
 
     config = {"workload": (f"heat hex8 {n}^3 unit cube, k = exp(N(0,1)) per cell (seed 1234), T=0 at x=0 and T=1 at "
-                           f"x=1, {args.levels}-level GMG ({args.coarse} coarse operators), {cheb_desc(args.degree, False)}, "
+                           f"x=1, {args.levels}-level GMG ({args.coarse} coarse operators), {cheb_desc(hdeg, False)}, "
                            "dense FP64 LDLT coarse, tol 1e-8")}
     config.update(heat)
     if elastic:
