@@ -141,7 +141,7 @@ EXPORTS = (
     "fem_model_tet_cells",
     "fem_solver_set_profile", "fem_solver_profile", "fem_solver_time_kernels",
     "fem_server_new", "fem_server_submit", "fem_server_wait", "fem_server_cancel", "fem_server_free",
-    "fem_solver_new_csr", "fem_solver_matrix",
+    "fem_solver_new_csr", "fem_solver_matrix", "fem_dense_solve_device",
 )
 
 _lib = None
@@ -202,6 +202,7 @@ def lib() -> C.CDLL:
         "fem_server_cancel": (C.c_int, [P, C.c_uint64]),
         "fem_server_free": (None, [P]),
         "fem_solver_matrix": (C.c_int, [P, C.POINTER(C.c_uint64), P, P, P, ERR, SZ]),
+        "fem_dense_solve_device": (C.c_int, [C.c_int, P, P, P, ERR, SZ]),
         "fem_solver_new_csr": (C.c_int, [C.POINTER(_Csr), C.POINTER(_Unknowns), C.POINTER(_Options), C.POINTER(P),
                                          ERR, SZ]),
         "fem_test_csr_levels_new": (C.c_int, [C.POINTER(_Csr), C.POINTER(_GridLevel), C.c_uint64, C.c_int,
@@ -608,6 +609,64 @@ def _stats(st: _Stats) -> Stats:
                  float(st.setup_seconds), int(st.levels), int(st.coarse_unknowns), int(st.restarts),
                  ("direct", "multigrid")[st.method] if st.method in (0, 1) else "?",
                  {v: k for k, v in PRECISIONS.items()}.get(st.precision, "?"))
+
+
+class DeviceArray:
+    """A float64 array in device memory (hipMalloc through the HIP runtime), for the entry points
+    that take device pointers."""
+
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        if cls._hip is None:
+            h = C.CDLL("libamdhip64.so") if not os.path.exists("/opt/rocm/lib/libamdhip64.so") \
+                else C.CDLL("/opt/rocm/lib/libamdhip64.so")
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipFree.argtypes = [C.c_void_p]
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, host: np.ndarray):
+        host = np.ascontiguousarray(host, dtype=np.float64)
+        self.shape, self.nbytes = host.shape, host.nbytes
+        p = C.c_void_p()
+        if self.hip().hipMalloc(C.byref(p), max(self.nbytes, 8)):
+            raise FemError("hipMalloc failed")
+        self.ptr = p.value
+        if self.nbytes and self.hip().hipMemcpy(self.ptr, host.ctypes.data, self.nbytes, 1):
+            raise FemError("hipMemcpy to the device failed")
+
+    def get(self) -> np.ndarray:
+        out = np.empty(self.shape)
+        if self.nbytes and self.hip().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, 2):
+            raise FemError("hipMemcpy from the device failed")
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.hip().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+
+def dense_solve(a, b, entry: str = "fem_dense_solve_device") -> np.ndarray:
+    """x = A^-1 b for a dense SPD matrix through the device's dense LDL^T (fem_dense_solve_device:
+    Cholesky::factor and Cholesky::solve, crates/fem/src/cholesky.rs:44-160, for a dense matrix).
+    The arrays are copied to device memory for the call; `entry` names the C entry point."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    n = b.shape[0]
+    if a.shape != (n, n):
+        raise FemError("the matrix must be n x n for a right side of n values")
+    da, db, dx = DeviceArray(a), DeviceArray(b), DeviceArray(np.zeros(n))
+    err = _err()
+    if getattr(lib(), entry)(n, da.ptr, db.ptr, dx.ptr, err, len(err)) != 0:
+        raise FemError(err.value.decode())
+    return dx.get()
 
 
 class Server:

@@ -107,3 +107,28 @@ class DArray {
 };
 
 }  // namespace fem
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+
+namespace fem {
+
+// Rust's `{:e}` of an f64 (the reference's error texts, e.g. crates/fem/src/cholesky.rs:94-98):
+// the shortest digits that read back to the same value, in scientific notation with the exponent
+// written without sign padding or leading zeros ("1e0", "2.5e-3", "-4.25e12"; "NaN", "inf").
+inline std::string rust_e(double v) {
+    if (std::isnan(v)) return "NaN";
+    if (std::isinf(v)) return v > 0 ? "inf" : "-inf";
+    char num[64];
+    for (int prec = 0; prec <= 17; ++prec) {
+        std::snprintf(num, sizeof num, "%.*e", prec, v);
+        if (std::strtod(num, nullptr) == v) break;
+    }
+    std::string t(num);
+    const size_t e = t.find('e');
+    if (e == std::string::npos) return t;
+    return t.substr(0, e) + "e" + std::to_string(std::atoi(t.c_str() + e + 1));
+}
+
+}  // namespace fem

@@ -251,6 +251,36 @@ __global__ void __launch_bounds__(256) k_coarse_gemv(const T* __restrict__ inv, 
     if (lane == 0) x[map[row]] = s;
 }
 
+// x[i] (= or +=) sum_j inv[i][j] b[j] for i < n: FP64, one wave per row, 4 rows per block. The
+// rows of inv have leading dimension ld (a multiple of 64); b has n entries.
+template <bool ACC>
+__global__ void __launch_bounds__(256) k_dense_gemv(const double* __restrict__ inv, int n, int ld,
+                                                    const double* __restrict__ b, double* __restrict__ x) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    const double* a = inv + (size_t)row * ld;
+    double s = 0.0;
+    for (int j = lane; j < n; j += 64) s += a[j] * b[j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) x[row] = ACC ? x[row] + s : s;
+}
+
+// r = b - A x for the n x n row-major matrix A (leading dimension n): one wave per row.
+__global__ void __launch_bounds__(256) k_dense_residual(const double* __restrict__ A, int n, const double* __restrict__ b,
+                                                        const double* __restrict__ x, double* __restrict__ r) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    const double* a = A + (size_t)row * n;
+    double s = 0.0;
+    for (int j = lane; j < n; j += 64) s += a[j] * x[j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) r[row] = b[row] - s;
+}
+
 void check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw Error(std::string("HIP error ") + hipGetErrorString(e) + " in " + what);
 }
@@ -270,85 +300,120 @@ void dense_gemm_nt(int m, int n, int k, double alpha, const double* A, int lda, 
     launch_gemm(g, dim3(m / NB, n / NB), s);
 }
 
-void DenseCoarse::factor(double* a, int n, bool fp32, hipStream_t s) {
-    const auto t0 = std::chrono::steady_clock::now();
-    n_ = n;
-    fp32_ = fp32;
-    const int nb = (n + NB - 1) / NB;
-    np_ = nb * NB;
-    const int np = np_;
+void DenseWork::carve(void* base, int n_) {
+    n = n_;
+    np = (n + NB - 1) / NB * NB;
+    nb = np / NB;
+    char* p = static_cast<char*>(base);
+    auto take = [&](size_t b) {
+        char* q = p;
+        p += align(b);
+        return q;
+    };
+    A = (double*)take((size_t)np * np * 8);
+    U = (double*)take((size_t)np * np * 8);
+    inv = (double*)take((size_t)np * np * 8);
+    P = (double*)take((size_t)np * NB * 8);
+    linv = (double*)take((size_t)nb * NB * NB * 8);
+    bk = (double*)take((size_t)NB * NB * 8);
+    d = (double*)take((size_t)np * 8);
+    dg = (double*)take((size_t)np * 8);
+    status = (int*)take(sizeof(int));
+}
+
+void dense_factor_inverse(const double* a, int n, const DenseWork& w, hipStream_t s) {
+    if (n <= 0 || w.n != n) throw Error("internal: the dense workspace is not carved for this size");
+    const int nb = w.nb, np = w.np;
     const size_t nn = (size_t)np * np;
-    DArray<double> A(nn), U(nn), P((size_t)np * NB), linv((size_t)nb * NB * NB), bk((size_t)NB * NB), d(np), dg(np);
-    DArray<int> status(1);
-    FEM_HIP(hipMemsetAsync(status.data(), 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_zero<int>, dim3(1), dim3(256), 0, s, w.status, 1);
     const unsigned blocks = (unsigned)((nn + 255) / 256);
-    hipLaunchKernelGGL(k_pad, dim3(blocks), dim3(256), 0, s, a, n, A.data(), np);
-    hipLaunchKernelGGL(k_diag_of, dim3((np + 255) / 256), dim3(256), 0, s, A.data(), np, dg.data());
+    hipLaunchKernelGGL(k_pad, dim3(blocks), dim3(256), 0, s, a, n, w.A, np);
+    hipLaunchKernelGGL(k_diag_of, dim3((np + 255) / 256), dim3(256), 0, s, (const double*)w.A, np, w.dg);
     for (int kb = 0; kb < nb; ++kb) {
-        hipLaunchKernelGGL(k_ldlt_diag, dim3(1), dim3(256), 0, s, A.data(), np, kb, dg.data(), d.data(), linv.data(),
-                           bk.data(), status.data());
+        hipLaunchKernelGGL(k_ldlt_diag, dim3(1), dim3(256), 0, s, w.A, np, kb, (const double*)w.dg, w.d, w.linv, w.bk,
+                           w.status);
         const int below = nb - kb - 1;
         if (below == 0) break;
         // L_ik = A_ik (D^-1 L^-1)^T ; W_ik = L_ik D_k
-        GemmArgs pg{A.data() + (size_t)kb * NB, np, bk.data(), NB, A.data() + (size_t)kb * NB, np, NB, 0.0, 1.0, 0,
-                    kb + 1, 0, 0, P.data(), d.data() + (size_t)kb * NB};
+        GemmArgs pg{w.A + (size_t)kb * NB, np, w.bk, NB, w.A + (size_t)kb * NB, np, NB, 0.0, 1.0, 0,
+                    kb + 1, 0, 0, w.P, w.d + (size_t)kb * NB};
         launch_gemm(pg, dim3(below, 1), s);
         // A_ij -= W_ik L_jk^T for kb < j <= i
-        GemmArgs ug{P.data(), NB, A.data() + (size_t)kb * NB, np, A.data(), np, NB, 1.0, -1.0, 1, kb + 1, kb + 1, below,
+        GemmArgs ug{w.P, NB, w.A + (size_t)kb * NB, np, w.A, np, NB, 1.0, -1.0, 1, kb + 1, kb + 1, below,
                     nullptr, nullptr};
         launch_gemm(ug, dim3(below * (below + 1) / 2, 1), s);
     }
     int bad = 0;
-    FEM_HIP(hipMemcpyAsync(&bad, status.data(), sizeof(int), hipMemcpyDeviceToHost, s));
+    FEM_HIP(hipMemcpyAsync(&bad, w.status, sizeof(int), hipMemcpyDeviceToHost, s));
     FEM_HIP(hipStreamSynchronize(s));
     if (bad) {
         const int col = bad - 1;
         double diag = 0.0;
-        FEM_HIP(hipMemcpy(&diag, dg.data() + col, sizeof(double), hipMemcpyDeviceToHost));
-        char buf[256];
-        std::snprintf(buf, sizeof buf,
-                      "the matrix is not positive definite: the pivot of column %d is 0 or less to the precision of "
-                      "the arithmetic (its diagonal is %g)",
-                      col, diag);
-        throw Error(buf);
+        FEM_HIP(hipMemcpyAsync(&diag, w.dg + col, sizeof(double), hipMemcpyDeviceToHost, s));
+        FEM_HIP(hipStreamSynchronize(s));
+        // cholesky.rs:94-98, the diagonal in Rust's {:e}
+        throw Error("the matrix is not positive definite: the pivot of column " + std::to_string(col) +
+                    " is 0 or less to the precision of the arithmetic (its diagonal is " + rust_e(diag) + ")");
     }
     // U = L^-T: solve on the identity, working on the transpose (all products are A B^T).
-    hipLaunchKernelGGL(k_identity, dim3(blocks), dim3(256), 0, s, U.data(), np);
+    hipLaunchKernelGGL(k_identity, dim3(blocks), dim3(256), 0, s, w.U, np);
     for (int k = 0; k < nb; ++k) {
         // U_jk = U_jk Linv_kk^T for j <= k
-        GemmArgs s1{U.data() + (size_t)k * NB, np, linv.data() + (size_t)k * NB * NB, NB, U.data() + (size_t)k * NB, np,
+        GemmArgs s1{w.U + (size_t)k * NB, np, w.linv + (size_t)k * NB * NB, NB, w.U + (size_t)k * NB, np,
                     NB, 0.0, 1.0, 0, 0, 0, 0, nullptr, nullptr};
         launch_gemm(s1, dim3(k + 1, 1), s);
         // U_ji -= U_jk L_ik^T for j <= k < i
         const int right = nb - k - 1;
         if (right == 0) continue;
-        GemmArgs s2{U.data() + (size_t)k * NB, np, A.data() + (size_t)k * NB, np, U.data(), np, NB, 1.0, -1.0, 0, 0,
+        GemmArgs s2{w.U + (size_t)k * NB, np, w.A + (size_t)k * NB, np, w.U, np, NB, 1.0, -1.0, 0, 0,
                     k + 1, 0, nullptr, nullptr};
         launch_gemm(s2, dim3(k + 1, right), s);
     }
-    hipLaunchKernelGGL(k_scale_cols, dim3(blocks), dim3(256), 0, s, U.data(), np, d.data());
+    hipLaunchKernelGGL(k_scale_cols, dim3(blocks), dim3(256), 0, s, w.U, np, (const double*)w.d);
     // A^-1 = V V^T with V upper triangular: K from max(i, j) of the tile.
-    DArray<double> inv(nn);
-    GemmArgs fg{U.data(), np, U.data(), np, inv.data(), np, np, 0.0, 1.0, 2, 0, 0, 0, nullptr, nullptr};
+    GemmArgs fg{w.U, np, w.U, np, w.inv, np, np, 0.0, 1.0, 2, 0, 0, 0, nullptr, nullptr};
     launch_gemm(fg, dim3(nb, nb), s);
-    ld_ = np;  // rows zero padded to a multiple of 64
+    FEM_HIP(hipGetLastError());
+}
+
+void dense_solve_refined(const double* a, int n, const DenseWork& w, const double* b, double* x, double* r,
+                         hipStream_t s) {
+    const unsigned rows = (unsigned)((n + 3) / 4);
+    // x = inv b (rows of inv are padded to np with zeros past n: only the first n columns are read)
+    hipLaunchKernelGGL((k_dense_gemv<false>), dim3(rows), dim3(256), 0, s, (const double*)w.inv, n, w.np, b, x);
+    // r = b - A x on the original matrix, e = inv r, x += e
+    hipLaunchKernelGGL(k_dense_residual, dim3(rows), dim3(256), 0, s, a, n, b, (const double*)x, r);
+    hipLaunchKernelGGL((k_dense_gemv<true>), dim3(rows), dim3(256), 0, s, (const double*)w.inv, n, w.np, (const double*)r, x);
+    FEM_HIP(hipGetLastError());
+}
+
+void DenseCoarse::factor(double* a, int n, bool fp32, hipStream_t s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    n_ = n;
+    fp32_ = fp32;
+    Buffer work(DenseWork::bytes(n));
+    DenseWork w;
+    w.carve(work.get(), n);
+    dense_factor_inverse(a, n, w, s);
+    np_ = w.np;
+    ld_ = w.np;  // rows zero padded to a multiple of 64
     const unsigned eb = (unsigned)(((size_t)n * ld_ + 255) / 256);
     const unsigned zb = (unsigned)((ld_ + 255) / 256);
     if (fp32) {
         inv32_.resize((size_t)n * ld_);
         bc32_.resize(ld_);
-        hipLaunchKernelGGL(k_extract<float>, dim3(eb), dim3(256), 0, s, inv.data(), np, n, ld_, inv32_.data());
+        hipLaunchKernelGGL(k_extract<float>, dim3(eb), dim3(256), 0, s, (const double*)w.inv, w.np, n, ld_, inv32_.data());
         hipLaunchKernelGGL(k_zero<float>, dim3(zb), dim3(256), 0, s, bc32_.data(), ld_);
     } else {
         inv64_.resize((size_t)n * ld_);
         bc64_.resize(ld_);
-        hipLaunchKernelGGL(k_extract<double>, dim3(eb), dim3(256), 0, s, inv.data(), np, n, ld_, inv64_.data());
+        hipLaunchKernelGGL(k_extract<double>, dim3(eb), dim3(256), 0, s, (const double*)w.inv, w.np, n, ld_, inv64_.data());
         hipLaunchKernelGGL(k_zero<double>, dim3(zb), dim3(256), 0, s, bc64_.data(), ld_);
     }
     FEM_HIP(hipGetLastError());
-    FEM_HIP(hipStreamSynchronize(s));
+    FEM_HIP(hipStreamSynchronize(s));  // the workspace is freed at the end of this scope
     factor_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    const double dn = (double)np;
+    const double dn = (double)np_;
     factor_flops_ = dn * dn * dn / 3.0 + dn * dn * dn / 3.0 + dn * dn * dn / 3.0;
 }
 

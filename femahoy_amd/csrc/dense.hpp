@@ -12,6 +12,41 @@
 
 namespace fem {
 
+// The device workspace of one dense factor of n unknowns (np = n rounded up to 64, nb = np / 64
+// blocks), carved from one contiguous region of bytes(n) bytes at 256-byte aligned offsets:
+// A (np x np, the padded copy that the factor overwrites with L and D), U (np x np, L^-T),
+// inv (np x np, the FP64 inverse), P (np x 64, the scaled block column of a step), linv
+// (nb blocks of 64 x 64, the inverse of each diagonal block's L), bk (64 x 64), d and dg (np:
+// the pivots and the original diagonal) and status (the first failed pivot's column + 1, or 0).
+struct DenseWork {
+    int n = 0, np = 0, nb = 0;
+    double *A = nullptr, *U = nullptr, *inv = nullptr, *P = nullptr, *linv = nullptr, *bk = nullptr, *d = nullptr,
+           *dg = nullptr;
+    int* status = nullptr;
+
+    static constexpr size_t align(size_t b) { return (b + 255) / 256 * 256; }
+    static constexpr size_t bytes(int n) {
+        const size_t np = (size_t)(n + 63) / 64 * 64, nb = np / 64;
+        return 3 * align(np * np * 8) + align(np * 64 * 8) + align(nb * 64 * 64 * 8) + align(64 * 64 * 8) +
+               2 * align(np * 8) + align(sizeof(int));
+    }
+    // Sets the pointers for n unknowns from `base` (bytes(n) bytes of device memory, 256-byte aligned).
+    void carve(void* base, int n_);
+};
+
+// Factors the n x n symmetric positive definite matrix `a` (FP64, row-major, leading dimension n,
+// device memory, not changed) into the workspace and forms the FP64 inverse in w.inv (np x np,
+// the padding rows and columns those of the identity). Waits for the stream (the pivot test) and
+// throws fem::Error with the reference's "not positive definite" text when a pivot fails.
+// Allocates nothing.
+void dense_factor_inverse(const double* a, int n, const DenseWork& w, hipStream_t s);
+
+// x = A^-1 b with one step of refinement on the FP64 residual of `a`: x = inv b, r = b - A x,
+// x += inv r. inv is w.inv (FP64, leading dimension w.np); r is an n-vector workspace. Allocates
+// nothing.
+void dense_solve_refined(const double* a, int n, const DenseWork& w, const double* b, double* x, double* r,
+                         hipStream_t s);
+
 class DenseCoarse {
    public:
     // Factors the n x n symmetric matrix `a` (FP64, row-major, device memory, overwritten) and

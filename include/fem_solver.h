@@ -191,6 +191,19 @@ typedef struct fem_unknowns {
 int fem_solver_new_csr(const fem_csr* a, const fem_unknowns* u, const fem_options* opt, fem_solver** out, char* err,
                        size_t errlen);
 
+/* ---- a dense direct solve on device memory ---- */
+
+/* x = A^-1 b for a dense symmetric positive definite n x n matrix A: the device's dense FP64
+ * LDL^T (blocked, on the FP64 matrix cores), its explicit inverse, and one step of refinement on
+ * the FP64 residual, the same factor and solve as the direct method of fem_solver_new_box and
+ * fem_solver_new_csr. It replaces Cholesky::factor and Cholesky::solve
+ * (crates/fem/src/cholesky.rs:44-160) for a matrix held dense. d_a (row-major, n x n, FP64),
+ * d_b and d_x (n) are device pointers; d_a and d_b are not changed. The device workspace (three
+ * n^2 arrays) is allocated for the call and freed before it returns; the call waits for the device.
+ * A failed pivot gives the reference's "the matrix is not positive definite: ..." text
+ * (cholesky.rs:94-98). */
+int fem_dense_solve_device(int n, const double* d_a, const double* d_b, double* d_x, char* err, size_t errlen);
+
 /* ---- the model's integer structures and right side, for the caller and for parity checks ---- */
 
 /* The number of free unknowns (the length of b and x), and of all unknowns. */

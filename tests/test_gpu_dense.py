@@ -85,3 +85,50 @@ def test_the_fp32_coarse_solve_matches_numpy(fem, n):
     # the FP32 rounding of each entry and FP32 sums of up to n / 64 + 6 terms per lane and wave
     bound = (n / 64 + 8) * 2.0 ** -24 * (np.abs(inv) @ np.abs(b))
     assert np.all(np.abs(x - ref) <= bound)
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 200, 1111])
+def test_dense_solve_device_matches_numpy(fem, n):
+    """fem_dense_solve_device (factor, inverse, one refinement step) against numpy's solve: an
+    SPD matrix with entries over several decades, error within a few FP64 roundings of cond(A)."""
+    rng = np.random.default_rng(100 + n)
+    k = np.exp(rng.standard_normal(n + 1))
+    A = np.diag(k[:-1] + k[1:])
+    for i in range(n - 1):
+        A[i, i + 1] = A[i + 1, i] = -k[i + 1]
+    M = rng.standard_normal((n, 4))
+    A += 0.05 * M @ M.T
+    b = rng.standard_normal(n)
+    x = fem.dense_solve(A, b)
+    ref = np.linalg.solve(A, b)
+    assert np.linalg.norm(x - ref) <= 1e-10 * np.linalg.norm(ref) * max(1.0, np.linalg.cond(A) / 1e4)
+    assert np.linalg.norm(b - A @ x) <= 1e-12 * np.linalg.norm(b) * np.linalg.cond(A)
+
+
+def test_dense_solve_device_refuses_a_singular_matrix(fem, oracle):
+    """The reference's text, with the diagonal in Rust's {:e} (cholesky.rs:94-98): a path
+    Laplacian with no shift fails at its last pivot, as the oracle's multifrontal factor does."""
+    n = 70
+    A = np.zeros((n, n))
+    for i in range(n - 1):
+        A[i, i] += 1
+        A[i + 1, i + 1] += 1
+        A[i, i + 1] -= 1
+        A[i + 1, i] -= 1
+    with pytest.raises(fem.FemError) as e:
+        fem.dense_solve(A, np.ones(n))
+    text = ("the matrix is not positive definite: the pivot of column 69 is 0 or less to the "
+            "precision of the arithmetic (its diagonal is 1e0)")
+    assert str(e.value) == text
+    # the oracle's multifrontal factor in the natural order stops at the same column, same text
+    ptr = np.zeros(n + 1, np.int64)
+    idx, val = [], []
+    for i in range(n):
+        cols = np.nonzero(A[i])[0]
+        idx += list(cols)
+        val += list(A[i, cols])
+        ptr[i + 1] = len(idx)
+    a = oracle.Csr.from_arrays(n, ptr, np.array(idx, np.uint32), np.array(val))
+    with pytest.raises(oracle.OracleError) as eo:
+        oracle.Cholesky(oracle.Symbolic(a, np.arange(n, dtype=np.int64)), a)
+    assert str(eo.value) == text
