@@ -142,6 +142,7 @@ EXPORTS = (
     "fem_solver_set_profile", "fem_solver_profile", "fem_solver_time_kernels",
     "fem_server_new", "fem_server_submit", "fem_server_wait", "fem_server_cancel", "fem_server_free",
     "fem_solver_new_csr", "fem_solver_matrix", "fem_dense_solve_device",
+    "fem_dense_solve_device_noalloc",
 )
 
 _lib = None
@@ -203,6 +204,7 @@ def lib() -> C.CDLL:
         "fem_server_free": (None, [P]),
         "fem_solver_matrix": (C.c_int, [P, C.POINTER(C.c_uint64), P, P, P, ERR, SZ]),
         "fem_dense_solve_device": (C.c_int, [C.c_int, P, P, P, ERR, SZ]),
+        "fem_dense_solve_device_noalloc": (C.c_int, [C.c_int, P, P, P, ERR, SZ]),
         "fem_solver_new_csr": (C.c_int, [C.POINTER(_Csr), C.POINTER(_Unknowns), C.POINTER(_Options), C.POINTER(P),
                                          ERR, SZ]),
         "fem_test_csr_levels_new": (C.c_int, [C.POINTER(_Csr), C.POINTER(_GridLevel), C.c_uint64, C.c_int,

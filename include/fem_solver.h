@@ -204,6 +204,17 @@ int fem_solver_new_csr(const fem_csr* a, const fem_unknowns* u, const fem_option
  * (cholesky.rs:94-98). */
 int fem_dense_solve_device(int n, const double* d_a, const double* d_b, double* d_x, char* err, size_t errlen);
 
+/* The same solve with no device allocation: no counterpart in the reference. It is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Its workspace is one fixed array of device memory (25.1 MiB)
+ * that each device the library runs on holds until the process ends, so it takes matrices of at
+ * most 1024 unknowns and refuses larger ones with a message. Calls on one device share that
+ * array: calls from several host threads of a process on one device run one after another (a
+ * lock per device), as they would otherwise race on it. Results equal fem_dense_solve_device's
+ * bit for bit. The call waits for the device. */
+int fem_dense_solve_device_noalloc(int n, const double* d_a, const double* d_b, double* d_x, char* err,
+                                   size_t errlen);
+
 /* ---- the model's integer structures and right side, for the caller and for parity checks ---- */
 
 /* The number of free unknowns (the length of b and x), and of all unknowns. */
