@@ -372,6 +372,26 @@ void* orc_grid_solver_new(void* csr, const double* px, size_t nx, const double* 
         return nullptr;
     }
 }
+// One step of coarsen (multigrid.rs:329-452) on a grid level and its matrix: the prolongation
+// (a Rect handle, or null when no axis has more than 2 planes) and the coarse points (3 per coarse
+// unknown, written to `coarse_points`, which holds 3 x rows entries at most).
+void* orc_coarsen(void* csr, const double* px, size_t nx, const double* py, size_t ny, const double* pz, size_t nz,
+                  const uint32_t* points, const uint8_t* comp, uint32_t* coarse_points, uint8_t* coarse_comp) {
+    const Csr& a = *static_cast<Csr*>(csr);
+    Level level;
+    level.planes = {std::vector<double>(px, px + nx), std::vector<double>(py, py + ny), std::vector<double>(pz, pz + nz)};
+    for (usize i = 0; i < a.n; ++i) {
+        level.points.push_back({points[3 * i], points[3 * i + 1], points[3 * i + 2]});
+        level.comp.push_back(comp[i]);
+    }
+    auto c = coarsen(level, a);
+    if (!c) return nullptr;
+    for (usize i = 0; i < c->coarse.points.size(); ++i) {
+        for (int d = 0; d < 3; ++d) coarse_points[3 * i + d] = c->coarse.points[i][d];
+        coarse_comp[i] = c->coarse.comp[i];
+    }
+    return new Rect(std::move(c->p));
+}
 int orc_solver_method(void* h) { return (int)static_cast<Solver*>(h)->method(); }
 size_t orc_solver_levels(void* h) {
     const Solver* s = static_cast<Solver*>(h);

@@ -35,12 +35,19 @@
 //   box problems (loads, rhs)      closed forms solved through the direct solver: layered wall
 //                                  (tests/heat.rs:112), uniform strain with face tractions, a hanging
 //                                  bar under body force (quadratic, exact on tet10)
+//   numbering and coarsen          literal vectors written out by hand from the reference's rules
+//                                  (tests/test_pinned_numbering.py): the first-seen points and cells
+//                                  of a 2 x 2 x 1 hex grid; the points, tetrahedra and Kuhn types of
+//                                  1 x 1 x 1 and 2 x 1 x 1 extruded boxes; the edge nodes of a tet10
+//                                  cell; coarsen on a two-material 4 x 2 x 2 grid (operator weights
+//                                  3/4 and 1/4, geometric 1/2 next to the fixed plane, the first-seen
+//                                  coarse numbering with coarse points on that plane)
 // Parity unpinned (no reference vector exists and none of the checks above fixes it):
-//   - the first-seen order of points and edge nodes of the box problems (heat.rs:1165-1208,
-//     tet10.rs:244-272) beyond being a permutation that covers each point and edge once;
-//   - the nested-dissection order on meshes with ties (order.rs:85-101), the supernode partition,
-//     the operator-dependent coarsen weights and first-seen coarse numbering (multigrid.rs:329-452),
-//     and therefore the reference's iteration counts;
+//   - the footprint triangulation of an extruded box: the reference meshes a footprint with its
+//     Delaunay mesher (mesh2d.rs), not restated here; the box problems take quads row by row cut
+//     along their (0,0)-(1,1) diagonal, and every rule after the footprint is the reference's;
+//   - the nested-dissection order on meshes with ties (order.rs:85-101) and the supernode partition
+//     beyond the known answers, and therefore the reference's iteration counts;
 //   - bitwise equality with the Rust binary.
 #pragma once
 

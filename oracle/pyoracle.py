@@ -101,6 +101,9 @@ def lib() -> C.CDLL:
         "orc_problem_coarse_xyz": (_SZ, [_P, C.c_void_p]),
         "orc_problem_free_handle": (None, [_P]),
         "orc_solver_new": (_P, [_P, C.c_int, _SZ, C.c_char_p, _SZ]),
+        "orc_coarsen": (_P, [_P, _F64P, _SZ, _F64P, _SZ, _F64P, _SZ, _U32P,
+                             np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS"), _U32P,
+                             np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")]),
         "orc_grid_solver_new": (_P, [_P, _F64P, _SZ, _F64P, _SZ, _F64P, _SZ, _U32P,
                                      np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS"), C.c_int, _SZ,
                                      C.c_char_p, _SZ]),
@@ -238,6 +241,21 @@ class Rect:
         y = np.zeros(self.cols)
         lib().orc_rect_mul_transpose(self.h, np.ascontiguousarray(x, np.float64), y)
         return y
+
+
+def coarsen(a: Csr, planes, points, comp):
+    """One step of coarsen (multigrid.rs:329-452): the prolongation (Rect) and the coarse points
+    and components, or None when no axis has more than 2 planes."""
+    pl = [np.ascontiguousarray(p, np.float64) for p in planes]
+    cp = np.zeros(3 * a.n, np.uint32)
+    cc = np.zeros(a.n, np.uint8)
+    h = lib().orc_coarsen(a.h, pl[0], pl[0].size, pl[1], pl[1].size, pl[2], pl[2].size,
+                          np.ascontiguousarray(points, np.uint32).reshape(-1), np.ascontiguousarray(comp, np.uint8),
+                          cp, cc)
+    if not h:
+        return None
+    r = Rect(h)
+    return r, cp[:3 * r.cols].reshape(-1, 3), cc[:r.cols]
 
 
 def nested_dissection(a: Csr, xyz) -> np.ndarray:
