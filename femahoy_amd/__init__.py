@@ -74,6 +74,10 @@ class _Options(C.Structure):
         ("refine", C.c_int32),
         ("device", C.c_int32),
         ("coarse_galerkin", C.c_int32),
+        ("crossover_kind", C.c_int32),
+        ("crossover_flops", C.c_double),
+        ("crossover_entries", C.c_uint64),
+        ("crossover_flat", C.c_int32),
     ]
 
 
@@ -107,9 +111,13 @@ class _GridLevel(C.Structure):
                 ("points", C.POINTER(C.c_uint32)), ("comp", C.POINTER(C.c_uint8))]
 
 
+class _Cells(C.Structure):
+    _fields_ = [("count", C.c_uint64), ("origin", C.c_void_p), ("element", C.c_void_p)]
+
+
 class _Unknowns(C.Structure):
     _fields_ = [("kind", C.c_int32), ("grid", _GridLevel), ("p", _Csr), ("coarse_has_grid", C.c_int32),
-                ("coarse_grid", _GridLevel)]
+                ("coarse_grid", _GridLevel), ("xyz", C.c_void_p), ("cells", C.POINTER(_Cells))]
 
 class _ServerOptions(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_active", C.c_int32), ("device", C.c_int32),
@@ -142,7 +150,7 @@ EXPORTS = (
     "fem_solver_set_profile", "fem_solver_profile", "fem_solver_time_kernels",
     "fem_server_new", "fem_server_submit", "fem_server_wait", "fem_server_cancel", "fem_server_free",
     "fem_solver_new_csr", "fem_solver_matrix", "fem_dense_solve_device",
-    "fem_dense_solve_device_noalloc",
+    "fem_dense_solve_device_noalloc", "fem_solver_crossover_counts",
 )
 
 _lib = None
@@ -205,6 +213,9 @@ def lib() -> C.CDLL:
         "fem_solver_matrix": (C.c_int, [P, C.POINTER(C.c_uint64), P, P, P, ERR, SZ]),
         "fem_dense_solve_device": (C.c_int, [C.c_int, P, P, P, ERR, SZ]),
         "fem_dense_solve_device_noalloc": (C.c_int, [C.c_int, P, P, P, ERR, SZ]),
+        "fem_solver_crossover_counts": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
+        "fem_test_symbolic": (C.c_int, [C.POINTER(_Csr), P, P, C.POINTER(C.c_uint64), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), ERR, SZ]),
         "fem_solver_new_csr": (C.c_int, [C.POINTER(_Csr), C.POINTER(_Unknowns), C.POINTER(_Options), C.POINTER(P),
                                          ERR, SZ]),
         "fem_test_csr_levels_new": (C.c_int, [C.POINTER(_Csr), C.POINTER(_GridLevel), C.c_uint64, C.c_int,
@@ -348,13 +359,28 @@ class Model:
         return (ijk + frac[kind]) * h
 
 
+@dataclass
+class Flops:
+    """Crossover::Flops (solve.rs:28-39): direct when the nested-dissection factor has at most
+    `entries` entries and (`flat`, or at most `flops` flops per unknown). A plain int crossover is
+    Crossover::Unknowns."""
+    flops: float = 1.5e5          # elastic.rs:263 DIRECT_FLOPS
+    entries: int = 250_000_000    # elastic.rs:269 DIRECT_ENTRIES
+    flat: bool = False
+
+
 def _options(method=None, precision="fp32", tol=PCG_TOL, max_iter=PCG_MAX_ITER, crossover=STEADY_DIRECT_LIMIT,
              levels=0, cheb_degree=0, cheb_lower=0.1, cheb_upper=1.1, power_steps=20, coarse_max=6000, refine=True,
              device=0, coarse_galerkin=True) -> "_Options":
     o = _Options()
     lib().fem_options_default(C.byref(o))
     o.tol, o.max_iter, o.method, o.precision = tol, max_iter, METHODS[method], PRECISIONS[precision]
-    o.crossover, o.levels, o.cheb_degree = crossover, levels, cheb_degree
+    if isinstance(crossover, Flops):
+        o.crossover_kind, o.crossover_flat = 1, int(crossover.flat)
+        o.crossover_flops, o.crossover_entries = float(crossover.flops), int(crossover.entries)
+    else:
+        o.crossover_kind, o.crossover = 0, int(crossover)
+    o.levels, o.cheb_degree = levels, cheb_degree
     o.cheb_lower, o.cheb_upper, o.power_steps = cheb_lower, cheb_upper, power_steps
     o.coarse_max, o.refine, o.device = coarse_max, int(refine), device
     o.coarse_galerkin = int(coarse_galerkin)
@@ -385,6 +411,14 @@ class Solver:
         if getattr(self, "h", None):
             lib().fem_solver_free(self.h)
             self.h = None
+
+    def crossover_counts(self):
+        """(entries, flops) of the nested-dissection factor that a FLOPS crossover compared, or
+        None when the solver's choice did not analyse the matrix."""
+        e, f = C.c_uint64(), C.c_double()
+        if lib().fem_solver_crossover_counts(self.h, C.byref(e), C.byref(f)):
+            return None
+        return int(e.value), float(f.value)
 
     def method(self) -> str:
         return ("direct", "multigrid")[lib().fem_solver_method(self.h)]
@@ -541,9 +575,10 @@ class CsrSolver(Solver):
 
     def __init__(self, ptr, idx, val, unknowns: str = "grid", grid: Optional[Grid] = None, p=None,
                  coarse_grid: Optional[Grid] = None, method: Optional[str] = None, precision: str = "fp32",
-                 tol: float = PCG_TOL, max_iter: int = PCG_MAX_ITER, crossover: int = STEADY_DIRECT_LIMIT,
+                 tol: float = PCG_TOL, max_iter: int = PCG_MAX_ITER, crossover=STEADY_DIRECT_LIMIT,
                  levels: int = 0, cheb_degree: int = 0, cheb_lower: float = 0.1, cheb_upper: float = 1.1,
-                 power_steps: int = 20, coarse_max: int = 6000, refine: bool = True, device: int = 0):
+                 power_steps: int = 20, coarse_max: int = 6000, refine: bool = True, device: int = 0,
+                 xyz=None, cells=None):
         n = len(ptr) - 1
         o = _options(method, precision, tol, max_iter, crossover, levels, cheb_degree, cheb_lower, cheb_upper,
                      power_steps, coarse_max, refine, device, True)
@@ -565,6 +600,17 @@ class CsrSolver(Solver):
                 u.coarse_has_grid = 1
                 u.coarse_grid, k = coarse_grid._struct(int(pcols))
                 keep += k
+        if xyz is not None:
+            xyz = np.ascontiguousarray(xyz, np.float64).reshape(n, 3)
+            u.xyz = xyz.ctypes.data
+            keep.append(xyz)
+        if cells is not None:
+            origin, element = cells
+            origin = np.ascontiguousarray(origin, np.uint32).reshape(-1, 3)
+            element = np.ascontiguousarray(element, np.float64).reshape(-1, 576)
+            cs = _Cells(len(origin), origin.ctypes.data, element.ctypes.data)
+            u.cells = C.pointer(cs)
+            keep += [origin, element, cs]
         h = C.c_void_p()
         err = _err()
         if lib().fem_solver_new_csr(C.byref(a), C.byref(u), C.byref(o), C.byref(h), err, 2048):
@@ -572,6 +618,23 @@ class CsrSolver(Solver):
         del keep
         self.h = h
         self.n = int(lib().fem_solver_free_count(self.h))
+
+
+def symbolic(ptr, idx, xyz):
+    """The library's host nested dissection (order.rs:19-53) and symbolic counts (symbolic.rs) of a
+    square pattern, as the FLOPS crossover computes them: (order, entries, flops, largest_front,
+    supernodes). No device is touched."""
+    n = len(ptr) - 1
+    a, keep = _csr_struct(ptr, idx, np.zeros(int(ptr[-1])), n, n)
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(n, 3)
+    order = np.zeros(n, np.uint64)
+    e, f, lf, sn = C.c_uint64(), C.c_double(), C.c_uint64(), C.c_uint64()
+    err = _err()
+    if lib().fem_test_symbolic(C.byref(a), xyz.ctypes.data, order.ctypes.data, C.byref(e), C.byref(f), C.byref(lf),
+                               C.byref(sn), err, 2048):
+        raise FemError(err.value.decode())
+    del keep
+    return order.astype(np.int64), int(e.value), float(f.value), int(lf.value), int(sn.value)
 
 
 def csr_levels(ptr, idx, val, grid: Grid, coarsest: int = 3000, max_levels: int = 0):
@@ -727,11 +790,11 @@ class Server:
 
 
 def inclusion_box(n: int, count: int, seed: int, e_ratio: float = 1e3, nu_matrix: float = 0.49,
-                  nu_inclusion: float = 0.3) -> "tuple[Box, np.ndarray]":
+                  nu_inclusion: float = 0.3, radius=(0.02, 0.05)) -> "tuple[Box, np.ndarray]":
     """Nearly incompressible heterogeneous elasticity: tet10 on n x n/2 x n/2 unit cells (6 Kuhn
     tetrahedra per cell), a matrix of E = 1 and nu = nu_matrix holding `count` spheres of E =
-    e_ratio and nu = nu_inclusion, centres uniform in the box and radii uniform in 0.02-0.05 of
-    the box length, from numpy's default_rng(seed); a tetrahedron is in a sphere when its centroid
+    e_ratio and nu = nu_inclusion, centres uniform in the box and radii uniform in `radius`
+    (0.02-0.05 by default) of the box length, from numpy's default_rng(seed); a tetrahedron is in a sphere when its centroid
     is. Clamped at x = 0, traction (1, 0, 0) on x = 1. Returns the box and the inclusion mask
     (reference numbering of tetrahedra)."""
     cells = [n, n // 2, n // 2]
@@ -739,7 +802,7 @@ def inclusion_box(n: int, count: int, seed: int, e_ratio: float = 1e3, nu_matrix
     c = Model(Box("elastic_tet10", cells, length, 1.0, 0.3, fixed={"x0": True})).tet_centroids()
     rng = np.random.default_rng(seed)
     centres = rng.uniform(0.0, 1.0, (count, 3)) * np.asarray(length)
-    radii = rng.uniform(0.02, 0.05, count) * max(length)
+    radii = rng.uniform(radius[0], radius[1], count) * max(length)
     inside = np.zeros(len(c), bool)
     for p, r in zip(centres, radii):
         inside |= ((c - p) ** 2).sum(axis=1) <= r * r

@@ -191,4 +191,13 @@ __global__ void __launch_bounds__(BLOCK) k_csr_pcg_start(uint64_t n, SellMat<dou
     }
 }
 
+// The rows of a CSR (n rows, columns < n) into a dense row-major n x n matrix (zero elsewhere).
+__global__ void __launch_bounds__(BLOCK) k_csr_scatter_dense(uint64_t n, const uint64_t* __restrict__ ptr,
+                                                             const uint32_t* __restrict__ idx,
+                                                             const double* __restrict__ val, double* __restrict__ A) {
+    const uint64_t r = blockIdx.x * (uint64_t)BLOCK + threadIdx.x;
+    if (r >= n) return;
+    for (uint64_t k = ptr[r]; k < ptr[r + 1]; ++k) A[r * n + idx[k]] = val[k];
+}
+
 }  // namespace fem

@@ -1,7 +1,8 @@
 // The solver of the assembled boundary (fem_solver_new_csr): the reference's Solver::new for a
 // CSR matrix of the free unknowns (crates/fem/src/solve.rs:92-183) on the device.
 //
-//  - Direct: the dense FP64 LDL^T of the whole matrix (dense.hip), up to 20000 unknowns.
+//  - Direct: the dense FP64 LDL^T of the whole matrix (dense.hip), up to DENSE_DIRECT_LIMIT
+//    (40000) unknowns.
 //  - Grid multigrid: the reference's hierarchy (csr_host.cpp: coarsen and Galerkin products, the
 //    level loop of multigrid.rs:84-139), the matrices and transfers of every level in CSR form on
 //    the device, a Chebyshev-Jacobi smoother, the dense LDL^T on the coarsest level.
@@ -31,7 +32,6 @@ namespace fem {
 
 namespace {
 
-constexpr uint64_t DIRECT_LIMIT = 20000;  // the dense direct solve (as the box solver's)
 constexpr uint64_t COARSEST = 3000;       // multigrid.rs:50
 
 int threads_for_setup() {
@@ -205,13 +205,24 @@ class CsrSolverImpl {
         FEM_HIP(hipHostMalloc((void**)&pinned_, 16 * sizeof(double), hipHostMallocDefault));
         FEM_HIP(hipEventCreate(&ev0_));
         FEM_HIP(hipEventCreate(&ev1_));
-        // solve.rs:98-183 with Crossover::Unknowns: the direct solver when forced or small enough
+        // solve.rs:98-183: the direct solver when forced, or as the crossover chooses
+        if (u.cells)
+            throw Error("coarse matrices from cells (multigrid.rs:458-679) are not built: give no cells, and the "
+                        "coarse matrices are the Galerkin products of the prolongations");
         if (u.kind == FEM_UNKNOWNS_POINTS) {
             if (o.method == FEM_METHOD_MULTIGRID)
                 throw Error("multigrid needs the box grid: a mesh of tetrahedra has the direct solver");
             direct_ = true;
         } else if (u.kind == FEM_UNKNOWNS_GRID || u.kind == FEM_UNKNOWNS_QUADRATIC) {
-            direct_ = o.method == FEM_METHOD_DIRECT || (o.method == FEM_METHOD_AUTO && n <= o.crossover);
+            if (o.method == FEM_METHOD_AUTO) {
+                if (u.kind == FEM_UNKNOWNS_GRID) {
+                    csr::validate(u.grid, n);
+                    u.xyz = grid_xyz(u.grid);
+                }
+                direct_ = auto_picks_direct(a, u.xyz, o, counts_, analysed_);
+            } else {
+                direct_ = o.method == FEM_METHOD_DIRECT;
+            }
         } else {
             throw Error("unknown kind of unknowns");
         }
@@ -224,9 +235,14 @@ class CsrSolverImpl {
         scal_.resize(1);
         red_.resize(4);
         if (direct_) {
-            if (n > DIRECT_LIMIT)
-                throw Error("the direct solve holds at most " + std::to_string(DIRECT_LIMIT) + " unknowns; this matrix has " +
-                            std::to_string(n));
+            if (n > DENSE_DIRECT_LIMIT) {
+                std::string why;
+                if (analysed_)
+                    why = " (the crossover chose it: the factor has " + std::to_string(counts_.entries) + " entries and " +
+                          rust_e(counts_.flops) + " flops; ask for multigrid)";
+                throw Error("the direct solve holds at most " + std::to_string(DENSE_DIRECT_LIMIT) +
+                            " unknowns; this matrix has " + std::to_string(n) + why);
+            }
             factor_dense(a, direct_coarse_, false);
         } else {
             build_hierarchy(std::move(a), std::move(u));
@@ -245,6 +261,12 @@ class CsrSolverImpl {
 
     int method() const { return direct_ ? FEM_METHOD_DIRECT : FEM_METHOD_MULTIGRID; }
     uint64_t n() const { return n_; }
+    bool crossover_counts(uint64_t* entries, double* flops) const {
+        if (!analysed_) return false;
+        if (entries) *entries = counts_.entries;
+        if (flops) *flops = counts_.flops;
+        return true;
+    }
     Options& options() { return opt_; }
     const char* device_name() const { return device_name_.c_str(); }
     const csr::GridHierarchy& host_levels() const { return host_; }
@@ -327,13 +349,22 @@ class CsrSolverImpl {
 
    private:
     // ---------------------------------------------------------------- setup
+    // The dense matrix of a CSR on the device (rows scattered by a kernel, so no n^2 host array),
+    // then its factor and inverse.
     void factor_dense(const csr::Csr& a, DenseCoarse& dc, bool fp32) {
         const uint64_t n = a.n;
-        std::vector<double> dense(n * n, 0.0);
-        for (uint64_t i = 0; i < n; ++i)
-            for (uint64_t k = a.ptr[i]; k < a.ptr[i + 1]; ++k) dense[i * n + a.idx[k]] = a.val[k];
+        DArray<uint64_t> dptr(n + 1);
+        DArray<uint32_t> didx(std::max<uint64_t>(a.ptr[n], 1));
+        DArray<double> dval(std::max<uint64_t>(a.ptr[n], 1));
+        dptr.upload(a.ptr.data(), n + 1, s_);
+        if (a.ptr[n]) {
+            didx.upload(a.idx.data(), a.ptr[n], s_);
+            dval.upload(a.val.data(), a.ptr[n], s_);
+        }
         DArray<double> A(n * n);
-        A.upload(dense.data(), n * n, s_);
+        launch(k_fill<double>, n * n, s_, n * n, A.data(), 0.0);
+        launch(k_csr_scatter_dense, n, s_, n, (const uint64_t*)dptr.data(), (const uint32_t*)didx.data(),
+               (const double*)dval.data(), A.data());
         FEM_HIP(hipStreamSynchronize(s_));
         dc.factor(A.data(), (int)n, fp32, s_);
         std::vector<int64_t> iota(n);
@@ -375,9 +406,15 @@ class CsrSolverImpl {
             }
         }
         if (host_.a.size() < 2) throw Error("the matrix is too small for multigrid: use the direct solver");
-        if (host_.a.back().n > cmax)
+        // A quadratic caller with no grid below its linear level (the reference's tet10 caller,
+        // elastic_tet.rs:294-313, passes coordinates only) has that level factored whole, as the
+        // reference does (pmultigrid.rs:46-60): the dense LDL^T up to DENSE_DIRECT_LIMIT unknowns,
+        // whatever coarse_max says. Below a grid hierarchy the coarsest level keeps coarse_max.
+        const bool whole_linear = u.kind == FEM_UNKNOWNS_QUADRATIC && !u.coarse_grid;
+        const uint64_t limit = whole_linear ? std::max<uint64_t>(cmax, DENSE_DIRECT_LIMIT) : cmax;
+        if (host_.a.back().n > limit)
             throw Error("the coarsest level has " + std::to_string(host_.a.back().n) +
-                        " unknowns, more than the dense limit " + std::to_string(cmax));
+                        " unknowns, more than the dense limit " + std::to_string(limit));
         for (const csr::Csr& m : host_.a)
             for (double d : csr::diagonal(m))
                 if (!(d > 0.0)) throw Error("the matrix has a diagonal entry that is not above 0");
@@ -652,6 +689,8 @@ class CsrSolverImpl {
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     hipGraphExec_t graph_ = nullptr;
     bool direct_ = false;
+    bool analysed_ = false;  // a FLOPS crossover analysed the matrix (counts_)
+    sym::Counts counts_;
     uint64_t n_ = 0, parts_ = 0;
     double setup_seconds_ = 0.0;
     DevCsr<double> a64_;
@@ -679,6 +718,25 @@ void CsrSolver::apply_host(const double* x, double* y) { impl_->apply_host(x, y)
 void CsrSolver::precondition_host(const double* r, double* z) { impl_->precondition_host(r, z); }
 void CsrSolver::level_info(int l, uint64_t* n, double* lmax) const { impl_->level_info(l, n, lmax); }
 const char* CsrSolver::device_name() const { return impl_->device_name(); }
+bool CsrSolver::crossover_counts(uint64_t* e, double* f) const { return impl_->crossover_counts(e, f); }
+
+std::vector<std::array<double, 3>> grid_xyz(const csr::GridLevel& g) {
+    std::vector<std::array<double, 3>> xyz(g.points.size());
+    for (size_t u = 0; u < g.points.size(); ++u)
+        for (int d = 0; d < 3; ++d) xyz[u][d] = g.planes[d][g.points[u][d]];
+    return xyz;
+}
+
+bool auto_picks_direct(const csr::Csr& a, const std::vector<std::array<double, 3>>& xyz, const Options& o,
+                       sym::Counts& counts, bool& analysed) {
+    analysed = false;
+    if (o.crossover_kind == FEM_CROSSOVER_UNKNOWNS) return a.n <= o.crossover;
+    if (o.crossover_kind != FEM_CROSSOVER_FLOPS) throw Error("unknown crossover");
+    if (xyz.size() != a.n) throw Error("the FLOPS crossover needs the coordinates of each unknown");
+    counts = sym::analyse(a, sym::nested_dissection(a, xyz));
+    analysed = true;
+    return sym::flops_picks_direct(counts, a.n, o.crossover_flops, o.crossover_entries, o.crossover_flat);
+}
 const csr::GridHierarchy& CsrSolver::host_levels() const { return impl_->host_levels(); }
 
 }  // namespace fem
@@ -741,6 +799,12 @@ CsrUnknowns unknowns_from(const fem_unknowns* u, uint64_t n) {
     } else if (u->kind != FEM_UNKNOWNS_POINTS) {
         throw Error("unknown kind of unknowns");
     }
+    if (u->xyz && u->kind != FEM_UNKNOWNS_GRID) {
+        out.xyz.resize(n);
+        for (uint64_t k = 0; k < n; ++k)
+            for (int d = 0; d < 3; ++d) out.xyz[k][d] = u->xyz[3 * k + d];
+    }
+    out.cells = u->cells != nullptr;
     return out;
 }
 
@@ -817,5 +881,28 @@ int fem_test_csr_level_arrays(const fem_csr_levels* h, int l, uint64_t* a_ptr, u
 }
 
 void fem_test_csr_levels_free(fem_csr_levels* h) { delete h; }
+
+int fem_test_symbolic(const fem_csr* a, const double* xyz, uint64_t* order_out, uint64_t* entries, double* flops,
+                      uint64_t* largest_front, uint64_t* supernodes, char* err, size_t errlen) {
+    try {
+        fem::csr::Csr m = fem::csr_from(a);
+        fem::csr::validate(m);
+        if (!xyz) throw fem::Error("the coordinates are needed");
+        std::vector<std::array<double, 3>> p(m.n);
+        for (uint64_t k = 0; k < m.n; ++k)
+            for (int d = 0; d < 3; ++d) p[k][d] = xyz[3 * k + d];
+        const std::vector<uint64_t> order = fem::sym::nested_dissection(m, p);
+        const fem::sym::Counts c = fem::sym::analyse(m, order);
+        if (order_out) std::copy(order.begin(), order.end(), order_out);
+        if (entries) *entries = c.entries;
+        if (flops) *flops = c.flops;
+        if (largest_front) *largest_front = c.largest_front;
+        if (supernodes) *supernodes = c.supernodes;
+        return 0;
+    } catch (const std::exception& e) {
+        put_error(err, errlen, e.what());
+        return 1;
+    }
+}
 
 }  // extern "C"

@@ -9,6 +9,7 @@
 #include "../../include/fem_solver.h"
 #include "csr_host.hpp"
 #include "solver_common.hpp"
+#include "symbolic.hpp"
 
 namespace fem {
 
@@ -19,7 +20,18 @@ struct CsrUnknowns {
     csr::Rect p;                        // QUADRATIC: the prolongation from the linear unknowns
     bool coarse_grid = false;           // QUADRATIC: the linear unknowns' grid level is given
     csr::GridLevel coarse;              // ... and here it is
+    std::vector<std::array<double, 3>> xyz;  // POINTS, QUADRATIC: the coordinates of each unknown, if given
+    bool cells = false;                      // Solver::new's cells were given (refused)
 };
+
+// The coordinates of each unknown of a grid level (solve.rs:143-150: the planes of its point).
+std::vector<std::array<double, 3>> grid_xyz(const csr::GridLevel& g);
+
+// The direct-or-multigrid choice of Solver::new for FEM_METHOD_AUTO (solve.rs:106-118, 158-170):
+// Crossover::Unknowns, or Crossover::Flops on the counts of the nested-dissection factor of `a`
+// (which are returned in `counts`, with `analysed` set).
+bool auto_picks_direct(const csr::Csr& a, const std::vector<std::array<double, 3>>& xyz, const Options& o,
+                       sym::Counts& counts, bool& analysed);
 
 class CsrSolverImpl;
 
@@ -36,6 +48,8 @@ class CsrSolver {
     void precondition_host(const double* r, double* z);
     void level_info(int level, uint64_t* unknowns, double* lmax) const;
     const char* device_name() const;
+    // The factor counts a FLOPS crossover compared; false when it did not analyse the matrix.
+    bool crossover_counts(uint64_t* entries, double* flops) const;
     // The host hierarchy (levels' matrices and prolongations) the device levels were made from.
     const csr::GridHierarchy& host_levels() const;
 

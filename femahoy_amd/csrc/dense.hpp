@@ -12,6 +12,12 @@
 
 namespace fem {
 
+// The most unknowns of a dense direct solve (the direct method of both boundaries, and the linear
+// level of a p-multigrid with no grid below it): the reference's TRANSIENT_DIRECT_LIMIT
+// (crates/fem/src/solve.rs:24), its largest direct crossover. A factor of n unknowns holds four
+// n^2 FP64 arrays on the device, 51 GB at this limit.
+constexpr uint64_t DENSE_DIRECT_LIMIT = 40000;
+
 // The device workspace of one dense factor of n unknowns (np = n rounded up to 64, nb = np / 64
 // blocks), carved from one contiguous region of bytes(n) bytes at 256-byte aligned offsets:
 // A (np x np, the padded copy that the factor overwrites with L and D), U (np x np, L^-T),

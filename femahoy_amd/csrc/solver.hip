@@ -561,14 +561,24 @@ class Solver {
         free_grid_.upload(model_.free_grid.data(), nfree, s_);
         fb_.resize(nfree);
         fx_.resize(nfree);
-        // direct or multigrid (solve.rs:92-183 with Crossover::Unknowns)
+        // direct or multigrid (solve.rs:92-183): forced, or as the crossover chooses (a FLOPS
+        // crossover analyses the reduced matrix's pattern in the nested dissection order)
         bool direct = false;
-        if (o.method == FEM_METHOD_DIRECT) direct = true;
-        else if (o.method == FEM_METHOD_AUTO) direct = nfree <= o.crossover;
+        if (o.method == FEM_METHOD_DIRECT) {
+            direct = true;
+        } else if (o.method == FEM_METHOD_AUTO) {
+            if (o.crossover_kind == FEM_CROSSOVER_FLOPS) direct = auto_picks_direct(reduced_pattern(), free_xyz(), o, counts_, analysed_);
+            else direct = nfree <= o.crossover;
+        }
         if (direct) {
-            if (nfree > 20000)
-                throw Error("the direct solver on the device holds up to 20000 unknowns; this model has " +
-                            std::to_string(nfree));
+            if (nfree > DENSE_DIRECT_LIMIT) {
+                std::string why;
+                if (analysed_)
+                    why = " (the crossover chose it: the factor has " + std::to_string(counts_.entries) + " entries and " +
+                          rust_e(counts_.flops) + " flops; ask for multigrid)";
+                throw Error("the direct solver on the device holds up to " + std::to_string(DENSE_DIRECT_LIMIT) +
+                            " unknowns; this model has " + std::to_string(nfree) + why);
+            }
             direct_ = std::make_unique<DenseCoarse>();
             assemble_and_factor(lv64_[0], *direct_, false);
             method_ = FEM_METHOD_DIRECT;
@@ -604,6 +614,12 @@ class Solver {
     }
 
     int method() const { return method_; }
+    bool crossover_counts(uint64_t* entries, double* flops) const {
+        if (!analysed_) return false;
+        if (entries) *entries = counts_.entries;
+        if (flops) *flops = counts_.flops;
+        return true;
+    }
 
     void set_profile(bool on) {
         prof_.on = on;
@@ -663,39 +679,44 @@ class Solver {
         FEM_HIP(hipStreamSynchronize(s_));
     }
 
-    // The reduced matrix of the model (Csr::submatrix(free), sparse.rs:108-135, of the matrix the
-    // reference's callers assemble): its pattern is every pair of free unknowns whose nodes share
-    // an element (Csr::pattern, sparse.rs:22-65: 27 neighbours on hex8 cells, the 15 of the Kuhn
-    // edges on tetrahedra), columns in increasing order; its values come from probing the FP64
-    // operator with one unit vector per colour of nodes 3 apart. Returns the entry count; fills
-    // the arrays when they are given. Not for tet10.
-    uint64_t export_matrix(uint64_t* out_ptr, uint32_t* out_idx, double* out_val) {
+    // The pattern of the model's reduced matrix (values empty): every pair of free unknowns whose
+    // nodes share an element, columns increasing (Csr::pattern, sparse.rs:22-65, then
+    // Csr::submatrix(free), sparse.rs:108-135). fidx, if given, gets the free index of each grid
+    // unknown (-1 for a fixed one).
+    csr::Csr reduced_pattern(std::vector<int64_t>* fidx_out = nullptr) const {
         const BoxModel& m = model_;
-        if (m.in.kind == Kind::ElasticTet10) throw Error("the matrix export covers hex8 and tet4 models");
+        const bool quad = m.in.kind == Kind::ElasticTet10;
         const int C = m.comps;
         const int nx = m.gn[0], ny = m.gn[1], nz = m.gn[2];
         const uint64_t nf = m.free.size();
         std::vector<int64_t> fidx(m.grid_unknowns(), -1);
         for (uint64_t r = 0; r < nf; ++r) fidx[m.free_grid[r]] = (int64_t)r;
-        // neighbour offsets of the pattern
-        std::vector<std::array<int, 3>> offs;
-        for (int dz = -1; dz <= 1; ++dz)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
-                    if (m.in.kind != Kind::HeatHex8) {  // a Kuhn edge: all components >= 0 or all <= 0
-                        const bool up = dx >= 0 && dy >= 0 && dz >= 0, down = dx <= 0 && dy <= 0 && dz <= 0;
-                        if (!up && !down) continue;
+        // neighbour offsets of the pattern, per class of node (its coordinates mod 2 on tet10;
+        // one class otherwise)
+        std::vector<std::array<int, 3>> offs[8];
+        if (quad) {
+            t10_pattern_offsets(offs);
+        } else {
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        if (m.in.kind != Kind::HeatHex8) {  // a Kuhn edge: all components >= 0 or all <= 0
+                            const bool up = dx >= 0 && dy >= 0 && dz >= 0, down = dx <= 0 && dy <= 0 && dz <= 0;
+                            if (!up && !down) continue;
+                        }
+                        offs[0].push_back({dx, dy, dz});
                     }
-                    offs.push_back({dx, dy, dz});
-                }
-        std::vector<uint64_t> ptr(nf + 1, 0);
-        std::vector<uint32_t> idx;
+        }
+        csr::Csr out;
+        out.n = nf;
+        out.ptr.assign(nf + 1, 0);
         auto row_cols = [&](uint64_t r, std::vector<uint32_t>& cols) {
             cols.clear();
             const int64_t gu = m.free_grid[r];
             const int64_t node = gu / C;
             const int i = (int)(node % nx), j = (int)((node / nx) % ny), k = (int)(node / ((int64_t)nx * ny));
-            for (const auto& o : offs) {
+            const int cls = quad ? (i & 1) + 2 * (j & 1) + 4 * (k & 1) : 0;
+            for (const auto& o : offs[cls]) {
                 const int a = i + o[0], b = j + o[1], c = k + o[2];
                 if (a < 0 || b < 0 || c < 0 || a >= nx || b >= ny || c >= nz) continue;
                 const int64_t nb = a + (int64_t)nx * (b + (int64_t)ny * c);
@@ -709,15 +730,50 @@ class Solver {
         std::vector<uint32_t> cols;
         for (uint64_t r = 0; r < nf; ++r) {
             row_cols(r, cols);
-            ptr[r + 1] = ptr[r] + cols.size();
+            out.ptr[r + 1] = out.ptr[r] + cols.size();
         }
-        const uint64_t nnz = ptr[nf];
-        if (!out_ptr) return nnz;
-        idx.resize(nnz);
+        out.idx.resize(out.ptr[nf]);
         for (uint64_t r = 0; r < nf; ++r) {
             row_cols(r, cols);
-            std::copy(cols.begin(), cols.end(), idx.begin() + ptr[r]);
+            std::copy(cols.begin(), cols.end(), out.idx.begin() + out.ptr[r]);
         }
+        if (fidx_out) *fidx_out = std::move(fidx);
+        return out;
+    }
+
+    // The coordinates of each free unknown's node for the nested dissection of the crossover: its
+    // grid indices (refined twice on tet10). The order reads coordinates only through comparisons
+    // along one axis at a time (order.rs:66-139), so these give the order of the reference's
+    // coordinates, which increase with the grid index along each axis.
+    std::vector<std::array<double, 3>> free_xyz() const {
+        const BoxModel& m = model_;
+        const int nx = m.gn[0], ny = m.gn[1];
+        std::vector<std::array<double, 3>> xyz(m.free.size());
+        for (size_t r = 0; r < m.free.size(); ++r) {
+            const int64_t node = m.free_grid[r] / m.comps;
+            xyz[r] = {(double)(node % nx), (double)((node / nx) % ny), (double)(node / ((int64_t)nx * ny))};
+        }
+        return xyz;
+    }
+
+    // The reduced matrix of the model (Csr::submatrix(free), sparse.rs:108-135, of the matrix the
+    // reference's callers assemble): its pattern is every pair of free unknowns whose nodes share
+    // an element (Csr::pattern, sparse.rs:22-65: 27 neighbours on hex8 cells, the 15 of the Kuhn
+    // edges on tetrahedra, and on tet10 the nodes of the tetrahedra around each node of the grid
+    // refined twice), columns in increasing order; its values come from probing the FP64 operator
+    // with one unit vector per colour of nodes S apart (S = 3, or 5 on tet10, whose couplings
+    // reach 2 nodes). Returns the entry count; fills the arrays when they are given.
+    uint64_t export_matrix(uint64_t* out_ptr, uint32_t* out_idx, double* out_val) {
+        const BoxModel& m = model_;
+        const bool quad = m.in.kind == Kind::ElasticTet10;
+        const int C = m.comps;
+        const uint64_t nf = m.free.size();
+        std::vector<int64_t> fidx;
+        const csr::Csr pat = reduced_pattern(&fidx);
+        const std::vector<uint64_t>& ptr = pat.ptr;
+        const std::vector<uint32_t>& idx = pat.idx;
+        const uint64_t nnz = ptr[nf];
+        if (!out_ptr) return nnz;
         DArray<uint64_t> dptr(nf + 1);
         DArray<uint32_t> didx(std::max<uint64_t>(nnz, 1));
         DArray<double> dval(std::max<uint64_t>(nnz, 1)), px(n0_), py(n0_);
@@ -726,13 +782,14 @@ class Solver {
         if (nnz) didx.upload(idx.data(), nnz, s_);
         dof.upload(fidx.data(), fidx.size(), s_);
         Level<double>& L = lv64_[0];
-        for (int c0 = 0; c0 < 3; ++c0)
-            for (int c1 = 0; c1 < 3; ++c1)
-                for (int c2 = 0; c2 < 3; ++c2)
+        const int S = quad ? 5 : 3;
+        for (int c0 = 0; c0 < S; ++c0)
+            for (int c1 = 0; c1 < S; ++c1)
+                for (int c2 = 0; c2 < S; ++c2)
                     for (int comp = 0; comp < C; ++comp) {
-                        launch(k_probe_fill, L.dofs(), s_, L.g, C, 3, c0, c1, c2, comp, (const int64_t*)dof.data(), px.data());
+                        launch(k_probe_fill, L.dofs(), s_, L.g, C, S, c0, c1, c2, comp, (const int64_t*)dof.data(), px.data());
                         op_apply(L, (const double*)px.data(), py.data(), s_);
-                        launch(k_probe_csr, nf, s_, nf, L.g, C, 3, c0, c1, c2, comp, (const uint64_t*)dptr.data(),
+                        launch(k_probe_csr, nf, s_, nf, L.g, C, S, c0, c1, c2, comp, (const uint64_t*)dptr.data(),
                                (const uint32_t*)didx.data(), (const int64_t*)free_grid_.data(), (const double*)py.data(),
                                dval.data());
                     }
@@ -741,6 +798,37 @@ class Solver {
         if (out_val && nnz) dval.download(out_val, nnz, s_);
         FEM_HIP(hipStreamSynchronize(s_));
         return nnz;
+    }
+
+    // The pattern offsets of tet10 on the Kuhn box, on the grid refined twice: the mesh repeats
+    // with a period of one cell (2 nodes), so the nodes that share a tetrahedron with a node are
+    // its position plus a set of offsets that depends only on its coordinates mod 2 (class
+    // x + 2 y + 4 z of those parities). The sets come from the 6 tetrahedra of one cell, all pairs of
+    // their 10 nodes. Near the box's faces the offsets that leave the grid are dropped; the rest
+    // still share a tetrahedron inside the box, because the split is conforming across cell faces.
+    static void t10_pattern_offsets(std::vector<std::array<int, 3>> offs[8]) {
+        static const int ORDER[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+        for (int t = 0; t < 6; ++t) {
+            int v[4][3] = {{0, 0, 0}};
+            for (int s2 = 0; s2 < 3; ++s2) {
+                for (int d = 0; d < 3; ++d) v[s2 + 1][d] = v[s2][d];
+                v[s2 + 1][ORDER[t][s2]] += 2;
+            }
+            std::array<int, 3> nd[10];
+            int q = 0;
+            for (int a = 0; a < 4; ++a) nd[q++] = {v[a][0], v[a][1], v[a][2]};
+            for (int a = 0; a < 4; ++a)
+                for (int b = a + 1; b < 4; ++b) nd[q++] = {(v[a][0] + v[b][0]) / 2, (v[a][1] + v[b][1]) / 2, (v[a][2] + v[b][2]) / 2};
+            for (int a = 0; a < 10; ++a)
+                for (int b = 0; b < 10; ++b) {
+                    const int cls = (nd[a][0] & 1) + 2 * (nd[a][1] & 1) + 4 * (nd[a][2] & 1);
+                    offs[cls].push_back({nd[b][0] - nd[a][0], nd[b][1] - nd[a][1], nd[b][2] - nd[a][2]});
+                }
+        }
+        for (int c = 0; c < 8; ++c) {
+            std::sort(offs[c].begin(), offs[c].end());
+            offs[c].erase(std::unique(offs[c].begin(), offs[c].end()), offs[c].end());
+        }
     }
 
     void precondition_host(const double* rin, double* zout) {
@@ -1023,6 +1111,8 @@ class Solver {
         std::string error;
     } run_;
     int64_t device_bytes_ = 0;  // peak device bytes of the construction
+    bool analysed_ = false;     // a FLOPS crossover analysed the reduced matrix (counts_)
+    sym::Counts counts_;
     size_t n0_ = 0;
     int parts_ = 0;
     DArray<double> x_, b_, r_, z_, p_, q_, tmp_, partA_, partB_, red_, fb_, fx_;
@@ -2002,7 +2092,13 @@ fem::Options options_from(const fem_options& opt) {
     o.refine = opt.refine;
     o.device = opt.device;
     o.coarse_galerkin = opt.coarse_galerkin;
+    o.crossover_kind = opt.crossover_kind;
+    o.crossover_flops = opt.crossover_flops;
+    o.crossover_entries = opt.crossover_entries;
+    o.crossover_flat = opt.crossover_flat != 0;
     if (o.precision < 0 || o.precision > 2) throw fem::Error("unknown precision");
+    if (o.crossover_kind != FEM_CROSSOVER_UNKNOWNS && o.crossover_kind != FEM_CROSSOVER_FLOPS)
+        throw fem::Error("unknown crossover");
     return o;
 }
 
@@ -2038,6 +2134,10 @@ void fem_options_default(fem_options* o) {
     o->refine = d.refine;
     o->device = d.device;
     o->coarse_galerkin = d.coarse_galerkin;
+    o->crossover_kind = d.crossover_kind;
+    o->crossover_flops = d.crossover_flops;
+    o->crossover_entries = d.crossover_entries;
+    o->crossover_flat = d.crossover_flat ? 1 : 0;
 }
 
 int fem_solver_new_box(const fem_box* box, const fem_options* opt, fem_solver** out, char* err, size_t errlen) {
@@ -2096,6 +2196,12 @@ int fem_solver_solve_device(fem_solver* s, const double* d_b, double* d_x, int (
         if (s->csr) s->csr->solve_device(d_b, d_x, cancelled, ctx, stats);
         else s->impl->solve_device(d_b, d_x, cancelled, ctx, stats);
     });
+}
+
+int fem_solver_crossover_counts(const fem_solver* s, uint64_t* entries, double* flops) {
+    if (!s) return 1;
+    const bool ok = s->impl ? s->impl->crossover_counts(entries, flops) : s->csr->crossover_counts(entries, flops);
+    return ok ? 0 : 1;
 }
 
 int fem_solver_method(const fem_solver* s) {

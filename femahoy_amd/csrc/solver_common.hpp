@@ -55,6 +55,10 @@ struct Options {
     int refine = 1;
     int device = 0;
     int coarse_galerkin = 1;
+    int crossover_kind = FEM_CROSSOVER_UNKNOWNS;  // solve.rs:28-39
+    double crossover_flops = 1.5e5;               // elastic.rs:263 DIRECT_FLOPS
+    uint64_t crossover_entries = 250000000;       // elastic.rs:269 DIRECT_ENTRIES
+    bool crossover_flat = false;
 };
 
 // The smoothing steps (Chebyshev degree, pre and post) of level l of a box hierarchy. An explicit

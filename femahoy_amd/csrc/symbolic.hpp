@@ -1,0 +1,37 @@
+// The fill-reducing order and the factor counts that the reference's crossover reads
+// (crates/fem/src/solve.rs:106-118, 158-170: Crossover::Flops), on the host: nested dissection
+// with coordinate bisection (order.rs:19-179) and the counts of the symbolic analysis
+// (symbolic.rs:41-192: elimination tree, postorder, column counts, fundamental supernodes).
+// Integer work; the results must equal the reference's exactly, so the steps are the reference's.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <vector>
+
+#include "csr_host.hpp"
+
+namespace fem {
+namespace sym {
+
+// order.rs:19-53: item k is the old index of the new node k. `xyz` holds one point per node.
+std::vector<uint64_t> nested_dissection(const csr::Csr& a, const std::vector<std::array<double, 3>>& xyz);
+
+// What Symbolic::analyse (symbolic.rs:41-150) gives the crossover: entries() (symbolic.rs:162-172),
+// flops() (174-183), largest_front() (186-192), and the supernode count.
+struct Counts {
+    uint64_t entries = 0;
+    double flops = 0.0;
+    uint64_t largest_front = 0;
+    uint64_t supernodes = 0;
+};
+Counts analyse(const csr::Csr& a, const std::vector<uint64_t>& order);
+
+// Crossover::Flops (solve.rs:113-117): direct when the factor has at most `entries` entries and
+// (`flat`, or its flops are at most `flops` per unknown).
+inline bool flops_picks_direct(const Counts& c, uint64_t n, double flops, uint64_t entries, bool flat) {
+    return c.entries <= entries && (flat || c.flops <= flops * (double)n);
+}
+
+}  // namespace sym
+}  // namespace fem

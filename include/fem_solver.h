@@ -39,6 +39,12 @@ enum fem_kind {
 /* solve.rs:47-50 Method, plus "let the crossover choose". */
 enum fem_method { FEM_METHOD_AUTO = -1, FEM_METHOD_DIRECT = 0, FEM_METHOD_MULTIGRID = 1 };
 
+/* solve.rs:28-39 Crossover: how FEM_METHOD_AUTO chooses. UNKNOWNS: direct up to `crossover`
+ * unknowns. FLOPS: direct when the factor of the nested dissection order (order.rs:19-53,
+ * symbolic.rs:41-192) has at most `crossover_entries` entries, and at most `crossover_flops` flops
+ * per unknown or `crossover_flat` (solve.rs:106-118, 158-170). */
+enum fem_crossover { FEM_CROSSOVER_UNKNOWNS = 0, FEM_CROSSOVER_FLOPS = 1 };
+
 /* The arithmetic of the preconditioner. The outer PCG (residual recurrence, dots, search
  * direction, operator of the outer iteration, true residual) is FP64 in every mode. */
 enum fem_precision {
@@ -83,6 +89,11 @@ typedef struct fem_options {
     int32_t coarse_galerkin; /* hex8: 1 Galerkin coarse operators (27-point stencils, as the reference's
                                 P^T A P, multigrid.rs:109); 0 the hex8 operator rediscretized with the
                                 mean conductivity of the 8 children */
+    int32_t crossover_kind;     /* enum fem_crossover (default UNKNOWNS) */
+    double crossover_flops;     /* FLOPS: the most factor flops per unknown of a direct solve
+                                   (elastic.rs:263 DIRECT_FLOPS = 1.5e5; infinite: the entries decide) */
+    uint64_t crossover_entries; /* FLOPS: the most factor entries (elastic.rs:269 DIRECT_ENTRIES = 2.5e8) */
+    int32_t crossover_flat;     /* FLOPS: 1 when the cells are flat (direct up to crossover_entries) */
 } fem_options;
 
 /* solve.rs:81-86 Stats first ({iterations, residual}: the recurrence residual of PCG, the true
@@ -102,17 +113,19 @@ typedef struct fem_stats {
 
 typedef struct fem_solver fem_solver;
 
-/* Fills the defaults: tol 1e-10, max_iter 500, AUTO, FP32 V-cycle, crossover 3000, automatic
- * levels, automatic Chebyshev degree on [0.1, 1.1] lmax with 20 power steps, coarse_max 6000, refine 1,
- * Galerkin coarse operators. */
+/* Fills the defaults: tol 1e-10, max_iter 500, AUTO, FP32 V-cycle, crossover UNKNOWNS 3000 (with
+ * the FLOPS values of elastic.rs: 1.5e5 flops per unknown, 2.5e8 entries, not flat), automatic
+ * levels, automatic Chebyshev degree on [0.1, 1.1] lmax with 20 power steps, coarse_max 6000,
+ * refine 1, Galerkin coarse operators. */
 void fem_options_default(fem_options* opt);
 
 /* Replaces Solver::new (crates/fem/src/solve.rs:92-183) for a box model: builds the model, its
  * numbering and free unknowns, and then the direct factor (a dense FP64 LDL^T on the device,
- * up to 20000 unknowns) or the multigrid hierarchy. With FEM_METHOD_AUTO the direct solver is
- * chosen up to `crossover` unknowns, as Crossover::Unknowns (solve.rs:106, 158). Unlike the
- * reference, multigrid also runs on tetrahedra (solve.rs:132-137 refuses it there). Material
- * errors use the texts of hex.rs:79 and hex.rs:93. */
+ * up to 40000 unknowns) or the multigrid hierarchy. With FEM_METHOD_AUTO the crossover chooses as
+ * the reference's does (solve.rs:106-118, 158-170): Crossover::Unknowns up to `crossover`
+ * unknowns, or Crossover::Flops on the counts of the nested-dissection factor of the model's
+ * reduced matrix pattern. Unlike the reference, multigrid also runs on tetrahedra (solve.rs:132-137
+ * refuses it there). Material errors use the texts of hex.rs:79 and hex.rs:93. */
 int fem_solver_new_box(const fem_box* box, const fem_options* opt, fem_solver** out, char* err, size_t errlen);
 
 /* Replaces Solver::solve (solve.rs:199-250) with host vectors: `b` and `x` hold the free
@@ -167,6 +180,16 @@ typedef struct fem_grid_level {
     const uint8_t* comp;     /* 1 per unknown */
 } fem_grid_level;
 
+/* multigrid.rs:67-70 Cells: the cells of the fine level of a displacement model, the lowest grid
+ * point of each cell and its 24 x 24 element matrix (row-major, unknown 3 n + c for corner n in the
+ * order of hex.rs:34 BITS and component c), from which the reference builds the coarse matrices
+ * (multigrid.rs:458-679 by_cells). */
+typedef struct fem_cells {
+    uint64_t count;
+    const uint32_t* origin;  /* 3 per cell */
+    const double* element;   /* 576 per cell */
+} fem_cells;
+
 typedef struct fem_unknowns {
     int32_t kind;              /* enum fem_unknowns_kind */
     fem_grid_level grid;       /* GRID: the level of the matrix's unknowns */
@@ -176,17 +199,27 @@ typedef struct fem_unknowns {
                                   that grid multigrid continues below the linear level (not in the
                                   reference, which factors that level: pmultigrid.rs:46-60) */
     fem_grid_level coarse_grid;
+    const double* xyz;         /* POINTS and QUADRATIC: the coordinates of each unknown (3 per unknown;
+                                  Unknowns::Points(xyz), Unknowns::Quadratic { points }), read by the
+                                  FLOPS crossover's nested dissection; may be NULL otherwise (GRID
+                                  unknowns take theirs from the grid level, solve.rs:143-150) */
+    const fem_cells* cells;    /* Solver::new's `cells` (solve.rs:97): NULL for None */
 } fem_unknowns;
 
-/* Replaces Solver::new(a, unknowns, method, crossover, None) (crates/fem/src/solve.rs:92-183)
+/* Replaces Solver::new(a, unknowns, method, crossover, cells) (crates/fem/src/solve.rs:92-183)
  * for an assembled matrix of the free unknowns (symmetric positive definite, both triangles).
- * The method follows the reference: direct with FEM_METHOD_DIRECT, or with FEM_METHOD_AUTO up to
- * `crossover` unknowns (Crossover::Unknowns); else grid multigrid (GRID) or p-multigrid
- * (QUADRATIC); POINTS with FEM_METHOD_MULTIGRID fails with the text of solve.rs:135. The grid
- * hierarchy is the reference's (multigrid.rs:84-139: coarsen, 329-452, and Galerkin products,
- * sparse.rs:233-292, down to 3000 unknowns; `levels` > 0 stops earlier); its smoother is
- * Chebyshev-Jacobi and its coarsest level the dense LDL^T (up to coarse_max unknowns). The direct
- * solver is the dense LDL^T (up to 20000 unknowns). b and x of fem_solver_solve are the matrix's
+ * The method follows the reference: direct with FEM_METHOD_DIRECT; with FEM_METHOD_AUTO as the
+ * crossover chooses (Crossover::Unknowns up to `crossover` unknowns, or Crossover::Flops from the
+ * nested dissection factor's counts, which need the unknowns' coordinates: the grid level's, or
+ * `xyz`); else grid multigrid (GRID) or p-multigrid (QUADRATIC); POINTS is always direct, and
+ * POINTS with FEM_METHOD_MULTIGRID fails with the text of solve.rs:135. The grid hierarchy is the
+ * reference's (multigrid.rs:84-139: coarsen, 329-452, and Galerkin products, sparse.rs:233-292,
+ * down to 3000 unknowns; `levels` > 0 stops earlier); its smoother is Chebyshev-Jacobi and its
+ * coarsest level the dense LDL^T (up to coarse_max unknowns; a QUADRATIC linear level with no grid
+ * below it up to 40000). The direct solver is the dense LDL^T, up to 40000 unknowns (the
+ * reference's TRANSIENT_DIRECT_LIMIT, solve.rs:24); when the crossover picks it for a larger matrix
+ * the call fails with a message that gives the factor's counts. Cells (the coarse matrices of
+ * multigrid.rs:458-679) are refused with a message. b and x of fem_solver_solve are the matrix's
  * unknowns in its order; fem_solver_rhs and fem_solver_point_grid do not apply. */
 int fem_solver_new_csr(const fem_csr* a, const fem_unknowns* u, const fem_options* opt, fem_solver** out, char* err,
                        size_t errlen);
@@ -215,6 +248,11 @@ int fem_dense_solve_device(int n, const double* d_a, const double* d_b, double* 
 int fem_dense_solve_device_noalloc(int n, const double* d_a, const double* d_b, double* d_x, char* err,
                                    size_t errlen);
 
+/* The counts of the nested-dissection factor that a FLOPS crossover compared (symbolic.rs:162-183
+ * entries() and flops()), when the solver's choice read them. Returns 0 and the counts, or 1 when
+ * the crossover did not analyse the matrix (UNKNOWNS, or a forced method). */
+int fem_solver_crossover_counts(const fem_solver* s, uint64_t* entries, double* flops);
+
 /* ---- the model's integer structures and right side, for the caller and for parity checks ---- */
 
 /* The number of free unknowns (the length of b and x), and of all unknowns. */
@@ -241,7 +279,7 @@ int fem_solver_apply(fem_solver* s, const double* x, double* y, char* err, size_
  * Csr::submatrix(free) (sparse.rs:108-135) of the Csr::pattern of the elements (sparse.rs:22-65),
  * over the free unknowns in reference order, columns increasing, FP64 values of the device's
  * operator. Call with ptr NULL for the entry count (*nnz); then with ptr (free count + 1), idx and
- * val (nnz each). hex8 and tet4 models. */
+ * val (nnz each). hex8, tet4 and tet10 models. */
 int fem_solver_matrix(fem_solver* s, uint64_t* nnz, uint64_t* ptr, uint32_t* idx, double* val, char* err,
                       size_t errlen);
 

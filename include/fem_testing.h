@@ -51,6 +51,14 @@ int fem_test_csr_level_arrays(const fem_csr_levels* h, int l, uint64_t* a_ptr, u
                               uint64_t* p_ptr, uint32_t* p_idx, double* p_val, uint32_t* points, uint8_t* comp);
 void fem_test_csr_levels_free(fem_csr_levels* h);
 
+/* The nested dissection order of a square pattern from the coordinates of its unknowns (3 each;
+ * order.rs:19-53; order_out holds rows values, item k the old index of new unknown k), and the
+ * counts of its symbolic factor (symbolic.rs:41-192: entries(), flops(), largest_front() and the
+ * number of supernodes), by the library's host restatement that the FLOPS crossover uses. No
+ * device is touched. */
+int fem_test_symbolic(const fem_csr* a, const double* xyz, uint64_t* order_out, uint64_t* entries, double* flops,
+                      uint64_t* largest_front, uint64_t* supernodes, char* err, size_t errlen);
+
 #ifdef __cplusplus
 }
 #endif

@@ -147,7 +147,8 @@ def test_assembled_solves_are_bitwise_reproducible_and_report_the_reference_erro
         s.solve(b, cancelled=lambda: True)
 
 
-@pytest.mark.parametrize("kind,n", [("heat_hex8", [8, 6, 5]), ("heat_tet4", [6, 5, 4]), ("elastic_tet4", [6, 4, 3])])
+@pytest.mark.parametrize("kind,n", [("heat_hex8", [8, 6, 5]), ("heat_tet4", [6, 5, 4]), ("elastic_tet4", [6, 4, 3]),
+                                    ("elastic_tet10", [4, 3, 3]), ("elastic_tet10", [5, 2, 3])])
 def test_the_exported_matrix_is_the_reference_reduced_matrix(fem, oracle, kind, n):
     """fem_solver_matrix of a box model against the oracle's Csr::submatrix(free): the same pattern
     bit for bit, the same values to 1e-12; and the assembled boundary on that matrix solves as the
@@ -162,7 +163,8 @@ def test_the_exported_matrix_is_the_reference_reduced_matrix(fem, oracle, kind, 
     else:
         nt = 6 * int(np.prod(n))
         e, nu = rng.uniform(1.0, 2.0, nt), rng.uniform(0.2, 0.35, nt)
-        pr = oracle.Problem.elastic(4, n, length, e, nu, clamped=["x0"], body=(0, 0, -1))
+        pr = oracle.Problem.elastic(10 if kind == "elastic_tet10" else 4, n, length, e, nu, clamped=["x0"],
+                                    body=(0, 0, -1))
         box = fem.Box(kind, n, length, e, nu, fixed={"x0": True}, body=(0, 0, -1))
     s = fem.Solver(box, method="multigrid")
     ptr, idx, val = s.matrix()
@@ -209,3 +211,116 @@ def test_unknowns_numbered_far_apart_solve_as_the_grid_numbering_does(fem, oracl
     assert rel(back, x0) < 1e-9
     v = np.random.default_rng(13).standard_normal(m)
     assert rel(s1.apply(v[perm]), (a @ v)[perm]) < 1e-14
+
+
+def test_p_multigrid_with_a_large_linear_level_and_no_grid(fem, oracle):
+    """The reference's tet10 caller (elastic_tet.rs:294-313) passes coordinates only, so the linear
+    level has no grid below it and the reference factors it whole (pmultigrid.rs:46-60). Here it
+    holds 12,168 unknowns, above the default coarse_max of 6,000: the device factors it dense. The
+    solution against the oracle's own p-multigrid PCG (multifrontal Cholesky on the linear level) to
+    1e-12."""
+    n, length = [24, 12, 12], [2.0, 1.0, 1.0]
+    pr, p = quadratic(oracle, n, length, 6)
+    assert p[3] == 12168
+    ptr, idx, val = pr.reduced().arrays()
+    s = fem.CsrSolver(ptr, idx, val, "quadratic", p=p, method="multigrid", precision="fp32", tol=1e-12)
+    assert s.level_info(1)[0] == p[3]
+    b = pr.rhs()
+    x, st = s.solve(b)
+    print(st, flush=True)
+    assert st.levels == 2 and st.coarse_unknowns == p[3]
+    assert st.true_residual < 1e-11
+    xo, it, res = oracle.Solver(pr, "multigrid").solve(b, tol=1e-12)
+    print("oracle p-multigrid iterations", it, flush=True)
+    assert rel(x, xo) < TOL_X
+
+
+def test_a_direct_solve_of_about_34k_unknowns_matches_oracle_direct(fem, oracle):
+    """Direct solves up to the reference's TRANSIENT_DIRECT_LIMIT of 40,000 unknowns (solve.rs:24):
+    33,759 heat unknowns (32^3 cells) forced direct through the assembled boundary, against the
+    oracle's multifrontal Cholesky."""
+    n, length = [32, 32, 32], [1.0, 1.0, 1.0]
+    pr, planes, pts = heat(oracle, n, length, 21)
+    ptr, idx, val = pr.reduced().arrays()
+    assert len(ptr) - 1 == 33759
+    s = fem.CsrSolver(ptr, idx, val, "grid", fem.Grid(planes, pts, np.zeros(len(pts), np.uint8)), method="direct")
+    assert s.method() == "direct"
+    b = pr.rhs()
+    x, st = s.solve(b)
+    print(st, flush=True)
+    xd, _, _ = oracle.Solver(pr, "direct").solve(b)
+    assert st.iterations == 0 and st.residual < 1e-12
+    assert rel(x, xd) < 1e-10
+
+
+def test_a_direct_solve_beyond_the_limit_is_refused(fem, oracle):
+    n, length = [34, 34, 34], [1.0, 1.0, 1.0]
+    pr, planes, pts = heat(oracle, n, length, 22)
+    ptr, idx, val = pr.reduced().arrays()
+    with pytest.raises(fem.FemError, match="^the direct solve holds at most 40000 unknowns; this matrix has 40425$"):
+        fem.CsrSolver(ptr, idx, val, "grid", fem.Grid(planes, pts, np.zeros(len(pts), np.uint8)), method="direct")
+
+
+def _oracle_counts(oracle, a, xyz):
+    s = oracle.Symbolic(a, oracle.nested_dissection(a, xyz))
+    return s.entries(), s.flops()
+
+
+@pytest.mark.parametrize("n,flat,want", [(24, False, "direct"), (30, False, "multigrid"), (30, True, "direct")])
+def test_the_flops_crossover_routes_an_assembled_grid_as_the_reference(fem, oracle, n, flat, want):
+    """Crossover::Flops { flops: 1.5e5, entries: 2.5e8, flat } (elastic.rs:541 with DIRECT_FLOPS and
+    DIRECT_ENTRIES): the nested-dissection factor of 24^3 heat cells costs 8.5e4 flops per unknown
+    (direct), of 30^3 cells 1.7e5 (multigrid, unless the cells are flat). The counts the library
+    compared equal the oracle's, the method follows solve.rs:158-170, and the solve matches."""
+    length = [1.0, 1.0, 1.0]
+    pr, planes, pts = heat(oracle, [n, n, n], length, 4)
+    a = pr.reduced()
+    ptr, idx, val = a.arrays()
+    xyz = np.stack([planes[d][pts[:, d]] for d in range(3)], axis=1)
+    e, f = _oracle_counts(oracle, a, xyz)
+    cross = fem.Flops(1.5e5, 250_000_000, flat)
+    assert (e <= cross.entries and (flat or f <= cross.flops * a.n)) == (want == "direct")
+    s = fem.CsrSolver(ptr, idx, val, "grid", fem.Grid(planes, pts, np.zeros(len(pts), np.uint8)), crossover=cross,
+                      tol=1e-12)
+    assert s.method() == want
+    assert s.crossover_counts() == (e, f)
+    x, st = s.solve(pr.rhs())
+    xd, _, _ = oracle.Solver(pr, "direct").solve(pr.rhs())
+    assert rel(x, xd) < TOL_X
+
+
+@pytest.mark.parametrize("n,cross,want", [
+    ([16, 16, 1], dict(flops=float("inf"), entries=250_000_000, flat=True), "direct"),
+    ([16, 16, 1], dict(flops=float("inf"), entries=1_000_000, flat=True), "multigrid"),
+    ([8, 8, 8], dict(flops=1.5e5, entries=250_000_000, flat=False), "multigrid"),
+])
+def test_the_flops_crossover_routes_a_tet10_box_as_the_reference(fem, oracle, n, cross, want):
+    """The tet10 caller's crossover (elastic_tet.rs:300-312): on flat cells Flops { flops: inf,
+    entries, flat: true } (direct up to `entries` factor entries), else the flop test. A flat plate of
+    16 x 16 x 1 cells 4 times thinner than wide (9,504 unknowns, 1.7e6 factor entries) is direct, or
+    multigrid with a limit of 1e6 entries; a cube of 8^3 cells (2.3e5 flops per unknown) is multigrid.
+    The counts equal the oracle's; each solve matches the oracle's direct solve."""
+    nt = 6 * int(np.prod(n))
+    length = [float(n[0]), float(n[1]), 0.25 * n[2] if n[2] == 1 else float(n[2])]
+    rng = np.random.default_rng(17)
+    e_, nu = rng.uniform(1.0, 2.0, nt), rng.uniform(0.2, 0.35, nt)
+    pr = oracle.Problem.elastic(10, n, length, e_, nu, clamped=["x0"], body=(0, 0, -1))
+    box = fem.Box("elastic_tet10", n, length, e_, nu, fixed={"x0": True}, body=(0, 0, -1))
+    a = pr.reduced()
+    e, f = _oracle_counts(oracle, a, pr.points()[pr.free() // 3])
+    c = fem.Flops(**cross)
+    assert (e <= c.entries and (c.flat or f <= c.flops * a.n)) == (want == "direct")
+    s = fem.Solver(box, crossover=c, tol=1e-12)
+    assert s.method() == want
+    assert s.crossover_counts() == (e, f)
+    x, st = s.solve(s.rhs())
+    xd, _, _ = oracle.Solver(pr, "direct").solve(pr.rhs())
+    assert rel(x, xd) < TOL_X
+
+
+def test_cells_are_refused_with_a_message(fem, oracle):
+    pr, planes, pts = heat(oracle, [6, 5, 4], [1.0, 1.0, 1.0], 2)
+    ptr, idx, val = pr.reduced().arrays()
+    with pytest.raises(fem.FemError, match="^coarse matrices from cells"):
+        fem.CsrSolver(ptr, idx, val, "grid", fem.Grid(planes, pts, np.zeros(len(pts), np.uint8)),
+                      cells=(np.zeros((1, 3)), np.zeros((1, 576))))
