@@ -143,10 +143,12 @@ def serve_requests(F, n_solves, seed):
     """The serving stream: refinement pairs (h, h/2) of heat tet4 or elasticity tet10 boxes (50/50),
     the coarse member's size log-uniform so that both members lie in about 5k-500k DOF. Boxes of
     2m x m x m cells, m the integer whose size is nearest the drawn one (tet4 heat: m in 13..31,
-    fine 2m; tet10: m in 4..10, fine 2m). Materials seeded."""
+    fine 2m; tet10: m in 5..10, fine 2m: 4,900 and 3,888 free DOF at m = 13 and m = 4 fall below
+    the stream's 5k floor, so heat starts at m = 14, 6,075 free DOF, and tet10 at m = 5, 7,260).
+    Materials seeded."""
     rng = np.random.default_rng(seed)
-    heat_m = list(range(13, 32))
-    elas_m = list(range(4, 11))
+    heat_m = list(range(14, 32))
+    elas_m = list(range(5, 11))
 
     def dofs(kind, m):
         if kind == "heat_tet4":
@@ -241,10 +243,12 @@ def measured_traffic(n):
     return {"bytes_per_launch": t["bytes_per_launch"], "source": os.path.relpath(path, ROOT)}
 
 
-def cpu_baseline(n_cpu):
+def cpu_baseline(n_cpu, reference_tol=None):
     """The oracle (the reference method restated in C++: assembled CSR, line Gauss-Seidel GMG with
     Galerkin coarse operators and a sparse Cholesky coarsest level, FP64 PCG), one thread, on a
-    smaller box of the same workload; solve time only (setup reported apart)."""
+    box of the same workload; solve time to 1e-8 only (setup reported apart). With reference_tol,
+    the oracle's PCG then continues from that solution to reference_tol, and its solution is
+    returned beside the baseline (the FP64 oracle solution of the headline)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
 
@@ -255,10 +259,15 @@ def cpu_baseline(n_cpu):
     t1 = time.perf_counter()
     s = O.Solver(pr, "multigrid")
     t2 = time.perf_counter()
-    _, it, res = s.solve(pr.rhs(), tol=1e-8)
+    x, it, res = s.solve(pr.rhs(), tol=1e-8)
     t3 = time.perf_counter()
     nfree = pr.sizes.n_free
-    return {
+    x_ref = None
+    if reference_tol is not None:
+        x_ref, it_ref, res_ref = s.solve(pr.rhs(), x0=x, tol=reference_tol)
+        log(f"  oracle FP64 MG-PCG continued to {res_ref:.1e} ({it_ref} more iterations, "
+            f"{time.perf_counter() - t3:.1f} s)")
+    return x_ref, {
         "value": nfree / (t3 - t2) / 1e6,
         "unit": "MDOF/s",
         "cores": 1,
@@ -268,6 +277,31 @@ def cpu_baseline(n_cpu):
                    f"assembly {t1 - t0:.2f} s and setup {t2 - t1:.2f} s excluded; host {os.uname().machine}, "
                    f"nproc {os.cpu_count()}"),
     }
+
+
+def tet10_smoother(prof):
+    """The fused tet10 Chebyshev step (tile kernel and boundary pass) of a kernel timing."""
+    gbs = prof["smoother_bytes"] / prof["smoother_s"] / 1e9
+    return {"kernel": "k_t10_tile<float> + k_t10_bound<float> (fused tet10 operator and Chebyshev update, FP32)",
+            "bytes_per_launch": int(prof["smoother_bytes"]), "us_per_launch": round(prof["smoother_s"] * 1e6, 2),
+            "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def streaming_point(F, n):
+    """The headline's dominant kernel (the FP32 hex8 Chebyshev step) on a heat box of n^3 cells,
+    whose level-0 arrays (about 28 B per node) are several times the 256 MB last-level cache: its
+    rate streams from HBM."""
+    box, _ = heat_box(F, n)
+    log(f"hbm streaming point: heat hex8 {n}^3, the finest Chebyshev step timed alone")
+    s = F.Solver(box, method="multigrid", precision="fp32", levels=5, cheb_degree=3)
+    tk = s.time_kernels(50)
+    sec = tk.smoother_seconds / tk.smoother_launches
+    gbs = tk.smoother_bytes / sec / 1e9
+    log(f"  {tk.smoother_bytes / 1e6:.0f} MB in {sec * 1e6:.1f} us: {gbs:.0f} GB/s")
+    return {"workload": f"heat hex8 {n}^3 (the same field law), the finest FP32 Chebyshev step",
+            "bytes_per_launch": int(tk.smoother_bytes), "us_per_launch": round(sec * 1e6, 2),
+            "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "measured": "HIP events around 50 back-to-back launches (fem_solver_time_kernels)"}
 
 
 def run_workload(F, hip, box, levels, args, label, degree, auto_leg=False):
@@ -344,11 +378,16 @@ def run_workload(F, hip, box, levels, args, label, degree, auto_leg=False):
         f"{st_ref.iterations} it")
 
     def leg(lat, st, err, setup):
-        return {"mdof_s": round(nfree / float(np.mean(lat)) / 1e6, 2), "ms_mean": round(1e3 * float(np.mean(lat)), 3),
+        mean = float(np.mean(lat))
+        return {"mdof_s": round(nfree / mean / 1e6, 2), "ms_mean": round(1e3 * mean, 3),
                 "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
                 "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3), "iterations": st.iterations,
                 "residual": st.residual, "true_residual": st.true_residual, "restarts": st.restarts,
-                "error_vs_fp64": err, "setup_s": round(setup, 3)}
+                "error_vs_fp64": err, "setup_s": round(setup, 3),
+                # the algorithmic bytes of every launch of one solve (the library's ledger) over the
+                # mean solve time, against the 8 TB/s HBM peak
+                "solve_bytes": int(st.hbm_bytes), "solve_gbs": round(st.hbm_bytes / mean / 1e9, 1),
+                "solve_frac": round(st.hbm_bytes / mean / 1e9 / HBM_PEAK_GBS, 4)}
 
     res = {"free_dof": nfree, "levels": [v[0] for v in lv], "mixed": leg(lat_m, st_m, err_mixed, t1 - t0),
            "mixed_bf16": leg(lat_h, st_h, err_bf16, t3 - t2),
@@ -364,6 +403,9 @@ def run_workload(F, hip, box, levels, args, label, degree, auto_leg=False):
                                             "us_per_launch": round(prof_h["smoother_s"] * 1e6, 2),
                                             "achieved_gbs": round(prof_h["smoother_bytes"] / prof_h["smoother_s"] / 1e9, 1)}
     res["_x_ref"] = x_ref
+    res["_x"] = {"mixed": x_mixed, "mixed_bf16": x_bf16, "fp64": x_fp64}
+    if auto is not None:
+        res["_x"]["mixed_automatic_degree"] = auto[2]
     return res, prof, lat_m
 
 
@@ -416,7 +458,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--levels", type=int, default=4)
-    ap.add_argument("--cpu-n", type=int, default=96)
+    ap.add_argument("--cpu-n", type=int, default=128,
+                    help="cells per axis of the oracle's heat box (at --n it also gives the error against the "
+                         "oracle's FP64 solution to 1e-13)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--degree", type=int, default=None,
                     help="Chebyshev degree on every level (0: automatic); default: 3 for the heat box as its "
@@ -424,6 +468,11 @@ def main():
     ap.add_argument("--coarse", choices=["galerkin", "mean"], default="galerkin")
     ap.add_argument("--elastic-n", type=int, default=60,
                     help="cells along x of the tet10 elasticity box reported beside the headline (0: skip)")
+    ap.add_argument("--target-n", type=int, default=70,
+                    help="cells along x of the 2M-DOF tet10 target box (70: 2,117,220 unknowns; 0: skip)")
+    ap.add_argument("--stream-n", type=int, default=256,
+                    help="cells per axis of the heat box whose finest smoother step is timed past the last-level "
+                         "cache (0: skip)")
     ap.add_argument("--inclusions-n", type=int, default=80,
                     help="cells along x of the nearly incompressible inclusion box (0: skip)")
     ap.add_argument("--inclusions", type=int, default=200, help="spheres in the inclusion box")
@@ -460,6 +509,7 @@ def main():
                                      auto_leg=args.degree is None)
     value = heat["mixed"]["mdof_s"]
     x_ref = heat.pop("_x_ref")
+    heat_x = heat.pop("_x")
     if not args.no_assembled:
         heat["assembled_csr"] = assembled_leg(F, hip, F.Solver(box, method="multigrid", levels=args.levels), box,
                                               x_ref, args, vdeg)
@@ -477,11 +527,20 @@ def main():
                     "us_per_launch": round(prof["smoother_s"] * 1e6, 2), "launches": int(prof["smoother_n"]),
                     "measured": "HIP events on the solver stream around 100 back-to-back launches of the kernel "
                                 "on the solver's own level-0 vectors (fem_solver_time_kernels)",
+                    "working_set": ("cache-resident: the launch's 60 MB of level-0 arrays stay in the 256 MB "
+                                    "last-level cache across back-to-back launches, so this is not an HBM-streaming "
+                                    "rate; see hbm_streaming for the same kernel on a working set past the cache, "
+                                    "and solve_frac for the whole solve"),
+                    "solve_frac": heat["mixed"]["solve_frac"], "solve_bytes": heat["mixed"]["solve_bytes"],
+                    "solve_gbs": heat["mixed"]["solve_gbs"],
                     "traffic_source": traffic["source"] if traffic else None,
                     "outer_apply": {"kernel": "k_march_hex8<double, MARCH_PCG_Q> (FP64 operator, q = A p, p.q partials)",
                                     "bytes_per_launch": int(prof["outer_bytes"]),
                                     "us_per_launch": round(prof["outer_s"] * 1e6, 2),
                                     "achieved": round(prof["outer_bytes"] / prof["outer_s"] / 1e9, 1)}}
+
+    if roofline is not None and args.stream_n > 0:
+        roofline["hbm_streaming"] = streaming_point(F, args.stream_n)
 
     elastic = None
     if args.elastic_n > 0:
@@ -489,14 +548,35 @@ def main():
         ebox = elastic_box(F, en)
         elastic, eprof, _ = run_workload(F, hip, ebox, 0, args, f"elasticity tet10 {en}x{en // 2}x{en // 2}", vdeg)
         elastic.pop("_x_ref")
+        elastic.pop("_x")
         elastic["workload"] = (f"linear elasticity, tet10 on {en}x{en // 2}x{en // 2} unit cells (6 Kuhn tets per cell), "
                                "E=1, nu=0.3, clamped at x=0, body force (0,0,-1); p-MG tet10->tet4 then geometric tet4 "
                                f"levels, dense FP64 LDLT coarse, {cheb_desc(vdeg, True)}, tol 1e-8")
         if eprof:
-            elastic["finest_smoother"] = {"kernel": "k_cheb<VectorOp<Tet10Op<float>>> (+ its element pass)",
-                                          "bytes_per_launch": int(eprof["smoother_bytes"]),
-                                          "us_per_launch": round(eprof["smoother_s"] * 1e6, 2),
-                                          "achieved_gbs": round(eprof["smoother_bytes"] / eprof["smoother_s"] / 1e9, 1)}
+            elastic["finest_smoother"] = tet10_smoother(eprof)
+
+    target = None
+    if args.target_n > 0:
+        tn = args.target_n
+        tbox = elastic_box(F, tn)
+        target, tprof, _ = run_workload(F, hip, tbox, 0, args, f"elasticity tet10 {tn}x{tn // 2}x{tn // 2}", vdeg)
+        target.pop("_x_ref")
+        target.pop("_x")
+        target["workload"] = (f"the 2M-DOF tet10 target: linear elasticity, tet10 on {tn}x{tn // 2}x{tn // 2} unit cells, "
+                              "E=1, nu=0.3, clamped at x=0, body force (0,0,-1); p-MG tet10->tet4 then geometric tet4 "
+                              f"levels, dense FP64 LDLT coarse, {cheb_desc(vdeg, True)}, tol 1e-8")
+        if tprof:
+            target["finest_smoother"] = tet10_smoother(tprof)
+        frac = target["finest_smoother"]["frac"] if tprof else None
+        target["targets"] = {
+            "speedup_mixed_vs_fp64": {"want": 2.0, "got": target["speedup_mixed_vs_fp64"],
+                                      "met": target["speedup_mixed_vs_fp64"] >= 2.0},
+            "fused_step_roofline_frac": {"want": 0.6, "got": frac, "met": frac is not None and frac >= 0.6},
+            "solve_roofline_frac": {"want": 0.6, "got": target["mixed"]["solve_frac"],
+                                    "met": target["mixed"]["solve_frac"] >= 0.6},
+            "error_vs_fp64": {"want": 1e-8, "got": target["mixed"]["error_vs_fp64"],
+                              "met": target["mixed"]["error_vs_fp64"] <= 1e-8},
+        }
 
     inclusions = None
     if args.inclusions_n > 0:
@@ -504,6 +584,7 @@ def main():
         ibox, inside = F.inclusion_box(ni, args.inclusions, 7)
         inclusions, _, _ = run_workload(F, hip, ibox, 0, args, f"inclusions tet10 {ni}x{ni // 2}x{ni // 2}", vdeg)
         inclusions.pop("_x_ref")
+        inclusions.pop("_x")
         inclusions["workload"] = (f"nearly incompressible heterogeneous elasticity, tet10 on {ni}x{ni // 2}x{ni // 2} unit "
                                   f"cells, nu=0.49 matrix (E=1) with {args.inclusions} seeded spheres (radius 0.02-0.05 "
                                   "of the box length, E=1e3, nu=0.3; a tet is inside when its centroid is), clamped at "
@@ -517,8 +598,18 @@ def main():
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline: oracle on heat hex8 {args.cpu_n}^3, 1 thread")
-        cpu = cpu_baseline(args.cpu_n)
+        x_oracle, cpu = cpu_baseline(args.cpu_n, reference_tol=1e-13 if args.cpu_n == n else None)
         log("  " + cpu["sample"])
+        if x_oracle is not None:
+            # every heat leg against the oracle's FP64 MG-PCG solution to 1e-13 (the same vector
+            # layout: the free unknowns in the reference order)
+            on = float(np.linalg.norm(x_oracle))
+            for name, xv in heat_x.items():
+                heat[name]["error_vs_oracle"] = float(np.linalg.norm(xv - x_oracle) / on)
+            heat["oracle_reference"] = ("the oracle's FP64 MG-PCG (line Gauss-Seidel GMG, Galerkin levels, sparse "
+                                        "Cholesky coarsest level) on the same box to 1e-13 relative residual")
+            heat["error_fp64_reference_vs_oracle"] = float(np.linalg.norm(x_ref - x_oracle) / on)
+            log("  error vs the oracle: " + ", ".join(f"{k} {heat[k]['error_vs_oracle']:.2e}" for k in heat_x))
 
     config = {"workload": (f"heat hex8 {n}^3 unit cube, k = exp(N(0,1)) per cell (seed 1234), T=0 at x=0 and T=1 at "
                            f"x=1, {args.levels}-level GMG ({args.coarse} coarse operators), {cheb_desc(hdeg, False)}, "
@@ -526,6 +617,8 @@ def main():
     config.update(heat)
     if elastic:
         config["elasticity_tet10"] = elastic
+    if target:
+        config["elasticity_tet10_2m"] = target
     if inclusions:
         config["inclusions_tet10"] = inclusions
     if serve:

@@ -93,6 +93,7 @@ class _Stats(C.Structure):
         ("restarts", C.c_uint64),
         ("method", C.c_int32),
         ("precision", C.c_int32),
+        ("hbm_bytes", C.c_uint64),
     ]
 
 
@@ -297,6 +298,7 @@ class Stats:
     restarts: int
     method: str
     precision: str
+    hbm_bytes: int = 0  # algorithmic bytes of the solve's launches (box solvers)
 
 
 class Model:
@@ -672,8 +674,8 @@ def csr_levels(ptr, idx, val, grid: Grid, coarsest: int = 3000, max_levels: int 
 def _stats(st: _Stats) -> Stats:
     return Stats(int(st.iterations), float(st.residual), float(st.true_residual), float(st.seconds),
                  float(st.setup_seconds), int(st.levels), int(st.coarse_unknowns), int(st.restarts),
-                 ("direct", "multigrid")[st.method] if st.method in (0, 1) else "?",
-                 {v: k for k, v in PRECISIONS.items()}.get(st.precision, "?"))
+                  ("direct", "multigrid")[st.method] if st.method in (0, 1) else "?",
+                 {v: k for k, v in PRECISIONS.items()}.get(st.precision, "?"), int(st.hbm_bytes))
 
 
 class DeviceArray:

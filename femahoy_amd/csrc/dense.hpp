@@ -77,6 +77,12 @@ class DenseCoarse {
     const DArray<int32_t>& dense_of() const { return dense_of_; }
 
     int n() const { return n_; }
+    // Algorithmic bytes of one solve in storage of vb bytes: the inverse's rows, the dense right
+    // side and the map in, x out; with `gather` also the gather (map, b in, the dense right side out).
+    uint64_t solve_bytes(size_t vb, bool gather) const {
+        const uint64_t n = (uint64_t)n_, ld = (uint64_t)ld_;
+        return n * ld * vb + ld * vb + n * 8 + n * vb + (gather ? n * 8 + 2 * n * vb : 0);
+    }
     const DArray<int64_t>& map() const { return map_; }
     DArray<int64_t>& map() { return map_; }
     double factor_seconds() const { return factor_seconds_; }

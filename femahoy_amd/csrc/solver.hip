@@ -62,6 +62,9 @@ struct Level {
     double lmax = 0.0;
     std::vector<T> c1, c2;     // Chebyshev step coefficients
     size_t nodes() const { return g.nodes(); }
+    // stored bytes of D^-1 and of the operator's coefficients as the smoothing kernels read them
+    size_t invd_bytes() const { return bf16 ? invd16.bytes() : invd.bytes(); }
+    size_t coef_bytes() const { return (bf16 && op != OpKind::Stencil27) ? coef16.bytes() : coef.bytes(); }
     // the hex8 operator through the column-marching kernels (31-bit byte offsets)
     bool march() const { return op == OpKind::Hex8K && nodes() * sizeof(T) < MARCH_MAX_BYTES; }
     size_t dofs() const { return g.nodes() * comps; }
@@ -373,6 +376,12 @@ class Hierarchy {
     // its output) into march_blocks partials at rz_part, when fuses_rz().
     const double* rz_r = nullptr;
     double* rz_part = nullptr;
+    // Algorithmic bytes of the launches (every array a launch reads or writes, once, at its stored
+    // precision), added to *ledger as they are launched.
+    uint64_t* ledger = nullptr;
+    void spend(uint64_t b) const {
+        if (ledger) *ledger += b;
+    }
 
     bool fuses_rz() const {
         return rz_part && lv.size() > 1 && lv[0].march() && lv[0].nodes() * sizeof(double) < MARCH_MAX_BYTES;
@@ -396,11 +405,14 @@ class Hierarchy {
             // first_done on the dense level: the restriction wrote its dense right side
             if (first_done) coarse.solve_rhs<T>(L.x(), s);
             else coarse.solve<T>(L.b.data(), L.x(), s);
+            spend(coarse.solve_bytes(sizeof(T), !first_done));
             return;
         }
         const size_t n = L.dofs();
+        const size_t vb = sizeof(T);
         // pre-smoothing from x = 0
         if (!first_done) {
+            spend(3 * n * vb + L.invd_bytes());
             with_invd(L, [&](auto iv) {
                 launch(k_cheb_first<T, elem_t<decltype(iv)>>, n, s, n, (const T*)L.b.data(), iv, L.x(), L.d.data(),
                        L.c2[0]);
@@ -423,6 +435,7 @@ class Hierarchy {
         } else {
             op_residual(L, (const T*)L.x(), (const T*)L.b.data(), L.r.data(), s);
         }
+        spend(3 * n * vb + L.coef_bytes());
         Level<T>& Cc = lv[l + 1];
         const bool smoothed = l + 2 < lv.size();  // the next level smooths (it is not the dense one)
         transfer(L, Cc, true, s, smoothed);
@@ -440,6 +453,8 @@ class Hierarchy {
 
     // d = c1 d + c2 D^-1 (b - A x), x2 = x + d; with rz (a march level), also the r.z partials.
     void cheb_step(Level<T>& L, T c1, T c2, hipStream_t s, bool rz = false) {
+        // x, b, d in; d, x out; D^-1 and the coefficients; with rz the outer r (FP64) in
+        spend(4 * L.dofs() * sizeof(T) + L.invd_bytes() + L.coef_bytes() + (rz ? L.dofs() * sizeof(double) : 0));
         if (L.march()) {
             with_march(L, [&](auto m) {
                 m.x = L.x();
@@ -482,10 +497,14 @@ class Hierarchy {
     // its buffer 0 (the buffer its cycle starts from); else x_f += P x_c.
     void transfer(Level<T>& F, Level<T>& Cc, bool restrict_, hipStream_t s, bool first_step = false) {
         const int C = F.comps;
+        const size_t vb = sizeof(T);
         if (restrict_) {
             const size_t n = Cc.nodes();
             // into the dense level (not smoothed): straight to its dense right side
             const bool dense = !first_step;
+            // r_f in, b_c out; the coarse first step's D^-1 in and x, d out; or the dense index map in
+            spend(F.dofs() * vb + Cc.dofs() * vb +
+                  (first_step ? Cc.invd_bytes() + 2 * Cc.dofs() * vb : Cc.dofs() * sizeof(int32_t)));
             const int32_t* of = dense ? coarse.dense_of().data() : nullptr;
             T* db = dense ? coarse.rhs<T>() : nullptr;
             with_invd(Cc, [&](auto iv) {
@@ -505,6 +524,7 @@ class Hierarchy {
             });
         } else {
             const size_t n = F.nodes();
+            spend(Cc.dofs() * vb + 2 * F.dofs() * vb);  // x_c in, x_f in and out
             if (F.kuhn) {
                 if (C == 1) launch((k_prolong_add<T, 1, true>), n, s, F.g, Cc.g, (const T*)Cc.x(), F.x());
                 else launch((k_prolong_add<T, 3, true>), n, s, F.g, Cc.g, (const T*)Cc.x(), F.x());
@@ -589,12 +609,14 @@ class Solver {
                 build_hierarchy(*h64_);
                 h64_->rz_r = r_.data();
                 h64_->rz_part = partA_.data();
+                h64_->ledger = &ledger_;
             } else {
                 h32_ = std::make_unique<Hierarchy<float>>();
                 h32_->bf16 = o.precision == FEM_PREC_BF16;
                 build_hierarchy(*h32_);
                 h32_->rz_r = r_.data();
                 h32_->rz_part = partA_.data();
+                h32_->ledger = &ledger_;
                 r32_.resize(n0_);
             }
         }
@@ -879,6 +901,9 @@ class Solver {
         run_.ctx = ctx;
         run_.d_out = d_x;
         FEM_HIP(hipEventRecord(ev_t0_, s_));
+        ledger_ = 0;
+        // two fills of the grid vectors, two scatters of the free vectors (map, source, target)
+        ledger_ += 2 * n0_ * sizeof(double) + 2 * nf * 3 * sizeof(double);
         launch(k_fill<double>, n0_, s_, n0_, b_.data(), 0.0);
         launch(k_fill<double>, n0_, s_, n0_, x_.data(), 0.0);
         launch(k_scatter, nf, s_, nf, (const int64_t*)free_grid_.data(), d_b, b_.data());
@@ -950,6 +975,8 @@ class Solver {
                     const unsigned nbv = rblocks(n);
                     PcgScalars* sc = scal_.data();
                     precondition();
+                    // r.z (r, z in, partials out), the scalars, p = z (z in, p and x in and out)
+                    ledger_ += 2 * n * sizeof(double) + 2 * nbv * sizeof(double) + 5 * n * sizeof(double);
                     launch_blocks((k_pcg_rz<float>), nbv, s_, n, (const float*)nullptr, z_.data(),
                                   (const double*)r_.data(), partA_.data());
                     launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr,
@@ -1113,6 +1140,9 @@ class Solver {
     int64_t device_bytes_ = 0;  // peak device bytes of the construction
     bool analysed_ = false;     // a FLOPS crossover analysed the reduced matrix (counts_)
     sym::Counts counts_;
+    // Algorithmic bytes of the launches of the running solve (the hierarchies add theirs), and of
+    // one captured PCG iteration (each replay of its graph adds them).
+    uint64_t ledger_ = 0, iter_bytes_ = 0;
     size_t n0_ = 0;
     int parts_ = 0;
     DArray<double> x_, b_, r_, z_, p_, q_, tmp_, partA_, partB_, red_, fb_, fx_;
@@ -1485,11 +1515,13 @@ class Solver {
     void precondition() {
         if (h32_) {
             Level<float>& L = h32_->lv[0];
+            ledger_ += 2 * n0_ * (sizeof(double) + sizeof(float));  // r to FP32 and x back to FP64
             launch((k_convert<double, float>), n0_, s_, n0_, (const double*)r_.data(), L.b.data());
             h32_->vcycle(s_);
             launch((k_convert<float, double>), n0_, s_, n0_, (const float*)L.x(), z_.data());
         } else {
             Level<double>& L = h64_->lv[0];
+            ledger_ += 4 * n0_ * sizeof(double);  // the two copies
             FEM_HIP(hipMemcpyAsync(L.b.data(), r_.data(), n0_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
             h64_->vcycle(s_);
             FEM_HIP(hipMemcpyAsync(z_.data(), L.x(), n0_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
@@ -1500,6 +1532,8 @@ class Solver {
     // returns their count.
     unsigned residual64(const double* x, double* r) {
         Level<double>& L = lv64_[0];
+        // x, b in, r out, the coefficients (and the partials of b.b and r.r, counted by the caller)
+        ledger_ += 3 * n0_ * sizeof(double) + L.coef.bytes();
         if (L.op == OpKind::Tet10) {
             op_residual(L, x, (const double*)b_.data(), r, s_);
             const unsigned pb = std::min<unsigned>(nblocks(n0_), 1024);
@@ -1528,6 +1562,7 @@ class Solver {
     double true_residual() {
         // |b - A x| / |b| in FP64 over the free unknowns (fixed rows are 0 in both)
         const unsigned nb = residual64(x_.data(), tmp_.data());
+        ledger_ += 4 * (uint64_t)nb * sizeof(double);  // the partials, written and summed
         launch_reduce(k_sum, s_, (const double*)partA_.data(), (int)nb, red_.data());
         launch_reduce(k_sum, s_, (const double*)partB_.data(), (int)nb, red_.data() + 1);
         double v[2];
@@ -1545,6 +1580,7 @@ class Solver {
     // q = A p in FP64 with the block partials of p.q; returns the number of partials.
     unsigned outer_apply() {
         Level<double>& L0 = lv64_[0];
+        ledger_ += 2 * n0_ * sizeof(double) + L0.coef.bytes();  // p in, q out, the FP64 coefficients
         if (L0.march()) {
             MarchArgs<double> m = march_args(L0);
             m.x = p_.data();
@@ -1618,6 +1654,9 @@ class Solver {
         const size_t n = n0_;
         Level<T>& L = H.lv[0];
         const bool smooths = H.lv.size() > 1;
+        // r, q in; r out; the V-cycle's right side out (FP32); with its first step D^-1 in, x, d out
+        ledger_ += 3 * n * sizeof(double) + n * sizeof(T) + (smooths ? L.invd_bytes() + 2 * n * sizeof(T) : 0) +
+                   rblocks(n) * sizeof(double);
         with_invd(L, [&](auto iv) {
             using S = elem_t<decltype(iv)>;
             launch_blocks((k_pcg_update<T, S>), rblocks(n), s_, n, (const PcgScalars*)scal_.data(), r_.data(),
@@ -1636,11 +1675,13 @@ class Solver {
         prof_.begin(1, s_);
         const unsigned nq = outer_apply();
         prof_.end(s_);
+        ledger_ += 2 * (uint64_t)nq * sizeof(double);  // the p.q partials, written and summed
         launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr, (int)nq, (int)nq, 1);
         // the update, r.r and the stopping test, and the V-cycle's first step on level 0 (when
         // level 0 smooths); then the V-cycle, then r.z and beta
         // r.z: summed by level 0's last smoothing step where it fuses it, else here
         unsigned nrz = nbv;
+        const size_t zb = h32_ ? sizeof(float) : sizeof(double);
         if (h32_) {
             update_and_vcycle(*h32_);
             if (h32_->fuses_rz())
@@ -1656,6 +1697,10 @@ class Solver {
                 launch_blocks((k_pcg_rz<double>), nbv, s_, n, (const double*)h64_->lv[0].x(), (double*)nullptr,
                               (const double*)r_.data(), partA_.data());
         }
+        const bool fused_rz = h32_ ? h32_->fuses_rz() : h64_->fuses_rz();
+        if (!fused_rz) ledger_ += n * zb + n * sizeof(double);  // z and r in
+        // the r.z and r.r partials, summed; then z in, p and x in and out
+        ledger_ += ((uint64_t)nrz + nbv) * sizeof(double) + (uint64_t)nrz * sizeof(double) + n * zb + 4 * n * sizeof(double);
         launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)partB_.data(), (int)nrz, (int)nbv,
                       5);
         // z straight from the V-cycle's level-0 iterate (kept until the next V-cycle)
@@ -1675,11 +1720,15 @@ class Solver {
         }
         if (!graph_exec_) {
             hipGraph_t graph = nullptr;
+            const uint64_t before = ledger_;
             FEM_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
             iteration_body();
             FEM_HIP(hipStreamEndCapture(s_, &graph));
+            iter_bytes_ = ledger_ - before;
             FEM_HIP(hipGraphInstantiate(&graph_exec_, graph, nullptr, nullptr, 0));
             FEM_HIP(hipGraphDestroy(graph));
+        } else {
+            ledger_ += iter_bytes_;
         }
         FEM_HIP(hipGraphLaunch(graph_exec_, s_));
     }
@@ -1697,6 +1746,7 @@ class Solver {
         const double limit = (double)(opt_.max_iter - std::min<uint64_t>(opt_.max_iter, run_.total_it));
         hipLaunchKernelGGL(k_pcg_config, dim3(1), dim3(64), 0, s_, scal_.data(), opt_.tol, limit);
         const unsigned nbn = residual64(x_.data(), r_.data());
+        ledger_ += 4 * (uint64_t)nbn * sizeof(double);  // the b.b and r.r partials, written and summed
         launch_reduce(k_pcg_scalars, s_, scal_.data(), (const double*)partA_.data(), (const double*)partB_.data(),
                       (int)nbn, (int)nbn, 0);
         read_back();
@@ -1759,6 +1809,8 @@ class Solver {
             run_.out.coarse_unknowns = h32_ ? h32_->coarse.n() : h64_->coarse.n();
         }
         launch(k_gather, nf, s_, nf, (const int64_t*)free_grid_.data(), (const double*)x_.data(), run_.d_out);
+        ledger_ += 3 * nf * sizeof(double);
+        run_.out.hbm_bytes = ledger_;
         FEM_HIP(hipEventRecord(ev_t1_, s_));
         FEM_HIP(hipEventSynchronize(ev_t1_));
         float ms = 0.f;

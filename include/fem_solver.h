@@ -109,6 +109,9 @@ typedef struct fem_stats {
     uint64_t restarts;       /* refinement restarts */
     int32_t method;          /* enum fem_method (never AUTO) */
     int32_t precision;       /* enum fem_precision */
+    uint64_t hbm_bytes;      /* algorithmic bytes of the solve's launches: every array a launch reads or
+                                writes, once per launch at its stored precision (box solvers; 0 for
+                                the assembled boundary and the direct method) */
 } fem_stats;
 
 typedef struct fem_solver fem_solver;
