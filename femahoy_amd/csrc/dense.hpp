@@ -77,6 +77,9 @@ class DenseCoarse {
     const DArray<int32_t>& dense_of() const { return dense_of_; }
 
     int n() const { return n_; }
+    // The device arrays of the per-cycle solve (the inverse and the dense right side).
+    const void* inverse_data() const { return fp32_ ? (const void*)inv32_.data() : (const void*)inv64_.data(); }
+    const void* rhs_data() const { return fp32_ ? (const void*)bc32_.data() : (const void*)bc64_.data(); }
     // Algorithmic bytes of one solve in storage of vb bytes: the inverse's rows, the dense right
     // side and the map in, x out; with `gather` also the gather (map, b in, the dense right side out).
     uint64_t solve_bytes(size_t vb, bool gather) const {

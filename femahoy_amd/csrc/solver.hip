@@ -1143,6 +1143,7 @@ class Solver {
     // Algorithmic bytes of the launches of the running solve (the hierarchies add theirs), and of
     // one captured PCG iteration (each replay of its graph adds them).
     uint64_t ledger_ = 0, iter_bytes_ = 0;
+    std::vector<const void*> graph_bufs_;  // graph_buffers() at the capture of graph_exec_
     size_t n0_ = 0;
     int parts_ = 0;
     DArray<double> x_, b_, r_, z_, p_, q_, tmp_, partA_, partB_, red_, fb_, fx_;
@@ -1712,13 +1713,41 @@ class Solver {
                    x_.data(), 0);
     }
 
+    // Every device allocation that a captured iteration's nodes read or write, and the pinned
+    // record of the readback, by address: they are made in the constructor and freed after the
+    // graph in the destructor, so the list must stay the same from capture to every replay.
+    std::vector<const void*> graph_buffers() const {
+        std::vector<const void*> v = {x_.data(), b_.data(), r_.data(), z_.data(), p_.data(), q_.data(), tmp_.data(),
+                                      partA_.data(), partB_.data(), red_.data(), scal_.data(), r32_.data(), pinned_};
+        for (const Level<double>& L : lv64_)
+            for (const void* q : {(const void*)L.coef.data(), (const void*)L.P.data(), (const void*)L.blist.data()}) v.push_back(q);
+        auto levels = [&](const auto& H) {
+            for (const auto& L : H.lv)
+                for (const void* q : {(const void*)L.coef.data(), (const void*)L.coef16.data(), (const void*)L.invd.data(),
+                                      (const void*)L.invd16.data(), (const void*)L.b.data(), (const void*)L.xa.data(),
+                                      (const void*)L.xb.data(), (const void*)L.d.data(), (const void*)L.r.data(),
+                                      (const void*)L.P.data(), (const void*)L.blist.data()})
+                    v.push_back(q);
+            v.push_back(H.coarse.map().data());
+            v.push_back(H.coarse.dense_of().data());
+            v.push_back(H.coarse.inverse_data());
+            v.push_back(H.coarse.rhs_data());
+        };
+        if (h32_) levels(*h32_);
+        if (h64_) levels(*h64_);
+        return v;
+    }
+
     void iterate() {
         if (!use_graph_ || prof_.on) {  // HIP cannot time events recorded by graph nodes
             iteration_body();
             prof_.flush_eager();
             return;
         }
+        if (graph_exec_ && graph_buffers() != graph_bufs_)
+            throw Error("internal: a buffer of the captured PCG iteration was reallocated after its capture");
         if (!graph_exec_) {
+            graph_bufs_ = graph_buffers();
             hipGraph_t graph = nullptr;
             const uint64_t before = ledger_;
             FEM_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
