@@ -195,8 +195,9 @@ def run_stream(F, reqs, arrivals, budget, max_active):
         phases.append((info.admit_time - info.submit_time, info.start_time - info.admit_time,
                        info.finish_time - info.start_time))
     makespan = time.perf_counter() - t0
+    mem = srv.memory()
     srv.close()
-    return makespan, np.array(lat), its, np.array(phases)
+    return makespan, np.array(lat), its, np.array(phases), mem
 
 
 def serving(F, n_solves, seed):
@@ -204,7 +205,7 @@ def serving(F, n_solves, seed):
     total_dof = sum(r[4] for r in reqs)
     budget, max_active = 64 << 30, 16
     log(f"serving: {len(reqs)} solves ({total_dof} DOF), capacity run (all submitted at once)")
-    cap_span, cap_lat, _, _ = run_stream(F, reqs, None, budget, max_active)
+    cap_span, cap_lat, _, _, _ = run_stream(F, reqs, None, budget, max_active)
     capacity = len(reqs) / cap_span
     rate = 0.8 * capacity
     rng = np.random.default_rng(seed + 1)
@@ -212,7 +213,7 @@ def serving(F, n_solves, seed):
     arrivals = [pair_t[r[0]] for r in reqs]
     log(f"  capacity {capacity:.1f} solves/s ({total_dof / cap_span / 1e6:.1f} MDOF/s); Poisson arrivals at "
         f"{rate:.1f} solves/s")
-    span, lat, its, ph = run_stream(F, reqs, arrivals, budget, max_active)
+    span, lat, its, ph, mem = run_stream(F, reqs, arrivals, budget, max_active)
     out = {
         "workload": (f"{len(reqs)} solves in refinement pairs (h, h/2), heat tet4 or elasticity tet10 (50/50), "
                      f"{min(r[4] for r in reqs)}-{max(r[4] for r in reqs)} DOF, Poisson pair arrivals at 80% of the "
@@ -227,6 +228,9 @@ def serving(F, n_solves, seed):
         "mean_queue_ms": round(1e3 * float(ph[:, 0].mean()), 2), "mean_setup_ms": round(1e3 * float(ph[:, 1].mean()), 2),
         "mean_solve_ms": round(1e3 * float(ph[:, 2].mean()), 2),
         "latency": "submit to solution on the host, including setup (model, hierarchy, coarse factor)",
+        "device_memory": {"arena_bytes": mem[0], "peak_bytes": mem[1], "allocations_outside_arena": mem[2],
+                          "note": "every solve's device memory comes from one arena reserved at fem_server_new: "
+                                  "no hipMalloc or hipFree while solves run"},
     }
     log(f"  stream: {out['mdof_s']} MDOF/s, latency p50 {out['p50_ms']} ms, p99 {out['p99_ms']} ms")
     return out

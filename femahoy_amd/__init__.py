@@ -151,7 +151,7 @@ EXPORTS = (
     "fem_solver_set_profile", "fem_solver_profile", "fem_solver_time_kernels",
     "fem_server_new", "fem_server_submit", "fem_server_wait", "fem_server_cancel", "fem_server_free",
     "fem_solver_new_csr", "fem_solver_matrix", "fem_dense_solve_device",
-    "fem_dense_solve_device_noalloc", "fem_solver_crossover_counts",
+    "fem_dense_solve_device_noalloc", "fem_solver_crossover_counts", "fem_server_memory",
 )
 
 _lib = None
@@ -211,6 +211,7 @@ def lib() -> C.CDLL:
                                       C.POINTER(RequestInfo), ERR, SZ]),
         "fem_server_cancel": (C.c_int, [P, C.c_uint64]),
         "fem_server_free": (None, [P]),
+        "fem_server_memory": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "fem_solver_matrix": (C.c_int, [P, C.POINTER(C.c_uint64), P, P, P, ERR, SZ]),
         "fem_dense_solve_device": (C.c_int, [C.c_int, P, P, P, ERR, SZ]),
         "fem_dense_solve_device_noalloc": (C.c_int, [C.c_int, P, P, P, ERR, SZ]),
@@ -788,6 +789,12 @@ class Server:
 
     def cancel(self, rid: int) -> bool:
         return lib().fem_server_cancel(self.h, rid) == 0
+
+    def memory(self):
+        """(arena bytes, peak bytes in use, allocations made outside the arena)."""
+        a, p, m = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        lib().fem_server_memory(self.h, C.byref(a), C.byref(p), C.byref(m))
+        return int(a.value), int(p.value), int(m.value)
 
 
 

@@ -5,6 +5,8 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -31,6 +33,81 @@ struct AllocMeter {
 };
 inline thread_local AllocMeter* alloc_meter = nullptr;
 
+// One device allocation made up front and handed out in pieces: the serving loop's memory, so
+// that no hipMalloc or hipFree (which in ROCm waits for the whole device) runs while solves run.
+// Pieces are 256-byte aligned; a free piece merges with its free neighbours. Thread-safe.
+class DeviceArena {
+   public:
+    explicit DeviceArena(size_t bytes) : size_((bytes + 255) / 256 * 256) {
+        if (size_) FEM_HIP(hipMalloc((void**)&base_, size_));
+        if (size_) free_[0] = size_;
+    }
+    ~DeviceArena() {
+        if (base_) (void)hipFree(base_);
+    }
+    DeviceArena(const DeviceArena&) = delete;
+    DeviceArena& operator=(const DeviceArena&) = delete;
+
+    // The smallest free piece that holds `bytes` (best fit), or nullptr when none does.
+    void* take(size_t bytes) {
+        const size_t need = (bytes + 255) / 256 * 256;
+        std::lock_guard<std::mutex> g(m_);
+        auto best = free_.end();
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+            if (it->second >= need && (best == free_.end() || it->second < best->second)) best = it;
+        if (best == free_.end()) {
+            ++misses_;
+            return nullptr;
+        }
+        const size_t off = best->first, len = best->second;
+        free_.erase(best);
+        if (len > need) free_[off + need] = len - need;
+        used_ += need;
+        if (used_ > peak_) peak_ = used_;
+        return base_ + off;
+    }
+    void give(void* p, size_t bytes) {
+        const size_t need = (bytes + 255) / 256 * 256;
+        std::lock_guard<std::mutex> g(m_);
+        size_t off = (size_t)(static_cast<char*>(p) - base_), len = need;
+        auto next = free_.lower_bound(off);
+        if (next != free_.end() && off + len == next->first) {
+            len += next->second;
+            next = free_.erase(next);
+        }
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                off = prev->first;
+                len += prev->second;
+                free_.erase(prev);
+            }
+        }
+        free_[off] = len;
+        used_ -= need;
+    }
+    size_t size() const { return size_; }
+    size_t peak() {
+        std::lock_guard<std::mutex> g(m_);
+        return peak_;
+    }
+    // Requests that found no piece and went to hipMalloc instead.
+    uint64_t misses() {
+        std::lock_guard<std::mutex> g(m_);
+        return misses_;
+    }
+
+   private:
+    std::mutex m_;
+    char* base_ = nullptr;
+    size_t size_ = 0, used_ = 0, peak_ = 0;
+    uint64_t misses_ = 0;
+    std::map<size_t, size_t> free_;  // offset -> length of each free piece
+};
+// While set on a thread, the Buffers allocated there come from this arena (and go back to it when
+// freed, from any thread); a request the arena cannot hold falls back to hipMalloc.
+inline thread_local DeviceArena* alloc_arena = nullptr;
+
 class Buffer {
    public:
     Buffer() = default;
@@ -38,23 +115,30 @@ class Buffer {
     ~Buffer() { release(); }
     Buffer(const Buffer&) = delete;
     Buffer& operator=(const Buffer&) = delete;
-    Buffer(Buffer&& o) noexcept : p_(o.p_), n_(o.n_) {
+    Buffer(Buffer&& o) noexcept : p_(o.p_), n_(o.n_), arena_(o.arena_) {
         o.p_ = nullptr;
         o.n_ = 0;
+        o.arena_ = nullptr;
     }
     Buffer& operator=(Buffer&& o) noexcept {
         if (this != &o) {
             release();
             p_ = o.p_;
             n_ = o.n_;
+            arena_ = o.arena_;
             o.p_ = nullptr;
             o.n_ = 0;
+            o.arena_ = nullptr;
         }
         return *this;
     }
     void alloc(size_t bytes) {
         release();
-        if (bytes) FEM_HIP(hipMalloc(&p_, bytes));
+        if (bytes && alloc_arena) {
+            p_ = alloc_arena->take(bytes);
+            if (p_) arena_ = alloc_arena;
+        }
+        if (bytes && !p_) FEM_HIP(hipMalloc(&p_, bytes));
         n_ = bytes;
         if (alloc_meter) {
             alloc_meter->live += (int64_t)bytes;
@@ -62,10 +146,14 @@ class Buffer {
         }
     }
     void release() {
-        if (p_) (void)hipFree(p_);
+        if (p_) {
+            if (arena_) arena_->give(p_, n_);
+            else (void)hipFree(p_);
+        }
         if (alloc_meter) alloc_meter->live -= (int64_t)n_;
         p_ = nullptr;
         n_ = 0;
+        arena_ = nullptr;
     }
     template <class T>
     T* as() const {
@@ -77,6 +165,7 @@ class Buffer {
    private:
     void* p_ = nullptr;
     size_t n_ = 0;
+    DeviceArena* arena_ = nullptr;  // the arena the piece came from, or none (hipMalloc)
 };
 
 // A typed device array.

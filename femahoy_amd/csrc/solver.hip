@@ -1875,7 +1875,8 @@ struct ServeRequest {
     std::unique_ptr<Solver> solver;
     std::vector<double> x;
     fem_stats st{};
-    std::string err;
+    std::string err;          // written only under the server's lock
+    std::string setup_err;    // the setup thread's failure, handed to the loop with the request
     bool done = false;
     uint64_t footprint = 0;
     int64_t device_bytes = 0;
@@ -1884,11 +1885,17 @@ struct ServeRequest {
 
 class Server {
    public:
+    // The device memory of every admitted solve (its vectors, hierarchy, coarse factor and the
+    // factor's temporaries) comes from one arena of `budget` bytes reserved here.
     Server(uint64_t budget, int max_active, int device, int setup_threads)
         : budget_(budget), max_active_(max_active > 0 ? max_active : 1), device_(device), t0_(clock::now()) {
+        FEM_HIP(hipSetDevice(device_));
+        arena_ = std::make_unique<DeviceArena>(budget_);
         loop_ = std::thread([this] { loop(); });
         for (int k = 0; k < std::max(1, setup_threads); ++k) setup_.emplace_back([this] { setup_loop(); });
     }
+    // Stops the loop and the setup threads; a solve that had not finished ends with an error, and
+    // the destructor returns once no thread is still inside wait().
     ~Server() {
         {
             std::lock_guard<std::mutex> g(m_);
@@ -1898,7 +1905,19 @@ class Server {
         setup_cv_.notify_all();
         loop_.join();
         for (auto& t : setup_) t.join();
+        std::unique_lock<std::mutex> g(m_);
+        for (auto& kv : all_)
+            if (!kv.second->done) {
+                kv.second->err = "the server stopped before the solve finished";
+                kv.second->done = true;
+            }
+        done_cv_.notify_all();
+        quiet_cv_.wait(g, [&] { return waiters_ == 0; });
     }
+
+    uint64_t arena_bytes() const { return arena_->size(); }
+    uint64_t arena_peak() { return arena_->peak(); }
+    uint64_t arena_misses() { return arena_->misses(); }
 
     uint64_t submit(const BoxInput& in, const Options& o, const double* b) {
         auto r = std::make_shared<ServeRequest>();
@@ -1926,6 +1945,13 @@ class Server {
     int wait(uint64_t id, double timeout_s, double* x, uint64_t n, fem_stats* st, fem_request_info* info,
              std::string& err) {
         std::unique_lock<std::mutex> g(m_);
+        ++waiters_;
+        struct Leave {
+            Server* s;
+            ~Leave() {
+                if (--s->waiters_ == 0) s->quiet_cv_.notify_all();
+            }
+        } leave{this};
         auto it = all_.find(id);
         if (it == all_.end()) {
             err = "no such request";
@@ -1987,21 +2013,31 @@ class Server {
         if (!to_setup_.empty()) setup_cv_.notify_one();
     }
 
-    void complete(const std::shared_ptr<ServeRequest>& r) {
-        // the result leaves the device before the solver (and its memory) goes
+    // The end of a request on the loop thread: its result leaves the device before the solver
+    // (whose memory goes back to the arena) does, then it is published under the lock, unless a
+    // failure of the whole server has already ended it.
+    void complete(const std::shared_ptr<ServeRequest>& r, std::string err) {
+        fem_stats st{};
+        std::vector<double> x;
+        if (err.empty()) err = r->setup_err;
         if (r->solver) {
-            r->st = r->solver->run_stats();
-            if (r->err.empty()) r->err = r->solver->run_error();
+            st = r->solver->run_stats();
+            if (err.empty()) err = r->solver->run_error();
             const size_t nf = r->solver->free_count_host();
-            r->x.resize(nf);
-            FEM_HIP(hipMemcpyAsync(r->x.data(), r->solver->stage_x(), nf * sizeof(double), hipMemcpyDeviceToHost,
+            x.resize(nf);
+            FEM_HIP(hipMemcpyAsync(x.data(), r->solver->stage_x(), nf * sizeof(double), hipMemcpyDeviceToHost,
                                    r->solver->stream()));
             FEM_HIP(hipStreamSynchronize(r->solver->stream()));
             r->solver.reset();
         }
         std::lock_guard<std::mutex> g(m_);
         r->t_finish = now();
-        r->done = true;
+        if (!r->done) {
+            r->st = st;
+            r->x = std::move(x);
+            r->err = err;
+            r->done = true;
+        }
         used_ -= r->footprint;
         --admitted_;
         admit();
@@ -2034,6 +2070,7 @@ class Server {
 
     void setup_body() {
         FEM_HIP(hipSetDevice(device_));
+        alloc_arena = arena_.get();  // every solver this thread builds takes its memory from the arena
         for (;;) {
             std::shared_ptr<ServeRequest> r;
             {
@@ -2060,7 +2097,7 @@ class Server {
                 r->t_start = now();
                 s.start_run(s.stage_b(), s.stage_x(), cancel_fn, r.get());
             } catch (const std::exception& e) {
-                r->err = e.what();
+                r->setup_err = e.what();
                 r->solver.reset();
             }
             std::lock_guard<std::mutex> g(m_);
@@ -2099,16 +2136,17 @@ class Server {
             for (size_t k = 0; k < active.size();) {
                 auto& r = active[k];
                 bool finished = !r->solver;  // setup failed
+                std::string err;
                 if (!finished) {
                     try {
                         finished = r->solver->advance(false);
                     } catch (const std::exception& e) {
-                        r->err = e.what();
+                        err = e.what();
                         finished = true;
                     }
                 }
                 if (finished) {
-                    complete(r);
+                    complete(r, err);
                     active[k] = active.back();
                     active.pop_back();
                     progressed = true;
@@ -2124,8 +2162,10 @@ class Server {
     const int max_active_;
     const int device_;
     const clock::time_point t0_;
+    std::unique_ptr<DeviceArena> arena_;
     std::mutex m_;
-    std::condition_variable cv_, setup_cv_, done_cv_;
+    std::condition_variable cv_, setup_cv_, done_cv_, quiet_cv_;
+    int waiters_ = 0;  // threads inside wait()
     bool stop_ = false;
     std::string fatal_;
     uint64_t next_id_ = 0, used_ = 0;
@@ -2443,6 +2483,14 @@ int fem_server_wait(fem_server* srv, uint64_t id, double timeout_s, double* x, u
 }
 
 int fem_server_cancel(fem_server* srv, uint64_t id) { return srv && srv->impl->cancel(id) ? 0 : 1; }
+
+int fem_server_memory(fem_server* srv, uint64_t* arena_bytes, uint64_t* peak_bytes, uint64_t* misses) {
+    if (!srv) return 1;
+    if (arena_bytes) *arena_bytes = srv->impl->arena_bytes();
+    if (peak_bytes) *peak_bytes = srv->impl->arena_peak();
+    if (misses) *misses = srv->impl->arena_misses();
+    return 0;
+}
 
 void fem_server_free(fem_server* srv) { delete srv; }
 

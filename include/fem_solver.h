@@ -361,7 +361,16 @@ int fem_server_wait(fem_server* srv, uint64_t id, double timeout_s, double* x, u
 /* Asks a queued or running solve to stop; it ends with the error "cancelled". */
 int fem_server_cancel(fem_server* srv, uint64_t id);
 
-/* Stops the loop (after the running solves' current step) and frees every solve. */
+/* The server's device memory: every admitted solve takes its memory (vectors, hierarchy, coarse
+ * factor and its temporaries) from one arena of hbm_budget bytes that fem_server_new reserves, so
+ * that no device allocation or free (each of which waits for the whole device) runs while solves
+ * run. Gives the arena's size, the most of it in use at once, and the allocations that found no
+ * room in it and were made apart (0 when the footprint estimates hold). */
+int fem_server_memory(fem_server* srv, uint64_t* arena_bytes, uint64_t* peak_bytes, uint64_t* misses);
+
+/* Stops the loop (after the running solves' current step) and frees every solve. A solve that had
+ * not finished ends with the error "the server stopped before the solve finished"; threads inside
+ * fem_server_wait return with it before this call returns. */
 void fem_server_free(fem_server* srv);
 
 /* ---- the model alone, on the host (no device is touched) ---- */

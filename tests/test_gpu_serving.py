@@ -53,6 +53,10 @@ def test_each_result_is_the_same_alone_and_in_the_mix(fem, oracle):
             assert np.linalg.norm(x - xa) <= 1e-12 * np.linalg.norm(xa)
             assert np.array_equal(x, xa)
             assert info.device_bytes <= info.footprint
+        # every solve took its memory from the arena reserved at the start: no device allocation
+        # ran beside the running solves
+        arena, peak, misses = srv.memory()
+        assert arena >= 8 << 30 and 0 < peak <= arena and misses == 0
     finally:
         srv.close()
     # and each agrees with the oracle's direct solve
@@ -117,3 +121,36 @@ def test_a_solve_beyond_the_budget_is_refused(fem):
             srv.submit(make_box(fem, kind, n, kw), **OPTS)
     finally:
         srv.close()
+
+
+def test_freeing_the_server_ends_a_waiting_request_and_its_waiter(fem):
+    """A waiter blocked on an unfinished request returns with the server's stop message when another
+    thread frees the server. One solve at a time: request A's setup (a forced direct factor of
+    27,869 unknowns) outlasts the 0.3 s before the free, so request B is still queued then."""
+    import threading
+    import time
+
+    n = [30, 30, 30]
+    k = np.exp(np.random.default_rng(3).standard_normal(int(np.prod(n))))
+    big = fem.Box("heat_hex8", n, [1.0, 1.0, 1.0], k, fixed={"x0": 0.0, "x1": 1.0})
+    kind, m, kw = specs()[0]
+    small = make_box(fem, kind, m, kw)
+    srv = fem.Server(hbm_budget=48 << 30, max_active=1)
+    srv.submit(big, method="direct")
+    rid = srv.submit(small, **OPTS)
+    got = {}
+
+    def waiter():
+        try:
+            srv.wait(rid, fem.Model(small).free_list().size, timeout=-1)
+            got["result"] = "finished"
+        except fem.FemError as e:
+            got["result"] = str(e)
+
+    t = threading.Thread(target=waiter)
+    t.start()
+    time.sleep(0.3)
+    srv.close()
+    t.join(timeout=120)
+    assert not t.is_alive()
+    assert got["result"] == "the server stopped before the solve finished"
