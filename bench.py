@@ -455,6 +455,40 @@ def assembled_leg(F, hip, solver, box, x_ref, args, degree):
                         f"{cheb_desc(degree, True)}, dense LDLT coarse, tol 1e-8"}
 
 
+FP64_MFMA_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix-core peak
+
+
+def dense_factor_leg(F, hip, n, seed=5):
+    """The dense FP64 LDL^T alone (the coarse level's and the direct method's factor, through
+    fem_dense_solve_device): a random SPD matrix of n unknowns, the call timed from the host
+    (factor, explicit inverse and a refined solve; the factor and inverse are n^3 flops, the
+    solves n^2). Returns the timing and the achieved FP64 TFLOP/s against the matrix-core peak."""
+    rng = np.random.default_rng(seed)
+    m = rng.standard_normal((n, 16))
+    a = np.diag(2.0 + rng.uniform(0.0, 1.0, n))
+    a += 0.1 * (m @ m.T) / 16
+    b = rng.standard_normal(n)
+    da, db, dx = F.DeviceArray(a), F.DeviceArray(b), F.DeviceArray(np.zeros(n))
+    err = C.create_string_buffer(512)
+    times = []
+    for rep_ in range(3):
+        hip.sync()
+        t0 = time.perf_counter()
+        if F.lib().fem_dense_solve_device(n, da.ptr, db.ptr, dx.ptr, err, 512):
+            raise RuntimeError(err.value.decode())
+        hip.sync()
+        times.append(time.perf_counter() - t0)
+    x = dx.get()
+    resid = float(np.linalg.norm(b - a @ x) / np.linalg.norm(b))
+    np_ = (n + 63) // 64 * 64
+    flops = 3 * np_ ** 3 / 3.0  # the factor (n^3 / 3) and the two products of the inverse (n^3 / 3 each)
+    t = min(times)
+    log(f"dense LDL^T n={n}: {t * 1e3:.1f} ms, {flops / t / 1e12:.2f} TFLOP/s, residual {resid:.1e}")
+    return {"n": n, "seconds": round(t, 5), "flops": flops, "tflops": round(flops / t / 1e12, 3),
+            "frac_of_fp64_mfma_peak": round(flops / t / 1e12 / FP64_MFMA_PEAK_TFLOPS, 4), "residual": resid,
+            "note": "host time of fem_dense_solve_device (factor, explicit inverse, refined solve), best of 3"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -482,6 +516,9 @@ def main():
     ap.add_argument("--inclusions", type=int, default=200, help="spheres in the inclusion box")
     ap.add_argument("--serve-n", type=int, default=256, help="solves of the serving stream (0: skip)")
     ap.add_argument("--no-assembled", action="store_true", help="skip the assembled (CSR) boundary leg of the heat box")
+    ap.add_argument("--dense", type=str, default="4335,20000",
+                    help="sizes of the dense LDL^T factor timed alone (comma-separated; empty: skip)")
+    ap.add_argument("--dense-only", action="store_true", help="time the dense factors of --dense and stop")
     ap.add_argument("--calibrate", action="store_true",
                     help="run one plain 256 MiB device copy first (the yardstick of HBM byte counters)")
     args = ap.parse_args()
@@ -503,6 +540,11 @@ def main():
         log(f"calibration: one copy kernel of {cal} bytes read + {cal} bytes written")
         hip.free(d_c0)
         hip.free(d_c1)
+
+    dense = [dense_factor_leg(F, hip, int(v)) for v in args.dense.split(",") if v.strip()]
+    if args.dense_only:
+        print(json.dumps({"dense_factor": dense}), flush=True)
+        return
 
     # the heat configuration names Chebyshev(3)-Jacobi; the elasticity ones name no degree
     hdeg = 3 if args.degree is None else args.degree
@@ -627,6 +669,8 @@ def main():
         config["inclusions_tet10"] = inclusions
     if serve:
         config["serving"] = serve
+    if dense:
+        config["dense_factor"] = dense
     out = {
         "metric": METRIC,
         "value": value,
