@@ -261,6 +261,72 @@ void* orc_problem_elastic(int order, const int* n, const double* len, const doub
         return nullptr;
     }
 }
+// Box-grid hex elasticity: E and nu per cell (elastic.rs:287-563).
+void* orc_problem_elastic_hex(const int* n, const double* len, const double* e, const double* nu, const int* clamped,
+                              const double* body, char* err, size_t errlen) {
+    try {
+        ElasticHexSpec s;
+        s.box = box_of(n, len);
+        const size_t nc = (size_t)n[0] * n[1] * n[2];
+        s.e.assign(e, e + nc);
+        s.nu.assign(nu, nu + nc);
+        for (int f = 0; f < 6; ++f) s.clamped[f] = clamped[f] != 0;
+        s.body = {body[0], body[1], body[2]};
+        return new Problem(elastic_hex8_box(s));
+    } catch (const std::exception& ex) {
+        set_err(err, errlen, ex.what());
+        return nullptr;
+    }
+}
+// The cells of a hex elasticity problem: each cell's lowest grid point (3) and matrix (576).
+size_t orc_problem_cell_count(void* h) { return static_cast<Problem*>(h)->cell_matrices.origin.size(); }
+void orc_problem_cell_matrices(void* h, uint32_t* origin, double* k) {
+    const Problem* p = static_cast<Problem*>(h);
+    for (size_t c = 0; c < p->cell_matrices.origin.size(); ++c) {
+        for (int d = 0; d < 3; ++d) origin[3 * c + d] = p->cell_matrices.origin[c][d];
+        std::copy(p->cell_matrices.k[c].begin(), p->cell_matrices.k[c].end(), k + 576 * c);
+    }
+}
+// hex.rs:199-279: the condensed 24 x 24 element of a box cell of an isotropic material.
+void orc_hex_stiffness(const double* size, double e, double nu, double* out, char* err, size_t errlen) {
+    try {
+        const auto k = hex_stiffness({size[0], size[1], size[2]}, Orthotropic::isotropic(e, nu).stiffness());
+        std::copy(k.begin(), k.end(), out);
+    } catch (const std::exception& ex) {
+        set_err(err, errlen, ex.what());
+    }
+}
+// One coarsen and by_cells step (multigrid.rs:96-125) on a hex elasticity problem's reduced
+// matrix, its grid level and its cells, with `all_free` the whole box free (no support): the coarse
+// matrix (a Csr handle) and the coarse points and components (3 and 1 per coarse unknown).
+void* orc_by_cells_step(void* h, uint32_t* coarse_points, uint8_t* coarse_comp, size_t* coarse_n) {
+    const Problem* p = static_cast<Problem*>(h);
+    const Csr a = p->k.submatrix(p->free);
+    auto c = coarsen(p->unknowns.level, a);
+    if (!c) return nullptr;
+    ByCells bc = by_cells(p->unknowns.level, p->cell_matrices, c->coarse, c->kept);
+    for (size_t i = 0; i < c->coarse.points.size(); ++i) {
+        for (int d = 0; d < 3; ++d) coarse_points[3 * i + d] = c->coarse.points[i][d];
+        coarse_comp[i] = c->coarse.comp[i];
+    }
+    *coarse_n = c->coarse.points.size();
+    return new Csr(std::move(bc.a));
+}
+// Solver::new on a problem with its cells given (multigrid.rs:84-139 by_cells levels).
+void* orc_solver_new_cells(void* h, int method, size_t crossover_unknowns, char* err, size_t errlen) {
+    try {
+        const Problem* p = static_cast<Problem*>(h);
+        std::optional<Method> m;
+        if (method == 0) m = Method::Direct;
+        if (method == 1) m = Method::Multigrid;
+        Crossover c;
+        c.unknowns = crossover_unknowns;
+        return new Solver(Solver::make(p->k.submatrix(p->free), p->unknowns, m, c, &p->cell_matrices));
+    } catch (const std::exception& ex) {
+        set_err(err, errlen, ex.what());
+        return nullptr;
+    }
+}
 // sizes: n_points, comps, n_unknowns, n_free, npe, n_cells, corners, n_edges
 void orc_problem_sizes(void* h, size_t* out) {
     const Problem* p = static_cast<Problem*>(h);

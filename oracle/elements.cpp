@@ -248,4 +248,113 @@ Quadratic Quadratic::make(const std::vector<Vec3>& points, const std::vector<std
     return q;
 }
 
+
+// ------------------------------------------------------------------ hex.rs:146-260 (Hex::new, k)
+namespace {
+// hex.rs:146-161: the gradients of the 8 corner functions and the 3 modes at xi.
+std::array<Vec3, 11> hex_gradients(const Vec3& size, const Vec3& xi) {
+    std::array<Vec3, 11> g{};
+    for (usize n = 0; n < 8; ++n) {
+        double s[3], f[3];
+        for (usize d = 0; d < 3; ++d) {
+            s[d] = HEX_BITS[n][d] == 1 ? 1.0 : -1.0;
+            f[d] = 0.5 * (1.0 + s[d] * xi[d]);
+        }
+        for (usize d = 0; d < 3; ++d) {
+            const usize d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+            g[n][d] = s[d] / size[d] * f[d1] * f[d2];
+        }
+    }
+    for (usize d = 0; d < 3; ++d) g[8 + d][d] = -4.0 * xi[d] / size[d];
+    return g;
+}
+
+// hex.rs:364-382
+std::array<std::array<double, 9>, 9> cholesky9(const std::array<std::array<double, 9>, 9>& a) {
+    std::array<std::array<double, 9>, 9> l{};
+    for (usize j = 0; j < 9; ++j) {
+        double d = a[j][j];
+        for (usize k = 0; k < j; ++k) d -= l[j][k] * l[j][k];
+        d = std::sqrt(d);
+        l[j][j] = d;
+        for (usize i = j + 1; i < 9; ++i) {
+            double v = a[i][j];
+            for (usize k = 0; k < j; ++k) v -= l[i][k] * l[j][k];
+            l[i][j] = v / d;
+        }
+    }
+    return l;
+}
+
+// hex.rs:384-398
+std::array<double, 9> solve9(const std::array<std::array<double, 9>, 9>& l, std::array<double, 9> b) {
+    for (usize i = 0; i < 9; ++i) {
+        for (usize k = 0; k < i; ++k) b[i] -= l[i][k] * b[k];
+        b[i] /= l[i][i];
+    }
+    for (usize i = 9; i-- > 0;) {
+        for (usize k = i + 1; k < 9; ++k) b[i] -= l[k][i] * b[k];
+        b[i] /= l[i][i];
+    }
+    return b;
+}
+}  // namespace
+
+std::array<double, 8> hex_shape(const Vec3& xi) {  // hex.rs:164-178
+    std::array<double, 8> out{};
+    for (usize n = 0; n < 8; ++n) {
+        double p = 1.0;
+        for (usize d = 0; d < 3; ++d) p *= HEX_BITS[n][d] == 1 ? 0.5 * (1.0 + xi[d]) : 0.5 * (1.0 - xi[d]);
+        out[n] = p;
+    }
+    return out;
+}
+
+std::array<double, 576> hex_stiffness(const Vec3& size, const Stiffness& dm) {
+    static const usize SHEAR_AX[3][2] = {{1, 2}, {0, 2}, {0, 1}};  // hex.rs:141
+    std::array<std::array<double, 33>, 33> k{};
+    const double jac = size[0] * size[1] * size[2] / 8.0;
+    const double p = 1.0 / std::sqrt(3.0);
+    for (usize q = 0; q < 8; ++q) {  // hex.rs:193-196: the 2-point Gauss points in BITS order
+        const Vec3 xi{HEX_BITS[q][0] == 1 ? p : -p, HEX_BITS[q][1] == 1 ? p : -p, HEX_BITS[q][2] == 1 ? p : -p};
+        const auto g = hex_gradients(size, xi);
+        for (usize i = 0; i < 11; ++i)
+            for (usize j = 0; j < 11; ++j)
+                for (usize a = 0; a < 3; ++a)
+                    for (usize b = 0; b < 3; ++b) {
+                        double v = g[i][a] * g[j][b] * dm.normal[a][b];
+                        if (a == b) {
+                            for (usize s = 0; s < 3; ++s) {
+                                if (SHEAR_AX[s][0] == a) v += g[i][SHEAR_AX[s][1]] * g[j][SHEAR_AX[s][1]] * dm.shear[s];
+                                else if (SHEAR_AX[s][1] == a) v += g[i][SHEAR_AX[s][0]] * g[j][SHEAR_AX[s][0]] * dm.shear[s];
+                            }
+                        } else {
+                            const usize s = 3 - a - b;
+                            v += g[i][b] * g[j][a] * dm.shear[s];
+                        }
+                        k[3 * i + a][3 * j + b] += jac * v;
+                    }
+    }
+    // condense the 9 modes (hex.rs:238-279)
+    std::array<std::array<double, 9>, 9> kmm{};
+    for (usize i = 0; i < 9; ++i)
+        for (usize j = 0; j < 9; ++j) kmm[i][j] = k[24 + i][24 + j];
+    const auto chol = cholesky9(kmm);
+    std::array<double, 216> mode_u{};
+    for (usize c = 0; c < 24; ++c) {
+        std::array<double, 9> rhs{};
+        for (usize i = 0; i < 9; ++i) rhs[i] = k[24 + i][c];
+        const auto col = solve9(chol, rhs);
+        for (usize i = 0; i < 9; ++i) mode_u[i * 24 + c] = col[i];
+    }
+    std::array<double, 576> kc{};
+    for (usize r = 0; r < 24; ++r)
+        for (usize c = 0; c < 24; ++c) {
+            double v = k[r][c];
+            for (usize i = 0; i < 9; ++i) v -= k[r][24 + i] * mode_u[i * 24 + c];
+            kc[r * 24 + c] = v;
+        }
+    return kc;
+}
+
 }  // namespace orc

@@ -168,14 +168,33 @@ struct Coarsened {
 };
 std::optional<Coarsened> coarsen(const Level& fine, const Csr& a);  // multigrid.rs:329-452
 
+// multigrid.rs:67-70 Cells, with its element closure as the list of element matrices: the lowest
+// grid point and the 24 x 24 matrix (unknown 3 n + c, corners in BITS order) of each cell.
+struct CellMatrices {
+    std::vector<std::array<u32, 3>> origin;
+    std::vector<std::array<double, 576>> k;
+};
+
+// multigrid.rs:458-600 by_cells: the coarse matrix of a displacement from the cells, and the
+// coarse cells and their matrices for the next level. The fine cells of one coarse cell are
+// summed in increasing fine-cell order (the reference's unstable sort leaves that order open).
+struct ByCells {
+    Csr a;
+    CellMatrices cells;
+};
+ByCells by_cells(const Level& fine, const CellMatrices& fc, const Level& coarse, const std::array<std::vector<usize>, 3>& keep);
+std::array<bool, 9> coarse_modes(const std::array<std::array<std::array<double, 11>, 11>, 3>& gram);  // multigrid.rs:609-643
+std::array<double, 576> condense(std::vector<double>& k, const std::array<bool, 9>& keep);            // multigrid.rs:648-671
+
 struct Multigrid {
     std::vector<Csr> a;
     std::vector<Rect> p;
     std::vector<std::array<Lines, 3>> lines;
     Cholesky coarse;
 
-    // multigrid.rs:84-139 for a scalar (cells = None): Galerkin coarse matrices.
-    static Multigrid make(const Csr& a, Level level);
+    // multigrid.rs:84-139: Galerkin coarse matrices (cells = None), or with `cells` (a
+    // displacement) the coarse matrices of by_cells.
+    static Multigrid make(const Csr& a, Level level, const CellMatrices* cells = nullptr);
     usize levels() const { return a.size(); }
     void apply(const double* r, double* z) const;  // multigrid.rs:147
     void cycle(usize l, const double* b, double* x) const;  // multigrid.rs:151-179
@@ -234,7 +253,8 @@ struct Solver {
     usize max_iter = PCG_MAX_ITER;
     double tol = PCG_TOL;  // the reference fixes PCG_TOL; the oracle lets a test tighten it
 
-    static Solver make(Csr a, Unknowns unknowns, std::optional<Method> method, Crossover crossover);
+    static Solver make(Csr a, Unknowns unknowns, std::optional<Method> method, Crossover crossover,
+                       const CellMatrices* cells = nullptr);
     Method method() const { return inner == Direct ? Method::Direct : Method::Multigrid; }
     Stats solve(const double* b, double* x, const Cancelled& cancelled) const;  // solve.rs:199-250
 };
@@ -272,6 +292,9 @@ std::vector<double> tet4_stiffness(const std::array<Vec3, 4>& g, double volume, 
 
 std::array<std::array<usize, 4>, 3> split_prism(const std::array<std::uint64_t, 6>& g);  // prism.rs:144
 
+std::array<double, 8> hex_shape(const Vec3& xi);                              // hex.rs:164-178
+std::array<double, 576> hex_stiffness(const Vec3& size, const Stiffness& d);  // hex.rs:199-279 Hex::new (k)
+
 struct Quadratic {  // tet10.rs:232-272
     std::vector<Vec3> points;
     usize corners = 0;
@@ -308,6 +331,14 @@ struct ElasticBoxSpec {
     std::array<Vec3, 6> traction{};  // force per area on each face
 };
 
+// Box-grid linear elasticity on hex8 cells (elastic.rs:287-563): E and nu per cell.
+struct ElasticHexSpec {
+    BoxSpec box;
+    std::vector<double> e, nu;       // per cell
+    std::array<bool, 6> clamped{};
+    Vec3 body{0, 0, 0};              // force per volume, V/8 on each corner (elastic.rs:402)
+};
+
 struct Problem {
     usize n_points = 0;
     usize comps = 1;
@@ -322,6 +353,7 @@ struct Problem {
     std::vector<usize> free;         // increasing
     std::vector<double> rhs;         // reduced right side over free unknowns
     Unknowns unknowns;
+    CellMatrices cell_matrices;      // hex elasticity: each cell's lowest grid point and element matrix
     // tet meshes
     std::vector<std::array<u32, 4>> tets4;
     usize corners = 0;
@@ -331,6 +363,7 @@ struct Problem {
 Problem heat_hex8_box(const HeatBoxSpec& spec);
 Problem heat_tet4_box(const HeatBoxSpec& spec);
 Problem elastic_tet_box(const ElasticBoxSpec& spec);
+Problem elastic_hex8_box(const ElasticHexSpec& spec);
 
 // The tet4 corner mesh of a box: the footprint (x, y) of quads cut at their lowest-corner
 // diagonal, extruded along z, each prism split by split_prism, numbered first-seen

@@ -4,6 +4,9 @@
 //                   V/8 per corner), heat.rs:503 (free), heat.rs:661-673 (submatrix, reduced_rhs,
 //                   Unknowns::Grid)
 //   tet4 heat:      heat.rs:880-968 (tet_model: tet_conduction), Unknowns::Points
+//   hex elasticity: elastic.rs:287-563 (points numbered as heat's, unknown 3 point + c at
+//                   elastic.rs:366-369, Hex elements, body force V/8 per corner at elastic.rs:402,
+//                   free, Unknowns::Grid with the cells of multigrid.rs:67-70)
 //   tet elasticity: elastic_tet.rs:173-313 (dofs 3 node + c, stiffness, body force -V/20 / V/5,
 //                   face traction A/3 on the edge points, free, prolongation, Unknowns::Quadratic)
 // The tet mesh of a box is the reference's extrusion (prism.rs:244-284) of a footprint of quads,
@@ -109,6 +112,68 @@ Problem heat_hex8_box(const HeatBoxSpec& spec) {
     for (usize i : pr.free) {
         pr.unknowns.level.points.push_back(pr.ijk[i]);
         pr.unknowns.level.comp.push_back(0);
+    }
+    return pr;
+}
+
+Problem elastic_hex8_box(const ElasticHexSpec& spec) {
+    // the points and cells of the heat box (the same first-seen numbering)
+    HeatBoxSpec hs;
+    hs.box = spec.box;
+    hs.k.assign((usize)spec.box.nx * spec.box.ny * spec.box.nz, 1.0);
+    Problem grid = heat_hex8_box(hs);
+    const BoxSpec& b = spec.box;
+    const usize ncell = (usize)b.nx * b.ny * b.nz;
+    if (spec.e.size() != ncell || spec.nu.size() != ncell) throw Error("one material for each cell");
+    Problem pr;
+    pr.comps = 3;
+    pr.npe = 8;
+    pr.n_points = grid.n_points;
+    pr.points = grid.points;
+    pr.ijk = grid.ijk;
+    pr.cells = grid.cells;
+    const usize n = 3 * pr.n_points;
+    std::vector<usize> eptr(ncell + 1);
+    std::vector<u32> conn;
+    conn.reserve(24 * ncell);
+    for (usize e = 0; e < ncell; ++e) {
+        eptr[e] = 24 * e;
+        for (int c = 0; c < 8; ++c)
+            for (u32 q = 0; q < 3; ++q) conn.push_back(3 * pr.cells[8 * e + c] + q);
+    }
+    eptr[ncell] = 24 * ncell;
+    pr.k = Csr::pattern(n, eptr, conn);
+    pr.load.assign(n, 0.0);
+    const auto& planes = grid.unknowns.level.planes;
+    for (usize e = 0; e < ncell; ++e) {
+        const int i = (int)(e % b.nx), j = (int)((e / b.nx) % b.ny), k = (int)(e / ((usize)b.nx * b.ny));
+        const Vec3 size = {planes[0][i + 1] - planes[0][i], planes[1][j + 1] - planes[1][j], planes[2][k + 1] - planes[2][k]};
+        const Stiffness d = Orthotropic::isotropic(spec.e[e], spec.nu[e]).stiffness();
+        const auto ke = hex_stiffness(size, d);
+        pr.k.add_element(conn.data() + 24 * e, 24, ke.data());
+        pr.cell_matrices.origin.push_back({(u32)i, (u32)j, (u32)k});
+        pr.cell_matrices.k.push_back(ke);
+        const double vol = size[0] * size[1] * size[2];
+        for (int c = 0; c < 8; ++c)
+            for (int q = 0; q < 3; ++q) pr.load[conn[24 * e + 3 * c + q]] += spec.body[q] * vol / 8.0;
+    }
+    const std::array<u32, 3> top = {(u32)b.nx, (u32)b.ny, (u32)b.nz};
+    pr.is_fixed.assign(n, 0);
+    pr.fixed_value.assign(n, 0.0);
+    for (int f = 0; f < 6; ++f) {
+        if (!spec.clamped[f]) continue;
+        for (usize v = 0; v < pr.n_points; ++v)
+            if (on_face(pr.ijk[v], top, f))
+                for (int q = 0; q < 3; ++q) pr.is_fixed[3 * v + q] = 1;
+    }
+    for (usize u = 0; u < n; ++u)
+        if (!pr.is_fixed[u]) pr.free.push_back(u);
+    pr.rhs = reduced_rhs(pr.k, pr.load, pr.fixed_value, pr.free);
+    pr.unknowns.kind = Unknowns::Grid;
+    pr.unknowns.level.planes = planes;
+    for (usize u : pr.free) {
+        pr.unknowns.level.points.push_back(pr.ijk[u / 3]);
+        pr.unknowns.level.comp.push_back((std::uint8_t)(u % 3));
     }
     return pr;
 }

@@ -101,6 +101,13 @@ def lib() -> C.CDLL:
         "orc_problem_coarse_xyz": (_SZ, [_P, C.c_void_p]),
         "orc_problem_free_handle": (None, [_P]),
         "orc_solver_new": (_P, [_P, C.c_int, _SZ, C.c_char_p, _SZ]),
+        "orc_problem_elastic_hex": (_P, [_I32P, _F64P, _F64P, _F64P, _I32P, _F64P, C.c_char_p, _SZ]),
+        "orc_problem_cell_count": (_SZ, [_P]),
+        "orc_problem_cell_matrices": (None, [_P, _U32P, _F64P]),
+        "orc_hex_stiffness": (None, [_F64P, C.c_double, C.c_double, _F64P, C.c_char_p, _SZ]),
+        "orc_solver_new_cells": (_P, [_P, C.c_int, _SZ, C.c_char_p, _SZ]),
+        "orc_by_cells_step": (_P, [_P, _U32P, np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS"),
+                                   C.POINTER(_SZ)]),
         "orc_coarsen": (_P, [_P, _F64P, _SZ, _F64P, _SZ, _F64P, _SZ, _U32P,
                              np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS"), _U32P,
                              np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")]),
@@ -241,6 +248,30 @@ class Rect:
         y = np.zeros(self.cols)
         lib().orc_rect_mul_transpose(self.h, np.ascontiguousarray(x, np.float64), y)
         return y
+
+
+def by_cells_step(problem: "Problem"):
+    """One coarsen + by_cells step on a hex elasticity problem (multigrid.rs:96-125): the coarse
+    matrix (Csr) and the coarse points (n x 3) and components."""
+    nf = problem.sizes.n_free
+    cp = np.zeros(3 * nf, np.uint32)
+    cc = np.zeros(nf, np.uint8)
+    cn = C.c_size_t()
+    h = lib().orc_by_cells_step(problem.h, cp, cc, C.byref(cn))
+    if not h:
+        return None
+    m = int(cn.value)
+    return Csr(h), cp[:3 * m].reshape(-1, 3), cc[:m]
+
+
+def hex_stiffness(size, e, nu):
+    """hex.rs:199-279: the condensed 24 x 24 element of a box cell, isotropic E and nu."""
+    out = np.zeros(576)
+    err = _err()
+    lib().orc_hex_stiffness(np.asarray(size, np.float64), float(e), float(nu), out, err, 1024)
+    if err.value:
+        raise OracleError(err.value.decode())
+    return out.reshape(24, 24)
 
 
 def coarsen(a: Csr, planes, points, comp):
@@ -454,6 +485,31 @@ class Problem:
             raise OracleError(err.value.decode())
         return Problem(h)
 
+    @staticmethod
+    def elastic_hex(n, length, e, nu, clamped=(), body=(0, 0, 0)) -> "Problem":
+        """Box-grid linear elasticity on hex8 cells (elastic.rs:287-563), E and nu per cell."""
+        nc = int(np.prod(n))
+        e = np.broadcast_to(np.asarray(e, np.float64), (nc,)).copy()
+        nu = np.broadcast_to(np.asarray(nu, np.float64), (nc,)).copy()
+        cl = np.zeros(6, np.int32)
+        for f in clamped:
+            cl[FACES.index(f)] = 1
+        err = _err()
+        h = lib().orc_problem_elastic_hex(np.asarray(n, np.int32), np.asarray(length, np.float64), e, nu, cl,
+                                          np.asarray(body, np.float64), err, 1024)
+        if not h:
+            raise OracleError(err.value.decode())
+        return Problem(h)
+
+    def cell_matrices(self):
+        """The cells of a hex elasticity problem (multigrid.rs:67-70): origins (count x 3) and
+        element matrices (count x 24 x 24)."""
+        c = lib().orc_problem_cell_count(self.h)
+        origin = np.zeros(3 * c, np.uint32)
+        k = np.zeros(576 * c)
+        lib().orc_problem_cell_matrices(self.h, origin, k)
+        return origin.reshape(c, 3), k.reshape(c, 24, 24)
+
     def points(self):
         out = np.zeros(3 * self.sizes.n_points)
         lib().orc_problem_points(self.h, out)
@@ -515,6 +571,19 @@ class Solver:
         if not self.h:
             raise OracleError(err.value.decode())
         self.n = problem.sizes.n_free
+
+    @classmethod
+    def with_cells(cls, problem: Problem, method=None, crossover=3000) -> "Solver":
+        """Solver::new(.., Some(cells)) on a hex elasticity problem: the coarse matrices of
+        by_cells (multigrid.rs:458-600)."""
+        m = {None: -1, "direct": 0, "multigrid": 1}[method]
+        err = _err()
+        self = cls.__new__(cls)
+        self.h = lib().orc_solver_new_cells(problem.h, m, crossover, err, 1024)
+        if not self.h:
+            raise OracleError(err.value.decode())
+        self.n = problem.sizes.n_free
+        return self
 
     @classmethod
     def grid(cls, a: "Csr", planes, points, comp, method=None, crossover=3000) -> "Solver":

@@ -277,9 +277,171 @@ static std::vector<Vec3> level_xyz(const Level& level) {
     return xyz;
 }
 
-Multigrid Multigrid::make(const Csr& a0, Level level) {
+// ------------------------------------------------------------------ by_cells, multigrid.rs:458-671
+ByCells by_cells(const Level& fine, const CellMatrices& fc, const Level& coarse,
+                 const std::array<std::vector<usize>, 3>& keep) {
+    static const u32 BITS[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    const usize fd[3] = {fine.planes[0].size(), fine.planes[1].size(), fine.planes[2].size()};
+    const usize cd[3] = {coarse.planes[0].size(), coarse.planes[1].size(), coarse.planes[2].size()};
+    auto index = [](const std::array<u32, 3>& q, usize c, const usize* d) {
+        return c + 3 * (q[0] + d[0] * (q[1] + d[1] * (usize)q[2]));
+    };
+    std::vector<bool> free(3 * fd[0] * fd[1] * fd[2], false);
+    for (usize k = 0; k < fine.points.size(); ++k) free[index(fine.points[k], fine.comp[k], fd)] = true;
+    std::vector<u32> number(3 * cd[0] * cd[1] * cd[2], UINT32_MAX);
+    for (usize k = 0; k < coarse.points.size(); ++k) number[index(coarse.points[k], coarse.comp[k], cd)] = (u32)k;
+    // the coarse cell of each fine cell along each axis
+    std::array<std::vector<u32>, 3> interval;
+    for (usize d = 0; d < 3; ++d) {
+        usize c = 0;
+        for (usize i = 0; i < fd[d]; ++i) {
+            while (c + 2 < keep[d].size() && keep[d][c + 1] <= i) ++c;
+            interval[d].push_back((u32)c);
+        }
+    }
+    std::vector<std::pair<u32, std::array<u32, 3>>> order;
+    for (usize f = 0; f < fc.origin.size(); ++f) {
+        const auto& o = fc.origin[f];
+        order.push_back({(u32)f, {interval[0][o[0]], interval[1][o[1]], interval[2][o[2]]}});
+    }
+    std::stable_sort(order.begin(), order.end(), [](const auto& x, const auto& y) {
+        return std::make_tuple(x.second[2], x.second[1], x.second[0]) < std::make_tuple(y.second[2], y.second[1], y.second[0]);
+    });
+    ByCells out;
+    usize start = 0;
+    while (start < order.size()) {
+        const auto cc = order[start].second;
+        usize end = start;
+        while (end < order.size() && order[end].second == cc) ++end;
+        double lo[3], hi[3];
+        for (usize d = 0; d < 3; ++d) {
+            lo[d] = coarse.planes[d][cc[d]];
+            hi[d] = coarse.planes[d][cc[d] + 1];
+        }
+        std::vector<double> k(33 * 33, 0.0);
+        std::array<std::array<std::array<double, 11>, 11>, 3> gram{};
+        for (usize t = start; t < end; ++t) {
+            const u32 f = order[t].first;
+            const auto& kf = fc.k[f];
+            const auto& o = fc.origin[f];
+            double phi[8][3][11] = {};
+            for (usize n = 0; n < 8; ++n) {
+                const std::array<u32, 3> q{o[0] + BITS[n][0], o[1] + BITS[n][1], o[2] + BITS[n][2]};
+                Vec3 xi;
+                for (usize d = 0; d < 3; ++d) xi[d] = 2.0 * (fine.planes[d][q[d]] - lo[d]) / (hi[d] - lo[d]) - 1.0;
+                const auto corner = hex_shape(xi);
+                for (usize c = 0; c < 3; ++c) {
+                    if (!free[index(q, c, fd)]) continue;
+                    double* fv = phi[n][c];
+                    for (usize a = 0; a < 8; ++a) fv[a] = corner[a];
+                    for (usize d = 0; d < 3; ++d) fv[8 + d] = 1.0 - xi[d] * xi[d];
+                    for (usize a = 0; a < 11; ++a)
+                        for (usize b = 0; b < 11; ++b) gram[c][a][b] += fv[a] * fv[b];
+                }
+            }
+            // kt = Kf T, then K += T^T kt
+            double kt[24][33] = {};
+            for (usize r = 0; r < 24; ++r)
+                for (usize m = 0; m < 8; ++m)
+                    for (usize c = 0; c < 3; ++c) {
+                        const double v = kf[r * 24 + 3 * m + c];
+                        if (v != 0.0)
+                            for (usize a = 0; a < 11; ++a) kt[r][3 * a + c] += v * phi[m][c][a];
+                    }
+            for (usize n = 0; n < 8; ++n)
+                for (usize c = 0; c < 3; ++c) {
+                    const double* row = kt[3 * n + c];
+                    for (usize a = 0; a < 11; ++a) {
+                        const double w = phi[n][c][a];
+                        if (w == 0.0) continue;
+                        double* o2 = &k[(3 * a + c) * 33];
+                        for (usize j = 0; j < 33; ++j) o2[j] += w * row[j];
+                    }
+                }
+        }
+        out.cells.k.push_back(condense(k, coarse_modes(gram)));
+        out.cells.origin.push_back(cc);
+        start = end;
+    }
+    // assemble on the coarse unknowns
+    std::vector<std::vector<std::pair<usize, u32>>> ids(out.cells.origin.size());
+    std::vector<usize> eptr{0};
+    std::vector<u32> conn;
+    for (usize e = 0; e < out.cells.origin.size(); ++e) {
+        const auto& o = out.cells.origin[e];
+        for (usize n = 0; n < 8; ++n) {
+            const std::array<u32, 3> q{o[0] + BITS[n][0], o[1] + BITS[n][1], o[2] + BITS[n][2]};
+            for (usize c = 0; c < 3; ++c) {
+                const u32 g = number[index(q, c, cd)];
+                if (g != UINT32_MAX) {
+                    ids[e].push_back({3 * n + c, g});
+                    conn.push_back(g);
+                }
+            }
+        }
+        eptr.push_back(conn.size());
+    }
+    out.a = Csr::pattern(coarse.points.size(), eptr, conn);
+    std::vector<double> ke;
+    for (usize e = 0; e < ids.size(); ++e) {
+        ke.clear();
+        for (const auto& r : ids[e])
+            for (const auto& c : ids[e]) ke.push_back(out.cells.k[e][r.first * 24 + c.first]);
+        out.a.add_element(conn.data() + eptr[e], ids[e].size(), ke.data());
+    }
+    return out;
+}
+
+std::array<bool, 9> coarse_modes(const std::array<std::array<std::array<double, 11>, 11>, 3>& gram) {
+    std::array<bool, 9> keep{};
+    for (usize c = 0; c < 3; ++c) {
+        const auto& g = gram[c];
+        double l[11][11] = {};
+        bool used[11] = {};
+        for (usize j = 0; j < 11; ++j) {
+            double d = g[j][j];
+            for (usize k = 0; k < j; ++k)
+                if (used[k]) d -= l[j][k] * l[j][k];
+            if (!(d > 1e-6 * g[j][j])) continue;
+            used[j] = true;
+            d = std::sqrt(d);
+            l[j][j] = d;
+            for (usize i = j + 1; i < 11; ++i) {
+                double v = g[i][j];
+                for (usize k = 0; k < j; ++k)
+                    if (used[k]) v -= l[i][k] * l[j][k];
+                l[i][j] = v / d;
+            }
+            if (j >= 8) keep[3 * (j - 8) + c] = true;
+        }
+    }
+    return keep;
+}
+
+std::array<double, 576> condense(std::vector<double>& k, const std::array<bool, 9>& keep) {
+    for (usize m = 24; m < 33; ++m) {
+        if (!keep[m - 24]) continue;
+        const double pivot = k[m * 33 + m];
+        double scale = 0.0;
+        for (usize i = 0; i < 33; ++i) scale = std::max(scale, std::abs(k[i * 33 + i]));
+        if (!(pivot > 1e-12 * scale)) continue;
+        for (usize i = 0; i < 33; ++i) {
+            const double f = k[i * 33 + m] / pivot;
+            if (i == m || f == 0.0) continue;
+            for (usize j = 0; j < 33; ++j) k[i * 33 + j] -= f * k[m * 33 + j];
+        }
+    }
+    std::array<double, 576> out{};
+    for (usize r = 0; r < 24; ++r)
+        for (usize c = 0; c < 24; ++c) out[r * 24 + c] = k[r * 33 + c];
+    return out;
+}
+
+Multigrid Multigrid::make(const Csr& a0, Level level, const CellMatrices* cells) {
     Multigrid mg;
     mg.a.push_back(a0);
+    CellMatrices stored;
+    bool have_stored = false;
     for (;;) {
         const Csr& a = mg.a.back();
         if (a.n <= COARSEST || mg.p.size() >= 30) break;
@@ -287,7 +449,15 @@ Multigrid Multigrid::make(const Csr& a0, Level level) {
         if (!c) break;
         if (c->coarse.points.size() * 10 > a.n * 9) break;
         mg.lines.push_back({Lines::make(level, 0, a), Lines::make(level, 1, a), Lines::make(level, 2, a)});
-        Csr ac = c->p.galerkin(a);
+        Csr ac;
+        if (!cells) {
+            ac = c->p.galerkin(a);
+        } else {
+            ByCells bc = by_cells(level, have_stored ? stored : *cells, c->coarse, c->kept);
+            ac = std::move(bc.a);
+            stored = std::move(bc.cells);
+            have_stored = true;
+        }
         mg.a.push_back(std::move(ac));
         mg.p.push_back(std::move(c->p));
         level = std::move(c->coarse);
@@ -401,7 +571,8 @@ std::pair<Rect, std::vector<Vec3>> prolongation(const std::vector<Vec3>& points,
 }
 
 // ------------------------------------------------------------------ solve.rs:92-250
-Solver Solver::make(Csr a, Unknowns unknowns, std::optional<Method> method, Crossover crossover) {
+Solver Solver::make(Csr a, Unknowns unknowns, std::optional<Method> method, Crossover crossover,
+                    const CellMatrices* cells) {
     Solver s;
     auto choose = [&](const std::function<Symbolic()>& symbolic) -> std::optional<Symbolic> {
         if (method == Method::Direct) return symbolic();
@@ -442,7 +613,7 @@ Solver Solver::make(Csr a, Unknowns unknowns, std::optional<Method> method, Cros
         s.chol = Cholesky::factor(*sym, a);
         s.inner = Direct;
     } else {
-        s.mg = Multigrid::make(a, level);
+        s.mg = Multigrid::make(a, level, cells);
         s.inner = MG;
     }
     s.a = std::move(a);
