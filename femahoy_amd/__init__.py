@@ -220,7 +220,7 @@ def lib() -> C.CDLL:
                                         C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), ERR, SZ]),
         "fem_solver_new_csr": (C.c_int, [C.POINTER(_Csr), C.POINTER(_Unknowns), C.POINTER(_Options), C.POINTER(P),
                                          ERR, SZ]),
-        "fem_test_csr_levels_new": (C.c_int, [C.POINTER(_Csr), C.POINTER(_GridLevel), C.c_uint64, C.c_int,
+        "fem_test_csr_levels_new": (C.c_int, [C.POINTER(_Csr), C.POINTER(_GridLevel), C.c_void_p, C.c_uint64, C.c_int,
                                               C.POINTER(P), ERR, SZ]),
         "fem_test_csr_levels_count": (C.c_int, [P]),
         "fem_test_csr_level_sizes": (C.c_int, [P, C.c_int] + [C.POINTER(C.c_uint64)] * 4),
@@ -640,7 +640,7 @@ def symbolic(ptr, idx, xyz):
     return order.astype(np.int64), int(e.value), float(f.value), int(lf.value), int(sn.value)
 
 
-def csr_levels(ptr, idx, val, grid: Grid, coarsest: int = 3000, max_levels: int = 0):
+def csr_levels(ptr, idx, val, grid: Grid, coarsest: int = 3000, max_levels: int = 0, cells=None):
     """The host setup of the assembled grid multigrid alone (no device): the reference's level loop
     with coarsen and Galerkin products. Returns a list of dicts per level: n, a = (ptr, idx, val),
     p = (ptr, idx, val, cols) from the next level (None on the last), points, comp."""
@@ -651,7 +651,15 @@ def csr_levels(ptr, idx, val, grid: Grid, coarsest: int = 3000, max_levels: int 
     h = C.c_void_p()
     err = _err()
     L = lib()
-    if L.fem_test_csr_levels_new(C.byref(a), C.byref(g), coarsest, max_levels, C.byref(h), err, 2048):
+    cp = None
+    if cells is not None:
+        origin, element = cells
+        origin = np.ascontiguousarray(origin, np.uint32).reshape(-1, 3)
+        element = np.ascontiguousarray(element, np.float64).reshape(-1, 576)
+        cs = _Cells(len(origin), origin.ctypes.data, element.ctypes.data)
+        keep += [origin, element, cs]
+        cp = C.cast(C.pointer(cs), C.c_void_p)
+    if L.fem_test_csr_levels_new(C.byref(a), C.byref(g), cp, coarsest, max_levels, C.byref(h), err, 2048):
         raise FemError(err.value.decode())
     try:
         out = []

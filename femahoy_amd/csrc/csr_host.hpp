@@ -52,7 +52,25 @@ void validate(const GridLevel& g, uint64_t n);
 // geometric ones, or for a scalar the operator's (its row's couplings to the plane below and
 // above) when both are positive. Coarse unknowns are numbered in the order the fine rows first
 // reach them; each row of the prolongation is sorted by column.
-bool coarsen(const GridLevel& fine, const Csr& a, Rect& p, GridLevel& coarse);
+bool coarsen(const GridLevel& fine, const Csr& a, Rect& p, GridLevel& coarse,
+             std::array<std::vector<size_t>, 3>* kept = nullptr);
+
+// multigrid.rs:67-70 Cells: the lowest grid point and the 24 x 24 element matrix (unknown 3 n + c,
+// corners in hex.rs:34 BITS order) of each cell of a displacement level.
+struct Cells {
+    std::vector<std::array<uint32_t, 3>> origin;
+    std::vector<std::array<double, 576>> k;
+};
+void validate(const Cells& cells, const GridLevel& g);
+
+// multigrid.rs:458-671 by_cells (with modes and condense): the coarse matrix of a displacement
+// from the cells of the fine level: each coarse cell sums its fine cells' energies on its 8
+// trilinear corner fields and 9 incompatible modes at the free fine unknowns, drops the modes the
+// corner fields span there (Gram-Cholesky test, 1e-6), condenses the rest and is assembled on
+// the coarse unknowns. `next` gets the coarse cells and matrices for the next level. The fine
+// cells of a coarse cell are summed in increasing fine-cell order.
+Csr by_cells(const GridLevel& fine, const Cells& cells, const GridLevel& coarse, const std::array<std::vector<size_t>, 3>& kept,
+             Cells& next);
 
 // sparse.rs:233-292: P^T A P by Gustavson's method (a dense accumulator per row of P^T), rows
 // sorted. Rows are independent; `threads` host threads share them (the result does not depend
@@ -65,16 +83,17 @@ Rect transpose(const Rect& p);
 // The diagonal (sparse.rs:89); 0 where a row has no diagonal entry.
 std::vector<double> diagonal(const Csr& a);
 
-// The grid multigrid hierarchy of multigrid.rs:84-139 for a matrix without cells (Galerkin
-// coarse matrices): coarsen while the level has more than `coarsest` unknowns, fewer than 30
-// prolongations exist, a coarsening exists and keeps at most 90 % of the unknowns. max_levels
-// > 0 also stops at that many levels.
+// The grid multigrid hierarchy of multigrid.rs:84-139: Galerkin coarse matrices, or with
+// `cells` (a displacement) those of by_cells; coarsen while the level has more than `coarsest`
+// unknowns, fewer than 30 prolongations exist, a coarsening exists and keeps at most 90 % of the
+// unknowns. max_levels > 0 also stops at that many levels.
 struct GridHierarchy {
     std::vector<Csr> a;        // level 0 (the input) to the coarsest
     std::vector<Rect> p;       // p[l]: level l + 1 to level l
     std::vector<GridLevel> level;
 };
-GridHierarchy grid_hierarchy(const Csr& a0, const GridLevel& level0, uint64_t coarsest, int max_levels, int threads);
+GridHierarchy grid_hierarchy(const Csr& a0, const GridLevel& level0, uint64_t coarsest, int max_levels, int threads,
+                             const Cells* cells = nullptr);
 
 }  // namespace csr
 }  // namespace fem

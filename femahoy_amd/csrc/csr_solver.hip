@@ -206,9 +206,8 @@ class CsrSolverImpl {
         FEM_HIP(hipEventCreate(&ev0_));
         FEM_HIP(hipEventCreate(&ev1_));
         // solve.rs:98-183: the direct solver when forced, or as the crossover chooses
-        if (u.cells)
-            throw Error("coarse matrices from cells (multigrid.rs:458-679) are not built: give no cells, and the "
-                        "coarse matrices are the Galerkin products of the prolongations");
+        if (u.cells && u.kind != FEM_UNKNOWNS_GRID)
+            throw Error("cells give the coarse matrices of grid unknowns (a displacement on the box grid)");
         if (u.kind == FEM_UNKNOWNS_POINTS) {
             if (o.method == FEM_METHOD_MULTIGRID)
                 throw Error("multigrid needs the box grid: a mesh of tetrahedra has the direct solver");
@@ -379,7 +378,8 @@ class CsrSolverImpl {
         const uint64_t cmax = opt_.coarse_max;
         if (u.kind == FEM_UNKNOWNS_GRID) {
             csr::validate(u.grid, a.n);
-            host_ = csr::grid_hierarchy(a, u.grid, COARSEST, opt_.levels, threads);
+            if (u.cells) csr::validate(u.cell_data, u.grid);
+            host_ = csr::grid_hierarchy(a, u.grid, COARSEST, opt_.levels, threads, u.cells ? &u.cell_data : nullptr);
         } else {
             // pmultigrid.rs:46-60: the linear level is P^T A P; the diagonal must be above 0
             csr::validate(u.p);
@@ -784,6 +784,18 @@ csr::GridLevel grid_from(const fem_grid_level* g, uint64_t n) {
     return l;
 }
 
+csr::Cells cells_from(const fem_cells* c) {
+    if (!c || (c->count && (!c->origin || !c->element))) throw Error("the cells need their origins and matrices");
+    csr::Cells out;
+    out.origin.resize(c->count);
+    out.k.resize(c->count);
+    for (uint64_t e = 0; e < c->count; ++e) {
+        for (int d = 0; d < 3; ++d) out.origin[e][d] = c->origin[3 * e + d];
+        std::copy(c->element + 576 * e, c->element + 576 * (e + 1), out.k[e].begin());
+    }
+    return out;
+}
+
 CsrUnknowns unknowns_from(const fem_unknowns* u, uint64_t n) {
     if (!u) throw Error("no unknowns");
     CsrUnknowns out;
@@ -805,6 +817,7 @@ CsrUnknowns unknowns_from(const fem_unknowns* u, uint64_t n) {
             for (int d = 0; d < 3; ++d) out.xyz[k][d] = u->xyz[3 * k + d];
     }
     out.cells = u->cells != nullptr;
+    if (u->cells) out.cell_data = cells_from(u->cells);
     return out;
 }
 
@@ -825,8 +838,8 @@ void put_error(char* err, size_t len, const char* msg) {
 
 extern "C" {
 
-int fem_test_csr_levels_new(const fem_csr* a, const fem_grid_level* g, uint64_t coarsest, int max_levels,
-                            fem_csr_levels** out, char* err, size_t errlen) {
+int fem_test_csr_levels_new(const fem_csr* a, const fem_grid_level* g, const fem_cells* cells, uint64_t coarsest,
+                            int max_levels, fem_csr_levels** out, char* err, size_t errlen) {
     try {
         if (!out) throw fem::Error("an output handle is needed");
         *out = nullptr;
@@ -834,8 +847,13 @@ int fem_test_csr_levels_new(const fem_csr* a, const fem_grid_level* g, uint64_t 
         fem::csr::validate(m);
         fem::csr::GridLevel l = fem::grid_from(g, m.n);
         fem::csr::validate(l, m.n);
+        fem::csr::Cells cl;
+        if (cells) {
+            cl = fem::cells_from(cells);
+            fem::csr::validate(cl, l);
+        }
         auto* h = new fem_csr_levels;
-        h->h = fem::csr::grid_hierarchy(m, l, coarsest, max_levels, 8);
+        h->h = fem::csr::grid_hierarchy(m, l, coarsest, max_levels, 8, cells ? &cl : nullptr);
         *out = h;
         return 0;
     } catch (const std::exception& e) {

@@ -21,7 +21,8 @@ struct CsrUnknowns {
     bool coarse_grid = false;           // QUADRATIC: the linear unknowns' grid level is given
     csr::GridLevel coarse;              // ... and here it is
     std::vector<std::array<double, 3>> xyz;  // POINTS, QUADRATIC: the coordinates of each unknown, if given
-    bool cells = false;                      // Solver::new's cells were given (refused)
+    bool cells = false;                      // Solver::new's cells were given
+    csr::Cells cell_data;                    // ... and here they are
 };
 
 // The coordinates of each unknown of a grid level (solve.rs:143-150: the planes of its point).
@@ -61,6 +62,7 @@ class CsrSolver {
 csr::Csr csr_from(const fem_csr* a);
 csr::Rect rect_from(const fem_csr* p);
 csr::GridLevel grid_from(const fem_grid_level* g, uint64_t n);
+csr::Cells cells_from(const fem_cells* c);
 CsrUnknowns unknowns_from(const fem_unknowns* u, uint64_t n);
 
 }  // namespace fem

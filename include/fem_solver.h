@@ -221,8 +221,10 @@ typedef struct fem_unknowns {
  * coarsest level the dense LDL^T (up to coarse_max unknowns; a QUADRATIC linear level with no grid
  * below it up to 40000). The direct solver is the dense LDL^T, up to 40000 unknowns (the
  * reference's TRANSIENT_DIRECT_LIMIT, solve.rs:24); when the crossover picks it for a larger matrix
- * the call fails with a message that gives the factor's counts. Cells (the coarse matrices of
- * multigrid.rs:458-679) are refused with a message. b and x of fem_solver_solve are the matrix's
+ * the call fails with a message that gives the factor's counts. With `cells` (GRID unknowns of a
+ * displacement) the coarse matrices are those of by_cells (multigrid.rs:458-671: each coarse cell's
+ * energy on its trilinear corner fields and incompatible modes, condensed), as the reference's
+ * box-grid elasticity caller asks (elastic.rs:537-547). b and x of fem_solver_solve are the matrix's
  * unknowns in its order; fem_solver_rhs and fem_solver_point_grid do not apply. */
 int fem_solver_new_csr(const fem_csr* a, const fem_unknowns* u, const fem_options* opt, fem_solver** out, char* err,
                        size_t errlen);

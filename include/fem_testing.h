@@ -34,11 +34,12 @@ int fem_test_copy(const void* d_src, void* d_dst, size_t bytes, char* err, size_
 
 /* The host setup of the grid multigrid of the assembled boundary (fem_solver_new_csr), alone: the
  * reference's level loop (multigrid.rs:84-139) with coarsen (329-452) and Galerkin products
- * (sparse.rs:233-292), down to `coarsest` unknowns (max_levels > 0: at most that many levels).
+ * (sparse.rs:233-292), or with `cells` (may be NULL) the coarse matrices of by_cells
+ * (multigrid.rs:458-671), down to `coarsest` unknowns (max_levels > 0: at most that many levels).
  * No device is touched. */
 typedef struct fem_csr_levels fem_csr_levels;
-int fem_test_csr_levels_new(const fem_csr* a, const fem_grid_level* g, uint64_t coarsest, int max_levels,
-                            fem_csr_levels** out, char* err, size_t errlen);
+int fem_test_csr_levels_new(const fem_csr* a, const fem_grid_level* g, const fem_cells* cells, uint64_t coarsest,
+                            int max_levels, fem_csr_levels** out, char* err, size_t errlen);
 int fem_test_csr_levels_count(const fem_csr_levels* h);
 /* Level l: its unknowns n, the entries of its matrix, and the columns and entries of the
  * prolongation from level l + 1 (0 and 0 on the last level). */

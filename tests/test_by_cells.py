@@ -105,3 +105,42 @@ def test_multigrid_from_cells_solves_a_clamped_box(oracle):
     x, it, res = s.solve(pr.rhs(), tol=1e-12)
     assert it <= 20
     assert np.linalg.norm(x - xd) <= 1e-10 * np.linalg.norm(xd)
+
+
+def _hex_problem(oracle, n, length, seed, clamped=("x0",)):
+    rng = np.random.default_rng(seed)
+    nc = int(np.prod(n))
+    return oracle.Problem.elastic_hex(n, length, rng.uniform(1, 3, nc), rng.uniform(0.2, 0.35, nc),
+                                      clamped=list(clamped), body=(0, 0, -1))
+
+
+def _grid(fem, pr, n, length):
+    planes = [np.linspace(0.0, length[d], n[d] + 1) for d in range(3)]
+    free = pr.free()
+    return fem.Grid(planes, pr.ijk()[free // 3], (free % 3).astype(np.uint8))
+
+
+@pytest.mark.parametrize("n,length", [([24, 12, 12], [2.0, 1.0, 1.0]), ([29, 9, 11], [3.0, 0.9, 1.1])])
+def test_the_library_builds_the_oracle_levels_from_cells_bit_for_bit(fem, oracle, n, length):
+    """The host setup of fem_solver_new_csr with cells against the oracle's Multigrid::new with
+    cells: the same prolongations and coarse matrices (pattern and FP64 values) on every level;
+    odd counts give coarse cells of one fine layer and planes kept at the ends."""
+    pr = _hex_problem(oracle, n, length, 4)
+    a = pr.reduced()
+    ptr, idx, val = a.arrays()
+    ours = fem.csr_levels(ptr.astype(np.uint64), idx, val, _grid(fem, pr, n, length), cells=pr.cell_matrices())
+    ref = oracle.Solver.with_cells(pr, "multigrid")
+    assert ref.levels() == len(ours) >= 2
+    for l, lev in enumerate(ours):
+        rp, ri, rv = ref.level_matrix(l).arrays()
+        assert np.array_equal(rp.astype(np.uint64), lev["a"][0])
+        assert np.array_equal(ri, lev["a"][1])
+        assert np.array_equal(rv, lev["a"][2])
+        if l + 1 < len(ours):
+            pp, pi, pv = ref.level_p(l).arrays()
+            assert np.array_equal(pp.astype(np.uint64), lev["p"][0])
+            assert np.array_equal(pi, lev["p"][1])
+            assert np.array_equal(pv, lev["p"][2])
+    # the coarse matrices differ from the Galerkin products of the same prolongations
+    gal = fem.csr_levels(ptr.astype(np.uint64), idx, val, _grid(fem, pr, n, length))
+    assert not np.array_equal(gal[1]["a"][2], ours[1]["a"][2])
