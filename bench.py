@@ -272,6 +272,7 @@ def cpu_baseline(n_cpu, reference_tol=None):
         log(f"  oracle FP64 MG-PCG continued to {res_ref:.1e} ({it_ref} more iterations, "
             f"{time.perf_counter() - t3:.1f} s)")
     return x_ref, {
+        "assembly_s": round(t1 - t0, 3),
         "value": nfree / (t3 - t2) / 1e6,
         "unit": "MDOF/s",
         "cores": 1,
@@ -413,7 +414,59 @@ def run_workload(F, hip, box, levels, args, label, degree, auto_leg=False):
     return res, prof, lat_m
 
 
-def assembled_leg(F, hip, solver, box, x_ref, args, degree):
+def device_assembly(F, box, k, solver, ptr, idx, val):
+    """fem_assemble on the heat box's mesh in the model's numbering (the reference's first-seen
+    points, cells x fastest with VTK corners): the pattern and free list equal the export's bit for
+    bit, the values and the right side equal it to the export's probe rounding; the host and device
+    times of the assembly."""
+    n = list(box.n)
+    h = [box.length[d] / n[d] for d in range(3)]
+    m = F.Model(box)
+    pg = m.point_grid().astype(np.int64)
+    at = np.full((n[0] + 1) * (n[1] + 1) * (n[2] + 1), -1, np.int64)
+    at[pg[:, 0] + (n[0] + 1) * (pg[:, 1] + (n[1] + 1) * pg[:, 2])] = np.arange(pg.shape[0])
+    i, j, kk = np.meshgrid(np.arange(n[0]), np.arange(n[1]), np.arange(n[2]), indexing="ij")
+    i, j, kk = (v.transpose(2, 1, 0).ravel() for v in (i, j, kk))  # cells x fastest
+    bits = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    conn = np.stack([at[(i + a) + (n[0] + 1) * ((j + b) + (n[1] + 1) * (kk + c))] for a, b, c in bits], axis=1)
+    xyz = pg * np.array(h)
+    fixed = (pg[:, 0] == 0) | (pg[:, 0] == n[0])
+    value = np.where(pg[:, 0] == n[0], 1.0, 0.0)
+    mesh = F.Mesh("heat_hex8", xyz, conn.astype(np.uint32), k, fixed.astype(np.uint8), value)
+    t0 = time.perf_counter()
+    a = F.Assembly(mesh)
+    t1 = time.perf_counter()
+    aptr, aidx, aval, arhs, afree = a.arrays()
+    st = a.stats
+    same_pattern = bool(np.array_equal(afree, solver.free_list()) and np.array_equal(aptr, ptr.astype(np.uint64))
+                        and np.array_equal(aidx, idx))
+    val_err = float(np.abs(aval - val).max() / np.abs(val).max())
+    b = solver.rhs()
+    rhs_err = float(np.abs(arhs - b).max() / np.abs(b).max())
+    cells = conn.shape[0]
+    # algorithmic bytes of the device part: the element matrices written and read back by the
+    # gather (8 B a slot), the gather maps (4 B a slot, 8 B an entry), the values written, the
+    # reduced copy and right side (their arrays once), the mesh read by the element kernel
+    slots = 64 * cells
+    nbytes = (slots * (8 + 8 + 4) + 8 * int(st.full_entries) * 2 + 8 * cells * 8 * 2 + 12 * int(st.entries)
+              + 8 * cells * 4 + 24 * pg.shape[0])
+    log(f"  device assembly: {int(st.entries)} reduced entries; host {st.host_seconds:.3f} s, device "
+        f"{st.device_seconds * 1e3:.2f} ms ({nbytes / st.device_seconds / 1e9:.0f} GB/s algorithmic); pattern equal to "
+        f"the export: {same_pattern}; values {val_err:.1e}, right side {rhs_err:.1e} from the export")
+    out = {"workload": (f"fem_assemble of the same box's hex8 mesh ({cells} cells, {pg.shape[0]} points) in the "
+                        "model's numbering: pattern, free list and gather maps on the host, element matrices, "
+                        "gathers, reduced matrix and right side on the device"),
+           "entries": int(st.entries), "host_s": round(st.host_seconds, 3),
+           "device_ms": round(st.device_seconds * 1e3, 3), "call_s": round(t1 - t0, 3),
+           "device_bytes": int(nbytes), "device_gbs": round(nbytes / st.device_seconds / 1e9, 1),
+           "device_frac": round(nbytes / st.device_seconds / 1e9 / HBM_PEAK_GBS, 4),
+           "pattern_equal_to_export": same_pattern, "values_rel_diff_vs_export": val_err,
+           "rhs_rel_diff_vs_export": rhs_err}
+    a.close()
+    return out
+
+
+def assembled_leg(F, hip, solver, box, x_ref, args, degree, k=None):
     """The assembled boundary on the same heat box: the model's reduced CSR (fem_solver_matrix, as
     the reference's caller assembles it), its grid unknowns, and fem_solver_new_csr with the
     reference's grid hierarchy (coarsen and Galerkin products down to 3000 unknowns), mixed
@@ -435,6 +488,7 @@ def assembled_leg(F, hip, solver, box, x_ref, args, degree):
         except F.FemError:
             break
     log(f"  {int(ptr[-1])} entries; export {t1 - t0:.2f} s, setup {t2 - t1:.2f} s; levels {lv}")
+    assembly = device_assembly(F, box, k, solver, ptr, idx, val) if k is not None else None
     b = solver.rhs()
     d_b = hip.malloc(b.nbytes)
     d_x = hip.malloc(b.nbytes)
@@ -449,7 +503,7 @@ def assembled_leg(F, hip, solver, box, x_ref, args, degree):
     return {"mdof_s": round(solver.n / float(np.mean(lat)) / 1e6, 2), "ms_mean": round(1e3 * float(np.mean(lat)), 3),
             "iterations": st.iterations, "residual": st.residual, "true_residual": st.true_residual,
             "error_vs_fp64": err, "entries": int(ptr[-1]), "levels": lv, "export_s": round(t1 - t0, 3),
-            "setup_s": round(t2 - t1, 3),
+            "setup_s": round(t2 - t1, 3), "device_assembly": assembly,
             "workload": "the same heat box through fem_solver_new_csr: CSR SpMV operators on every level (FP32 "
                         "V-cycle, FP64 PCG), the reference's coarsen and Galerkin levels down to 3000 unknowns, "
                         f"{cheb_desc(degree, True)}, dense LDLT coarse, tol 1e-8"}
@@ -550,7 +604,7 @@ def main():
     hdeg = 3 if args.degree is None else args.degree
     vdeg = 0 if args.degree is None else args.degree
     n = args.n
-    box, _ = heat_box(F, n)
+    box, heat_k = heat_box(F, n)
     heat, prof, lat_m = run_workload(F, hip, box, args.levels, args, f"heat hex8 {n}^3", hdeg,
                                      auto_leg=args.degree is None)
     value = heat["mixed"]["mdof_s"]
@@ -558,7 +612,7 @@ def main():
     heat_x = heat.pop("_x")
     if not args.no_assembled:
         heat["assembled_csr"] = assembled_leg(F, hip, F.Solver(box, method="multigrid", levels=args.levels), box,
-                                              x_ref, args, vdeg)
+                                              x_ref, args, vdeg, k=heat_k)
 
     roofline = None
     traffic = measured_traffic(n)
@@ -655,6 +709,10 @@ def main():
             heat["oracle_reference"] = ("the oracle's FP64 MG-PCG (line Gauss-Seidel GMG, Galerkin levels, sparse "
                                         "Cholesky coarsest level) on the same box to 1e-13 relative residual")
             heat["error_fp64_reference_vs_oracle"] = float(np.linalg.norm(x_ref - x_oracle) / on)
+        da = heat.get("assembled_csr", {}).get("device_assembly")
+        if da and args.cpu_n == n:
+            # the oracle's model of the same box: numbering, pattern, add_element, reduced right side
+            da["oracle_cpu_model_s"] = cpu["assembly_s"]
             log("  error vs the oracle: " + ", ".join(f"{k} {heat[k]['error_vs_oracle']:.2e}" for k in heat_x))
 
     config = {"workload": (f"heat hex8 {n}^3 unit cube, k = exp(N(0,1)) per cell (seed 1234), T=0 at x=0 and T=1 at "

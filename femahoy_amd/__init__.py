@@ -120,6 +120,19 @@ class _Unknowns(C.Structure):
     _fields_ = [("kind", C.c_int32), ("grid", _GridLevel), ("p", _Csr), ("coarse_has_grid", C.c_int32),
                 ("coarse_grid", _GridLevel), ("xyz", C.c_void_p), ("cells", C.POINTER(_Cells))]
 
+class _Mesh(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("points", C.c_uint64), ("xyz", C.c_void_p), ("cells", C.c_uint64),
+                ("conn", C.c_void_p), ("coef0", C.c_void_p), ("coef1", C.c_void_p), ("source", C.c_double),
+                ("body", C.c_double * 3), ("fixed", C.c_void_p), ("value", C.c_void_p), ("load", C.c_void_p)]
+
+
+class AssemblyStats(C.Structure):
+    """fem_assembly_stats: the sizes of an assembled system and the time of its two parts."""
+
+    _fields_ = [("free", C.c_uint64), ("entries", C.c_uint64), ("full_entries", C.c_uint64),
+                ("host_seconds", C.c_double), ("device_seconds", C.c_double)]
+
+
 class _ServerOptions(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_active", C.c_int32), ("device", C.c_int32),
                 ("setup_threads", C.c_int32)]
@@ -152,6 +165,7 @@ EXPORTS = (
     "fem_server_new", "fem_server_submit", "fem_server_wait", "fem_server_cancel", "fem_server_free",
     "fem_solver_new_csr", "fem_solver_matrix", "fem_dense_solve_device",
     "fem_dense_solve_device_noalloc", "fem_solver_crossover_counts", "fem_server_memory",
+    "fem_assemble", "fem_assembly_stats_of", "fem_assembly_download", "fem_assembly_device", "fem_assembly_free",
 )
 
 _lib = None
@@ -226,6 +240,13 @@ def lib() -> C.CDLL:
         "fem_test_csr_level_sizes": (C.c_int, [P, C.c_int] + [C.POINTER(C.c_uint64)] * 4),
         "fem_test_csr_level_arrays": (C.c_int, [P, C.c_int, P, P, P, P, P, P, P, P]),
         "fem_test_csr_levels_free": (None, [P]),
+        "fem_assemble": (C.c_int, [C.POINTER(_Mesh), C.c_int, C.POINTER(P), ERR, SZ]),
+        "fem_assembly_stats_of": (None, [P, C.POINTER(AssemblyStats)]),
+        "fem_assembly_download": (C.c_int, [P, P, P, P, P, P, ERR, SZ]),
+        "fem_assembly_device": (None, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
+        "fem_assembly_free": (None, [P]),
+        "fem_test_assembly_pattern": (C.c_int, [C.POINTER(_Mesh), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), P, P,
+                                                P, ERR, SZ]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -743,6 +764,119 @@ def dense_solve(a, b, entry: str = "fem_dense_solve_device") -> np.ndarray:
     if getattr(lib(), entry)(n, da.ptr, db.ptr, dx.ptr, err, len(err)) != 0:
         raise FemError(err.value.decode())
     return dx.get()
+
+
+MESH_KINDS = {"heat_hex8": 0, "heat_tet4": 1, "elastic_tet10": 2}
+
+
+@dataclass
+class Mesh:
+    """A mesh for fem_assemble: points (n x 3), cells (c x 8, 4 or 10 point numbers), a material
+    value per cell (conductivity, or Young's modulus with `coef1` Poisson's ratio), the held
+    unknowns and their values (one per unknown: a point, or 3 point + component), a heat source or
+    body force per volume, and optionally a further load per unknown added after them."""
+
+    kind: str
+    xyz: np.ndarray
+    cells: np.ndarray
+    coef0: np.ndarray
+    fixed: np.ndarray
+    value: np.ndarray
+    coef1: Optional[np.ndarray] = None
+    source: float = 0.0
+    body: Sequence[float] = (0.0, 0.0, 0.0)
+    load: Optional[np.ndarray] = None
+
+    def _struct(self):
+        if self.kind not in MESH_KINDS:
+            raise FemError(f"unknown mesh kind {self.kind!r}")
+        keep = {
+            "xyz": np.ascontiguousarray(self.xyz, np.float64).reshape(-1),
+            "conn": np.ascontiguousarray(self.cells, np.uint32).reshape(-1),
+            "coef0": np.ascontiguousarray(self.coef0, np.float64).reshape(-1),
+            "fixed": np.ascontiguousarray(self.fixed, np.uint8).reshape(-1),
+            "value": np.ascontiguousarray(self.value, np.float64).reshape(-1),
+        }
+        if self.coef1 is not None:
+            keep["coef1"] = np.ascontiguousarray(self.coef1, np.float64).reshape(-1)
+        if self.load is not None:
+            keep["load"] = np.ascontiguousarray(self.load, np.float64).reshape(-1)
+        m = _Mesh()
+        m.kind = MESH_KINDS[self.kind]
+        m.points = keep["xyz"].size // 3
+        m.cells = np.asarray(self.cells).shape[0]
+        for k, v in keep.items():
+            setattr(m, k, v.ctypes.data)
+        m.source = float(self.source)
+        for d in range(3):
+            m.body[d] = float(self.body[d])
+        units = m.points * (3 if self.kind == "elastic_tet10" else 1)
+        if keep["fixed"].size != units or keep["value"].size != units:
+            raise FemError(f"fixed and value need one entry per unknown ({units})")
+        if "load" in keep and keep["load"].size != units:
+            raise FemError(f"load needs one entry per unknown ({units})")
+        return m, keep
+
+
+class Assembly:
+    """The system of a mesh assembled on the device (fem_assemble): the reduced CSR matrix over the
+    free unknowns (increasing) and its right side."""
+
+    def __init__(self, mesh: Mesh, device: int = 0):
+        m, self._keep = mesh._struct()
+        h = C.c_void_p()
+        err = _err()
+        if lib().fem_assemble(C.byref(m), int(device), C.byref(h), err, 2048):
+            raise FemError(err.value.decode())
+        self.h = h
+        st = AssemblyStats()
+        lib().fem_assembly_stats_of(self.h, C.byref(st))
+        self.stats = st
+        self.n = int(st.free)
+
+    def arrays(self):
+        """(ptr, idx, val, rhs, free) of the reduced system, on the host."""
+        n, ne = self.n, int(self.stats.entries)
+        ptr = np.zeros(n + 1, np.uint64)
+        idx = np.zeros(max(ne, 1), np.uint32)
+        val = np.zeros(max(ne, 1))
+        rhs = np.zeros(max(n, 1))
+        free = np.zeros(max(n, 1), np.int64)
+        err = _err()
+        if lib().fem_assembly_download(self.h, ptr.ctypes.data, idx.ctypes.data, val.ctypes.data, rhs.ctypes.data,
+                                       free.ctypes.data, err, 2048):
+            raise FemError(err.value.decode())
+        return ptr, idx[:ne], val[:ne], rhs[:n], free[:n]
+
+    def device(self):
+        """Device pointers (ptr, idx, val, rhs) of the reduced system, valid while this lives."""
+        out = [C.c_void_p() for _ in range(4)]
+        lib().fem_assembly_device(self.h, *[C.byref(o) for o in out])
+        return tuple(o.value for o in out)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fem_assembly_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def assembly_pattern(mesh: Mesh):
+    """The host part of fem_assemble alone (no device): (ptr, idx, free) of the reduced pattern."""
+    m, keep = mesh._struct()
+    nf, ne = C.c_uint64(), C.c_uint64()
+    err = _err()
+    if lib().fem_test_assembly_pattern(C.byref(m), C.byref(nf), C.byref(ne), None, None, None, err, 2048):
+        raise FemError(err.value.decode())
+    ptr = np.zeros(nf.value + 1, np.uint64)
+    idx = np.zeros(max(ne.value, 1), np.uint32)
+    free = np.zeros(max(nf.value, 1), np.int64)
+    if lib().fem_test_assembly_pattern(C.byref(m), C.byref(nf), C.byref(ne), ptr.ctypes.data, idx.ctypes.data,
+                                       free.ctypes.data, err, 2048):
+        raise FemError(err.value.decode())
+    return ptr, idx[:ne.value], free[:nf.value]
 
 
 class Server:

@@ -229,6 +229,67 @@ typedef struct fem_unknowns {
 int fem_solver_new_csr(const fem_csr* a, const fem_unknowns* u, const fem_options* opt, fem_solver** out, char* err,
                        size_t errlen);
 
+/* ---- assembly on the device: a mesh to the reduced matrix and right side ---- */
+
+/* The element kinds of an assembled mesh. */
+enum fem_mesh_kind {
+    FEM_MESH_HEAT_HEX8 = 0,     /* conduction of axis-aligned box cells (heat.rs:1246-1278), 8 corners
+                                   in VTK order (hex.rs:34-43); cell sizes from corners 0 and 6 */
+    FEM_MESH_HEAT_TET4 = 1,     /* conduction of straight tetrahedra (heat.rs:972-1000) */
+    FEM_MESH_ELASTIC_TET10 = 2  /* isotropic elasticity of straight tet10 (tet10.rs:64-120): 4 corners, then
+                                   the 6 edge nodes in the order of tet10.rs:38 */
+};
+
+/* A mesh and its loads and held unknowns, as the reference's callers hold them before
+ * Csr::pattern / add_element / submatrix / reduced_rhs (sparse.rs:22-86; heat.rs:661-673 and
+ * 1219-1242; elastic_tet.rs:178-331). Unknowns: one per node (heat), 3 node + component
+ * (elasticity, elastic_tet.rs:178-183). */
+typedef struct fem_mesh {
+    int32_t kind;            /* enum fem_mesh_kind */
+    uint64_t points;
+    const double* xyz;       /* 3 per point */
+    uint64_t cells;
+    const uint32_t* conn;    /* 8, 4 or 10 points per cell */
+    const double* coef0;     /* per cell: the isotropic conductivity, or Young's modulus */
+    const double* coef1;     /* per cell: Poisson's ratio (elasticity); may be NULL for heat */
+    double source;           /* heat: source per volume, V/8 (hex8) or V/4 (tet4) to each corner */
+    double body[3];          /* elasticity: force per volume, -V/20 to each corner and V/5 to each edge node */
+    const uint8_t* fixed;    /* per unknown: 1 when held */
+    const double* value;     /* per unknown: its held value (read where fixed) */
+    const double* load;      /* per unknown: a further load (face loads) added after the volume loads; may be NULL */
+} fem_mesh;
+
+/* The assembled system of a mesh on the device (no counterpart type in the reference). */
+typedef struct fem_assembly fem_assembly;
+
+/* Timings of an assembly. */
+typedef struct fem_assembly_stats {
+    uint64_t free;           /* unknowns of the reduced system */
+    uint64_t entries;        /* its stored entries (both triangles) */
+    uint64_t full_entries;   /* the entries of the whole pattern */
+    double host_seconds;     /* validation, the pattern, the free list and the gather maps (host) */
+    double device_seconds;   /* element matrices and loads, the gathers, the reduced matrix and right side */
+} fem_assembly_stats;
+
+/* Assembles a mesh on the device: the pattern of Csr::pattern (sparse.rs:22-62) and the free
+ * unknowns (increasing) on the host, then on the device every element's matrix and loads (one
+ * thread a matrix row), each matrix entry and load summed from its elements in element order as
+ * add_element and the callers' load loops sum them (sparse.rs:64-86), the reduced matrix
+ * (Csr::submatrix over the free unknowns) and the reduced right side of heat.rs:1219-1242. FP64
+ * throughout, with no contraction of products into fused multiply-adds, so the values equal a
+ * sequential CPU assembly in the reference's order bit for bit. A material outside the
+ * reference's range gives its text (hex.rs:79 and 93). The call waits for the device. */
+int fem_assemble(const fem_mesh* mesh, int device, fem_assembly** out, char* err, size_t errlen);
+void fem_assembly_stats_of(const fem_assembly* a, fem_assembly_stats* out);
+/* The reduced system to host arrays: ptr (free + 1), idx and val (entries), rhs (free), and the
+ * free unknowns (free); any may be NULL. */
+int fem_assembly_download(const fem_assembly* a, uint64_t* ptr, uint32_t* idx, double* val, double* rhs,
+                          int64_t* free_list, char* err, size_t errlen);
+/* The device arrays of the reduced system, valid while the assembly lives. */
+void fem_assembly_device(const fem_assembly* a, const uint64_t** ptr, const uint32_t** idx, const double** val,
+                         const double** rhs);
+void fem_assembly_free(fem_assembly* a);
+
 /* ---- a dense direct solve on device memory ---- */
 
 /* x = A^-1 b for a dense symmetric positive definite n x n matrix A: the device's dense FP64

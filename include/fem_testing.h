@@ -60,6 +60,12 @@ void fem_test_csr_levels_free(fem_csr_levels* h);
 int fem_test_symbolic(const fem_csr* a, const double* xyz, uint64_t* order_out, uint64_t* entries, double* flops,
                       uint64_t* largest_front, uint64_t* supernodes, char* err, size_t errlen);
 
+/* The host part of fem_assemble alone: the reduced pattern (ptr: free + 1 values, idx: entries)
+ * and the free unknowns of the mesh, with their counts first (call with NULL arrays to size
+ * them). No device is touched. */
+int fem_test_assembly_pattern(const fem_mesh* mesh, uint64_t* free_count, uint64_t* entries, uint64_t* ptr,
+                              uint32_t* idx, int64_t* free_list, char* err, size_t errlen);
+
 #ifdef __cplusplus
 }
 #endif
