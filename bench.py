@@ -174,10 +174,10 @@ def serve_requests(F, n_solves, seed):
     return reqs
 
 
-def run_stream(F, reqs, arrivals, budget, max_active):
+def run_stream(F, reqs, arrivals, budget, max_active, batched=True):
     """Submits reqs at their arrival times (seconds from the start; None = all at once) and waits
-    for all; returns (makespan s, latencies s, per-request stats)."""
-    srv = F.Server(hbm_budget=budget, max_active=max_active, setup_threads=SETUP_THREADS)
+    for all; returns (makespan s, latencies s, iterations, phases, memory, batching counts)."""
+    srv = F.Server(hbm_budget=budget, max_active=max_active, setup_threads=SETUP_THREADS, batched=batched)
     t0 = time.perf_counter()
     ids = []
     opts = dict(method="multigrid", precision="fp32", tol=1e-8)
@@ -196,16 +196,18 @@ def run_stream(F, reqs, arrivals, budget, max_active):
                        info.finish_time - info.start_time))
     makespan = time.perf_counter() - t0
     mem = srv.memory()
+    bst = srv.batching()
     srv.close()
-    return makespan, np.array(lat), its, np.array(phases), mem
+    return makespan, np.array(lat), its, np.array(phases), mem, bst
 
 
 def serving(F, n_solves, seed):
     reqs = serve_requests(F, n_solves, seed)
     total_dof = sum(r[4] for r in reqs)
-    budget, max_active = 64 << 30, 16
-    log(f"serving: {len(reqs)} solves ({total_dof} DOF), capacity run (all submitted at once)")
-    cap_span, cap_lat, _, _, _ = run_stream(F, reqs, None, budget, max_active)
+    budget, max_active = 64 << 30, 32
+    log(f"serving: {len(reqs)} solves ({total_dof} DOF), capacity run (all submitted at once), unbatched then batched")
+    solo_span, _, _, _, _, _ = run_stream(F, reqs, None, budget, max_active, batched=False)
+    cap_span, cap_lat, _, _, _, cap_bst = run_stream(F, reqs, None, budget, max_active)
     capacity = len(reqs) / cap_span
     rate = 0.8 * capacity
     rng = np.random.default_rng(seed + 1)
@@ -213,7 +215,7 @@ def serving(F, n_solves, seed):
     arrivals = [pair_t[r[0]] for r in reqs]
     log(f"  capacity {capacity:.1f} solves/s ({total_dof / cap_span / 1e6:.1f} MDOF/s); Poisson arrivals at "
         f"{rate:.1f} solves/s")
-    span, lat, its, ph, mem = run_stream(F, reqs, arrivals, budget, max_active)
+    span, lat, its, ph, mem, bst = run_stream(F, reqs, arrivals, budget, max_active)
     out = {
         "workload": (f"{len(reqs)} solves in refinement pairs (h, h/2), heat tet4 or elasticity tet10 (50/50), "
                      f"{min(r[4] for r in reqs)}-{max(r[4] for r in reqs)} DOF, Poisson pair arrivals at 80% of the "
@@ -228,6 +230,17 @@ def serving(F, n_solves, seed):
         "mean_queue_ms": round(1e3 * float(ph[:, 0].mean()), 2), "mean_setup_ms": round(1e3 * float(ph[:, 1].mean()), 2),
         "mean_solve_ms": round(1e3 * float(ph[:, 2].mean()), 2),
         "latency": "submit to solution on the host, including setup (model, hierarchy, coarse factor)",
+        "batching": {
+            "capacity_unbatched_solves_s": round(len(reqs) / solo_span, 2),
+            "capacity_batched_solves_s": round(capacity, 2),
+            "stream": bst, "capacity_run": cap_bst,
+            "solves_per_batch": round(bst["iterations"] / max(bst["batches"], 1), 2),
+            "launches_per_batch": round(bst["launches"] / max(bst["batches"], 1), 1),
+            "launches_per_batch_unbatched": round(bst["solo_launches"] / max(bst["batches"], 1), 1),
+            "note": "continuous batching: each PCG iteration due of every running solve goes out in one batch, each "
+                    "launch running one kernel (the same level of each V-cycle, the same PCG step) over all members; "
+                    "one captured hipGraph per set of members; unbatched: each solve replays its own iteration graph "
+                    "on its own stream"},
         "device_memory": {"arena_bytes": mem[0], "peak_bytes": mem[1], "allocations_outside_arena": mem[2],
                           "note": "every solve's device memory comes from one arena reserved at fem_server_new: "
                                   "no hipMalloc or hipFree while solves run"},

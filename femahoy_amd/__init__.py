@@ -135,7 +135,7 @@ class AssemblyStats(C.Structure):
 
 class _ServerOptions(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_active", C.c_int32), ("device", C.c_int32),
-                ("setup_threads", C.c_int32)]
+                ("setup_threads", C.c_int32), ("unbatched", C.c_int32)]
 
 
 class RequestInfo(C.Structure):
@@ -164,7 +164,7 @@ EXPORTS = (
     "fem_solver_set_profile", "fem_solver_profile", "fem_solver_time_kernels",
     "fem_server_new", "fem_server_submit", "fem_server_wait", "fem_server_cancel", "fem_server_free",
     "fem_solver_new_csr", "fem_solver_matrix", "fem_dense_solve_device",
-    "fem_dense_solve_device_noalloc", "fem_solver_crossover_counts", "fem_server_memory",
+    "fem_dense_solve_device_noalloc", "fem_solver_crossover_counts", "fem_server_memory", "fem_server_batching",
     "fem_assemble", "fem_assembly_stats_of", "fem_assembly_download", "fem_assembly_device", "fem_assembly_free",
 )
 
@@ -226,6 +226,7 @@ def lib() -> C.CDLL:
         "fem_server_cancel": (C.c_int, [P, C.c_uint64]),
         "fem_server_free": (None, [P]),
         "fem_server_memory": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+        "fem_server_batching": (C.c_int, [P, C.POINTER(BatchStats)]),
         "fem_solver_matrix": (C.c_int, [P, C.POINTER(C.c_uint64), P, P, P, ERR, SZ]),
         "fem_dense_solve_device": (C.c_int, [C.c_int, P, P, P, ERR, SZ]),
         "fem_dense_solve_device_noalloc": (C.c_int, [C.c_int, P, P, P, ERR, SZ]),
@@ -879,12 +880,23 @@ def assembly_pattern(mesh: Mesh):
     return ptr, idx[:ne.value], free[:nf.value]
 
 
+class BatchStats(C.Structure):
+    """fem_server_batch_stats: the serving loop's batched iterations."""
+
+    _fields_ = [("batches", C.c_uint64), ("iterations", C.c_uint64), ("launches", C.c_uint64),
+                ("solo_launches", C.c_uint64), ("graphs", C.c_uint64), ("graph_seconds", C.c_double),
+                ("launch_seconds", C.c_double), ("finish_seconds", C.c_double)]
+
+
 class Server:
     """The serving loop (fem_server_*): solves of uneven sizes admitted by device footprint, set up
-    on a thread of their own and advanced together, one PCG iteration per solve at a time."""
+    on a thread of their own and advanced together, one PCG iteration per solve at a time; with
+    batching (the default) the iterations due go out together, one launch per kernel of each level
+    over every running solve."""
 
-    def __init__(self, hbm_budget: int, max_active: int = 8, device: int = 0, setup_threads: int = 2):
-        o = _ServerOptions(int(hbm_budget), int(max_active), int(device), int(setup_threads))
+    def __init__(self, hbm_budget: int, max_active: int = 8, device: int = 0, setup_threads: int = 2,
+                 batched: bool = True):
+        o = _ServerOptions(int(hbm_budget), int(max_active), int(device), int(setup_threads), 0 if batched else 1)
         h = C.c_void_p()
         err = _err()
         if lib().fem_server_new(C.byref(o), C.byref(h), err, 2048):
@@ -931,6 +943,12 @@ class Server:
 
     def cancel(self, rid: int) -> bool:
         return lib().fem_server_cancel(self.h, rid) == 0
+
+    def batching(self) -> dict:
+        """Counts of the batched iterations (fem_server_batching)."""
+        b = BatchStats()
+        lib().fem_server_batching(self.h, C.byref(b))
+        return {f: (float if t is C.c_double else int)(getattr(b, f)) for f, t in BatchStats._fields_}
 
     def memory(self):
         """(arena bytes, peak bytes in use, allocations made outside the arena)."""

@@ -10,6 +10,7 @@
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <vector>
 
 namespace fem {
 
@@ -107,6 +108,68 @@ class DeviceArena {
 // While set on a thread, the Buffers allocated there come from this arena (and go back to it when
 // freed, from any thread); a request the arena cannot hold falls back to hipMalloc.
 inline thread_local DeviceArena* alloc_arena = nullptr;
+
+// The HIP objects of one solve: its stream, the pinned record its scalars are read back into, and
+// its events. A serving loop keeps them in a pool (HandlePool) and hands them from one solve to
+// the next, since creating and destroying them can wait for the whole device.
+struct SolveHandles {
+    hipStream_t s = nullptr;
+    double* pinned = nullptr;  // 16 doubles
+    hipEvent_t ev = nullptr, ev_sync = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
+
+    static SolveHandles create() {
+        SolveHandles h;
+        FEM_HIP(hipStreamCreateWithFlags(&h.s, hipStreamNonBlocking));
+        FEM_HIP(hipHostMalloc((void**)&h.pinned, 16 * sizeof(double), hipHostMallocDefault));
+        FEM_HIP(hipEventCreateWithFlags(&h.ev, hipEventDisableTiming));
+        FEM_HIP(hipEventCreateWithFlags(&h.ev_sync, hipEventDisableTiming));
+        FEM_HIP(hipEventCreate(&h.ev_t0));
+        FEM_HIP(hipEventCreate(&h.ev_t1));
+        return h;
+    }
+    void destroy() {
+        if (s) (void)hipStreamSynchronize(s);
+        if (pinned) (void)hipHostFree(pinned);
+        for (hipEvent_t e : {ev, ev_sync, ev_t0, ev_t1})
+            if (e) (void)hipEventDestroy(e);
+        if (s) (void)hipStreamDestroy(s);
+        *this = SolveHandles{};
+    }
+};
+
+// Handles given back by finished solves, taken by new ones (created when none is free). Thread-safe.
+class HandlePool {
+   public:
+    HandlePool() = default;
+    HandlePool(const HandlePool&) = delete;
+    HandlePool& operator=(const HandlePool&) = delete;
+    ~HandlePool() {
+        for (SolveHandles& h : free_) h.destroy();
+    }
+    SolveHandles take() {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            if (!free_.empty()) {
+                SolveHandles h = free_.back();
+                free_.pop_back();
+                return h;
+            }
+        }
+        return SolveHandles::create();
+    }
+    // h's stream must be idle.
+    void give(const SolveHandles& h) {
+        std::lock_guard<std::mutex> g(m_);
+        free_.push_back(h);
+    }
+
+   private:
+    std::mutex m_;
+    std::vector<SolveHandles> free_;
+};
+// While set on a thread, the solvers built there take their handles from this pool (and give them
+// back when destroyed, on any thread).
+inline thread_local HandlePool* solve_handles = nullptr;
 
 class Buffer {
    public:

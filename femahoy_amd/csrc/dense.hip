@@ -10,6 +10,7 @@
 // The pivot test is the reference's (crates/fem/src/cholesky.rs:22 and 92-99): a pivot at or
 // below 1e-10 of its diagonal in A stops the factor with "the matrix is not positive definite".
 #include "dense.hpp"
+#include "batch.hpp"
 
 #include <chrono>
 #include <cmath>
@@ -214,19 +215,34 @@ __global__ void k_extract(const double* Ainv, int np, int n, int ld, T* out) {
 
 // bc[j] = b[map[j]] for the n dense unknowns (bc is zero beyond n).
 template <class T>
-__global__ void k_coarse_gather(int n, const int64_t* __restrict__ map, const T* __restrict__ b, T* __restrict__ bc) {
-    const int j = blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void coarse_gather_body(unsigned blk, unsigned nblk, int n, const int64_t* __restrict__ map, const T* __restrict__ b, T* __restrict__ bc) {
+    const int j = blk * 256 + threadIdx.x;
     if (j < n) bc[j] = b[map[j]];
 }
+template <class T>
+__global__ void k_coarse_gather(int n, const int64_t* __restrict__ map, const T* __restrict__ b, T* __restrict__ bc) {
+    coarse_gather_body<T>(blockIdx.x, gridDim.x, n, map, b, bc);
+}
+template <class T>
+struct KCoarseGather {
+    static constexpr int NT = 256, MINB = 0;
+    int n;
+    const int64_t* map;
+    const T* b;
+    T* bc;
+    __device__ __forceinline__ void operator()(unsigned blk, unsigned nblk) const {
+        coarse_gather_body<T>(blk, nblk, n, map, b, bc);
+    }
+};
 
 // x[map[i]] = sum_j inv[i][j] bc[j]: one wave per row with 16-byte loads, 4 rows per block. Rows
 // and bc are zero padded to ld (a multiple of 64), so whole vectors are read.
 template <class T>
-__global__ void __launch_bounds__(256) k_coarse_gemv(const T* __restrict__ inv, int n, int ld, const T* __restrict__ bc,
+__device__ __forceinline__ void coarse_gemv_body(unsigned blk, unsigned nblk, const T* __restrict__ inv, int n, int ld, const T* __restrict__ bc,
                                                      const int64_t* __restrict__ map, T* __restrict__ x) {
     constexpr int W = 16 / sizeof(T);
     const int lane = threadIdx.x & 63;
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int row = blk * 4 + (threadIdx.x >> 6);
     if (row >= n) return;
     const T* a = inv + (size_t)row * ld;
     T s = T(0);
@@ -250,6 +266,24 @@ __global__ void __launch_bounds__(256) k_coarse_gemv(const T* __restrict__ inv, 
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if (lane == 0) x[map[row]] = s;
 }
+template <class T>
+__global__ void __launch_bounds__(256) k_coarse_gemv(const T* __restrict__ inv, int n, int ld, const T* __restrict__ bc,
+                                                     const int64_t* __restrict__ map, T* __restrict__ x) {
+    coarse_gemv_body<T>(blockIdx.x, gridDim.x, inv, n, ld, bc, map, x);
+}
+template <class T>
+struct KCoarseGemv {
+    static constexpr int NT = 256, MINB = 0;
+    const T* inv;
+    int n;
+    int ld;
+    const T* bc;
+    const int64_t* map;
+    T* x;
+    __device__ __forceinline__ void operator()(unsigned blk, unsigned nblk) const {
+        coarse_gemv_body<T>(blk, nblk, inv, n, ld, bc, map, x);
+    }
+};
 
 // x[i] (= or +=) sum_j inv[i][j] b[j] for i < n: FP64, one wave per row, 4 rows per block. The
 // rows of inv have leading dimension ld (a multiple of 64); b has n entries.
@@ -430,8 +464,8 @@ void DenseCoarse::solve(const T* b, T* x, hipStream_t s) const {
         inv = inv64_.data();
         bc = bc64_.data();
     }
-    hipLaunchKernelGGL(k_coarse_gather<T>, dim3((n_ + 255) / 256), dim3(256), 0, s, n_, map_.data(), b, bc);
-    hipLaunchKernelGGL(k_coarse_gemv<T>, dim3((n_ + 3) / 4), dim3(256), 0, s, inv, n_, ld_, (const T*)bc, map_.data(), x);
+    run_k(KCoarseGather<T>{n_, map_.data(), b, bc}, (unsigned)((n_ + 255) / 256), s);
+    run_k(KCoarseGemv<T>{inv, n_, ld_, (const T*)bc, map_.data(), x}, (unsigned)((n_ + 3) / 4), s);
 }
 
 template <class T>
@@ -450,8 +484,7 @@ void DenseCoarse::solve_rhs(T* x, hipStream_t s) const {
     const T* inv;
     if constexpr (sizeof(T) == 4) inv = inv32_.data();
     else inv = inv64_.data();
-    hipLaunchKernelGGL(k_coarse_gemv<T>, dim3((n_ + 3) / 4), dim3(256), 0, s, inv, n_, ld_, (const T*)rhs<T>(),
-                       map_.data(), x);
+    run_k(KCoarseGemv<T>{inv, n_, ld_, (const T*)rhs<T>(), map_.data(), x}, (unsigned)((n_ + 3) / 4), s);
 }
 
 template void DenseCoarse::solve<float>(const float*, float*, hipStream_t) const;

@@ -77,10 +77,11 @@ constexpr int RBLOCK = 1024;
 // The logical block of a 1-D launch, XCD-aware: workgroups go to the 8 XCDs round-robin by id
 // and each XCD has its own L2, so id p takes block (p % 8) * (G / 8) + p / 8 and neighbouring
 // blocks, which read each other's halo, share an L2. The tail beyond a multiple of 8 keeps its id.
-__device__ __forceinline__ size_t xcd_block() {
-    const unsigned G = gridDim.x, p = blockIdx.x, per = G / 8;
+__device__ __forceinline__ size_t xcd_block(unsigned p, unsigned G) {
+    const unsigned per = G / 8;
     return p < per * 8 ? (size_t)(p % 8) * per + p / 8 : (size_t)p;
 }
+__device__ __forceinline__ size_t xcd_block() { return xcd_block(blockIdx.x, gridDim.x); }
 
 // y = A x on the unknowns; 0 on fixed nodes.
 template <class OP, class T>
@@ -103,10 +104,10 @@ __global__ void __launch_bounds__(BLOCK) k_apply(OP op, const T* __restrict__ x,
 
 // r = b - A x.
 template <class OP, class T>
-__global__ void __launch_bounds__(BLOCK) k_residual(OP op, const T* __restrict__ x, const T* __restrict__ b,
+__device__ __forceinline__ void residual_body(unsigned blk, unsigned nblk, OP op, const T* __restrict__ x, const T* __restrict__ b,
                                                     T* __restrict__ r) {
     const GridDims g = op.dims();
-    const size_t node = xcd_block() * BLOCK + threadIdx.x;
+    const size_t node = xcd_block(blk, nblk) * BLOCK + threadIdx.x;
     if (node >= g.nodes()) return;
     int i, j, k;
     node_ijk(g, node, i, j, k);
@@ -120,15 +121,31 @@ __global__ void __launch_bounds__(BLOCK) k_residual(OP op, const T* __restrict__
 #pragma unroll
     for (int c = 0; c < OP::C; ++c) r[OP::C * node + c] = b[OP::C * node + c] - out[c];
 }
+template <class OP, class T>
+__global__ void __launch_bounds__(BLOCK) k_residual(OP op, const T* __restrict__ x, const T* __restrict__ b,
+                                                    T* __restrict__ r) {
+    residual_body<OP, T>(blockIdx.x, gridDim.x, op, x, b, r);
+}
+template <class OP, class T>
+struct KResidual {
+    static constexpr int NT = BLOCK, MINB = 0;
+    OP op;
+    const T* x;
+    const T* b;
+    T* r;
+    __device__ __forceinline__ void operator()(unsigned blk, unsigned nblk) const {
+        residual_body<OP, T>(blk, nblk, op, x, b, r);
+    }
+};
 
 // One Chebyshev step from a nonzero x: r = b - A x_in, d = c1 d + c2 D^-1 r, x_out = x_in + d.
 // D^-1 stored in S (T, or BF16).
 template <class OP, class T, class S = T>
-__global__ void __launch_bounds__(BLOCK) k_cheb(OP op, const T* __restrict__ xin, T* __restrict__ xout,
+__device__ __forceinline__ void cheb_body(unsigned blk, unsigned nblk, OP op, const T* __restrict__ xin, T* __restrict__ xout,
                                                 T* __restrict__ d, const T* __restrict__ b,
                                                 const S* __restrict__ invd, T c1, T c2) {
     const GridDims g = op.dims();
-    const size_t node = xcd_block() * BLOCK + threadIdx.x;
+    const size_t node = xcd_block(blk, nblk) * BLOCK + threadIdx.x;
     if (node >= g.nodes()) return;
     int i, j, k;
     node_ijk(g, node, i, j, k);
@@ -151,17 +168,56 @@ __global__ void __launch_bounds__(BLOCK) k_cheb(OP op, const T* __restrict__ xin
         xout[u] = xin[u] + dn;
     }
 }
+template <class OP, class T, class S = T>
+__global__ void __launch_bounds__(BLOCK) k_cheb(OP op, const T* __restrict__ xin, T* __restrict__ xout,
+                                                T* __restrict__ d, const T* __restrict__ b,
+                                                const S* __restrict__ invd, T c1, T c2) {
+    cheb_body<OP, T, S>(blockIdx.x, gridDim.x, op, xin, xout, d, b, invd, c1, c2);
+}
+template <class OP, class T, class S = T>
+struct KCheb {
+    static constexpr int NT = BLOCK, MINB = 0;
+    OP op;
+    const T* xin;
+    T* xout;
+    T* d;
+    const T* b;
+    const S* invd;
+    T c1;
+    T c2;
+    __device__ __forceinline__ void operator()(unsigned blk, unsigned nblk) const {
+        cheb_body<OP, T, S>(blk, nblk, op, xin, xout, d, b, invd, c1, c2);
+    }
+};
 
 // The first Chebyshev step from x = 0: d = c0 D^-1 b, x = d.
 template <class T, class S = T>
-__global__ void __launch_bounds__(BLOCK) k_cheb_first(size_t n, const T* __restrict__ b, const S* __restrict__ invd,
+__device__ __forceinline__ void cheb_first_body(unsigned blk, unsigned nblk, size_t n, const T* __restrict__ b, const S* __restrict__ invd,
                                                       T* __restrict__ x, T* __restrict__ d, T c0) {
-    const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    const size_t u = blk * (size_t)BLOCK + threadIdx.x;
     if (u >= n) return;
     const T v = c0 * ld<T>(invd, u) * b[u];
     d[u] = v;
     x[u] = v;
 }
+template <class T, class S = T>
+__global__ void __launch_bounds__(BLOCK) k_cheb_first(size_t n, const T* __restrict__ b, const S* __restrict__ invd,
+                                                      T* __restrict__ x, T* __restrict__ d, T c0) {
+    cheb_first_body<T, S>(blockIdx.x, gridDim.x, n, b, invd, x, d, c0);
+}
+template <class T, class S = T>
+struct KChebFirst {
+    static constexpr int NT = BLOCK, MINB = 0;
+    size_t n;
+    const T* b;
+    const S* invd;
+    T* x;
+    T* d;
+    T c0;
+    __device__ __forceinline__ void operator()(unsigned blk, unsigned nblk) const {
+        cheb_first_body<T, S>(blk, nblk, n, b, invd, x, d, c0);
+    }
+};
 
 // invd = 1 / diag(A); 0 on fixed nodes.
 template <class OP, class T>
@@ -185,9 +241,9 @@ __global__ void __launch_bounds__(BLOCK) k_inv_diag(OP op, T* __restrict__ invd)
 // as a tensor product. Kuhn P1 (tet4, and tet10 -> tet4 on the same cells): fine node f is the
 // midpoint of the Kuhn edge from floor(f/2) to ceil(f/2), componentwise.
 template <class T, int C, bool KUHN_P1>
-__global__ void __launch_bounds__(BLOCK) k_prolong_add(GridDims fg, GridDims cg, const T* __restrict__ ec,
+__device__ __forceinline__ void prolong_add_body(unsigned blk, unsigned nblk, GridDims fg, GridDims cg, const T* __restrict__ ec,
                                                        T* __restrict__ xf) {
-    const size_t node = xcd_block() * BLOCK + threadIdx.x;
+    const size_t node = xcd_block(blk, nblk) * BLOCK + threadIdx.x;
     if (node >= fg.nodes()) return;
     int i, j, k;
     node_ijk(fg, node, i, j, k);
@@ -233,16 +289,32 @@ __global__ void __launch_bounds__(BLOCK) k_prolong_add(GridDims fg, GridDims cg,
 #pragma unroll
     for (int c = 0; c < C; ++c) xf[C * node + c] += acc[c];
 }
+template <class T, int C, bool KUHN_P1>
+__global__ void __launch_bounds__(BLOCK) k_prolong_add(GridDims fg, GridDims cg, const T* __restrict__ ec,
+                                                       T* __restrict__ xf) {
+    prolong_add_body<T, C, KUHN_P1>(blockIdx.x, gridDim.x, fg, cg, ec, xf);
+}
+template <class T, int C, bool KUHN_P1>
+struct KProlongAdd {
+    static constexpr int NT = BLOCK, MINB = 0;
+    GridDims fg;
+    GridDims cg;
+    const T* ec;
+    T* xf;
+    __device__ __forceinline__ void operator()(unsigned blk, unsigned nblk) const {
+        prolong_add_body<T, C, KUHN_P1>(blk, nblk, fg, cg, ec, xf);
+    }
+};
 
 // The transpose: b_c = P^T r_f, gathered per coarse node; 0 on fixed coarse nodes. With invd,
 // also the coarse level's first Chebyshev step from x = 0: d_c = x_c = c0 D^-1 b_c. With dense_of
 // (the coarse level is the dense one), b_c goes to its dense index in dense_b instead of bc.
 template <class T, int C, bool KUHN_P1, class S>
-__global__ void __launch_bounds__(BLOCK) k_restrict(GridDims fg, GridDims cg, const T* __restrict__ rf,
+__device__ __forceinline__ void restrict_body(unsigned blk, unsigned nblk, GridDims fg, GridDims cg, const T* __restrict__ rf,
                                                     T* __restrict__ bc, const S* __restrict__ invd, T c0,
                                                     T* __restrict__ xc, T* __restrict__ dc,
                                                     const int32_t* __restrict__ dense_of, T* __restrict__ dense_b) {
-    const size_t node = xcd_block() * BLOCK + threadIdx.x;
+    const size_t node = xcd_block(blk, nblk) * BLOCK + threadIdx.x;
     if (node >= cg.nodes()) return;
     int I, J, K;
     node_ijk(cg, node, I, J, K);
@@ -310,6 +382,30 @@ __global__ void __launch_bounds__(BLOCK) k_restrict(GridDims fg, GridDims cg, co
         }
     }
 }
+template <class T, int C, bool KUHN_P1, class S>
+__global__ void __launch_bounds__(BLOCK) k_restrict(GridDims fg, GridDims cg, const T* __restrict__ rf,
+                                                    T* __restrict__ bc, const S* __restrict__ invd, T c0,
+                                                    T* __restrict__ xc, T* __restrict__ dc,
+                                                    const int32_t* __restrict__ dense_of, T* __restrict__ dense_b) {
+    restrict_body<T, C, KUHN_P1, S>(blockIdx.x, gridDim.x, fg, cg, rf, bc, invd, c0, xc, dc, dense_of, dense_b);
+}
+template <class T, int C, bool KUHN_P1, class S>
+struct KRestrict {
+    static constexpr int NT = BLOCK, MINB = 0;
+    GridDims fg;
+    GridDims cg;
+    const T* rf;
+    T* bc;
+    const S* invd;
+    T c0;
+    T* xc;
+    T* dc;
+    const int32_t* dense_of;
+    T* dense_b;
+    __device__ __forceinline__ void operator()(unsigned blk, unsigned nblk) const {
+        restrict_body<T, C, KUHN_P1, S>(blk, nblk, fg, cg, rf, bc, invd, c0, xc, dc, dense_of, dense_b);
+    }
+};
 
 // ------------------------------------------------------------------ FP64 PCG pieces
 // Scalars of one PCG solve, in device memory. The stopping test of the loop head
@@ -414,10 +510,10 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_start(OP op, const double* __rest
 
 // q = A p; partial sums of p.q.
 template <class OP>
-__global__ void __launch_bounds__(BLOCK) k_pcg_q(OP op, const double* __restrict__ p, double* __restrict__ q,
+__device__ __forceinline__ void pcg_q_body(unsigned blk, unsigned nblk, OP op, const double* __restrict__ p, double* __restrict__ q,
                                                  double* __restrict__ part) {
     const GridDims g = op.dims();
-    const size_t node = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    const size_t node = blk * (size_t)BLOCK + threadIdx.x;
     double pq = 0.0;
     if (node < g.nodes()) {
         int i, j, k;
@@ -434,14 +530,30 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_q(OP op, const double* __restrict
         }
     }
     pq = block_sum(pq);
-    if (threadIdx.x == 0) part[blockIdx.x] = pq;
+    if (threadIdx.x == 0) part[blk] = pq;
 }
+template <class OP>
+__global__ void __launch_bounds__(BLOCK) k_pcg_q(OP op, const double* __restrict__ p, double* __restrict__ q,
+                                                 double* __restrict__ part) {
+    pcg_q_body<OP>(blockIdx.x, gridDim.x, op, p, q, part);
+}
+template <class OP>
+struct KPcgQ {
+    static constexpr int NT = BLOCK, MINB = 0;
+    OP op;
+    const double* p;
+    double* q;
+    double* part;
+    __device__ __forceinline__ void operator()(unsigned blk, unsigned nblk) const {
+        pcg_q_body<OP>(blk, nblk, op, p, q, part);
+    }
+};
 
 // r -= alpha q, rv = (V) r for the V-cycle; partial sums of r.r (grid-stride: any grid size). With invd, also the first
 // Chebyshev step of the V-cycle's level 0 from x = 0: d0 = x0 = c0 D^-1 rv (the step k_cheb_first
 // takes, in the same arithmetic). x += alpha p is left to k_pcg_dir, which reads p anyway.
 template <class V, class S>
-__global__ void __launch_bounds__(BLOCK) k_pcg_update(size_t n, const PcgScalars* __restrict__ s,
+__device__ __forceinline__ void pcg_update_body(unsigned blk, unsigned nblk, size_t n, const PcgScalars* __restrict__ s,
                                                       double* __restrict__ r, const double* __restrict__ q,
                                                       V* __restrict__ rv, double* __restrict__ part,
                                                       const S* __restrict__ invd, V c0, V* __restrict__ x0,
@@ -449,7 +561,7 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_update(size_t n, const PcgScalars
     double rr = 0.0;
     if (s->go != 0.0) {
         const double alpha = s->alpha;
-        for (size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x; u < n; u += (size_t)gridDim.x * BLOCK) {
+        for (size_t u = blk * (size_t)BLOCK + threadIdx.x; u < n; u += (size_t)nblk * BLOCK) {
             const double ru = r[u] - alpha * q[u];
             r[u] = ru;
             const V bu = (V)ru;
@@ -463,16 +575,41 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_update(size_t n, const PcgScalars
         }
     }
     rr = block_sum(rr);
-    if (threadIdx.x == 0) part[blockIdx.x] = rr;
+    if (threadIdx.x == 0) part[blk] = rr;
 }
+template <class V, class S>
+__global__ void __launch_bounds__(BLOCK) k_pcg_update(size_t n, const PcgScalars* __restrict__ s,
+                                                      double* __restrict__ r, const double* __restrict__ q,
+                                                      V* __restrict__ rv, double* __restrict__ part,
+                                                      const S* __restrict__ invd, V c0, V* __restrict__ x0,
+                                                      V* __restrict__ d0) {
+    pcg_update_body<V, S>(blockIdx.x, gridDim.x, n, s, r, q, rv, part, invd, c0, x0, d0);
+}
+template <class V, class S>
+struct KPcgUpdate {
+    static constexpr int NT = BLOCK, MINB = 0;
+    size_t n;
+    const PcgScalars* s;
+    double* r;
+    const double* q;
+    V* rv;
+    double* part;
+    const S* invd;
+    V c0;
+    V* x0;
+    V* d0;
+    __device__ __forceinline__ void operator()(unsigned blk, unsigned nblk) const {
+        pcg_update_body<V, S>(blk, nblk, n, s, r, q, rv, part, invd, c0, x0, d0);
+    }
+};
 
 // Partial sums of r.z (grid-stride), z read from zv (if zv; also stored to z as FP64 if z) or
 // from z.
 template <class V>
-__global__ void __launch_bounds__(BLOCK) k_pcg_rz(size_t n, const V* __restrict__ zv, double* __restrict__ z,
+__device__ __forceinline__ void pcg_rz_body(unsigned blk, unsigned nblk, size_t n, const V* __restrict__ zv, double* __restrict__ z,
                                                   const double* __restrict__ r, double* __restrict__ part) {
     double rz = 0.0;
-    for (size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x; u < n; u += (size_t)gridDim.x * BLOCK) {
+    for (size_t u = blk * (size_t)BLOCK + threadIdx.x; u < n; u += (size_t)nblk * BLOCK) {
         double zu;
         if (zv) {
             zu = (double)zv[u];
@@ -483,15 +620,32 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_rz(size_t n, const V* __restrict_
         rz += r[u] * zu;
     }
     rz = block_sum(rz);
-    if (threadIdx.x == 0) part[blockIdx.x] = rz;
+    if (threadIdx.x == 0) part[blk] = rz;
 }
+template <class V>
+__global__ void __launch_bounds__(BLOCK) k_pcg_rz(size_t n, const V* __restrict__ zv, double* __restrict__ z,
+                                                  const double* __restrict__ r, double* __restrict__ part) {
+    pcg_rz_body<V>(blockIdx.x, gridDim.x, n, zv, z, r, part);
+}
+template <class V>
+struct KPcgRz {
+    static constexpr int NT = BLOCK, MINB = 0;
+    size_t n;
+    const V* zv;
+    double* z;
+    const double* r;
+    double* part;
+    __device__ __forceinline__ void operator()(unsigned blk, unsigned nblk) const {
+        pcg_rz_body<V>(blk, nblk, n, zv, z, r, part);
+    }
+};
 
 // x += alpha p (the iteration's update of x, multigrid.rs:741), then p = z + beta p; first: p = z
 // alone. z in the preconditioner's precision.
 template <class V>
-__global__ void __launch_bounds__(BLOCK) k_pcg_dir(size_t n, const PcgScalars* __restrict__ s, const V* __restrict__ z,
+__device__ __forceinline__ void pcg_dir_body(unsigned blk, unsigned nblk, size_t n, const PcgScalars* __restrict__ s, const V* __restrict__ z,
                                                    double* __restrict__ p, double* __restrict__ x, int first) {
-    const size_t u = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    const size_t u = blk * (size_t)BLOCK + threadIdx.x;
     if (u >= n) return;
     if (first) {
         p[u] = (double)z[u];
@@ -501,6 +655,24 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_dir(size_t n, const PcgScalars* _
         p[u] = (double)z[u] + s->beta * pu;
     }
 }
+template <class V>
+__global__ void __launch_bounds__(BLOCK) k_pcg_dir(size_t n, const PcgScalars* __restrict__ s, const V* __restrict__ z,
+                                                   double* __restrict__ p, double* __restrict__ x, int first) {
+    pcg_dir_body<V>(blockIdx.x, gridDim.x, n, s, z, p, x, first);
+}
+template <class V>
+struct KPcgDir {
+    static constexpr int NT = BLOCK, MINB = 0;
+    size_t n;
+    const PcgScalars* s;
+    const V* z;
+    double* p;
+    double* x;
+    int first;
+    __device__ __forceinline__ void operator()(unsigned blk, unsigned nblk) const {
+        pcg_dir_body<V>(blk, nblk, n, s, z, p, x, first);
+    }
+};
 
 // Sum of `count` partials in a fixed order, by one block of RBLOCK threads.
 __device__ __forceinline__ double sum_partials(const double* part, int count) {
@@ -510,13 +682,29 @@ __device__ __forceinline__ double sum_partials(const double* part, int count) {
 }
 
 // The scalars from block partials (pcg_scalars_apply): count_a of part_a, count_b of part_b.
-static __global__ void __launch_bounds__(RBLOCK) k_pcg_scalars(PcgScalars* s, const double* part_a, const double* part_b,
+__device__ __forceinline__ void pcg_scalars_body(unsigned blk, unsigned nblk, PcgScalars* s, const double* part_a, const double* part_b,
                                                        int count_a, int count_b, int what) {
     const double a = sum_partials(part_a, count_a);
     double b = 0.0;
     if (what == 0 || what == 5) b = sum_partials(part_b, count_b);
     if (threadIdx.x == 0) pcg_scalars_apply(s, a, b, what);
 }
+static __global__ void __launch_bounds__(RBLOCK) k_pcg_scalars(PcgScalars* s, const double* part_a, const double* part_b,
+                                                       int count_a, int count_b, int what) {
+    pcg_scalars_body(blockIdx.x, gridDim.x, s, part_a, part_b, count_a, count_b, what);
+}
+struct KPcgScalars {
+    static constexpr int NT = RBLOCK, MINB = 0;
+    PcgScalars* s;
+    const double* part_a;
+    const double* part_b;
+    int count_a;
+    int count_b;
+    int what;
+    __device__ __forceinline__ void operator()(unsigned blk, unsigned nblk) const {
+        pcg_scalars_body(blk, nblk, s, part_a, part_b, count_a, count_b, what);
+    }
+};
 
 // ------------------------------------------------------------------ small vector kernels
 template <class S, class D>

@@ -176,6 +176,7 @@ MarchArgs<T> march_args(Level<T>& L) {
 
 template <class T, int MODE, class S>
 void launch_march(const MarchArgs<T, S>& m, hipStream_t s) {
+    if (recording_unbatchable()) return;
     hipLaunchKernelGGL((k_march_hex8<T, MODE, true, MarchShape<T>::shuffle, MarchShape<T>::kz, S>), march_grid<T>(m.g),
                        dim3(BLOCK), 0, s, m);
 }
@@ -234,8 +235,8 @@ unsigned t10_run(T10Args<T, S> a, hipStream_t s) {
     const unsigned n1 = (unsigned)a.tx * a.ty * a.tz;
     const unsigned n2 = (unsigned)((a.nb + BLOCK - 1) / BLOCK);
     a.nb1 = (int)n1;
-    hipLaunchKernelGGL((k_t10_tile<T, TL::X, TL::Y, TL::Z, MODE, S>), dim3(n1), dim3(TL::X * TL::Y * TL::Z), 0, s, a);
-    if (n2) hipLaunchKernelGGL((k_t10_bound<T, TL::X, TL::Y, TL::Z, MODE, S>), dim3(n2), dim3(BLOCK), 0, s, a);
+    run_k(KT10Tile<T, TL::X, TL::Y, TL::Z, MODE, S>{a}, n1, s);
+    run_k(KT10Bound<T, TL::X, TL::Y, TL::Z, MODE, S>{a}, n2, s);
     return n1 + n2;
 }
 
@@ -277,7 +278,7 @@ void op_residual(Level<T>& L, const T* x, const T* b, T* r, hipStream_t s) {
         });
         return;
     }
-    with_op(L, [&](auto op) { launch((k_residual<decltype(op), T>), L.nodes(), s, op, x, b, r); });
+    with_op(L, [&](auto op) { run_k(KResidual<decltype(op), T>{op, x, b, r}, nblocks(L.nodes()), s); });
 }
 
 // invd = 1 / diag(A) on a level (0 on fixed nodes).
@@ -414,8 +415,8 @@ class Hierarchy {
         if (!first_done) {
             spend(3 * n * vb + L.invd_bytes());
             with_invd(L, [&](auto iv) {
-                launch(k_cheb_first<T, elem_t<decltype(iv)>>, n, s, n, (const T*)L.b.data(), iv, L.x(), L.d.data(),
-                       L.c2[0]);
+                run_k(KChebFirst<T, elem_t<decltype(iv)>>{n, (const T*)L.b.data(), iv, L.x(), L.d.data(), L.c2[0]},
+                      nblocks(n), s);
             });
         }
         const int deg = (int)L.c1.size();  // the level's smoothing steps
@@ -487,8 +488,9 @@ class Hierarchy {
         }
         with_op(L, [&](auto op) {
             with_invd(L, [&](auto iv) {
-                launch((k_cheb<decltype(op), T, elem_t<decltype(iv)>>), L.nodes(), s, op, (const T*)L.x(), L.x2(),
-                       L.d.data(), (const T*)L.b.data(), iv, c1, c2);
+                run_k(KCheb<decltype(op), T, elem_t<decltype(iv)>>{op, (const T*)L.x(), L.x2(), L.d.data(),
+                                                                 (const T*)L.b.data(), iv, c1, c2},
+                      nblocks(L.nodes()), s);
             });
         });
     }
@@ -515,22 +517,22 @@ class Hierarchy {
                 T* dc = Cc.d.data();
                 const T* rf = F.r.data();
                 if (F.kuhn) {
-                    if (C == 1) launch((k_restrict<T, 1, true, S>), n, s, F.g, Cc.g, rf, Cc.b.data(), inv, c0, xc, dc, of, db);
-                    else launch((k_restrict<T, 3, true, S>), n, s, F.g, Cc.g, rf, Cc.b.data(), inv, c0, xc, dc, of, db);
+                    if (C == 1) run_k(KRestrict<T, 1, true, S>{F.g, Cc.g, rf, Cc.b.data(), inv, c0, xc, dc, of, db}, nblocks(n), s);
+                    else run_k(KRestrict<T, 3, true, S>{F.g, Cc.g, rf, Cc.b.data(), inv, c0, xc, dc, of, db}, nblocks(n), s);
                 } else {
-                    if (C == 1) launch((k_restrict<T, 1, false, S>), n, s, F.g, Cc.g, rf, Cc.b.data(), inv, c0, xc, dc, of, db);
-                    else launch((k_restrict<T, 3, false, S>), n, s, F.g, Cc.g, rf, Cc.b.data(), inv, c0, xc, dc, of, db);
+                    if (C == 1) run_k(KRestrict<T, 1, false, S>{F.g, Cc.g, rf, Cc.b.data(), inv, c0, xc, dc, of, db}, nblocks(n), s);
+                    else run_k(KRestrict<T, 3, false, S>{F.g, Cc.g, rf, Cc.b.data(), inv, c0, xc, dc, of, db}, nblocks(n), s);
                 }
             });
         } else {
             const size_t n = F.nodes();
             spend(Cc.dofs() * vb + 2 * F.dofs() * vb);  // x_c in, x_f in and out
             if (F.kuhn) {
-                if (C == 1) launch((k_prolong_add<T, 1, true>), n, s, F.g, Cc.g, (const T*)Cc.x(), F.x());
-                else launch((k_prolong_add<T, 3, true>), n, s, F.g, Cc.g, (const T*)Cc.x(), F.x());
+                if (C == 1) run_k(KProlongAdd<T, 1, true>{F.g, Cc.g, (const T*)Cc.x(), F.x()}, nblocks(n), s);
+                else run_k(KProlongAdd<T, 3, true>{F.g, Cc.g, (const T*)Cc.x(), F.x()}, nblocks(n), s);
             } else {
-                if (C == 1) launch((k_prolong_add<T, 1, false>), n, s, F.g, Cc.g, (const T*)Cc.x(), F.x());
-                else launch((k_prolong_add<T, 3, false>), n, s, F.g, Cc.g, (const T*)Cc.x(), F.x());
+                if (C == 1) run_k(KProlongAdd<T, 1, false>{F.g, Cc.g, (const T*)Cc.x(), F.x()}, nblocks(n), s);
+                else run_k(KProlongAdd<T, 3, false>{F.g, Cc.g, (const T*)Cc.x(), F.x()}, nblocks(n), s);
             }
         }
     }
@@ -539,17 +541,20 @@ class Hierarchy {
 // ---------------------------------------------------------------------------------------------
 class Solver {
    public:
-    Solver(const BoxInput& in, const Options& o) : opt_(o) {
+    Solver(const BoxInput& in, const Options& o) : opt_(o), uid_(next_uid()) {
         const auto t0 = std::chrono::steady_clock::now();
         FEM_HIP(hipSetDevice(o.device));
         hipDeviceProp_t prop;
         FEM_HIP(hipGetDeviceProperties(&prop, o.device));
         device_name_ = std::string(prop.name[0] ? prop.name : "AMD GPU") + " (" + prop.gcnArchName + ")";
-        FEM_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
-        FEM_HIP(hipHostMalloc((void**)&pinned_, 16 * sizeof(double), hipHostMallocDefault));
-        FEM_HIP(hipEventCreateWithFlags(&ev_, hipEventDisableTiming));
-        FEM_HIP(hipEventCreate(&ev_t0_));
-        FEM_HIP(hipEventCreate(&ev_t1_));
+        handles_from_ = solve_handles;
+        handles_ = handles_from_ ? handles_from_->take() : SolveHandles::create();
+        s_ = handles_.s;
+        pinned_ = handles_.pinned;
+        ev_ = handles_.ev;
+        ev_sync_ = handles_.ev_sync;
+        ev_t0_ = handles_.ev_t0;
+        ev_t1_ = handles_.ev_t1;
         AllocMeter meter;
         AllocMeter* const outer_meter = alloc_meter;
         alloc_meter = &meter;
@@ -628,14 +633,13 @@ class Solver {
     ~Solver() {
         if (s_) (void)hipStreamSynchronize(s_);  // nothing of this solver may still run on the device
         if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-        if (pinned_) (void)hipHostFree(pinned_);
-        if (ev_) (void)hipEventDestroy(ev_);
-        if (ev_t0_) (void)hipEventDestroy(ev_t0_);
-        if (ev_t1_) (void)hipEventDestroy(ev_t1_);
-        if (s_) (void)hipStreamDestroy(s_);
+        if (handles_from_) handles_from_->give(handles_);
+        else handles_.destroy();
     }
 
     int method() const { return method_; }
+    // A number no other solver of this process has (addresses can be reused).
+    uint64_t uid() const { return uid_; }
     bool crossover_counts(uint64_t* entries, double* flops) const {
         if (!analysed_) return false;
         if (entries) *entries = counts_.entries;
@@ -942,20 +946,23 @@ class Solver {
     bool advance(bool wait) {
         for (;;) {
             if (run_.phase == Phase::Finished) return true;
+            if (run_.want_iter) return false;  // the serving loop's next batch runs the iteration
             if (run_.pending) {
+                const hipEvent_t ev = pending_ev_ ? pending_ev_ : ev_;
                 if (wait) {
                     // a blocking caller polls: the batch's readback reaches the host sooner than
                     // through hipEventSynchronize (heat 128^3: 1.0-1.5 % faster solves)
                     hipError_t q;
-                    while ((q = hipEventQuery(ev_)) == hipErrorNotReady) {
+                    while ((q = hipEventQuery(ev)) == hipErrorNotReady) {
                     }
                     FEM_HIP(q);
                 } else {
-                    const hipError_t q = hipEventQuery(ev_);
+                    const hipError_t q = hipEventQuery(ev);
                     if (q == hipErrorNotReady) return false;
                     FEM_HIP(q);
                 }
                 run_.pending = false;
+                pending_ev_ = nullptr;
             }
             try {
                 const PcgScalars ps = *reinterpret_cast<const PcgScalars*>(pinned_);
@@ -986,6 +993,7 @@ class Solver {
                     run_.it = 0;
                     run_.residual = std::sqrt(rr) / run_.norm_b;
                     run_.rate = 0.0;
+                    s_dirty_ = true;  // a batched iteration after these launches waits for them
                 } else {  // Iterating: a batch of iterations has completed (or stopped on the device)
                     const uint64_t it = (uint64_t)ps.it;
                     const double res = std::sqrt(ps.rr) / run_.norm_b;
@@ -1090,6 +1098,52 @@ class Solver {
     // 1: streams share the hardware queues, and a long batch on one holds up the others.
     void set_max_batch(int k) { max_batch_ = std::max(1, k); }
 
+    // ---- batched iterations (the serving loop) ----
+    // Records the launches of one PCG iteration (none of them runs): the iteration is batchable
+    // when every launch in it is a run_k one (not the hex8 marching kernels, not the direct method).
+    void record_iteration() {
+        if (method_ != FEM_METHOD_MULTIGRID) return;
+        Recorder rec;
+        const uint64_t before = ledger_;
+        tl_recorder = &rec;
+        try {
+            iteration_body();
+        } catch (...) {
+            tl_recorder = nullptr;
+            ledger_ = before;
+            throw;
+        }
+        tl_recorder = nullptr;
+        rec_bytes_ = ledger_ - before;
+        ledger_ = before;
+        if (!rec.unbatchable) recs_ = std::move(rec.recs);
+    }
+    bool batchable() const { return !recs_.empty(); }
+    // With on, each PCG iteration waits for the serving loop to run it in a batch.
+    void set_batched(bool on) { batched_ = on && batchable(); }
+    bool batched() const { return batched_; }
+    const std::vector<BatchRec>& iteration_records() const { return recs_; }
+    bool wants_iteration() const { return run_.want_iter && run_.phase != Phase::Finished; }
+    // Makes stream bs wait for the work this solve queued on its own stream since its last batched
+    // iteration.
+    void order_before(hipStream_t bs) {
+        if (!s_dirty_) return;
+        FEM_HIP(hipEventRecord(ev_sync_, s_));
+        FEM_HIP(hipStreamWaitEvent(bs, ev_sync_, 0));
+        s_dirty_ = false;
+    }
+    // The readback of the scalars on stream bs after a batched iteration.
+    void read_back_on(hipStream_t bs) {
+        FEM_HIP(hipMemcpyAsync(pinned_, scal_.data(), sizeof(PcgScalars), hipMemcpyDeviceToHost, bs));
+    }
+    // A batched iteration and its readback are queued; ev completes after them.
+    void iteration_queued(hipEvent_t ev) {
+        run_.want_iter = false;
+        run_.pending = true;
+        pending_ev_ = ev;
+        ledger_ += rec_bytes_;
+    }
+
     void solve_host(const double* b, double* x, int (*cancelled)(void*), void* ctx, fem_stats* st) {
         const size_t nf = model_.free.size();
         fb_.upload(b, nf, s_);
@@ -1106,7 +1160,14 @@ class Solver {
     }
 
    private:
+    SolveHandles handles_;                // s_, pinned_ and the events below
+    HandlePool* handles_from_ = nullptr;  // the pool they came from (and go back to), if any
+    static uint64_t next_uid() {
+        static std::atomic<uint64_t> n{0};
+        return ++n;
+    }
     Options opt_;
+    const uint64_t uid_;
     BoxModel model_;
     hipStream_t s_ = nullptr;
     std::string device_name_;
@@ -1121,11 +1182,19 @@ class Solver {
     int max_batch_ = 8;
     hipGraphExec_t graph_exec_ = nullptr;
     double* pinned_ = nullptr;  // host-visible scalars read back each iteration
+    // Batched iterations (the serving loop): the recorded launches of one PCG iteration and their
+    // algorithmic bytes; batched_ routes iterations to the loop's batches; the event of the batch
+    // whose readback is pending; s_dirty_: work was queued on s_ since the last batched iteration.
+    std::vector<BatchRec> recs_;
+    uint64_t rec_bytes_ = 0;
+    bool batched_ = false, s_dirty_ = false;
+    hipEvent_t pending_ev_ = nullptr, ev_sync_ = nullptr;
     hipEvent_t ev_ = nullptr, ev_t0_ = nullptr, ev_t1_ = nullptr;  // readback; device time of a solve
     enum class Phase { Idle, Start, Iterating, Finished };
     struct Run {
         Phase phase = Phase::Idle;
-        bool pending = false;  // a readback into pinned_ is in flight (ev_)
+        bool pending = false;  // a readback into pinned_ is in flight (ev_, or pending_ev_)
+        bool want_iter = false;  // batched: the next PCG iteration waits for the serving loop's batch
         uint64_t total_it = 0, it = 0;
         int restarts = 0;
         double norm_b = 0.0, residual = 0.0;
@@ -1599,8 +1668,7 @@ class Solver {
             return t10_run<T10_PCGQ>(a, s_);
         }
         with_op(L0, [&](auto op) {
-            launch_blocks((k_pcg_q<decltype(op)>), nblocks(L0.nodes()), s_, op, (const double*)p_.data(), q_.data(),
-                          partA_.data());
+            run_k(KPcgQ<decltype(op)>{op, (const double*)p_.data(), q_.data(), partA_.data()}, nblocks(L0.nodes()), s_);
         });
         return nblocks(L0.nodes());
     }
@@ -1660,9 +1728,10 @@ class Solver {
                    rblocks(n) * sizeof(double);
         with_invd(L, [&](auto iv) {
             using S = elem_t<decltype(iv)>;
-            launch_blocks((k_pcg_update<T, S>), rblocks(n), s_, n, (const PcgScalars*)scal_.data(), r_.data(),
-                          (const double*)q_.data(), L.b.data(), partB_.data(), smooths ? iv : (const S*)nullptr,
-                          smooths ? L.c2[0] : T(0), L.xa.data(), L.d.data());
+            run_k(KPcgUpdate<T, S>{n, (const PcgScalars*)scal_.data(), r_.data(), (const double*)q_.data(), L.b.data(),
+                                   partB_.data(), smooths ? iv : (const S*)nullptr, smooths ? L.c2[0] : T(0),
+                                   L.xa.data(), L.d.data()},
+                  rblocks(n), s_);
         });
         // r.r (partB_) waits for the scalars after r.z (what 5): the stopping test is read at the
         // next iteration's head
@@ -1677,7 +1746,7 @@ class Solver {
         const unsigned nq = outer_apply();
         prof_.end(s_);
         ledger_ += 2 * (uint64_t)nq * sizeof(double);  // the p.q partials, written and summed
-        launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)nullptr, (int)nq, (int)nq, 1);
+        run_k(KPcgScalars{sc, (const double*)partA_.data(), (const double*)nullptr, (int)nq, (int)nq, 1}, 1, s_);
         // the update, r.r and the stopping test, and the V-cycle's first step on level 0 (when
         // level 0 smooths); then the V-cycle, then r.z and beta
         // r.z: summed by level 0's last smoothing step where it fuses it, else here
@@ -1688,29 +1757,30 @@ class Solver {
             if (h32_->fuses_rz())
                 nrz = (unsigned)march_blocks<float>(h32_->lv[0].g);
             else
-                launch_blocks((k_pcg_rz<float>), nbv, s_, n, (const float*)h32_->lv[0].x(), (double*)nullptr,
-                              (const double*)r_.data(), partA_.data());
+                run_k(KPcgRz<float>{n, (const float*)h32_->lv[0].x(), (double*)nullptr, (const double*)r_.data(),
+                                    partA_.data()},
+                      nbv, s_);
         } else {
             update_and_vcycle(*h64_);
             if (h64_->fuses_rz())
                 nrz = (unsigned)march_blocks<double>(h64_->lv[0].g);
             else
-                launch_blocks((k_pcg_rz<double>), nbv, s_, n, (const double*)h64_->lv[0].x(), (double*)nullptr,
-                              (const double*)r_.data(), partA_.data());
+                run_k(KPcgRz<double>{n, (const double*)h64_->lv[0].x(), (double*)nullptr, (const double*)r_.data(),
+                                     partA_.data()},
+                      nbv, s_);
         }
         const bool fused_rz = h32_ ? h32_->fuses_rz() : h64_->fuses_rz();
         if (!fused_rz) ledger_ += n * zb + n * sizeof(double);  // z and r in
         // the r.z and r.r partials, summed; then z in, p and x in and out
         ledger_ += ((uint64_t)nrz + nbv) * sizeof(double) + (uint64_t)nrz * sizeof(double) + n * zb + 4 * n * sizeof(double);
-        launch_reduce(k_pcg_scalars, s_, sc, (const double*)partA_.data(), (const double*)partB_.data(), (int)nrz, (int)nbv,
-                      5);
+        run_k(KPcgScalars{sc, (const double*)partA_.data(), (const double*)partB_.data(), (int)nrz, (int)nbv, 5}, 1, s_);
         // z straight from the V-cycle's level-0 iterate (kept until the next V-cycle)
         if (h32_)
-            launch((k_pcg_dir<float>), n, s_, n, (const PcgScalars*)sc, (const float*)h32_->lv[0].x(), p_.data(),
-                   x_.data(), 0);
+            run_k(KPcgDir<float>{n, (const PcgScalars*)sc, (const float*)h32_->lv[0].x(), p_.data(), x_.data(), 0},
+                  nblocks(n), s_);
         else
-            launch((k_pcg_dir<double>), n, s_, n, (const PcgScalars*)sc, (const double*)h64_->lv[0].x(), p_.data(),
-                   x_.data(), 0);
+            run_k(KPcgDir<double>{n, (const PcgScalars*)sc, (const double*)h64_->lv[0].x(), p_.data(), x_.data(), 0},
+                  nblocks(n), s_);
     }
 
     // Every device allocation that a captured iteration's nodes read or write, and the pinned
@@ -1824,6 +1894,11 @@ class Solver {
                         std::to_string(opt_.max_iter) + " iterations");
         }
         if (run_.cancelled && run_.cancelled(run_.ctx)) throw Error("cancelled");
+        if (batched_) {
+            run_.want_iter = true;
+            run_.phase = Phase::Iterating;
+            return;
+        }
         const int k = batch_size();
         for (int q = 0; q < k; ++q) iterate();
         read_back();
@@ -1887,11 +1962,22 @@ class Server {
    public:
     // The device memory of every admitted solve (its vectors, hierarchy, coarse factor and the
     // factor's temporaries) comes from one arena of `budget` bytes reserved here.
-    Server(uint64_t budget, int max_active, int device, int setup_threads)
-        : budget_(budget), max_active_(max_active > 0 ? max_active : 1), device_(device), t0_(clock::now()) {
+    Server(uint64_t budget, int max_active, int device, int setup_threads, bool batch)
+        : budget_(budget), max_active_(max_active > 0 ? max_active : 1), device_(device), batch_(batch),
+          t0_(clock::now()) {
         FEM_HIP(hipSetDevice(device_));
         arena_ = std::make_unique<DeviceArena>(budget_);
+        FEM_HIP(hipStreamCreateWithFlags(&bs_, hipStreamNonBlocking));
+        FEM_HIP(hipMalloc(&dtable_, SLOTS * TABLE_BYTES));
+        FEM_HIP(hipHostMalloc((void**)&htable_, SLOTS * TABLE_BYTES, hipHostMallocDefault));
+        for (int k = 0; k < SLOTS; ++k) {
+            Slot& sl = slots_[k];
+            FEM_HIP(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
+            sl.dtable = static_cast<char*>(dtable_) + k * TABLE_BYTES;
+            sl.htable = htable_ + k * TABLE_BYTES;
+        }
         loop_ = std::thread([this] { loop(); });
+        reaper_ = std::thread([this] { reap_loop(); });
         for (int k = 0; k < std::max(1, setup_threads); ++k) setup_.emplace_back([this] { setup_loop(); });
     }
     // Stops the loop and the setup threads; a solve that had not finished ends with an error, and
@@ -1904,7 +1990,17 @@ class Server {
         cv_.notify_all();
         setup_cv_.notify_all();
         loop_.join();
+        reap_cv_.notify_all();
+        reaper_.join();  // after the loop: it drains the requests the loop handed over
         for (auto& t : setup_) t.join();
+        (void)hipStreamSynchronize(bs_);
+        for (Slot& sl : slots_) {
+            if (sl.graph) (void)hipGraphExecDestroy(sl.graph);
+            (void)hipEventDestroy(sl.ev);
+        }
+        (void)hipFree(dtable_);
+        (void)hipHostFree(htable_);
+        (void)hipStreamDestroy(bs_);
         std::unique_lock<std::mutex> g(m_);
         for (auto& kv : all_)
             if (!kv.second->done) {
@@ -1916,6 +2012,10 @@ class Server {
     }
 
     uint64_t arena_bytes() const { return arena_->size(); }
+    void batch_stats(fem_server_batch_stats* o) {
+        std::lock_guard<std::mutex> g(m_);
+        *o = bstats_;
+    }
     uint64_t arena_peak() { return arena_->peak(); }
     uint64_t arena_misses() { return arena_->misses(); }
 
@@ -2013,10 +2113,41 @@ class Server {
         if (!to_setup_.empty()) setup_cv_.notify_one();
     }
 
-    // The end of a request on the loop thread: its result leaves the device before the solver
-    // (whose memory goes back to the arena) does, then it is published under the lock, unless a
-    // failure of the whole server has already ended it.
+    // The end of a request: the loop thread hands it to the reaper thread, which takes its result
+    // off the device and releases the solver (a copy, a stream synchronization and frees that can
+    // wait for the device), so that the loop thread goes on launching batches meanwhile.
     void complete(const std::shared_ptr<ServeRequest>& r, std::string err) {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            reap_.emplace_back(r, std::move(err));
+        }
+        reap_cv_.notify_one();
+    }
+
+    void reap_loop() {
+        try {
+            FEM_HIP(hipSetDevice(device_));
+            for (;;) {
+                std::pair<std::shared_ptr<ServeRequest>, std::string> job;
+                {
+                    std::unique_lock<std::mutex> g(m_);
+                    reap_cv_.wait(g, [&] { return !reap_.empty() || (stop_ && loop_done_); });
+                    if (reap_.empty()) return;
+                    job = std::move(reap_.front());
+                    reap_.pop_front();
+                }
+                finish(job.first, std::move(job.second));
+            }
+        } catch (const std::exception& e) {
+            fail_all(e.what());
+        }
+    }
+
+    // The end of a request on the reaper thread: its result leaves the device before the solver
+    // (whose memory goes back to the arena) does, then it is published under the lock, unless a
+    // failure of the whole server has already ended it; then its slot and footprint are free.
+    void finish(const std::shared_ptr<ServeRequest>& r, std::string err) {
+        const auto t0 = clock::now();
         fem_stats st{};
         std::vector<double> x;
         if (err.empty()) err = r->setup_err;
@@ -2031,6 +2162,7 @@ class Server {
             r->solver.reset();
         }
         std::lock_guard<std::mutex> g(m_);
+        bstats_.finish_seconds += std::chrono::duration<double>(clock::now() - t0).count();
         r->t_finish = now();
         if (!r->done) {
             r->st = st;
@@ -2071,6 +2203,7 @@ class Server {
     void setup_body() {
         FEM_HIP(hipSetDevice(device_));
         alloc_arena = arena_.get();  // every solver this thread builds takes its memory from the arena
+        solve_handles = &handles_;   // and its stream, readback record and events from the pool
         for (;;) {
             std::shared_ptr<ServeRequest> r;
             {
@@ -2085,6 +2218,10 @@ class Server {
                 r->device_bytes = r->solver->device_bytes();
                 Solver& s = *r->solver;
                 s.set_max_batch(1);
+                if (batch_) {
+                    s.record_iteration();
+                    s.set_batched(true);
+                }
                 const size_t nf = s.free_count_host();
                 if (r->b.empty()) {
                     r->b.resize(nf);
@@ -2112,6 +2249,9 @@ class Server {
         } catch (const std::exception& e) {
             fail_all(e.what());
         }
+        std::lock_guard<std::mutex> g(m_);
+        loop_done_ = true;
+        reap_cv_.notify_all();
     }
 
     void loop_body() {
@@ -2131,6 +2271,12 @@ class Server {
                     });
                     continue;
                 }
+            }
+            for (Slot& sl : slots_) {
+                if (!sl.in_flight) continue;
+                const hipError_t q = hipEventQuery(sl.ev);
+                if (q == hipSuccess) sl.in_flight = false;
+                else if (q != hipErrorNotReady) FEM_HIP(q);
             }
             bool progressed = false;
             for (size_t k = 0; k < active.size();) {
@@ -2154,17 +2300,177 @@ class Server {
                     ++k;
                 }
             }
+            for (Slot& sl : slots_) {
+                if (sl.in_flight) continue;
+                std::vector<Solver*> members;
+                for (auto& r : active)
+                    if (r->solver && r->solver->wants_iteration()) members.push_back(r->solver.get());
+                if (members.empty()) break;
+                launch_tick(sl, members);
+                progressed = true;
+            }
             if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
         }
     }
 
+    // ---- continuous batching ----
+    static constexpr size_t TABLE_BYTES = 4u << 20;  // the functor tables of one batch
+    static constexpr int SLOTS = 2;                  // batches queued at once
+    struct TickStep {              // one launch of a batch: functors and block offsets in the table
+        void (*launch)(const void*, const unsigned*, unsigned, unsigned, hipStream_t);
+        size_t off_k, off_s;
+        unsigned n, blocks;
+    };
+    struct Slot {
+        hipEvent_t ev = nullptr;          // the end of its last batch
+        char* dtable = nullptr;           // its table (device) and host staging
+        uint8_t* htable = nullptr;
+        hipGraphExec_t graph = nullptr;   // its planned batch, once captured
+        std::vector<uint64_t> members;    // the solvers (uid) of the planned batch
+        std::vector<TickStep> steps;
+        size_t solo = 0;                  // their launches one solve at a time
+        bool in_flight = false;
+    };
+    // One batched PCG iteration of every running solve that waits for one; the set changes as
+    // solves start and finish. Two batches can be queued at once (slots, each with its own table
+    // and graph, both on the batch stream): the device runs one while the loop reads back and
+    // plans the other. The members' recorded iterations are merged in order: at each step
+    // the functor type at the head of the most members goes out as one launch over all of them
+    // (k_run_batch), and every member's own launches keep their order. The launches and the
+    // members' scalar readbacks are captured in one hipGraph, replayed while the members stay the
+    // same and rebuilt when they change. Each member's blocks run exactly its own launches, so a
+    // result is bitwise the same batched and alone.
+    // A set of members runs its first batch with direct launches (the set changes as often as
+    // solves start and finish, and capturing and instantiating a graph costs more host time than
+    // launching once); a set that comes due again in the next batch is captured and replayed.
+    void launch_tick(Slot& sl, const std::vector<Solver*>& members) {
+        const auto t0 = clock::now();
+        std::vector<uint64_t> ids;
+        for (Solver* s : members) ids.push_back(s->uid());
+        for (Solver* s : members) s->order_before(bs_);
+        double graph_s = 0.0;
+        if (ids == sl.members && sl.graph) {
+            FEM_HIP(hipGraphLaunch(sl.graph, bs_));
+        } else if (ids == sl.members) {
+            const auto tg = clock::now();
+            capture_tick(sl, members);
+            graph_s = std::chrono::duration<double>(clock::now() - tg).count();
+            FEM_HIP(hipGraphLaunch(sl.graph, bs_));
+        } else {
+            plan_tick(sl, members, ids);
+            enqueue_tick(sl, members);
+        }
+        FEM_HIP(hipEventRecord(sl.ev, bs_));
+        for (Solver* s : members) s->iteration_queued(sl.ev);
+        sl.in_flight = true;
+        std::lock_guard<std::mutex> g(m_);
+        ++bstats_.batches;
+        bstats_.iterations += members.size();
+        bstats_.launches += sl.steps.size();
+        bstats_.solo_launches += sl.solo;
+        bstats_.graph_seconds += graph_s;
+        bstats_.launch_seconds += std::chrono::duration<double>(clock::now() - t0).count();
+    }
+
+    // The batch's launches (from the device table) and the members' readbacks, on bs_.
+    void enqueue_tick(const Slot& sl, const std::vector<Solver*>& members) {
+        for (const TickStep& st : sl.steps)
+            st.launch(sl.dtable + st.off_k, reinterpret_cast<const unsigned*>(sl.dtable + st.off_s), st.n, st.blocks, bs_);
+        for (Solver* s : members) s->read_back_on(bs_);
+    }
+
+    void capture_tick(Slot& sl, const std::vector<Solver*>& members) {
+        hipGraph_t graph = nullptr;
+        FEM_HIP(hipStreamBeginCapture(bs_, hipStreamCaptureModeThreadLocal));
+        enqueue_tick(sl, members);
+        FEM_HIP(hipStreamEndCapture(bs_, &graph));
+        FEM_HIP(hipGraphInstantiate(&sl.graph, graph, nullptr, nullptr, 0));
+        FEM_HIP(hipGraphDestroy(graph));
+        std::lock_guard<std::mutex> g(m_);
+        ++bstats_.graphs;
+    }
+
+    // The merged launches of a new set of members and their device table.
+    void plan_tick(Slot& sl, const std::vector<Solver*>& members, const std::vector<uint64_t>& ids) {
+        const size_t m = members.size();
+        std::vector<size_t> head(m, 0);
+        using Step = TickStep;
+        std::vector<Step> steps;
+        size_t bytes = 0, solo = 0;
+        auto align = [&](size_t a) { bytes = (bytes + a - 1) / a * a; };
+        for (Solver* s : members) solo += s->iteration_records().size();
+        for (;;) {
+            // the functor type at the head of the most members (the first such in member order)
+            decltype(Step::launch) best = nullptr;
+            size_t best_count = 0;
+            for (size_t i = 0; i < m; ++i) {
+                const auto& ri = members[i]->iteration_records();
+                if (head[i] >= ri.size()) continue;
+                const auto f = ri[head[i]].launch;
+                size_t c = 0;
+                for (size_t j = 0; j < m; ++j) {
+                    const auto& rj = members[j]->iteration_records();
+                    if (head[j] < rj.size() && rj[head[j]].launch == f) ++c;
+                }
+                if (c > best_count) {
+                    best = f;
+                    best_count = c;
+                }
+            }
+            if (!best) break;
+            Step st{best, 0, 0, 0, 0};
+            align(64);
+            st.off_k = bytes;
+            std::vector<unsigned> starts(1, 0);
+            for (size_t i = 0; i < m; ++i) {
+                const auto& ri = members[i]->iteration_records();
+                if (head[i] >= ri.size() || ri[head[i]].launch != best) continue;
+                const BatchRec& rec = ri[head[i]];
+                if (bytes + rec.size > TABLE_BYTES) throw Error("internal: the batch table outgrew its buffer");
+                std::memcpy(sl.htable + bytes, rec.bytes.data(), rec.size);
+                bytes += rec.size;
+                starts.push_back(starts.back() + rec.blocks);
+                ++head[i];
+                ++st.n;
+            }
+            align(4);
+            st.off_s = bytes;
+            if (bytes + starts.size() * sizeof(unsigned) > TABLE_BYTES)
+                throw Error("internal: the batch table outgrew its buffer");
+            std::memcpy(sl.htable + bytes, starts.data(), starts.size() * sizeof(unsigned));
+            bytes += starts.size() * sizeof(unsigned);
+            st.blocks = starts.back();
+            steps.push_back(st);
+        }
+        if (sl.graph) {
+            FEM_HIP(hipGraphExecDestroy(sl.graph));
+            sl.graph = nullptr;
+        }
+        // the slot's previous batch has completed, so its table is free
+        FEM_HIP(hipMemcpyAsync(sl.dtable, sl.htable, bytes, hipMemcpyHostToDevice, bs_));
+        sl.members = ids;
+        sl.steps = std::move(steps);
+        sl.solo = solo;
+    }
+
+
     const uint64_t budget_;
     const int max_active_;
     const int device_;
+    const bool batch_;
+    hipStream_t bs_ = nullptr;     // the batches' stream
+    void* dtable_ = nullptr;       // the slots' functor tables (device) and their host staging
+    uint8_t* htable_ = nullptr;
+    Slot slots_[SLOTS];
+    fem_server_batch_stats bstats_{};
     const clock::time_point t0_;
     std::unique_ptr<DeviceArena> arena_;
+    HandlePool handles_;  // declared before the requests: outlives every solver
     std::mutex m_;
-    std::condition_variable cv_, setup_cv_, done_cv_, quiet_cv_;
+    std::condition_variable cv_, setup_cv_, done_cv_, quiet_cv_, reap_cv_;
+    std::deque<std::pair<std::shared_ptr<ServeRequest>, std::string>> reap_;  // finished, not yet released
+    bool loop_done_ = false;
+    std::thread reaper_;
     int waiters_ = 0;  // threads inside wait()
     bool stop_ = false;
     std::string fatal_;
@@ -2455,7 +2761,8 @@ int fem_server_new(const fem_server_options* opt, fem_server** out, char* err, s
     return guarded(err, errlen, [&] {
         if (!opt || !out) throw fem::Error("fem_server_new needs options and an output handle");
         auto* srv = new fem_server;
-        srv->impl = std::make_unique<fem::Server>(opt->hbm_budget, opt->max_active, opt->device, opt->setup_threads);
+        srv->impl = std::make_unique<fem::Server>(opt->hbm_budget, opt->max_active, opt->device, opt->setup_threads,
+                                                  opt->unbatched == 0);
         *out = srv;
     });
 }
@@ -2489,6 +2796,12 @@ int fem_server_memory(fem_server* srv, uint64_t* arena_bytes, uint64_t* peak_byt
     if (arena_bytes) *arena_bytes = srv->impl->arena_bytes();
     if (peak_bytes) *peak_bytes = srv->impl->arena_peak();
     if (misses) *misses = srv->impl->arena_misses();
+    return 0;
+}
+
+int fem_server_batching(fem_server* srv, fem_server_batch_stats* out) {
+    if (!srv || !out) return 1;
+    srv->impl->batch_stats(out);
     return 0;
 }
 

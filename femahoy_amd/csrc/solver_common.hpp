@@ -11,6 +11,7 @@
 #include <utility>
 
 #include "../../include/fem_solver.h"
+#include "batch.hpp"
 #include "kernels.hpp"
 
 namespace fem {
@@ -21,15 +22,18 @@ inline unsigned nblocks(size_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); 
 inline unsigned rblocks(size_t n) { return nblocks(n) < 2048u ? nblocks(n) : 2048u; }
 
 // Launches `kernel` on `blocks` blocks of BLOCK threads.
+// (These plain launches cannot be replayed from a Recorder's record: under one they launch
+// nothing and mark the recording unbatchable.)
 template <class... KArgs, class... Args>
 void launch_blocks(void (*kernel)(KArgs...), unsigned blocks, hipStream_t s, Args&&... args) {
-    if (blocks == 0) return;
+    if (blocks == 0 || recording_unbatchable()) return;
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(BLOCK), 0, s, static_cast<KArgs>(args)...);
 }
 
 // Launches a partial-sum kernel: one block of RBLOCK threads.
 template <class... KArgs, class... Args>
 void launch_reduce(void (*kernel)(KArgs...), hipStream_t s, Args&&... args) {
+    if (recording_unbatchable()) return;
     hipLaunchKernelGGL(kernel, dim3(1), dim3(RBLOCK), 0, s, static_cast<KArgs>(args)...);
 }
 

@@ -182,14 +182,14 @@ __device__ __forceinline__ void t10_exchange(T F[27][3], T* xch, int tid, bool s
 // FP32 asks for 2 waves per SIMD (at most 256 registers per lane), which two tiles of LDS per CU
 // allow; FP64 takes 1. Index arithmetic is 32-bit: the host keeps every byte offset below 2^31.
 template <class T, int TX, int TY, int TZ, int MODE, class S = T>
-__global__ void __launch_bounds__(TX * TY * TZ, sizeof(T) == 4 ? 2 : 1) k_t10_tile(T10Args<T, S> a) {
+__device__ __forceinline__ void t10_tile_body(unsigned blk, unsigned nblk, T10Args<T, S> a) {
     constexpr int NT = TX * TY * TZ, NW = NT / 64;
     constexpr int LX = 2 * TX + 1, LY = 2 * TY + 1, LZ = 2 * TZ + 1;
     constexpr int RS = 3 * LX;  // values per box row
     __shared__ T xs[LZ * LY * RS];
     __shared__ T xch[NT * 27];
     const int tid = threadIdx.x;
-    const int tile = (int)xcd_block();
+    const int tile = (int)xcd_block(blk, nblk);
     const int X0 = (tile % a.tx) * TX, Y0 = ((tile / a.tx) % a.ty) * TY, Z0 = (tile / (a.tx * a.ty)) * TZ;
     const unsigned NX = a.g.nx, NY = a.g.ny, NXY = NX * NY;
     const unsigned N3 = 3 * NXY * a.g.nz;
@@ -314,14 +314,26 @@ __global__ void __launch_bounds__(TX * TY * TZ, sizeof(T) == 4 ? 2 : 1) k_t10_ti
     }
     if (MODE == T10_PCGQ) {
         pq = block_sum_t<NT>(pq);
-        if (tid == 0) a.part[blockIdx.x] = pq;
+        if (tid == 0) a.part[blk] = pq;
     }
 }
+template <class T, int TX, int TY, int TZ, int MODE, class S = T>
+__global__ void __launch_bounds__(TX * TY * TZ, sizeof(T) == 4 ? 2 : 1) k_t10_tile(T10Args<T, S> a) {
+    t10_tile_body<T, TX, TY, TZ, MODE, S>(blockIdx.x, gridDim.x, a);
+}
+template <class T, int TX, int TY, int TZ, int MODE, class S = T>
+struct KT10Tile {
+    static constexpr int NT = TX * TY * TZ, MINB = (sizeof(T) == 4 ? 2 : 1);
+    T10Args<T, S> a;
+    __device__ __forceinline__ void operator()(unsigned blk, unsigned nblk) const {
+        t10_tile_body<T, TX, TY, TZ, MODE, S>(blk, nblk, a);
+    }
+};
 
 // The boundary nodes: the sum of their slots in slot order, then the mode's update.
 template <class T, int TX, int TY, int TZ, int MODE, class S = T>
-__global__ void __launch_bounds__(BLOCK) k_t10_bound(T10Args<T, S> a) {
-    const int e = blockIdx.x * BLOCK + threadIdx.x;
+__device__ __forceinline__ void t10_bound_body(unsigned blk, unsigned nblk, T10Args<T, S> a) {
+    const int e = blk * BLOCK + threadIdx.x;
     double pq = 0.0;
     if (e < a.nb) {
         const unsigned NX = a.g.nx, NY = a.g.ny, NXY = NX * NY;
@@ -350,9 +362,21 @@ __global__ void __launch_bounds__(BLOCK) k_t10_bound(T10Args<T, S> a) {
     }
     if (MODE == T10_PCGQ) {
         pq = block_sum(pq);
-        if (threadIdx.x == 0) a.part[a.nb1 + blockIdx.x] = pq;
+        if (threadIdx.x == 0) a.part[a.nb1 + blk] = pq;
     }
 }
+template <class T, int TX, int TY, int TZ, int MODE, class S = T>
+__global__ void __launch_bounds__(BLOCK) k_t10_bound(T10Args<T, S> a) {
+    t10_bound_body<T, TX, TY, TZ, MODE, S>(blockIdx.x, gridDim.x, a);
+}
+template <class T, int TX, int TY, int TZ, int MODE, class S = T>
+struct KT10Bound {
+    static constexpr int NT = BLOCK, MINB = 0;
+    T10Args<T, S> a;
+    __device__ __forceinline__ void operator()(unsigned blk, unsigned nblk) const {
+        t10_bound_body<T, TX, TY, TZ, MODE, S>(blk, nblk, a);
+    }
+};
 
 // A refined coordinate that the boundary launch finishes: on a face shared by two tiles, or the
 // grid's last.

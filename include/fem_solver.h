@@ -396,6 +396,10 @@ typedef struct fem_server_options {
     int32_t max_active;  /* solves admitted at once */
     int32_t device;      /* HIP device ordinal of every solve */
     int32_t setup_threads; /* host threads building admitted solves (model, hierarchy, factor); 0: 1 */
+    int32_t unbatched;   /* 0 (default): continuous batching, the running solves' PCG iterations go out
+                            together, each launch of a level covering that level of every solve
+                            (fem_server_batching); 1: each solve launches its own iterations on its
+                            own stream */
 } fem_server_options;
 
 typedef struct fem_request_info {
@@ -430,6 +434,26 @@ int fem_server_cancel(fem_server* srv, uint64_t id);
  * run. Gives the arena's size, the most of it in use at once, and the allocations that found no
  * room in it and were made apart (0 when the footprint estimates hold). */
 int fem_server_memory(fem_server* srv, uint64_t* arena_bytes, uint64_t* peak_bytes, uint64_t* misses);
+
+/* Continuous batching (no counterpart in the reference). Every running solve whose next PCG
+ * iteration is due joins the next batch: the launches of one iteration of each member (recorded
+ * once per solve at its setup) are merged in order, so that one launch runs the same kernel for
+ * every member at once (the same level of each solve's V-cycle, the same PCG step), each member's
+ * blocks doing exactly the work of its own launch. A batch with its readbacks is one captured
+ * hipGraph, rebuilt only when the set of members changes. Solves whose iteration holds a launch
+ * that cannot be batched (the hex8 marching kernels) run their own iterations as before. */
+typedef struct fem_server_batch_stats {
+    uint64_t batches;       /* batched PCG iterations launched */
+    uint64_t iterations;    /* solve iterations they carried (members summed over batches) */
+    uint64_t launches;      /* kernel launches of those batches */
+    uint64_t solo_launches; /* the launches the same iterations take one solve at a time */
+    uint64_t graphs;        /* batch graphs captured: a set of members due twice in a row is captured
+                               once and replayed; a new set is launched directly */
+    double graph_seconds;   /* loop-thread time capturing and instantiating batch graphs */
+    double launch_seconds;  /* loop-thread time planning and launching batches (graph time included) */
+    double finish_seconds;  /* loop-thread time finishing solves: the solution's copy, the solver's release */
+} fem_server_batch_stats;
+int fem_server_batching(fem_server* srv, fem_server_batch_stats* out);
 
 /* Stops the loop (after the running solves' current step) and frees every solve. A solve that had
  * not finished ends with the error "the server stopped before the solve finished"; threads inside

@@ -1,9 +1,10 @@
 """The serving loop (fem_server_*): each solve's result is the same alone and in a mix, admission
 respects the device-memory budget, and cancellation and refusal behave as documented.
 
-Tolerance: a solve in the mix must equal the same solve alone to 1e-12 relative L2; the loop runs
-the same launches on the solve's own stream and buffers, so the test also expects bitwise
-equality, and checks each against the oracle's direct solution to the 1e-8 parity bar.
+Tolerance: a solve in the mix must equal the same solve alone to 1e-12 relative L2. In a batch
+each solve's blocks run exactly the launches of its own iteration (on its own buffers), and
+unbatched each solve runs them on its own stream, so the tests also expect bitwise equality, and
+check each against the oracle's direct solution to the 1e-8 parity bar.
 """
 import numpy as np
 import pytest
@@ -57,6 +58,12 @@ def test_each_result_is_the_same_alone_and_in_the_mix(fem, oracle):
         # ran beside the running solves
         arena, peak, misses = srv.memory()
         assert arena >= 8 << 30 and 0 < peak <= arena and misses == 0
+        # the iterations went out in batches: more solve iterations than batches, and fewer
+        # launches than the same iterations one solve at a time
+        bs = srv.batching()
+        print(bs)
+        assert bs["batches"] > 0 and bs["iterations"] > bs["batches"]
+        assert bs["launches"] < bs["solo_launches"] and bs["graphs"] >= 1
     finally:
         srv.close()
     # and each agrees with the oracle's direct solve
@@ -154,3 +161,35 @@ def test_freeing_the_server_ends_a_waiting_request_and_its_waiter(fem):
     t.join(timeout=120)
     assert not t.is_alive()
     assert got["result"] == "the server stopped before the solve finished"
+
+
+def test_unbatched_and_marching_solves_match_their_solves_alone(fem):
+    """The loop without batching (each solve launches its own iterations), and a batching loop whose
+    mix holds hex8 boxes (their marching kernels are not batched: those solves run their own
+    iterations beside the batches): every result equals the same solve alone, bitwise."""
+    rng = np.random.default_rng(23)
+    cases = [(k, n, kw) for k, n, kw in specs() if max(n) <= 8]
+    hexes = []
+    for n in ([8, 8, 8], [12, 6, 6]):
+        k = np.exp(rng.standard_normal(int(np.prod(n))))
+        hexes.append(fem.Box("heat_hex8", n, [1.0, 1.0, 1.0], k, fixed={"x0": 0.0, "x1": 1.0}))
+    boxes = [make_box(fem, kind, n, kw) for kind, n, kw in cases] + hexes
+    alone = []
+    for b in boxes:
+        s = fem.Solver(b, **OPTS)
+        alone.append(s.solve(s.rhs())[0])
+    for batched in (False, True):
+        srv = fem.Server(hbm_budget=8 << 30, max_active=4, batched=batched)
+        try:
+            ids = [srv.submit(b, **OPTS) for b in boxes]
+            for b, rid, xa in zip(boxes, ids, alone):
+                x, st, _ = srv.wait(rid, len(xa), timeout=120)
+                assert np.array_equal(x, xa), (batched, b.kind)
+            bs = srv.batching()
+            print("batched" if batched else "unbatched", bs)
+            if batched:
+                assert bs["batches"] > 0
+            else:
+                assert bs["batches"] == 0
+        finally:
+            srv.close()
