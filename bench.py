@@ -525,6 +525,31 @@ def assembled_leg(F, hip, solver, box, x_ref, args, degree, k=None):
 FP64_MFMA_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix-core peak
 
 
+def many_leg(F, box, k, seed=11):
+    """k right sides against one setup: k solves one after another (fem_solver_solve), then the same
+    k at once (fem_solver_solve_many: shadow solvers sharing the hierarchy, their iterations
+    batched). Host vectors on both sides; wall time of the calls."""
+    s = F.Solver(box, method="multigrid", precision="fp32", tol=1e-8)
+    rng = np.random.default_rng(seed)
+    b = s.rhs()
+    scale = np.linalg.norm(b) / np.sqrt(s.n)
+    B = np.array([(1.0 + 0.1 * i) * b + 0.05 * scale * rng.standard_normal(s.n) for i in range(k)])
+    s.solve_many(B)  # the shadows are made on the first call; warm both paths
+    s.solve(B[0])
+    t0 = time.perf_counter()
+    seq = [s.solve(B[i]) for i in range(k)]
+    t_seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    X, stats = s.solve_many(B)
+    t_many = time.perf_counter() - t0
+    equal = all(np.array_equal(X[i], seq[i][0]) for i in range(k))
+    return {"right_sides": k, "free_dof": int(s.n), "iterations": [int(st.iterations) for st in stats],
+            "one_after_another_ms": round(1e3 * t_seq, 2), "together_ms": round(1e3 * t_many, 2),
+            "one_after_another_mdof_s": round(k * s.n / t_seq / 1e6, 2), "together_mdof_s": round(k * s.n / t_many / 1e6, 2),
+            "speedup": round(t_seq / t_many, 3), "bitwise_equal_to_alone": bool(equal),
+            "note": "wall time of the calls with host vectors (PCIe copies on both sides)"}
+
+
 def dense_factor_leg(F, hip, n, seed=5):
     """The dense FP64 LDL^T alone (the coarse level's and the direct method's factor, through
     fem_dense_solve_device): a random SPD matrix of n unknowns, the call timed from the host
@@ -582,6 +607,7 @@ def main():
                     help="cells along x of the nearly incompressible inclusion box (0: skip)")
     ap.add_argument("--inclusions", type=int, default=200, help="spheres in the inclusion box")
     ap.add_argument("--serve-n", type=int, default=256, help="solves of the serving stream (0: skip)")
+    ap.add_argument("--many", type=int, default=8, help="right sides of the multi-RHS leg on the tet10 block (0: skip)")
     ap.add_argument("--no-assembled", action="store_true", help="skip the assembled (CSR) boundary leg of the heat box")
     ap.add_argument("--dense", type=str, default="4335,20000",
                     help="sizes of the dense LDL^T factor timed alone (comma-separated; empty: skip)")
@@ -667,6 +693,9 @@ def main():
                                f"levels, dense FP64 LDLT coarse, {cheb_desc(vdeg, True)}, tol 1e-8")
         if eprof:
             elastic["finest_smoother"] = tet10_smoother(eprof)
+        if args.many > 0:
+            log(f"many right sides: {args.many} against one tet10 setup")
+            elastic["many_right_sides"] = many_leg(F, ebox, args.many)
 
     target = None
     if args.target_n > 0:

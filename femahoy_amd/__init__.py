@@ -155,6 +155,7 @@ class Profile(C.Structure):
 
 EXPORTS = (
     "fem_options_default", "fem_solver_new_box", "fem_solver_solve", "fem_solver_solve_device",
+    "fem_solver_solve_many", "fem_solver_solve_many_device",
     "fem_solver_method", "fem_solver_set_max_iter", "fem_solver_set_tol", "fem_solver_free",
     "fem_solver_free_count", "fem_solver_unknown_count", "fem_solver_free_list", "fem_solver_point_grid",
     "fem_solver_rhs", "fem_solver_apply", "fem_solver_precondition", "fem_solver_level_info",
@@ -194,6 +195,8 @@ def lib() -> C.CDLL:
         "fem_solver_new_box": (C.c_int, [C.POINTER(_Box), C.POINTER(_Options), C.POINTER(P), ERR, SZ]),
         "fem_solver_solve": (C.c_int, [P, F64P, F64P, CANCEL_FN, P, C.POINTER(_Stats), ERR, SZ]),
         "fem_solver_solve_device": (C.c_int, [P, P, P, CANCEL_FN, P, C.POINTER(_Stats), ERR, SZ]),
+        "fem_solver_solve_many": (C.c_int, [P, C.c_uint64, F64P, F64P, CANCEL_FN, P, P, ERR, SZ]),
+        "fem_solver_solve_many_device": (C.c_int, [P, C.c_uint64, P, P, CANCEL_FN, P, P, ERR, SZ]),
         "fem_solver_method": (C.c_int, [P]),
         "fem_solver_set_max_iter": (None, [P, C.c_uint64]),
         "fem_solver_set_tol": (None, [P, C.c_double]),
@@ -540,6 +543,27 @@ class Solver:
         if rc:
             raise FemError(err.value.decode())
         return x, _stats(st)
+
+    def solve_many(self, B, X=None, cancelled: Optional[Callable[[], bool]] = None):
+        """Solve A x_i = b_i for the k rows of B (k x n) against this one setup
+        (fem_solver_solve_many: the k solves run together, their iterations batched); X holds the
+        start values (zeros if None). Returns (X, [Stats]); each row equals solve() of that right
+        side. Raises the first right side's error after all have ended."""
+        B = np.ascontiguousarray(B, np.float64)
+        if B.ndim != 2 or B.shape[1] != self.n:
+            raise FemError(f"B needs k rows of {self.n} values")
+        k = B.shape[0]
+        X = np.zeros_like(B) if X is None else np.array(X, np.float64, copy=True, order="C")
+        if X.shape != B.shape:
+            raise FemError("X needs the shape of B")
+        cb = CANCEL_FN(lambda _ctx: 1 if cancelled() else 0) if cancelled else CANCEL_FN()
+        st = (_Stats * max(k, 1))()
+        err = _err()
+        rc = lib().fem_solver_solve_many(self.h, k, B.reshape(-1), X.reshape(-1), cb, None, C.cast(st, C.c_void_p), err,
+                                         2048)
+        if rc:
+            raise FemError(err.value.decode())
+        return X, [_stats(st[i]) for i in range(k)]
 
     def solve_device(self, d_b: int, d_x: int, cancelled: Optional[Callable[[], bool]] = None) -> Stats:
         """Solve with b and x already in device memory (raw HIP pointers)."""

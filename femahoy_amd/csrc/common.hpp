@@ -178,10 +178,11 @@ class Buffer {
     ~Buffer() { release(); }
     Buffer(const Buffer&) = delete;
     Buffer& operator=(const Buffer&) = delete;
-    Buffer(Buffer&& o) noexcept : p_(o.p_), n_(o.n_), arena_(o.arena_) {
+    Buffer(Buffer&& o) noexcept : p_(o.p_), n_(o.n_), arena_(o.arena_), owned_(o.owned_) {
         o.p_ = nullptr;
         o.n_ = 0;
         o.arena_ = nullptr;
+        o.owned_ = true;
     }
     Buffer& operator=(Buffer&& o) noexcept {
         if (this != &o) {
@@ -189,11 +190,20 @@ class Buffer {
             p_ = o.p_;
             n_ = o.n_;
             arena_ = o.arena_;
+            owned_ = o.owned_;
             o.p_ = nullptr;
             o.n_ = 0;
             o.arena_ = nullptr;
+            o.owned_ = true;
         }
         return *this;
+    }
+    // Refers to another buffer's memory without owning it (the owner must outlive this view).
+    void view(const Buffer& o) {
+        release();
+        p_ = o.p_;
+        n_ = o.n_;
+        owned_ = false;
     }
     void alloc(size_t bytes) {
         release();
@@ -209,6 +219,12 @@ class Buffer {
         }
     }
     void release() {
+        if (!owned_) {  // a view: nothing to free
+            p_ = nullptr;
+            n_ = 0;
+            owned_ = true;
+            return;
+        }
         if (p_) {
             if (arena_) arena_->give(p_, n_);
             else (void)hipFree(p_);
@@ -229,6 +245,7 @@ class Buffer {
     void* p_ = nullptr;
     size_t n_ = 0;
     DeviceArena* arena_ = nullptr;  // the arena the piece came from, or none (hipMalloc)
+    bool owned_ = true;             // false for a view of another buffer's memory
 };
 
 // A typed device array.
@@ -243,6 +260,16 @@ class DArray {
     }
     T* data() const { return buf_.as<T>(); }
     size_t size() const { return n_; }
+    // This array refers to o's memory (o must outlive it).
+    void view(const DArray& o) {
+        buf_.view(o.buf_);
+        n_ = o.n_;
+    }
+    // A zero-filled array of o's size.
+    void like(const DArray& o, hipStream_t s) {
+        resize(o.n_);
+        if (n_) FEM_HIP(hipMemsetAsync(data(), 0, bytes(), s));
+    }
     size_t bytes() const { return n_ * sizeof(T); }
     void upload(const T* host, size_t n, hipStream_t s = nullptr) {
         if (n > n_) throw Error("upload beyond the array");

@@ -451,6 +451,21 @@ void DenseCoarse::factor(double* a, int n, bool fp32, hipStream_t s) {
     factor_flops_ = dn * dn * dn / 3.0 + dn * dn * dn / 3.0 + dn * dn * dn / 3.0;
 }
 
+void DenseCoarse::shadow_of(const DenseCoarse& o, hipStream_t s) {
+    n_ = o.n_;
+    np_ = o.np_;
+    ld_ = o.ld_;
+    fp32_ = o.fp32_;
+    factor_seconds_ = 0.0;
+    factor_flops_ = 0.0;
+    inv64_.view(o.inv64_);
+    inv32_.view(o.inv32_);
+    map_.view(o.map_);
+    dense_of_.view(o.dense_of_);
+    bc64_.like(o.bc64_, s);
+    bc32_.like(o.bc32_, s);
+}
+
 template <class T>
 void DenseCoarse::solve(const T* b, T* x, hipStream_t s) const {
     const T* inv;

@@ -60,6 +60,10 @@ class DenseCoarse {
     // with the reference's "not positive definite" text when a pivot fails.
     void factor(double* a, int n, bool fp32, hipStream_t s);
 
+    // The factor of o for another solve of the same level: its inverse and maps viewed (o must
+    // outlive this one), its dense right side of its own (zero).
+    void shadow_of(const DenseCoarse& o, hipStream_t s);
+
     // x[map[i]] = sum_j inv[i][j] b[map[j]] for the n dense unknowns; map holds the grid unknown
     // of each dense unknown. T is float or double (the storage of the inverse must match).
     template <class T>

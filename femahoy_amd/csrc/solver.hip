@@ -71,6 +71,36 @@ struct Level {
     size_t ncell() const { return (size_t)cells[0] * cells[1] * cells[2]; }
     T* x() { return cur ? xb.data() : xa.data(); }
     T* x2() { return cur ? xa.data() : xb.data(); }
+    // The same level for another solve of the same model: the operator and smoother data
+    // (coefficients, D^-1, the boundary node list) are o's (viewed: o must outlive this level),
+    // the vectors and the partial-sum workspace its own (zero).
+    void shadow_of(const Level& o, hipStream_t s) {
+        op = o.op;
+        comps = o.comps;
+        g = o.g;
+        for (int d = 0; d < 3; ++d) {
+            cells[d] = o.cells[d];
+            h[d] = o.h[d];
+        }
+        kuhn = o.kuhn;
+        bf16 = o.bf16;
+        nbound = o.nbound;
+        lmax = o.lmax;
+        c1 = o.c1;
+        c2 = o.c2;
+        cur = 0;
+        coef.view(o.coef);
+        invd.view(o.invd);
+        coef16.view(o.coef16);
+        invd16.view(o.invd16);
+        blist.view(o.blist);
+        b.like(o.b, s);
+        xa.like(o.xa, s);
+        xb.like(o.xb, s);
+        d.like(o.d, s);
+        r.like(o.r, s);
+        P.like(o.P, s);
+    }
 };
 
 template <class T>
@@ -388,6 +418,15 @@ class Hierarchy {
         return rz_part && lv.size() > 1 && lv[0].march() && lv[0].nodes() * sizeof(double) < MARCH_MAX_BYTES;
     }
 
+    // The hierarchy for another solve of the same model: o's levels and coarse inverse, viewed,
+    // with vectors of its own (Level::shadow_of). rz_r, rz_part and ledger are the caller's to set.
+    void shadow_of(const Hierarchy& o, hipStream_t s) {
+        lv.resize(o.lv.size());
+        for (size_t l = 0; l < lv.size(); ++l) lv[l].shadow_of(o.lv[l], s);
+        coarse.shadow_of(o.coarse, s);
+        bf16 = o.bf16;
+    }
+
     void mark_begin(size_t l, hipStream_t s) {
         if (l == 0 && prof) prof->begin(0, s);
     }
@@ -538,6 +577,8 @@ class Hierarchy {
     }
 };
 
+class Batcher;
+
 // ---------------------------------------------------------------------------------------------
 class Solver {
    public:
@@ -630,12 +671,62 @@ class Solver {
         device_bytes_ = meter.peak;
     }
 
-    ~Solver() {
-        if (s_) (void)hipStreamSynchronize(s_);  // nothing of this solver may still run on the device
-        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-        if (handles_from_) handles_from_->give(handles_);
-        else handles_.destroy();
+    // A solver of the same model and hierarchy for another right side: the model, the operator and
+    // smoother data of every level and the coarse inverse are the base's (viewed: the base must
+    // outlive it), its vectors, workspaces, stream and events its own. It runs exactly the base's
+    // launches on its own buffers, so its result equals the base's for the same right side, bit for
+    // bit. No setup is repeated.
+    struct ShadowTag {};
+    Solver(const Solver& base, ShadowTag) : opt_(base.opt_), uid_(next_uid()) {
+        const auto t0 = std::chrono::steady_clock::now();
+        if (base.method_ != FEM_METHOD_MULTIGRID) throw Error("internal: only a multigrid solver has shadows");
+        FEM_HIP(hipSetDevice(opt_.device));
+        handles_ = SolveHandles::create();
+        s_ = handles_.s;
+        pinned_ = handles_.pinned;
+        ev_ = handles_.ev;
+        ev_sync_ = handles_.ev_sync;
+        ev_t0_ = handles_.ev_t0;
+        ev_t1_ = handles_.ev_t1;
+        model_ = base.model_;
+        device_name_ = base.device_name_;
+        method_ = base.method_;
+        use_graph_ = base.use_graph_;
+        max_batch_ = base.max_batch_;
+        analysed_ = base.analysed_;
+        counts_ = base.counts_;
+        n0_ = base.n0_;
+        parts_ = base.parts_;
+        lv64_.resize(base.lv64_.size());
+        for (size_t l = 0; l < lv64_.size(); ++l) lv64_[l].shadow_of(base.lv64_[l], s_);
+        const std::pair<DArray<double>*, const DArray<double>*> vecs[] = {
+            {&x_, &base.x_},         {&b_, &base.b_},         {&r_, &base.r_},   {&z_, &base.z_},
+            {&p_, &base.p_},         {&q_, &base.q_},         {&tmp_, &base.tmp_}, {&partA_, &base.partA_},
+            {&partB_, &base.partB_}, {&red_, &base.red_},     {&fb_, &base.fb_},   {&fx_, &base.fx_}};
+        for (const auto& v : vecs) v.first->like(*v.second, s_);
+        r32_.like(base.r32_, s_);
+        scal_.like(base.scal_, s_);
+        free_grid_.view(base.free_grid_);
+        auto hook = [&](auto& H) {
+            H.rz_r = r_.data();
+            H.rz_part = partA_.data();
+            H.ledger = &ledger_;
+        };
+        if (base.h32_) {
+            h32_ = std::make_unique<Hierarchy<float>>();
+            h32_->shadow_of(*base.h32_, s_);
+            hook(*h32_);
+        }
+        if (base.h64_) {
+            h64_ = std::make_unique<Hierarchy<double>>();
+            h64_->shadow_of(*base.h64_, s_);
+            hook(*h64_);
+        }
+        FEM_HIP(hipStreamSynchronize(s_));
+        setup_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
+
+    ~Solver();
 
     int method() const { return method_; }
     // A number no other solver of this process has (addresses can be reused).
@@ -1144,6 +1235,35 @@ class Solver {
         ledger_ += rec_bytes_;
     }
 
+    // k right sides at once (multi-RHS reuse of one hierarchy: transient steps, load cases):
+    // b_i = d_b + i n and x_i = d_x + i n on the device (n free unknowns; x_i the start value in,
+    // the solution out). This solver takes the first and k - 1 shadows (kept for later calls) the
+    // others; their PCG iterations run in batches, each launch covering the same level of every
+    // right side's V-cycle. Each result equals the solve of that right side alone, bit for bit.
+    // A direct solver, or a hierarchy whose iteration cannot be batched (hex8 boxes), solves them
+    // concurrently on their own streams instead (still from one setup). Stats per right side;
+    // throws the first right side's error after all have ended.
+    void solve_many_device(size_t k, const double* d_b, double* d_x, int (*cancelled)(void*), void* ctx,
+                           fem_stats* st);
+
+    // solve_many_device on host vectors (k * n values each).
+    void solve_many_host(size_t k, const double* b, double* x, int (*cancelled)(void*), void* ctx, fem_stats* st) {
+        const size_t nf = model_.free.size();
+        DArray<double> db(k * nf), dx(k * nf);
+        db.upload(b, k * nf, s_);
+        dx.upload(x, k * nf, s_);
+        FEM_HIP(hipStreamSynchronize(s_));
+        std::string error;
+        try {
+            solve_many_device(k, db.data(), dx.data(), cancelled, ctx, st);
+        } catch (const Error& e) {
+            error = e.what();
+        }
+        dx.download(x, k * nf, s_);
+        FEM_HIP(hipStreamSynchronize(s_));
+        if (!error.empty()) throw Error(error);
+    }
+
     void solve_host(const double* b, double* x, int (*cancelled)(void*), void* ctx, fem_stats* st) {
         const size_t nf = model_.free.size();
         fb_.upload(b, nf, s_);
@@ -1219,6 +1339,10 @@ class Solver {
     DArray<float> r32_;
     DArray<PcgScalars> scal_;
     DArray<int64_t> free_grid_;
+    // solve_many: the shadows of this solver (declared last: destroyed before what they view) and
+    // the batches of their iterations
+    std::unique_ptr<Batcher> batcher_;
+    std::vector<std::unique_ptr<Solver>> shadows_;
 
     // Automatic levels stop at the first level of at most this many unknowns. The dense coarse
     // solve costs n^2 per cycle and n^3 at setup; n0 / 256 keeps its per-cycle bytes within about
@@ -1926,6 +2050,282 @@ class Solver {
 };
 
 
+
+// ---------------------------------------------------------------------------------------------
+// Continuous batching of PCG iterations (no counterpart in the reference). One batch runs one
+// iteration of each member solve: their recorded iterations (Solver::record_iteration) are merged
+// in order, so that at each step the functor type at the head of the most members goes out as one
+// launch over all of them (k_run_batch), each member's launches keeping their order. One launch
+// then covers the same level of every member's V-cycle, or the same PCG step. Each member's blocks
+// run exactly its own launches on its own buffers, so a result is bitwise the same batched and
+// alone.
+//  - A new set of members is launched directly from a device table; a set that comes due again is
+//    captured with its readbacks into a hipGraph and replayed (capturing costs more host time than
+//    launching a set once, and sets change as often as solves start and finish).
+//  - Two batches can be queued at once (slots, each with its own table and graph, on one stream):
+//    the device runs one while the caller reads back and plans the other.
+class Batcher {
+   public:
+    Batcher() {
+        FEM_HIP(hipStreamCreateWithFlags(&bs_, hipStreamNonBlocking));
+        FEM_HIP(hipMalloc(&dtable_, SLOTS * TABLE_BYTES));
+        FEM_HIP(hipHostMalloc((void**)&htable_, SLOTS * TABLE_BYTES, hipHostMallocDefault));
+        for (int k = 0; k < SLOTS; ++k) {
+            Slot& sl = slots_[k];
+            FEM_HIP(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
+            sl.dtable = static_cast<char*>(dtable_) + k * TABLE_BYTES;
+            sl.htable = htable_ + k * TABLE_BYTES;
+        }
+    }
+    ~Batcher() {
+        (void)hipStreamSynchronize(bs_);
+        for (Slot& sl : slots_) {
+            if (sl.graph) (void)hipGraphExecDestroy(sl.graph);
+            (void)hipEventDestroy(sl.ev);
+        }
+        (void)hipFree(dtable_);
+        (void)hipHostFree(htable_);
+        (void)hipStreamDestroy(bs_);
+    }
+    Batcher(const Batcher&) = delete;
+    Batcher& operator=(const Batcher&) = delete;
+
+    // Marks the slots whose batch has completed as free (their members' readbacks are then on the
+    // host).
+    void poll() {
+        for (Slot& sl : slots_) {
+            if (!sl.in_flight) continue;
+            const hipError_t q = hipEventQuery(sl.ev);
+            if (q == hipSuccess) sl.in_flight = false;
+            else if (q != hipErrorNotReady) FEM_HIP(q);
+        }
+    }
+    bool has_free_slot() const {
+        for (const Slot& sl : slots_)
+            if (!sl.in_flight) return true;
+        return false;
+    }
+    bool idle() const { return !slots_[0].in_flight && !slots_[1].in_flight; }
+
+    // One batched iteration of each member (every member wants one) in a free slot.
+    void launch(const std::vector<Solver*>& members) {
+        const auto t0 = std::chrono::steady_clock::now();
+        Slot* free_slot = nullptr;
+        for (Slot& sl : slots_)
+            if (!sl.in_flight) {
+                free_slot = &sl;
+                break;
+            }
+        if (!free_slot) throw Error("internal: a batch was launched with no free slot");
+        Slot& sl = *free_slot;
+        std::vector<uint64_t> ids;
+        for (Solver* s : members) ids.push_back(s->uid());
+        for (Solver* s : members) s->order_before(bs_);
+        double graph_s = 0.0;
+        if (ids == sl.members && sl.graph) {
+            FEM_HIP(hipGraphLaunch(sl.graph, bs_));
+        } else if (ids == sl.members) {
+            const auto tg = std::chrono::steady_clock::now();
+            capture(sl, members);
+            graph_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - tg).count();
+            FEM_HIP(hipGraphLaunch(sl.graph, bs_));
+        } else {
+            plan(sl, members, ids);
+            enqueue(sl, members);
+        }
+        FEM_HIP(hipEventRecord(sl.ev, bs_));
+        for (Solver* s : members) s->iteration_queued(sl.ev);
+        sl.in_flight = true;
+        std::lock_guard<std::mutex> g(m_);
+        ++st_.batches;
+        st_.iterations += members.size();
+        st_.launches += sl.steps.size();
+        st_.solo_launches += sl.solo;
+        st_.graph_seconds += graph_s;
+        st_.launch_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    fem_server_batch_stats stats() {
+        std::lock_guard<std::mutex> g(m_);
+        return st_;
+    }
+
+   private:
+    static constexpr size_t TABLE_BYTES = 4u << 20;  // the functor tables of one batch
+    static constexpr int SLOTS = 2;                  // batches queued at once
+    struct TickStep {  // one launch of a batch: functors and block offsets in the table
+        void (*launch)(const void*, const unsigned*, unsigned, unsigned, hipStream_t);
+        size_t off_k, off_s;
+        unsigned n, blocks;
+    };
+    struct Slot {
+        hipEvent_t ev = nullptr;         // the end of its last batch
+        char* dtable = nullptr;          // its table (device) and host staging
+        uint8_t* htable = nullptr;
+        hipGraphExec_t graph = nullptr;  // its planned batch, once captured
+        std::vector<uint64_t> members;   // the solvers (uid) of the planned batch
+        std::vector<TickStep> steps;
+        size_t solo = 0;                 // their launches one solve at a time
+        bool in_flight = false;
+    };
+
+    // The batch's launches (from the device table) and the members' readbacks, on bs_.
+    void enqueue(const Slot& sl, const std::vector<Solver*>& members) {
+        for (const TickStep& st : sl.steps)
+            st.launch(sl.dtable + st.off_k, reinterpret_cast<const unsigned*>(sl.dtable + st.off_s), st.n, st.blocks, bs_);
+        for (Solver* s : members) s->read_back_on(bs_);
+    }
+
+    void capture(Slot& sl, const std::vector<Solver*>& members) {
+        hipGraph_t graph = nullptr;
+        FEM_HIP(hipStreamBeginCapture(bs_, hipStreamCaptureModeThreadLocal));
+        enqueue(sl, members);
+        FEM_HIP(hipStreamEndCapture(bs_, &graph));
+        FEM_HIP(hipGraphInstantiate(&sl.graph, graph, nullptr, nullptr, 0));
+        FEM_HIP(hipGraphDestroy(graph));
+        std::lock_guard<std::mutex> g(m_);
+        ++st_.graphs;
+    }
+
+    // The merged launches of a new set of members and their device table.
+    void plan(Slot& sl, const std::vector<Solver*>& members, const std::vector<uint64_t>& ids) {
+        const size_t m = members.size();
+        std::vector<size_t> head(m, 0);
+        std::vector<TickStep> steps;
+        size_t bytes = 0, solo = 0;
+        auto align = [&](size_t a) { bytes = (bytes + a - 1) / a * a; };
+        for (Solver* s : members) solo += s->iteration_records().size();
+        for (;;) {
+            // the functor type at the head of the most members (the first such in member order)
+            decltype(TickStep::launch) best = nullptr;
+            size_t best_count = 0;
+            for (size_t i = 0; i < m; ++i) {
+                const auto& ri = members[i]->iteration_records();
+                if (head[i] >= ri.size()) continue;
+                const auto f = ri[head[i]].launch;
+                size_t c = 0;
+                for (size_t j = 0; j < m; ++j) {
+                    const auto& rj = members[j]->iteration_records();
+                    if (head[j] < rj.size() && rj[head[j]].launch == f) ++c;
+                }
+                if (c > best_count) {
+                    best = f;
+                    best_count = c;
+                }
+            }
+            if (!best) break;
+            TickStep st{best, 0, 0, 0, 0};
+            align(64);
+            st.off_k = bytes;
+            std::vector<unsigned> starts(1, 0);
+            for (size_t i = 0; i < m; ++i) {
+                const auto& ri = members[i]->iteration_records();
+                if (head[i] >= ri.size() || ri[head[i]].launch != best) continue;
+                const BatchRec& rec = ri[head[i]];
+                if (bytes + rec.size > TABLE_BYTES) throw Error("internal: the batch table outgrew its buffer");
+                std::memcpy(sl.htable + bytes, rec.bytes.data(), rec.size);
+                bytes += rec.size;
+                starts.push_back(starts.back() + rec.blocks);
+                ++head[i];
+                ++st.n;
+            }
+            align(4);
+            st.off_s = bytes;
+            if (bytes + starts.size() * sizeof(unsigned) > TABLE_BYTES)
+                throw Error("internal: the batch table outgrew its buffer");
+            std::memcpy(sl.htable + bytes, starts.data(), starts.size() * sizeof(unsigned));
+            bytes += starts.size() * sizeof(unsigned);
+            st.blocks = starts.back();
+            steps.push_back(st);
+        }
+        if (sl.graph) {
+            FEM_HIP(hipGraphExecDestroy(sl.graph));
+            sl.graph = nullptr;
+        }
+        // the slot's previous batch has completed, so its table is free
+        FEM_HIP(hipMemcpyAsync(sl.dtable, sl.htable, bytes, hipMemcpyHostToDevice, bs_));
+        sl.members = ids;
+        sl.steps = std::move(steps);
+        sl.solo = solo;
+    }
+
+    hipStream_t bs_ = nullptr;  // the batches' stream
+    void* dtable_ = nullptr;    // the slots' functor tables (device) and their host staging
+    uint8_t* htable_ = nullptr;
+    Slot slots_[SLOTS];
+    std::mutex m_;  // st_
+    fem_server_batch_stats st_{};
+};
+
+Solver::~Solver() {
+    if (s_) (void)hipStreamSynchronize(s_);  // nothing of this solver may still run on the device
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (handles_from_) handles_from_->give(handles_);
+    else handles_.destroy();
+}
+
+void Solver::solve_many_device(size_t k, const double* d_b, double* d_x, int (*cancelled)(void*), void* ctx,
+                           fem_stats* st) {
+    const size_t nf = model_.free.size();
+    if (k == 0) return;
+    if (method_ != FEM_METHOD_MULTIGRID || k == 1) {
+        std::string first;
+        for (size_t i = 0; i < k; ++i) {
+            try {
+                solve_device(d_b + i * nf, d_x + i * nf, cancelled, ctx, st ? st + i : nullptr);
+            } catch (const Error& e) {
+                if (st) st[i] = run_.out;
+                if (first.empty()) first = e.what();
+            }
+        }
+        if (!first.empty()) throw Error(first);
+        return;
+    }
+    FEM_HIP(hipSetDevice(opt_.device));
+    while (shadows_.size() + 1 < k) shadows_.push_back(std::make_unique<Solver>(*this, ShadowTag{}));
+    std::vector<Solver*> all{this};
+    for (size_t i = 0; i + 1 < k; ++i) all.push_back(shadows_[i].get());
+    bool batch = true;
+    for (Solver* p : all) {
+        if (!p->batchable()) p->record_iteration();
+        batch = batch && p->batchable();
+    }
+    if (batch && !batcher_) batcher_ = std::make_unique<Batcher>();
+    struct Unbatch {  // the solvers run alone again after this call, whatever happens
+        std::vector<Solver*>& all;
+        ~Unbatch() {
+            for (Solver* p : all) p->set_batched(false);
+        }
+    } unbatch{all};
+    for (size_t i = 0; i < k; ++i) {
+        all[i]->set_batched(batch);
+        all[i]->start_run(d_b + i * nf, d_x + i * nf, cancelled, ctx);
+    }
+    for (;;) {
+        if (batch) batcher_->poll();
+        bool done = true;
+        for (Solver* p : all) {
+            if (!p->finished()) p->advance(false);
+            done = done && p->finished();
+        }
+        if (done) break;
+        if (!batch) continue;
+        while (batcher_->has_free_slot()) {
+            std::vector<Solver*> members;
+            for (Solver* p : all)
+                if (p->wants_iteration()) members.push_back(p);
+            if (members.empty()) break;
+            batcher_->launch(members);
+        }
+    }
+    std::string first;
+    for (size_t i = 0; i < k; ++i) {
+        if (st) st[i] = all[i]->run_.out;
+        if (first.empty() && !all[i]->run_.error.empty()) first = all[i]->run_.error;
+    }
+    if (!first.empty()) throw Error(first);
+}
+
 // ---------------------------------------------------------------------------------------------
 // The serving loop: a persistent request loop over Solver runs (no counterpart in the reference;
 // its analogues are Solver reuse across right sides, heat.rs:697-735, the cancel closure of
@@ -1967,15 +2367,7 @@ class Server {
           t0_(clock::now()) {
         FEM_HIP(hipSetDevice(device_));
         arena_ = std::make_unique<DeviceArena>(budget_);
-        FEM_HIP(hipStreamCreateWithFlags(&bs_, hipStreamNonBlocking));
-        FEM_HIP(hipMalloc(&dtable_, SLOTS * TABLE_BYTES));
-        FEM_HIP(hipHostMalloc((void**)&htable_, SLOTS * TABLE_BYTES, hipHostMallocDefault));
-        for (int k = 0; k < SLOTS; ++k) {
-            Slot& sl = slots_[k];
-            FEM_HIP(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
-            sl.dtable = static_cast<char*>(dtable_) + k * TABLE_BYTES;
-            sl.htable = htable_ + k * TABLE_BYTES;
-        }
+        batcher_ = std::make_unique<Batcher>();
         loop_ = std::thread([this] { loop(); });
         reaper_ = std::thread([this] { reap_loop(); });
         for (int k = 0; k < std::max(1, setup_threads); ++k) setup_.emplace_back([this] { setup_loop(); });
@@ -1993,14 +2385,7 @@ class Server {
         reap_cv_.notify_all();
         reaper_.join();  // after the loop: it drains the requests the loop handed over
         for (auto& t : setup_) t.join();
-        (void)hipStreamSynchronize(bs_);
-        for (Slot& sl : slots_) {
-            if (sl.graph) (void)hipGraphExecDestroy(sl.graph);
-            (void)hipEventDestroy(sl.ev);
-        }
-        (void)hipFree(dtable_);
-        (void)hipHostFree(htable_);
-        (void)hipStreamDestroy(bs_);
+        batcher_.reset();
         std::unique_lock<std::mutex> g(m_);
         for (auto& kv : all_)
             if (!kv.second->done) {
@@ -2013,8 +2398,9 @@ class Server {
 
     uint64_t arena_bytes() const { return arena_->size(); }
     void batch_stats(fem_server_batch_stats* o) {
+        *o = batcher_->stats();
         std::lock_guard<std::mutex> g(m_);
-        *o = bstats_;
+        o->finish_seconds = finish_seconds_;
     }
     uint64_t arena_peak() { return arena_->peak(); }
     uint64_t arena_misses() { return arena_->misses(); }
@@ -2162,7 +2548,7 @@ class Server {
             r->solver.reset();
         }
         std::lock_guard<std::mutex> g(m_);
-        bstats_.finish_seconds += std::chrono::duration<double>(clock::now() - t0).count();
+        finish_seconds_ += std::chrono::duration<double>(clock::now() - t0).count();
         r->t_finish = now();
         if (!r->done) {
             r->st = st;
@@ -2272,12 +2658,7 @@ class Server {
                     continue;
                 }
             }
-            for (Slot& sl : slots_) {
-                if (!sl.in_flight) continue;
-                const hipError_t q = hipEventQuery(sl.ev);
-                if (q == hipSuccess) sl.in_flight = false;
-                else if (q != hipErrorNotReady) FEM_HIP(q);
-            }
+            batcher_->poll();
             bool progressed = false;
             for (size_t k = 0; k < active.size();) {
                 auto& r = active[k];
@@ -2300,169 +2681,24 @@ class Server {
                     ++k;
                 }
             }
-            for (Slot& sl : slots_) {
-                if (sl.in_flight) continue;
+            while (batcher_->has_free_slot()) {
                 std::vector<Solver*> members;
                 for (auto& r : active)
                     if (r->solver && r->solver->wants_iteration()) members.push_back(r->solver.get());
                 if (members.empty()) break;
-                launch_tick(sl, members);
+                batcher_->launch(members);
                 progressed = true;
             }
             if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
         }
     }
 
-    // ---- continuous batching ----
-    static constexpr size_t TABLE_BYTES = 4u << 20;  // the functor tables of one batch
-    static constexpr int SLOTS = 2;                  // batches queued at once
-    struct TickStep {              // one launch of a batch: functors and block offsets in the table
-        void (*launch)(const void*, const unsigned*, unsigned, unsigned, hipStream_t);
-        size_t off_k, off_s;
-        unsigned n, blocks;
-    };
-    struct Slot {
-        hipEvent_t ev = nullptr;          // the end of its last batch
-        char* dtable = nullptr;           // its table (device) and host staging
-        uint8_t* htable = nullptr;
-        hipGraphExec_t graph = nullptr;   // its planned batch, once captured
-        std::vector<uint64_t> members;    // the solvers (uid) of the planned batch
-        std::vector<TickStep> steps;
-        size_t solo = 0;                  // their launches one solve at a time
-        bool in_flight = false;
-    };
-    // One batched PCG iteration of every running solve that waits for one; the set changes as
-    // solves start and finish. Two batches can be queued at once (slots, each with its own table
-    // and graph, both on the batch stream): the device runs one while the loop reads back and
-    // plans the other. The members' recorded iterations are merged in order: at each step
-    // the functor type at the head of the most members goes out as one launch over all of them
-    // (k_run_batch), and every member's own launches keep their order. The launches and the
-    // members' scalar readbacks are captured in one hipGraph, replayed while the members stay the
-    // same and rebuilt when they change. Each member's blocks run exactly its own launches, so a
-    // result is bitwise the same batched and alone.
-    // A set of members runs its first batch with direct launches (the set changes as often as
-    // solves start and finish, and capturing and instantiating a graph costs more host time than
-    // launching once); a set that comes due again in the next batch is captured and replayed.
-    void launch_tick(Slot& sl, const std::vector<Solver*>& members) {
-        const auto t0 = clock::now();
-        std::vector<uint64_t> ids;
-        for (Solver* s : members) ids.push_back(s->uid());
-        for (Solver* s : members) s->order_before(bs_);
-        double graph_s = 0.0;
-        if (ids == sl.members && sl.graph) {
-            FEM_HIP(hipGraphLaunch(sl.graph, bs_));
-        } else if (ids == sl.members) {
-            const auto tg = clock::now();
-            capture_tick(sl, members);
-            graph_s = std::chrono::duration<double>(clock::now() - tg).count();
-            FEM_HIP(hipGraphLaunch(sl.graph, bs_));
-        } else {
-            plan_tick(sl, members, ids);
-            enqueue_tick(sl, members);
-        }
-        FEM_HIP(hipEventRecord(sl.ev, bs_));
-        for (Solver* s : members) s->iteration_queued(sl.ev);
-        sl.in_flight = true;
-        std::lock_guard<std::mutex> g(m_);
-        ++bstats_.batches;
-        bstats_.iterations += members.size();
-        bstats_.launches += sl.steps.size();
-        bstats_.solo_launches += sl.solo;
-        bstats_.graph_seconds += graph_s;
-        bstats_.launch_seconds += std::chrono::duration<double>(clock::now() - t0).count();
-    }
-
-    // The batch's launches (from the device table) and the members' readbacks, on bs_.
-    void enqueue_tick(const Slot& sl, const std::vector<Solver*>& members) {
-        for (const TickStep& st : sl.steps)
-            st.launch(sl.dtable + st.off_k, reinterpret_cast<const unsigned*>(sl.dtable + st.off_s), st.n, st.blocks, bs_);
-        for (Solver* s : members) s->read_back_on(bs_);
-    }
-
-    void capture_tick(Slot& sl, const std::vector<Solver*>& members) {
-        hipGraph_t graph = nullptr;
-        FEM_HIP(hipStreamBeginCapture(bs_, hipStreamCaptureModeThreadLocal));
-        enqueue_tick(sl, members);
-        FEM_HIP(hipStreamEndCapture(bs_, &graph));
-        FEM_HIP(hipGraphInstantiate(&sl.graph, graph, nullptr, nullptr, 0));
-        FEM_HIP(hipGraphDestroy(graph));
-        std::lock_guard<std::mutex> g(m_);
-        ++bstats_.graphs;
-    }
-
-    // The merged launches of a new set of members and their device table.
-    void plan_tick(Slot& sl, const std::vector<Solver*>& members, const std::vector<uint64_t>& ids) {
-        const size_t m = members.size();
-        std::vector<size_t> head(m, 0);
-        using Step = TickStep;
-        std::vector<Step> steps;
-        size_t bytes = 0, solo = 0;
-        auto align = [&](size_t a) { bytes = (bytes + a - 1) / a * a; };
-        for (Solver* s : members) solo += s->iteration_records().size();
-        for (;;) {
-            // the functor type at the head of the most members (the first such in member order)
-            decltype(Step::launch) best = nullptr;
-            size_t best_count = 0;
-            for (size_t i = 0; i < m; ++i) {
-                const auto& ri = members[i]->iteration_records();
-                if (head[i] >= ri.size()) continue;
-                const auto f = ri[head[i]].launch;
-                size_t c = 0;
-                for (size_t j = 0; j < m; ++j) {
-                    const auto& rj = members[j]->iteration_records();
-                    if (head[j] < rj.size() && rj[head[j]].launch == f) ++c;
-                }
-                if (c > best_count) {
-                    best = f;
-                    best_count = c;
-                }
-            }
-            if (!best) break;
-            Step st{best, 0, 0, 0, 0};
-            align(64);
-            st.off_k = bytes;
-            std::vector<unsigned> starts(1, 0);
-            for (size_t i = 0; i < m; ++i) {
-                const auto& ri = members[i]->iteration_records();
-                if (head[i] >= ri.size() || ri[head[i]].launch != best) continue;
-                const BatchRec& rec = ri[head[i]];
-                if (bytes + rec.size > TABLE_BYTES) throw Error("internal: the batch table outgrew its buffer");
-                std::memcpy(sl.htable + bytes, rec.bytes.data(), rec.size);
-                bytes += rec.size;
-                starts.push_back(starts.back() + rec.blocks);
-                ++head[i];
-                ++st.n;
-            }
-            align(4);
-            st.off_s = bytes;
-            if (bytes + starts.size() * sizeof(unsigned) > TABLE_BYTES)
-                throw Error("internal: the batch table outgrew its buffer");
-            std::memcpy(sl.htable + bytes, starts.data(), starts.size() * sizeof(unsigned));
-            bytes += starts.size() * sizeof(unsigned);
-            st.blocks = starts.back();
-            steps.push_back(st);
-        }
-        if (sl.graph) {
-            FEM_HIP(hipGraphExecDestroy(sl.graph));
-            sl.graph = nullptr;
-        }
-        // the slot's previous batch has completed, so its table is free
-        FEM_HIP(hipMemcpyAsync(sl.dtable, sl.htable, bytes, hipMemcpyHostToDevice, bs_));
-        sl.members = ids;
-        sl.steps = std::move(steps);
-        sl.solo = solo;
-    }
-
-
     const uint64_t budget_;
     const int max_active_;
     const int device_;
     const bool batch_;
-    hipStream_t bs_ = nullptr;     // the batches' stream
-    void* dtable_ = nullptr;       // the slots' functor tables (device) and their host staging
-    uint8_t* htable_ = nullptr;
-    Slot slots_[SLOTS];
-    fem_server_batch_stats bstats_{};
+    std::unique_ptr<Batcher> batcher_;  // the batches of the running solves' iterations
+    double finish_seconds_ = 0.0;       // the reaper's time finishing solves
     const clock::time_point t0_;
     std::unique_ptr<DeviceArena> arena_;
     HandlePool handles_;  // declared before the requests: outlives every solver
@@ -2622,6 +2858,48 @@ int fem_solver_solve_device(fem_solver* s, const double* d_b, double* d_x, int (
         if (!s) throw fem::Error("no solver");
         if (s->csr) s->csr->solve_device(d_b, d_x, cancelled, ctx, stats);
         else s->impl->solve_device(d_b, d_x, cancelled, ctx, stats);
+    });
+}
+
+int fem_solver_solve_many(fem_solver* s, uint64_t k, const double* b, double* x, int (*cancelled)(void*), void* ctx,
+                          fem_stats* stats, char* err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!s) throw fem::Error("no solver");
+        if (s->csr) {
+            const size_t n = s->csr->n();
+            std::string first;
+            for (uint64_t i = 0; i < k; ++i) {
+                try {
+                    s->csr->solve_host(b + i * n, x + i * n, cancelled, ctx, stats ? stats + i : nullptr);
+                } catch (const fem::Error& e) {
+                    if (first.empty()) first = e.what();
+                }
+            }
+            if (!first.empty()) throw fem::Error(first);
+            return;
+        }
+        s->impl->solve_many_host(k, b, x, cancelled, ctx, stats);
+    });
+}
+
+int fem_solver_solve_many_device(fem_solver* s, uint64_t k, const double* d_b, double* d_x, int (*cancelled)(void*),
+                                 void* ctx, fem_stats* stats, char* err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!s) throw fem::Error("no solver");
+        if (s->csr) {
+            const size_t n = s->csr->n();
+            std::string first;
+            for (uint64_t i = 0; i < k; ++i) {
+                try {
+                    s->csr->solve_device(d_b + i * n, d_x + i * n, cancelled, ctx, stats ? stats + i : nullptr);
+                } catch (const fem::Error& e) {
+                    if (first.empty()) first = e.what();
+                }
+            }
+            if (!first.empty()) throw fem::Error(first);
+            return;
+        }
+        s->impl->solve_many_device(k, d_b, d_x, cancelled, ctx, stats);
     });
 }
 

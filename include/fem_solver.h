@@ -144,6 +144,21 @@ int fem_solver_solve(fem_solver* s, const double* b, double* x, int (*cancelled)
 int fem_solver_solve_device(fem_solver* s, const double* d_b, double* d_x, int (*cancelled)(void*), void* ctx,
                             fem_stats* stats, char* err, size_t errlen);
 
+/* Several right sides against one setup (no single counterpart in the reference; it replaces the
+ * loop of Solver::solve calls on one Solver of its transient steps, heat.rs:697-735, DC terminals,
+ * dc.rs:262-271, and shift-invert blocks, modes.rs:412). b and x hold k vectors of the free
+ * unknowns one after another (b + i n, x + i n); each x_i is that solve's start value and gets its
+ * solution; stats holds k entries. A box multigrid solver runs the k PCG solves together: it keeps
+ * k - 1 shadow solvers that share its hierarchy (coefficients, D^-1, coarse inverse) and own only
+ * their vectors, and batches their iterations (one launch per kernel over the same level of every
+ * right side). Each result equals fem_solver_solve of that right side, bit for bit. Errors as
+ * fem_solver_solve; after all k have ended, the first error is returned. */
+int fem_solver_solve_many(fem_solver* s, uint64_t k, const double* b, double* x, int (*cancelled)(void*), void* ctx,
+                          fem_stats* stats, char* err, size_t errlen);
+/* The same with b and x in device memory (k n values each). */
+int fem_solver_solve_many_device(fem_solver* s, uint64_t k, const double* d_b, double* d_x, int (*cancelled)(void*),
+                                 void* ctx, fem_stats* stats, char* err, size_t errlen);
+
 /* Solver::method (solve.rs:191). */
 int fem_solver_method(const fem_solver* s);
 
