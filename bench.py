@@ -300,7 +300,7 @@ def cpu_baseline(n_cpu, reference_tol=None):
 def tet10_smoother(prof):
     """The fused tet10 Chebyshev step (tile kernel and boundary pass) of a kernel timing."""
     gbs = prof["smoother_bytes"] / prof["smoother_s"] / 1e9
-    return {"kernel": "k_t10_tile<float> + k_t10_bound<float> (fused tet10 operator and Chebyshev update, FP32)",
+    return {"kernel": "KT10Tile<float> + KT10Bound<float> (fused tet10 operator and Chebyshev update, FP32, run by k_run)",
             "bytes_per_launch": int(prof["smoother_bytes"]), "us_per_launch": round(prof["smoother_s"] * 1e6, 2),
             "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
