@@ -116,61 +116,112 @@ __global__ void __launch_bounds__(256) k_gemm_nt(GemmArgs g) {
             }
 }
 
-// Factors the diagonal block kb in LDS: A_kk = L D L^T (L unit lower). Writes L into the block
-// (strict lower part), D into d[kb*64 ..], L^-1 into linv[kb] and D^-1 L^-1 into bk.
-// The trailing update takes 4 lanes per row with the pivot's reciprocal; L^-1 is formed one column
-// per thread in registers (fully unrolled forward substitution on the unit lower factor in LDS).
+// 1 / x in FP64: the hardware reciprocal refined by two Newton steps (the result is within an
+// ulp of the quotient; the IEEE division sequence is about three times as long, and it sits on the
+// pivot chain of the factor).
+__device__ __forceinline__ double recip(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+
+// Factors the diagonal block kb: A_kk = L D L^T (L unit lower). Writes L into the block (strict
+// lower part), D into d[kb*64 ..], L^-1 into linv[kb] and D^-1 L^-1 into bk.
+// Four lanes per row hold the row in registers (lane l: columns l, l + 4, ..., l + 60). Pivot step
+// j publishes the unscaled column j through a double-buffered LDS column behind one barrier (a
+// buffer is written again two steps later, after every thread has passed the barrier between), and
+// every row r > j takes S[r][c] -= (S[r][j] / d_j) S[c][j] for j < c <= r, then scales S[r][j] by
+// 1 / d_j: the order of a right-looking LDL^T. The 64 steps are unrolled so that every register
+// index is known at compile time (a register array indexed at run time lives in scratch memory;
+// a loop over the steps with selects in its place measured 2.4 times slower), and every update is
+// made with few branches: an FMA whose multiplier is 0 where an entry takes no update leaves it
+// as it is (a branch per entry cost more than the arithmetic, and a test per entry and step makes
+// the compiler hold every step's tests at once, in scratch). The first failing pivot is kept in a
+// register and reported once at the end. L^-1 is then formed column by column with four lanes per
+// column: lane l sums over the rows k = l mod 4 in four independent partial sums, and the lanes'
+// sums meet by two lane exchanges.
 __global__ void __launch_bounds__(256) k_ldlt_diag(double* A, int lda, int kb, const double* orig_diag, double* d,
                                                    double* linv, double* bk, int* status) {
-    __shared__ double S[NB][NB + 1];
-    const int tid = threadIdx.x;
+    __shared__ double col[2][NB];
+    __shared__ double L[NB][NB + 1];
+    __shared__ double piv[NB];
+    __shared__ double D[NB];
+    const int tid = threadIdx.x, row = tid >> 2, lane = tid & 3;
     double* blk = A + (size_t)kb * NB * lda + (size_t)kb * NB;
-    for (int e = tid; e < NB * NB; e += 256) {
-        const int r = e / NB, c = e % NB;
-        S[r][c] = r >= c ? blk[(size_t)r * lda + c] : 0.0;
+    double a[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int c = lane + 4 * q;
+        a[q] = c <= row ? blk[(size_t)row * lda + c] : 0.0;
+    }
+    if (tid < NB) piv[tid] = PIVOT * fabs(orig_diag[kb * NB + tid]);  // read after the first barrier
+    int bad = 0;  // the first failing pivot's column + 1
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const int q = j / 4;
+        const bool holds = lane == (j & 3);
+        double* cj = col[j & 1];
+        if (holds && row >= j) cj[row] = a[q];
+        __syncthreads();
+        double dj = cj[j];
+        const bool ok = dj > piv[j];
+        bad = bad == 0 && !ok ? kb * NB + j + 1 : bad;
+        dj = ok ? dj : 1.0;  // keep going with a harmless pivot; the host reports the failure
+        const double inv = recip(dj);
+        // rows past the pivot take S[r][c] -= lr S[c][j] on the columns past j: in the group of 4
+        // columns holding j the lanes past j's, in the groups above it every lane (the entries
+        // above the diagonal, c > r, take it too and are never read); rows up to the pivot take a
+        // multiplier of 0, which leaves them as they are
+        const double lr = row > j ? cj[row] * inv : 0.0;
+        const double lq = lane > (j & 3) ? lr : 0.0;
+#pragma unroll
+        for (int qq = 0; qq < 16; ++qq) {
+            if (qq < q) continue;  // groups of columns below j (known at compile time)
+            a[qq] = fma(qq == q ? -lq : -lr, cj[lane + 4 * qq], a[qq]);
+        }
+        if (holds) {
+            if (row > j) a[q] *= inv;
+            else if (row == j) a[q] = dj;
+        }
+    }
+    if (tid == 0 && bad && *status == 0) *status = bad;
+    // L (unit lower) and D to LDS, the strict lower part of L back to the block
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int c = lane + 4 * q;
+        if (c < row) blk[(size_t)row * lda + c] = a[q];
+        L[row][c] = c < row ? a[q] : (c == row ? 1.0 : 0.0);
+        if (c == row) D[row] = a[q];
     }
     __syncthreads();
-    const int row = tid >> 2, lane = tid & 3;
-    for (int j = 0; j < NB; ++j) {
-        double dj = S[j][j];
-        if (!(dj > PIVOT * fabs(orig_diag[kb * NB + j]))) {
-            if (tid == 0 && *status == 0) *status = kb * NB + j + 1;
-            dj = 1.0;  // keep going with a harmless pivot; the host reports the failure
-        }
-        const double inv = 1.0 / dj;
-        // trailing update of the lower triangle: S[r][c] -= S[r][j] S[c][j] / dj, j < c <= r
-        if (row > j) {
-            const double lr = S[row][j] * inv;
-            for (int c = j + 1 + lane; c <= row; c += 4) S[row][c] -= lr * S[c][j];
-        }
-        __syncthreads();
-        if (tid > j && tid < NB) S[tid][j] *= inv;
-        if (tid == 0) S[j][j] = dj;
-        __syncthreads();
-    }
-    // L^-1 (unit lower): thread c forms column c by forward substitution, in registers.
-    if (tid < NB) {
-        const int c = tid;
-        double li[NB];
+    if (tid < NB) d[kb * NB + tid] = D[tid];
+    // L^-1: column cc by four lanes, lane l holding rows r = l mod 4 (li[r / 4])
+    const int cc = tid >> 2;
+    double li[16];
 #pragma unroll
-        for (int r = 0; r < NB; ++r) {
-            double s = 0.0;
+    for (int q = 0; q < 16; ++q) li[q] = 0.0;
 #pragma unroll
-            for (int k = 0; k < r; ++k) s += S[r][k] * li[k];
-            li[r] = r == c ? 1.0 : (r < c ? 0.0 : -s);
-        }
-        double* lc = linv + (size_t)kb * NB * NB;
+    for (int r = 0; r < NB; ++r) {
+        double p4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int r = 0; r < NB; ++r) {
-            lc[r * NB + c] = li[r];
-            bk[r * NB + c] = li[r] / S[r][r];
+        for (int q = 0; q < 16; ++q) {
+            if (4 * q >= r) continue;  // rows k = l + 4 q < r only (known at compile time for 4 q >= r)
+            const int k = lane + 4 * q;
+            if (k < r) p4[q & 3] += L[r][k] * li[q];
         }
+        double part = (p4[0] + p4[1]) + (p4[2] + p4[3]);
+        part += __shfl_xor(part, 1, 64);
+        part += __shfl_xor(part, 2, 64);
+        if (lane == (r & 3)) li[r / 4] = r < cc ? 0.0 : (r == cc ? 1.0 : -part);
     }
-    for (int e = tid; e < NB * NB; e += 256) {
-        const int r = e / NB, c = e % NB;
-        if (r > c) blk[(size_t)r * lda + c] = S[r][c];
+    double* lc = linv + (size_t)kb * NB * NB;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int r = lane + 4 * q;
+        lc[r * NB + cc] = li[q];
+        bk[r * NB + cc] = li[q] / D[r];
     }
-    if (tid < NB) d[kb * NB + tid] = S[tid][tid];
 }
 
 __global__ void k_pad(const double* a, int n, double* A, int np) {
