@@ -26,8 +26,11 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfemahoy_amd.so")
 
-HEAT_HEX8, HEAT_TET4, ELASTIC_TET4, ELASTIC_TET10 = 0, 1, 2, 3
-KINDS = {"heat_hex8": HEAT_HEX8, "heat_tet4": HEAT_TET4, "elastic_tet4": ELASTIC_TET4, "elastic_tet10": ELASTIC_TET10}
+HEAT_HEX8, HEAT_TET4, ELASTIC_TET4, ELASTIC_TET10, ELASTIC_HEX8 = 0, 1, 2, 3, 4
+KINDS = {"heat_hex8": HEAT_HEX8, "heat_tet4": HEAT_TET4, "elastic_tet4": ELASTIC_TET4, "elastic_tet10": ELASTIC_TET10,
+         "elastic_hex8": ELASTIC_HEX8}
+HEX_KINDS = ("heat_hex8", "elastic_hex8")  # one material value per cell (the others: per tetrahedron)
+MULTIGRID_ASPECT = 2.0  # elastic.rs:276: cells flatter than this are 'flat' to the FLOPS crossover
 METHODS = {None: -1, "direct": 0, "multigrid": 1}
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2}
 FACES = ("x0", "x1", "y0", "y1", "z0", "z1")
@@ -251,6 +254,9 @@ def lib() -> C.CDLL:
         "fem_assembly_free": (None, [P]),
         "fem_test_assembly_pattern": (C.c_int, [C.POINTER(_Mesh), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), P, P,
                                                 P, ERR, SZ]),
+        "fem_test_hex_element": (C.c_int, [F64P, C.c_double, C.c_double, F64P, ERR, SZ]),
+        "fem_test_hex_box_system": (C.c_int, [C.POINTER(_Box), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), P, P, P,
+                                              P, C.POINTER(C.c_double), ERR, SZ]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -288,7 +294,7 @@ class Box:
         for d in range(3):
             b.n[d] = int(self.n[d])
             b.length[d] = float(self.length[d])
-        count = self.cells() * (1 if self.kind == "heat_hex8" else 6)
+        count = self.cells() * (1 if self.kind in HEX_KINDS else 6)
         c0 = np.ascontiguousarray(np.broadcast_to(np.asarray(self.coef0, np.float64), (count,)))
         keep = [c0]
         b.coef0 = c0.ctypes.data_as(C.POINTER(C.c_double))
@@ -387,6 +393,18 @@ class Model:
         return (ijk + frac[kind]) * h
 
 
+def box_aspect(box: "Box") -> float:
+    """The largest ratio of a cell's longest side to its shortest (elastic.rs:525-536), the cell
+    sides being the differences of the planes h i (h = length / n) as the library makes them."""
+    sides = []
+    for d in range(3):
+        h = float(box.length[d]) / float(box.n[d])
+        pl = np.arange(int(box.n[d]) + 1, dtype=np.float64) * h
+        sides.append(np.unique(np.diff(pl)))
+    g = np.stack(np.meshgrid(*sides, indexing="ij"), axis=-1).reshape(-1, 3)
+    return float(np.max(g.max(axis=1) / g.min(axis=1)))
+
+
 @dataclass
 class Flops:
     """Crossover::Flops (solve.rs:28-39): direct when the nested-dissection factor has at most
@@ -419,9 +437,14 @@ class Solver:
     """Solver::new / solve / method / with_max_iter over the device library."""
 
     def __init__(self, box: Box, method: Optional[str] = None, precision: str = "fp32", tol: float = PCG_TOL,
-                 max_iter: int = PCG_MAX_ITER, crossover: int = STEADY_DIRECT_LIMIT, levels: int = 0,
+                 max_iter: int = PCG_MAX_ITER, crossover=None, levels: int = 0,
                  cheb_degree: int = 0, cheb_lower: float = 0.1, cheb_upper: float = 1.1, power_steps: int = 20,
                  coarse_max: int = 6000, refine: bool = True, device: int = 0, coarse_galerkin: bool = True):
+        if crossover is None:
+            # the box-grid elasticity caller's choice (elastic.rs:525-547): Crossover::Flops, flat when
+            # a cell's longest side is more than MULTIGRID_ASPECT times its shortest; else Unknowns
+            crossover = (Flops(flat=box_aspect(box) > MULTIGRID_ASPECT) if box.kind == "elastic_hex8"
+                         else STEADY_DIRECT_LIMIT)
         o = _options(method, precision, tol, max_iter, crossover, levels, cheb_degree, cheb_lower, cheb_upper,
                      power_steps, coarse_max, refine, device, coarse_galerkin)
         b, keep = box._struct()
@@ -902,6 +925,37 @@ def assembly_pattern(mesh: Mesh):
                                        free.ctypes.data, err, 2048):
         raise FemError(err.value.decode())
     return ptr, idx[:ne.value], free[:nf.value]
+
+
+def hex_element(size, e: float, nu: float) -> np.ndarray:
+    """The condensed incompatible-mode hex of a box cell (hex.rs:199-279), 24 x 24, by the library's
+    host code (no device)."""
+    out = np.zeros(576)
+    err = _err()
+    if lib().fem_test_hex_element(np.ascontiguousarray(size, np.float64), float(e), float(nu), out, err, 2048):
+        raise FemError(err.value.decode())
+    return out.reshape(24, 24)
+
+
+def hex_box_system(box: Box):
+    """The reduced system fem_solver_new_box assembles for an elastic_hex8 box, on the host (no
+    device): ((ptr, idx, val), rhs, aspect)."""
+    b, keep = box._struct()
+    n, nnz, aspect = C.c_uint64(), C.c_uint64(), C.c_double()
+    err = _err()
+    L = lib()
+    if L.fem_test_hex_box_system(C.byref(b), C.byref(n), C.byref(nnz), None, None, None, None, C.byref(aspect), err,
+                                 2048):
+        raise FemError(err.value.decode())
+    ptr = np.zeros(n.value + 1, np.uint64)
+    idx = np.zeros(max(nnz.value, 1), np.uint32)
+    val = np.zeros(max(nnz.value, 1))
+    rhs = np.zeros(max(n.value, 1))
+    if L.fem_test_hex_box_system(C.byref(b), C.byref(n), C.byref(nnz), ptr.ctypes.data, idx.ctypes.data,
+                                 val.ctypes.data, rhs.ctypes.data, C.byref(aspect), err, 2048):
+        raise FemError(err.value.decode())
+    del keep
+    return (ptr, idx[:nnz.value], val[:nnz.value]), rhs[:n.value], aspect.value
 
 
 class BatchStats(C.Structure):

@@ -27,6 +27,7 @@
 #include "../../include/fem_solver.h"
 #include "../../include/fem_testing.h"
 #include "common.hpp"
+#include "hex_elastic.hpp"
 
 #pragma clang fp contract(off)
 
@@ -39,32 +40,6 @@ constexpr uint32_t NONE32 = 0xffffffffu;
 // Points per cell and unknowns per point of a mesh kind.
 int points_per_cell(int kind) { return kind == FEM_MESH_HEAT_HEX8 ? 8 : kind == FEM_MESH_HEAT_TET4 ? 4 : 10; }
 int comps_of(int kind) { return kind == FEM_MESH_ELASTIC_TET10 ? 3 : 1; }
-
-// The stiffness of an isotropic material by the inverse of its compliance (hex.rs:58-115), with
-// the reference's texts for a material outside its range: normal[3][3], then G_yz, G_xz, G_xy.
-std::array<double, 12> isotropic_stiffness(double e, double nu) {
-    const double g = e / (2.0 * (1.0 + nu));
-    const bool moduli = std::isfinite(e) && e > 0.0 && std::isfinite(g) && g > 0.0;
-    if (!moduli || !std::isfinite(nu)) throw Error("the moduli must be above 0 and the Poisson ratios finite");
-    const double s[3][3] = {{1.0 / e, -nu / e, -nu / e}, {-nu / e, 1.0 / e, -nu / e}, {-nu / e, -nu / e, 1.0 / e}};
-    const double det = s[0][0] * (s[1][1] * s[2][2] - s[1][2] * s[2][1]) -
-                       s[0][1] * (s[1][0] * s[2][2] - s[1][2] * s[2][0]) +
-                       s[0][2] * (s[1][0] * s[2][1] - s[1][1] * s[2][0]);
-    const double minor = s[0][0] * s[1][1] - s[0][1] * s[1][0];
-    if (!(minor > 0.0 && det > 1e-12 * s[0][0] * s[1][1] * s[2][2]))
-        throw Error(
-            "the Poisson ratios give no stable material (for an isotropic material, nu must be between -1 and 0.5)");
-    std::array<double, 12> d{};
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
-            d[3 * i + j] = (s[j1][i1] * s[j2][i2] - s[j1][i2] * s[j2][i1]) / det;
-        }
-    d[9] = g;  // G_yz
-    d[10] = g; // G_xz
-    d[11] = g; // G_xy
-    return d;
-}
 
 // ------------------------------------------------------------------ device: element rows
 // Each thread computes row r of the matrix of one element into ke[e m^2 + r m ...] and the row's

@@ -18,6 +18,7 @@
 #include <cmath>
 
 #include "common.hpp"
+#include "hex_elastic.hpp"
 
 namespace fem {
 
@@ -127,6 +128,57 @@ bool on_face(const uint32_t q[3], const int top[3], int f) {
     return (f % 2 == 0) ? q[d] == 0 : (int)q[d] == top[d];
 }
 
+// The loads of the box-grid elasticity caller on hex cells (elastic.rs:366-393 and 942-963):
+// each cell puts body x V / 8 on each of its corners, in cell order; then each face with a
+// traction t puts t A / 4 on each of the 4 points of each of its cell faces (A the cell face's
+// area), the faces in the order x-, x+, y-, y+, z-, z+ and each face's cells in cell order.
+void hex_loads(BoxModel& m) {
+    const auto& in = m.in;
+    const int nx = in.n[0], ny = in.n[1], nz = in.n[2];
+    std::array<std::vector<double>, 3> pl;
+    for (int d = 0; d < 3; ++d) {
+        pl[d].resize((size_t)in.n[d] + 1);
+        for (size_t i = 0; i < pl[d].size(); ++i) pl[d][i] = m.h[d] * (double)i;
+    }
+    auto gidx = [&](int i, int j, int k) { return (size_t)i + (size_t)m.gn[0] * ((size_t)j + (size_t)m.gn[1] * k); };
+    const bool body = in.body[0] != 0.0 || in.body[1] != 0.0 || in.body[2] != 0.0;
+    if (body)
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    const double size[3] = {pl[0][i + 1] - pl[0][i], pl[1][j + 1] - pl[1][j], pl[2][k + 1] - pl[2][k]};
+                    const double volume = size[0] * size[1] * size[2];
+                    for (const auto& b : VTK_BITS)
+                        for (int d = 0; d < 3; ++d)
+                            m.load[3 * gidx(i + b[0], j + b[1], k + b[2]) + d] += in.body[d] * volume / 8.0;
+                }
+    for (int f = 0; f < 6; ++f) {
+        const double* t = in.traction[f];
+        if (t[0] == 0.0 && t[1] == 0.0 && t[2] == 0.0) continue;
+        const int ax = f / 2, a = ax == 0 ? 1 : 0, b = ax == 2 ? 1 : 2;
+        const int plane = (f % 2 == 0) ? 0 : in.n[ax];
+        const int cell_plane = (f % 2 == 0) ? 0 : in.n[ax] - 1;
+        int lo[3] = {0, 0, 0}, hi[3] = {nx, ny, nz};
+        lo[ax] = cell_plane;
+        hi[ax] = cell_plane + 1;
+        for (int k = lo[2]; k < hi[2]; ++k)
+            for (int j = lo[1]; j < hi[1]; ++j)
+                for (int i = lo[0]; i < hi[0]; ++i) {
+                    const int c[3] = {i, j, k};
+                    const double area = (pl[a][c[a] + 1] - pl[a][c[a]]) * (pl[b][c[b] + 1] - pl[b][c[b]]);
+                    // the 4 points of the cell face: (0,0), (1,0), (1,1), (0,1) along its two axes
+                    const int PTS[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+                    for (const auto& q : PTS) {
+                        int p[3] = {i, j, k};
+                        p[ax] = plane;
+                        p[a] += q[0];
+                        p[b] += q[1];
+                        for (int d = 0; d < 3; ++d) m.load[3 * gidx(p[0], p[1], p[2]) + d] += t[d] * area / 4.0;
+                    }
+                }
+    }
+}
+
 }  // namespace
 
 BoxModel box_shape(const BoxInput& in) {
@@ -137,8 +189,8 @@ BoxModel box_shape(const BoxInput& in) {
     BoxModel m;
     m.in = in;
     const size_t cells = (size_t)in.n[0] * in.n[1] * in.n[2];
-    const bool tets = in.kind != Kind::HeatHex8;
-    m.comps = (in.kind == Kind::ElasticTet4 || in.kind == Kind::ElasticTet10) ? 3 : 1;
+    const bool tets = !hex_kind(in.kind);
+    m.comps = (in.kind == Kind::ElasticTet4 || in.kind == Kind::ElasticTet10 || in.kind == Kind::ElasticHex8) ? 3 : 1;
     m.quadratic = in.kind == Kind::ElasticTet10;
     const int refine = m.quadratic ? 2 : 1;
     for (int d = 0; d < 3; ++d) {
@@ -148,8 +200,11 @@ BoxModel box_shape(const BoxInput& in) {
     for (int f = 0; f < 6; ++f)
         if (in.fixed[f]) m.fix |= 1u << f;
     const size_t need = tets ? 6 * cells : cells;
-    if (in.coef0.size() != need) throw Error(tets ? "one material value for each tetrahedron" : "one conductivity for each cell");
-    if (m.comps == 3 && in.coef1.size() != need) throw Error("one Poisson ratio for each tetrahedron");
+    if (in.coef0.size() != need)
+        throw Error(tets ? "one material value for each tetrahedron"
+                         : m.comps == 3 ? "one Young's modulus for each cell" : "one conductivity for each cell");
+    if (m.comps == 3 && in.coef1.size() != need)
+        throw Error(tets ? "one Poisson ratio for each tetrahedron" : "one Poisson ratio for each cell");
     // a node is free when it lies on no fixed face, that is, off the fixed faces along every axis
     size_t free_nodes = 1;
     for (int d = 0; d < 3; ++d) free_nodes *= (size_t)(m.gn[d] - (in.fixed[2 * d] ? 1 : 0) - (in.fixed[2 * d + 1] ? 1 : 0));
@@ -161,7 +216,7 @@ BoxModel build_model(const BoxInput& in) {
     BoxModel m = box_shape(in);
     const int nx = in.n[0], ny = in.n[1], nz = in.n[2];
     const size_t cells = (size_t)nx * ny * nz;
-    const bool tets = in.kind != Kind::HeatHex8;
+    const bool tets = !hex_kind(in.kind);
 
     const size_t gnodes = m.grid_nodes();
     auto gidx = [&](uint32_t i, uint32_t j, uint32_t k) { return i + (size_t)m.gn[0] * (j + (size_t)m.gn[1] * k); };
@@ -247,7 +302,11 @@ BoxModel build_model(const BoxInput& in) {
     if (m.points != gnodes) throw Error("the numbering does not cover the grid");
 
     // ---------------------------------------------------------------- materials
-    if (!tets) {
+    if (in.kind == Kind::ElasticHex8) {
+        m.cell_k = in.coef0;
+        m.cell_nu = in.coef1;
+        for (size_t e = 0; e < cells; ++e) (void)isotropic_stiffness(m.cell_k[e], m.cell_nu[e]);  // its texts
+    } else if (!tets) {
         m.cell_k = in.coef0;
         for (double k : m.cell_k)
             if (!(k > 0.0 && std::isfinite(k))) throw Error("the conductivity must be a finite number above 0");
@@ -325,7 +384,8 @@ BoxModel build_model(const BoxInput& in) {
                         }
                     }
     }
-    if (m.comps == 3) {
+    if (in.kind == Kind::ElasticHex8) hex_loads(m);
+    if (m.comps == 3 && tets) {
         const bool body = in.body[0] != 0.0 || in.body[1] != 0.0 || in.body[2] != 0.0;
         const double v = hx * hy * hz / 6.0;
         const uint32_t s = m.quadratic ? 2 : 1;

@@ -10,7 +10,10 @@
 
 namespace fem {
 
-enum class Kind { HeatHex8 = 0, HeatTet4 = 1, ElasticTet4 = 2, ElasticTet10 = 3 };
+enum class Kind { HeatHex8 = 0, HeatTet4 = 1, ElasticTet4 = 2, ElasticTet10 = 3, ElasticHex8 = 4 };
+
+// Kinds on the box cells themselves (hex8) rather than their 6 Kuhn tetrahedra.
+inline bool hex_kind(Kind k) { return k == Kind::HeatHex8 || k == Kind::ElasticHex8; }
 
 struct BoxInput {
     Kind kind = Kind::HeatHex8;
@@ -37,7 +40,8 @@ struct BoxModel {
     std::vector<int64_t> free;         // free unknowns, increasing (reference numbering)
     std::vector<int64_t> free_grid;    // free unknown k -> level-0 grid unknown (comps * node + c)
     // materials in grid layout, FP64
-    std::vector<double> cell_k;        // hex8: k per cell (x fastest)
+    std::vector<double> cell_k;        // hex8: k (heat) or Young's modulus (elasticity) per cell (x fastest)
+    std::vector<double> cell_nu;       // hex8 elasticity: Poisson ratio per cell
     std::vector<double> tet_coef;      // Kuhn tets: [(q * 6 + type) * cells + cell], q = 0 (k or lambda), 1 (mu)
     std::vector<uint32_t> tet_cell;    // tetrahedron (reference order) -> cell (x fastest)
     std::vector<uint8_t> tet_type;     // tetrahedron (reference order) -> Kuhn type (chain axis order)

@@ -21,8 +21,10 @@
 #include <vector>
 
 #include "../../include/fem_solver.h"
+#include "../../include/fem_testing.h"
 #include "common.hpp"
 #include "dense.hpp"
+#include "hex_elastic.hpp"
 #include "kernels.hpp"
 #include "model.hpp"
 #include "march.hpp"
@@ -583,6 +585,8 @@ class Batcher;
 class Solver {
    public:
     Solver(const BoxInput& in, const Options& o) : opt_(o), uid_(next_uid()) {
+        if (in.kind == Kind::ElasticHex8)
+            throw Error("hex8 elasticity is solved through its assembled matrix (fem_solver_new_box), not by the matrix-free solver");
         const auto t0 = std::chrono::steady_clock::now();
         FEM_HIP(hipSetDevice(o.device));
         hipDeviceProp_t prop;
@@ -1404,6 +1408,8 @@ class Solver {
                 L0.kuhn = true;
                 t10_setup(L0, s_);
                 break;
+            case Kind::ElasticHex8:
+                throw Error("internal: hex8 elasticity has no matrix-free level");
         }
         if (m.in.kind != Kind::HeatHex8) {
             L0.coef.resize(m.tet_coef.size());
@@ -2726,6 +2732,11 @@ class Server {
 struct fem_solver {
     std::unique_ptr<fem::Solver> impl;
     std::unique_ptr<fem::CsrSolver> csr;
+    // A box model solved through its assembled matrix (hex8 elasticity): the model, its right
+    // side and its reduced matrix, beside `csr`.
+    std::unique_ptr<fem::BoxModel> box;
+    std::vector<double> box_rhs;
+    fem::csr::Csr box_matrix;
 };
 
 namespace {
@@ -2813,7 +2824,22 @@ int fem_solver_new_box(const fem_box* box, const fem_options* opt, fem_solver** 
         const fem::Options o = options_from(opt ? *opt : def);
         auto* s = new fem_solver;
         try {
-            s->impl = std::make_unique<fem::Solver>(in, o);
+            if (in.kind == fem::Kind::ElasticHex8) {
+                // elastic.rs:287-547: the assembled matrix, the grid level and the cells to Solver::new
+                auto m = std::make_unique<fem::BoxModel>(fem::build_model(in));
+                fem::HexBoxSystem sys = fem::hex_box_system(*m);
+                fem::CsrUnknowns un;
+                un.kind = FEM_UNKNOWNS_GRID;
+                un.grid = std::move(sys.level);
+                un.cells = true;
+                un.cell_data = std::move(sys.cells);
+                s->box_matrix = sys.a;
+                s->csr = std::make_unique<fem::CsrSolver>(std::move(sys.a), std::move(un), o);
+                s->box_rhs = std::move(sys.rhs);
+                s->box = std::move(m);
+            } else {
+                s->impl = std::make_unique<fem::Solver>(in, o);
+            }
         } catch (...) {
             delete s;
             throw;
@@ -2931,11 +2957,16 @@ uint64_t fem_solver_free_count(const fem_solver* s) {
 
 uint64_t fem_solver_unknown_count(const fem_solver* s) {
     if (!s) return 0;
+    if (s->box) return s->box->unknowns();
     return s->csr ? s->csr->n() : s->impl->model().unknowns();
 }
 
 int fem_solver_free_list(const fem_solver* s, int64_t* out) {
     if (!s || !out) return 1;
+    if (s->box) {
+        std::memcpy(out, s->box->free.data(), s->box->free.size() * sizeof(int64_t));
+        return 0;
+    }
     if (s->csr) {  // the matrix's unknowns are all free, in its order
         for (uint64_t i = 0; i < s->csr->n(); ++i) out[i] = (int64_t)i;
         return 0;
@@ -2946,8 +2977,8 @@ int fem_solver_free_list(const fem_solver* s, int64_t* out) {
 }
 
 int fem_solver_point_grid(const fem_solver* s, uint32_t* out) {
-    if (!s || !out || s->csr) return 1;
-    const auto& m = s->impl->model();
+    if (!s || !out || (s->csr && !s->box)) return 1;
+    const auto& m = s->box ? *s->box : s->impl->model();
     for (size_t p = 0; p < m.points; ++p) {
         const uint32_t g = m.point_grid[p];
         out[3 * p] = g % m.gn[0];
@@ -2960,6 +2991,10 @@ int fem_solver_point_grid(const fem_solver* s, uint32_t* out) {
 int fem_solver_rhs(fem_solver* s, double* out, char* err, size_t errlen) {
     return guarded(err, errlen, [&] {
         if (!s) throw fem::Error("no solver");
+        if (s->box) {
+            std::copy(s->box_rhs.begin(), s->box_rhs.end(), out);
+            return;
+        }
         if (s->csr) throw fem::Error("an assembled solver has no model: its right side is the caller's");
         s->impl->rhs(out);
     });
@@ -2985,6 +3020,14 @@ int fem_solver_matrix(fem_solver* s, uint64_t* nnz, uint64_t* ptr, uint32_t* idx
                       size_t errlen) {
     return guarded(err, errlen, [&] {
         if (!s) throw fem::Error("no solver");
+        if (s->box) {  // the assembled matrix of the box
+            const auto& a = s->box_matrix;
+            if (ptr) std::copy(a.ptr.begin(), a.ptr.end(), ptr);
+            if (idx) std::copy(a.idx.begin(), a.idx.end(), idx);
+            if (val) std::copy(a.val.begin(), a.val.end(), val);
+            if (nnz) *nnz = a.nnz();
+            return;
+        }
         if (s->csr) throw fem::Error("an assembled solver holds the caller's matrix already");
         const uint64_t n = s->impl->export_matrix(ptr, idx, val);
         if (nnz) *nnz = n;
@@ -3097,7 +3140,7 @@ namespace {
 fem::BoxInput box_input(const fem_box* box) {
     if (!box) throw fem::Error("a box is needed");
     fem::BoxInput in;
-    if (box->kind < 0 || box->kind > 3) throw fem::Error("unknown model kind");
+    if (box->kind < 0 || box->kind > 4) throw fem::Error("unknown model kind");
     in.kind = (fem::Kind)box->kind;
     for (int d = 0; d < 3; ++d) {
         in.n[d] = box->n[d];
@@ -3105,10 +3148,10 @@ fem::BoxInput box_input(const fem_box* box) {
     }
     if (box->n[0] < 1 || box->n[1] < 1 || box->n[2] < 1) throw fem::Error("each axis needs 1 cell or more");
     const size_t cells = (size_t)box->n[0] * box->n[1] * box->n[2];
-    const size_t count = in.kind == fem::Kind::HeatHex8 ? cells : 6 * cells;
+    const size_t count = fem::hex_kind(in.kind) ? cells : 6 * cells;
     if (!box->coef0) throw fem::Error("the material values are missing");
     in.coef0.assign(box->coef0, box->coef0 + count);
-    if (in.kind == fem::Kind::ElasticTet4 || in.kind == fem::Kind::ElasticTet10) {
+    if (in.kind == fem::Kind::ElasticTet4 || in.kind == fem::Kind::ElasticTet10 || in.kind == fem::Kind::ElasticHex8) {
         if (!box->coef1) throw fem::Error("the Poisson ratios are missing");
         in.coef1.assign(box->coef1, box->coef1 + count);
     }
@@ -3124,6 +3167,30 @@ fem::BoxInput box_input(const fem_box* box) {
 }  // namespace
 
 extern "C" {
+
+int fem_test_hex_element(const double* size, double e, double nu, double* out, char* err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!size || !out) throw fem::Error("the sizes and the output are needed");
+        const auto k = fem::hex_element({size[0], size[1], size[2]}, fem::isotropic_stiffness(e, nu));
+        std::copy(k.begin(), k.end(), out);
+    });
+}
+
+int fem_test_hex_box_system(const fem_box* box, uint64_t* n, uint64_t* nnz, uint64_t* ptr, uint32_t* idx, double* val,
+                            double* rhs, double* aspect, char* err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        const fem::BoxInput in = box_input(box);
+        if (in.kind != fem::Kind::ElasticHex8) throw fem::Error("a hex8 elasticity box is needed");
+        const fem::HexBoxSystem sys = fem::hex_box_system(fem::build_model(in));
+        if (n) *n = sys.a.n;
+        if (nnz) *nnz = sys.a.nnz();
+        if (ptr) std::copy(sys.a.ptr.begin(), sys.a.ptr.end(), ptr);
+        if (idx) std::copy(sys.a.idx.begin(), sys.a.idx.end(), idx);
+        if (val) std::copy(sys.a.val.begin(), sys.a.val.end(), val);
+        if (rhs) std::copy(sys.rhs.begin(), sys.rhs.end(), rhs);
+        if (aspect) *aspect = sys.aspect;
+    });
+}
 
 int fem_model_new_box(const fem_box* box, fem_model** out, char* err, size_t errlen) {
     return guarded(err, errlen, [&] {

@@ -33,7 +33,8 @@ enum fem_kind {
     FEM_HEAT_HEX8 = 0,     /* heat.rs:1246 conduction() on box cells */
     FEM_HEAT_TET4 = 1,     /* heat.rs:972 tet_conduction() */
     FEM_ELASTIC_TET4 = 2,  /* constant-strain tetrahedron (the p-multigrid coarse operator, pmultigrid.rs:7) */
-    FEM_ELASTIC_TET10 = 3  /* tet10.rs:90 Tet10::stiffness() */
+    FEM_ELASTIC_TET10 = 3, /* tet10.rs:90 Tet10::stiffness() */
+    FEM_ELASTIC_HEX8 = 4   /* hex.rs:199-279 Hex::new: the box-grid elasticity caller (elastic.rs:287-547), assembled */
 };
 
 /* solve.rs:47-50 Method, plus "let the crossover choose". */
@@ -59,13 +60,15 @@ typedef struct fem_box {
     int32_t n[3];         /* cells along x, y, z */
     double length[3];     /* box size */
     const double* coef0;  /* heat: conductivity of each cell (hex8) or tetrahedron (tet4);
-                             elasticity: Young's modulus of each tetrahedron. Reference order. */
-    const double* coef1;  /* elasticity: Poisson ratio of each tetrahedron; heat: unused */
+                             elasticity: Young's modulus of each tetrahedron (tet4, tet10) or cell
+                             (hex8). Reference order. */
+    const double* coef1;  /* elasticity: Poisson ratio of each tetrahedron or cell; heat: unused */
     int32_t fixed[6];     /* heat: a face at a fixed temperature; elasticity: a clamped face */
     double value[6];      /* heat: the temperature of a fixed face */
     double source;        /* heat: heat source per volume */
     double body[3];       /* elasticity: force per volume */
-    double traction[18];  /* elasticity: force per area on each face, 3 per face */
+    double traction[18];  /* elasticity: force per area on each face, 3 per face (hex8: a quarter of
+                             each cell face's force on each of its 4 points, elastic.rs:942-963) */
 } fem_box;
 
 /* Options of a solver. `fem_options_default` fills the reference's values where it has them. */
@@ -128,7 +131,15 @@ void fem_options_default(fem_options* opt);
  * the reference's does (solve.rs:106-118, 158-170): Crossover::Unknowns up to `crossover`
  * unknowns, or Crossover::Flops on the counts of the nested-dissection factor of the model's
  * reduced matrix pattern. Unlike the reference, multigrid also runs on tetrahedra (solve.rs:132-137
- * refuses it there). Material errors use the texts of hex.rs:79 and hex.rs:93. */
+ * refuses it there). Material errors use the texts of hex.rs:79 and hex.rs:93.
+ * FEM_ELASTIC_HEX8 is solved as the reference's box-grid elasticity caller solves it
+ * (elastic.rs:287-547): the condensed incompatible-mode hex of each cell (hex.rs:199-279) is
+ * assembled on the host into the reduced matrix of the free unknowns, which goes with the grid
+ * level of its unknowns and the cells (for the coarse matrices of multigrid.rs:458-671) to the
+ * solver of fem_solver_new_csr; the caller chooses the crossover (the reference's caller passes
+ * Crossover::Flops with `flat` when a cell's sides differ by more than 2x, elastic.rs:525-547).
+ * fem_solver_rhs, _free_list, _point_grid and _matrix give its right side, free unknowns, points
+ * and reduced matrix. The serving loop takes the matrix-free kinds only. */
 int fem_solver_new_box(const fem_box* box, const fem_options* opt, fem_solver** out, char* err, size_t errlen);
 
 /* Replaces Solver::solve (solve.rs:199-250) with host vectors: `b` and `x` hold the free

@@ -66,6 +66,17 @@ int fem_test_symbolic(const fem_csr* a, const double* xyz, uint64_t* order_out, 
 int fem_test_assembly_pattern(const fem_mesh* mesh, uint64_t* free_count, uint64_t* entries, uint64_t* ptr,
                               uint32_t* idx, int64_t* free_list, char* err, size_t errlen);
 
+/* The condensed incompatible-mode hex of a box cell with sides size[3] and an isotropic material
+ * (hex.rs:199-279 Hex::new's k: 24 x 24 row-major, unknown 3 corner + component, corners in VTK
+ * order), by the library's host restatement that FEM_ELASTIC_HEX8 boxes assemble. No device. */
+int fem_test_hex_element(const double* size, double e, double nu, double* out, char* err, size_t errlen);
+
+/* The reduced system that fem_solver_new_box assembles for a FEM_ELASTIC_HEX8 box, on the host:
+ * its counts first (call with NULL arrays to size them), then the reduced matrix (ptr: n + 1,
+ * idx and val: nnz), the right side (n) and the largest aspect ratio of a cell. No device. */
+int fem_test_hex_box_system(const fem_box* box, uint64_t* n, uint64_t* nnz, uint64_t* ptr, uint32_t* idx, double* val,
+                            double* rhs, double* aspect, char* err, size_t errlen);
+
 #ifdef __cplusplus
 }
 #endif

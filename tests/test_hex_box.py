@@ -1,0 +1,91 @@
+"""Box-grid elasticity on hex cells, host side (no device): the library's condensed
+incompatible-mode hex (hex.rs:199-279) and the reduced system that fem_solver_new_box assembles
+for an elastic_hex8 box (elastic.rs:287-547: Csr::pattern + add_element in cell order,
+submatrix(free), the body force V/8 per corner, a quarter of each cell face's traction force on
+its 4 points, elastic.rs:942-963).
+
+Tolerance: bit for bit. The element against the oracle's restatement (itself pinned by the
+reference's hex tests: rigid modes, uniform strain, one-cell bending, in test_by_cells.py); the
+matrix against the oracle's assembly of the same box; the right side against the oracle's body
+load with the tractions added in the reference's order (faces in load order, each face's cells in
+cell order, a cell face's points in the order of grid.rs:272-289).
+"""
+import numpy as np
+import pytest
+
+FACE_AXIS = {"x": 0, "y": 1, "z": 2}
+
+
+@pytest.mark.parametrize("size", [(1.0, 1.0, 1.0), (2.0, 0.5, 0.1), (0.3, 0.7, 1.9)])
+@pytest.mark.parametrize("e,nu", [(210e3, 0.3), (1.0, 0.0), (5.0, 0.45), (1.0, -0.5)])
+def test_hex_element_is_the_oracles_bit_for_bit(fem, oracle, size, e, nu):
+    k = fem.hex_element(size, e, nu)
+    assert np.array_equal(k, oracle.hex_stiffness(size, e, nu).reshape(24, 24))
+    # symmetric to rounding: the condensation K_uu - K_um K_mm^-1 K_mu is not symmetrized
+    assert np.abs(k - k.T).max() <= 1e-12 * np.abs(k).max()
+
+
+def traction_load(pr, n, length, traction):
+    """The oracle's body load plus the face tractions in the reference's order."""
+    load = pr.load().copy()
+    node = {tuple(p): i for i, p in enumerate(pr.ijk())}
+    planes = [np.arange(n[d] + 1, dtype=np.float64) * (length[d] / n[d]) for d in range(3)]
+    for face, t in traction:
+        ax = FACE_AXIS[face[0]]
+        a, b = [d for d in range(3) if d != ax]
+        top = face[1] == "1"
+        for k in range(n[2]):
+            for j in range(n[1]):
+                for i in range(n[0]):
+                    c = [i, j, k]
+                    if c[ax] != (n[ax] - 1 if top else 0):
+                        continue
+                    area = (planes[a][c[a] + 1] - planes[a][c[a]]) * (planes[b][c[b] + 1] - planes[b][c[b]])
+                    for qa, qb in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                        p = list(c)
+                        p[ax] = n[ax] if top else 0
+                        p[a] += qa
+                        p[b] += qb
+                        for d in range(3):
+                            load[3 * node[tuple(p)] + d] += t[d] * area / 4.0
+    return load
+
+
+@pytest.mark.parametrize("n,length,clamped,traction", [
+    ([5, 3, 4], [2.0, 0.7, 1.1], ["x0"], [("x1", (0.5, -0.25, 1.5)), ("z1", (0.0, 0.0, -2.0))]),
+    ([4, 4, 2], [1.0, 1.0, 0.3], ["z0", "y1"], [("x0", (1.0, 0.0, 0.0)), ("x1", (-1.0, 0.5, 0.0))]),
+    ([1, 1, 1], [1.0, 2.0, 3.0], ["x0"], []),
+    ([7, 1, 1], [7.0, 1.0, 1.0], ["x0", "x1"], [("y1", (0.0, 1.0, 0.0))]),
+])
+def test_hex_box_system_is_the_oracles_bit_for_bit(fem, oracle, n, length, clamped, traction):
+    rng = np.random.default_rng(3)
+    nc = int(np.prod(n))
+    e = rng.uniform(1.0, 3.0, nc)
+    nu = rng.uniform(0.1, 0.4, nc)
+    body = (0.3, -1.0, 2.0)
+    box = fem.Box("elastic_hex8", n, length, e, nu, fixed={f: True for f in clamped}, body=body,
+                  traction=dict(traction))
+    (ptr, idx, val), rhs, aspect = fem.hex_box_system(box)
+    pr = oracle.Problem.elastic_hex(n, length, e, nu, clamped=clamped, body=body)
+    optr, oidx, oval = pr.reduced().arrays()
+    assert np.array_equal(ptr, optr) and np.array_equal(idx, oidx) and np.array_equal(val, oval)
+    free = pr.free()
+    assert np.array_equal(rhs, traction_load(pr, n, length, traction)[free])
+    model = fem.Model(box)
+    assert np.array_equal(model.free_list(), free)
+    assert np.array_equal(model.point_grid(), pr.ijk())
+    assert aspect == pytest.approx(fem.box_aspect(box), rel=0, abs=0)
+
+
+def test_hex_box_materials_are_checked_with_the_references_texts(fem):
+    with pytest.raises(fem.FemError, match="^the Poisson ratios give no stable material"):
+        fem.hex_box_system(fem.Box("elastic_hex8", [2, 1, 1], [2.0, 1.0, 1.0], 1.0, 0.5, fixed={"x0": True}))
+    with pytest.raises(fem.FemError, match="^the moduli must be above 0 and the Poisson ratios finite$"):
+        fem.hex_box_system(fem.Box("elastic_hex8", [2, 1, 1], [2.0, 1.0, 1.0], 0.0, 0.3, fixed={"x0": True}))
+    with pytest.raises(fem.FemError, match="^the Poisson ratios are missing$"):
+        fem.hex_box_system(fem.Box("elastic_hex8", [2, 1, 1], [2.0, 1.0, 1.0], 1.0, None, fixed={"x0": True}))
+
+
+def test_hex_box_aspect_is_the_references_flatness(fem):
+    assert fem.box_aspect(fem.Box("elastic_hex8", [4, 4, 1], [4.0, 4.0, 0.25], 1.0, 0.3)) == 4.0
+    assert fem.box_aspect(fem.Box("elastic_hex8", [3, 3, 3], [1.0, 1.0, 1.0], 1.0, 0.3)) < 1.0 + 1e-12
