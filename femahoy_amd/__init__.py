@@ -814,7 +814,7 @@ def dense_solve(a, b, entry: str = "fem_dense_solve_device") -> np.ndarray:
     return dx.get()
 
 
-MESH_KINDS = {"heat_hex8": 0, "heat_tet4": 1, "elastic_tet10": 2}
+MESH_KINDS = {"heat_hex8": 0, "heat_tet4": 1, "elastic_tet10": 2, "elastic_hex8": 3}
 
 
 @dataclass
@@ -858,7 +858,7 @@ class Mesh:
         m.source = float(self.source)
         for d in range(3):
             m.body[d] = float(self.body[d])
-        units = m.points * (3 if self.kind == "elastic_tet10" else 1)
+        units = m.points * (3 if self.kind.startswith("elastic") else 1)
         if keep["fixed"].size != units or keep["value"].size != units:
             raise FemError(f"fixed and value need one entry per unknown ({units})")
         if "load" in keep and keep["load"].size != units:

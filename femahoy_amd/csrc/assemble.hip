@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <memory>
 #include <string>
 #include <thread>
@@ -38,8 +39,10 @@ constexpr int ABLOCK = 256;
 constexpr uint32_t NONE32 = 0xffffffffu;
 
 // Points per cell and unknowns per point of a mesh kind.
-int points_per_cell(int kind) { return kind == FEM_MESH_HEAT_HEX8 ? 8 : kind == FEM_MESH_HEAT_TET4 ? 4 : 10; }
-int comps_of(int kind) { return kind == FEM_MESH_ELASTIC_TET10 ? 3 : 1; }
+int points_per_cell(int kind) {
+    return kind == FEM_MESH_HEAT_HEX8 || kind == FEM_MESH_ELASTIC_HEX8 ? 8 : kind == FEM_MESH_HEAT_TET4 ? 4 : 10;
+}
+int comps_of(int kind) { return kind == FEM_MESH_ELASTIC_TET10 || kind == FEM_MESH_ELASTIC_HEX8 ? 3 : 1; }
 
 // ------------------------------------------------------------------ device: element rows
 // Each thread computes row r of the matrix of one element into ke[e m^2 + r m ...] and the row's
@@ -178,6 +181,25 @@ __global__ void __launch_bounds__(ABLOCK) k_row_tet10(size_t cells, const double
     fe[e * 30 + row] = body[a] * wl;
 }
 
+// hex8 elasticity (hex.rs:199-279, elastic.rs:366-393): row r of cell e is row r of its element,
+// one of the distinct elements the host computed (table[which[e]]), and its load the body force
+// times V/8 (V from corners 0 and 6), as the box-grid caller adds it.
+__global__ void __launch_bounds__(ABLOCK) k_row_hex8e(size_t cells, const double* xyz, const uint32_t* conn,
+                                                      const uint32_t* which, const double* table, const double* body,
+                                                      double* ke, double* fe) {
+    const size_t t = (size_t)blockIdx.x * ABLOCK + threadIdx.x;
+    if (t >= cells * 24) return;
+    const size_t e = t / 24;
+    const int row = (int)(t % 24);
+    const double* src = table + (size_t)which[e] * 576 + (size_t)row * 24;
+    double* dst = ke + e * 576 + (size_t)row * 24;
+    for (int c = 0; c < 24; ++c) dst[c] = src[c];
+    const uint32_t c0 = conn[8 * e], c6 = conn[8 * e + 6];
+    double s[3];
+    for (int d = 0; d < 3; ++d) s[d] = xyz[3 * (size_t)c6 + d] - xyz[3 * (size_t)c0 + d];
+    fe[e * 24 + row] = body[row % 3] * (s[0] * s[1] * s[2]) / 8.0;
+}
+
 // s = 0; s += src[slot] over the slots of item r in map order (the additions of the element loop).
 __global__ void __launch_bounds__(ABLOCK) k_gather(size_t n, const uint64_t* ptr, const uint32_t* slot,
                                                    const double* src, const double* extra, double* out) {
@@ -227,14 +249,17 @@ struct HostPlan {
     std::vector<uint32_t> cslot;
     std::vector<uint64_t> lptr;               // per unknown: its element-load slots
     std::vector<uint32_t> lslot;
-    std::vector<double> dmat;                 // elasticity: the 12 stiffness values of each cell
+    std::vector<double> dmat;                 // tet10 elasticity: the 12 stiffness values of each cell
+    std::vector<uint32_t> which;              // hex8 elasticity: each cell's element in `table`
+    std::vector<double> table;                // hex8 elasticity: the distinct elements, 576 each
 };
 
 void validate(const fem_mesh& m) {
-    if (m.kind < FEM_MESH_HEAT_HEX8 || m.kind > FEM_MESH_ELASTIC_TET10) throw Error("unknown mesh kind");
+    if (m.kind < FEM_MESH_HEAT_HEX8 || m.kind > FEM_MESH_ELASTIC_HEX8) throw Error("unknown mesh kind");
     if (m.points == 0 || m.cells == 0) throw Error("the mesh needs points and cells");
     if (!m.xyz || !m.conn || !m.coef0 || !m.fixed || !m.value) throw Error("the mesh needs xyz, conn, coef0, fixed and value");
-    if (m.kind == FEM_MESH_ELASTIC_TET10 && !m.coef1) throw Error("elasticity needs Poisson's ratio per cell (coef1)");
+    if ((m.kind == FEM_MESH_ELASTIC_TET10 || m.kind == FEM_MESH_ELASTIC_HEX8) && !m.coef1)
+        throw Error("elasticity needs Poisson's ratio per cell (coef1)");
     const int npc = points_per_cell(m.kind);
     const uint64_t m2 = (uint64_t)(npc * comps_of(m.kind)) * (uint64_t)(npc * comps_of(m.kind));
     if (m.cells * m2 >= (uint64_t)NONE32) throw Error("the mesh has too many cells for 32-bit element slots");
@@ -360,6 +385,33 @@ void plan(const fem_mesh& mesh, HostPlan& P, bool maps) {
     group_by_target(target, P.idx.size(), P.cptr, P.cslot);
     // the unknown that each element-load slot adds into (the callers' load loops, element order)
     group_by_target(P.eu, n, P.lptr, P.lslot);
+    if (P.kind == FEM_MESH_ELASTIC_HEX8) {
+        // the element of each distinct (sizes, material): sizes from corners 0 and 6 (grid.rs:195-198)
+        std::map<std::array<uint64_t, 5>, uint32_t> seen;
+        P.which.resize(P.cells);
+        P.table.clear();
+        auto bits = [](double v) {
+            uint64_t b;
+            std::memcpy(&b, &v, sizeof b);
+            return b;
+        };
+        for (size_t e = 0; e < P.cells; ++e) {
+            const uint32_t c0 = mesh.conn[8 * e], c6 = mesh.conn[8 * e + 6];
+            std::array<double, 3> size;
+            for (int d = 0; d < 3; ++d) size[d] = mesh.xyz[3 * (size_t)c6 + d] - mesh.xyz[3 * (size_t)c0 + d];
+            const std::array<uint64_t, 5> key = {bits(size[0]), bits(size[1]), bits(size[2]), bits(mesh.coef0[e]),
+                                                 bits(mesh.coef1[e])};
+            auto it = seen.find(key);
+            if (it == seen.end()) {
+                if (!(size[0] > 0.0 && size[1] > 0.0 && size[2] > 0.0))
+                    throw Error("a hex cell needs corner 6 above corner 0 along every axis");
+                it = seen.emplace(key, (uint32_t)(P.table.size() / 576)).first;
+                const auto k = hex_element(size, isotropic_stiffness(mesh.coef0[e], mesh.coef1[e]));
+                P.table.insert(P.table.end(), k.begin(), k.end());
+            }
+            P.which[e] = it->second;
+        }
+    }
     if (P.kind == FEM_MESH_ELASTIC_TET10) {
         P.dmat.resize(P.cells * 12);
         for (size_t e = 0; e < P.cells; ++e) {
@@ -441,7 +493,7 @@ class Assembly {
         // the mesh and the maps to the device (the uploads are outside the device time)
         DArray<double> xyz = up(mesh.xyz, mesh.points * 3, s);
         DArray<uint32_t> conn = up(mesh.conn, nc * npc, s);
-        DArray<double> coef = P.kind == FEM_MESH_ELASTIC_TET10 ? up(P.dmat, s) : up(mesh.coef0, nc, s);
+        DArray<double> coef = P.kind == FEM_MESH_ELASTIC_TET10 ? up(P.dmat, s) : up(mesh.coef0, nc, s);  // unused by hex8 elasticity
         DArray<double> body = up(mesh.body, 3, s);
         DArray<uint8_t> fixed = up(mesh.fixed, n, s);
         DArray<double> value = up(mesh.value, n, s);
@@ -455,6 +507,12 @@ class Assembly {
         DArray<double> ke(nc * m * m), fe(nc * m), val(std::max<size_t>(P.idx.size(), 1)), load(n);
         rval_.resize(std::max<size_t>(P.ridx.size(), 1));
         rhs_.resize(std::max<size_t>(P.free_list.size(), 1));
+        DArray<uint32_t> which;
+        DArray<double> table;
+        if (P.kind == FEM_MESH_ELASTIC_HEX8) {
+            which = up(P.which, s);
+            table = up(P.table, s);
+        }
         FEM_HIP(hipEventRecord(e0, s));
         const size_t rows = nc * m;
         switch (P.kind) {
@@ -467,6 +525,11 @@ class Assembly {
                 hipLaunchKernelGGL(k_row_tet4, dim3(grid_of(rows)), dim3(ABLOCK), 0, s, nc, (const double*)xyz.data(),
                                    (const uint32_t*)conn.data(), (const double*)coef.data(), mesh.source, ke.data(),
                                    fe.data());
+                break;
+            case FEM_MESH_ELASTIC_HEX8:
+                hipLaunchKernelGGL(k_row_hex8e, dim3(grid_of(rows)), dim3(ABLOCK), 0, s, nc, (const double*)xyz.data(),
+                                   (const uint32_t*)conn.data(), (const uint32_t*)which.data(),
+                                   (const double*)table.data(), (const double*)body.data(), ke.data(), fe.data());
                 break;
             default:
                 hipLaunchKernelGGL(k_row_tet10, dim3(grid_of(rows)), dim3(ABLOCK), 0, s, nc, (const double*)xyz.data(),

@@ -1,0 +1,805 @@
+"""The reference's case files (`grid.toml`) for steady heat conduction and linear static stress in
+box parts, read and solved with this library on the device.
+
+The reader follows crates/core/src/grid.rs (`read`, `read_grid`, `read_parts`, `read_faces`, the
+loads and probes of `read_heat`) for the keys listed below; the model is built as
+crates/fem/src/grid.rs builds the graded grid (`axis_planes`, `place`, `assign`, `faces`, `crop`,
+`locate`), as heat.rs:1483-1597 `build_grid` applies symmetry, heat.rs:1165-1208 `number`
+numbers the points, and as heat.rs:328-700 (steady) and elastic.rs:287-563 (static) assemble and
+solve. The matrices and right sides are assembled on the device (`fem_assemble`), the solve is the
+library's (`fem_solver_new_csr` with the grid level, and the cells for a displacement).
+
+Keys read: `units`; `[grid]` max_cell, max_growth, min_cells, symmetry; `[material.*]`
+conductivity (a number) or youngs_modulus and poisson_ratio (numbers); `[part.*]` material, x, y,
+z, priority; `[face.*]` side, box, parts; `[[load]]` temperature, source, flux, convection with
+sink (steady), ux / uy / uz, traction, pressure (static); `[step]` type "steady" or "static";
+`[probe.*]` point. Anything else (tables, thermal strain, orthotropic materials, footprints,
+radiation, transient and other steps) is refused with a message.
+"""
+from __future__ import annotations
+
+import bisect
+import math
+import os
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+try:
+    import tomllib as _toml  # Python 3.11+
+except ImportError:  # pragma: no cover
+    import tomli as _toml
+
+VOID = -1
+# crates/fem/src/grid.rs:55-81 Side::ALL: x-, x+, y-, y+, z-, z+.
+SIDES = [(0, False), (0, True), (1, False), (1, True), (2, False), (2, True)]
+SIDE_NAMES = {"x-": 0, "x+": 1, "y-": 2, "y+": 3, "z-": 4, "z+": 5}
+VTK = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+STEADY_DIRECT_LIMIT = 3000     # solve.rs:18
+DIRECT_FLOPS = 1.5e5           # elastic.rs:263
+DIRECT_ENTRIES = 250_000_000   # elastic.rs:269
+MULTIGRID_ASPECT = 2.0         # elastic.rs:276
+
+
+class CaseError(Exception):
+    pass
+
+
+@dataclass
+class Part:
+    name: str
+    material: int
+    priority: int
+    lo: list
+    hi: list
+
+
+@dataclass
+class FaceSet:
+    name: str
+    sides: list          # indices into SIDES
+    region: Optional[tuple]
+    parts: list
+
+
+@dataclass
+class Case:
+    units: str
+    step: str                       # "steady" or "static"
+    max_cell: float
+    max_growth: float
+    min_cells: int
+    symmetry: list
+    materials: list                 # dicts: name, and conductivity or youngs_modulus + poisson_ratio
+    parts: list
+    faces: list
+    loads: list                     # dicts: kind and its values
+    probes: list                    # (name, point)
+    name: str = ""
+
+
+# ------------------------------------------------------------------------------- reading
+
+def _only(table, where, keys):
+    for k in table:
+        if k not in keys:
+            raise CaseError(f"{where}: `{k}` is not read by this front end (it reads {', '.join(keys)})")
+
+
+def _number(table, key, where):
+    v = table.get(key)
+    if isinstance(v, bool) or not isinstance(v, (int, float)):
+        raise CaseError(f"{where}: `{key}` must be a number")
+    return float(v)
+
+
+def _numbers(v, n, where, key):
+    if not isinstance(v, list) or len(v) != n or not all(isinstance(x, (int, float)) and not isinstance(x, bool)
+                                                         for x in v):
+        raise CaseError(f"{where}: `{key}` must be {n} numbers")
+    return [float(x) for x in v]
+
+
+def read(text: str, name: str = "") -> Case:
+    """The case of a grid.toml text (core/src/grid.rs:79-152 `read`, :153 `read_heat`), for the
+    keys this front end reads."""
+    top = _toml.loads(text)
+    _only(top, "the top", ["units", "grid", "material", "part", "face", "load", "step", "probe"])
+    units = top.get("units")
+    if units not in ("SI-m", "SI-mm"):
+        raise CaseError(f"`units` must be SI-m or SI-mm, not {units}")
+    grid = top.get("grid", {})
+    _only(grid, "[grid]", ["max_cell", "max_growth", "min_cells", "symmetry"])
+    max_cell = _number(grid, "max_cell", "[grid]")
+    max_growth = float(grid["max_growth"]) if "max_growth" in grid else 1.3
+    min_cells = grid.get("min_cells", 2)
+    if not isinstance(min_cells, int) or min_cells < 1:
+        raise CaseError("[grid]: `min_cells` must be a whole number, 1 or more")
+    symmetry = [False] * 3
+    for a in grid.get("symmetry", []):
+        if a not in ("x", "y", "z"):
+            raise CaseError("[grid]: `symmetry` must list axes x, y or z")
+        symmetry["xyz".index(a)] = True
+    step = top.get("step", {})
+    _only(step, "[step]", ["type"])
+    kind = step.get("type")
+    if kind not in ("steady", "static"):
+        raise CaseError(f"[step]: type {kind!r}: this front end solves steady heat and static stress")
+    materials, mindex = [], {}
+    for mname, m in top.get("material", {}).items():
+        where = f"[material.{mname}]"
+        if kind == "steady":
+            _only(m, where, ["conductivity"])
+            materials.append({"name": mname, "conductivity": _number(m, "conductivity", where)})
+        else:
+            _only(m, where, ["youngs_modulus", "poisson_ratio", "density"])
+            materials.append({"name": mname, "youngs_modulus": _number(m, "youngs_modulus", where),
+                              "poisson_ratio": _number(m, "poisson_ratio", where)})
+        mindex[mname] = len(materials) - 1
+    parts = []
+    for pname, p in top.get("part", {}).items():
+        where = f"[part.{pname}]"
+        _only(p, where, ["material", "x", "y", "z", "priority"])
+        if p.get("material") not in mindex:
+            raise CaseError(f"{where}: no material {p.get('material')}")
+        lo, hi = [0.0] * 3, [0.0] * 3
+        for d, ax in enumerate("xyz"):
+            r = _numbers(p.get(ax), 2, where, ax)
+            lo[d], hi[d] = r
+        prio = p.get("priority", 0)
+        if not isinstance(prio, int):
+            raise CaseError(f"{where}: `priority` must be a whole number")
+        parts.append(Part(pname, mindex[p["material"]], prio, lo, hi))
+    if not parts:
+        raise CaseError("the model has no part")
+    pindex = {p.name: i for i, p in enumerate(parts)}
+    faces, findex = [], {}
+    for fname, f in top.get("face", {}).items():
+        where = f"[face.{fname}]"
+        _only(f, where, ["side", "box", "parts"])
+        sides = f.get("side", [])
+        sides = [sides] if isinstance(sides, str) else sides
+        for s in sides:
+            if s not in SIDE_NAMES:
+                raise CaseError(f"{where}: side `{s}` must be x-, x+, y-, y+, z- or z+")
+        region = None
+        if "box" in f:
+            v = _numbers(f["box"], 6, where, "box")
+            region = (v[:3], v[3:])
+        fparts = []
+        for pn in f.get("parts", []):
+            if pn not in pindex:
+                raise CaseError(f"{where}: no part {pn}")
+            fparts.append(pindex[pn])
+        faces.append(FaceSet(fname, [SIDE_NAMES[s] for s in sides], region, fparts))
+        findex[fname] = len(faces) - 1
+    loads = []
+    for i, l in enumerate(top.get("load", [])):
+        where = f"[[load]] {i + 1}"
+
+        def face_of(l=l, where=where):
+            if l.get("face") not in findex:
+                raise CaseError(f"{where}: no face set {l.get('face')}")
+            return findex[l["face"]]
+        if kind == "steady":
+            if "temperature" in l:
+                _only(l, where, ["temperature", "face"])
+                loads.append({"kind": "temperature", "faces": face_of(), "value": _number(l, "temperature", where)})
+            elif "source" in l:
+                _only(l, where, ["source", "part"])
+                if l.get("part") not in pindex:
+                    raise CaseError(f"{where}: no part {l.get('part')}")
+                loads.append({"kind": "source", "part": pindex[l["part"]], "value": _number(l, "source", where)})
+            elif "flux" in l:
+                _only(l, where, ["flux", "face"])
+                loads.append({"kind": "flux", "faces": face_of(), "value": _number(l, "flux", where)})
+            elif "convection" in l:
+                _only(l, where, ["convection", "sink", "face"])
+                film = _number(l, "convection", where)
+                if not film > 0.0:
+                    raise CaseError("a convection needs a film coefficient above 0")
+                loads.append({"kind": "convection", "faces": face_of(), "film": film, "sink": _number(l, "sink", where)})
+            else:
+                raise CaseError(f"{where}: a steady load here is temperature, source, flux or convection")
+        else:
+            if any(k in l for k in ("ux", "uy", "uz")):
+                _only(l, where, ["ux", "uy", "uz", "face"])
+                value = [(_number(l, k, where) if k in l else None) for k in ("ux", "uy", "uz")]
+                loads.append({"kind": "displacement", "faces": face_of(), "value": value})
+            elif "traction" in l:
+                _only(l, where, ["traction", "face"])
+                loads.append({"kind": "traction", "faces": face_of(), "value": _numbers(l["traction"], 3, where, "traction")})
+            elif "pressure" in l:
+                _only(l, where, ["pressure", "face"])
+                loads.append({"kind": "pressure", "faces": face_of(), "value": _number(l, "pressure", where)})
+            else:
+                raise CaseError(f"{where}: a static load here is ux / uy / uz, traction or pressure")
+    probes = []
+    for pname, p in top.get("probe", {}).items():
+        where = f"[probe.{pname}]"
+        _only(p, where, ["point"])
+        probes.append((pname, _numbers(p.get("point"), 3, where, "point")))
+    return Case(units, kind, max_cell, max_growth, min_cells, symmetry, materials, parts, faces, loads, probes, name)
+
+
+def load_case(path: str) -> Case:
+    """The case of a grid.toml file, named after its folder."""
+    with open(path, "rb") as f:
+        text = f.read().decode()
+    return read(text, os.path.basename(os.path.dirname(os.path.abspath(path))))
+
+
+# ------------------------------------------------------------------------------- the grid
+
+def tol(p) -> float:
+    """grid.rs:405-408: 1e-9 of the length of the planes."""
+    return 1e-9 * max(abs(p[-1] - p[0]), 2.2250738585072014e-308)
+
+
+def place(keys, target, at_key, slope):
+    """grid.rs:492-521: the planes of an axis from the size function h."""
+    def h(x, i):
+        v = target[i]
+        for k, s in zip(keys, at_key):
+            v = min(v, s + slope * abs(x - k))
+        return v
+    planes = [keys[0]]
+    for i in range(len(keys) - 1):
+        a, b = keys[i], keys[i + 1]
+        samples = 400
+        step = (b - a) / float(samples)
+        phi = [0.0] * (samples + 1)
+        for s in range(samples):
+            x0, x1 = a + s * step, a + (s + 1) * step
+            phi[s + 1] = phi[s] + 0.5 * step * (1.0 / h(x0, i) + 1.0 / h(x1, i))
+        total = phi[samples]
+        cells = max(int(math.ceil(total - 1e-6)), 1)
+        for c in range(1, cells):
+            want = total * c / cells
+            s = min(max(bisect.bisect_left(phi, want), 1), samples)
+            f = (want - phi[s - 1]) / (phi[s] - phi[s - 1])
+            planes.append(a + (s - 1.0 + f) * step)
+        planes.append(b)
+    return planes
+
+
+def axis_planes(parts, d, max_cell, max_growth, min_cells, extra):
+    """grid.rs:411-489: the graded planes of axis d."""
+    keys = sorted([v for p in parts for v in (p.lo[d], p.hi[d])] + list(extra))
+    t = tol(keys)
+    dedup = [keys[0]]
+    for v in keys[1:]:
+        if abs(v - dedup[-1]) > t:
+            dedup.append(v)
+    keys = dedup
+    target = []
+    for w0, w1 in zip(keys[:-1], keys[1:]):
+        v = max_cell
+        for p in parts:
+            if p.lo[d] <= w0 + t and p.hi[d] >= w1 - t:
+                v = min(v, (p.hi[d] - p.lo[d]) / float(min_cells))
+        target.append(v)
+    at_key = []
+    for k in range(len(keys)):
+        left = target[k - 1] if k > 0 else math.inf
+        right = target[k] if k < len(target) else math.inf
+        at_key.append(min(left, right))
+    slope = 0.5 * (max_growth - 1.0)
+    for _ in range(60):
+        planes = place(keys, target, at_key, slope)
+        cells = [b - a for a, b in zip(planes[:-1], planes[1:])]
+        fixed = False
+        for c in range(len(cells) - 1):
+            small, large = (cells[c], cells[c + 1]) if cells[c] < cells[c + 1] else (cells[c + 1], cells[c])
+            if large > max_growth * small * (1.0 + 1e-9):
+                x = planes[c + 1]
+                k = min(bisect.bisect_left(keys, x - t), len(keys) - 1)
+                for kk in (max(k - 1, 0), k, min(k + 1, len(keys) - 1)):
+                    if abs(keys[kk] - x) <= 2.0 * large and at_key[kk] > small * 1.0001:
+                        at_key[kk] = small
+                        fixed = True
+        if not fixed:
+            return planes
+    raise CaseError(f"the {'xyz'[d]} grid planes break the growth limit max_growth = {max_growth}: use a larger "
+                    "max_growth or a smaller max_cell")
+
+
+class Grid:
+    """grid.rs:108-403: the planes of each axis and the part of each cell (x fastest) or VOID."""
+
+    def __init__(self, planes, part):
+        self.planes = [list(p) for p in planes]
+        self.part = np.asarray(part, np.int64)
+
+    @classmethod
+    def build(cls, case: Case, extra):
+        planes = [axis_planes(case.parts, d, case.max_cell, case.max_growth, case.min_cells, extra[d]) for d in range(3)]
+        g = cls(planes, np.full((len(planes[0]) - 1) * (len(planes[1]) - 1) * (len(planes[2]) - 1), VOID))
+        g._assign(case.parts)
+        return g
+
+    def dims(self):
+        return [len(p) - 1 for p in self.planes]
+
+    def cell_index(self, c):
+        nx, ny, _ = self.dims()
+        return c[0] + nx * (c[1] + ny * c[2])
+
+    def cell_ijk(self, cell):
+        nx, ny, _ = self.dims()
+        return [cell % nx, (cell // nx) % ny, cell // (nx * ny)]
+
+    def point_index(self, p):
+        nx, ny, _ = self.dims()
+        return p[0] + (nx + 1) * (p[1] + (ny + 1) * p[2])
+
+    def size(self, cell):
+        c = self.cell_ijk(cell)
+        return [self.planes[d][c[d] + 1] - self.planes[d][c[d]] for d in range(3)]
+
+    def centre(self, cell):
+        c = self.cell_ijk(cell)
+        return [0.5 * (self.planes[d][c[d]] + self.planes[d][c[d] + 1]) for d in range(3)]
+
+    def corners(self, cell):
+        i, j, k = self.cell_ijk(cell)
+        return [(i + b[0], j + b[1], k + b[2]) for b in VTK]
+
+    def _assign(self, parts):
+        """grid.rs:290-327: each cell takes the part that holds its centre (priority, then order)."""
+        best = [None] * len(self.part)
+        for index, p in enumerate(parts):
+            rng = []
+            for d in range(3):
+                pl = self.planes[d]
+                t = tol(pl)
+                rng.append((bisect.bisect_left(pl, p.lo[d] - t), bisect.bisect_left(pl, p.hi[d] - t)))
+            for k in range(*rng[2]):
+                for j in range(*rng[1]):
+                    for i in range(*rng[0]):
+                        cell = self.cell_index([i, j, k])
+                        other = best[cell]
+                        if other is not None and parts[other].priority > p.priority:
+                            continue
+                        if other is not None and parts[other].priority == p.priority and \
+                                parts[other].material != p.material:
+                            x = self.centre(cell)
+                            raise CaseError(f"parts {parts[other].name} and {p.name} overlap at ({x[0]}, {x[1]}, {x[2]}) "
+                                            "with the same priority and other materials: give the part that must "
+                                            "hold the overlap a higher priority")
+                        best[cell] = index
+        self.part = np.array([VOID if b is None else b for b in best], np.int64)
+
+    def neighbour(self, cell, side):
+        axis, positive = SIDES[side]
+        c = self.cell_ijk(cell)
+        n = self.dims()[axis]
+        if positive:
+            if c[axis] + 1 >= n:
+                return None
+            c[axis] += 1
+        else:
+            if c[axis] == 0:
+                return None
+            c[axis] -= 1
+        return self.cell_index(c)
+
+    def faces(self):
+        """grid.rs:233-253: each side of a cell with a part and no part beyond it, cells in order."""
+        out = []
+        for cell in range(len(self.part)):
+            if self.part[cell] == VOID:
+                continue
+            for s in range(6):
+                n = self.neighbour(cell, s)
+                if n is None or self.part[n] == VOID:
+                    out.append((cell, s))
+        return out
+
+    def face_centre(self, face):
+        cell, s = face
+        axis, positive = SIDES[s]
+        c = self.cell_ijk(cell)
+        x = self.centre(cell)
+        x[axis] = self.planes[axis][c[axis] + (1 if positive else 0)]
+        return x
+
+    def face_area(self, face):
+        s = self.size(face[0])
+        axis = SIDES[face[1]][0]
+        v = 1.0
+        for d in range(3):
+            if d != axis:
+                v *= s[d]
+        return v
+
+    def face_points(self, face):
+        """grid.rs:272-289: (0, 0), (1, 0), (1, 1), (0, 1) along the face's two axes."""
+        cell, s = face
+        axis, positive = SIDES[s]
+        a, b = {0: (1, 2), 1: (0, 2), 2: (0, 1)}[axis]
+        base = self.cell_ijk(cell)
+        base[axis] += 1 if positive else 0
+        out = []
+        for da, db in ((0, 0), (1, 0), (1, 1), (0, 1)):
+            p = list(base)
+            p[a] += da
+            p[b] += db
+            out.append(tuple(p))
+        return out
+
+    def select(self, all_faces, fs: FaceSet):
+        """heat.rs:1575-1604 `select`."""
+        out = []
+        for f in all_faces:
+            if fs.sides and f[1] not in fs.sides:
+                continue
+            if fs.parts and int(self.part[f[0]]) not in fs.parts:
+                continue
+            if fs.region is not None:
+                c = self.face_centre(f)
+                if any(c[d] < fs.region[0][d] - tol(self.planes[d]) or c[d] > fs.region[1][d] + tol(self.planes[d])
+                       for d in range(3)):
+                    continue
+            out.append(f)
+        return out
+
+    def planes_mirror(self, axis, at):
+        p = self.planes[axis]
+        n = len(p)
+        return all(abs((p[i] - at) + (p[n - 1 - i] - at)) <= tol(p) for i in range(n))
+
+    def mirror_cell(self, cell, axis):
+        c = self.cell_ijk(cell)
+        c[axis] = self.dims()[axis] - 1 - c[axis]
+        return self.cell_index(c)
+
+    def crop(self, axis, at):
+        """grid.rs:329-357: only the cells above the plane `at` stay."""
+        p = self.planes[axis]
+        first = next(i for i, x in enumerate(p) if abs(x - at) <= tol(p))
+        planes = [list(q) for q in self.planes]
+        planes[axis] = p[first:]
+        nx, ny, nz = self.dims()
+        part, old = [], []
+        for k in range(nz):
+            for j in range(ny):
+                for i in range(nx):
+                    if (i, j, k)[axis] >= first:
+                        cell = self.cell_index([i, j, k])
+                        part.append(int(self.part[cell]))
+                        old.append(cell)
+        return Grid(planes, part), old
+
+    def locate(self, x):
+        """grid.rs:361-402: the cell of a part that holds x, and x's local coordinates (0 to 1)."""
+        choices = []
+        for d in range(3):
+            p = self.planes[d]
+            t = tol(p)
+            if x[d] < p[0] - t or x[d] > p[-1] + t:
+                return None
+            i = min(max(bisect.bisect_right(p, x[d]), 1), len(p) - 1) - 1
+            ch = [i]
+            if i > 0 and abs(x[d] - p[i]) <= t:
+                ch.append(i - 1)
+            if i + 2 < len(p) and abs(x[d] - p[i + 1]) <= t:
+                ch.append(i + 1)
+            choices.append(ch)
+        for i in choices[0]:
+            for j in choices[1]:
+                for k in choices[2]:
+                    cell = self.cell_index([i, j, k])
+                    if self.part[cell] != VOID:
+                        c = (i, j, k)
+                        local = []
+                        for d in range(3):
+                            a, b = self.planes[d][c[d]], self.planes[d][c[d] + 1]
+                            local.append(min(max((x[d] - a) / (b - a), 0.0), 1.0))
+                        return cell, local
+        return None
+
+
+def build_grid(case: Case, used, keys):
+    """heat.rs:1483-1573: the grid with the mid planes of symmetry, the faces of each face set,
+    and the grid cut at each plane of symmetry (the part above it stays)."""
+    mids = [None] * 3
+    for d in range(3):
+        if case.symmetry[d]:
+            lo = min(p.lo[d] for p in case.parts)
+            hi = max(p.hi[d] for p in case.parts)
+            mids[d] = 0.5 * (lo + hi)
+    extra = [[m] if m is not None else [] for m in mids]
+    grid = Grid.build(case, extra)
+    all_faces = grid.faces()
+    faces = [grid.select(all_faces, fs) for fs in case.faces]
+    for fs, lst, u in zip(case.faces, faces, used):
+        if u and not lst:
+            raise CaseError(f"face set {fs.name} selects no face of the model: a face of the model has no part beyond it")
+    for d in range(3):
+        mid = mids[d]
+        if mid is None:
+            continue
+        axis = "xyz"[d]
+        if not grid.planes_mirror(d, mid):
+            raise CaseError(f"symmetry at {axis} = {mid}: the grid planes do not mirror at this plane. Solve the full model")
+
+        def sign(c):
+            p = int(grid.part[c])
+            return None if p == VOID else (case.parts[p].material, keys[p])
+        if any(sign(c) != sign(grid.mirror_cell(c, d)) for c in range(len(grid.part))):
+            raise CaseError(f"symmetry at {axis} = {mid}: the materials or the loads of the parts do not mirror at "
+                            "this plane. Solve the full model")
+        for fs, lst in zip(case.faces, faces):
+            have = set(lst)
+            for cell, s in lst:
+                ax, pos = SIDES[s]
+                ms = SIDES.index((ax, not pos)) if ax == d else s
+                if (grid.mirror_cell(cell, d), ms) not in have:
+                    raise CaseError(f"symmetry at {axis} = {mid}: the faces of face set {fs.name} do not mirror at "
+                                    "this plane. Solve the full model")
+        cut, old = grid.crop(d, mid)
+        new = {o: k for k, o in enumerate(old)}
+        faces = [[(new[c], s) for c, s in lst if c in new] for lst in faces]
+        grid = cut
+    return grid, faces, mids
+
+
+def number(grid: Grid):
+    """heat.rs:1165-1208: points first seen over the cells with a part, corners in VTK order."""
+    number_of = {}
+    ijk, cells, cell_of = [], [], []
+    for cell in range(len(grid.part)):
+        if grid.part[cell] == VOID:
+            continue
+        for p in grid.corners(cell):
+            slot = number_of.get(p)
+            if slot is None:
+                slot = len(ijk)
+                number_of[p] = slot
+                ijk.append(p)
+            cells.append(slot)
+        cell_of.append(cell)
+    return np.array(ijk, np.int64).reshape(-1, 3), np.array(cells, np.int64).reshape(-1, 8), cell_of, number_of
+
+
+def probe_weights(case: Case, grid: Grid, mids, number_of):
+    """heat.rs:1312-1358: for each probe its (node, weight) pairs at the cell's corners (the
+    trilinear functions at its local point), and the axes on which it reads its mirror point."""
+    out = []
+    for name, point in case.probes:
+        x = list(point)
+        mirrored = [False] * 3
+        for d in range(3):
+            if mids[d] is not None and x[d] < mids[d]:
+                x[d] = 2.0 * mids[d] - x[d]
+                mirrored[d] = True
+        hit = grid.locate(x)
+        if hit is None:
+            raise CaseError(f"probe {name} at ({point[0]}, {point[1]}, {point[2]}) is in no part")
+        cell, local = hit
+        xi = [2.0 * f - 1.0 for f in local]
+        w = []
+        for b in VTK:
+            v = 1.0
+            for d in range(3):
+                v *= 0.5 * (1.0 + xi[d]) if b[d] == 1 else 0.5 * (1.0 - xi[d])
+            w.append(v)
+        out.append((name, [(number_of[c], wc) for c, wc in zip(grid.corners(cell), w)], mirrored))
+    return out
+
+
+# ------------------------------------------------------------------------------- the solves
+
+@dataclass
+class Result:
+    """The probes of a solve (column name -> value), the field at the points, and the solve."""
+    probes: dict
+    field: np.ndarray
+    ijk: np.ndarray
+    grid: Grid
+    method: str
+    stats: object = None
+    unknowns: int = 0
+    notes: list = field(default_factory=list)
+
+
+def _face_mass(a, b):
+    """heat.rs:1280-1292 face_mass(a, b, 1): the 4 x 4 mass of a face in face_points order."""
+    bits = ((0, 0), (1, 0), (1, 1), (0, 1))
+    ms = lambda l, p, q: l / 3.0 if p == q else l / 6.0
+    return [[1.0 * ms(a, bp[0], bq[0]) * ms(b, bp[1], bq[1]) for bq in bits] for bp in bits]
+
+
+def solve(case: Case, precision: str = "fp32", method: Optional[str] = None, tol: float = 1e-10,
+          device: int = 0) -> Result:
+    """The case solved on the device: steady heat (heat.rs:328-700) or static stress
+    (elastic.rs:287-563)."""
+    if case.step == "steady":
+        return _solve_heat(case, precision, method, tol, device)
+    return _solve_elastic(case, precision, method, tol, device)
+
+
+def _entry(ptr, idx, r, c):
+    lo, hi = int(ptr[r]), int(ptr[r + 1])
+    k = lo + int(np.searchsorted(idx[lo:hi], c))
+    assert k < hi and idx[k] == c
+    return k
+
+
+def _solve_heat(case, precision, method, tol_, device):
+    import femahoy_amd as fem
+    used = [any(l.get("faces") == f for l in case.loads) for f in range(len(case.faces))]
+    part_source = [0.0] * len(case.parts)
+    for l in case.loads:
+        if l["kind"] == "source":
+            part_source[l["part"]] += l["value"]
+    keys = [(q,) for q in part_source]
+    grid, faces, mids = build_grid(case, used, keys)
+    ijk, cells, cell_of, number_of = number(grid)
+    n = len(ijk)
+    xyz = np.array([[grid.planes[d][p[d]] for d in range(3)] for p in ijk])
+    kc = np.array([case.materials[case.parts[int(grid.part[c])].material]["conductivity"] for c in cell_of])
+    # heat.rs:384-426: the volume sources, cell by cell, V/8 to each corner
+    source = np.zeros(n)
+    for e, c in enumerate(cell_of):
+        s = grid.size(c)
+        vol = s[0] * s[1] * s[2]
+        q = part_source[int(grid.part[c])]
+        for node in cells[e]:
+            source[node] += q * vol / 8.0
+    flux = np.zeros(n)
+    film = np.zeros(n)
+    fixed = [None] * n
+    convection = []
+    for l in case.loads:
+        if l["kind"] == "source":
+            continue
+        for face in faces[l["faces"]]:
+            nodes = [number_of[p] for p in grid.face_points(face)]
+            share = grid.face_area(face) / 4.0
+            if l["kind"] == "temperature":
+                for v in nodes:
+                    if fixed[v] is not None and fixed[v][0] != l["value"]:
+                        raise CaseError(f"the face sets {case.faces[fixed[v][1]].name} and {case.faces[l['faces']].name} "
+                                        f"share a point with 2 fixed temperatures, {fixed[v][0]} and {l['value']}")
+                    fixed[v] = (l["value"], l["faces"])
+            elif l["kind"] == "flux":
+                for v in nodes:
+                    flux[v] += l["value"] * share
+            else:
+                s = grid.size(face[0])
+                axis = SIDES[face[1]][0]
+                a, b = {0: (s[1], s[2]), 1: (s[0], s[2]), 2: (s[0], s[1])}[axis]
+                convection.append((nodes, _face_mass(a, b), l["film"]))
+                for v in nodes:
+                    film[v] += l["film"] * l["sink"] * share
+    load = source + flux + film
+    isfixed = np.array([f is not None for f in fixed], np.uint8)
+    value = np.array([f[0] if f is not None else 0.0 for f in fixed])
+    mesh = fem.Mesh("heat_hex8", xyz, cells, kc, isfixed, value, load=load)
+    asm = fem.Assembly(mesh, device)
+    ptr, idx, val, rhs, free = asm.arrays()
+    renum = -np.ones(n, np.int64)
+    renum[free] = np.arange(len(free))
+    # heat.rs:465-473: the convection matrices h m added after the conduction, face by face; the
+    # held points' share moves to the right side
+    for nodes, m, h in convection:
+        for p in range(4):
+            rp = renum[nodes[p]]
+            if rp < 0:
+                continue
+            for q in range(4):
+                rq = renum[nodes[q]]
+                hm = h * m[p][q]
+                if rq >= 0:
+                    val[_entry(ptr, idx, rp, rq)] += hm
+                else:
+                    rhs[rp] -= hm * value[nodes[q]]
+    planes = [np.asarray(p, np.float64) for p in grid.planes]
+    g = fem.Grid(planes, ijk[free].astype(np.uint32), np.zeros(len(free), np.uint8))
+    s = fem.CsrSolver(ptr, idx, val, "grid", g, method=method, precision=precision, tol=tol_,
+                      crossover=STEADY_DIRECT_LIMIT, device=device)
+    x, st = s.solve(rhs)
+    t = value.copy()
+    t[free] = x
+    probes = {}
+    for name, pairs, _ in probe_weights(case, grid, mids, number_of):
+        probes[name] = sum(t[v] * w for v, w in pairs)
+    return Result(probes, t, ijk, grid, s.method(), st, len(free))
+
+
+def _solve_elastic(case, precision, method, tol_, device):
+    import femahoy_amd as fem
+    used = [any(l.get("faces") == f for l in case.loads) for f in range(len(case.faces))]
+    keys = [(0.0, 0.0, 0.0) for _ in case.parts]
+    grid, faces, mids = build_grid(case, used, keys)
+    ijk, cells, cell_of, number_of = number(grid)
+    n = len(ijk)
+    xyz = np.array([[grid.planes[d][p[d]] for d in range(3)] for p in ijk])
+    mat = [case.materials[case.parts[int(grid.part[c])].material] for c in cell_of]
+    e_mod = np.array([m["youngs_modulus"] for m in mat])
+    nu = np.array([m["poisson_ratio"] for m in mat])
+    force = np.zeros(3 * n)
+    fixed = [None] * (3 * n)
+    for l in case.loads:
+        for face in faces[l["faces"]]:
+            pts = grid.face_points(face)
+            if l["kind"] == "displacement":
+                for p in pts:
+                    v = number_of[p]
+                    for d in range(3):
+                        val = l["value"][d]
+                        if val is None:
+                            continue
+                        if fixed[3 * v + d] is not None and fixed[3 * v + d] != val:
+                            x = [grid.planes[a][p[a]] for a in range(3)]
+                            raise CaseError(f"2 fixed displacements give 2 values of u{'xyz'[d]} at the point "
+                                            f"({x[0]:e}, {x[1]:e}, {x[2]:e}): {fixed[3 * v + d]} and {val}")
+                        fixed[3 * v + d] = val
+            else:
+                if l["kind"] == "traction":
+                    t = list(l["value"])
+                else:  # elastic.rs:448-459: a pressure pushes along the inner normal
+                    axis, positive = SIDES[face[1]]
+                    t = [0.0, 0.0, 0.0]
+                    t[axis] = -l["value"] if positive else l["value"]
+                area = grid.face_area(face)
+                for p in pts:
+                    v = number_of[p]
+                    for d in range(3):
+                        force[3 * v + d] += t[d] * area / 4.0
+    # elastic.rs:493-511: the planes of symmetry hold the normal component
+    for i in range(n):
+        for d in range(3):
+            if mids[d] is not None and ijk[i][d] == 0:
+                f = fixed[3 * i + d]
+                if f is not None and f != 0.0:
+                    raise CaseError(f"a fixed u{'xyz'[d]} = {f} is on the plane of symmetry: it does not mirror. "
+                                    "Solve the full model")
+                fixed[3 * i + d] = 0.0
+    isfixed = np.array([f is not None for f in fixed], np.uint8)
+    value = np.array([f if f is not None else 0.0 for f in fixed])
+    mesh = fem.Mesh("elastic_hex8", xyz, cells, e_mod, isfixed, value, coef1=nu, load=force)
+    asm = fem.Assembly(mesh, device)
+    ptr, idx, val, rhs, free = asm.arrays()
+    planes = [np.asarray(p, np.float64) for p in grid.planes]
+    g = fem.Grid(planes, ijk[free // 3].astype(np.uint32), (free % 3).astype(np.uint8))
+    # elastic.rs:525-547: Crossover::Flops, flat when a cell's sides differ by more than 2 times
+    aspect = 1.0
+    for c in cell_of:
+        s = grid.size(c)
+        aspect = max(aspect, max(s) / min(s))
+    origin = np.array([grid.cell_ijk(c) for c in cell_of], np.uint32)
+    element = np.stack([fem.hex_element(grid.size(c), m["youngs_modulus"], m["poisson_ratio"])
+                        for c, m in zip(cell_of, mat)])
+    crossover = fem.Flops(flops=DIRECT_FLOPS, entries=DIRECT_ENTRIES, flat=aspect > MULTIGRID_ASPECT)
+    s = fem.CsrSolver(ptr, idx, val, "grid", g, method=method, precision=precision, tol=tol_, crossover=crossover,
+                      device=device, cells=(origin, element))
+    x, st = s.solve(rhs)
+    u = value.copy()
+    u[free] = x
+    probes = {}
+    for name, pairs, mirrored in probe_weights(case, grid, mids, number_of):
+        for d, comp in enumerate(("ux", "uy", "uz")):
+            v = sum(u[3 * node + d] * w for node, w in pairs)
+            probes[f"{name}.{comp}"] = -v if mirrored[d] else v
+    return Result(probes, u, ijk, grid, s.method(), st, len(free))
+
+
+def check(result: Result, check_text: str, case_name: str):
+    """The entries of a check.toml (crates/fh/src/validation/run.rs:38: pass when |computed -
+    reference| <= rel_tol |reference| + abs_tol) that name this case and a column of its probes:
+    a list of (entry, column, computed, reference, limit, passed)."""
+    doc = _toml.loads(check_text)
+    rel, ab = float(doc.get("rel_tol", 0.0)), float(doc.get("abs_tol", 0.0))
+    out = []
+    for key, item in doc.items():
+        if not isinstance(item, dict) or item.get("case") != case_name or item.get("column") not in result.probes:
+            continue
+        got = result.probes[item["column"]]
+        ref = float(item["value"])
+        limit = rel * abs(ref) + ab
+        out.append((key, item["column"], got, ref, limit, abs(got - ref) <= limit))
+    return out

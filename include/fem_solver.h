@@ -262,8 +262,12 @@ enum fem_mesh_kind {
     FEM_MESH_HEAT_HEX8 = 0,     /* conduction of axis-aligned box cells (heat.rs:1246-1278), 8 corners
                                    in VTK order (hex.rs:34-43); cell sizes from corners 0 and 6 */
     FEM_MESH_HEAT_TET4 = 1,     /* conduction of straight tetrahedra (heat.rs:972-1000) */
-    FEM_MESH_ELASTIC_TET10 = 2  /* isotropic elasticity of straight tet10 (tet10.rs:64-120): 4 corners, then
+    FEM_MESH_ELASTIC_TET10 = 2, /* isotropic elasticity of straight tet10 (tet10.rs:64-120): 4 corners, then
                                    the 6 edge nodes in the order of tet10.rs:38 */
+    FEM_MESH_ELASTIC_HEX8 = 3   /* isotropic elasticity of axis-aligned box cells with incompatible modes
+                                   (hex.rs:199-279, the box-grid caller elastic.rs:287-547), 8 corners in
+                                   VTK order, sizes from corners 0 and 6; the element of each distinct
+                                   (sizes, material) on the host, the rest on the device */
 };
 
 /* A mesh and its loads and held unknowns, as the reference's callers hold them before
@@ -279,7 +283,8 @@ typedef struct fem_mesh {
     const double* coef0;     /* per cell: the isotropic conductivity, or Young's modulus */
     const double* coef1;     /* per cell: Poisson's ratio (elasticity); may be NULL for heat */
     double source;           /* heat: source per volume, V/8 (hex8) or V/4 (tet4) to each corner */
-    double body[3];          /* elasticity: force per volume, -V/20 to each corner and V/5 to each edge node */
+    double body[3];          /* elasticity: force per volume; tet10: -V/20 to each corner and V/5 to each
+                                edge node; hex8: V/8 to each corner (elastic.rs:384-392) */
     const uint8_t* fixed;    /* per unknown: 1 when held */
     const double* value;     /* per unknown: its held value (read where fixed) */
     const double* load;      /* per unknown: a further load (face loads) added after the volume loads; may be NULL */

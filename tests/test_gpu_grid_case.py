@@ -1,0 +1,47 @@
+"""The reference's own validation cases, read from their grid.toml files and solved on the device
+through the grid.toml front end (femahoy_amd.grid_case): each probe passes the case's check.toml
+with the reference's pass rule (|computed - reference| <= rel_tol |reference| + abs_tol,
+crates/fh/src/validation/run.rs:38), and where a case's derivation notes print what the
+reference's own grid solver gave on this mesh, the probe matches it to half a unit of its last
+printed digit.
+
+The case files are fixtures copied from the reference's validation/cases (grid.toml and
+check.toml as they are): a heated square bar with quarter symmetry (grid-heated-bar), a hex
+cantilever with incompatible modes (grid-cantilever), a wall of 3 layers with a convection on
+each side on a graded grid (grid-layered-wall), and a steel plate with a heated copper block on
+it, void cells beside the block and half symmetry in x and y (grid-heated-block-ccx).
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CASES = os.path.join(os.path.dirname(__file__), "golden", "cases")
+
+# derivation.md of each case: the reference grid solver's value on the case's own mesh
+PRINTED = {
+    ("grid-heated-bar", "centre"): (14.745623, 0.5e-6),      # max_cell 0.625 mm
+    ("grid-cantilever", "tip.uz"): (-0.190113, 0.5e-6),      # max_cell 2.5 mm
+}
+
+
+@pytest.mark.parametrize("case_dir", ["grid-heated-bar/bar", "grid-cantilever/beam", "grid-layered-wall/wall",
+                                      "grid-heated-block-ccx/block"])
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_reference_case_passes_its_check(fem, case_dir, precision):
+    from femahoy_amd import grid_case
+    folder = case_dir.split("/")[0]
+    case = grid_case.load_case(os.path.join(CASES, case_dir, "grid.toml"))
+    res = grid_case.solve(case, precision=precision, tol=1e-12)
+    print(case_dir, precision, res.method, res.unknowns, res.stats)
+    with open(os.path.join(CASES, folder, "check.toml")) as f:
+        rows = grid_case.check(res, f.read(), case.name)
+    assert rows, "no check of this case reads a probe"
+    for key, column, got, ref, limit, ok in rows:
+        print(f"  {key}: {column} = {got!r} (reference {ref}, limit {limit:.3g})")
+        assert ok, (key, got, ref, limit)
+    for (c, column), (value, half) in PRINTED.items():
+        if c == folder:
+            assert abs(res.probes[column] - value) <= half, (column, res.probes[column], value)
