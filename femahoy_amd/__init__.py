@@ -81,7 +81,11 @@ class _Options(C.Structure):
         ("crossover_flops", C.c_double),
         ("crossover_entries", C.c_uint64),
         ("crossover_flat", C.c_int32),
+        ("direct_solver", C.c_int32),
     ]
+
+
+DIRECT_SOLVERS = {"auto": 0, "dense": 1, "multifrontal": 2}
 
 
 class _Stats(C.Structure):
@@ -650,10 +654,11 @@ class CsrSolver(Solver):
                  tol: float = PCG_TOL, max_iter: int = PCG_MAX_ITER, crossover=STEADY_DIRECT_LIMIT,
                  levels: int = 0, cheb_degree: int = 0, cheb_lower: float = 0.1, cheb_upper: float = 1.1,
                  power_steps: int = 20, coarse_max: int = 6000, refine: bool = True, device: int = 0,
-                 xyz=None, cells=None):
+                 xyz=None, cells=None, direct_solver: str = "auto"):
         n = len(ptr) - 1
         o = _options(method, precision, tol, max_iter, crossover, levels, cheb_degree, cheb_lower, cheb_upper,
                      power_steps, coarse_max, refine, device, True)
+        o.direct_solver = DIRECT_SOLVERS[direct_solver]
         a, keep = _csr_struct(ptr, idx, val, n, n)
         u = _Unknowns()
         u.kind = UNKNOWNS[unknowns]

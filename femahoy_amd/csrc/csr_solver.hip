@@ -1,8 +1,9 @@
 // The solver of the assembled boundary (fem_solver_new_csr): the reference's Solver::new for a
 // CSR matrix of the free unknowns (crates/fem/src/solve.rs:92-183) on the device.
 //
-//  - Direct: the dense FP64 LDL^T of the whole matrix (dense.hip), up to DENSE_DIRECT_LIMIT
-//    (40000) unknowns.
+//  - Direct: the dense FP64 LDL^T of the whole matrix (dense.hip) up to MULTIFRONTAL_FROM unknowns,
+//    the multifrontal LDL^T (multifrontal.hip) above, as the reference's factor is its multifrontal
+//    Cholesky (cholesky.rs:44-160); fem_options.direct_solver picks one.
 //  - Grid multigrid: the reference's hierarchy (csr_host.cpp: coarsen and Galerkin products, the
 //    level loop of multigrid.rs:84-139), the matrices and transfers of every level in CSR form on
 //    the device, a Chebyshev-Jacobi smoother, the dense LDL^T on the coarsest level.
@@ -27,6 +28,7 @@
 #include "csr_kernels.hpp"
 #include "csr_solver.hpp"
 #include "dense.hpp"
+#include "multifrontal.hpp"
 
 namespace fem {
 
@@ -233,7 +235,22 @@ class CsrSolverImpl {
         partB_.resize(parts_);
         scal_.resize(1);
         red_.resize(4);
-        if (direct_) {
+        if (direct_ && o.direct_solver != FEM_DIRECT_DENSE &&
+            (o.direct_solver == FEM_DIRECT_MULTIFRONTAL || n > MULTIFRONTAL_FROM)) {
+            // the multifrontal factor orders by nested dissection, which needs the unknowns' points
+            if (u.kind == FEM_UNKNOWNS_GRID && u.xyz.empty()) {
+                csr::validate(u.grid, n);
+                u.xyz = grid_xyz(u.grid);
+            }
+            if (u.xyz.size() == n) {
+                size_t free_b = 0, total_b = 0;
+                FEM_HIP(hipMemGetInfo(&free_b, &total_b));
+                mf_ = std::make_unique<Multifrontal>(a, u.xyz, s_, (uint64_t)(0.8 * (double)free_b));
+            } else if (o.direct_solver == FEM_DIRECT_MULTIFRONTAL) {
+                throw Error("the multifrontal factor orders by nested dissection: it needs the unknowns' coordinates");
+            }
+        }
+        if (direct_ && !mf_) {
             if (n > DENSE_DIRECT_LIMIT) {
                 std::string why;
                 if (analysed_)
@@ -325,7 +342,8 @@ class CsrSolverImpl {
         try {
             if (direct_) {
                 // solve.rs:207-213: x = A^-1 b, the true relative residual
-                direct_coarse_.solve<double>(b_.data(), x_.data(), s_);
+                if (mf_) mf_->solve(b_.data(), x_.data(), s_);
+                else direct_coarse_.solve<double>(b_.data(), x_.data(), s_);
                 out.iterations = 0;
                 out.levels = 1;
                 out.coarse_unknowns = n_;
@@ -697,6 +715,7 @@ class CsrSolverImpl {
     DArray<double> x_, b_, r_, p_, q_, z_, partA_, partB_, red_;
     DArray<PcgScalars> scal_;
     DenseCoarse direct_coarse_;
+    std::unique_ptr<Multifrontal> mf_;
     csr::GridHierarchy host_;
     std::unique_ptr<CHierarchy<float>> h32_;
     std::unique_ptr<CHierarchy<double>> h64_;

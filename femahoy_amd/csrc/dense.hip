@@ -406,6 +406,20 @@ void DenseWork::carve(void* base, int n_) {
     status = (int*)take(sizeof(int));
 }
 
+void ldlt_panel(double* A, int lda, int nb, int kb, const double* dg, double* d, double* linv, double* bk, double* P,
+                int* status, hipStream_t s) {
+    hipLaunchKernelGGL(k_ldlt_diag, dim3(1), dim3(256), 0, s, A, lda, kb, dg, d, linv, bk, status);
+    const int below = nb - kb - 1;
+    if (below == 0) return;
+    // L_ik = A_ik (D^-1 L^-1)^T ; W_ik = L_ik D_k
+    GemmArgs pg{A + (size_t)kb * NB, lda, bk, NB, A + (size_t)kb * NB, lda, NB, 0.0, 1.0, 0,
+                kb + 1, 0, 0, P, d + (size_t)kb * NB};
+    launch_gemm(pg, dim3(below, 1), s);
+    // A_ij -= W_ik L_jk^T for kb < j <= i
+    GemmArgs ug{P, NB, A + (size_t)kb * NB, lda, A, lda, NB, 1.0, -1.0, 1, kb + 1, kb + 1, below, nullptr, nullptr};
+    launch_gemm(ug, dim3(below * (below + 1) / 2, 1), s);
+}
+
 void dense_factor_inverse(const double* a, int n, const DenseWork& w, hipStream_t s) {
     if (n <= 0 || w.n != n) throw Error("internal: the dense workspace is not carved for this size");
     const int nb = w.nb, np = w.np;
@@ -414,20 +428,7 @@ void dense_factor_inverse(const double* a, int n, const DenseWork& w, hipStream_
     const unsigned blocks = (unsigned)((nn + 255) / 256);
     hipLaunchKernelGGL(k_pad, dim3(blocks), dim3(256), 0, s, a, n, w.A, np);
     hipLaunchKernelGGL(k_diag_of, dim3((np + 255) / 256), dim3(256), 0, s, (const double*)w.A, np, w.dg);
-    for (int kb = 0; kb < nb; ++kb) {
-        hipLaunchKernelGGL(k_ldlt_diag, dim3(1), dim3(256), 0, s, w.A, np, kb, (const double*)w.dg, w.d, w.linv, w.bk,
-                           w.status);
-        const int below = nb - kb - 1;
-        if (below == 0) break;
-        // L_ik = A_ik (D^-1 L^-1)^T ; W_ik = L_ik D_k
-        GemmArgs pg{w.A + (size_t)kb * NB, np, w.bk, NB, w.A + (size_t)kb * NB, np, NB, 0.0, 1.0, 0,
-                    kb + 1, 0, 0, w.P, w.d + (size_t)kb * NB};
-        launch_gemm(pg, dim3(below, 1), s);
-        // A_ij -= W_ik L_jk^T for kb < j <= i
-        GemmArgs ug{w.P, NB, w.A + (size_t)kb * NB, np, w.A, np, NB, 1.0, -1.0, 1, kb + 1, kb + 1, below,
-                    nullptr, nullptr};
-        launch_gemm(ug, dim3(below * (below + 1) / 2, 1), s);
-    }
+    for (int kb = 0; kb < nb; ++kb) ldlt_panel(w.A, np, nb, kb, w.dg, w.d, w.linv, w.bk, w.P, w.status, s);
     int bad = 0;
     FEM_HIP(hipMemcpyAsync(&bad, w.status, sizeof(int), hipMemcpyDeviceToHost, s));
     FEM_HIP(hipStreamSynchronize(s));

@@ -17,6 +17,8 @@ namespace fem {
 // (crates/fem/src/solve.rs:24), its largest direct crossover. A factor of n unknowns holds four
 // n^2 FP64 arrays on the device, 51 GB at this limit.
 constexpr uint64_t DENSE_DIRECT_LIMIT = 40000;
+// The direct method of the assembled boundary factors larger matrices by the multifrontal LDL^T.
+constexpr uint64_t MULTIFRONTAL_FROM = 8000;
 
 // The device workspace of one dense factor of n unknowns (np = n rounded up to 64, nb = np / 64
 // blocks), carved from one contiguous region of bytes(n) bytes at 256-byte aligned offsets:
@@ -46,6 +48,14 @@ struct DenseWork {
 // throws fem::Error with the reference's "not positive definite" text when a pivot fails.
 // Allocates nothing.
 void dense_factor_inverse(const double* a, int n, const DenseWork& w, hipStream_t s);
+
+// One step of the blocked right-looking LDL^T of the row-major matrix A (lda, nb blocks of 64 per
+// side, lower triangle): factor diagonal block kb (its L into the block, D into d[64 kb ..], its
+// L^-1 into linv[kb]; a pivot at or below 1e-10 |dg| sets *status to its column + 1 when *status
+// is 0), form the block column below it, and subtract its product from every block (i, j) with
+// kb < j <= i. bk (64 x 64) and P (64 (nb - kb - 1) x 64) are workspace.
+void ldlt_panel(double* A, int lda, int nb, int kb, const double* dg, double* d, double* linv, double* bk, double* P,
+                int* status, hipStream_t s);
 
 // x = A^-1 b with one step of refinement on the FP64 residual of `a`: x = inv b, r = b - A x,
 // x += inv r. inv is w.inv (FP64, leading dimension w.np); r is an n-vector workspace. Allocates

@@ -472,9 +472,11 @@ __device__ __forceinline__ void pcg_scalars_apply(PcgScalars* s, double a, doubl
     }
 }
 
+// The loop goes on while the residual is not at or below tol (multigrid.rs:721-727): a NaN
+// residual goes on to the iteration limit, as the reference's comparison does.
 __device__ __forceinline__ double pcg_next_active(const PcgScalars* s, double rr) {
-    const double res = s->bb > 0.0 ? sqrt(rr) / sqrt(s->bb) : 0.0;
-    return (res > s->tol && s->it < s->it_limit) ? 1.0 : 0.0;
+    const double res = s->bb == 0.0 ? 0.0 : sqrt(rr) / sqrt(s->bb);
+    return (!(res <= s->tol) && s->it < s->it_limit) ? 1.0 : 0.0;
 }
 
 // r = b - A x (FP64 operator); partial sums of b.b and r.r.

@@ -2770,6 +2770,8 @@ fem::Options options_from(const fem_options& opt) {
     o.crossover_flops = opt.crossover_flops;
     o.crossover_entries = opt.crossover_entries;
     o.crossover_flat = opt.crossover_flat != 0;
+    o.direct_solver = opt.direct_solver;
+    if (o.direct_solver < FEM_DIRECT_AUTO || o.direct_solver > FEM_DIRECT_MULTIFRONTAL) throw fem::Error("unknown direct solver");
     if (o.precision < 0 || o.precision > 2) throw fem::Error("unknown precision");
     if (o.crossover_kind != FEM_CROSSOVER_UNKNOWNS && o.crossover_kind != FEM_CROSSOVER_FLOPS)
         throw fem::Error("unknown crossover");
@@ -2812,6 +2814,7 @@ void fem_options_default(fem_options* o) {
     o->crossover_flops = d.crossover_flops;
     o->crossover_entries = d.crossover_entries;
     o->crossover_flat = d.crossover_flat ? 1 : 0;
+    o->direct_solver = d.direct_solver;
 }
 
 int fem_solver_new_box(const fem_box* box, const fem_options* opt, fem_solver** out, char* err, size_t errlen) {

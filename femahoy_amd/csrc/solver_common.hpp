@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cmath>
 #include <string>
 #include <utility>
 
@@ -63,6 +64,7 @@ struct Options {
     double crossover_flops = 1.5e5;               // elastic.rs:263 DIRECT_FLOPS
     uint64_t crossover_entries = 250000000;       // elastic.rs:269 DIRECT_ENTRIES
     bool crossover_flat = false;
+    int direct_solver = FEM_DIRECT_AUTO;
 };
 
 // The smoothing steps (Chebyshev degree, pre and post) of level l of a box hierarchy. An explicit
@@ -79,6 +81,8 @@ inline int level_degree(int degree, size_t l, bool vector) {
 
 // Rust's {:.2e} (multigrid.rs:730-733): 2 decimals, the exponent without sign padding or zeros.
 inline std::string rust_e2(double v) {
+    if (std::isnan(v)) return "NaN";
+    if (std::isinf(v)) return v > 0 ? "inf" : "-inf";
     char buf[64];
     std::snprintf(buf, sizeof buf, "%.2e", v);
     std::string t(buf);

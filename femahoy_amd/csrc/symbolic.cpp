@@ -293,5 +293,83 @@ Counts analyse(const csr::Csr& a, const std::vector<uint64_t>& order) {
     return out;
 }
 
+Supernodal supernodal(const csr::Csr& a, const std::vector<uint64_t>& order) {
+    const uint64_t n = a.n;
+    if (order.size() != n) throw Error("the order must hold each column once");
+    Supernodal S;
+    S.n = n;
+    S.counts = analyse(a, order);
+    const std::vector<uint64_t> iorder = inverse(order);
+    const std::vector<uint64_t> parent0 = etree(a, order, iorder);
+    const std::vector<uint64_t> post = postorder(parent0);
+    S.perm.resize(n);
+    for (uint64_t k = 0; k < n; ++k) S.perm[k] = order[post[k]];
+    S.iperm = inverse(S.perm);
+    const std::vector<uint64_t> ipost = inverse(post);
+    std::vector<uint64_t> parent(n);
+    for (uint64_t k = 0; k < n; ++k) parent[k] = parent0[post[k]] == NONE ? NONE : ipost[parent0[post[k]]];
+    const std::vector<uint64_t> counts = column_counts(a, S.perm, S.iperm, parent);
+    std::vector<uint64_t> nchild(n, 0);
+    for (uint64_t p : parent)
+        if (p != NONE) nchild[p] += 1;
+    // the fundamental supernodes of analyse (symbolic.rs:73-80)
+    S.start.push_back(0);
+    for (uint64_t j = 1; j < n; ++j) {
+        const bool joins = parent[j - 1] == j && counts[j - 1] == counts[j] + 1 && nchild[j] == 1;
+        if (!joins) S.start.push_back(j);
+    }
+    if (n > 0) S.start.push_back(n);
+    const uint64_t ns = S.supernodes();
+    std::vector<uint64_t> sup_of(n);
+    for (uint64_t s = 0; s < ns; ++s)
+        for (uint64_t j = S.start[s]; j < S.start[s + 1]; ++j) sup_of[j] = s;
+    S.sparent.assign(ns, -1);
+    std::vector<std::vector<uint64_t>> kids(ns);
+    for (uint64_t s = 0; s < ns; ++s) {
+        const uint64_t p = parent[S.start[s + 1] - 1];
+        if (p != NONE) {
+            S.sparent[s] = (int64_t)sup_of[p];
+            kids[sup_of[p]].push_back(s);
+        }
+    }
+    // the rows of each supernode: its columns' entries of A at or below its first column, and its
+    // children's rows past their own columns (all at or below its first column)
+    S.rptr.assign(ns + 1, 0);
+    std::vector<uint64_t> mark(n, NONE), list;
+    for (uint64_t s = 0; s < ns; ++s) {
+        const uint64_t first = S.start[s], end = S.start[s + 1];
+        list.clear();
+        for (uint64_t j = first; j < end; ++j) {
+            mark[j] = s;
+            list.push_back(j);
+        }
+        for (uint64_t j = first; j < end; ++j) {
+            const uint64_t old = S.perm[j];
+            for (uint64_t e = a.ptr[old]; e < a.ptr[old + 1]; ++e) {
+                const uint64_t i = S.iperm[a.idx[e]];
+                if (i >= first && mark[i] != s) {
+                    mark[i] = s;
+                    list.push_back(i);
+                }
+            }
+        }
+        for (uint64_t c : kids[s]) {
+            const uint64_t kc = S.start[c + 1] - S.start[c];
+            for (uint64_t t = S.rptr[c] + kc; t < S.rptr[c + 1]; ++t) {
+                const uint64_t i = S.rows[t];
+                if (mark[i] != s) {
+                    mark[i] = s;
+                    list.push_back(i);
+                }
+            }
+        }
+        std::sort(list.begin(), list.end());
+        if (list.size() != counts[first]) throw Error("internal: a supernode's rows differ from its column count");
+        S.rows.insert(S.rows.end(), list.begin(), list.end());
+        S.rptr[s + 1] = S.rows.size();
+    }
+    return S;
+}
+
 }  // namespace sym
 }  // namespace fem

@@ -27,6 +27,23 @@ struct Counts {
 };
 Counts analyse(const csr::Csr& a, const std::vector<uint64_t>& order);
 
+// The structure of the supernodal factor that the multifrontal factor fills (symbolic.rs:41-150):
+// the fill-reducing order with its elimination-tree postorder (perm: new -> old), the fundamental
+// supernodes (columns start[s] .. start[s + 1] - 1 in the new order), the rows of each (rows[rptr[s]
+// .. rptr[s + 1]), new indices, increasing; the first start[s + 1] - start[s] are its own columns),
+// the supernode tree (sparent, -1 at a root; a supernode's number is above its children's), and
+// the counts of `analyse`.
+struct Supernodal {
+    uint64_t n = 0;
+    std::vector<uint64_t> perm, iperm;
+    std::vector<uint64_t> start;
+    std::vector<uint64_t> rptr, rows;
+    std::vector<int64_t> sparent;
+    Counts counts;
+    uint64_t supernodes() const { return start.empty() ? 0 : start.size() - 1; }
+};
+Supernodal supernodal(const csr::Csr& a, const std::vector<uint64_t>& order);
+
 // Crossover::Flops (solve.rs:113-117): direct when the factor has at most `entries` entries and
 // (`flat`, or its flops are at most `flops` per unknown).
 inline bool flops_picks_direct(const Counts& c, uint64_t n, double flops, uint64_t entries, bool flat) {

@@ -97,7 +97,14 @@ typedef struct fem_options {
                                    (elastic.rs:263 DIRECT_FLOPS = 1.5e5; infinite: the entries decide) */
     uint64_t crossover_entries; /* FLOPS: the most factor entries (elastic.rs:269 DIRECT_ENTRIES = 2.5e8) */
     int32_t crossover_flat;     /* FLOPS: 1 when the cells are flat (direct up to crossover_entries) */
+    int32_t direct_solver;      /* enum fem_direct: the factor of the direct method of fem_solver_new_csr */
 } fem_options;
+
+/* The factor of the direct method (fem_solver_new_csr). The reference's is its multifrontal sparse
+ * Cholesky (cholesky.rs:44-160) in the nested-dissection order (order.rs:19-53). AUTO: the dense
+ * LDL^T up to 8000 unknowns, the multifrontal LDL^T above (and where the unknowns have no
+ * coordinates, the dense one up to 40000). */
+enum fem_direct { FEM_DIRECT_AUTO = 0, FEM_DIRECT_DENSE = 1, FEM_DIRECT_MULTIFRONTAL = 2 };
 
 /* solve.rs:81-86 Stats first ({iterations, residual}: the recurrence residual of PCG, the true
  * residual of the direct solver), then what the GPU solve adds. */

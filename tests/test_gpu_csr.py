@@ -253,12 +253,15 @@ def test_a_direct_solve_of_about_34k_unknowns_matches_oracle_direct(fem, oracle)
     assert rel(x, xd) < 1e-10
 
 
-def test_a_direct_solve_beyond_the_limit_is_refused(fem, oracle):
+def test_a_dense_direct_solve_beyond_its_limit_is_refused(fem, oracle):
+    """The dense factor holds 40,000 unknowns; asked for beyond that, it refuses with the counts
+    (the automatic choice takes the multifrontal factor there, test_gpu_multifrontal.py)."""
     n, length = [34, 34, 34], [1.0, 1.0, 1.0]
     pr, planes, pts = heat(oracle, n, length, 22)
     ptr, idx, val = pr.reduced().arrays()
     with pytest.raises(fem.FemError, match="^the direct solve holds at most 40000 unknowns; this matrix has 40425$"):
-        fem.CsrSolver(ptr, idx, val, "grid", fem.Grid(planes, pts, np.zeros(len(pts), np.uint8)), method="direct")
+        fem.CsrSolver(ptr, idx, val, "grid", fem.Grid(planes, pts, np.zeros(len(pts), np.uint8)), method="direct",
+                      direct_solver="dense")
 
 
 def _oracle_counts(oracle, a, xyz):

@@ -129,3 +129,24 @@ def test_the_automatic_smoother_degree_is_two_for_elasticity_and_three_then_one_
         print(deg, st)
         assert rel(xs[deg], xd) < 1e-8
     assert not np.array_equal(xs[0], xs[2]) and not np.array_equal(xs[0], xs[3])
+
+
+@pytest.mark.parametrize("boundary", ["box", "assembled"])
+def test_a_nan_right_side_runs_to_the_limit_as_the_reference(fem, oracle, boundary):
+    """A NaN in the right side makes every residual NaN. The reference's loop goes on while the
+    residual is not at or below tol (multigrid.rs:721-733), so it runs to max_iter and fails with the
+    residual printed as Rust prints a NaN."""
+    box, k = heat_box(fem, [16, 16, 16])
+    if boundary == "box":
+        s = fem.Solver(box, method="multigrid", levels=3)
+    else:
+        pr = oracle.Problem.heat([16, 16, 16], [1.0, 1.0, 1.0], k, fixed={"x0": 0.0, "x1": 1.0})
+        ptr, idx, val = pr.reduced().arrays()
+        planes = [np.arange(17, dtype=np.float64) / 16.0 for _ in range(3)]
+        pts = pr.ijk()[pr.free()]
+        s = fem.CsrSolver(ptr, idx, val, "grid", fem.Grid(planes, pts, np.zeros(len(pts), np.uint8)), method="multigrid")
+    b = s.rhs() if boundary == "box" else pr.rhs()
+    b[3] = np.nan
+    s.with_max_iter(4)
+    with pytest.raises(fem.FemError, match="^conjugate gradients did not converge: the residual is NaN after 4 iterations$"):
+        s.solve(b)
