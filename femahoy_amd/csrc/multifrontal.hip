@@ -10,8 +10,9 @@
 //  3. add each child's update matrix (the Schur complement left in the child's front), child slot
 //     by child slot, so that no entry takes two additions at once and every sum has a fixed order;
 //  4. factor the k pivots: a front of at most 64 rows in one workgroup, its rows in registers (the
-//     diagonal-block kernel of the dense factor with the pivot count known only at run time);
-//     a larger front by the dense factor's blocked steps (ldlt_panel: the diagonal block, the block
+//     diagonal-block kernel of the dense factor with the pivot count known only at run time); a
+//     taller front of at most 48 pivots by a panel workgroup and one tiled update of the rows past
+//     its pivots; a front of more pivots by the dense factor's blocked steps (ldlt_panel: the diagonal block, the block
 //     column and the trailing update on the FP64 matrix cores), with its pivots padded to a
 //     multiple of 64 by unit pivots that couple to nothing, and its other rows after them, so the
 //     trailing blocks end as the update matrix.
@@ -32,6 +33,7 @@ namespace fem {
 namespace {
 
 constexpr int MB = 64;           // fronts of at most 64 rows are factored in one workgroup
+constexpr int KMED = 48;         // other fronts of at most 48 pivots: a panel workgroup and tiled updates
 constexpr double PIVOT = 1e-10;  // cholesky.rs:22
 constexpr uint32_t CHUNK = 4096; // items per block of the segmented launches
 
@@ -153,6 +155,85 @@ __global__ void __launch_bounds__(256) k_mf_small(const uint32_t* list, const Fr
     if (tid == 0 && bad) atomicMin(status, (unsigned long long)(f.first + bad));
 }
 
+// The k pivots of a front of more than 64 rows and at most KMED pivots (row-major N = m, pivots
+// first), one workgroup: right-looking over the panel of its k columns only (the rows below take
+// the updates of the panel columns), L scaled into the panel, D on its diagonal. The rows past
+// the pivots take their Schur update afterwards, in one pass (k_mf_update).
+__global__ void __launch_bounds__(256) k_mf_panel(const uint32_t* list, const FrontDesc* fronts, const double* dA,
+                                                   double* F, double* Dv, unsigned long long* status) {
+    const FrontDesc f = fronts[list[blockIdx.x]];
+    double* P = F + f.off;
+    const uint32_t N = f.N;
+    int bad = 0;
+    for (uint32_t j = 0; j < f.k; ++j) {
+        double d = P[(uint64_t)j * N + j];
+        const bool ok = d > PIVOT * fabs(dA[f.first + j]);
+        bad = bad == 0 && !ok ? (int)j + 1 : bad;
+        d = ok ? d : 1.0;
+        const double inv = recip64(d);
+        // rows below the pivot: the panel columns past j, up to the row's own
+        for (uint32_t i = j + 1 + threadIdx.x; i < f.m; i += 256) {
+            double* row = P + (uint64_t)i * N;
+            const double a = row[j] * inv;
+            const uint32_t cend = i < f.k - 1 ? i : f.k - 1;
+            for (uint32_t c = j + 1; c <= cend; ++c) row[c] = fma(-a, P[(uint64_t)c * N + j], row[c]);
+        }
+        __syncthreads();
+        for (uint32_t i = j + 1 + threadIdx.x; i < f.m; i += 256) P[(uint64_t)i * N + j] *= inv;
+        if (threadIdx.x == 0) {
+            P[(uint64_t)j * N + j] = d;
+            Dv[f.doff + j] = d;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && bad) atomicMin(status, (unsigned long long)(f.first + bad));
+}
+
+// The Schur update of a panel front's rows past its pivots, one 64 x 64 tile (rows I, columns C
+// of the rows past the pivots, C <= I) per workgroup: S_ic -= sum over j of L_ij d_j L_cj, the L
+// rows of the tile staged in LDS, 4 x 4 entries per thread, j in increasing order.
+__global__ void __launch_bounds__(256) k_mf_update(const Task* tasks, const FrontDesc* fronts, const double* Dv,
+                                                    double* F) {
+    __shared__ double Li[KMED][MB + 1];
+    __shared__ double Lc[KMED][MB + 1];
+    const Task tk = tasks[blockIdx.x];
+    const FrontDesc f = fronts[tk.front];
+    double* P = F + f.off;
+    const uint32_t N = f.N, k = f.k, mu = f.m - f.k;
+    const uint32_t I0 = tk.start * MB, C0 = tk.count * MB;  // tile rows and columns (rows past the pivots)
+    for (uint32_t e = threadIdx.x; e < k * MB; e += 256) {
+        const uint32_t j = e / MB, r = e % MB;
+        const uint32_t ri = I0 + r, rc = C0 + r;
+        Li[j][r] = ri < mu ? P[(uint64_t)(k + ri) * N + j] : 0.0;
+        Lc[j][r] = rc < mu ? P[(uint64_t)(k + rc) * N + j] * Dv[f.doff + j] : 0.0;
+    }
+    __syncthreads();
+    const uint32_t r0 = (threadIdx.x / 16) * 4, c0 = (threadIdx.x % 16) * 4;
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    for (uint32_t j = 0; j < k; ++j) {
+        double li[4], lc[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) li[a] = Li[j][r0 + a];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) lc[b] = Lc[j][c0 + b];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = fma(li[a], lc[b], acc[a][b]);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const uint32_t i = I0 + r0 + a, c = C0 + c0 + b;
+            if (i < mu && c <= i) P[(uint64_t)(k + i) * N + (k + c)] -= acc[a][b];
+        }
+}
+
 // Forward solve of the fronts of a level: the front's vector takes y at its pivots and 0 below,
 // then its children's update vectors in child order; the unit lower L of its pivots runs over
 // it, and its pivots go back to y (z = L^-1 b at those columns). What stays below the pivots is
@@ -233,12 +314,12 @@ struct Multifrontal::Impl {
     sym::Counts counts;
     uint64_t ns = 0, nlevels = 0, pool = 0;
     // per level: its fronts (small and large), its tasks and scatter entries
-    std::vector<uint64_t> lv_small, lv_all;      // offsets into small_list / all_list (nlevels + 1)
-    std::vector<uint32_t> all_list, small_list;  // host copies
+    std::vector<uint64_t> lv_small, lv_med, lv_all;       // offsets into the lists (nlevels + 1)
+    std::vector<uint32_t> all_list, small_list, med_list;  // host copies
     std::vector<std::vector<uint32_t>> lv_large;
     std::vector<FrontDesc> fr;
     DArray<FrontDesc> d_fr;
-    DArray<uint32_t> d_all, d_small, d_kids, d_rel, d_rows;
+    DArray<uint32_t> d_all, d_small, d_med, d_kids, d_rel, d_rows;
     DArray<uint64_t> d_perm;
     DArray<double> F, Dv, w, y;
 
@@ -268,6 +349,9 @@ void Multifrontal::Impl::build(const csr::Csr& a, const std::vector<std::array<d
         if (f.m <= (uint32_t)MB) {
             f.N = MB;
             f.tpos = f.k;
+        } else if (f.k <= (uint32_t)KMED) {  // a panel front: rows in order, no padding
+            f.N = f.m;
+            f.tpos = f.k;
         } else {
             const uint32_t kp = (f.k + MB - 1) / MB * MB, mp = (f.m - f.k + MB - 1) / MB * MB;
             f.N = kp + mp;
@@ -276,7 +360,7 @@ void Multifrontal::Impl::build(const csr::Csr& a, const std::vector<std::array<d
         f.off = off;
         off += (uint64_t)f.N * f.N;
         f.doff = doff;
-        doff += f.m <= (uint32_t)MB ? (uint64_t)MB : (uint64_t)f.tpos;
+        doff += f.m <= (uint32_t)MB ? (uint64_t)MB : (uint64_t)f.tpos;  // a panel front: tpos = k
         f.woff = woff;
         woff += f.N;
         f.reloff = reloff;
@@ -311,20 +395,24 @@ void Multifrontal::Impl::build(const csr::Csr& a, const std::vector<std::array<d
     for (uint64_t q = 0; q < ns; ++q) by_level[level[q]].push_back((uint32_t)q);
     lv_all.assign(nlevels + 1, 0);
     lv_small.assign(nlevels + 1, 0);
+    lv_med.assign(nlevels + 1, 0);
     lv_large.assign(nlevels, {});
     for (uint64_t l = 0; l < nlevels; ++l) {
         for (uint32_t q : by_level[l]) {
             all_list.push_back(q);
             if (fr[q].m <= (uint32_t)MB) small_list.push_back(q);
+            else if (fr[q].k <= (uint32_t)KMED) med_list.push_back(q);
             else lv_large[l].push_back(q);
         }
         lv_all[l + 1] = all_list.size();
         lv_small[l + 1] = small_list.size();
+        lv_med[l + 1] = med_list.size();
     }
     // device structures
     d_fr = upload(fr, s);
     d_all = upload(all_list, s);
     d_small = upload(small_list, s);
+    d_med = upload(med_list, s);
     d_kids = upload(kidlist, s);
     d_rel = upload(rel, s);
     std::vector<uint32_t> rows32(S.rows.begin(), S.rows.end());
@@ -343,7 +431,7 @@ void Multifrontal::Impl::build(const csr::Csr& a, const std::vector<std::array<d
     // workspace of the large fronts' blocked steps
     uint32_t maxN = 0, maxkp = 0;
     for (const FrontDesc& f : fr)
-        if (f.m > (uint32_t)MB) {
+        if (f.m > (uint32_t)MB && f.k > (uint32_t)KMED) {
             maxN = std::max(maxN, f.N);
             maxkp = std::max(maxkp, f.tpos);
         }
@@ -413,7 +501,19 @@ void Multifrontal::Impl::build(const csr::Csr& a, const std::vector<std::array<d
             ext_at[l].push_back(ext_tasks.size());
         }
     }
-    DArray<Task> d_init = upload(init_tasks, s), d_ext = upload(ext_tasks, s);
+    // the tiles of the panel fronts' Schur updates, level by level (lower-triangle tile pairs)
+    std::vector<Task> upd_tasks;
+    std::vector<uint64_t> upd_at(nlevels + 1, 0);
+    for (uint64_t l = 0; l < nlevels; ++l) {
+        for (uint64_t i = lv_med[l]; i < lv_med[l + 1]; ++i) {
+            const FrontDesc& f = fr[med_list[i]];
+            const uint32_t tiles = (f.m - f.k + MB - 1) / MB;
+            for (uint32_t I = 0; I < tiles; ++I)
+                for (uint32_t C = 0; C <= I; ++C) upd_tasks.push_back({med_list[i], I, C, 0});
+        }
+        upd_at[l + 1] = upd_tasks.size();
+    }
+    DArray<Task> d_init = upload(init_tasks, s), d_ext = upload(ext_tasks, s), d_upd = upload(upd_tasks, s);
     DArray<uint64_t> d_dst = upload(dst, s), d_src = upload(src, s);
     size_t large_index = 0;
     std::vector<uint32_t> large_order;
@@ -442,6 +542,16 @@ void Multifrontal::Impl::build(const csr::Csr& a, const std::vector<std::array<d
             hipLaunchKernelGGL(k_mf_small, dim3((unsigned)nsmall), dim3(256), 0, s,
                                (const uint32_t*)d_small.data() + lv_small[l], (const FrontDesc*)d_fr.data(),
                                (const double*)d_dA.data(), F.data(), Dv.data(), sstat.data());
+        const uint64_t nmed = lv_med[l + 1] - lv_med[l];
+        if (nmed) {
+            hipLaunchKernelGGL(k_mf_panel, dim3((unsigned)nmed), dim3(256), 0, s,
+                               (const uint32_t*)d_med.data() + lv_med[l], (const FrontDesc*)d_fr.data(),
+                               (const double*)d_dA.data(), F.data(), Dv.data(), sstat.data());
+            const uint64_t nu = upd_at[l + 1] - upd_at[l];
+            if (nu)
+                hipLaunchKernelGGL(k_mf_update, dim3((unsigned)nu), dim3(256), 0, s, (const Task*)d_upd.data() + upd_at[l],
+                                   (const FrontDesc*)d_fr.data(), (const double*)Dv.data(), F.data());
+        }
         FEM_HIP(hipGetLastError());
         for (uint32_t q : lv_large[l]) {
             const FrontDesc& f = fr[q];
