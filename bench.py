@@ -581,6 +581,43 @@ def dense_factor_leg(F, hip, n, seed=5):
             "note": "host time of fem_dense_solve_device (factor, explicit inverse, refined solve), best of 3"}
 
 
+def multifrontal_leg(F, hip, n):
+    """The direct method beyond the dense factor: the multifrontal LDL^T (fem_solver_new_csr with
+    the direct method) of the heat hex8 box of n^3 cells (the field law of the headline), its matrix
+    exported by fem_solver_matrix. Reports the setup (nested dissection, symbolic analysis, the
+    fronts' factor on the device), the solve, the true residual, and the factor's counts (the
+    library's host symbolic analysis)."""
+    box, _ = heat_box(F, n)
+    sb = F.Solver(box, method="multigrid")
+    ptr, idx, val = sb.matrix()
+    pg = F.Model(box).point_grid()[sb.free_list()]
+    planes = [np.arange(n + 1, dtype=np.float64) / n for _ in range(3)]
+    g = F.Grid(planes, pg.astype(np.uint32), np.zeros(len(pg), np.uint8))
+    b = sb.rhs()
+    del sb
+    xyz = np.stack([planes[d][pg[:, d]] for d in range(3)], axis=1)
+    t0 = time.perf_counter()
+    _, entries, flops, largest, supernodes = F.symbolic(ptr, idx, xyz)
+    t_sym = time.perf_counter() - t0
+    hip.sync()
+    t0 = time.perf_counter()
+    s = F.CsrSolver(ptr, idx, val, "grid", g, method="direct", direct_solver="multifrontal")
+    hip.sync()
+    t_setup = time.perf_counter() - t0
+    sts = [s.solve(b)[1] for _ in range(3)]
+    st = min(sts, key=lambda q: q.seconds)
+    log(f"multifrontal heat {n}^3 ({len(b)} unknowns): setup {t_setup:.3f} s (host symbolic {t_sym:.3f} s), "
+        f"solve {st.seconds * 1e3:.2f} ms, residual {st.true_residual:.1e}")
+    return {"workload": f"heat hex8 {n}^3 box matrix (exported by fem_solver_matrix), direct method, multifrontal LDL^T",
+            "unknowns": int(len(b)), "factor_entries": int(entries), "factor_flops": float(flops),
+            "supernodes": int(supernodes), "largest_front_entries": int(largest),
+            "setup_s": round(t_setup, 4), "host_symbolic_s": round(t_sym, 4),
+            "solve_ms": round(st.seconds * 1e3, 3), "true_residual": float(st.true_residual),
+            "factor_tflops_incl_host": round(flops / t_setup / 1e12, 4),
+            "note": "setup is the host time of fem_solver_new_csr (order, symbolic analysis, level lists, the "
+                    "device factor); solve is the device time of one solve (best of 3)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -612,6 +649,9 @@ def main():
     ap.add_argument("--dense", type=str, default="4335,20000",
                     help="sizes of the dense LDL^T factor timed alone (comma-separated; empty: skip)")
     ap.add_argument("--dense-only", action="store_true", help="time the dense factors of --dense and stop")
+    ap.add_argument("--multifrontal", type=str, default="34,50",
+                    help="cells per axis of the heat boxes whose direct solve by the multifrontal factor is timed "
+                         "(comma-separated; empty: skip)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run one plain 256 MiB device copy first (the yardstick of HBM byte counters)")
     args = ap.parse_args()
@@ -638,6 +678,7 @@ def main():
     if args.dense_only:
         print(json.dumps({"dense_factor": dense}), flush=True)
         return
+    multifrontal = [multifrontal_leg(F, hip, int(v)) for v in args.multifrontal.split(",") if v.strip()]
 
     # the heat configuration names Chebyshev(3)-Jacobi; the elasticity ones name no degree
     hdeg = 3 if args.degree is None else args.degree
@@ -771,6 +812,8 @@ def main():
         config["serving"] = serve
     if dense:
         config["dense_factor"] = dense
+    if multifrontal:
+        config["direct_multifrontal"] = multifrontal
     out = {
         "metric": METRIC,
         "value": value,

@@ -235,56 +235,80 @@ __global__ void __launch_bounds__(256) k_mf_update(const Task* tasks, const Fron
 }
 
 // Forward solve of the fronts of a level: the front's vector takes y at its pivots and 0 below,
-// then its children's update vectors in child order; the unit lower L of its pivots runs over
-// it, and its pivots go back to y (z = L^-1 b at those columns). What stays below the pivots is
-// the front's update vector for its parent.
+// then its children's update vectors in child order; the unit lower L of its pivots runs over it
+// in blocks of 64 pivots (the first wave solves a block with lane shuffles, then every thread
+// takes rows below it: one 64-term sum per row), and its pivots go back to y (z = L^-1 b at those
+// columns). What stays below the pivots is the front's update vector for its parent.
 __global__ void __launch_bounds__(256) k_mf_forward(const uint32_t* list, const FrontDesc* fronts, const uint32_t* kids,
                                                      const uint32_t* rel, const double* F, double* w, double* y) {
     const FrontDesc f = fronts[list[blockIdx.x]];
     double* wf = w + f.woff;
-    for (uint32_t t = threadIdx.x; t < f.m; t += 256) wf[pos_of(f, t)] = t < f.k ? y[f.first + t] : 0.0;
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    for (uint32_t t = tid; t < f.m; t += 256) wf[pos_of(f, t)] = t < f.k ? y[f.first + t] : 0.0;
     __syncthreads();
     for (uint32_t q = 0; q < f.nkids; ++q) {
         const FrontDesc c = fronts[kids[f.kid0 + q]];
         const uint32_t* r = rel + c.reloff;
         const double* wc = w + c.woff + c.tpos;
-        for (uint32_t i = threadIdx.x; i < c.m - c.k; i += 256) wf[r[i]] += wc[i];
+        for (uint32_t i = tid; i < c.m - c.k; i += 256) wf[r[i]] += wc[i];
         __syncthreads();
     }
     const double* L = F + f.off;
-    for (uint32_t j = 0; j < f.k; ++j) {
-        const double v = wf[j];
-        for (uint32_t t = j + 1 + threadIdx.x; t < f.m; t += 256) {
+    for (uint32_t j0 = 0; j0 < f.k; j0 += 64) {
+        const uint32_t jn = f.k - j0 < 64 ? f.k - j0 : 64;
+        if (tid < 64) {
+            double wl = lane < jn ? wf[j0 + lane] : 0.0;
+            for (uint32_t t = 0; t < jn; ++t) {
+                const double v = __shfl(wl, (int)t, 64);
+                if (lane > t && lane < jn) wl = fma(-L[(uint64_t)(j0 + lane) * f.N + j0 + t], v, wl);
+            }
+            if (lane < jn) wf[j0 + lane] = wl;
+        }
+        __syncthreads();
+        for (uint32_t t = j0 + jn + tid; t < f.m; t += 256) {
             const uint32_t p = pos_of(f, t);
-            wf[p] -= L[(uint64_t)p * f.N + j] * v;
+            const double* lr = L + (uint64_t)p * f.N + j0;
+            double sum = 0.0;
+            for (uint32_t u = 0; u < jn; ++u) sum = fma(lr[u], wf[j0 + u], sum);
+            wf[p] -= sum;
         }
         __syncthreads();
     }
-    for (uint32_t t = threadIdx.x; t < f.k; t += 256) y[f.first + t] = wf[t];
+    for (uint32_t t = tid; t < f.k; t += 256) y[f.first + t] = wf[t];
 }
 
-// Backward solve of the fronts of a level, from the last pivot: x_j = z_j / d_j - sum over the
-// rows t > j of L_tj x_t, the rows past the pivots read from y (their fronts, higher in the tree,
-// are done). The sum over t is split over the threads in a fixed pattern and met by a fixed tree.
+// Backward solve of the fronts of a level, in blocks of 64 pivots from the last: each block's
+// sums over the rows below it (x_t of the pivots past the block and of the rows past the pivots,
+// whose fronts, higher in the tree, are done) run over the four waves in a fixed split, met in a
+// fixed order; then the first wave solves the block's L^T with lane shuffles:
+// x_j = z_j / d_j - sum over t > j of L_tj x_t.
 __global__ void __launch_bounds__(256) k_mf_backward(const uint32_t* list, const FrontDesc* fronts,
                                                       const uint32_t* rows, const double* F, const double* Dv, double* y) {
-    __shared__ double red[256];
+    __shared__ double red[4][64];
     const FrontDesc f = fronts[list[blockIdx.x]];
     const double* L = F + f.off;
     const uint32_t* rw = rows + f.roff;
-    for (int j = (int)f.k - 1; j >= 0; --j) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (f.k == 0) return;
+    for (int64_t jb = (int64_t)((f.k - 1) / 64) * 64; jb >= 0; jb -= 64) {
+        const uint32_t j0 = (uint32_t)jb, jn = f.k - j0 < 64 ? f.k - j0 : 64;
         double s = 0.0;
-        for (uint32_t t = (uint32_t)j + 1 + threadIdx.x; t < f.m; t += 256) {
-            const double xt = t < f.k ? y[f.first + t] : y[rw[t]];
-            s += L[(uint64_t)pos_of(f, t) * f.N + j] * xt;
-        }
-        red[threadIdx.x] = s;
+        if (lane < jn)
+            for (uint32_t t = j0 + jn + wave; t < f.m; t += 4) {
+                const double xt = t < f.k ? y[f.first + t] : y[rw[t]];
+                s = fma(L[(uint64_t)pos_of(f, t) * f.N + j0 + lane], xt, s);
+            }
+        red[wave][lane] = s;
         __syncthreads();
-        for (int h = 128; h > 0; h >>= 1) {
-            if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-            __syncthreads();
+        if (tid < 64) {
+            const double below = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+            double xv = lane < jn ? y[f.first + j0 + lane] / Dv[f.doff + j0 + lane] - below : 0.0;
+            for (int t = (int)jn - 1; t >= 0; --t) {
+                const double xt = __shfl(xv, t, 64);
+                if ((int)lane < t) xv = fma(-L[(uint64_t)(j0 + t) * f.N + j0 + lane], xt, xv);
+            }
+            if (lane < jn) y[f.first + j0 + lane] = xv;
         }
-        if (threadIdx.x == 0) y[f.first + j] = y[f.first + j] / Dv[f.doff + j] - red[0];
         __syncthreads();
     }
 }
