@@ -27,6 +27,7 @@
 #include "hex_elastic.hpp"
 #include "kernels.hpp"
 #include "model.hpp"
+#include "multifrontal.hpp"
 #include "march.hpp"
 #include "tet10_fused.hpp"
 #include "tet10_kuhn.hpp"
@@ -641,16 +642,33 @@ class Solver {
             else direct = nfree <= o.crossover;
         }
         if (direct) {
-            if (nfree > DENSE_DIRECT_LIMIT) {
-                std::string why;
-                if (analysed_)
-                    why = " (the crossover chose it: the factor has " + std::to_string(counts_.entries) + " entries and " +
-                          rust_e(counts_.flops) + " flops; ask for multigrid)";
-                throw Error("the direct solver on the device holds up to " + std::to_string(DENSE_DIRECT_LIMIT) +
-                            " unknowns; this model has " + std::to_string(nfree) + why);
+            const bool multifrontal = o.direct_solver == FEM_DIRECT_MULTIFRONTAL ||
+                                      (o.direct_solver == FEM_DIRECT_AUTO && nfree > MULTIFRONTAL_FROM);
+            if (multifrontal) {
+                // the reduced matrix (the reference's pattern, values probed from the FP64 operator)
+                // to the multifrontal factor, ordered by the free unknowns' points
+                csr::Csr a;
+                a.n = nfree;
+                a.ptr.resize(nfree + 1);
+                const uint64_t nnz = export_matrix(nullptr, nullptr, nullptr);
+                a.idx.resize(nnz);
+                a.val.resize(nnz);
+                export_matrix(a.ptr.data(), a.idx.data(), a.val.data());
+                size_t free_b = 0, total_b = 0;
+                FEM_HIP(hipMemGetInfo(&free_b, &total_b));
+                mf_ = std::make_unique<Multifrontal>(a, free_xyz(), s_, (uint64_t)(0.8 * (double)free_b));
+            } else {
+                if (nfree > DENSE_DIRECT_LIMIT) {
+                    std::string why;
+                    if (analysed_)
+                        why = " (the crossover chose it: the factor has " + std::to_string(counts_.entries) + " entries and " +
+                              rust_e(counts_.flops) + " flops; ask for multigrid)";
+                    throw Error("the direct solver on the device holds up to " + std::to_string(DENSE_DIRECT_LIMIT) +
+                                " unknowns; this model has " + std::to_string(nfree) + why);
+                }
+                direct_ = std::make_unique<DenseCoarse>();
+                assemble_and_factor(lv64_[0], *direct_, false);
             }
-            direct_ = std::make_unique<DenseCoarse>();
-            assemble_and_factor(lv64_[0], *direct_, false);
             method_ = FEM_METHOD_DIRECT;
         } else {
             method_ = FEM_METHOD_MULTIGRID;
@@ -1015,11 +1033,21 @@ class Solver {
         if (method_ == FEM_METHOD_DIRECT) {
             try {
                 // x = A^-1 b, then one step of refinement on the FP64 residual: x += A^-1 (b - A x)
-                direct_->solve<double>(b_.data(), x_.data(), s_);
+                // (the multifrontal factor works on the free unknowns: gathered, solved, scattered)
+                auto dsolve = [&](const double* rhs, double* out) {
+                    if (!mf_) {
+                        direct_->solve<double>(rhs, out, s_);
+                        return;
+                    }
+                    launch(k_gather, nf, s_, nf, (const int64_t*)free_grid_.data(), rhs, fb_.data());
+                    mf_->solve(fb_.data(), fx_.data(), s_);
+                    launch(k_scatter, nf, s_, nf, (const int64_t*)free_grid_.data(), (const double*)fx_.data(), out);
+                };
+                dsolve(b_.data(), x_.data());
                 apply64(x_.data(), tmp_.data());
                 launch(k_axpby, n0_, s_, n0_, 1.0, (const double*)b_.data(), -1.0, tmp_.data());
                 launch(k_fill<double>, n0_, s_, n0_, z_.data(), 0.0);  // the solve writes the free rows only
-                direct_->solve<double>(tmp_.data(), z_.data(), s_);
+                dsolve(tmp_.data(), z_.data());
                 launch(k_axpby, n0_, s_, n0_, 1.0, (const double*)z_.data(), 1.0, x_.data());
                 const double res = true_residual();
                 run_.out.iterations = 0;
@@ -1301,6 +1329,7 @@ class Solver {
     std::unique_ptr<Hierarchy<float>> h32_;
     std::unique_ptr<Hierarchy<double>> h64_;
     std::unique_ptr<DenseCoarse> direct_;
+    std::unique_ptr<Multifrontal> mf_;  // the direct method's factor above MULTIFRONTAL_FROM unknowns
     Profiler prof_;
     bool use_graph_ = true;
     int max_batch_ = 8;

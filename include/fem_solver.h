@@ -252,9 +252,13 @@ typedef struct fem_unknowns {
  * reference's (multigrid.rs:84-139: coarsen, 329-452, and Galerkin products, sparse.rs:233-292,
  * down to 3000 unknowns; `levels` > 0 stops earlier); its smoother is Chebyshev-Jacobi and its
  * coarsest level the dense LDL^T (up to coarse_max unknowns; a QUADRATIC linear level with no grid
- * below it up to 40000). The direct solver is the dense LDL^T, up to 40000 unknowns (the
- * reference's TRANSIENT_DIRECT_LIMIT, solve.rs:24); when the crossover picks it for a larger matrix
- * the call fails with a message that gives the factor's counts. With `cells` (GRID unknowns of a
+ * below it up to 40000). The direct solver (fem_options.direct_solver) is the dense LDL^T up to
+ * 8000 unknowns and the multifrontal LDL^T above: the nested-dissection order and fundamental
+ * supernodes of the reference's factor (order.rs:19-53, symbolic.rs:41-150, cholesky.rs:44-160),
+ * its fronts factored on the device level by level of the supernode tree, its pivot failure in the
+ * reference's text (cholesky.rs:92-99); it needs the unknowns' coordinates (GRID, or `xyz`), and
+ * without them the dense LDL^T takes up to 40000 unknowns and refuses more with the factor's counts.
+ * With `cells` (GRID unknowns of a
  * displacement) the coarse matrices are those of by_cells (multigrid.rs:458-671: each coarse cell's
  * energy on its trilinear corner fields and incompatible modes, condensed), as the reference's
  * box-grid elasticity caller asks (elastic.rs:537-547). b and x of fem_solver_solve are the matrix's

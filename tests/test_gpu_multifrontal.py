@@ -94,3 +94,20 @@ def test_a_singular_matrix_fails_as_the_reference_fails(fem, oracle):
         fem.CsrSolver(ptr, idx, val, "grid", grid_of(fem, planes, pts), method="direct", direct_solver="multifrontal")
     print(str(want.value), "|", str(got.value))
     assert str(got.value) == str(want.value)
+
+
+def test_a_box_model_forced_direct_beyond_the_dense_limit(fem, oracle):
+    """The box boundary's direct method (fem_solver_new_box with FEM_METHOD_DIRECT) above 8,000
+    unknowns factors the model's reduced matrix (its pattern, the values probed from the FP64
+    operator) by the multifrontal LDL^T: 47,915 heat unknowns, beyond the dense 40,000."""
+    n, length = [36, 36, 36], [1.0, 1.0, 1.0]
+    k = np.exp(np.random.default_rng(45).standard_normal(int(np.prod(n))))
+    box = fem.Box("heat_hex8", n, length, k, fixed={"x0": 0.0, "x1": 1.0})
+    s = fem.Solver(box, method="direct")
+    assert s.n == 47915 and s.method() == "direct"
+    x, st = s.solve(s.rhs())
+    print(st, flush=True)
+    assert st.true_residual < 1e-12
+    pr = oracle.Problem.heat(n, length, k, fixed={"x0": 0.0, "x1": 1.0})
+    xd, _, _ = oracle.Solver(pr, "direct").solve(pr.rhs())
+    assert rel(x, xd) < 1e-10
