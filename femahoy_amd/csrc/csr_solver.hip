@@ -942,4 +942,28 @@ int fem_test_symbolic(const fem_csr* a, const double* xyz, uint64_t* order_out, 
     }
 }
 
+int fem_test_supernodal(const fem_csr* a, const double* xyz, uint64_t* nsup, uint64_t* nrows, uint64_t* perm,
+                        uint64_t* start, uint64_t* rptr, uint64_t* rows, int64_t* sparent, char* err, size_t errlen) {
+    try {
+        fem::csr::Csr m = fem::csr_from(a);
+        fem::csr::validate(m);
+        if (!xyz) throw fem::Error("the coordinates are needed");
+        std::vector<std::array<double, 3>> p(m.n);
+        for (uint64_t k = 0; k < m.n; ++k)
+            for (int d = 0; d < 3; ++d) p[k][d] = xyz[3 * k + d];
+        const fem::sym::Supernodal S = fem::sym::supernodal(m, fem::sym::nested_dissection(m, p));
+        if (nsup) *nsup = S.supernodes();
+        if (nrows) *nrows = S.rows.size();
+        if (perm) std::copy(S.perm.begin(), S.perm.end(), perm);
+        if (start) std::copy(S.start.begin(), S.start.end(), start);
+        if (rptr) std::copy(S.rptr.begin(), S.rptr.end(), rptr);
+        if (rows) std::copy(S.rows.begin(), S.rows.end(), rows);
+        if (sparent) std::copy(S.sparent.begin(), S.sparent.end(), sparent);
+        return 0;
+    } catch (const std::exception& e) {
+        put_error(err, errlen, e.what());
+        return 1;
+    }
+}
+
 }  // extern "C"

@@ -66,6 +66,14 @@ int fem_test_symbolic(const fem_csr* a, const double* xyz, uint64_t* order_out, 
 int fem_test_assembly_pattern(const fem_mesh* mesh, uint64_t* free_count, uint64_t* entries, uint64_t* ptr,
                               uint32_t* idx, int64_t* free_list, char* err, size_t errlen);
 
+/* The structure of the multifrontal factor of a square pattern (symbolic.rs:41-150 in the nested
+ * dissection order of the points xyz, 3 per unknown): the supernode and row counts first (call
+ * with NULL arrays to size them), then the order (perm: n, new -> old), the supernodes' first
+ * columns (start: supernodes + 1), their rows (rptr: supernodes + 1, rows: the factor's order,
+ * each supernode's own columns first) and their parents (-1 at a root). No device is touched. */
+int fem_test_supernodal(const fem_csr* a, const double* xyz, uint64_t* nsup, uint64_t* nrows, uint64_t* perm,
+                        uint64_t* start, uint64_t* rptr, uint64_t* rows, int64_t* sparent, char* err, size_t errlen);
+
 /* The condensed incompatible-mode hex of a box cell with sides size[3] and an isotropic material
  * (hex.rs:199-279 Hex::new's k: 24 x 24 row-major, unknown 3 corner + component, corners in VTK
  * order), by the library's host restatement that FEM_ELASTIC_HEX8 boxes assemble. No device. */

@@ -110,6 +110,14 @@ void orc_symbolic_perm(void* h, int64_t* out) {
     const Symbolic* s = static_cast<Symbolic*>(h);
     for (size_t k = 0; k < s->n; ++k) out[k] = (int64_t)s->perm[k];
 }
+size_t orc_symbolic_rows_len(void* h) { return static_cast<Symbolic*>(h)->rows.size(); }
+void orc_symbolic_structure(void* h, int64_t* start, int64_t* rows_ptr, uint32_t* rows, int64_t* parent) {
+    const Symbolic* s = static_cast<Symbolic*>(h);
+    for (size_t k = 0; k < s->start.size(); ++k) start[k] = (int64_t)s->start[k];
+    for (size_t k = 0; k < s->rows_ptr.size(); ++k) rows_ptr[k] = (int64_t)s->rows_ptr[k];
+    for (size_t k = 0; k < s->rows.size(); ++k) rows[k] = s->rows[k];
+    for (size_t k = 0; k < s->parent.size(); ++k) parent[k] = s->parent[k] == (usize)-1 ? -1 : (int64_t)s->parent[k];
+}
 void orc_symbolic_free(void* h) { delete static_cast<Symbolic*>(h); }
 void* orc_cholesky(void* sym, void* a, char* err, size_t errlen) {
     try {

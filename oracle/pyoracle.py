@@ -69,6 +69,9 @@ def lib() -> C.CDLL:
         "orc_symbolic_largest_front": (C.c_uint64, [_P]),
         "orc_symbolic_perm": (None, [_P, _I64P]),
         "orc_symbolic_free": (None, [_P]),
+        "orc_symbolic_rows_len": (_SZ, [_P]),
+        "orc_symbolic_structure": (None, [_P, _I64P, _I64P, np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS"),
+                                          _I64P]),
         "orc_cholesky": (_P, [_P, _P, C.c_char_p, _SZ]),
         "orc_cholesky_solve": (None, [_P, _F64P]),
         "orc_cholesky_entries": (_SZ, [_P]),
@@ -316,6 +319,18 @@ class Symbolic:
 
     def largest_front(self):
         return lib().orc_symbolic_largest_front(self.h)
+
+    def structure(self):
+        """(perm, start, rows_ptr, rows, parent) of the supernodal factor (symbolic.rs:41-150);
+        parent -1 at a root."""
+        ns = self.supernodes()
+        n = int(lib().orc_symbolic_rows_len(self.h))
+        start, rptr = np.zeros(ns + 1, np.int64), np.zeros(ns + 1, np.int64)
+        rows, parent = np.zeros(max(n, 1), np.uint32), np.zeros(max(ns, 1), np.int64)
+        lib().orc_symbolic_structure(self.h, start, rptr, rows, parent)
+        perm = np.zeros(int(start[-1]), np.int64)
+        lib().orc_symbolic_perm(self.h, perm)
+        return perm, start, rptr, rows[:n], parent[:ns]
 
 
 class Cholesky:

@@ -122,3 +122,41 @@ def test_grid_indices_give_the_order_of_the_coordinates(fem, oracle):
     warped = np.stack([np.round(xyz[:, 0] * 11 / 1.3), np.exp(xyz[:, 1]), xyz[:, 2] ** 3], axis=1)
     o2 = fem.symbolic(ptr.astype(np.uint64), idx, warped)[0]
     assert np.array_equal(o1, o2)
+
+
+@pytest.mark.parametrize("case", ["heat", "elastic_hex", "tet10"])
+def test_the_multifrontal_structure_is_the_oracles(fem, oracle, case):
+    """The supernodes of the multifrontal factor and their rows (symbolic.rs:41-150: the postordered
+    nested-dissection order, the fundamental supernodes, each one's own columns then its rows below
+    them, the supernode tree) equal the oracle's, integer for integer."""
+    rng = np.random.default_rng(51)
+    if case == "heat":
+        n = [9, 7, 6]
+        pr = oracle.Problem.heat(n, [1.0, 0.8, 0.6], np.exp(rng.standard_normal(int(np.prod(n)))),
+                                 fixed={"x0": 0.0, "x1": 1.0})
+    elif case == "elastic_hex":
+        n = [8, 4, 3]
+        nc = int(np.prod(n))
+        pr = oracle.Problem.elastic_hex(n, [4.0, 2.0, 1.5], rng.uniform(1, 2, nc), rng.uniform(0.2, 0.4, nc),
+                                        clamped=["x0"])
+    else:
+        n = [4, 3, 2]
+        nt = 6 * int(np.prod(n))
+        pr = oracle.Problem.elastic(10, n, [4.0, 3.0, 2.0], rng.uniform(1, 2, nt), rng.uniform(0.2, 0.4, nt),
+                                    clamped=["x0"])
+    a = pr.reduced()
+    ptr, idx, _ = a.arrays()
+    comps = 1 if case == "heat" else 3
+    xyz = pr.points()[pr.free() // comps]
+    perm, start, rptr, rows, sparent = fem.supernodal(ptr, idx, xyz)
+    operm, ostart, orptr, orows, oparent = oracle.Symbolic(a, oracle.nested_dissection(a, xyz)).structure()
+    assert np.array_equal(perm, operm)
+    assert np.array_equal(start, ostart)
+    assert np.array_equal(rptr, orptr)
+    assert np.array_equal(rows, orows.astype(np.int64))
+    assert np.array_equal(sparent, oparent)
+    # each supernode's rows start with its own columns and increase
+    for s in range(len(start) - 1):
+        r = rows[rptr[s]:rptr[s + 1]]
+        k = start[s + 1] - start[s]
+        assert np.array_equal(r[:k], np.arange(start[s], start[s + 1])) and np.all(np.diff(r) > 0)
