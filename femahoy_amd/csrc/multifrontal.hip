@@ -292,13 +292,27 @@ __global__ void __launch_bounds__(256) k_mf_backward(const uint32_t* list, const
     if (f.k == 0) return;
     for (int64_t jb = (int64_t)((f.k - 1) / 64) * 64; jb >= 0; jb -= 64) {
         const uint32_t j0 = (uint32_t)jb, jn = f.k - j0 < 64 ? f.k - j0 : 64;
-        double s = 0.0;
-        if (lane < jn)
-            for (uint32_t t = j0 + jn + wave; t < f.m; t += 4) {
-                const double xt = t < f.k ? y[f.first + t] : y[rw[t]];
-                s = fma(L[(uint64_t)pos_of(f, t) * f.N + j0 + lane], xt, s);
+        // four rows in flight per wave (independent partial sums, met in a fixed order)
+        double s4[4] = {0.0, 0.0, 0.0, 0.0};
+        if (lane < jn) {
+            uint32_t t = j0 + jn + wave;
+            for (; t + 12 < f.m; t += 16) {
+                double xt[4], lv[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t tq = t + 4 * q;
+                    xt[q] = tq < f.k ? y[f.first + tq] : y[rw[tq]];
+                    lv[q] = L[(uint64_t)pos_of(f, tq) * f.N + j0 + lane];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) s4[q] = fma(lv[q], xt[q], s4[q]);
             }
-        red[wave][lane] = s;
+            for (; t < f.m; t += 4) {
+                const double xt = t < f.k ? y[f.first + t] : y[rw[t]];
+                s4[0] = fma(L[(uint64_t)pos_of(f, t) * f.N + j0 + lane], xt, s4[0]);
+            }
+        }
+        red[wave][lane] = (s4[0] + s4[1]) + (s4[2] + s4[3]);
         __syncthreads();
         if (tid < 64) {
             const double below = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
