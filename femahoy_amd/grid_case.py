@@ -1,20 +1,22 @@
-"""The reference's case files (`grid.toml`) for steady heat conduction and linear static stress in
-box parts, read and solved with this library on the device.
+"""The reference's case files (`grid.toml`) for steady and transient heat conduction and linear
+static stress in box parts, read and solved with this library on the device.
 
 The reader follows crates/core/src/grid.rs (`read`, `read_grid`, `read_parts`, `read_faces`, the
 loads and probes of `read_heat`) for the keys listed below; the model is built as
 crates/fem/src/grid.rs builds the graded grid (`axis_planes`, `place`, `assign`, `faces`, `crop`,
 `locate`), as heat.rs:1483-1597 `build_grid` applies symmetry, heat.rs:1165-1208 `number`
-numbers the points, and as heat.rs:328-700 (steady) and elastic.rs:287-563 (static) assemble and
+numbers the points, and as heat.rs:328-700 (steady), heat.rs:684-735 (transient: backward Euler
+with the lumped capacity, one solver for every step) and elastic.rs:287-563 (static) assemble and
 solve. The matrices and right sides are assembled on the device (`fem_assemble`), the solve is the
 library's (`fem_solver_new_csr` with the grid level, and the cells for a displacement).
 
 Keys read: `units`; `[grid]` max_cell, max_growth, min_cells, symmetry; `[material.*]`
-conductivity (a number) or youngs_modulus and poisson_ratio (numbers); `[part.*]` material, x, y,
-z, priority; `[face.*]` side, box, parts; `[[load]]` temperature, source, flux, convection with
-sink (steady), ux / uy / uz, traction, pressure (static); `[step]` type "steady" or "static";
-`[probe.*]` point. Anything else (tables, thermal strain, orthotropic materials, footprints,
-radiation, transient and other steps) is refused with a message.
+conductivity (a number; with density and specific_heat for a transient step) or youngs_modulus
+and poisson_ratio (numbers); `[part.*]` material, x, y, z, priority; `[face.*]` side, box, parts;
+`[[load]]` temperature, source, flux, convection with sink (heat), ux / uy / uz, traction, pressure
+(static); `[step]` type "steady", "transient" (end, dt, initial) or "static"; `[probe.*]` point.
+Anything else (tables, thermal strain, orthotropic materials, footprints, radiation and other
+steps) is refused with a message.
 """
 from __future__ import annotations
 
@@ -37,6 +39,7 @@ SIDES = [(0, False), (0, True), (1, False), (1, True), (2, False), (2, True)]
 SIDE_NAMES = {"x-": 0, "x+": 1, "y-": 2, "y+": 3, "z-": 4, "z+": 5}
 VTK = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
 STEADY_DIRECT_LIMIT = 3000     # solve.rs:18
+TRANSIENT_DIRECT_LIMIT = 40000  # solve.rs:24
 DIRECT_FLOPS = 1.5e5           # elastic.rs:263
 DIRECT_ENTRIES = 250_000_000   # elastic.rs:269
 MULTIGRID_ASPECT = 2.0         # elastic.rs:276
@@ -66,7 +69,7 @@ class FaceSet:
 @dataclass
 class Case:
     units: str
-    step: str                       # "steady" or "static"
+    step: str                       # "steady", "transient" or "static"
     max_cell: float
     max_growth: float
     min_cells: int
@@ -77,6 +80,7 @@ class Case:
     loads: list                     # dicts: kind and its values
     probes: list                    # (name, point)
     name: str = ""
+    transient: Optional[dict] = None  # step "transient": end, dt, initial (heat.rs Step::Transient)
 
 
 # ------------------------------------------------------------------------------- reading
@@ -122,16 +126,28 @@ def read(text: str, name: str = "") -> Case:
             raise CaseError("[grid]: `symmetry` must list axes x, y or z")
         symmetry["xyz".index(a)] = True
     step = top.get("step", {})
-    _only(step, "[step]", ["type"])
     kind = step.get("type")
-    if kind not in ("steady", "static"):
-        raise CaseError(f"[step]: type {kind!r}: this front end solves steady heat and static stress")
+    transient = None
+    if kind == "transient":
+        _only(step, "[step]", ["type", "end", "dt", "initial", "output_every"])
+        transient = {"end": _number(step, "end", "[step]"), "dt": _number(step, "dt", "[step]"),
+                     "initial": _number(step, "initial", "[step]")}
+        if not (transient["dt"] > 0.0 and transient["end"] > 0.0):
+            raise CaseError("[step]: `end` and `dt` must be numbers above 0")
+    else:
+        _only(step, "[step]", ["type"])
+    if kind not in ("steady", "static", "transient"):
+        raise CaseError(f"[step]: type {kind!r}: this front end solves steady and transient heat and static stress")
     materials, mindex = [], {}
     for mname, m in top.get("material", {}).items():
         where = f"[material.{mname}]"
         if kind == "steady":
             _only(m, where, ["conductivity"])
             materials.append({"name": mname, "conductivity": _number(m, "conductivity", where)})
+        elif kind == "transient":
+            _only(m, where, ["conductivity", "density", "specific_heat"])
+            materials.append({"name": mname, "conductivity": _number(m, "conductivity", where),
+                              "capacity": _number(m, "density", where) * _number(m, "specific_heat", where)})
         else:
             _only(m, where, ["youngs_modulus", "poisson_ratio", "density"])
             materials.append({"name": mname, "youngs_modulus": _number(m, "youngs_modulus", where),
@@ -182,7 +198,7 @@ def read(text: str, name: str = "") -> Case:
             if l.get("face") not in findex:
                 raise CaseError(f"{where}: no face set {l.get('face')}")
             return findex[l["face"]]
-        if kind == "steady":
+        if kind in ("steady", "transient"):
             if "temperature" in l:
                 _only(l, where, ["temperature", "face"])
                 loads.append({"kind": "temperature", "faces": face_of(), "value": _number(l, "temperature", where)})
@@ -220,7 +236,8 @@ def read(text: str, name: str = "") -> Case:
         where = f"[probe.{pname}]"
         _only(p, where, ["point"])
         probes.append((pname, _numbers(p.get("point"), 3, where, "point")))
-    return Case(units, kind, max_cell, max_growth, min_cells, symmetry, materials, parts, faces, loads, probes, name)
+    return Case(units, kind, max_cell, max_growth, min_cells, symmetry, materials, parts, faces, loads, probes, name,
+                transient)
 
 
 def load_case(path: str) -> Case:
@@ -616,7 +633,7 @@ def solve(case: Case, precision: str = "fp32", method: Optional[str] = None, tol
           device: int = 0) -> Result:
     """The case solved on the device: steady heat (heat.rs:328-700) or static stress
     (elastic.rs:287-563)."""
-    if case.step == "steady":
+    if case.step in ("steady", "transient"):
         return _solve_heat(case, precision, method, tol, device)
     return _solve_elastic(case, precision, method, tol, device)
 
@@ -699,11 +716,34 @@ def _solve_heat(case, precision, method, tol_, device):
                     rhs[rp] -= hm * value[nodes[q]]
     planes = [np.asarray(p, np.float64) for p in grid.planes]
     g = fem.Grid(planes, ijk[free].astype(np.uint32), np.zeros(len(free), np.uint8))
-    s = fem.CsrSolver(ptr, idx, val, "grid", g, method=method, precision=precision, tol=tol_,
-                      crossover=STEADY_DIRECT_LIMIT, device=device)
-    x, st = s.solve(rhs)
-    t = value.copy()
-    t[free] = x
+    if case.transient is None:
+        s = fem.CsrSolver(ptr, idx, val, "grid", g, method=method, precision=precision, tol=tol_,
+                          crossover=STEADY_DIRECT_LIMIT, device=device)
+        x, st = s.solve(rhs)
+        t = value.copy()
+        t[free] = x
+    else:
+        # heat.rs:684-735: the lumped capacity (rho c V / 8 to each corner, cell by cell) over dt on
+        # the diagonal, one solver for every step, each step's right side the load, the capacity
+        # term of the last temperatures and the fixed points' couplings, its start the last solution
+        dt, steps = case.transient["dt"], int(round(case.transient["end"] / case.transient["dt"]))
+        capacity = np.zeros(n)
+        for e, c in enumerate(cell_of):
+            sz = grid.size(c)
+            vol = sz[0] * sz[1] * sz[2]
+            q = case.materials[case.parts[int(grid.part[c])].material]["capacity"]
+            for node in cells[e]:
+                capacity[node] += q * vol / 8.0
+        for f_, i in enumerate(free):
+            val[_entry(ptr, idx, f_, f_)] += capacity[i] / dt
+        s = fem.CsrSolver(ptr, idx, val, "grid", g, method=method, precision=precision, tol=tol_,
+                          crossover=TRANSIENT_DIRECT_LIMIT, device=device)
+        t = np.where(isfixed.astype(bool), value, case.transient["initial"])
+        x = t[free].copy()
+        cf = capacity[free] / dt
+        for _ in range(steps):
+            x, st = s.solve(rhs + cf * t[free], x=x)
+            t[free] = x
     probes = {}
     for name, pairs, _ in probe_weights(case, grid, mids, number_of):
         probes[name] = sum(t[v] * w for v, w in pairs)

@@ -45,3 +45,30 @@ def test_reference_case_passes_its_check(fem, case_dir, precision):
     for (c, column), (value, half) in PRINTED.items():
         if c == folder:
             assert abs(res.probes[column] - value) <= half, (column, res.probes[column], value)
+
+
+# grid-cooling-slab (derivation.md, "Discretization error and tolerance"): the reference's grid solver's
+# centre temperature at t = 20 s for 3 grids and 4 time steps (backward Euler), printed to 5 decimals.
+COOLING = {0.001: (77.16018, 77.13918, 77.12847, 77.12306),
+           0.0005: (77.24674, 77.22512, 77.21410, 77.20853),
+           0.00025: (77.26825, 77.24647, 77.23536, 77.22976)}
+
+
+@pytest.mark.parametrize("max_cell", [0.001, 0.0005, 0.00025])
+def test_transient_cooling_slab_reproduces_the_references_table(fem, max_cell):
+    """A slab at 100 C whose faces go to 0 C (transient heat, heat.rs:684-735: the lumped capacity
+    over dt on the diagonal, one solver for every step, each step started from the last solution):
+    every entry of the reference's table of grids and time steps to half a unit of its last digit,
+    and the case file's own grid and step within its check.toml."""
+    from femahoy_amd import grid_case
+    case = grid_case.load_case(os.path.join(CASES, "grid-cooling-slab/slab/grid.toml"))
+    for dt, printed in zip((0.2, 0.1, 0.05, 0.025), COOLING[max_cell]):
+        case.max_cell = max_cell
+        case.transient["dt"] = dt
+        res = grid_case.solve(case, precision="fp64", tol=1e-12)
+        print(max_cell, dt, res.method, res.unknowns, res.probes["centre"])
+        assert abs(res.probes["centre"] - printed) <= 0.5e-5, (max_cell, dt, res.probes["centre"], printed)
+        if max_cell == 0.0005 and dt == 0.1:
+            with open(os.path.join(CASES, "grid-cooling-slab", "check.toml")) as f:
+                rows = grid_case.check(res, f.read(), case.name)
+            assert rows and all(r[-1] for r in rows)

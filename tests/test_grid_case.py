@@ -44,7 +44,7 @@ def test_the_four_cases_read():
      "`expansion` is not read"),
     ('units = "SI-m"\n[grid]\nmax_cell = 1.0\n[material.a]\nyoungs_modulus = 1.0\npoisson_ratio = 0.3\n'
      '[part.p]\nmaterial = "a"\nx = [0.0, 1.0]\ny = [0.0, 1.0]\nz = [0.0, 1.0]\n[step]\ntype = "modes"\n',
-     "this front end solves steady heat and static stress"),
+     "this front end solves steady and transient heat and static stress"),
     ('units = "SI-cm"\n[grid]\nmax_cell = 1.0\n[step]\ntype = "steady"\n', "`units` must be SI-m or SI-mm"),
     ('units = "SI-m"\n[grid]\nmax_cell = 1.0\n[material.a]\nconductivity = 1.0\n'
      '[part.p]\nmaterial = "b"\nx = [0.0, 1.0]\ny = [0.0, 1.0]\nz = [0.0, 1.0]\n[step]\ntype = "steady"\n',
