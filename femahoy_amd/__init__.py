@@ -161,7 +161,7 @@ class Profile(C.Structure):
 
 
 EXPORTS = (
-    "fem_options_default", "fem_solver_new_box", "fem_solver_solve", "fem_solver_solve_device",
+    "fem_options_default", "fem_hex_element", "fem_solver_new_box", "fem_solver_solve", "fem_solver_solve_device",
     "fem_solver_solve_many", "fem_solver_solve_many_device",
     "fem_solver_method", "fem_solver_set_max_iter", "fem_solver_set_tol", "fem_solver_free",
     "fem_solver_free_count", "fem_solver_unknown_count", "fem_solver_free_list", "fem_solver_point_grid",
@@ -259,6 +259,7 @@ def lib() -> C.CDLL:
         "fem_test_assembly_pattern": (C.c_int, [C.POINTER(_Mesh), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), P, P,
                                                 P, ERR, SZ]),
         "fem_test_hex_element": (C.c_int, [F64P, C.c_double, C.c_double, F64P, ERR, SZ]),
+        "fem_hex_element": (C.c_int, [F64P, C.c_double, C.c_double, C.c_double, F64P, F64P, ERR, SZ]),
         "fem_test_supernodal": (C.c_int, [C.POINTER(_Csr), P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), P, P, P, P,
                                           P, ERR, SZ]),
         "fem_test_hex_box_system": (C.c_int, [C.POINTER(_Box), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), P, P, P,
@@ -957,14 +958,17 @@ def assembly_pattern(mesh: Mesh):
     return ptr, idx[:ne.value], free[:nf.value]
 
 
-def hex_element(size, e: float, nu: float) -> np.ndarray:
-    """The condensed incompatible-mode hex of a box cell (hex.rs:199-279), 24 x 24, by the library's
-    host code (no device)."""
-    out = np.zeros(576)
+def hex_element(size, e: float, nu: float, alpha: Optional[float] = None):
+    """The condensed incompatible-mode hex of a box cell (hex.rs:199-279, fem_hex_element), 24 x 24;
+    with an expansion coefficient `alpha`, also the load of a unit temperature change at each corner
+    (24 x 8): (k, load). Host code, no device."""
+    k = np.zeros(576)
+    load = np.zeros(192)
     err = _err()
-    if lib().fem_test_hex_element(np.ascontiguousarray(size, np.float64), float(e), float(nu), out, err, 2048):
+    if lib().fem_hex_element(np.ascontiguousarray(size, np.float64), float(e), float(nu),
+                             0.0 if alpha is None else float(alpha), k, load, err, 2048):
         raise FemError(err.value.decode())
-    return out.reshape(24, 24)
+    return k.reshape(24, 24) if alpha is None else (k.reshape(24, 24), load.reshape(24, 8))
 
 
 def hex_box_system(box: Box):

@@ -113,8 +113,25 @@ std::array<double, 12> isotropic_stiffness(double e, double nu) {
 }
 
 std::array<double, 576> hex_element(const std::array<double, 3>& size, const std::array<double, 12>& d) {
+    std::array<double, 576> k;
+    std::array<double, 192> load;
+    hex_element_thermal(size, d, 0.0, k, load);
+    return k;
+}
+
+void hex_element_thermal(const std::array<double, 3>& size, const std::array<double, 12>& d, double alpha,
+                         std::array<double, 576>& out, std::array<double, 192>& lout) {
     const double* normal = d.data();
     const double* shear = d.data() + 9;
+    // the thermal stress of a unit temperature change: D alpha on the normal stresses (a sum from 0)
+    double thermal[3];
+    for (int i = 0; i < 3; ++i) {
+        double t = 0.0;
+        for (int j = 0; j < 3; ++j) t += normal[3 * i + j] * alpha;
+        thermal[i] = t;
+    }
+    std::vector<double> lbuf(33 * 8, 0.0);
+    auto Lm = [&](int r, int c) -> double& { return lbuf[(size_t)r * 8 + c]; };
     std::vector<double> kbuf(33 * 33, 0.0);
     auto K = [&](int r, int c) -> double& { return kbuf[(size_t)r * 33 + c]; };
     const double jac = size[0] * size[1] * size[2] / 8.0;
@@ -123,6 +140,12 @@ std::array<double, 576> hex_element(const std::array<double, 3>& size, const std
         const double xi[3] = {CORNER[gp][0] == 1 ? p : -p, CORNER[gp][1] == 1 ? p : -p, CORNER[gp][2] == 1 ? p : -p};
         double g[11][3];
         gradients11(size, xi, g);
+        double nshape[8];  // the corner functions at xi (hex.rs:148-163)
+        for (int c = 0; c < 8; ++c) {
+            double v = 1.0;
+            for (int dd = 0; dd < 3; ++dd) v *= CORNER[c][dd] == 1 ? 0.5 * (1.0 + xi[dd]) : 0.5 * (1.0 - xi[dd]);
+            nshape[c] = v;
+        }
         for (int i = 0; i < 11; ++i) {
             const double* gi = g[i];
             for (int j = 0; j < 11; ++j) {
@@ -142,6 +165,8 @@ std::array<double, 576> hex_element(const std::array<double, 3>& size, const std
                         K(3 * i + a, 3 * j + b) += jac * v;
                     }
             }
+            for (int a = 0; a < 3; ++a)
+                for (int c = 0; c < 8; ++c) Lm(3 * i + a, c) += jac * gi[a] * thermal[a] * nshape[c];
         }
     }
     // the modes condensed: X = K_mm^-1 K_mu, then K_uu - K_um X
@@ -156,14 +181,25 @@ std::array<double, 576> hex_element(const std::array<double, 3>& size, const std
         solve9(l, col);
         for (int i = 0; i < 9; ++i) x[i][c] = col[i];
     }
-    std::array<double, 576> out{};
-    for (int r = 0; r < 24; ++r)
+    double xt[9][8];
+    for (int c = 0; c < 8; ++c) {
+        double col[9];
+        for (int i = 0; i < 9; ++i) col[i] = Lm(24 + i, c);
+        solve9(l, col);
+        for (int i = 0; i < 9; ++i) xt[i][c] = col[i];
+    }
+    for (int r = 0; r < 24; ++r) {
         for (int c = 0; c < 24; ++c) {
             double v = K(r, c);
             for (int i = 0; i < 9; ++i) v -= K(r, 24 + i) * x[i][c];
             out[(size_t)r * 24 + c] = v;
         }
-    return out;
+        for (int c = 0; c < 8; ++c) {
+            double v = Lm(r, c);
+            for (int i = 0; i < 9; ++i) v -= K(r, 24 + i) * xt[i][c];
+            lout[(size_t)r * 8 + c] = v;
+        }
+    }
 }
 
 std::vector<double> box_planes(const BoxModel& m, int d) {

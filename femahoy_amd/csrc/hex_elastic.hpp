@@ -25,6 +25,12 @@ std::array<double, 12> isotropic_stiffness(double e, double nu);
 // Cholesky factor. Row-major, unknown 3 corner + component, corners in the order of a VTK hexahedron.
 std::array<double, 576> hex_element(const std::array<double, 3>& size, const std::array<double, 12>& d);
 
+// hex.rs:199-279 with expansion: the stiffness and the load of a unit temperature change at each
+// corner (24 x 8, row-major), the modes' share condensed with them, for the isotropic expansion
+// coefficient alpha.
+void hex_element_thermal(const std::array<double, 3>& size, const std::array<double, 12>& d, double alpha,
+                         std::array<double, 576>& k, std::array<double, 192>& load);
+
 // The reduced system of a box model of kind ElasticHex8.
 struct HexBoxSystem {
     csr::Csr a;             // Csr::pattern + add_element in cell order, then submatrix(free)

@@ -3208,6 +3208,18 @@ int fem_test_hex_element(const double* size, double e, double nu, double* out, c
     });
 }
 
+int fem_hex_element(const double* size, double e, double nu, double alpha, double* k, double* load, char* err,
+                    size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!size || !k) throw fem::Error("the sizes and the stiffness output are needed");
+        std::array<double, 576> kk;
+        std::array<double, 192> ll;
+        fem::hex_element_thermal({size[0], size[1], size[2]}, fem::isotropic_stiffness(e, nu), alpha, kk, ll);
+        std::copy(kk.begin(), kk.end(), k);
+        if (load) std::copy(ll.begin(), ll.end(), load);
+    });
+}
+
 int fem_test_hex_box_system(const fem_box* box, uint64_t* n, uint64_t* nnz, uint64_t* ptr, uint32_t* idx, double* val,
                             double* rhs, double* aspect, char* err, size_t errlen) {
     return guarded(err, errlen, [&] {

@@ -332,6 +332,15 @@ void fem_assembly_device(const fem_assembly* a, const uint64_t** ptr, const uint
                          const double** rhs);
 void fem_assembly_free(fem_assembly* a);
 
+/* The condensed incompatible-mode hex of an axis-aligned box cell with sides size[3] (hex.rs:199-279
+ * Hex::new), isotropic: its stiffness k (24 x 24) and, for the expansion coefficient alpha, the
+ * load of a unit temperature change at each corner (24 x 8; may be NULL), row-major, unknown
+ * 3 corner + component, corners in VTK order. The element that FEM_ELASTIC_HEX8 boxes and
+ * FEM_MESH_ELASTIC_HEX8 meshes assemble, for callers that build their own cells and thermal loads
+ * (the case-file front end). Material errors use the texts of hex.rs:79 and hex.rs:93. */
+int fem_hex_element(const double* size, double e, double nu, double alpha, double* k, double* load, char* err,
+                    size_t errlen);
+
 /* ---- a dense direct solve on device memory ---- */
 
 /* x = A^-1 b for a dense symmetric positive definite n x n matrix A: the device's dense FP64

@@ -89,3 +89,19 @@ def test_hex_box_materials_are_checked_with_the_references_texts(fem):
 def test_hex_box_aspect_is_the_references_flatness(fem):
     assert fem.box_aspect(fem.Box("elastic_hex8", [4, 4, 1], [4.0, 4.0, 0.25], 1.0, 0.3)) == 4.0
     assert fem.box_aspect(fem.Box("elastic_hex8", [3, 3, 3], [1.0, 1.0, 1.0], 1.0, 0.3)) < 1.0 + 1e-12
+
+
+@pytest.mark.parametrize("size,alpha", [((2.0, 0.5, 0.1), 12e-6), ((1.0, 1.0, 1.0), 1.0), ((0.3, 0.7, 1.9), 2e-5)])
+def test_hex_thermal_load_is_free_expansion(fem, size, alpha):
+    """The load of a unit temperature change (hex.rs:199-279, the modes' share condensed): a free
+    cell under a uniform change dt takes the strain alpha dt on each axis with no stress, so the
+    corner displacements u = alpha dt x solve K u = load dt exactly; a closed form (a sign, a factor
+    or a dropped condensation term fails it). The stiffness is the element's without expansion."""
+    k, load = fem.hex_element(size, 3.0, 0.3, alpha)
+    assert np.array_equal(k, fem.hex_element(size, 3.0, 0.3))
+    bits = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    u = np.array([alpha * b[d] * size[d] for b in bits for d in range(3)])
+    f = load @ np.ones(8)
+    assert np.abs(k @ u - f).max() <= 1e-12 * np.abs(f).max()
+    # and each corner's column is the load of a change at that corner alone: the 8 sum to the free expansion
+    assert np.abs(load.sum(axis=1) - f).max() <= 1e-15 * np.abs(f).max()
