@@ -7,16 +7,20 @@ crates/fem/src/grid.rs builds the graded grid (`axis_planes`, `place`, `assign`,
 `locate`), as heat.rs:1483-1597 `build_grid` applies symmetry, heat.rs:1165-1208 `number`
 numbers the points, and as heat.rs:328-700 (steady), heat.rs:684-735 (transient: backward Euler
 with the lumped capacity, one solver for every step) and elastic.rs:287-563 (static) assemble and
-solve. The matrices and right sides are assembled on the device (`fem_assemble`), the solve is the
-library's (`fem_solver_new_csr` with the grid level, and the cells for a displacement).
+solve. A static step with a temperature (uniform, or the steady heat solve of its heat loads on the
+same grid, elastic.rs:358-363) adds each cell's thermal force to the loads (elastic.rs:383-404);
+its surfaces give their shape (elastic.rs:1099-1250). The matrices and right sides are assembled
+on the device (`fem_assemble`), the solve is the library's (`fem_solver_new_csr` with the grid
+level, and the cells for a displacement).
 
 Keys read: `units`; `[grid]` max_cell, max_growth, min_cells, symmetry; `[material.*]`
-conductivity (a number; with density and specific_heat for a transient step) or youngs_modulus
-and poisson_ratio (numbers); `[part.*]` material, x, y, z, priority; `[face.*]` side, box, parts;
-`[[load]]` temperature, source, flux, convection with sink (heat), ux / uy / uz, traction, pressure
-(static); `[step]` type "steady", "transient" (end, dt, initial) or "static"; `[probe.*]` point.
-Anything else (tables, thermal strain, orthotropic materials, footprints, radiation and other
-steps) is refused with a message.
+conductivity (a number; with density and specific_heat for a transient step) or youngs_modulus,
+poisson_ratio, expansion and conductivity (numbers); `[part.*]` material, x, y, z, priority;
+`[face.*]` side, box, parts; `[[load]]` temperature, source, flux, convection with sink (heat, and
+the temperature of a static step), ux / uy / uz on a face or a point, traction, pressure (static);
+`[step]` type "steady", "transient" (end, dt, initial) or "static" (reference_temperature,
+temperature); `[probe.*]` point; `[surface.*]` face. Anything else (tables, orthotropic
+materials, footprints, radiation, gravity, Joule heat and other steps) is refused with a message.
 """
 from __future__ import annotations
 
@@ -81,6 +85,8 @@ class Case:
     probes: list                    # (name, point)
     name: str = ""
     transient: Optional[dict] = None  # step "transient": end, dt, initial (heat.rs Step::Transient)
+    thermal: Optional[dict] = None    # step "static": {"uniform": T} or {"heat": True}, and "reference"
+    surfaces: list = field(default_factory=list)  # (name, face set) of a static step (elastic.rs:93-97)
 
 
 # ------------------------------------------------------------------------------- reading
@@ -109,7 +115,7 @@ def read(text: str, name: str = "") -> Case:
     """The case of a grid.toml text (core/src/grid.rs:79-152 `read`, :153 `read_heat`), for the
     keys this front end reads."""
     top = _toml.loads(text)
-    _only(top, "the top", ["units", "grid", "material", "part", "face", "load", "step", "probe"])
+    _only(top, "the top", ["units", "grid", "material", "part", "face", "load", "step", "probe", "surface"])
     units = top.get("units")
     if units not in ("SI-m", "SI-mm"):
         raise CaseError(f"`units` must be SI-m or SI-mm, not {units}")
@@ -134,6 +140,8 @@ def read(text: str, name: str = "") -> Case:
                      "initial": _number(step, "initial", "[step]")}
         if not (transient["dt"] > 0.0 and transient["end"] > 0.0):
             raise CaseError("[step]: `end` and `dt` must be numbers above 0")
+    elif kind == "static":
+        _only(step, "[step]", ["type", "reference_temperature", "temperature"])
     else:
         _only(step, "[step]", ["type"])
     if kind not in ("steady", "static", "transient"):
@@ -149,9 +157,11 @@ def read(text: str, name: str = "") -> Case:
             materials.append({"name": mname, "conductivity": _number(m, "conductivity", where),
                               "capacity": _number(m, "density", where) * _number(m, "specific_heat", where)})
         else:
-            _only(m, where, ["youngs_modulus", "poisson_ratio", "density"])
+            _only(m, where, ["youngs_modulus", "poisson_ratio", "density", "expansion", "conductivity"])
             materials.append({"name": mname, "youngs_modulus": _number(m, "youngs_modulus", where),
-                              "poisson_ratio": _number(m, "poisson_ratio", where)})
+                              "poisson_ratio": _number(m, "poisson_ratio", where),
+                              "expansion": _number(m, "expansion", where) if "expansion" in m else 0.0,
+                              "conductivity": _number(m, "conductivity", where) if "conductivity" in m else None})
         mindex[mname] = len(materials) - 1
     parts = []
     for pname, p in top.get("part", {}).items():
@@ -220,9 +230,32 @@ def read(text: str, name: str = "") -> Case:
                 raise CaseError(f"{where}: a steady load here is temperature, source, flux or convection")
         else:
             if any(k in l for k in ("ux", "uy", "uz")):
-                _only(l, where, ["ux", "uy", "uz", "face"])
                 value = [(_number(l, k, where) if k in l else None) for k in ("ux", "uy", "uz")]
-                loads.append({"kind": "displacement", "faces": face_of(), "value": value})
+                if "point" in l:  # elastic.rs:419-423: a point support
+                    _only(l, where, ["ux", "uy", "uz", "point"])
+                    loads.append({"kind": "point", "point": _numbers(l["point"], 3, where, "point"), "value": value})
+                else:
+                    _only(l, where, ["ux", "uy", "uz", "face"])
+                    loads.append({"kind": "displacement", "faces": face_of(), "value": value})
+            elif any(k in l for k in ("temperature", "source", "flux", "convection")):
+                # heat loads of a static step: the temperature of its thermal strain (core grid.rs:814-830)
+                if "temperature" in l:
+                    _only(l, where, ["temperature", "face"])
+                    loads.append({"kind": "temperature", "faces": face_of(), "value": _number(l, "temperature", where),
+                                  "heat": True})
+                elif "source" in l:
+                    _only(l, where, ["source", "part"])
+                    if l.get("part") not in pindex:
+                        raise CaseError(f"{where}: no part {l.get('part')}")
+                    loads.append({"kind": "source", "part": pindex[l["part"]], "value": _number(l, "source", where),
+                                  "heat": True})
+                elif "flux" in l:
+                    _only(l, where, ["flux", "face"])
+                    loads.append({"kind": "flux", "faces": face_of(), "value": _number(l, "flux", where), "heat": True})
+                else:
+                    _only(l, where, ["convection", "sink", "face"])
+                    loads.append({"kind": "convection", "faces": face_of(), "film": _number(l, "convection", where),
+                                  "sink": _number(l, "sink", where), "heat": True})
             elif "traction" in l:
                 _only(l, where, ["traction", "face"])
                 loads.append({"kind": "traction", "faces": face_of(), "value": _numbers(l["traction"], 3, where, "traction")})
@@ -236,8 +269,36 @@ def read(text: str, name: str = "") -> Case:
         where = f"[probe.{pname}]"
         _only(p, where, ["point"])
         probes.append((pname, _numbers(p.get("point"), 3, where, "point")))
+    surfaces = []
+    for sname, sf in top.get("surface", {}).items():
+        where = f"[surface.{sname}]"
+        if kind != "static":
+            raise CaseError(f"{where}: a surface is a result of a static step: [step] type = \"static\"")
+        _only(sf, where, ["face"])
+        if sf.get("face") not in findex:
+            raise CaseError(f"{where}: no face set {sf.get('face')}")
+        surfaces.append((sname, findex[sf["face"]]))
+    thermal = None
+    if kind == "static":
+        heat = [l for l in loads if l.get("heat")]
+        uniform = step.get("temperature")
+        if uniform is not None and heat:
+            raise CaseError("[step]: give `temperature` or heat loads (temperature, flux, convection, emissivity, "
+                            "source), not both")
+        if uniform is not None or heat:
+            if "reference_temperature" not in step:
+                raise CaseError("[step]: missing key `reference_temperature`, the temperature with no thermal strain")
+            thermal = {"reference": _number(step, "reference_temperature", "[step]")}
+            if uniform is not None:
+                thermal["uniform"] = _number(step, "temperature", "[step]")
+            else:
+                thermal["heat"] = True
+                if any(m["conductivity"] is None for m in materials):
+                    raise CaseError("the heat loads of a static step need the conductivity of every material")
+        elif "reference_temperature" in step:
+            raise CaseError("[step]: `reference_temperature` needs `temperature` or heat loads")
     return Case(units, kind, max_cell, max_growth, min_cells, symmetry, materials, parts, faces, loads, probes, name,
-                transient)
+                transient, thermal, surfaces)
 
 
 def load_case(path: str) -> Case:
@@ -750,9 +811,157 @@ def _solve_heat(case, precision, method, tol_, device):
     return Result(probes, t, ijk, grid, s.method(), st, len(free))
 
 
+def grid_point(grid: Grid, number_of, mids, x):
+    """elastic.rs:1410-1444: the point of the grid at a point support (its mirror point above each
+    plane of symmetry), the nearest plane on each axis within 1e-9 of the model's span."""
+    y = list(x)
+    for d in range(3):
+        if mids[d] is not None and y[d] < mids[d]:
+            y[d] = 2.0 * mids[d] - y[d]
+    span = max(grid.planes[d][-1] - grid.planes[d][0] for d in range(3))
+    near = tuple(min(range(len(grid.planes[d])), key=lambda a: abs(grid.planes[d][a] - y[d])) for d in range(3))
+    node = number_of.get(near)
+    if node is None or any(abs(grid.planes[d][near[d]] - y[d]) > 1e-9 * span for d in range(3)):
+        raise CaseError(f"the point support at ({x[0]:e}, {x[1]:e}, {x[2]:e}) is not a point of the grid of the "
+                        "parts. A point of the grid is a corner of a part, or a point where the planes of 2 parts "
+                        "cross")
+    return node
+
+
+def _thermal_change(case, n, precision, tol_, device):
+    """elastic.rs:358-363, 869-906: the temperature change from the reference temperature at each
+    point: a uniform temperature, or the steady heat solve of the heat loads on the same parts
+    (core grid.rs:816-830), or none."""
+    if case.thermal is None:
+        return None
+    if "uniform" in case.thermal:
+        return np.full(n, case.thermal["uniform"] - case.thermal["reference"])
+    import dataclasses
+    heat = dataclasses.replace(case, step="steady", loads=[l for l in case.loads if l.get("heat")], probes=[],
+                               thermal=None, transient=None)
+    t = _solve_heat(heat, precision, None, tol_, device).field
+    if len(t) != n:
+        raise CaseError("the heat model has other parts or another mesh")
+    return t - case.thermal["reference"]
+
+
+def _solve_small(m, r, k):
+    """elastic.rs:1252-1286: the first k rows and columns of m x = r by Gauss-Jordan elimination
+    with partial pivoting; a column with no pivot above 1e-12 of the largest diagonal gives 0."""
+    a = [list(m[i][:k]) + [0.0] * (6 - k) + [r[i]] for i in range(k)]
+    scale = max(abs(m[i][i]) for i in range(k))
+    x = [0.0] * 6
+    pivot_of = [None] * 6
+    row = 0
+    for col in range(k):
+        if row >= k:
+            break
+        p = max(range(row, k), key=lambda i: (abs(a[i][col]), i))
+        if abs(a[p][col]) <= 1e-12 * scale:
+            continue
+        a[row], a[p] = a[p], a[row]
+        for i in range(k):
+            if i != row:
+                f = a[i][col] / a[row][col]
+                for j in range(7):
+                    a[i][j] -= f * a[row][j]
+        pivot_of[col] = row
+        row += 1
+    for col in range(k):
+        if pivot_of[col] is not None:
+            x[col] = a[pivot_of[col]][6] / a[pivot_of[col]][col]
+    return x
+
+
+def surface_shape(grid: Grid, faces, mids, number_of, u, name):
+    """elastic.rs:1099-1250: the shape of a surface from w, the displacement along its outward
+    normal: its faces and their mirrors across the planes of symmetry in its plane, the least
+    squares plane and quadratic of w on each face's 2 x 2 Gauss points, and the range of w and of w
+    minus the plane at the corners. The columns of surfaces.txt (fh grid.rs:846-862)."""
+    side = faces[0][1]
+    if any(f[1] != side for f in faces):
+        raise CaseError(f"surface {name}: its faces must all face 1 direction, for example z+: give its face set "
+                        "a `side`")
+    axis, positive = SIDES[side]
+    sign = 1.0 if positive else -1.0
+    ia, ib = {0: (1, 2), 1: (0, 2), 2: (0, 1)}[axis]
+    quads = []
+    for face in faces:
+        q = []
+        for p in grid.face_points(face):
+            x = [grid.planes[d][p[d]] for d in range(3)]
+            q.append([x[ia], x[ib], sign * u[3 * number_of[p] + axis]])
+        quads.append(q)
+    for k, d in enumerate((ia, ib)):
+        if mids[d] is not None:
+            m = mids[d]
+            copies = []
+            for q in quads:
+                c = [list(v) for v in q]
+                for v in c:
+                    v[k] = 2.0 * m - v[k]
+                copies.append(c)
+            quads.extend(copies)
+    g = 0.5 / math.sqrt(3.0)
+    samples = []
+    for q in quads:
+        da = q[1][0] - q[0][0]
+        db = q[3][1] - q[0][1]
+        weight = abs(da * db) / 4.0
+        for fa, fb in ((0.5 - g, 0.5 - g), (0.5 + g, 0.5 - g), (0.5 + g, 0.5 + g), (0.5 - g, 0.5 + g)):
+            nsh = ((1.0 - fa) * (1.0 - fb), fa * (1.0 - fb), fa * fb, (1.0 - fa) * fb)
+            w = 0.0
+            for k in range(4):
+                w += nsh[k] * q[k][2]
+            samples.append((q[0][0] + fa * da, q[0][1] + fb * db, w, weight))
+    nodes = [v for q in quads for v in q]
+    area = ca = cb = 0.0
+    for a_, b_, _, wt in samples:
+        area += wt
+        ca += wt * a_
+        cb += wt * b_
+    ca, cb = ca / area, cb / area
+    lo = [min(c[k] for c in nodes) for k in range(2)]
+    hi = [max(c[k] for c in nodes) for k in range(2)]
+    half = max(0.5 * max(hi[0] - lo[0], hi[1] - lo[1]), 2.2250738585072014e-308)
+
+    def basis(a_, b_):
+        s_, t_ = (a_ - ca) / half, (b_ - cb) / half
+        return (1.0, s_, t_, s_ * s_, s_ * t_, t_ * t_)
+    mm = [[0.0] * 6 for _ in range(6)]
+    rr = [0.0] * 6
+    for a_, b_, w, wt in samples:
+        phi = basis(a_, b_)
+        for i in range(6):
+            rr[i] += wt * phi[i] * w
+            for j in range(6):
+                mm[i][j] += wt * phi[i] * phi[j]
+    plane = _solve_small(mm, rr, 3)
+    quad = _solve_small(mm, rr, 6)
+    wmin = rmin = math.inf
+    wmax = rmax = -math.inf
+    for c in nodes:
+        phi = basis(c[0], c[1])
+        fit = 0.0
+        for k in range(3):
+            fit += plane[k] * phi[k]
+        wmin, wmax = min(wmin, c[2]), max(wmax, c[2])
+        rmin, rmax = min(rmin, c[2] - fit), max(rmax, c[2] - fit)
+    h2 = half * half
+    curv = (-2.0 * quad[3] / h2, -2.0 * quad[5] / h2, -quad[4] / h2)
+    mean = 0.5 * (curv[0] + curv[1])
+    radius = math.sqrt(0.25 * (curv[0] - curv[1]) ** 2 + curv[2] * curv[2])
+    a_n, b_n = "xyz"[ia], "xyz"[ib]
+    k1, k2 = mean + radius, mean - radius
+    return {f"{name}.range": wmax - wmin, f"{name}.flatness": rmax - rmin, f"{name}.k{a_n}{a_n}": curv[0],
+            f"{name}.k{b_n}{b_n}": curv[1], f"{name}.k{a_n}{b_n}": curv[2], f"{name}.k1": k1, f"{name}.k2": k2,
+            f"{name}.shape": "convex" if k1 + k2 > 0.0 else ("concave" if k1 + k2 < 0.0 else "flat")}
+
+
 def _solve_elastic(case, precision, method, tol_, device):
     import femahoy_amd as fem
-    used = [any(l.get("faces") == f for l in case.loads) for f in range(len(case.faces))]
+    used = [any(l.get("faces") == f for l in case.loads) or any(sf == f for _, sf in case.surfaces)
+            for f in range(len(case.faces))]
     keys = [(0.0, 0.0, 0.0) for _ in case.parts]
     grid, faces, mids = build_grid(case, used, keys)
     ijk, cells, cell_of, number_of = number(grid)
@@ -761,23 +970,47 @@ def _solve_elastic(case, precision, method, tol_, device):
     mat = [case.materials[case.parts[int(grid.part[c])].material] for c in cell_of]
     e_mod = np.array([m["youngs_modulus"] for m in mat])
     nu = np.array([m["poisson_ratio"] for m in mat])
+    # each distinct (cell size, material) once: the element and its thermal load
+    table = {}
+    which = []
+    for c in cell_of:
+        key = (tuple(grid.size(c)), case.parts[int(grid.part[c])].material)
+        if key not in table:
+            m = case.materials[key[1]]
+            table[key] = fem.hex_element(key[0], m["youngs_modulus"], m["poisson_ratio"], m["expansion"])
+        which.append(key)
+    dt = _thermal_change(case, n, precision, tol_, device)
     force = np.zeros(3 * n)
+    if dt is not None:
+        # elastic.rs:383-404: each cell's thermal force, load x the temperature change at its corners,
+        # in cell order before the surface loads
+        for e, key in enumerate(which):
+            f = table[key][1] @ dt[cells[e]]
+            for r in range(24):
+                force[3 * cells[e][r // 3] + r % 3] += f[r]
     fixed = [None] * (3 * n)
+
+    def fix(v, value):
+        for d in range(3):
+            val = value[d]
+            if val is None:
+                continue
+            if fixed[3 * v + d] is not None and fixed[3 * v + d] != val:
+                x = xyz[v]
+                raise CaseError(f"2 fixed displacements give 2 values of u{'xyz'[d]} at the point "
+                                f"({x[0]:e}, {x[1]:e}, {x[2]:e}): {fixed[3 * v + d]} and {val}")
+            fixed[3 * v + d] = val
     for l in case.loads:
+        if l.get("heat"):
+            continue
+        if l["kind"] == "point":
+            fix(grid_point(grid, number_of, mids, l["point"]), l["value"])
+            continue
         for face in faces[l["faces"]]:
             pts = grid.face_points(face)
             if l["kind"] == "displacement":
                 for p in pts:
-                    v = number_of[p]
-                    for d in range(3):
-                        val = l["value"][d]
-                        if val is None:
-                            continue
-                        if fixed[3 * v + d] is not None and fixed[3 * v + d] != val:
-                            x = [grid.planes[a][p[a]] for a in range(3)]
-                            raise CaseError(f"2 fixed displacements give 2 values of u{'xyz'[d]} at the point "
-                                            f"({x[0]:e}, {x[1]:e}, {x[2]:e}): {fixed[3 * v + d]} and {val}")
-                        fixed[3 * v + d] = val
+                    fix(number_of[p], l["value"])
             else:
                 if l["kind"] == "traction":
                     t = list(l["value"])
@@ -812,8 +1045,7 @@ def _solve_elastic(case, precision, method, tol_, device):
         s = grid.size(c)
         aspect = max(aspect, max(s) / min(s))
     origin = np.array([grid.cell_ijk(c) for c in cell_of], np.uint32)
-    element = np.stack([fem.hex_element(grid.size(c), m["youngs_modulus"], m["poisson_ratio"])
-                        for c, m in zip(cell_of, mat)])
+    element = np.stack([table[key][0] for key in which])
     crossover = fem.Flops(flops=DIRECT_FLOPS, entries=DIRECT_ENTRIES, flat=aspect > MULTIGRID_ASPECT)
     s = fem.CsrSolver(ptr, idx, val, "grid", g, method=method, precision=precision, tol=tol_, crossover=crossover,
                       device=device, cells=(origin, element))
@@ -825,6 +1057,8 @@ def _solve_elastic(case, precision, method, tol_, device):
         for d, comp in enumerate(("ux", "uy", "uz")):
             v = sum(u[3 * node + d] * w for node, w in pairs)
             probes[f"{name}.{comp}"] = -v if mirrored[d] else v
+    for name, sf in case.surfaces:
+        probes.update(surface_shape(grid, faces[sf], mids, number_of, u, name))
     return Result(probes, u, ijk, grid, s.method(), st, len(free))
 
 

@@ -9,7 +9,9 @@ The case files are fixtures copied from the reference's validation/cases (grid.t
 check.toml as they are): a heated square bar with quarter symmetry (grid-heated-bar), a hex
 cantilever with incompatible modes (grid-cantilever), a wall of 3 layers with a convection on
 each side on a graded grid (grid-layered-wall), and a steel plate with a heated copper block on
-it, void cells beside the block and half symmetry in x and y (grid-heated-block-ccx).
+it, void cells beside the block and half symmetry in x and y (grid-heated-block-ccx), and the
+same plate and block with the thermal stress of its steady temperature, a point support and the
+shape of the bottom surface (grid-heated-block-stress-ccx).
 """
 import os
 
@@ -24,11 +26,16 @@ CASES = os.path.join(os.path.dirname(__file__), "golden", "cases")
 PRINTED = {
     ("grid-heated-bar", "centre"): (14.745623, 0.5e-6),      # max_cell 0.625 mm
     ("grid-cantilever", "tip.uz"): (-0.190113, 0.5e-6),      # max_cell 2.5 mm
+    # max_cell 1.25 mm, 52,344 unknowns, values in 1e-6 m to 4 decimals
+    ("grid-heated-block-stress-ccx", "block_top.uz"): (12.7391e-6, 0.5e-10),
+    ("grid-heated-block-stress-ccx", "plate_corner.ux"): (10.2845e-6, 0.5e-10),
+    ("grid-heated-block-stress-ccx", "plate_corner.uz"): (-7.2154e-6, 0.5e-10),
+    ("grid-heated-block-stress-ccx", "plate_edge.uz"): (-4.8369e-6, 0.5e-10),
 }
 
 
 @pytest.mark.parametrize("case_dir", ["grid-heated-bar/bar", "grid-cantilever/beam", "grid-layered-wall/wall",
-                                      "grid-heated-block-ccx/block"])
+                                      "grid-heated-block-ccx/block", "grid-heated-block-stress-ccx/block"])
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
 def test_reference_case_passes_its_check(fem, case_dir, precision):
     from femahoy_amd import grid_case
@@ -45,6 +52,12 @@ def test_reference_case_passes_its_check(fem, case_dir, precision):
     for (c, column), (value, half) in PRINTED.items():
         if c == folder:
             assert abs(res.probes[column] - value) <= half, (column, res.probes[column], value)
+    if folder == "grid-heated-block-stress-ccx":
+        # the centre of the bottom holds uz = 0 and the corner is its lowest point: the range of
+        # the bottom surface is -plate_corner.uz, and the plate bends down at its edges (concave)
+        assert res.unknowns == 52344
+        assert abs(res.probes["bottom.range"] + res.probes["plate_corner.uz"]) <= 1e-12 * res.probes["bottom.range"]
+        assert res.probes["bottom.shape"] == "concave"
 
 
 # grid-cooling-slab (derivation.md, "Discretization error and tolerance"): the reference's grid solver's

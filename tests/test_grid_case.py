@@ -125,3 +125,105 @@ def test_probe_at_a_grid_point_reads_that_point():
     # middle_top at x = 51.25 is halfway between planes 20 and 21: 1/2 on each corner of the top edge
     mid = sorted(wt for _, wt in w["middle_top"] if abs(wt) > 1e-12)
     assert np.allclose(mid, [0.5, 0.5], rtol=0, atol=1e-12)
+
+
+def test_static_thermal_case_reads():
+    """grid-heated-block-stress-ccx: heat loads in a static step are the temperature of its thermal
+    strain (core grid.rs:814-850), a point support holds uz at the centre of the bottom, and the
+    bottom face set is a surface whose shape the result gives."""
+    c = case("grid-heated-block-stress-ccx/block")
+    assert c.step == "static" and c.thermal == {"reference": 20.0, "heat": True}
+    assert [(l["kind"], bool(l.get("heat"))) for l in c.loads] == [("convection", True), ("source", True),
+                                                                   ("point", False)]
+    assert c.loads[2]["point"] == [0.05, 0.05, 0.0] and c.loads[2]["value"] == [None, None, 0.0]
+    assert [m["expansion"] for m in c.materials] == [12e-6, 17e-6]
+    assert c.surfaces == [("bottom", 0)]
+    g, _, mids = built(c)
+    ijk, _, _, number_of = gc.number(g)
+    node = gc.grid_point(g, number_of, mids, [0.05, 0.05, 0.0])
+    assert tuple(ijk[node]) == (0, 0, 0)
+    # a point below a plane of symmetry reads its mirror point
+    assert gc.grid_point(g, number_of, mids, [0.0, 0.0, 0.0]) == number_of[(len(g.planes[0]) - 1,
+                                                                             len(g.planes[1]) - 1, 0)]
+    with pytest.raises(gc.CaseError, match="is not a point of the grid of the parts"):
+        gc.grid_point(g, number_of, mids, [0.0501, 0.05, 0.0])
+
+
+BLOCK = os.path.join(CASES, "grid-heated-block-stress-ccx", "block", "grid.toml")
+
+
+@pytest.mark.parametrize("edit,msg", [
+    (("reference_temperature = 20.0\n", ""), "missing key `reference_temperature`"),
+    (('type = "static"\n', 'type = "static"\ntemperature = 80.0\n'), "give `temperature` or heat loads"),
+    (("expansion = 17e-6\n", "expansion = 17e-6\n"), None),
+    (('[surface.bottom]\nface = "bottom"\n', '[surface.bottom]\nface = "top"\n'), "no face set top"),
+    (("conductivity = 400.0\n", ""), "need the conductivity of every material"),
+])
+def test_static_thermal_errors(edit, msg):
+    with open(BLOCK) as f:
+        text = f.read()
+    assert edit[0] in text
+    text = text.replace(edit[0], edit[1])
+    if msg is None:
+        gc.read(text)
+    else:
+        with pytest.raises(gc.CaseError, match=msg):
+            gc.read(text)
+
+
+def test_reference_temperature_needs_a_thermal_load():
+    text = open(os.path.join(CASES, "grid-cantilever", "beam", "grid.toml")).read()
+    text = text.replace('type = "static"', 'type = "static"\nreference_temperature = 20.0')
+    with pytest.raises(gc.CaseError, match="`reference_temperature` needs `temperature` or heat loads"):
+        gc.read(text)
+    uniform = gc.read(text.replace("reference_temperature = 20.0", "reference_temperature = 20.0\ntemperature = 70.0"))
+    assert uniform.thermal == {"reference": 20.0, "uniform": 70.0}
+
+
+def test_solve_small_is_gauss_elimination():
+    """elastic.rs:1252-1286 restated: a regular system equals numpy's solve; a singular one gives
+    0 for the unknowns with no pivot and solves the rest."""
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal((6, 6))
+    m = b @ b.T + 6 * np.eye(6)
+    r = rng.standard_normal(6)
+    for k in (3, 6):
+        x = gc._solve_small(m.tolist(), r.tolist(), k)
+        assert np.allclose(x[:k], np.linalg.solve(m[:k, :k], r[:k]), rtol=1e-12, atol=0)
+        assert x[k:] == [0.0] * (6 - k)
+    # samples on one line a = const: the column of s is zero, its unknown stays 0
+    m2 = np.diag([2.0, 0.0, 3.0, 1.0, 1.0, 1.0])
+    x = gc._solve_small(m2.tolist(), [4.0, 5.0, 6.0, 0, 0, 0], 3)
+    assert x[:3] == [2.0, 0.0, 2.0]
+
+
+def _shape_of(w_of_xy):
+    c = case("grid-heated-block-stress-ccx/block")
+    g, faces, mids = gc.build_grid(c, [True], [(0.0, 0.0, 0.0)] * len(c.parts))
+    ijk, _, _, number_of = gc.number(g)
+    u = np.zeros(3 * len(ijk))
+    for (i, j, k), v in number_of.items():
+        # the bottom face is z-: w = -uz
+        u[3 * v + 2] = -w_of_xy(g.planes[0][i], g.planes[1][j])
+    return gc.surface_shape(g, faces[0], mids, number_of, u, "bottom")
+
+
+def test_surface_shape_of_a_plane_and_a_bowl():
+    """elastic.rs:1099-1250 on the stress block's bottom face (a quarter, mirrored across x and y):
+    w mirrors with the faces, so a field of the quarter stands for an even field of the whole face.
+    A uniform w has no range, flatness or curvature; w = k/2 (x - 0.05)^2 + k/4 (y - 0.05)^2 has the
+    curvatures -k and -k/2 and no twist (to the error of the bilinear faces, h^2 / L^2), and its
+    range is the largest minus the smallest corner value. A quarter's tilted plane w = c (x - 0.05)
+    is the whole face's V, w = c |x - 0.05|: its least squares plane is flat (its mean), so its
+    flatness equals its range."""
+    s = _shape_of(lambda x, y: 1e-6)
+    assert s["bottom.range"] == 0.0 and abs(s["bottom.flatness"]) < 1e-18
+    assert max(abs(s[k]) for k in ("bottom.kxx", "bottom.kyy", "bottom.kxy")) < 1e-12
+    s = _shape_of(lambda x, y: 2e-5 * (x - 0.05))
+    assert abs(s["bottom.range"] - 1e-6) < 1e-18 and abs(s["bottom.flatness"] - 1e-6) < 1e-15
+    k = 3e-3
+    s = _shape_of(lambda x, y: 0.5 * k * (x - 0.05) ** 2 + 0.25 * k * (y - 0.05) ** 2)
+    assert abs(s["bottom.kxx"] + k) < 2e-3 * k and abs(s["bottom.kyy"] + 0.5 * k) < 2e-3 * k
+    assert abs(s["bottom.kxy"]) < 1e-9 * k
+    assert s["bottom.k1"] >= s["bottom.k2"] and s["bottom.shape"] == "concave"
+    assert abs(s["bottom.range"] - (0.5 * k + 0.25 * k) * 0.05 ** 2) < 1e-15
