@@ -36,6 +36,7 @@ constexpr int MB = 64;           // fronts of at most 64 rows are factored in on
 constexpr int KMED = 48;         // other fronts of at most 48 pivots: a panel workgroup and tiled updates
 constexpr double PIVOT = 1e-10;  // cholesky.rs:22
 constexpr uint32_t CHUNK = 4096; // items per block of the segmented launches
+constexpr uint32_t KSPLIT = 64; // fronts of at least this many pivots: the split solve
 
 struct FrontDesc {
     uint64_t off;     // the front: row-major N x N (lower triangle used) at F + off
@@ -56,6 +57,14 @@ struct Task {
 };
 
 __device__ __forceinline__ uint32_t pos_of(const FrontDesc& f, uint32_t t) { return t < f.k ? t : f.tpos + (t - f.k); }
+
+// Lane t's value to every lane of the wave (t the same in every lane): two v_readlane.
+__device__ __forceinline__ double lane_value(double v, uint32_t t) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, (int)t);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), (int)t);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
 
 __global__ void __launch_bounds__(256) k_mf_init(const Task* tasks, const FrontDesc* fronts, double* F) {
     const Task tk = tasks[blockIdx.x];
@@ -259,7 +268,7 @@ __global__ void __launch_bounds__(256) k_mf_forward(const uint32_t* list, const 
         if (tid < 64) {
             double wl = lane < jn ? wf[j0 + lane] : 0.0;
             for (uint32_t t = 0; t < jn; ++t) {
-                const double v = __shfl(wl, (int)t, 64);
+                const double v = lane_value(wl, t);
                 if (lane > t && lane < jn) wl = fma(-L[(uint64_t)(j0 + lane) * f.N + j0 + t], v, wl);
             }
             if (lane < jn) wf[j0 + lane] = wl;
@@ -318,13 +327,155 @@ __global__ void __launch_bounds__(256) k_mf_backward(const uint32_t* list, const
             const double below = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
             double xv = lane < jn ? y[f.first + j0 + lane] / Dv[f.doff + j0 + lane] - below : 0.0;
             for (int t = (int)jn - 1; t >= 0; --t) {
-                const double xt = __shfl(xv, t, 64);
+                const double xt = lane_value(xv, (uint32_t)t);
                 if ((int)lane < t) xv = fma(-L[(uint64_t)(j0 + t) * f.N + j0 + lane], xt, xv);
             }
             if (lane < jn) y[f.first + j0 + lane] = xv;
         }
         __syncthreads();
     }
+}
+
+// The split solve of the fronts with at least KSPLIT pivots: one launch per block of 64 pivots
+// over all such fronts of a level (blockIdx.y), the rows below the block in chunks of 64 rows,
+// one chunk a workgroup (blockIdx.x), so that the top of the tree, where a level holds a few large
+// fronts, streams its factor over many CUs instead of one workgroup a front. The workgroup of the
+// chunk right below a block holds the next block's rows, so it also solves that block within the
+// same launch: no workgroup waits for another. Sums run in a fixed order (a fixed butterfly in the
+// forward steps; the four waves' sums, then the chunks in order, in the backward ones).
+
+// Forward solve of the 64 pivots from j0 of a front's vector wf (its first wave): the unit lower L
+// of the block, z to wf and to y.
+__device__ __forceinline__ void mfs_forward_block(const FrontDesc& f, const double* L, double* wf, double* y,
+                                                  uint32_t j0, uint32_t lane) {
+    const uint32_t jn = f.k - j0 < 64 ? f.k - j0 : 64;
+    double wl = lane < jn ? wf[j0 + lane] : 0.0;
+    for (uint32_t t = 0; t < jn; ++t) {
+        const double v = lane_value(wl, t);
+        if (lane > t && lane < jn) wl = fma(-L[(uint64_t)(j0 + lane) * f.N + j0 + t], v, wl);
+    }
+    if (lane < jn) {
+        wf[j0 + lane] = wl;
+        y[f.first + j0 + lane] = wl;
+    }
+}
+
+// Start of the forward solve of a split front: its vector (y at its pivots, 0 below, then its
+// children's update vectors in child order, as k_mf_forward) and its first block.
+__global__ void __launch_bounds__(256) k_mfs_forward_start(const uint32_t* list, const FrontDesc* fronts,
+                                                           const uint32_t* kids, const uint32_t* rel, const double* F,
+                                                           double* w, double* y) {
+    const FrontDesc f = fronts[list[blockIdx.x]];
+    double* wf = w + f.woff;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t t = tid; t < f.m; t += 256) wf[pos_of(f, t)] = t < f.k ? y[f.first + t] : 0.0;
+    __syncthreads();
+    for (uint32_t q = 0; q < f.nkids; ++q) {
+        const FrontDesc c = fronts[kids[f.kid0 + q]];
+        const uint32_t* r = rel + c.reloff;
+        const double* wc = w + c.woff + c.tpos;
+        for (uint32_t i = tid; i < c.m - c.k; i += 256) wf[r[i]] += wc[i];
+        __syncthreads();
+    }
+    if (tid < 64) mfs_forward_block(f, F + f.off, wf, y, 0, tid);
+}
+
+// Forward step `blk` of the split fronts: the rows of chunk blockIdx.x below the block take
+// L[row, block] . z[block] (lane = row; wave w sums the columns 16 w .. 16 w + 15 of the block,
+// and the four sums meet in a fixed order); the first chunk then solves the next block.
+__global__ void __launch_bounds__(256) k_mfs_forward_step(const uint32_t* list, const FrontDesc* fronts,
+                                                          const double* F, double* w, double* y, uint32_t blk) {
+    __shared__ double zs[64];
+    __shared__ double red[4][64];
+    const FrontDesc f = fronts[list[blockIdx.y]];
+    const uint32_t j0 = blk * 64;
+    if (j0 >= f.k) return;
+    const uint32_t jn = f.k - j0 < 64 ? f.k - j0 : 64;
+    const uint32_t r0 = j0 + jn + blockIdx.x * 64;
+    if (r0 >= f.m) return;
+    const uint32_t rn = f.m - r0 < 64 ? f.m - r0 : 64;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const double* L = F + f.off;
+    double* wf = w + f.woff;
+    if (tid < 64) zs[lane] = lane < jn ? wf[j0 + lane] : 0.0;
+    __syncthreads();
+    const uint32_t p = lane < rn ? pos_of(f, r0 + lane) : 0;
+    double acc = 0.0;
+    if (lane < rn) {
+        const double* lr = L + (uint64_t)p * f.N + j0 + wave * 16;
+        const uint32_t u1 = jn > wave * 16 ? (jn - wave * 16 < 16 ? jn - wave * 16 : 16) : 0;
+        if (u1 == 16) {
+#pragma unroll
+            for (uint32_t u = 0; u < 16; ++u) acc = fma(lr[u], zs[wave * 16 + u], acc);
+        } else {
+            for (uint32_t u = 0; u < u1; ++u) acc = fma(lr[u], zs[wave * 16 + u], acc);
+        }
+    }
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (tid < 64 && lane < rn) wf[p] -= (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    if (blockIdx.x == 0 && j0 + jn < f.k) {
+        __syncthreads();
+        if (tid < 64) mfs_forward_block(f, L, wf, y, j0 + jn, lane);
+    }
+}
+
+// The number of 64-row chunks below block `blk` of a front.
+__device__ __forceinline__ uint32_t mfs_chunks(const FrontDesc& f, uint32_t blk) {
+    const uint32_t j0 = blk * 64, jn = f.k - j0 < 64 ? f.k - j0 : 64;
+    return (f.m - j0 - jn + 63) / 64;
+}
+
+// Backward step `st` of the split fronts (st = 0 takes each front's last block): block b = its
+// number of blocks - 1 - st. The first chunk first finishes block b + 1 (the last step's chunk sums
+// in chunk order, then L^T of the block: x_j = z_j / d_j - sum over t > j of L_tj x_t); then each
+// chunk's sums over its rows, L[t, block b] x_t per column, go to part. The step after a front's
+// block 0 only finishes that block.
+__global__ void __launch_bounds__(256) k_mfs_backward_step(const uint32_t* list, const FrontDesc* fronts,
+                                                           const uint32_t* rows, const double* F, const double* Dv,
+                                                           double* y, double* part, uint32_t maxch, uint32_t st) {
+    __shared__ double red[4][64];
+    const FrontDesc f = fronts[list[blockIdx.y]];
+    const int nb = (int)((f.k + 63) / 64);
+    const int b = nb - 1 - (int)st;
+    if (b < -1) return;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const double* L = F + f.off;
+    double* mine = part + (uint64_t)blockIdx.y * 2 * maxch * 64;
+    if (blockIdx.x == 0 && b + 1 < nb) {  // finish block b + 1
+        const uint32_t j0 = (uint32_t)(b + 1) * 64, jn = f.k - j0 < 64 ? f.k - j0 : 64;
+        if (tid < 64) {
+            const double* prev = mine + (uint64_t)((st - 1) & 1) * maxch * 64;
+            const uint32_t nc = mfs_chunks(f, (uint32_t)(b + 1));
+            double below = 0.0;
+            for (uint32_t c = 0; c < nc; ++c) below += prev[c * 64 + lane];
+            double xv = lane < jn ? y[f.first + j0 + lane] / Dv[f.doff + j0 + lane] - below : 0.0;
+            for (int t = (int)jn - 1; t >= 0; --t) {
+                const double xt = lane_value(xv, (uint32_t)t);
+                if ((int)lane < t) xv = fma(-L[(uint64_t)(j0 + t) * f.N + j0 + lane], xt, xv);
+            }
+            if (lane < jn) y[f.first + j0 + lane] = xv;
+        }
+        __syncthreads();
+    }
+    if (b < 0) return;
+    const uint32_t j0 = (uint32_t)b * 64, jn = f.k - j0 < 64 ? f.k - j0 : 64;
+    const uint32_t r0 = j0 + jn + blockIdx.x * 64;
+    if (r0 >= f.m) return;
+    const uint32_t rn = f.m - r0 < 64 ? f.m - r0 : 64;
+    const uint32_t* rw = rows + f.roff;
+    double acc = 0.0;
+    if (lane < jn)
+        for (uint32_t i = wave; i < rn; i += 4) {
+            const uint32_t t = r0 + i;
+            const double xt = t < f.k ? y[f.first + t] : y[rw[t]];
+            acc = fma(L[(uint64_t)pos_of(f, t) * f.N + j0 + lane], xt, acc);
+        }
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (tid < 64)
+        mine[(uint64_t)(st & 1) * maxch * 64 + blockIdx.x * 64 + lane] =
+            (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
 }
 
 __global__ void __launch_bounds__(256) k_mf_gather(uint64_t n, const uint64_t* perm, const double* b, double* y) {
@@ -354,6 +505,12 @@ struct Multifrontal::Impl {
     // per level: its fronts (small and large), its tasks and scatter entries
     std::vector<uint64_t> lv_small, lv_med, lv_all;       // offsets into the lists (nlevels + 1)
     std::vector<uint32_t> all_list, small_list, med_list;  // host copies
+    // the solve: per level the fronts of one workgroup each, and the split fronts with their
+    // largest number of blocks and of chunks below a block
+    std::vector<uint64_t> lv_solve, lv_split;
+    std::vector<uint32_t> split_blocks, split_chunks;
+    DArray<uint32_t> d_solve, d_split;
+    DArray<double> part;
     std::vector<std::vector<uint32_t>> lv_large;
     std::vector<FrontDesc> fr;
     DArray<FrontDesc> d_fr;
@@ -445,6 +602,35 @@ void Multifrontal::Impl::build(const csr::Csr& a, const std::vector<std::array<d
         lv_all[l + 1] = all_list.size();
         lv_small[l + 1] = small_list.size();
         lv_med[l + 1] = med_list.size();
+    }
+    // the solve's lists
+    {
+        std::vector<uint32_t> solve_list, split_list;
+        lv_solve.assign(nlevels + 1, 0);
+        lv_split.assign(nlevels + 1, 0);
+        split_blocks.assign(nlevels, 0);
+        split_chunks.assign(nlevels, 0);
+        uint64_t part_len = 1;
+        for (uint64_t l = 0; l < nlevels; ++l) {
+            for (uint32_t q : by_level[l]) {
+                const FrontDesc& f = fr[q];
+                if (f.k >= KSPLIT) {
+                    split_list.push_back(q);
+                    split_blocks[l] = std::max(split_blocks[l], (f.k + 63) / 64);
+                    const uint32_t jn = f.k < 64 ? f.k : 64;
+                    split_chunks[l] = std::max(split_chunks[l], (f.m - jn + 63) / 64);
+                } else {
+                    solve_list.push_back(q);
+                }
+            }
+            lv_solve[l + 1] = solve_list.size();
+            lv_split[l + 1] = split_list.size();
+            split_chunks[l] = std::max<uint32_t>(split_chunks[l], 1);
+            part_len = std::max<uint64_t>(part_len, (lv_split[l + 1] - lv_split[l]) * 2 * split_chunks[l] * 64);
+        }
+        d_solve = upload(solve_list, s);
+        d_split = upload(split_list, s);
+        part.resize(part_len);
     }
     // device structures
     d_fr = upload(fr, s);
@@ -630,18 +816,39 @@ void Multifrontal::solve(const double* b, double* x, hipStream_t s) const {
     if (n == 0) return;
     hipLaunchKernelGGL(k_mf_gather, dim3(blocks_of(n)), dim3(256), 0, s, n, (const uint64_t*)I.d_perm.data(), b,
                        I.y.data());
+    const FrontDesc* fr = (const FrontDesc*)I.d_fr.data();
+    const double* F = (const double*)I.F.data();
     for (uint64_t l = 0; l < I.nlevels; ++l) {
-        const uint64_t nf = I.lv_all[l + 1] - I.lv_all[l];
-        hipLaunchKernelGGL(k_mf_forward, dim3((unsigned)nf), dim3(256), 0, s, (const uint32_t*)I.d_all.data() + I.lv_all[l],
-                           (const FrontDesc*)I.d_fr.data(), (const uint32_t*)I.d_kids.data(),
-                           (const uint32_t*)I.d_rel.data(), (const double*)I.F.data(), I.w.data(), I.y.data());
+        const uint64_t nf = I.lv_solve[l + 1] - I.lv_solve[l];
+        if (nf)
+            hipLaunchKernelGGL(k_mf_forward, dim3((unsigned)nf), dim3(256), 0, s,
+                               (const uint32_t*)I.d_solve.data() + I.lv_solve[l], fr, (const uint32_t*)I.d_kids.data(),
+                               (const uint32_t*)I.d_rel.data(), F, I.w.data(), I.y.data());
+        const uint64_t nsp = I.lv_split[l + 1] - I.lv_split[l];
+        if (nsp) {
+            const uint32_t* list = (const uint32_t*)I.d_split.data() + I.lv_split[l];
+            hipLaunchKernelGGL(k_mfs_forward_start, dim3((unsigned)nsp), dim3(256), 0, s, list, fr,
+                               (const uint32_t*)I.d_kids.data(), (const uint32_t*)I.d_rel.data(), F, I.w.data(),
+                               I.y.data());
+            for (uint32_t blk = 0; blk < I.split_blocks[l]; ++blk)
+                hipLaunchKernelGGL(k_mfs_forward_step, dim3(I.split_chunks[l], (unsigned)nsp), dim3(256), 0, s, list, fr,
+                                   F, I.w.data(), I.y.data(), blk);
+        }
     }
     for (uint64_t l = I.nlevels; l-- > 0;) {
-        const uint64_t nf = I.lv_all[l + 1] - I.lv_all[l];
-        hipLaunchKernelGGL(k_mf_backward, dim3((unsigned)nf), dim3(256), 0, s,
-                           (const uint32_t*)I.d_all.data() + I.lv_all[l], (const FrontDesc*)I.d_fr.data(),
-                           (const uint32_t*)I.d_rows.data(), (const double*)I.F.data(), (const double*)I.Dv.data(),
-                           I.y.data());
+        const uint64_t nsp = I.lv_split[l + 1] - I.lv_split[l];
+        if (nsp) {
+            const uint32_t* list = (const uint32_t*)I.d_split.data() + I.lv_split[l];
+            for (uint32_t st = 0; st <= I.split_blocks[l]; ++st)
+                hipLaunchKernelGGL(k_mfs_backward_step, dim3(I.split_chunks[l], (unsigned)nsp), dim3(256), 0, s, list,
+                                   fr, (const uint32_t*)I.d_rows.data(), F, (const double*)I.Dv.data(), I.y.data(),
+                                   I.part.data(), I.split_chunks[l], st);
+        }
+        const uint64_t nf = I.lv_solve[l + 1] - I.lv_solve[l];
+        if (nf)
+            hipLaunchKernelGGL(k_mf_backward, dim3((unsigned)nf), dim3(256), 0, s,
+                               (const uint32_t*)I.d_solve.data() + I.lv_solve[l], fr, (const uint32_t*)I.d_rows.data(),
+                               F, (const double*)I.Dv.data(), I.y.data());
     }
     hipLaunchKernelGGL(k_mf_place, dim3(blocks_of(n)), dim3(256), 0, s, n, (const uint64_t*)I.d_perm.data(),
                        (const double*)I.y.data(), x);
