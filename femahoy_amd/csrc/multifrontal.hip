@@ -66,6 +66,17 @@ __device__ __forceinline__ double lane_value(double v, uint32_t t) {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+// The diagonal block of the pivots j0 .. j0 + jn - 1 of a front into LDS (row r, column c at
+// Ld[r][c]; rows padded to 65), in one coalesced pass of the 256 threads, so that the block's
+// solve chain reads LDS instead of waiting on a load at each step.
+__device__ __forceinline__ void stage_block(const FrontDesc& f, const double* L, uint32_t j0, uint32_t jn,
+                                            double (*Ld)[65]) {
+    for (uint32_t e = threadIdx.x; e < jn * 64; e += 256) {
+        const uint32_t r = e >> 6, c = e & 63;
+        if (c < jn) Ld[r][c] = L[(uint64_t)(j0 + r) * f.N + j0 + c];
+    }
+}
+
 __global__ void __launch_bounds__(256) k_mf_init(const Task* tasks, const FrontDesc* fronts, double* F) {
     const Task tk = tasks[blockIdx.x];
     const FrontDesc f = fronts[tk.front];
@@ -250,6 +261,7 @@ __global__ void __launch_bounds__(256) k_mf_update(const Task* tasks, const Fron
 // columns). What stays below the pivots is the front's update vector for its parent.
 __global__ void __launch_bounds__(256) k_mf_forward(const uint32_t* list, const FrontDesc* fronts, const uint32_t* kids,
                                                      const uint32_t* rel, const double* F, double* w, double* y) {
+    __shared__ double Ld[64][65];
     const FrontDesc f = fronts[list[blockIdx.x]];
     double* wf = w + f.woff;
     const uint32_t tid = threadIdx.x, lane = tid & 63;
@@ -265,11 +277,13 @@ __global__ void __launch_bounds__(256) k_mf_forward(const uint32_t* list, const 
     const double* L = F + f.off;
     for (uint32_t j0 = 0; j0 < f.k; j0 += 64) {
         const uint32_t jn = f.k - j0 < 64 ? f.k - j0 : 64;
+        stage_block(f, L, j0, jn, Ld);
+        __syncthreads();
         if (tid < 64) {
             double wl = lane < jn ? wf[j0 + lane] : 0.0;
             for (uint32_t t = 0; t < jn; ++t) {
                 const double v = lane_value(wl, t);
-                if (lane > t && lane < jn) wl = fma(-L[(uint64_t)(j0 + lane) * f.N + j0 + t], v, wl);
+                if (lane > t && lane < jn) wl = fma(-Ld[lane][t], v, wl);
             }
             if (lane < jn) wf[j0 + lane] = wl;
         }
@@ -294,6 +308,7 @@ __global__ void __launch_bounds__(256) k_mf_forward(const uint32_t* list, const 
 __global__ void __launch_bounds__(256) k_mf_backward(const uint32_t* list, const FrontDesc* fronts,
                                                       const uint32_t* rows, const double* F, const double* Dv, double* y) {
     __shared__ double red[4][64];
+    __shared__ double Ld[64][65];
     const FrontDesc f = fronts[list[blockIdx.x]];
     const double* L = F + f.off;
     const uint32_t* rw = rows + f.roff;
@@ -301,34 +316,36 @@ __global__ void __launch_bounds__(256) k_mf_backward(const uint32_t* list, const
     if (f.k == 0) return;
     for (int64_t jb = (int64_t)((f.k - 1) / 64) * 64; jb >= 0; jb -= 64) {
         const uint32_t j0 = (uint32_t)jb, jn = f.k - j0 < 64 ? f.k - j0 : 64;
-        // four rows in flight per wave (independent partial sums, met in a fixed order)
-        double s4[4] = {0.0, 0.0, 0.0, 0.0};
-        if (lane < jn) {
-            uint32_t t = j0 + jn + wave;
-            for (; t + 12 < f.m; t += 16) {
-                double xt[4], lv[4];
+        // sixteen rows in flight per wave (independent partial sums, met in a fixed order)
+        double s16[16];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < 16; ++q) s16[q] = 0.0;
+        if (lane < jn)
+            for (uint32_t t = j0 + jn + wave; t < f.m; t += 64) {  // a ragged last group is predicated
+                double xt[16], lv[16];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
                     const uint32_t tq = t + 4 * q;
-                    xt[q] = tq < f.k ? y[f.first + tq] : y[rw[tq]];
-                    lv[q] = L[(uint64_t)pos_of(f, tq) * f.N + j0 + lane];
+                    const bool in = tq < f.m;
+                    xt[q] = !in ? 0.0 : tq < f.k ? y[f.first + tq] : y[rw[tq]];
+                    lv[q] = in ? L[(uint64_t)pos_of(f, tq) * f.N + j0 + lane] : 0.0;
                 }
 #pragma unroll
-                for (int q = 0; q < 4; ++q) s4[q] = fma(lv[q], xt[q], s4[q]);
+                for (int q = 0; q < 16; ++q) s16[q] = fma(lv[q], xt[q], s16[q]);
             }
-            for (; t < f.m; t += 4) {
-                const double xt = t < f.k ? y[f.first + t] : y[rw[t]];
-                s4[0] = fma(L[(uint64_t)pos_of(f, t) * f.N + j0 + lane], xt, s4[0]);
-            }
-        }
-        red[wave][lane] = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+#pragma unroll
+        for (int h = 8; h > 0; h >>= 1)
+#pragma unroll
+            for (int q = 0; q < h; ++q) s16[q] += s16[q + h];
+        red[wave][lane] = s16[0];
+        stage_block(f, L, j0, jn, Ld);
         __syncthreads();
         if (tid < 64) {
             const double below = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
             double xv = lane < jn ? y[f.first + j0 + lane] / Dv[f.doff + j0 + lane] - below : 0.0;
             for (int t = (int)jn - 1; t >= 0; --t) {
                 const double xt = lane_value(xv, (uint32_t)t);
-                if ((int)lane < t) xv = fma(-L[(uint64_t)(j0 + t) * f.N + j0 + lane], xt, xv);
+                if ((int)lane < t) xv = fma(-Ld[t][lane], xt, xv);
             }
             if (lane < jn) y[f.first + j0 + lane] = xv;
         }
@@ -345,14 +362,14 @@ __global__ void __launch_bounds__(256) k_mf_backward(const uint32_t* list, const
 // forward steps; the four waves' sums, then the chunks in order, in the backward ones).
 
 // Forward solve of the 64 pivots from j0 of a front's vector wf (its first wave): the unit lower L
-// of the block, z to wf and to y.
-__device__ __forceinline__ void mfs_forward_block(const FrontDesc& f, const double* L, double* wf, double* y,
+// of the block (staged in Ld), z to wf and to y.
+__device__ __forceinline__ void mfs_forward_block(const FrontDesc& f, const double (*Ld)[65], double* wf, double* y,
                                                   uint32_t j0, uint32_t lane) {
     const uint32_t jn = f.k - j0 < 64 ? f.k - j0 : 64;
     double wl = lane < jn ? wf[j0 + lane] : 0.0;
     for (uint32_t t = 0; t < jn; ++t) {
         const double v = lane_value(wl, t);
-        if (lane > t && lane < jn) wl = fma(-L[(uint64_t)(j0 + lane) * f.N + j0 + t], v, wl);
+        if (lane > t && lane < jn) wl = fma(-Ld[lane][t], v, wl);
     }
     if (lane < jn) {
         wf[j0 + lane] = wl;
@@ -365,9 +382,11 @@ __device__ __forceinline__ void mfs_forward_block(const FrontDesc& f, const doub
 __global__ void __launch_bounds__(256) k_mfs_forward_start(const uint32_t* list, const FrontDesc* fronts,
                                                            const uint32_t* kids, const uint32_t* rel, const double* F,
                                                            double* w, double* y) {
+    __shared__ double Ld[64][65];
     const FrontDesc f = fronts[list[blockIdx.x]];
     double* wf = w + f.woff;
     const uint32_t tid = threadIdx.x;
+    stage_block(f, F + f.off, 0, f.k < 64 ? f.k : 64, Ld);
     for (uint32_t t = tid; t < f.m; t += 256) wf[pos_of(f, t)] = t < f.k ? y[f.first + t] : 0.0;
     __syncthreads();
     for (uint32_t q = 0; q < f.nkids; ++q) {
@@ -377,7 +396,8 @@ __global__ void __launch_bounds__(256) k_mfs_forward_start(const uint32_t* list,
         for (uint32_t i = tid; i < c.m - c.k; i += 256) wf[r[i]] += wc[i];
         __syncthreads();
     }
-    if (tid < 64) mfs_forward_block(f, F + f.off, wf, y, 0, tid);
+    __syncthreads();
+    if (tid < 64) mfs_forward_block(f, Ld, wf, y, 0, tid);
 }
 
 // Forward step `blk` of the split fronts: the rows of chunk blockIdx.x below the block take
@@ -387,6 +407,7 @@ __global__ void __launch_bounds__(256) k_mfs_forward_step(const uint32_t* list, 
                                                           const double* F, double* w, double* y, uint32_t blk) {
     __shared__ double zs[64];
     __shared__ double red[4][64];
+    __shared__ double Ld[64][65];
     const FrontDesc f = fronts[list[blockIdx.y]];
     const uint32_t j0 = blk * 64;
     if (j0 >= f.k) return;
@@ -398,6 +419,9 @@ __global__ void __launch_bounds__(256) k_mfs_forward_step(const uint32_t* list, 
     const double* L = F + f.off;
     double* wf = w + f.woff;
     if (tid < 64) zs[lane] = lane < jn ? wf[j0 + lane] : 0.0;
+    const bool next = blockIdx.x == 0 && j0 + jn < f.k;  // this chunk also solves the next block
+    const uint32_t jn1 = f.k - j0 - jn < 64 ? f.k - j0 - jn : 64;
+    if (next) stage_block(f, L, j0 + jn, jn1, Ld);
     __syncthreads();
     const uint32_t p = lane < rn ? pos_of(f, r0 + lane) : 0;
     double acc = 0.0;
@@ -414,9 +438,9 @@ __global__ void __launch_bounds__(256) k_mfs_forward_step(const uint32_t* list, 
     red[wave][lane] = acc;
     __syncthreads();
     if (tid < 64 && lane < rn) wf[p] -= (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
-    if (blockIdx.x == 0 && j0 + jn < f.k) {
+    if (next) {
         __syncthreads();
-        if (tid < 64) mfs_forward_block(f, L, wf, y, j0 + jn, lane);
+        if (tid < 64) mfs_forward_block(f, Ld, wf, y, j0 + jn, lane);
     }
 }
 
@@ -435,6 +459,7 @@ __global__ void __launch_bounds__(256) k_mfs_backward_step(const uint32_t* list,
                                                            const uint32_t* rows, const double* F, const double* Dv,
                                                            double* y, double* part, uint32_t maxch, uint32_t st) {
     __shared__ double red[4][64];
+    __shared__ double Ld[64][65];
     const FrontDesc f = fronts[list[blockIdx.y]];
     const int nb = (int)((f.k + 63) / 64);
     const int b = nb - 1 - (int)st;
@@ -444,15 +469,25 @@ __global__ void __launch_bounds__(256) k_mfs_backward_step(const uint32_t* list,
     double* mine = part + (uint64_t)blockIdx.y * 2 * maxch * 64;
     if (blockIdx.x == 0 && b + 1 < nb) {  // finish block b + 1
         const uint32_t j0 = (uint32_t)(b + 1) * 64, jn = f.k - j0 < 64 ? f.k - j0 : 64;
+        stage_block(f, L, j0, jn, Ld);
+        __syncthreads();
         if (tid < 64) {
             const double* prev = mine + (uint64_t)((st - 1) & 1) * maxch * 64;
             const uint32_t nc = mfs_chunks(f, (uint32_t)(b + 1));
             double below = 0.0;
-            for (uint32_t c = 0; c < nc; ++c) below += prev[c * 64 + lane];
+            uint32_t c = 0;
+            for (; c + 8 <= nc; c += 8) {  // eight loads in flight, added in chunk order
+                double v[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) v[q] = prev[(c + q) * 64 + lane];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) below += v[q];
+            }
+            for (; c < nc; ++c) below += prev[c * 64 + lane];
             double xv = lane < jn ? y[f.first + j0 + lane] / Dv[f.doff + j0 + lane] - below : 0.0;
             for (int t = (int)jn - 1; t >= 0; --t) {
                 const double xt = lane_value(xv, (uint32_t)t);
-                if ((int)lane < t) xv = fma(-L[(uint64_t)(j0 + t) * f.N + j0 + lane], xt, xv);
+                if ((int)lane < t) xv = fma(-Ld[t][lane], xt, xv);
             }
             if (lane < jn) y[f.first + j0 + lane] = xv;
         }
@@ -465,12 +500,18 @@ __global__ void __launch_bounds__(256) k_mfs_backward_step(const uint32_t* list,
     const uint32_t rn = f.m - r0 < 64 ? f.m - r0 : 64;
     const uint32_t* rw = rows + f.roff;
     double acc = 0.0;
-    if (lane < jn)
-        for (uint32_t i = wave; i < rn; i += 4) {
-            const uint32_t t = r0 + i;
-            const double xt = t < f.k ? y[f.first + t] : y[rw[t]];
-            acc = fma(L[(uint64_t)pos_of(f, t) * f.N + j0 + lane], xt, acc);
+    if (lane < jn) {  // sixteen rows a wave, their loads in flight together (a ragged chunk predicated)
+        double xt[16], lv[16];
+#pragma unroll
+        for (uint32_t i = 0; i < 16; ++i) {
+            const uint32_t t = r0 + wave + 4 * i;
+            const bool in = wave + 4 * i < rn;
+            xt[i] = !in ? 0.0 : t < f.k ? y[f.first + t] : y[rw[t]];
+            lv[i] = in ? L[(uint64_t)pos_of(f, t) * f.N + j0 + lane] : 0.0;
         }
+#pragma unroll
+        for (uint32_t i = 0; i < 16; ++i) acc = fma(lv[i], xt[i], acc);
+    }
     red[wave][lane] = acc;
     __syncthreads();
     if (tid < 64)
