@@ -424,7 +424,7 @@ class Flops:
 
 def _options(method=None, precision="fp32", tol=PCG_TOL, max_iter=PCG_MAX_ITER, crossover=STEADY_DIRECT_LIMIT,
              levels=0, cheb_degree=0, cheb_lower=0.1, cheb_upper=1.1, power_steps=20, coarse_max=6000, refine=True,
-             device=0, coarse_galerkin=True) -> "_Options":
+             device=0, coarse_galerkin=True, direct_solver="auto") -> "_Options":
     o = _Options()
     lib().fem_options_default(C.byref(o))
     o.tol, o.max_iter, o.method, o.precision = tol, max_iter, METHODS[method], PRECISIONS[precision]
@@ -437,6 +437,9 @@ def _options(method=None, precision="fp32", tol=PCG_TOL, max_iter=PCG_MAX_ITER, 
     o.cheb_lower, o.cheb_upper, o.power_steps = cheb_lower, cheb_upper, power_steps
     o.coarse_max, o.refine, o.device = coarse_max, int(refine), device
     o.coarse_galerkin = int(coarse_galerkin)
+    if direct_solver not in DIRECT_SOLVERS:
+        raise FemError(f"unknown direct solver {direct_solver}: auto, dense or multifrontal")
+    o.direct_solver = DIRECT_SOLVERS[direct_solver]
     return o
 
 
@@ -446,14 +449,15 @@ class Solver:
     def __init__(self, box: Box, method: Optional[str] = None, precision: str = "fp32", tol: float = PCG_TOL,
                  max_iter: int = PCG_MAX_ITER, crossover=None, levels: int = 0,
                  cheb_degree: int = 0, cheb_lower: float = 0.1, cheb_upper: float = 1.1, power_steps: int = 20,
-                 coarse_max: int = 6000, refine: bool = True, device: int = 0, coarse_galerkin: bool = True):
+                 coarse_max: int = 6000, refine: bool = True, device: int = 0, coarse_galerkin: bool = True,
+                 direct_solver: str = "auto"):
         if crossover is None:
             # the box-grid elasticity caller's choice (elastic.rs:525-547): Crossover::Flops, flat when
             # a cell's longest side is more than MULTIGRID_ASPECT times its shortest; else Unknowns
             crossover = (Flops(flat=box_aspect(box) > MULTIGRID_ASPECT) if box.kind == "elastic_hex8"
                          else STEADY_DIRECT_LIMIT)
         o = _options(method, precision, tol, max_iter, crossover, levels, cheb_degree, cheb_lower, cheb_upper,
-                     power_steps, coarse_max, refine, device, coarse_galerkin)
+                     power_steps, coarse_max, refine, device, coarse_galerkin, direct_solver)
         b, keep = box._struct()
         h = C.c_void_p()
         err = _err()

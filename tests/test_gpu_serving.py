@@ -158,8 +158,8 @@ def test_a_solve_beyond_the_budget_is_refused(fem):
 
 def test_freeing_the_server_ends_a_waiting_request_and_its_waiter(fem):
     """A waiter blocked on an unfinished request returns with the server's stop message when another
-    thread frees the server. One solve at a time: request A's setup (a forced direct factor of
-    27,869 unknowns) outlasts the 0.3 s before the free, so request B is still queued then."""
+    thread frees the server. One solve at a time: request A's setup (a forced dense direct factor
+    of 27,869 unknowns) outlasts the 0.3 s before the free, so request B is still queued then."""
     import threading
     import time
 
@@ -169,7 +169,7 @@ def test_freeing_the_server_ends_a_waiting_request_and_its_waiter(fem):
     kind, m, kw = specs()[0]
     small = make_box(fem, kind, m, kw)
     srv = fem.Server(hbm_budget=48 << 30, max_active=1)
-    srv.submit(big, method="direct")
+    srv.submit(big, method="direct", direct_solver="dense")
     rid = srv.submit(small, **OPTS)
     got = {}
 
