@@ -11,7 +11,9 @@ cantilever with incompatible modes (grid-cantilever), a wall of 3 layers with a 
 each side on a graded grid (grid-layered-wall), and a steel plate with a heated copper block on
 it, void cells beside the block and half symmetry in x and y (grid-heated-block-ccx), and the
 same plate and block with the thermal stress of its steady temperature, a point support and the
-shape of the bottom surface (grid-heated-block-stress-ccx).
+shape of the bottom surface (grid-heated-block-stress-ccx). The transient cooling slab
+(grid-cooling-slab) and the uniformly heated bimetal strip (grid-bimetal-strip) are checked against
+their derivations' whole tables of grids.
 """
 import os
 
@@ -85,3 +87,32 @@ def test_transient_cooling_slab_reproduces_the_references_table(fem, max_cell):
             with open(os.path.join(CASES, "grid-cooling-slab", "check.toml")) as f:
                 rows = grid_case.check(res, f.read(), case.name)
             assert rows and all(r[-1] for r in rows)
+
+
+# grid-bimetal-strip (derivation.md, "Mesh study"): the reference's grid solver on 4 grids of the
+# strip heated uniformly by 100 C: unknowns and the top surface's curvatures, to 7 digits.
+BIMETAL = {2.0: (1406, 1.122349e-3, 1.132433e-3), 1.0: (4301, 1.158686e-3, 1.157552e-3),
+           0.5: (16096, 1.158975e-3, 1.157798e-3), 0.25: (62180, 1.159359e-3, 1.158216e-3)}
+
+
+@pytest.mark.parametrize("max_cell", [2.0, 1.0, 0.5, 0.25])
+def test_bimetal_strip_reproduces_the_references_mesh_study(fem, max_cell):
+    """Two bonded layers heated by a uniform 100 C (a static step's `temperature`: each cell's
+    thermal force from the same change at every corner, elastic.rs:358-404), free but for uz at the
+    centre of the bottom, quarter symmetry: the curvatures of the top surface's quadratic fit
+    (elastic.rs:1099-1250) match the reference solver's table to half a unit of its 7th digit,
+    the top bulges outward, and the case file's own grid passes its check.toml (Timoshenko's
+    curvature 1.163636e-3 within 1 %)."""
+    from femahoy_amd import grid_case
+    case = grid_case.load_case(os.path.join(CASES, "grid-bimetal-strip/strip/grid.toml"))
+    case.max_cell = max_cell
+    res = grid_case.solve(case, precision="fp64", tol=1e-12)
+    unknowns, kxx, kyy = BIMETAL[max_cell]
+    print(max_cell, res.method, res.unknowns, res.probes["top.kxx"], res.probes["top.kyy"])
+    assert res.unknowns == unknowns
+    assert abs(res.probes["top.kxx"] - kxx) <= 0.5e-9 and abs(res.probes["top.kyy"] - kyy) <= 0.5e-9
+    assert res.probes["top.shape"] == "convex"
+    if max_cell == 1.0:
+        with open(os.path.join(CASES, "grid-bimetal-strip", "check.toml")) as f:
+            rows = grid_case.check(res, f.read(), case.name)
+        assert [r[1] for r in rows] == ["top.kxx", "top.kyy"] and all(r[-1] for r in rows)
