@@ -161,7 +161,7 @@ class Profile(C.Structure):
 
 
 EXPORTS = (
-    "fem_options_default", "fem_hex_element", "fem_solver_new_box", "fem_solver_solve", "fem_solver_solve_device",
+    "fem_options_default", "fem_hex_element", "fem_hex_stress", "fem_solver_new_box", "fem_solver_solve", "fem_solver_solve_device",
     "fem_solver_solve_many", "fem_solver_solve_many_device",
     "fem_solver_method", "fem_solver_set_max_iter", "fem_solver_set_tol", "fem_solver_free",
     "fem_solver_free_count", "fem_solver_unknown_count", "fem_solver_free_list", "fem_solver_point_grid",
@@ -260,6 +260,8 @@ def lib() -> C.CDLL:
                                                 P, ERR, SZ]),
         "fem_test_hex_element": (C.c_int, [F64P, C.c_double, C.c_double, F64P, ERR, SZ]),
         "fem_hex_element": (C.c_int, [F64P, C.c_double, C.c_double, C.c_double, F64P, F64P, ERR, SZ]),
+        "fem_hex_stress": (C.c_int, [F64P, C.c_double, C.c_double, C.c_double, F64P, F64P, F64P, F64P, F64P, ERR,
+                                     SZ]),
         "fem_test_supernodal": (C.c_int, [C.POINTER(_Csr), P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), P, P, P, P,
                                           P, ERR, SZ]),
         "fem_test_hex_box_system": (C.c_int, [C.POINTER(_Box), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), P, P, P,
@@ -973,6 +975,19 @@ def hex_element(size, e: float, nu: float, alpha: Optional[float] = None):
                              0.0 if alpha is None else float(alpha), k, load, err, 2048):
         raise FemError(err.value.decode())
     return k.reshape(24, 24) if alpha is None else (k.reshape(24, 24), load.reshape(24, 8))
+
+
+def hex_strain_stress(size, e: float, nu: float, alpha: float, xi, u, t):
+    """hex.rs:313-340 strain_stress of the condensed hex (fem_hex_stress): the engineering strain and
+    the stress (xx, yy, zz, yz, xz, xy) at the local point xi from the corner displacements u (24)
+    and temperature changes t (8). Host code, no device."""
+    strain, stress = np.zeros(6), np.zeros(6)
+    err = _err()
+    if lib().fem_hex_stress(np.ascontiguousarray(size, np.float64), float(e), float(nu), float(alpha),
+                            np.ascontiguousarray(xi, np.float64), np.ascontiguousarray(u, np.float64),
+                            np.ascontiguousarray(t, np.float64), strain, stress, err, 2048):
+        raise FemError(err.value.decode())
+    return strain, stress
 
 
 def hex_box_system(box: Box):

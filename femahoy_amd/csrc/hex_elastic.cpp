@@ -119,12 +119,64 @@ std::array<double, 576> hex_element(const std::array<double, 3>& size, const std
     return k;
 }
 
+namespace {
+// hex.rs:199-279: the element with its mode maps, mode_u (9 x 24) and mode_t (9 x 8): the modes'
+// amplitudes are mode_t t - mode_u u.
+void hex_element_modes(const std::array<double, 3>& size, const std::array<double, 12>& d, double alpha,
+                       std::array<double, 576>& out, std::array<double, 192>& lout, double x[9][24],
+                       double xt[9][8], double thermal[3]);
+}  // namespace
+
 void hex_element_thermal(const std::array<double, 3>& size, const std::array<double, 12>& d, double alpha,
                          std::array<double, 576>& out, std::array<double, 192>& lout) {
+    double x[9][24], xt[9][8], thermal[3];
+    hex_element_modes(size, d, alpha, out, lout, x, xt, thermal);
+}
+
+void hex_strain_stress(const std::array<double, 3>& size, const std::array<double, 12>& d, double alpha,
+                       const double xi[3], const double u[24], const double t[8], double strain[6], double stress[6]) {
+    std::array<double, 576> k;
+    std::array<double, 192> load;
+    double x[9][24], xt[9][8], thermal[3];
+    hex_element_modes(size, d, alpha, k, load, x, xt, thermal);
+    double all[11][3];
+    for (int n = 0; n < 8; ++n)
+        for (int c = 0; c < 3; ++c) all[n][c] = u[3 * n + c];
+    for (int i = 0; i < 9; ++i) {
+        double a = 0.0;
+        for (int c = 0; c < 8; ++c) a += xt[i][c] * t[c];
+        for (int c = 0; c < 24; ++c) a -= x[i][c] * u[c];
+        all[8 + i / 3][i % 3] = a;
+    }
+    double g[11][3];
+    gradients11(size, xi, g);
+    for (int i = 0; i < 6; ++i) strain[i] = 0.0;
+    for (int i = 0; i < 11; ++i) {  // hex.rs:180-193
+        for (int dd = 0; dd < 3; ++dd) strain[dd] += g[i][dd] * all[i][dd];
+        for (int sidx = 0; sidx < 3; ++sidx) {
+            const int a = SHEAR_AXES[sidx][0], b = SHEAR_AXES[sidx][1];
+            strain[3 + sidx] += g[i][a] * all[i][b] + g[i][b] * all[i][a];
+        }
+    }
+    double dt = 0.0;  // the temperature change at xi: the corner functions over t
+    for (int c = 0; c < 8; ++c) {
+        double v = 1.0;
+        for (int dd = 0; dd < 3; ++dd) v *= CORNER[c][dd] == 1 ? 0.5 * (1.0 + xi[dd]) : 0.5 * (1.0 - xi[dd]);
+        dt += v * t[c];
+    }
+    const double* n = d.data();
+    for (int i = 0; i < 3; ++i) stress[i] = n[3 * i] * strain[0] + n[3 * i + 1] * strain[1] + n[3 * i + 2] * strain[2];
+    for (int sidx = 0; sidx < 3; ++sidx) stress[3 + sidx] = d[9 + sidx] * strain[3 + sidx];
+    for (int i = 0; i < 3; ++i) stress[i] -= thermal[i] * dt;
+}
+
+namespace {
+void hex_element_modes(const std::array<double, 3>& size, const std::array<double, 12>& d, double alpha,
+                       std::array<double, 576>& out, std::array<double, 192>& lout, double x[9][24],
+                       double xt[9][8], double thermal[3]) {
     const double* normal = d.data();
     const double* shear = d.data() + 9;
     // the thermal stress of a unit temperature change: D alpha on the normal stresses (a sum from 0)
-    double thermal[3];
     for (int i = 0; i < 3; ++i) {
         double t = 0.0;
         for (int j = 0; j < 3; ++j) t += normal[3 * i + j] * alpha;
@@ -174,14 +226,12 @@ void hex_element_thermal(const std::array<double, 3>& size, const std::array<dou
     for (int i = 0; i < 9; ++i)
         for (int j = 0; j < 9; ++j) kmm[i][j] = K(24 + i, 24 + j);
     factor9(kmm, l);
-    double x[9][24];
     for (int c = 0; c < 24; ++c) {
         double col[9];
         for (int i = 0; i < 9; ++i) col[i] = K(24 + i, c);
         solve9(l, col);
         for (int i = 0; i < 9; ++i) x[i][c] = col[i];
     }
-    double xt[9][8];
     for (int c = 0; c < 8; ++c) {
         double col[9];
         for (int i = 0; i < 9; ++i) col[i] = Lm(24 + i, c);
@@ -201,6 +251,7 @@ void hex_element_thermal(const std::array<double, 3>& size, const std::array<dou
         }
     }
 }
+}  // namespace
 
 std::vector<double> box_planes(const BoxModel& m, int d) {
     std::vector<double> p((size_t)m.in.n[d] + 1);

@@ -31,6 +31,13 @@ std::array<double, 576> hex_element(const std::array<double, 3>& size, const std
 void hex_element_thermal(const std::array<double, 3>& size, const std::array<double, 12>& d, double alpha,
                          std::array<double, 576>& k, std::array<double, 192>& load);
 
+// hex.rs:313-340 strain_stress: the engineering strain (xx, yy, zz, yz, xz, xy) and the stress in
+// the same order at the local point xi (each coordinate -1 to 1) from the corner displacements u
+// (24) and temperature changes t (8): the modes' amplitudes recovered, the strain of all 11
+// functions, D times it less the thermal stress of the interpolated temperature change.
+void hex_strain_stress(const std::array<double, 3>& size, const std::array<double, 12>& d, double alpha,
+                       const double xi[3], const double u[24], const double t[8], double strain[6], double stress[6]);
+
 // The reduced system of a box model of kind ElasticHex8.
 struct HexBoxSystem {
     csr::Csr a;             // Csr::pattern + add_element in cell order, then submatrix(free)

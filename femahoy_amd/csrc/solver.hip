@@ -3220,6 +3220,15 @@ int fem_hex_element(const double* size, double e, double nu, double alpha, doubl
     });
 }
 
+int fem_hex_stress(const double* size, double e, double nu, double alpha, const double* xi, const double* u,
+                   const double* t, double* strain, double* stress, char* err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!size || !xi || !u || !t || !strain || !stress) throw fem::Error("every argument is needed");
+        fem::hex_strain_stress({size[0], size[1], size[2]}, fem::isotropic_stiffness(e, nu), alpha, xi, u, t, strain,
+                               stress);
+    });
+}
+
 int fem_test_hex_box_system(const fem_box* box, uint64_t* n, uint64_t* nnz, uint64_t* ptr, uint32_t* idx, double* val,
                             double* rhs, double* aspect, char* err, size_t errlen) {
     return guarded(err, errlen, [&] {

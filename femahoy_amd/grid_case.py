@@ -9,7 +9,8 @@ numbers the points, and as heat.rs:328-700 (steady), heat.rs:684-735 (transient:
 with the lumped capacity, one solver for every step) and elastic.rs:287-563 (static) assemble and
 solve. A static step with a temperature (uniform, or the steady heat solve of its heat loads on the
 same grid, elastic.rs:358-363) adds each cell's thermal force to the loads (elastic.rs:383-404);
-its surfaces give their shape (elastic.rs:1099-1250). The matrices and right sides are assembled
+its surfaces give their shape (elastic.rs:1099-1250), its probes their stress (elastic.rs:676-700)
+and its displacement loads their reactions (elastic.rs:773-839). The matrices and right sides are assembled
 on the device (`fem_assemble`), the solve is the library's (`fem_solver_new_csr` with the grid
 level, and the cells for a displacement).
 
@@ -644,7 +645,8 @@ def number(grid: Grid):
 
 def probe_weights(case: Case, grid: Grid, mids, number_of):
     """heat.rs:1312-1358: for each probe its (node, weight) pairs at the cell's corners (the
-    trilinear functions at its local point), and the axes on which it reads its mirror point."""
+    trilinear functions at its local point), the axes on which it reads its mirror point, its cell
+    and its local point (each coordinate -1 to 1)."""
     out = []
     for name, point in case.probes:
         x = list(point)
@@ -664,7 +666,7 @@ def probe_weights(case: Case, grid: Grid, mids, number_of):
             for d in range(3):
                 v *= 0.5 * (1.0 + xi[d]) if b[d] == 1 else 0.5 * (1.0 - xi[d])
             w.append(v)
-        out.append((name, [(number_of[c], wc) for c, wc in zip(grid.corners(cell), w)], mirrored))
+        out.append((name, [(number_of[c], wc) for c, wc in zip(grid.corners(cell), w)], mirrored, cell, xi))
     return out
 
 
@@ -806,7 +808,7 @@ def _solve_heat(case, precision, method, tol_, device):
             x, st = s.solve(rhs + cf * t[free], x=x)
             t[free] = x
     probes = {}
-    for name, pairs, _ in probe_weights(case, grid, mids, number_of):
+    for name, pairs, *_ in probe_weights(case, grid, mids, number_of):
         probes[name] = sum(t[v] * w for v, w in pairs)
     return Result(probes, t, ijk, grid, s.method(), st, len(free))
 
@@ -989,8 +991,9 @@ def _solve_elastic(case, precision, method, tol_, device):
             for r in range(24):
                 force[3 * cells[e][r // 3] + r % 3] += f[r]
     fixed = [None] * (3 * n)
+    owner = [-1] * (3 * n)  # the load that first fixes each unknown, for the reactions (elastic.rs:419)
 
-    def fix(v, value):
+    def fix(v, value, li):
         for d in range(3):
             val = value[d]
             if val is None:
@@ -999,18 +1002,20 @@ def _solve_elastic(case, precision, method, tol_, device):
                 x = xyz[v]
                 raise CaseError(f"2 fixed displacements give 2 values of u{'xyz'[d]} at the point "
                                 f"({x[0]:e}, {x[1]:e}, {x[2]:e}): {fixed[3 * v + d]} and {val}")
+            if fixed[3 * v + d] is None:
+                owner[3 * v + d] = li
             fixed[3 * v + d] = val
-    for l in case.loads:
+    for li, l in enumerate(case.loads):
         if l.get("heat"):
             continue
         if l["kind"] == "point":
-            fix(grid_point(grid, number_of, mids, l["point"]), l["value"])
+            fix(grid_point(grid, number_of, mids, l["point"]), l["value"], li)
             continue
         for face in faces[l["faces"]]:
             pts = grid.face_points(face)
             if l["kind"] == "displacement":
                 for p in pts:
-                    fix(number_of[p], l["value"])
+                    fix(number_of[p], l["value"], li)
             else:
                 if l["kind"] == "traction":
                     t = list(l["value"])
@@ -1053,10 +1058,46 @@ def _solve_elastic(case, precision, method, tol_, device):
     u = value.copy()
     u[free] = x
     probes = {}
-    for name, pairs, mirrored in probe_weights(case, grid, mids, number_of):
+    for name, pairs, mirrored, cell, xi in probe_weights(case, grid, mids, number_of):
         for d, comp in enumerate(("ux", "uy", "uz")):
             v = sum(u[3 * node + d] * w for node, w in pairs)
             probes[f"{name}.{comp}"] = -v if mirrored[d] else v
+        # elastic.rs:676-700: the stress of the probe's cell at its local point, in the order xx, yy,
+        # zz, xy, yz, xz (elastic.rs:967); a mirrored axis turns the shears that hold it
+        nodes = [node for node, _ in pairs]
+        ue = np.array([u[3 * node + c] for node in nodes for c in range(3)])
+        te = dt[nodes] if dt is not None else np.zeros(8)
+        m = case.materials[case.parts[int(grid.part[cell])].material]
+        _, sg = fem.hex_strain_stress(grid.size(cell), m["youngs_modulus"], m["poisson_ratio"], m["expansion"], xi,
+                                      ue, te)
+        sig = [sg[0], sg[1], sg[2], sg[5], sg[3], sg[4]]
+        for a in range(3):
+            if mirrored[a]:
+                for sidx, pair in enumerate(((0, 1), (1, 2), (0, 2))):
+                    if a in pair:
+                        sig[3 + sidx] = -sig[3 + sidx]
+        for comp, v in zip(("sxx", "syy", "szz", "sxy", "syz", "sxz"), sig):
+            probes[f"{name}.{comp}"] = v
+        xx, yy, zz, xy, yz, xz = sig
+        probes[f"{name}.mises"] = math.sqrt(0.5 * ((xx - yy) ** 2 + (yy - zz) ** 2 + (zz - xx) ** 2)
+                                            + 3.0 * (xy * xy + yz * yz + xz * xz))
+        if dt is not None:
+            probes[f"{name}.t"] = sum(w * (dt[node] + case.thermal["reference"]) for node, w in pairs)
+    # elastic.rs:773-839: the reactions K u - f at the unknowns that each displacement load fixes
+    # first (planes of symmetry and point supports have none), named after its face set
+    dofs = (3 * cells[:, :, None] + np.arange(3)).reshape(len(cells), 24)
+    ku = np.zeros(3 * n)
+    np.add.at(ku, dofs, np.einsum("eij,ej->ei", element, u[dofs]))
+    for li, l in enumerate(case.loads):
+        if l["kind"] != "displacement":
+            continue
+        f = [0.0, 0.0, 0.0]
+        for i in range(3 * n):
+            if owner[i] == li:
+                f[i % 3] += ku[i] - force[i]
+        name = case.faces[l["faces"]].name
+        for comp, v in zip(("fx", "fy", "fz"), f):
+            probes[f"{name}.{comp}"] = v
     for name, sf in case.surfaces:
         probes.update(surface_shape(grid, faces[sf], mids, number_of, u, name))
     return Result(probes, u, ijk, grid, s.method(), st, len(free))

@@ -341,6 +341,15 @@ void fem_assembly_free(fem_assembly* a);
 int fem_hex_element(const double* size, double e, double nu, double alpha, double* k, double* load, char* err,
                     size_t errlen);
 
+/* hex.rs:313-340 Hex::strain_stress of the same element: the engineering strain (xx, yy, zz, yz,
+ * xz, xy) and the stress in the same order at the local point xi[3] (each coordinate -1 to 1) of
+ * the cell, from its corner displacements u[24] (3 corner + component) and corner temperature
+ * changes t[8]: the incompatible modes' amplitudes recovered from u and t, the strain of the 11
+ * functions, D times it less the thermal stress of the interpolated temperature change. Host
+ * code. The probes' stresses of the case-file front end (elastic.rs:676-700). */
+int fem_hex_stress(const double* size, double e, double nu, double alpha, const double* xi, const double* u,
+                   const double* t, double* strain, double* stress, char* err, size_t errlen);
+
 /* ---- a dense direct solve on device memory ---- */
 
 /* x = A^-1 b for a dense symmetric positive definite n x n matrix A: the device's dense FP64

@@ -28,6 +28,7 @@ CASES = os.path.join(os.path.dirname(__file__), "golden", "cases")
 PRINTED = {
     ("grid-heated-bar", "centre"): (14.745623, 0.5e-6),      # max_cell 0.625 mm
     ("grid-cantilever", "tip.uz"): (-0.190113, 0.5e-6),      # max_cell 2.5 mm
+    ("grid-cantilever", "middle_top.sxx"): (29.25, 0.5e-7),   # "the grid gives 29.2500000 MPa"
     # max_cell 1.25 mm, 52,344 unknowns, values in 1e-6 m to 4 decimals
     ("grid-heated-block-stress-ccx", "block_top.uz"): (12.7391e-6, 0.5e-10),
     ("grid-heated-block-stress-ccx", "plate_corner.ux"): (10.2845e-6, 0.5e-10),
@@ -51,6 +52,9 @@ def test_reference_case_passes_its_check(fem, case_dir, precision):
     for key, column, got, ref, limit, ok in rows:
         print(f"  {key}: {column} = {got!r} (reference {ref}, limit {limit:.3g})")
         assert ok, (key, got, ref, limit)
+    # every entry of the check that names this case is computed (probes, stresses, surfaces)
+    doc = grid_case._toml.loads(open(os.path.join(CASES, folder, "check.toml")).read())
+    assert len(rows) == sum(1 for v in doc.values() if isinstance(v, dict) and v.get("case") == case.name)
     for (c, column), (value, half) in PRINTED.items():
         if c == folder:
             assert abs(res.probes[column] - value) <= half, (column, res.probes[column], value)
@@ -115,4 +119,6 @@ def test_bimetal_strip_reproduces_the_references_mesh_study(fem, max_cell):
     if max_cell == 1.0:
         with open(os.path.join(CASES, "grid-bimetal-strip", "check.toml")) as f:
             rows = grid_case.check(res, f.read(), case.name)
-        assert [r[1] for r in rows] == ["top.kxx", "top.kyy"] and all(r[-1] for r in rows)
+        # the surface's curvatures, and the reactions of the face that holds uz (elastic.rs:773-839):
+        # the thermal load has no net force, so they are 0 to abs_tol
+        assert [r[1] for r in rows] == ["top.kxx", "top.kyy", "centre.fz", "centre.fx"] and all(r[-1] for r in rows)

@@ -117,7 +117,7 @@ def test_probe_at_a_grid_point_reads_that_point():
     c = case("grid-cantilever/beam")
     g, _, mids = built(c)
     _, _, _, number_of = gc.number(g)
-    w = {name: pairs for name, pairs, _ in gc.probe_weights(c, g, mids, number_of)}
+    w = {name: pairs for name, pairs, *_ in gc.probe_weights(c, g, mids, number_of)}
     # the graded planes are placed by the size function's integral, so a plane is at its
     # coordinate to rounding: the weights are 1 and 0 to 1e-12
     tip = {v: wt for v, wt in w["tip"] if abs(wt) > 1e-12}

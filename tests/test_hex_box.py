@@ -105,3 +105,57 @@ def test_hex_thermal_load_is_free_expansion(fem, size, alpha):
     assert np.abs(k @ u - f).max() <= 1e-12 * np.abs(f).max()
     # and each corner's column is the load of a change at that corner alone: the 8 sum to the free expansion
     assert np.abs(load.sum(axis=1) - f).max() <= 1e-15 * np.abs(f).max()
+
+
+VTK_BITS = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+
+
+def _corners(size):
+    return np.array([[b[d] * size[d] for d in range(3)] for b in VTK_BITS])
+
+
+def test_strain_stress_of_closed_forms(fem):
+    """hex.rs:313-340 on fields with known answers: a uniform strain gives that strain and D times
+    it at any point (the modes stay 0); a free expansion alpha dT x and a rigid rotation give no
+    stress."""
+    size, e, nu, alpha = [2.0, 1.0, 0.5], 200.0, 0.3, 1e-3
+    x = _corners(size)
+    grad = np.array([[1e-3, 2e-3, -1e-3], [0.5e-3, -2e-3, 1e-3], [0.0, 3e-3, 1.5e-3]])
+    u = (x @ grad.T).ravel()
+    lam, mu = e * nu / ((1 + nu) * (1 - 2 * nu)), e / (2 * (1 + nu))
+    eps = np.array([grad[0, 0], grad[1, 1], grad[2, 2], grad[1, 2] + grad[2, 1], grad[0, 2] + grad[2, 0],
+                    grad[0, 1] + grad[1, 0]])
+    tr = eps[:3].sum()
+    sig = np.concatenate([2 * mu * eps[:3] + lam * tr, mu * eps[3:]])
+    for xi in ([0.0, 0.0, 0.0], [0.3, -0.7, 1.0], [-1.0, -1.0, -1.0]):
+        strain, stress = fem.hex_strain_stress(size, e, nu, alpha, xi, u, np.zeros(8))
+        assert np.allclose(strain, eps, rtol=0, atol=1e-15) and np.allclose(stress, sig, rtol=1e-12, atol=1e-12)
+    dt = 50.0
+    free = (alpha * dt * x).ravel()
+    _, stress = fem.hex_strain_stress(size, e, nu, alpha, [0.2, 0.1, -0.4], free, np.full(8, dt))
+    assert np.abs(stress).max() < 1e-12
+    w = np.array([0.3, -0.2, 0.1])
+    rot = np.cross(w, x).ravel()
+    strain, stress = fem.hex_strain_stress(size, e, nu, alpha, [0.5, 0.5, 0.5], rot, np.zeros(8))
+    assert np.abs(strain).max() < 1e-15 and np.abs(stress).max() < 1e-13
+
+
+def test_strain_stress_is_the_elements_virtual_work(fem):
+    """The modes recovered by strain_stress are the ones the stiffness and the thermal load condense
+    (hex.rs:199-279): on the element's own 2 x 2 x 2 Gauss rule, the virtual work of the stress of
+    (u, t) on the strain of du (t = 0) is du . (K u - L t), for random u, t and du. A wrong sign
+    or index in the mode recovery breaks it."""
+    size, e, nu, alpha = [1.5, 0.8, 1.2], 70.0, 0.33, 2e-3
+    k, load = fem.hex_element(size, e, nu, alpha)
+    rng = np.random.default_rng(11)
+    u, du, t = rng.standard_normal(24), rng.standard_normal(24), rng.standard_normal(8)
+    p = 1.0 / np.sqrt(3.0)
+    jac = size[0] * size[1] * size[2] / 8.0
+    work = 0.0
+    for b in VTK_BITS:
+        xi = [p if v else -p for v in b]
+        _, s = fem.hex_strain_stress(size, e, nu, alpha, xi, u, t)
+        ed, _ = fem.hex_strain_stress(size, e, nu, alpha, xi, du, np.zeros(8))
+        work += jac * float(s @ ed)
+    expect = float(du @ (k @ u - load @ t))
+    assert abs(work - expect) <= 1e-11 * (abs(expect) + np.abs(k).max())
