@@ -130,7 +130,8 @@ class _Unknowns(C.Structure):
 class _Mesh(C.Structure):
     _fields_ = [("kind", C.c_int32), ("points", C.c_uint64), ("xyz", C.c_void_p), ("cells", C.c_uint64),
                 ("conn", C.c_void_p), ("coef0", C.c_void_p), ("coef1", C.c_void_p), ("source", C.c_double),
-                ("body", C.c_double * 3), ("fixed", C.c_void_p), ("value", C.c_void_p), ("load", C.c_void_p)]
+                ("body", C.c_double * 3), ("fixed", C.c_void_p), ("value", C.c_void_p), ("load", C.c_void_p),
+                ("ortho", C.c_void_p)]
 
 
 class AssemblyStats(C.Structure):
@@ -259,9 +260,8 @@ def lib() -> C.CDLL:
         "fem_test_assembly_pattern": (C.c_int, [C.POINTER(_Mesh), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), P, P,
                                                 P, ERR, SZ]),
         "fem_test_hex_element": (C.c_int, [F64P, C.c_double, C.c_double, F64P, ERR, SZ]),
-        "fem_hex_element": (C.c_int, [F64P, C.c_double, C.c_double, C.c_double, F64P, F64P, ERR, SZ]),
-        "fem_hex_stress": (C.c_int, [F64P, C.c_double, C.c_double, C.c_double, F64P, F64P, F64P, F64P, F64P, ERR,
-                                     SZ]),
+        "fem_hex_element": (C.c_int, [F64P, F64P, F64P, F64P, F64P, ERR, SZ]),
+        "fem_hex_stress": (C.c_int, [F64P, F64P, F64P, F64P, F64P, F64P, F64P, F64P, ERR, SZ]),
         "fem_test_supernodal": (C.c_int, [C.POINTER(_Csr), P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), P, P, P, P,
                                           P, ERR, SZ]),
         "fem_test_hex_box_system": (C.c_int, [C.POINTER(_Box), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), P, P, P,
@@ -871,6 +871,7 @@ class Mesh:
     source: float = 0.0
     body: Sequence[float] = (0.0, 0.0, 0.0)
     load: Optional[np.ndarray] = None
+    ortho: Optional[np.ndarray] = None  # elasticity: per cell the 9 constants of material9, for coef0 / coef1
 
     def _struct(self):
         if self.kind not in MESH_KINDS:
@@ -878,10 +879,15 @@ class Mesh:
         keep = {
             "xyz": np.ascontiguousarray(self.xyz, np.float64).reshape(-1),
             "conn": np.ascontiguousarray(self.cells, np.uint32).reshape(-1),
-            "coef0": np.ascontiguousarray(self.coef0, np.float64).reshape(-1),
             "fixed": np.ascontiguousarray(self.fixed, np.uint8).reshape(-1),
             "value": np.ascontiguousarray(self.value, np.float64).reshape(-1),
         }
+        if self.coef0 is not None:
+            keep["coef0"] = np.ascontiguousarray(self.coef0, np.float64).reshape(-1)
+        if self.ortho is not None:
+            keep["ortho"] = np.ascontiguousarray(self.ortho, np.float64).reshape(-1)
+            if keep["ortho"].size != 9 * np.asarray(self.cells).shape[0]:
+                raise FemError("ortho needs 9 constants per cell")
         if self.coef1 is not None:
             keep["coef1"] = np.ascontiguousarray(self.coef1, np.float64).reshape(-1)
         if self.load is not None:
@@ -964,26 +970,46 @@ def assembly_pattern(mesh: Mesh):
     return ptr, idx[:ne.value], free[:nf.value]
 
 
-def hex_element(size, e: float, nu: float, alpha: Optional[float] = None):
-    """The condensed incompatible-mode hex of a box cell (hex.rs:199-279, fem_hex_element), 24 x 24;
-    with an expansion coefficient `alpha`, also the load of a unit temperature change at each corner
-    (24 x 8): (k, load). Host code, no device."""
+def material9(e, nu, g=None) -> np.ndarray:
+    """The 9 constants of an orthotropic material with its axes on x, y and z (hex.rs:47-64): E_x,
+    E_y, E_z, nu_xy, nu_xz, nu_yz, G_xy, G_xz, G_yz. Scalar e and nu: the isotropic material (E, nu
+    and E / (2 (1 + nu)) on every axis); 3 values each: orthotropic, with the shear moduli g."""
+    if np.ndim(e) == 0 and np.ndim(nu) == 0 and g is None:
+        e, nu = float(e), float(nu)
+        gi = e / (2.0 * (1.0 + nu))
+        return np.array([e, e, e, nu, nu, nu, gi, gi, gi])
+    if g is None or np.size(e) != 3 or np.size(nu) != 3 or np.size(g) != 3:
+        raise FemError("an orthotropic material needs 3 Young's moduli, 3 Poisson ratios and 3 shear moduli")
+    return np.concatenate([np.ravel(e), np.ravel(nu), np.ravel(g)]).astype(np.float64)
+
+
+def _alpha3(alpha) -> np.ndarray:
+    if alpha is None:
+        return np.zeros(3)
+    return np.full(3, float(alpha)) if np.ndim(alpha) == 0 else np.ascontiguousarray(alpha, np.float64).reshape(3)
+
+
+def hex_element(size, e, nu, alpha=None, g=None):
+    """The condensed incompatible-mode hex of a box cell (hex.rs:199-279, fem_hex_element), 24 x 24,
+    of an isotropic (scalar e, nu) or orthotropic (3 values each, and g) material; with expansion
+    coefficients `alpha` (a number, or 3 along x, y and z), also the load of a unit temperature
+    change at each corner (24 x 8): (k, load). Host code, no device."""
     k = np.zeros(576)
     load = np.zeros(192)
     err = _err()
-    if lib().fem_hex_element(np.ascontiguousarray(size, np.float64), float(e), float(nu),
-                             0.0 if alpha is None else float(alpha), k, load, err, 2048):
+    if lib().fem_hex_element(np.ascontiguousarray(size, np.float64), material9(e, nu, g), _alpha3(alpha), k, load, err,
+                             2048):
         raise FemError(err.value.decode())
     return k.reshape(24, 24) if alpha is None else (k.reshape(24, 24), load.reshape(24, 8))
 
 
-def hex_strain_stress(size, e: float, nu: float, alpha: float, xi, u, t):
+def hex_strain_stress(size, e, nu, alpha, xi, u, t, g=None):
     """hex.rs:313-340 strain_stress of the condensed hex (fem_hex_stress): the engineering strain and
     the stress (xx, yy, zz, yz, xz, xy) at the local point xi from the corner displacements u (24)
     and temperature changes t (8). Host code, no device."""
     strain, stress = np.zeros(6), np.zeros(6)
     err = _err()
-    if lib().fem_hex_stress(np.ascontiguousarray(size, np.float64), float(e), float(nu), float(alpha),
+    if lib().fem_hex_stress(np.ascontiguousarray(size, np.float64), material9(e, nu, g), _alpha3(alpha),
                             np.ascontiguousarray(xi, np.float64), np.ascontiguousarray(u, np.float64),
                             np.ascontiguousarray(t, np.float64), strain, stress, err, 2048):
         raise FemError(err.value.decode())

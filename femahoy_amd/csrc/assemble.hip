@@ -257,9 +257,11 @@ struct HostPlan {
 void validate(const fem_mesh& m) {
     if (m.kind < FEM_MESH_HEAT_HEX8 || m.kind > FEM_MESH_ELASTIC_HEX8) throw Error("unknown mesh kind");
     if (m.points == 0 || m.cells == 0) throw Error("the mesh needs points and cells");
-    if (!m.xyz || !m.conn || !m.coef0 || !m.fixed || !m.value) throw Error("the mesh needs xyz, conn, coef0, fixed and value");
-    if ((m.kind == FEM_MESH_ELASTIC_TET10 || m.kind == FEM_MESH_ELASTIC_HEX8) && !m.coef1)
-        throw Error("elasticity needs Poisson's ratio per cell (coef1)");
+    const bool elastic = m.kind == FEM_MESH_ELASTIC_TET10 || m.kind == FEM_MESH_ELASTIC_HEX8;
+    if (!m.xyz || !m.conn || !m.fixed || !m.value) throw Error("the mesh needs xyz, conn, fixed and value");
+    if (!m.coef0 && !(elastic && m.ortho)) throw Error("the mesh needs coef0 (or, for elasticity, ortho)");
+    if (elastic && !m.coef1 && !m.ortho) throw Error("elasticity needs Poisson's ratio per cell (coef1)");
+    if (m.ortho && !elastic) throw Error("orthotropic constants are for elasticity");
     const int npc = points_per_cell(m.kind);
     const uint64_t m2 = (uint64_t)(npc * comps_of(m.kind)) * (uint64_t)(npc * comps_of(m.kind));
     if (m.cells * m2 >= (uint64_t)NONE32) throw Error("the mesh has too many cells for 32-bit element slots");
@@ -387,7 +389,7 @@ void plan(const fem_mesh& mesh, HostPlan& P, bool maps) {
     group_by_target(P.eu, n, P.lptr, P.lslot);
     if (P.kind == FEM_MESH_ELASTIC_HEX8) {
         // the element of each distinct (sizes, material): sizes from corners 0 and 6 (grid.rs:195-198)
-        std::map<std::array<uint64_t, 5>, uint32_t> seen;
+        std::map<std::array<uint64_t, 12>, uint32_t> seen;
         P.which.resize(P.cells);
         P.table.clear();
         auto bits = [](double v) {
@@ -399,14 +401,18 @@ void plan(const fem_mesh& mesh, HostPlan& P, bool maps) {
             const uint32_t c0 = mesh.conn[8 * e], c6 = mesh.conn[8 * e + 6];
             std::array<double, 3> size;
             for (int d = 0; d < 3; ++d) size[d] = mesh.xyz[3 * (size_t)c6 + d] - mesh.xyz[3 * (size_t)c0 + d];
-            const std::array<uint64_t, 5> key = {bits(size[0]), bits(size[1]), bits(size[2]), bits(mesh.coef0[e]),
-                                                 bits(mesh.coef1[e])};
+            std::array<uint64_t, 12> key = {bits(size[0]), bits(size[1]), bits(size[2])};
+            if (mesh.ortho)
+                for (int q = 0; q < 9; ++q) key[3 + q] = bits(mesh.ortho[9 * e + q]);
+            else
+                key[3] = bits(mesh.coef0[e]), key[4] = bits(mesh.coef1[e]);
             auto it = seen.find(key);
             if (it == seen.end()) {
                 if (!(size[0] > 0.0 && size[1] > 0.0 && size[2] > 0.0))
                     throw Error("a hex cell needs corner 6 above corner 0 along every axis");
                 it = seen.emplace(key, (uint32_t)(P.table.size() / 576)).first;
-                const auto k = hex_element(size, isotropic_stiffness(mesh.coef0[e], mesh.coef1[e]));
+                const auto k = hex_element(size, mesh.ortho ? orthotropic_stiffness(mesh.ortho + 9 * e)
+                                                            : isotropic_stiffness(mesh.coef0[e], mesh.coef1[e]));
                 P.table.insert(P.table.end(), k.begin(), k.end());
             }
             P.which[e] = it->second;
@@ -415,7 +421,8 @@ void plan(const fem_mesh& mesh, HostPlan& P, bool maps) {
     if (P.kind == FEM_MESH_ELASTIC_TET10) {
         P.dmat.resize(P.cells * 12);
         for (size_t e = 0; e < P.cells; ++e) {
-            const auto d = isotropic_stiffness(mesh.coef0[e], mesh.coef1[e]);
+            const auto d = mesh.ortho ? orthotropic_stiffness(mesh.ortho + 9 * e)
+                                      : isotropic_stiffness(mesh.coef0[e], mesh.coef1[e]);
             std::copy(d.begin(), d.end(), P.dmat.begin() + (ptrdiff_t)(e * 12));
         }
     }
@@ -493,7 +500,9 @@ class Assembly {
         // the mesh and the maps to the device (the uploads are outside the device time)
         DArray<double> xyz = up(mesh.xyz, mesh.points * 3, s);
         DArray<uint32_t> conn = up(mesh.conn, nc * npc, s);
-        DArray<double> coef = P.kind == FEM_MESH_ELASTIC_TET10 ? up(P.dmat, s) : up(mesh.coef0, nc, s);  // unused by hex8 elasticity
+        // the D matrices of tet10, the conductivities of heat; hex8 elasticity reads its element table
+        DArray<double> coef = P.kind == FEM_MESH_ELASTIC_TET10 ? up(P.dmat, s)
+                              : P.kind == FEM_MESH_ELASTIC_HEX8 ? DArray<double>(1) : up(mesh.coef0, nc, s);
         DArray<double> body = up(mesh.body, 3, s);
         DArray<uint8_t> fixed = up(mesh.fixed, n, s);
         DArray<double> value = up(mesh.value, n, s);

@@ -90,9 +90,18 @@ uint64_t bits_of(double v) {
 
 std::array<double, 12> isotropic_stiffness(double e, double nu) {
     const double g = e / (2.0 * (1.0 + nu));
-    const bool moduli = std::isfinite(e) && e > 0.0 && std::isfinite(g) && g > 0.0;
-    if (!moduli || !std::isfinite(nu)) throw Error("the moduli must be above 0 and the Poisson ratios finite");
-    const double s[3][3] = {{1.0 / e, -nu / e, -nu / e}, {-nu / e, 1.0 / e, -nu / e}, {-nu / e, -nu / e, 1.0 / e}};
+    const double m[9] = {e, e, e, nu, nu, nu, g, g, g};
+    return orthotropic_stiffness(m);
+}
+
+std::array<double, 12> orthotropic_stiffness(const double m[9]) {
+    const double ex = m[0], ey = m[1], ez = m[2], nxy = m[3], nxz = m[4], nyz = m[5];
+    bool moduli = true;
+    for (int i : {0, 1, 2, 6, 7, 8}) moduli = moduli && std::isfinite(m[i]) && m[i] > 0.0;
+    if (!moduli || !std::isfinite(nxy) || !std::isfinite(nxz) || !std::isfinite(nyz))
+        throw Error("the moduli must be above 0 and the Poisson ratios finite");
+    const double s[3][3] = {{1.0 / ex, -nxy / ex, -nxz / ex}, {-nxy / ex, 1.0 / ey, -nyz / ey},
+                            {-nxz / ex, -nyz / ey, 1.0 / ez}};
     const double det = s[0][0] * (s[1][1] * s[2][2] - s[1][2] * s[2][1]) -
                        s[0][1] * (s[1][0] * s[2][2] - s[1][2] * s[2][0]) +
                        s[0][2] * (s[1][0] * s[2][1] - s[1][1] * s[2][0]);
@@ -106,35 +115,37 @@ std::array<double, 12> isotropic_stiffness(double e, double nu) {
             const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
             d[3 * i + j] = (s[j1][i1] * s[j2][i2] - s[j1][i2] * s[j2][i1]) / det;
         }
-    d[9] = g;   // G_yz
-    d[10] = g;  // G_xz
-    d[11] = g;  // G_xy
+    d[9] = m[8];   // G_yz
+    d[10] = m[7];  // G_xz
+    d[11] = m[6];  // G_xy
     return d;
 }
 
 std::array<double, 576> hex_element(const std::array<double, 3>& size, const std::array<double, 12>& d) {
     std::array<double, 576> k;
     std::array<double, 192> load;
-    hex_element_thermal(size, d, 0.0, k, load);
+    hex_element_thermal(size, d, {0.0, 0.0, 0.0}, k, load);
     return k;
 }
 
 namespace {
 // hex.rs:199-279: the element with its mode maps, mode_u (9 x 24) and mode_t (9 x 8): the modes'
 // amplitudes are mode_t t - mode_u u.
-void hex_element_modes(const std::array<double, 3>& size, const std::array<double, 12>& d, double alpha,
-                       std::array<double, 576>& out, std::array<double, 192>& lout, double x[9][24],
-                       double xt[9][8], double thermal[3]);
+void hex_element_modes(const std::array<double, 3>& size, const std::array<double, 12>& d,
+                       const std::array<double, 3>& alpha, std::array<double, 576>& out, std::array<double, 192>& lout,
+                       double x[9][24], double xt[9][8], double thermal[3]);
 }  // namespace
 
-void hex_element_thermal(const std::array<double, 3>& size, const std::array<double, 12>& d, double alpha,
-                         std::array<double, 576>& out, std::array<double, 192>& lout) {
+void hex_element_thermal(const std::array<double, 3>& size, const std::array<double, 12>& d,
+                         const std::array<double, 3>& alpha, std::array<double, 576>& out,
+                         std::array<double, 192>& lout) {
     double x[9][24], xt[9][8], thermal[3];
     hex_element_modes(size, d, alpha, out, lout, x, xt, thermal);
 }
 
-void hex_strain_stress(const std::array<double, 3>& size, const std::array<double, 12>& d, double alpha,
-                       const double xi[3], const double u[24], const double t[8], double strain[6], double stress[6]) {
+void hex_strain_stress(const std::array<double, 3>& size, const std::array<double, 12>& d,
+                       const std::array<double, 3>& alpha, const double xi[3], const double u[24], const double t[8],
+                       double strain[6], double stress[6]) {
     std::array<double, 576> k;
     std::array<double, 192> load;
     double x[9][24], xt[9][8], thermal[3];
@@ -171,15 +182,15 @@ void hex_strain_stress(const std::array<double, 3>& size, const std::array<doubl
 }
 
 namespace {
-void hex_element_modes(const std::array<double, 3>& size, const std::array<double, 12>& d, double alpha,
-                       std::array<double, 576>& out, std::array<double, 192>& lout, double x[9][24],
-                       double xt[9][8], double thermal[3]) {
+void hex_element_modes(const std::array<double, 3>& size, const std::array<double, 12>& d,
+                       const std::array<double, 3>& alpha, std::array<double, 576>& out, std::array<double, 192>& lout,
+                       double x[9][24], double xt[9][8], double thermal[3]) {
     const double* normal = d.data();
     const double* shear = d.data() + 9;
     // the thermal stress of a unit temperature change: D alpha on the normal stresses (a sum from 0)
     for (int i = 0; i < 3; ++i) {
         double t = 0.0;
-        for (int j = 0; j < 3; ++j) t += normal[3 * i + j] * alpha;
+        for (int j = 0; j < 3; ++j) t += normal[3 * i + j] * alpha[j];
         thermal[i] = t;
     }
     std::vector<double> lbuf(33 * 8, 0.0);

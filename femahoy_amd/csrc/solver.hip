@@ -3208,24 +3208,27 @@ int fem_test_hex_element(const double* size, double e, double nu, double* out, c
     });
 }
 
-int fem_hex_element(const double* size, double e, double nu, double alpha, double* k, double* load, char* err,
-                    size_t errlen) {
+int fem_hex_element(const double* size, const double* material, const double* alpha, double* k, double* load,
+                    char* err, size_t errlen) {
     return guarded(err, errlen, [&] {
-        if (!size || !k) throw fem::Error("the sizes and the stiffness output are needed");
+        if (!size || !material || !k) throw fem::Error("the sizes, the material and the stiffness output are needed");
         std::array<double, 576> kk;
         std::array<double, 192> ll;
-        fem::hex_element_thermal({size[0], size[1], size[2]}, fem::isotropic_stiffness(e, nu), alpha, kk, ll);
+        const std::array<double, 3> a = alpha ? std::array<double, 3>{alpha[0], alpha[1], alpha[2]}
+                                              : std::array<double, 3>{0.0, 0.0, 0.0};
+        fem::hex_element_thermal({size[0], size[1], size[2]}, fem::orthotropic_stiffness(material), a, kk, ll);
         std::copy(kk.begin(), kk.end(), k);
         if (load) std::copy(ll.begin(), ll.end(), load);
     });
 }
 
-int fem_hex_stress(const double* size, double e, double nu, double alpha, const double* xi, const double* u,
-                   const double* t, double* strain, double* stress, char* err, size_t errlen) {
+int fem_hex_stress(const double* size, const double* material, const double* alpha, const double* xi,
+                   const double* u, const double* t, double* strain, double* stress, char* err, size_t errlen) {
     return guarded(err, errlen, [&] {
-        if (!size || !xi || !u || !t || !strain || !stress) throw fem::Error("every argument is needed");
-        fem::hex_strain_stress({size[0], size[1], size[2]}, fem::isotropic_stiffness(e, nu), alpha, xi, u, t, strain,
-                               stress);
+        if (!size || !material || !alpha || !xi || !u || !t || !strain || !stress)
+            throw fem::Error("every argument is needed");
+        fem::hex_strain_stress({size[0], size[1], size[2]}, fem::orthotropic_stiffness(material),
+                               {alpha[0], alpha[1], alpha[2]}, xi, u, t, strain, stress);
     });
 }
 

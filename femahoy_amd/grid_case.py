@@ -15,13 +15,14 @@ on the device (`fem_assemble`), the solve is the library's (`fem_solver_new_csr`
 level, and the cells for a displacement).
 
 Keys read: `units`; `[grid]` max_cell, max_growth, min_cells, symmetry; `[material.*]`
-conductivity (a number; with density and specific_heat for a transient step) or youngs_modulus,
-poisson_ratio, expansion and conductivity (numbers); `[part.*]` material, x, y, z, priority;
+conductivity (a number; with density and specific_heat for a transient step) or youngs_modulus and
+poisson_ratio (1 number each, or 3 each with shear_modulus: orthotropic), expansion (1 or 3) and
+conductivity; `[part.*]` material, x, y, z, priority;
 `[face.*]` side, box, parts; `[[load]]` temperature, source, flux, convection with sink (heat, and
 the temperature of a static step), ux / uy / uz on a face or a point, traction, pressure (static);
 `[step]` type "steady", "transient" (end, dt, initial) or "static" (reference_temperature,
-temperature); `[probe.*]` point; `[surface.*]` face. Anything else (tables, orthotropic
-materials, footprints, radiation, gravity, Joule heat and other steps) is refused with a message.
+temperature); `[probe.*]` point; `[surface.*]` face. Anything else (tables, footprints, radiation,
+gravity, Joule heat and other steps) is refused with a message.
 """
 from __future__ import annotations
 
@@ -112,6 +113,40 @@ def _numbers(v, n, where, key):
     return [float(x) for x in v]
 
 
+def _elastic_constants(m, where):
+    """core grid.rs:1326-1345: youngs_modulus and poisson_ratio as 1 number each (isotropic), or
+    youngs_modulus (x, y, z), poisson_ratio (xy, xz, yz) and shear_modulus (xy, xz, yz) as 3 numbers
+    each (orthotropic); the 9 constants E_x, E_y, E_z, nu_xy, nu_xz, nu_yz, G_xy, G_xz, G_yz."""
+    def values(key):
+        v = m.get(key)
+        if v is None:
+            return None
+        v = v if isinstance(v, list) else [v]
+        if not all(isinstance(x, (int, float)) and not isinstance(x, bool) for x in v):
+            return []
+        return [float(x) for x in v]
+    y, p, g = values("youngs_modulus"), values("poisson_ratio"), values("shear_modulus")
+    if y is not None and p is not None and g is None and len(y) == 1 and len(p) == 1:
+        gi = y[0] / (2.0 * (1.0 + p[0]))
+        return [y[0]] * 3 + [p[0]] * 3 + [gi] * 3
+    if y is not None and p is not None and g is not None and len(y) == len(p) == len(g) == 3:
+        return y + p + g
+    raise CaseError(f"{where}: give youngs_modulus and poisson_ratio as 1 number each (isotropic), or "
+                    "youngs_modulus (x, y, z), poisson_ratio (xy, xz, yz) and shear_modulus (xy, xz, yz) as 3 "
+                    "numbers each (orthotropic)")
+
+
+def _expansion(m, where):
+    """core grid.rs:1347-1355: the expansion along x, y and z, 0 when the material has none."""
+    v = m.get("expansion")
+    if v is None:
+        return [0.0, 0.0, 0.0]
+    v = v if isinstance(v, list) else [v]
+    if not all(isinstance(x, (int, float)) and not isinstance(x, bool) for x in v) or len(v) not in (1, 3):
+        raise CaseError(f"{where}: missing key `expansion`")
+    return [float(x) for x in v] * (3 if len(v) == 1 else 1)
+
+
 def read(text: str, name: str = "") -> Case:
     """The case of a grid.toml text (core/src/grid.rs:79-152 `read`, :153 `read_heat`), for the
     keys this front end reads."""
@@ -158,10 +193,10 @@ def read(text: str, name: str = "") -> Case:
             materials.append({"name": mname, "conductivity": _number(m, "conductivity", where),
                               "capacity": _number(m, "density", where) * _number(m, "specific_heat", where)})
         else:
-            _only(m, where, ["youngs_modulus", "poisson_ratio", "density", "expansion", "conductivity"])
-            materials.append({"name": mname, "youngs_modulus": _number(m, "youngs_modulus", where),
-                              "poisson_ratio": _number(m, "poisson_ratio", where),
-                              "expansion": _number(m, "expansion", where) if "expansion" in m else 0.0,
+            _only(m, where, ["youngs_modulus", "poisson_ratio", "shear_modulus", "density", "expansion",
+                             "conductivity"])
+            materials.append({"name": mname, "elastic": _elastic_constants(m, where),
+                              "expansion": _expansion(m, where),
                               "conductivity": _number(m, "conductivity", where) if "conductivity" in m else None})
         mindex[mname] = len(materials) - 1
     parts = []
@@ -970,8 +1005,7 @@ def _solve_elastic(case, precision, method, tol_, device):
     n = len(ijk)
     xyz = np.array([[grid.planes[d][p[d]] for d in range(3)] for p in ijk])
     mat = [case.materials[case.parts[int(grid.part[c])].material] for c in cell_of]
-    e_mod = np.array([m["youngs_modulus"] for m in mat])
-    nu = np.array([m["poisson_ratio"] for m in mat])
+    ortho = np.array([m["elastic"] for m in mat])
     # each distinct (cell size, material) once: the element and its thermal load
     table = {}
     which = []
@@ -979,7 +1013,8 @@ def _solve_elastic(case, precision, method, tol_, device):
         key = (tuple(grid.size(c)), case.parts[int(grid.part[c])].material)
         if key not in table:
             m = case.materials[key[1]]
-            table[key] = fem.hex_element(key[0], m["youngs_modulus"], m["poisson_ratio"], m["expansion"])
+            el = m["elastic"]
+            table[key] = fem.hex_element(key[0], el[0:3], el[3:6], m["expansion"], g=el[6:9])
         which.append(key)
     dt = _thermal_change(case, n, precision, tol_, device)
     force = np.zeros(3 * n)
@@ -1039,7 +1074,7 @@ def _solve_elastic(case, precision, method, tol_, device):
                 fixed[3 * i + d] = 0.0
     isfixed = np.array([f is not None for f in fixed], np.uint8)
     value = np.array([f if f is not None else 0.0 for f in fixed])
-    mesh = fem.Mesh("elastic_hex8", xyz, cells, e_mod, isfixed, value, coef1=nu, load=force)
+    mesh = fem.Mesh("elastic_hex8", xyz, cells, None, isfixed, value, load=force, ortho=ortho)
     asm = fem.Assembly(mesh, device)
     ptr, idx, val, rhs, free = asm.arrays()
     planes = [np.asarray(p, np.float64) for p in grid.planes]
@@ -1068,8 +1103,8 @@ def _solve_elastic(case, precision, method, tol_, device):
         ue = np.array([u[3 * node + c] for node in nodes for c in range(3)])
         te = dt[nodes] if dt is not None else np.zeros(8)
         m = case.materials[case.parts[int(grid.part[cell])].material]
-        _, sg = fem.hex_strain_stress(grid.size(cell), m["youngs_modulus"], m["poisson_ratio"], m["expansion"], xi,
-                                      ue, te)
+        el = m["elastic"]
+        _, sg = fem.hex_strain_stress(grid.size(cell), el[0:3], el[3:6], m["expansion"], xi, ue, te, g=el[6:9])
         sig = [sg[0], sg[1], sg[2], sg[5], sg[3], sg[4]]
         for a in range(3):
             if mirrored[a]:

@@ -299,6 +299,8 @@ typedef struct fem_mesh {
     const uint8_t* fixed;    /* per unknown: 1 when held */
     const double* value;     /* per unknown: its held value (read where fixed) */
     const double* load;      /* per unknown: a further load (face loads) added after the volume loads; may be NULL */
+    const double* ortho;     /* elasticity: per cell 9 orthotropic constants (E_x, E_y, E_z, nu_xy, nu_xz, nu_yz,
+                                G_xy, G_xz, G_yz; hex.rs:47-115) in place of coef0 and coef1; may be NULL */
 } fem_mesh;
 
 /* The assembled system of a mesh on the device (no counterpart type in the reference). */
@@ -333,13 +335,16 @@ void fem_assembly_device(const fem_assembly* a, const uint64_t** ptr, const uint
 void fem_assembly_free(fem_assembly* a);
 
 /* The condensed incompatible-mode hex of an axis-aligned box cell with sides size[3] (hex.rs:199-279
- * Hex::new), isotropic: its stiffness k (24 x 24) and, for the expansion coefficient alpha, the
- * load of a unit temperature change at each corner (24 x 8; may be NULL), row-major, unknown
- * 3 corner + component, corners in VTK order. The element that FEM_ELASTIC_HEX8 boxes and
- * FEM_MESH_ELASTIC_HEX8 meshes assemble, for callers that build their own cells and thermal loads
- * (the case-file front end). Material errors use the texts of hex.rs:79 and hex.rs:93. */
-int fem_hex_element(const double* size, double e, double nu, double alpha, double* k, double* load, char* err,
-                    size_t errlen);
+ * Hex::new) of an orthotropic material with its axes on x, y and z (hex.rs:47-115 Orthotropic):
+ * material[9] = E_x, E_y, E_z, nu_xy, nu_xz, nu_yz, G_xy, G_xz, G_yz (an isotropic material: E, nu
+ * and E / (2 (1 + nu)) on every axis). Its stiffness k (24 x 24) and, for the expansion
+ * coefficients alpha[3] along x, y and z (may be NULL: none), the load of a unit temperature change
+ * at each corner (24 x 8; may be NULL), row-major, unknown 3 corner + component, corners in VTK
+ * order. The element that FEM_ELASTIC_HEX8 boxes and FEM_MESH_ELASTIC_HEX8 meshes assemble, for
+ * callers that build their own cells and thermal loads (the case-file front end). Material errors
+ * use the texts of hex.rs:79 and hex.rs:93. */
+int fem_hex_element(const double* size, const double* material, const double* alpha, double* k, double* load,
+                    char* err, size_t errlen);
 
 /* hex.rs:313-340 Hex::strain_stress of the same element: the engineering strain (xx, yy, zz, yz,
  * xz, xy) and the stress in the same order at the local point xi[3] (each coordinate -1 to 1) of
@@ -347,8 +352,8 @@ int fem_hex_element(const double* size, double e, double nu, double alpha, doubl
  * changes t[8]: the incompatible modes' amplitudes recovered from u and t, the strain of the 11
  * functions, D times it less the thermal stress of the interpolated temperature change. Host
  * code. The probes' stresses of the case-file front end (elastic.rs:676-700). */
-int fem_hex_stress(const double* size, double e, double nu, double alpha, const double* xi, const double* u,
-                   const double* t, double* strain, double* stress, char* err, size_t errlen);
+int fem_hex_stress(const double* size, const double* material, const double* alpha, const double* xi,
+                   const double* u, const double* t, double* strain, double* stress, char* err, size_t errlen);
 
 /* ---- a dense direct solve on device memory ---- */
 

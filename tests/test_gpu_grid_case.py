@@ -38,7 +38,8 @@ PRINTED = {
 
 
 @pytest.mark.parametrize("case_dir", ["grid-heated-bar/bar", "grid-cantilever/beam", "grid-layered-wall/wall",
-                                      "grid-heated-block-ccx/block", "grid-heated-block-stress-ccx/block"])
+                                      "grid-heated-block-ccx/block", "grid-heated-block-stress-ccx/block",
+                                      "grid-constrained-block/block"])
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
 def test_reference_case_passes_its_check(fem, case_dir, precision):
     from femahoy_amd import grid_case

@@ -136,7 +136,7 @@ def test_static_thermal_case_reads():
     assert [(l["kind"], bool(l.get("heat"))) for l in c.loads] == [("convection", True), ("source", True),
                                                                    ("point", False)]
     assert c.loads[2]["point"] == [0.05, 0.05, 0.0] and c.loads[2]["value"] == [None, None, 0.0]
-    assert [m["expansion"] for m in c.materials] == [12e-6, 17e-6]
+    assert [m["expansion"] for m in c.materials] == [[12e-6] * 3, [17e-6] * 3]
     assert c.surfaces == [("bottom", 0)]
     g, _, mids = built(c)
     ijk, _, _, number_of = gc.number(g)

@@ -159,3 +159,34 @@ def test_strain_stress_is_the_elements_virtual_work(fem):
         work += jac * float(s @ ed)
     expect = float(du @ (k @ u - load @ t))
     assert abs(work - expect) <= 1e-11 * (abs(expect) + np.abs(k).max())
+
+
+def test_isotropic_is_the_orthotropic_special_case(fem):
+    """hex.rs:58-64: an isotropic material is the orthotropic one with E, nu and E / (2 (1 + nu)) on
+    every axis; the element is the same bit for bit either way."""
+    size, e, nu = [1.25, 0.5, 2.0], 210e3, 0.29
+    k1 = fem.hex_element(size, e, nu)
+    k2 = fem.hex_element(size, [e] * 3, [nu] * 3, g=[e / (2.0 * (1.0 + nu))] * 3)
+    assert np.array_equal(k1, k2)
+
+
+def test_orthotropic_block_held_on_every_face(fem):
+    """grid-constrained-block (derivation.md, R. M. Jones 1999 section 2.5): an orthotropic block
+    heated by 100 C with every face held normal has no strain, so S sigma = -alpha dT. The stress of
+    the element at u = 0 and a uniform dT is -D alpha dT (hex.rs:66-115, 313-340): it equals the
+    derivation's values to their 9 decimals, and numpy's solve of the compliance. A swapped Poisson
+    ratio or shear modulus fails it."""
+    e3, nu3, g3, a3 = [100e3, 50e3, 20e3], [0.3, 0.25, 0.2], [20e3, 10e3, 8e3], [5e-6, 10e-6, 20e-6]
+    _, s = fem.hex_strain_stress([2.5, 2.0, 1.5], e3, nu3, a3, [0.1, -0.3, 0.7], np.zeros(24), np.full(8, 100.0), g=g3)
+    assert np.allclose(s[:3], [-84.302009777, -72.650733297, -50.027159153], rtol=0, atol=5e-10)
+    assert np.all(s[3:] == 0.0)
+    ex, ey, ez = e3
+    nxy, nxz, nyz = nu3
+    comp = np.array([[1 / ex, -nxy / ex, -nxz / ex], [-nxy / ex, 1 / ey, -nyz / ey], [-nxz / ex, -nyz / ey, 1 / ez]])
+    assert np.allclose(s[:3], np.linalg.solve(comp, -np.array(a3) * 100.0), rtol=1e-12, atol=0)
+    # the shear moduli: a pure engineering shear gamma_xy gives G_xy gamma_xy alone
+    x = _corners([2.5, 2.0, 1.5])
+    u = np.zeros((8, 3))
+    u[:, 0] = 1e-3 * x[:, 1]  # u_x = 1e-3 y
+    _, s = fem.hex_strain_stress([2.5, 2.0, 1.5], e3, nu3, a3, [0.0, 0.0, 0.0], u.ravel(), np.zeros(8), g=g3)
+    assert np.allclose(s, [0, 0, 0, 0, 0, 20e3 * 1e-3], rtol=1e-12, atol=1e-12)
