@@ -262,8 +262,8 @@ def lib() -> C.CDLL:
         "fem_test_hex_element": (C.c_int, [F64P, C.c_double, C.c_double, F64P, ERR, SZ]),
         "fem_hex_element": (C.c_int, [F64P, F64P, F64P, F64P, F64P, ERR, SZ]),
         "fem_hex_stress": (C.c_int, [F64P, F64P, F64P, F64P, F64P, F64P, F64P, F64P, ERR, SZ]),
-        "fem_test_supernodal": (C.c_int, [C.POINTER(_Csr), P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), P, P, P, P,
-                                          P, ERR, SZ]),
+        "fem_test_supernodal": (C.c_int, [C.POINTER(_Csr), P, C.c_uint64, C.POINTER(C.c_uint64),
+                                          C.POINTER(C.c_uint64), P, P, P, P, P, ERR, SZ]),
         "fem_test_hex_box_system": (C.c_int, [C.POINTER(_Box), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), P, P, P,
                                               P, C.POINTER(C.c_double), ERR, SZ]),
     }
@@ -723,7 +723,7 @@ def symbolic(ptr, idx, xyz):
     return order.astype(np.int64), int(e.value), float(f.value), int(lf.value), int(sn.value)
 
 
-def supernodal(ptr, idx, xyz):
+def supernodal(ptr, idx, xyz, join: int = 0):
     """The library's host structure of the multifrontal factor of a square pattern (symbolic.rs:
     41-150 in the nested-dissection order of xyz): (perm, start, rptr, rows, sparent). No device."""
     n = len(ptr) - 1
@@ -732,13 +732,14 @@ def supernodal(ptr, idx, xyz):
     ns, nr = C.c_uint64(), C.c_uint64()
     err = _err()
     L = lib()
-    if L.fem_test_supernodal(C.byref(a), xyz.ctypes.data, C.byref(ns), C.byref(nr), None, None, None, None, None,
+    if L.fem_test_supernodal(C.byref(a), xyz.ctypes.data, int(join), C.byref(ns), C.byref(nr), None, None, None, None,
+                             None,
                              err, 2048):
         raise FemError(err.value.decode())
     perm = np.zeros(n, np.uint64)
     start, rptr = np.zeros(ns.value + 1, np.uint64), np.zeros(ns.value + 1, np.uint64)
     rows, sparent = np.zeros(max(nr.value, 1), np.uint64), np.zeros(max(ns.value, 1), np.int64)
-    if L.fem_test_supernodal(C.byref(a), xyz.ctypes.data, C.byref(ns), C.byref(nr), perm.ctypes.data,
+    if L.fem_test_supernodal(C.byref(a), xyz.ctypes.data, int(join), C.byref(ns), C.byref(nr), perm.ctypes.data,
                              start.ctypes.data, rptr.ctypes.data, rows.ctypes.data, sparent.ctypes.data, err, 2048):
         raise FemError(err.value.decode())
     del keep

@@ -942,8 +942,9 @@ int fem_test_symbolic(const fem_csr* a, const double* xyz, uint64_t* order_out, 
     }
 }
 
-int fem_test_supernodal(const fem_csr* a, const double* xyz, uint64_t* nsup, uint64_t* nrows, uint64_t* perm,
-                        uint64_t* start, uint64_t* rptr, uint64_t* rows, int64_t* sparent, char* err, size_t errlen) {
+int fem_test_supernodal(const fem_csr* a, const double* xyz, uint64_t join, uint64_t* nsup, uint64_t* nrows,
+                        uint64_t* perm, uint64_t* start, uint64_t* rptr, uint64_t* rows, int64_t* sparent, char* err,
+                        size_t errlen) {
     try {
         fem::csr::Csr m = fem::csr_from(a);
         fem::csr::validate(m);
@@ -951,7 +952,8 @@ int fem_test_supernodal(const fem_csr* a, const double* xyz, uint64_t* nsup, uin
         std::vector<std::array<double, 3>> p(m.n);
         for (uint64_t k = 0; k < m.n; ++k)
             for (int d = 0; d < 3; ++d) p[k][d] = xyz[3 * k + d];
-        const fem::sym::Supernodal S = fem::sym::supernodal(m, fem::sym::nested_dissection(m, p));
+        const fem::sym::Supernodal S =
+            fem::sym::amalgamate(fem::sym::supernodal(m, fem::sym::nested_dissection(m, p)), join);
         if (nsup) *nsup = S.supernodes();
         if (nrows) *nrows = S.rows.size();
         if (perm) std::copy(S.perm.begin(), S.perm.end(), perm);

@@ -372,4 +372,49 @@ Supernodal supernodal(const csr::Csr& a, const std::vector<uint64_t>& order) {
 }
 
 }  // namespace sym
+// Amalgamation of small supernodes: a supernode whose columns end where its parent's begin (its
+// parent's last child in the post-order) joins its parent when the two hold at most `kmax` pivots
+// together. The joined front's rows are the pivots of both, then the parent's rows past its pivots
+// (the child's rows past its pivots are among them: its update goes to its parent), so the child's
+// columns only gain entries that stay exactly 0 in the factor, and the order is unchanged. Chains
+// of 1- and 2-pivot supernodes of the separators then take one level of the tree, not one each.
+sym::Supernodal sym::amalgamate(const sym::Supernodal& S, uint64_t kmax) {
+    const uint64_t ns = S.supernodes();
+    std::vector<uint64_t> first(S.start.begin(), S.start.end() - 1);  // first column, moves down on a join
+    std::vector<int64_t> into(ns, -1);                                 // the supernode a dead one joined
+    auto live = [&](int64_t q) {
+        while (q >= 0 && into[(size_t)q] >= 0) q = into[(size_t)q];
+        return q;
+    };
+    for (uint64_t q = 0; q < ns; ++q) {
+        const int64_t p = S.sparent[q];
+        if (p < 0) continue;
+        const uint64_t kq = S.start[q + 1] - first[q], kp = S.start[(size_t)p + 1] - first[(size_t)p];
+        if (S.start[q + 1] == first[(size_t)p] && kq + kp <= kmax) {
+            first[(size_t)p] = first[q];
+            into[q] = p;
+        }
+    }
+    sym::Supernodal R;
+    R.n = S.n;
+    R.perm = S.perm;
+    R.iperm = S.iperm;
+    R.counts = S.counts;
+    std::vector<int64_t> index(ns, -1);
+    for (uint64_t q = 0; q < ns; ++q)
+        if (into[q] < 0) index[q] = (int64_t)R.start.size(), R.start.push_back(first[q]);
+    R.start.push_back(S.n);
+    R.rptr.push_back(0);
+    for (uint64_t q = 0; q < ns; ++q) {
+        if (into[q] >= 0) continue;
+        const uint64_t kq = S.start[q + 1] - S.start[q];
+        for (uint64_t c = first[q]; c < S.start[q + 1]; ++c) R.rows.push_back(c);
+        R.rows.insert(R.rows.end(), S.rows.begin() + (ptrdiff_t)(S.rptr[q] + kq), S.rows.begin() + (ptrdiff_t)S.rptr[q + 1]);
+        R.rptr.push_back(R.rows.size());
+        const int64_t p = live(S.sparent[q]);
+        R.sparent.push_back(p < 0 ? -1 : index[(size_t)p]);
+    }
+    return R;
+}
+
 }  // namespace fem

@@ -43,6 +43,11 @@ struct Supernodal {
     uint64_t supernodes() const { return start.empty() ? 0 : start.size() - 1; }
 };
 Supernodal supernodal(const csr::Csr& a, const std::vector<uint64_t>& order);
+// The multifrontal factor's joins: a supernode whose columns end where its parent's begin (its
+// parent's last child) joins its parent when the two hold at most kmax pivots; its rows become
+// both pivot ranges and the parent's rows past its pivots. Same order, same counts; kmax 0 (or 1)
+// joins nothing.
+Supernodal amalgamate(const Supernodal& s, uint64_t kmax);
 
 // Crossover::Flops (solve.rs:113-117): direct when the factor has at most `entries` entries and
 // (`flat`, or its flops are at most `flops` per unknown).

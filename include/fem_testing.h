@@ -70,9 +70,12 @@ int fem_test_assembly_pattern(const fem_mesh* mesh, uint64_t* free_count, uint64
  * dissection order of the points xyz, 3 per unknown): the supernode and row counts first (call
  * with NULL arrays to size them), then the order (perm: n, new -> old), the supernodes' first
  * columns (start: supernodes + 1), their rows (rptr: supernodes + 1, rows: the factor's order,
- * each supernode's own columns first) and their parents (-1 at a root). No device is touched. */
-int fem_test_supernodal(const fem_csr* a, const double* xyz, uint64_t* nsup, uint64_t* nrows, uint64_t* perm,
-                        uint64_t* start, uint64_t* rptr, uint64_t* rows, int64_t* sparent, char* err, size_t errlen);
+ * each supernode's own columns first) and their parents (-1 at a root). join > 1: the supernodes
+ * after the multifrontal factor's joins of at most `join` pivots (0: the fundamental ones). No
+ * device is touched. */
+int fem_test_supernodal(const fem_csr* a, const double* xyz, uint64_t join, uint64_t* nsup, uint64_t* nrows,
+                        uint64_t* perm, uint64_t* start, uint64_t* rptr, uint64_t* rows, int64_t* sparent, char* err,
+                        size_t errlen);
 
 /* The condensed incompatible-mode hex of a box cell with sides size[3] and an isotropic material
  * (hex.rs:199-279 Hex::new's k: 24 x 24 row-major, unknown 3 corner + component, corners in VTK

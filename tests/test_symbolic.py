@@ -160,3 +160,46 @@ def test_the_multifrontal_structure_is_the_oracles(fem, oracle, case):
         r = rows[rptr[s]:rptr[s + 1]]
         k = start[s + 1] - start[s]
         assert np.array_equal(r[:k], np.arange(start[s], start[s + 1])) and np.all(np.diff(r) > 0)
+
+
+@pytest.mark.parametrize("join", [16, 48])
+def test_joined_supernodes_keep_the_factor(fem, oracle, join):
+    """The multifrontal factor's joins (a supernode that is its parent's last child joins it when the
+    two hold at most `join` pivots): the order is the fundamental one; pivot ranges stay contiguous
+    and cover every column once; each joined supernode's rows are its pivots then the rows of the
+    fundamental supernode it ends in; every fundamental supernode's rows lie in the rows of the
+    joined supernode that holds it, and every child's rows past its pivots lie in its parent's rows
+    (the extend-add of the factor); join 0 gives the fundamental supernodes unchanged."""
+    a, xyz = _heat(oracle, [14, 9, 7], [1.0, 0.8, 0.6], 12)
+    ptr, idx, _ = a.arrays()
+    ptr = ptr.astype(np.uint64)
+    f = fem.supernodal(ptr, idx, xyz)
+    z = fem.supernodal(ptr, idx, xyz, join=0)
+    assert all(np.array_equal(u, v) for u, v in zip(f, z))
+    perm, start, rptr, rows, sparent = fem.supernodal(ptr, idx, xyz, join=join)
+    fperm, fstart, frptr, frows, fsparent = f
+    assert np.array_equal(perm, fperm)
+    ns, fns = len(start) - 1, len(fstart) - 1
+    assert ns < fns and start[0] == 0 and start[-1] == len(perm) and np.all(np.diff(start) > 0)
+    # every joined supernode starts at a fundamental one's first column and ends at one's last
+    assert set(start.tolist()) <= set(fstart.tolist())
+    owner = np.searchsorted(start, fstart[:-1], side="right") - 1  # the joined supernode of each fundamental one
+    for q in range(ns):
+        own = rows[rptr[q]:rptr[q + 1]]
+        k = start[q + 1] - start[q]
+        assert np.array_equal(own[:k], np.arange(start[q], start[q + 1]))
+        assert np.all(np.diff(own) > 0)
+        # its last fundamental supernode's rows past its pivots follow
+        last = np.searchsorted(fstart, start[q + 1]) - 1
+        fk = fstart[last + 1] - fstart[last]
+        assert np.array_equal(own[k:], frows[frptr[last] + fk:frptr[last + 1]])
+        if k > 1 and len(np.flatnonzero(owner == q)) > 1:
+            assert k <= join
+        if sparent[q] >= 0:
+            p = sparent[q]
+            assert p > q
+            prow = set(rows[rptr[p]:rptr[p + 1]].tolist())
+            assert set(own[k:].tolist()) <= prow
+    for s in range(fns):
+        q = owner[s]
+        assert set(frows[frptr[s]:frptr[s + 1]].tolist()) <= set(rows[rptr[q]:rptr[q + 1]].tolist())
